@@ -1,0 +1,124 @@
+/* wpbr.h - C ABI of libwpbr_hip.so, the MI355X (gfx950) implementation of
+ * WoodenPBREngine's ray-tracing hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * wpbr_status, and never throws.  Pointers named d_* are device (HBM) pointers
+ * obtained from wpbr_malloc (or any hipMalloc'd memory on the context's
+ * device); all other pointers are host memory.  Calls are ordered on the
+ * context's stream and return after the work completes unless the name ends
+ * in _async.  A context is not re-entrant: use one per host thread.
+ *
+ * Reference interfaces each group replaces (WoodenPBREngine/...):
+ *   wpbr_shape_bounds        JobUpdateBoundsAndCentroidTriangle / ...Sphere
+ *                            (CSTriangle.cpp:190-216, CSSphere.h:126-153)
+ *   wpbr_lbvh_build          MEngine::buildLBVH (MEngine.cpp:341-350): the jobs
+ *                            JobGenerateShapesCentroidBound (SLBVH.cpp:164),
+ *                            JobGenerateShapesMortonCode (:186),
+ *                            JobSortsShapesByMortonCode (:231),
+ *                            JobEmitLBVH (:328), JobBuildUpperSAH (:410)
+ *   wpbr_collide             MEngine::runCollisionSystem (MEngine.cpp:352-361):
+ *                            JobProcessRayCasts (SLBVH.cpp:85), the sphere and
+ *                            triangle interaction-request jobs (CSSphere.cpp:16,
+ *                            CSTriangle.cpp:9), JobProcessRayCastsResults
+ *                            (SLBVH.cpp:9) and the full-interaction jobs
+ *                            (CSSphere.cpp:53, CSTriangle.cpp:149)
+ *   wpbr_render_*            MEngine::render (MEngine.cpp:380-448) job list
+ */
+#ifndef WPBR_H
+#define WPBR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t wpbr_status;
+enum {
+    WPBR_OK = 0,
+    WPBR_ERR_INVALID_ARG = 1,
+    WPBR_ERR_OOM = 2,
+    WPBR_ERR_HIP = 3,
+    WPBR_ERR_STACK_OVERFLOW = 4, /* traversal stack exceeded (reference: UB past 256) */
+    WPBR_ERR_TIMEOUT = 5,        /* a bounded device-side wait gave up */
+    WPBR_ERR_NOT_BUILT = 6
+};
+
+typedef struct wpbr_ctx wpbr_ctx;
+typedef struct wpbr_lbvh wpbr_lbvh;
+
+/* ---- context and device memory --------------------------------------- */
+wpbr_status wpbr_open(int device, wpbr_ctx** out);
+void wpbr_close(wpbr_ctx* ctx);
+const char* wpbr_status_string(wpbr_status s);
+const char* wpbr_last_error(wpbr_ctx* ctx);
+wpbr_status wpbr_malloc(wpbr_ctx* ctx, size_t bytes, void** d_ptr);
+wpbr_status wpbr_free(wpbr_ctx* ctx, void* d_ptr);
+wpbr_status wpbr_upload(wpbr_ctx* ctx, void* d_dst, const void* src, size_t bytes);
+wpbr_status wpbr_download(wpbr_ctx* ctx, void* dst, const void* d_src, size_t bytes);
+wpbr_status wpbr_sync(wpbr_ctx* ctx);
+/* Device time in milliseconds of the last completed entry point (HIP events on
+ * the context's stream). */
+float wpbr_last_time_ms(wpbr_ctx* ctx);
+
+/* ---- geometry ---------------------------------------------------------- *
+ * Shape order (the order the reference's ECS iterates CBounds/CCentroid):
+ * spheres first (created by SSphere::create while the scene is built), then
+ * triangles (created afterwards by JobTriangleMeshGenerateTriangles).
+ * Shape id s < n_spheres is sphere s; otherwise triangle s - n_spheres. */
+typedef struct {
+    uint32_t n_spheres;
+    const float* d_sphere_xform;  /* n_spheres x 24: world 3x4 row-major, then its inverse */
+    const float* d_sphere_radius; /* n_spheres */
+    uint32_t n_triangles;
+    const float* d_tri_verts;     /* n_triangles x 9: world-space v0, v1, v2 */
+} wpbr_geometry;
+
+/* a1: per-shape world bounds (6 floats: min xyz, max xyz) and centroid
+ * (3 floats) into d_bounds / d_centroid, shape order as above. */
+wpbr_status wpbr_shape_bounds(wpbr_ctx* ctx, const wpbr_geometry* geom, float* d_bounds,
+                              float* d_centroid);
+
+/* ---- LBVH (a2-a7) -------------------------------------------------------- */
+wpbr_status wpbr_lbvh_create(wpbr_ctx* ctx, wpbr_lbvh** out);
+void wpbr_lbvh_destroy(wpbr_lbvh* bvh);
+/* Builds the tree over the shapes of geom (bounds, centroid bounds, Morton
+ * codes, radix sort, treelets, upper SAH, flatten, traversal layout).  Device
+ * buffers are owned by the lbvh object and reused across builds. */
+wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* bvh);
+/* Counts: [0] shapes, [1] nodes, [2] upper SAH nodes, [3] treelets. */
+wpbr_status wpbr_lbvh_counts(wpbr_lbvh* bvh, uint32_t counts[4]);
+/* Downloads one array of CSceneTreeLBVH (SLBVH.h:17-40) or build state to host
+ * memory.  field: 0 Morton codes (u32, shape order), 1 sorted codes (u32),
+ * 2 sorted shape index (u32) = shapesEntities, 3 node offset (u32:
+ * shapeOffset for leaves, secondChildOffset for interior nodes), 4 node
+ * nShapes (u32), 5 node axis (u8), 6 node bounds (6 x f32), 7 centroid bounds
+ * (6 x f32), 8 treelet start (u32). */
+wpbr_status wpbr_lbvh_download(wpbr_ctx* ctx, wpbr_lbvh* bvh, int field, void* dst, size_t bytes);
+/* Per-kernel device times (ms) of the last build: [0] bounds, [1] morton+hist,
+ * [2..5] sort passes, [6] treelet marks, [7] emit, [8] upper SAH, [9] flatten,
+ * [10] traversal layout; returns the number written. */
+int wpbr_lbvh_kernel_times(wpbr_lbvh* bvh, float* ms, int cap);
+
+/* ---- collision (a8-a13) -------------------------------------------------- *
+ * Rays are SoA: d_rays = 6 x n floats (ox[n], oy[n], oz[n], dx[n], dy[n],
+ * dz[n]); tmax = +inf as in the reference.  Output per ray: the hit shape id
+ * (0xFFFFFFFF = INVALID_HANDLE, a miss) and t (+inf on a miss); optional
+ * barycentrics (3 x n, triangles only).  Closest hit, ties to the first
+ * candidate in the reference's depth-first order. */
+wpbr_status wpbr_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* bvh,
+                         uint32_t n, const float* d_rays, uint32_t* d_hit_shape, float* d_hit_t,
+                         float* d_bary);
+/* Same, without waiting; the caller synchronises with wpbr_sync. */
+wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* bvh,
+                               uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
+                               float* d_hit_t, float* d_bary);
+/* Traversal launch shape: block size (64..1024, multiple of 64) and LDS stack
+ * depth (8..64).  0 keeps the current value. */
+wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WPBR_H */

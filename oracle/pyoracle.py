@@ -1,0 +1,178 @@
+"""ctypes wrapper of oracle/_build/liboracle.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, as the checker and the timed CPU baseline; never by
+the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "liboracle.so"
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", str(HERE), "-s"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    vp, u32, f32 = C.c_void_p, C.c_uint32, C.c_float
+    lib.orc_shape_bounds.argtypes = [u32, vp, vp, u32, vp, vp, vp]
+    lib.orc_centroid_bounds.argtypes = [u32, vp, vp]
+    lib.orc_lbvh_build.argtypes = [u32, vp, vp, vp]
+    lib.orc_lbvh_build.restype = vp
+    lib.orc_lbvh_free.argtypes = [vp]
+    lib.orc_lbvh_info.argtypes = [vp, vp]
+    lib.orc_lbvh_copy.argtypes = [vp, C.c_int, vp]
+    lib.orc_lbvh_copy.restype = C.c_int
+    lib.orc_collide.argtypes = [u32, vp, vp, u32, vp, vp, u32, vp, vp, vp, C.c_int, vp]
+    lib.orc_collide.restype = C.c_uint64
+    lib.orc_brute_force.argtypes = [u32, vp, vp, u32, vp, u32, vp, vp, vp, C.c_int]
+    lib.orc_tri_intersect.argtypes = [vp, vp, vp, vp]
+    lib.orc_tri_intersect.restype = f32
+    lib.orc_sphere_intersect.argtypes = [vp, f32, vp, vp]
+    lib.orc_sphere_intersect.restype = f32
+    lib.orc_box_hit.argtypes = [vp, vp, vp, f32, vp]
+    lib.orc_box_hit.restype = C.c_int
+    lib.orc_partition_check.argtypes = [u32, vp, C.c_int, vp, vp, vp]
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(shape) if shape is not None else a
+
+
+def default_threads():
+    return int(os.environ.get("WPBR_ORACLE_THREADS", os.cpu_count() or 1))
+
+
+class OScene:
+    def __init__(self, sphere_xform, sphere_radius, tri_verts):
+        self.sx = _f32(sphere_xform, (-1, 24))
+        self.sr = _f32(sphere_radius, (-1,))
+        self.tv = _f32(tri_verts, (-1, 9))
+        self.ns, self.nt = len(self.sr), len(self.tv)
+
+    def args(self):
+        return (self.ns, _p(self.sx), _p(self.sr), self.nt, _p(self.tv))
+
+
+def shape_bounds(sc):
+    lib = load()
+    n = sc.ns + sc.nt
+    b = np.empty((n, 6), np.float32)
+    c = np.empty((n, 3), np.float32)
+    lib.orc_shape_bounds(*sc.args(), _p(b), _p(c))
+    return b, c
+
+
+class OLbvh:
+    def __init__(self, bounds, centroid, entities=None):
+        lib = load()
+        self.lib = lib
+        self.n = len(bounds)
+        self.bounds = _f32(bounds, (-1, 6))
+        self.centroid = _f32(centroid, (-1, 3))
+        ent = None if entities is None else np.ascontiguousarray(entities, np.uint32)
+        self.h = lib.orc_lbvh_build(self.n, _p(self.bounds), _p(self.centroid), _p(ent))
+        info = np.zeros(3, np.uint32)
+        lib.orc_lbvh_info(self.h, _p(info))
+        self.n_nodes, self.n_sah_nodes, self.n_treelets = (int(x) for x in info)
+
+    def get(self, name):
+        spec = {"morton": (0, self.n, np.uint32), "sorted_code": (1, self.n, np.uint32),
+                "sorted_index": (2, self.n, np.uint32), "shape_entities": (3, self.n, np.uint32),
+                "node_offset": (4, self.n_nodes, np.uint32),
+                "node_nshapes": (5, self.n_nodes, np.uint32),
+                "node_axis": (6, self.n_nodes, np.uint8),
+                "node_bounds": (7, 6 * self.n_nodes, np.float32),
+                "treelet_start": (8, self.n_treelets, np.uint32),
+                "centroid_bounds": (9, 6, np.float32)}[name]
+        out = np.empty(spec[1], dtype=spec[2])
+        self.lib.orc_lbvh_copy(self.h, spec[0], _p(out))
+        if name in ("node_bounds", "centroid_bounds"):
+            out = out.reshape(-1, 6)
+        return out
+
+    def __del__(self):
+        try:
+            self.lib.orc_lbvh_free(self.h)
+        except Exception:
+            pass
+
+
+def build_scene_bvh(sc):
+    b, c = shape_bounds(sc)
+    return OLbvh(b, c)
+
+
+def collide(sc, bvh, rays, nthreads=None):
+    lib = load()
+    rays = _f32(rays, (6, -1))
+    n = rays.shape[1]
+    shape = np.empty(n, np.uint32)
+    t = np.empty(n, np.float32)
+    ovf = np.zeros(1, np.int32)
+    cand = lib.orc_collide(*sc.args(), bvh.h, n, _p(rays), _p(shape), _p(t),
+                           nthreads or default_threads(), _p(ovf))
+    return shape, t, int(cand), bool(ovf[0])
+
+
+def brute_force(sc, rays, nthreads=None):
+    lib = load()
+    rays = _f32(rays, (6, -1))
+    n = rays.shape[1]
+    shape = np.empty(n, np.uint32)
+    t = np.empty(n, np.float32)
+    lib.orc_brute_force(*sc.args(), n, _p(rays), _p(shape), _p(t), nthreads or default_threads())
+    return shape, t
+
+
+def tri_intersect(v9, o, d):
+    lib = load()
+    v9, o, d = _f32(v9), _f32(o), _f32(d)
+    bary = np.zeros(3, np.float32)
+    t = lib.orc_tri_intersect(_p(v9), _p(o), _p(d), _p(bary))
+    return float(t), bary
+
+
+def sphere_intersect(xform24, radius, o, d):
+    lib = load()
+    return float(lib.orc_sphere_intersect(_p(_f32(xform24)), float(radius), _p(_f32(o)),
+                                          _p(_f32(d))))
+
+
+def box_hit(b6, o, d, tmax=np.inf):
+    lib = load()
+    tn = np.zeros(1, np.float32)
+    h = lib.orc_box_hit(_p(_f32(b6)), _p(_f32(o)), _p(_f32(d)), float(tmax), _p(tn))
+    return bool(h), float(tn[0])
+
+
+def partition_check(buckets, best):
+    lib = load()
+    b = np.ascontiguousarray(buckets, np.uint8)
+    n = len(b)
+    a1 = np.empty(n, np.uint32)
+    a2 = np.empty(n, np.uint32)
+    mids = np.empty(2, np.uint32)
+    lib.orc_partition_check(n, _p(b), int(best), _p(a1), _p(a2), _p(mids))
+    return a1, a2, mids

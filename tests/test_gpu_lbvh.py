@@ -1,0 +1,108 @@
+"""GPU parity: LBVH build and closest-hit traversal against the oracle.
+
+Bit-exact: Morton codes, sort permutation, every node array (bounds compared as
+float32 values), hit shape ids; t equal bit for bit (the north-star bound is
+2 ULP).  Calls go through the C ABI (libwpbr_hip.so)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from woodenpbrengine_amd import lib, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _spheres(k, seed):
+    g = np.random.default_rng(seed)
+    c = g.random((k, 3))
+    r = (0.02 + 0.08 * g.random(k)).astype(np.float32)
+    sx = np.stack([scenes.translate_xform(ci) for ci in c]) if k else np.zeros((0, 24), np.float32)
+    return sx, r
+
+
+def _case(name):
+    if name == "tiny":
+        tris = scenes.random_triangles(5, 11)
+        sx, sr = _spheres(1, 12)
+    elif name == "small":
+        tris = scenes.random_triangles(3000, 13, median_edge=0.02)
+        sx, sr = _spheres(3, 14)
+    elif name == "medium":
+        tris = scenes.random_triangles(100_000, 15, median_edge=0.01)
+        sx, sr = _spheres(8, 16)
+    elif name == "icosphere":
+        tris = scenes.icosphere(32)
+        sx, sr = _spheres(2, 17)
+    elif name == "clustered":
+        # many coincident centroids: duplicate Morton codes and >= 30-shape leaves
+        base = scenes.random_triangles(50, 18, median_edge=0.05)
+        tris = np.repeat(base, 80, axis=0)
+        sx, sr = _spheres(0, 19)
+    else:
+        raise KeyError(name)
+    return sx, sr, tris
+
+
+CASES = ["tiny", "small", "medium", "icosphere", "clustered"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_lbvh_build_bit_exact(ctx, name):
+    sx, sr, tris = _case(name)
+    print("case", name, len(sr), len(tris), flush=True)
+    osc = po.OScene(sx, sr, tris)
+    ob = po.build_scene_bvh(osc)
+    sc = lib.Scene(ctx, sx, sr, tris)
+    b = lib.Lbvh(ctx)
+    b.build(sc)
+    c = b.counts()
+    assert c["nodes"] == ob.n_nodes
+    assert c["treelets"] == ob.n_treelets
+    np.testing.assert_array_equal(b.download("centroid_bounds").ravel(),
+                                  ob.get("centroid_bounds").ravel())
+    np.testing.assert_array_equal(b.download("morton"), ob.get("morton"))
+    np.testing.assert_array_equal(b.download("sorted_code"), ob.get("sorted_code"))
+    np.testing.assert_array_equal(b.download("sorted_index"), ob.get("sorted_index"))
+    np.testing.assert_array_equal(b.download("treelet_start"), ob.get("treelet_start"))
+    ns = b.download("node_nshapes")
+    np.testing.assert_array_equal(ns, ob.get("node_nshapes"))
+    np.testing.assert_array_equal(b.download("node_offset"), ob.get("node_offset"))
+    interior = ns == 0
+    np.testing.assert_array_equal(b.download("node_axis")[interior], ob.get("node_axis")[interior])
+    np.testing.assert_array_equal(b.download("node_bounds"), ob.get("node_bounds"))
+    b.destroy()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_collide_matches_oracle(ctx, name):
+    sx, sr, tris = _case(name)
+    print("case", name, flush=True)
+    osc = po.OScene(sx, sr, tris)
+    ob = po.build_scene_bvh(osc)
+    rays = np.concatenate([scenes.pinhole_rays(128, 64, (0.5, 0.5, -1.0)),
+                           scenes.random_rays(8192, 21)], axis=1)
+    os_, ot, _, ovf = po.collide(osc, ob, rays)
+    assert not ovf
+    sc = lib.Scene(ctx, sx, sr, tris)
+    b = lib.Lbvh(ctx)
+    b.build(sc)
+    d_rays = ctx.array(rays)
+    d_s, d_t, _ = lib.collide(ctx, sc, b, d_rays, rays.shape[1])
+    gs, gt = d_s.numpy(), d_t.numpy()
+    np.testing.assert_array_equal(gs, os_)
+    np.testing.assert_array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    b.destroy()
+
+
+def test_rebuild_reuses_buffers(ctx):
+    """Two builds in a row (the second smaller) give the oracle's trees."""
+    b = lib.Lbvh(ctx)
+    for n, seed in ((20000, 31), (700, 32)):
+        tris = scenes.random_triangles(n, seed, median_edge=0.02)
+        sx, sr = _spheres(2, seed)
+        sc = lib.Scene(ctx, sx, sr, tris)
+        b.build(sc)
+        ob = po.build_scene_bvh(po.OScene(sx, sr, tris))
+        np.testing.assert_array_equal(b.download("node_offset"), ob.get("node_offset"))
+        np.testing.assert_array_equal(b.download("node_bounds"), ob.get("node_bounds"))
+    b.destroy()

@@ -1,0 +1,256 @@
+"""Pins the oracle's LBVH build and collision against checks that do not retype
+its formulas: closed forms, library routines, invariants and brute force.
+
+The reference ships no tests or golden vectors for this path (it cannot be built
+here: WoodenECS/WoodenMath are not vendored), so these are the pins."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from woodenpbrengine_amd import scenes
+
+
+def _interleave_ref(x, y, z):
+    """Bit interleave by loops over the 10 bits (independent of leftShift3)."""
+    code = np.zeros_like(x, dtype=np.uint64)
+    for b in range(10):
+        code |= ((x >> b) & 1).astype(np.uint64) << np.uint64(3 * b)
+        code |= ((y >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 1)
+        code |= ((z >> b) & 1).astype(np.uint64) << np.uint64(3 * b + 2)
+    return code.astype(np.uint32)
+
+
+def _scene(n=4000, seed=3, spheres=3):
+    tris = scenes.random_triangles(n, seed, median_edge=0.02)
+    g = np.random.default_rng(seed)
+    sx = np.stack([scenes.translate_xform(c) for c in g.random((spheres, 3))])
+    sr = (0.03 + 0.05 * g.random(spheres)).astype(np.float32)
+    return po.OScene(sx, sr, tris)
+
+
+def test_shape_bounds_closed_form():
+    sc = _scene(500, 5, 4)
+    b, c = po.shape_bounds(sc)
+    v = sc.tv.reshape(-1, 3, 3)
+    np.testing.assert_array_equal(b[sc.ns:, :3], v.min(axis=1))
+    np.testing.assert_array_equal(b[sc.ns:, 3:], v.max(axis=1))
+    # spheres: centre +- r exactly (translation-only transforms)
+    cen = sc.sx[:, [3, 7, 11]]
+    np.testing.assert_array_equal(b[:sc.ns, :3], cen - sc.sr[:, None])
+    np.testing.assert_array_equal(b[:sc.ns, 3:], cen + sc.sr[:, None])
+    # centroid = correctly rounded midpoint
+    mid = ((b[:, :3].astype(np.float64) + b[:, 3:].astype(np.float64)) / 2).astype(np.float32)
+    np.testing.assert_array_equal(c, mid)
+
+
+def test_morton_closed_form():
+    sc = _scene()
+    b, c = po.shape_bounds(sc)
+    bvh = po.OLbvh(b, c)
+    cb = bvh.get("centroid_bounds").ravel()
+    np.testing.assert_array_equal(cb[:3], c.min(axis=0))
+    np.testing.assert_array_equal(cb[3:], c.max(axis=0))
+    ext = (cb[3:] - cb[:3]).astype(np.float32)
+    o = ((c - cb[:3]) / ext).astype(np.float32)
+    q = np.minimum((o * np.float32(1024)).astype(np.uint32), 1023)
+    ref = _interleave_ref(q[:, 0], q[:, 1], q[:, 2])
+    np.testing.assert_array_equal(bvh.get("morton"), ref)
+    assert bvh.get("morton").max() < (1 << 30)
+
+
+def test_sort_is_stable_sort():
+    # duplicate codes on purpose: repeated triangles share centroids
+    tris = np.repeat(scenes.random_triangles(300, 7, median_edge=0.03), 7, axis=0)
+    sc = po.OScene(np.zeros((0, 24)), np.zeros(0), tris)
+    bvh = po.build_scene_bvh(sc)
+    m = bvh.get("morton")
+    order = np.argsort(m, kind="stable")
+    np.testing.assert_array_equal(bvh.get("sorted_index"), order)
+    np.testing.assert_array_equal(bvh.get("sorted_code"), m[order])
+
+
+def _walk(bvh):
+    """Depth-first walk of the flattened tree; returns leaves in visit order."""
+    off, ns, nb = bvh.get("node_offset"), bvh.get("node_nshapes"), bvh.get("node_bounds")
+    leaves, seen = [], np.zeros(bvh.n_nodes, bool)
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        assert not seen[i]
+        seen[i] = True
+        if ns[i] > 0:
+            leaves.append((off[i], ns[i], i))
+        else:
+            c0, c1 = i + 1, off[i]
+            assert c1 > c0
+            # interior bounds are the union of the children's
+            np.testing.assert_array_equal(nb[i, :3], np.minimum(nb[c0, :3], nb[c1, :3]))
+            np.testing.assert_array_equal(nb[i, 3:], np.maximum(nb[c0, 3:], nb[c1, 3:]))
+            stack.append(c1)
+            stack.append(c0)
+    assert seen.all(), "every node is reachable exactly once (preorder layout)"
+    return leaves
+
+
+@pytest.mark.parametrize("dup", [1, 40])
+def test_tree_invariants(dup):
+    tris = np.repeat(scenes.random_triangles(3000 // dup, 9, median_edge=0.02), dup, axis=0)
+    sc = po.OScene(np.zeros((0, 24)), np.zeros(0), tris)
+    b, c = po.shape_bounds(sc)
+    bvh = po.OLbvh(b, c)
+    codes = bvh.get("sorted_code")
+    idx = bvh.get("sorted_index")
+    leaves = _walk(bvh)
+    # leaf ranges partition [0, n): each primitive is in exactly one leaf (the
+    # upper SAH reorders treelets, so ranges are contiguous only after sorting)
+    pos = 0
+    nb = bvh.get("node_bounds")
+    for off, cnt, i in sorted(leaves):
+        assert off == pos
+        pos += cnt
+        if cnt >= 30:  # large leaves only for identical codes (SLBVH.cpp:357)
+            assert np.all(codes[off:off + cnt] == codes[off])
+        bb = b[idx[off:off + cnt]]
+        np.testing.assert_array_equal(nb[i, :3], bb[:, :3].min(axis=0))
+        np.testing.assert_array_equal(nb[i, 3:], bb[:, 3:].max(axis=0))
+    assert pos == len(tris)
+    # treelets are the runs of equal top-12 bits
+    starts = bvh.get("treelet_start")
+    top = codes >> 18
+    expect = np.concatenate([[0], np.nonzero(top[1:] != top[:-1])[0] + 1])
+    np.testing.assert_array_equal(starts, expect)
+    # node count = upper SAH nodes (k - 1) + treelet nodes
+    assert bvh.n_sah_nodes == bvh.n_treelets - 1
+
+
+def test_split_is_first_set_bit():
+    """Inside one treelet the first split is at the first index whose highest
+    differing bit is set (SLBVH.cpp:381-392), checked on a 1-treelet scene."""
+    g = np.random.default_rng(4)
+    # all centroids in one top-12-bit cell: tiny cube near the origin corner
+    c = (g.random((200, 3)) * 1e-3).astype(np.float32)
+    c[0] = 0.0
+    c[1] = 1.0  # stretch the centroid bounds so the cell is tiny
+    b = np.concatenate([c, c], axis=1)
+    bvh = po.OLbvh(b, c)
+    codes = bvh.get("sorted_code")
+    starts = bvh.get("treelet_start")
+    # find a treelet with >= 30 shapes and check its root split
+    ends = np.append(starts[1:], len(codes))
+    big = [(s, e) for s, e in zip(starts, ends) if e - s >= 30]
+    assert big
+    s, e = big[0]
+    x = int(codes[s]) ^ int(codes[e - 1])
+    if x:
+        hb = x.bit_length() - 1
+        first = s + np.argmax((codes[s:e] >> hb) & 1)
+        leaves = _walk(bvh)
+        # the split point is a leaf boundary
+        bounds = {off for off, _, _ in leaves}
+        assert first in bounds
+
+
+def test_partition_matches_libstdcxx():
+    g = np.random.default_rng(8)
+    for n in (2, 3, 10, 100, 4096):
+        for _ in range(5):
+            buckets = g.integers(0, 12, n).astype(np.uint8)
+            best = int(g.integers(0, 11))
+            a, b, mids = po.partition_check(buckets, best)
+            np.testing.assert_array_equal(a, b)
+            assert mids[0] == mids[1] == (buckets <= best).sum()
+
+
+def test_collide_equals_brute_force():
+    sc = _scene(6000, 11, 4)
+    bvh = po.build_scene_bvh(sc)
+    rays = np.concatenate([scenes.pinhole_rays(64, 32, (0.5, 0.5, -1.0)),
+                           scenes.random_rays(4096, 12)], axis=1)
+    s, t, _, ovf = po.collide(sc, bvh, rays)
+    s2, t2 = po.brute_force(sc, rays)
+    assert not ovf
+    np.testing.assert_array_equal(t.view(np.uint32), t2.view(np.uint32))
+    # ids agree except on exact t ties (none expected for random geometry)
+    np.testing.assert_array_equal(s, s2)
+    assert (s != 0xFFFFFFFF).sum() > 100
+
+
+def test_collide_tie_goes_to_first_candidate():
+    """Two identical triangles: the reference keeps the first strict minimum in
+    candidate order (SLBVH.cpp:32), i.e. the one earlier in its leaf."""
+    tri = np.array([[-1, -1, 0, 1, -1, 0, 0, 1, 0]], np.float32)
+    tris = np.concatenate([tri, tri])
+    sc = po.OScene(np.zeros((0, 24)), np.zeros(0), tris)
+    bvh = po.build_scene_bvh(sc)
+    rays = np.array([[0.1], [0.1], [-5], [0], [0], [1]], np.float32)
+    s, t, _, _ = po.collide(sc, bvh, rays)
+    assert t[0] == 5.0
+    assert s[0] == bvh.get("sorted_index")[0]
+
+
+def test_triangle_closed_form():
+    tri = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)
+    # axis-aligned hit at t = 2, barycentrics of (0.25, 0.25)
+    t, b = po.tri_intersect(tri, [0.25, 0.25, -2], [0, 0, 1])
+    assert t == 2.0
+    np.testing.assert_allclose(b, [0.5, 0.25, 0.25], rtol=1e-7)
+    # from behind
+    t, _ = po.tri_intersect(tri, [0.25, 0.25, 3], [0, 0, -1])
+    assert t == 3.0
+    # miss outside, ray behind the origin, parallel ray
+    assert po.tri_intersect(tri, [2, 2, -1], [0, 0, 1])[0] == 0.0
+    assert po.tri_intersect(tri, [0.25, 0.25, 1], [0, 0, 1])[0] == 0.0
+    assert po.tri_intersect(tri, [0.25, 0.25, 1], [1, 0, 0])[0] == 0.0
+    # y-dominant direction (the permutation case kz == 1) against a y-facing triangle
+    tri_y = np.array([0, 0, 0, 1, 0, 0, 0, 0, 1], np.float32)
+    t, _ = po.tri_intersect(tri_y, [0.2, 5, 0.2], [0, -1, 0])
+    # det is summed before the (1,2,0) permutation and tScale after it
+    # (CSTriangle.cpp:145-154), so t is within a few ULP of 5, not exact
+    assert abs(t - 5.0) <= 4 * np.spacing(np.float32(5.0))
+    # oblique: t equals the plane distance to float precision
+    d = np.array([0.3, 0.2, 1.0])
+    d /= np.linalg.norm(d)
+    t, _ = po.tri_intersect(tri, [0.1, 0.1, -1], d)
+    assert abs(t - 1.0 / d[2]) < 1e-6
+
+
+def test_sphere_closed_form():
+    xf = scenes.translate_xform((1.0, 2.0, 3.0))
+    t = po.sphere_intersect(xf, 0.5, [1, 2, 0], [0, 0, 1])
+    assert t == 2.5
+    t = po.sphere_intersect(xf, 0.5, [1, 2, 3], [0, 0, 1])  # inside: far root
+    assert t == 0.5
+    assert po.sphere_intersect(xf, 0.5, [1, 2, 5], [0, 0, 1]) == 0.0  # behind
+    assert po.sphere_intersect(xf, 0.5, [2, 2, 0], [0, 0, 1]) == 0.0  # miss
+    g = np.random.default_rng(2)
+    for _ in range(50):
+        o = g.normal(size=3) * 3
+        d = g.normal(size=3)
+        d /= np.linalg.norm(d)
+        t = po.sphere_intersect(xf, 0.5, o.astype(np.float32), d.astype(np.float32))
+        oc = o.astype(np.float32).astype(np.float64) - [1, 2, 3]
+        dd = d.astype(np.float32).astype(np.float64)
+        bq = np.dot(oc, dd)
+        disc = bq * bq - (np.dot(oc, oc) - 0.25) * np.dot(dd, dd)
+        if disc > 1e-9:
+            roots = sorted([(-bq - np.sqrt(disc)) / np.dot(dd, dd),
+                            (-bq + np.sqrt(disc)) / np.dot(dd, dd)])
+            pos = [r for r in roots if r > 0]
+            if pos:
+                assert abs(t - pos[0]) <= 1e-5 * max(1, pos[0])
+            else:
+                assert t == 0.0
+
+
+def test_box_hit_conservative():
+    g = np.random.default_rng(1)
+    for _ in range(300):
+        lo = g.random(3).astype(np.float32)
+        hi = (lo + g.random(3)).astype(np.float32)
+        p = (lo + (hi - lo) * g.random(3)).astype(np.float32)  # point inside
+        d = g.normal(size=3)
+        d /= np.linalg.norm(d)
+        o = (p - d * (1 + 2 * g.random())).astype(np.float32)
+        hit, tn = po.box_hit(np.concatenate([lo, hi]), o, d.astype(np.float32))
+        assert hit
+        assert tn <= np.linalg.norm(p - o) * (1 + 1e-5)

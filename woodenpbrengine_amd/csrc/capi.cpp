@@ -1,0 +1,426 @@
+// C ABI of libwpbr_hip.so: contexts, device memory, LBVH build and collision.
+// Host code only; kernels live in the .hip translation units.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+#include "wpbr_internal.hpp"
+#include "wpbr_ctx.hpp"
+
+using namespace wpbr;
+
+namespace wpbr {
+
+wpbr_status set_err(wpbr_ctx* ctx, wpbr_status s, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    return s;
+}
+
+wpbr_status hip_check(wpbr_ctx* ctx, hipError_t e, const char* what) {
+    if (e == hipSuccess) return WPBR_OK;
+    return set_err(ctx, e == hipErrorOutOfMemory ? WPBR_ERR_OOM : WPBR_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Bump allocator over one device allocation.
+struct Carver {
+    char* base;
+    size_t off = 0;
+    template <class T>
+    T* take(size_t count) {
+        off = (off + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + off);
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
+    Carver c{base};
+    const size_t N = std::max<uint32_t>(n, 1);
+    const size_t tiles = (N + 3839) / 3840;
+    s.bounds = c.take<float>(6 * N);
+    s.centroid = c.take<float>(3 * N);
+    s.cb_ord = c.take<uint32_t>(8);
+    s.morton = c.take<uint32_t>(N);
+    s.keys_a = c.take<uint32_t>(N);
+    s.vals_a = c.take<uint32_t>(N);
+    s.keys_b = c.take<uint32_t>(N);
+    s.vals_b = c.take<uint32_t>(N);
+    s.hist = c.take<uint32_t>(4 * 256);
+    s.status_words = (int)(tiles * 256);
+    s.status0 = c.take<uint32_t>(tiles * 256);
+    s.status1 = c.take<uint32_t>(tiles * 256);
+    s.tile_counter = c.take<uint32_t>(8);
+    s.err = c.take<uint32_t>(4);
+    s.bin_start = c.take<uint32_t>(4096);
+    s.presence = c.take<uint32_t>(128);
+    s.t_start = c.take<uint32_t>(4096);
+    s.t_count = c.take<uint32_t>(4096);
+    s.t_slot = c.take<uint32_t>(4096);
+    s.t_nodes = c.take<uint32_t>(4096);
+    s.t_final = c.take<uint32_t>(4096);
+    s.t_box = c.take<float>(6 * 4096);
+    s.r_start = c.take<uint32_t>(2 * N);
+    s.r_n = c.take<uint32_t>(2 * N);
+    s.r_c0 = c.take<uint32_t>(2 * N);
+    s.r_meta = c.take<uint32_t>(2 * N);
+    s.r_size = c.take<uint32_t>(2 * N);
+    s.r_off = c.take<uint32_t>(2 * N);
+    s.r_box = c.take<float>(12 * N);
+    s.perm = c.take<uint32_t>(4096);
+    s.bucket = c.take<uint8_t>(4096);
+    s.list_a = c.take<uint32_t>(4096);
+    s.list_b = c.take<uint32_t>(4096);
+    s.task_i0 = c.take<uint32_t>(8192);
+    s.task_i1 = c.take<uint32_t>(8192);
+    s.task_parent = c.take<uint32_t>(8192);
+    s.task_depth = c.take<uint32_t>(8192);
+    s.u_c0 = c.take<uint32_t>(4096);
+    s.u_c1 = c.take<uint32_t>(4096);
+    s.u_axis = c.take<uint32_t>(4096);
+    s.u_depth = c.take<uint32_t>(4096);
+    s.u_size = c.take<uint32_t>(4096);
+    s.u_off = c.take<uint32_t>(4096);
+    s.u_box = c.take<float>(6 * 4096);
+    s.upper_count = c.take<uint32_t>(1);
+    s.upper_root = c.take<uint32_t>(1);
+    s.n_nodes = c.take<uint32_t>(1);
+    s.n_treelets = c.take<uint32_t>(1);
+    s.node_offset = c.take<uint32_t>(2 * N);
+    s.node_nshapes = c.take<uint32_t>(2 * N);
+    s.node_axis = c.take<uint8_t>(2 * N);
+    s.node_bounds = c.take<float>(12 * N);
+    s.trav = c.take<float4>(4 * 2 * N);
+    s.prims = c.take<float4>(3 * N);
+    return c.off + 256;
+}
+
+GeomView to_view(const wpbr_geometry* g) {
+    GeomView v;
+    v.n_spheres = g->n_spheres;
+    v.sphere_xform = g->d_sphere_xform;
+    v.sphere_radius = g->d_sphere_radius;
+    v.n_triangles = g->n_triangles;
+    v.tri_verts = g->d_tri_verts;
+    return v;
+}
+
+}  // namespace wpbr
+
+extern "C" {
+
+wpbr_status wpbr_open(int device, wpbr_ctx** out) {
+    if (!out) return WPBR_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count)
+        return WPBR_ERR_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return WPBR_ERR_HIP;
+    wpbr_ctx* c = new wpbr_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return WPBR_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    *out = c;
+    return WPBR_OK;
+}
+
+void wpbr_close(wpbr_ctx* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->spill) hipFree(ctx->spill);
+    if (ctx->err_flag) hipFree(ctx->err_flag);
+    ctx_release_render(ctx);
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* wpbr_status_string(wpbr_status s) {
+    switch (s) {
+        case WPBR_OK: return "ok";
+        case WPBR_ERR_INVALID_ARG: return "invalid argument";
+        case WPBR_ERR_OOM: return "out of device memory";
+        case WPBR_ERR_HIP: return "HIP runtime error";
+        case WPBR_ERR_STACK_OVERFLOW: return "traversal stack overflow";
+        case WPBR_ERR_TIMEOUT: return "device-side wait timed out";
+        case WPBR_ERR_NOT_BUILT: return "acceleration structure not built";
+    }
+    return "unknown status";
+}
+
+const char* wpbr_last_error(wpbr_ctx* ctx) { return ctx ? ctx->last_error.c_str() : ""; }
+
+wpbr_status wpbr_malloc(wpbr_ctx* ctx, size_t bytes, void** d_ptr) {
+    if (!ctx || !d_ptr) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    return hip_check(ctx, hipMalloc(d_ptr, bytes ? bytes : 1), "hipMalloc");
+}
+
+wpbr_status wpbr_free(wpbr_ctx* ctx, void* d_ptr) {
+    if (!ctx) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    return hip_check(ctx, hipFree(d_ptr), "hipFree");
+}
+
+wpbr_status wpbr_upload(wpbr_ctx* ctx, void* d_dst, const void* src, size_t bytes) {
+    if (!ctx || (!d_dst && bytes) || (!src && bytes)) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    wpbr_status s = hip_check(ctx, hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, ctx->stream),
+                              "upload");
+    if (s) return s;
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "upload sync");
+}
+
+wpbr_status wpbr_download(wpbr_ctx* ctx, void* dst, const void* d_src, size_t bytes) {
+    if (!ctx || (!d_src && bytes) || (!dst && bytes)) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    wpbr_status s = hip_check(ctx, hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream),
+                              "download");
+    if (s) return s;
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "download sync");
+}
+
+wpbr_status wpbr_sync(wpbr_ctx* ctx) {
+    if (!ctx) return WPBR_ERR_INVALID_ARG;
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "sync");
+}
+
+float wpbr_last_time_ms(wpbr_ctx* ctx) { return ctx ? ctx->last_ms : 0.0f; }
+
+// ---- a1 -------------------------------------------------------------------
+wpbr_status wpbr_shape_bounds(wpbr_ctx* ctx, const wpbr_geometry* geom, float* d_bounds,
+                              float* d_centroid) {
+    if (!ctx || !geom || !d_bounds || !d_centroid) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    uint32_t* cb = nullptr;
+    wpbr_status s = hip_check(ctx, hipMalloc(&cb, 32), "hipMalloc");
+    if (s) return s;
+    s = shape_bounds_launch(to_view(geom), d_bounds, d_centroid, cb, ctx->stream);
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "shape bounds");
+    hipFree(cb);
+    return s;
+}
+
+// ---- LBVH -----------------------------------------------------------------
+wpbr_status wpbr_lbvh_create(wpbr_ctx* ctx, wpbr_lbvh** out) {
+    if (!ctx || !out) return WPBR_ERR_INVALID_ARG;
+    wpbr_lbvh* b = new wpbr_lbvh();
+    b->ctx = ctx;
+    hipSetDevice(ctx->device);
+    for (int i = 0; i < wpbr_lbvh::NEV; i++) hipEventCreate(&b->ev[i]);
+    *out = b;
+    return WPBR_OK;
+}
+
+void wpbr_lbvh_destroy(wpbr_lbvh* b) {
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->mem) hipFree(b->mem);
+    for (int i = 0; i < wpbr_lbvh::NEV; i++) hipEventDestroy(b->ev[i]);
+    delete b;
+}
+
+wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b) {
+    if (!ctx || !geom || !b) return WPBR_ERR_INVALID_ARG;
+    uint64_t n64 = (uint64_t)geom->n_spheres + geom->n_triangles;
+    if (n64 == 0 || n64 >= (1ull << 30)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count");
+    if ((geom->n_spheres && (!geom->d_sphere_xform || !geom->d_sphere_radius)) ||
+        (geom->n_triangles && !geom->d_tri_verts))
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
+    uint32_t n = (uint32_t)n64;
+    hipSetDevice(ctx->device);
+    if (n > b->cap) {
+        if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
+        LbvhScratch tmp;
+        size_t bytes = lbvh_layout(tmp, nullptr, n);
+        wpbr_status s = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc lbvh");
+        if (s) { b->cap = 0; return s; }
+        b->cap = n;
+    }
+    lbvh_layout(b->s, (char*)b->mem, n);
+    b->n = n;
+    b->geom = to_view(geom);
+    wpbr_status s = lbvh_launch(b->s, b->geom, n, ctx->stream, b->ev, wpbr_lbvh::NEV);
+    if (s) return set_err(ctx, s, "lbvh launch failed");
+    uint32_t info[2] = {0, 0};
+    s = hip_check(ctx, hipMemcpyAsync(&info[0], b->s.n_nodes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipMemcpyAsync(&info[1], b->s.n_treelets, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    uint32_t err = 0, ns0 = 0;
+    if (!s) s = hip_check(ctx, hipMemcpyAsync(&err, b->s.err, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipMemcpyAsync(&ns0, b->s.node_nshapes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "lbvh build");
+    if (s) return s;
+    b->n_nodes = info[0];
+    b->n_treelets = info[1];
+    b->root_is_leaf = ns0 > 0;
+    float ms = 0;
+    hipEventElapsedTime(&ms, b->ev[0], b->ev[wpbr_lbvh::NEV - 1]);
+    ctx->last_ms = ms;
+    b->built = err == 0;
+    if (err) return set_err(ctx, WPBR_ERR_TIMEOUT, "radix sort look-back timed out");
+    return WPBR_OK;
+}
+
+wpbr_status wpbr_lbvh_counts(wpbr_lbvh* b, uint32_t counts[4]) {
+    if (!b || !counts) return WPBR_ERR_INVALID_ARG;
+    counts[0] = b->n;
+    counts[1] = b->n_nodes;
+    counts[2] = b->n_treelets ? b->n_treelets - 1 : 0;
+    counts[3] = b->n_treelets;
+    return WPBR_OK;
+}
+
+int wpbr_lbvh_kernel_times(wpbr_lbvh* b, float* ms, int cap) {
+    if (!b || !ms) return 0;
+    int k = 0;
+    for (int i = 1; i < wpbr_lbvh::NEV && k < cap; i++, k++) {
+        float t = 0;
+        hipEventElapsedTime(&t, b->ev[i - 1], b->ev[i]);
+        ms[k] = t;
+    }
+    return k;
+}
+
+wpbr_status wpbr_lbvh_download(wpbr_ctx* ctx, wpbr_lbvh* b, int field, void* dst, size_t bytes) {
+    if (!ctx || !b || !dst) return WPBR_ERR_INVALID_ARG;
+    if (!b->built) return set_err(ctx, WPBR_ERR_NOT_BUILT, "lbvh not built");
+    const void* src = nullptr;
+    size_t need = 0;
+    const size_t n = b->n, nn = b->n_nodes;
+    switch (field) {
+        case 0: src = b->s.morton; need = 4 * n; break;
+        case 1: src = b->s.keys_a; need = 4 * n; break;
+        case 2: src = b->s.vals_a; need = 4 * n; break;
+        case 3: src = b->s.node_offset; need = 4 * nn; break;
+        case 4: src = b->s.node_nshapes; need = 4 * nn; break;
+        case 5: src = b->s.node_axis; need = nn; break;
+        case 6: src = b->s.node_bounds; need = 24 * nn; break;
+        case 7: src = b->s.cb_ord; need = 24; break;
+        case 8: src = b->s.t_start; need = 4 * (size_t)b->n_treelets; break;
+        default: return set_err(ctx, WPBR_ERR_INVALID_ARG, "unknown field");
+    }
+    if (bytes < need) return set_err(ctx, WPBR_ERR_INVALID_ARG, "destination too small");
+    wpbr_status s = wpbr_download(ctx, dst, src, need);
+    if (s || field != 7) return s;
+    // centroid bounds are stored as order-preserving uints
+    uint32_t* u = (uint32_t*)dst;
+    for (int i = 0; i < 6; i++) {
+        uint32_t v = u[i];
+        v = (v & 0x80000000u) ? (v & 0x7fffffffu) : ~v;
+        u[i] = v;
+    }
+    return WPBR_OK;
+}
+
+// ---- collision --------------------------------------------------------------
+static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* b,
+                                uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
+                                float* d_hit_t, float* d_bary, bool wait) {
+    if (!ctx || !geom || !b || (n && (!d_rays || !d_hit_shape || !d_hit_t)))
+        return WPBR_ERR_INVALID_ARG;
+    if (!b->built) return set_err(ctx, WPBR_ERR_NOT_BUILT, "lbvh not built");
+    if ((uint64_t)geom->n_spheres + geom->n_triangles != b->n)
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "geometry does not match the lbvh");
+    hipSetDevice(ctx->device);
+    wpbr_status s = ctx_ensure_trace(ctx, n);
+    if (s) return s;
+    TraceArgs a;
+    a.g = to_view(geom);
+    a.trav = (const float4*)b->s.trav;
+    a.prims = (const float4*)b->s.prims;
+    a.root_box = b->s.node_bounds;
+    a.root_is_leaf = b->root_is_leaf;
+    a.n = n;
+    a.rays = d_rays;
+    a.hit_shape = d_hit_shape;
+    a.hit_t = d_hit_t;
+    a.bary = d_bary;
+    a.spill = ctx->spill;
+    a.spill_depth = ctx->spill_depth;
+    a.err = ctx->err_flag;
+    hipEventRecord(ctx->ev0, ctx->stream);
+    s = trace_launch(a, ctx->trace_block, ctx->trace_stack, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, "trace launch failed");
+    if (!wait) return WPBR_OK;
+    return ctx_finish_trace(ctx);
+}
+
+wpbr_status wpbr_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* b, uint32_t n,
+                         const float* d_rays, uint32_t* d_hit_shape, float* d_hit_t, float* d_bary) {
+    return collide_impl(ctx, geom, b, n, d_rays, d_hit_shape, d_hit_t, d_bary, true);
+}
+
+wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* b,
+                               uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
+                               float* d_hit_t, float* d_bary) {
+    return collide_impl(ctx, geom, b, n, d_rays, d_hit_shape, d_hit_t, d_bary, false);
+}
+
+wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth) {
+    if (!ctx) return WPBR_ERR_INVALID_ARG;
+    if (block) {
+        if (block != 64 && block != 128 && block != 256 && block != 512) return WPBR_ERR_INVALID_ARG;
+        ctx->trace_block = block;
+    }
+    if (stack_depth) {
+        if (stack_depth != 8 && stack_depth != 16 && stack_depth != 24 && stack_depth != 32 &&
+            stack_depth != 48 && stack_depth != 64)
+            return WPBR_ERR_INVALID_ARG;
+        ctx->trace_stack = stack_depth;
+    }
+    return WPBR_OK;
+}
+
+}  // extern "C"
+
+namespace wpbr {
+
+wpbr_status ctx_ensure_trace(wpbr_ctx* ctx, uint32_t n) {
+    // Spill capacity so that LDS + spill = 256 entries, the reference's stack.
+    uint32_t depth = 256 - (uint32_t)ctx->trace_stack;
+    size_t need = (size_t)n * depth;
+    if (!ctx->err_flag) {
+        wpbr_status s = hip_check(ctx, hipMalloc(&ctx->err_flag, 16), "hipMalloc");
+        if (s) return s;
+        s = hip_check(ctx, hipMemsetAsync(ctx->err_flag, 0, 16, ctx->stream), "memset");
+        if (s) return s;
+    }
+    if (need > ctx->spill_words || depth != ctx->spill_depth) {
+        if (ctx->spill) hipFree(ctx->spill);
+        ctx->spill = nullptr;
+        ctx->spill_words = 0;
+        wpbr_status s = hip_check(ctx, hipMalloc(&ctx->spill, std::max<size_t>(need, 1) * 4), "hipMalloc spill");
+        if (s) return s;
+        ctx->spill_words = need;
+        ctx->spill_depth = depth;
+    }
+    return WPBR_OK;
+}
+
+wpbr_status ctx_finish_trace(wpbr_ctx* ctx) {
+    uint32_t err = 0;
+    wpbr_status s = hip_check(ctx, hipMemcpyAsync(&err, ctx->err_flag, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "collide");
+    if (s) return s;
+    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
+    if (err) {
+        hipMemsetAsync(ctx->err_flag, 0, 16, ctx->stream);
+        hipStreamSynchronize(ctx->stream);
+        return set_err(ctx, WPBR_ERR_STACK_OVERFLOW, "traversal stack overflow (256 entries)");
+    }
+    return WPBR_OK;
+}
+
+}  // namespace wpbr

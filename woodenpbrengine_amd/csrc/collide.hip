@@ -1,0 +1,207 @@
+// Closest-hit traversal for gfx950 (runCollisionSystem, MEngine.cpp:352-361).
+//
+// The reference collects every primitive of every leaf whose box the ray hits
+// (SLBVH.cpp:85-162), intersects them shape-major (CSTriangle.cpp:9-110,
+// CSSphere.cpp:84-120) and keeps the first strict minimum of t in candidate order
+// (SLBVH.cpp:27-37).  Here one lane owns one ray and walks the same depth-first
+// order (near child first by dirIsNeg[axis], SLBVH.cpp:135-143), testing both
+// child boxes at the parent from one 64-byte record, with closest-hit pruning:
+// a box whose entry distance exceeds the best t found so far (plus 2^-12
+// relative slack) holds no candidate that could win, so skipping it leaves the
+// winner unchanged (later candidates only win with a strictly smaller t).
+// The stack lives in LDS (lane-interleaved, conflict-free); entries beyond its
+// depth spill to a per-thread global array.  The reference's stack holds 256
+// entries and overflows silently; here an overflow is reported.
+#include "wpbr_device.hpp"
+#include "wpbr_internal.hpp"
+
+namespace wpbr {
+
+constexpr float PRUNE_SLACK = 1.0f + 1.0f / 4096.0f;
+
+// pbrt-v3 Bounds3::IntersectP with a precomputed 1/dir and dirIsNeg.
+WPBR_D bool slab_hit(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, f3 o,
+                     f3 inv, bool nx, bool ny, bool nz, float tmax) {
+    const float eps = 5.96046448e-08f;  // machine epsilon / 2
+    const float g3 = (3 * eps) / (1 - 3 * eps);
+    const float scale = 1 + 2 * g3;
+    float tMin = ((nx ? mxx : mnx) - o.x) * inv.x;
+    float tMx = ((nx ? mnx : mxx) - o.x) * inv.x;
+    float tyMin = ((ny ? mxy : mny) - o.y) * inv.y;
+    float tyMax = ((ny ? mny : mxy) - o.y) * inv.y;
+    tMx *= scale;
+    tyMax *= scale;
+    if (tMin > tyMax || tyMin > tMx) return false;
+    if (tyMin > tMin) tMin = tyMin;
+    if (tyMax < tMx) tMx = tyMax;
+    float tzMin = ((nz ? mxz : mnz) - o.z) * inv.z;
+    float tzMax = ((nz ? mnz : mxz) - o.z) * inv.z;
+    tzMax *= scale;
+    if (tMin > tzMax || tzMin > tMx) return false;
+    if (tzMin > tMin) tMin = tzMin;
+    if (tzMax < tMx) tMx = tzMax;
+    return (tMin < tmax) && (tMx > 0);
+}
+
+// CSTriangle.cpp:9-110 (watertight test without tMax and error bounds), with the
+// pbrt-v3 cyclic axis permutation.  Returns 0 for a miss.
+WPBR_D float tri_hit(float4 a, float4 b, float4 c, f3 o, f3 d, int kx, int ky, int kz, float sx,
+                     float sy, float sz, float* bary) {
+    f3 p0 = mk3(a.x - o.x, a.y - o.y, a.z - o.z);
+    f3 p1 = mk3(b.x - o.x, b.y - o.y, b.z - o.z);
+    f3 p2 = mk3(c.x - o.x, c.y - o.y, c.z - o.z);
+    float q0x = comp(p0, kx), q0y = comp(p0, ky), q0z = comp(p0, kz);
+    float q1x = comp(p1, kx), q1y = comp(p1, ky), q1z = comp(p1, kz);
+    float q2x = comp(p2, kx), q2y = comp(p2, ky), q2z = comp(p2, kz);
+    float x0 = q0x + sx * q0z, y0 = q0y + sy * q0z, z0 = q0z + 0.0f * q0z;
+    float x1 = q1x + sx * q1z, y1 = q1y + sy * q1z, z1 = q1z + 0.0f * q1z;
+    float x2 = q2x + sx * q2z, y2 = q2y + sy * q2z, z2 = q2z + 0.0f * q2z;
+    float r0 = x0 * y1 - x1 * y0;
+    float r1 = x1 * y2 - x2 * y1;
+    float r2 = x2 * y0 - x0 * y2;
+    if ((r0 < 0 || r1 < 0 || r2 < 0) && (r0 > 0 || r1 > 0 || r2 > 0)) return 0.0f;
+    float det = (r0 + r1) + r2;
+    if (det == 0) return 0.0f;
+    float tScale = (r1 * (z0 * sz) + r2 * (z1 * sz)) + r0 * (z2 * sz);
+    if (det < 0.0f && tScale >= 0) return 0.0f;
+    if (det > 0.0f && tScale <= 0) return 0.0f;
+    float invDet = 1.0f / det;
+    bary[0] = r1 * invDet;
+    bary[1] = r2 * invDet;
+    bary[2] = r0 * invDet;
+    return tScale * invDet;
+}
+
+// SSphere::intersect (CSSphere.cpp:84-120) with the pbrt-v3 FP64 quadratic.
+WPBR_D float sphere_hit(const GeomView& g, uint32_t id, f3 o, f3 d) {
+    affine inv = load_affine(g.sphere_xform + 24 * (size_t)id + 12);
+    float radius = g.sphere_radius[id];
+    f3 ro = xf_point(inv, o), rd = xf_vector(inv, d);
+    float A = len2(rd);
+    float B = 2 * dot3(rd, ro);
+    float C = len2(ro) - radius * radius;
+    double discrim = (double)B * (double)B - 4 * (double)A * (double)C;
+    if (discrim < 0) return 0.0f;
+    double rdsc = sqrt(discrim);
+    double q = (B < 0) ? -0.5 * (B - rdsc) : -0.5 * (B + rdsc);
+    float t0 = (float)(q / A), t1 = (float)(C / q);
+    if (t0 > t1) { float tmp = t0; t0 = t1; t1 = tmp; }
+    if (t1 <= 0) return 0.0f;
+    return (t0 <= 0) ? t1 : t0;
+}
+
+template <int BLOCK, int SD>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+    __shared__ uint32_t stack[SD * BLOCK];
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t n = a.n;
+    f3 o = mk3(a.rays[i], a.rays[n + i], a.rays[2 * (size_t)n + i]);
+    f3 d = mk3(a.rays[3 * (size_t)n + i], a.rays[4 * (size_t)n + i], a.rays[5 * (size_t)n + i]);
+    f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
+    // Triangle setup shared by every candidate (pbrt-v3 MaxDimension, shear).
+    float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
+    int kz = (adx > ady) ? ((adx > adz) ? 0 : 2) : ((ady > adz) ? 1 : 2);
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    float dkx = comp(d, kx), dky = comp(d, ky), dkz = comp(d, kz);
+    float sx = -dkx / dkz, sy = -dky / dkz, sz = 1.0f / dkz;
+
+    float tBest = INF;
+    uint32_t best = 0xFFFFFFFFu;
+    float bb[3] = {0.0f, 0.0f, 0.0f};
+    const float* rb = a.root_box;
+    uint32_t node = 0xFFFFFFFFu;  // 0xFFFFFFFF = done
+    if (slab_hit(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, inv, nx, ny, nz, INF))
+        node = a.root_is_leaf ? 0x80000000u : 0u;
+    uint32_t sp = 0;
+    const uint32_t spill_base = i;
+    auto push = [&](uint32_t v) {
+        if (sp < (uint32_t)SD) stack[sp * BLOCK + threadIdx.x] = v;
+        else if (sp - SD < a.spill_depth) a.spill[(size_t)(sp - SD) * n + spill_base] = v;
+        else { atomicOr(a.err, 1u); return; }
+        sp++;
+    };
+    auto pop = [&]() -> uint32_t {
+        if (sp == 0) return 0xFFFFFFFFu;
+        sp--;
+        return sp < (uint32_t)SD ? stack[sp * BLOCK + threadIdx.x]
+                                 : a.spill[(size_t)(sp - SD) * n + spill_base];
+    };
+    while (node != 0xFFFFFFFFu) {
+        // Interior nodes.
+        while (!(node & 0x80000000u)) {
+            const float4* r = a.trav + 4 * (size_t)node;
+            float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            float lim = tBest * PRUNE_SLACK;
+            bool h0 = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
+            bool h1 = slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
+            uint32_t w0 = __float_as_uint(r3.x), w1 = __float_as_uint(r3.y);
+            uint32_t axis = __float_as_uint(r3.z);
+            bool negAxis = (axis == 0) ? nx : ((axis == 1) ? ny : nz);
+            uint32_t nearW = negAxis ? w1 : w0, farW = negAxis ? w0 : w1;
+            bool hn = negAxis ? h1 : h0, hf = negAxis ? h0 : h1;
+            if (hn && hf) { push(farW); node = nearW; }
+            else if (hn) node = nearW;
+            else if (hf) node = farW;
+            else { node = pop(); if (node == 0xFFFFFFFFu) break; }
+        }
+        // Leaves.
+        while (node != 0xFFFFFFFFu && (node & 0x80000000u)) {
+            const float4 lr = a.trav[4 * (size_t)(node & 0x7FFFFFFFu) + 3];
+            uint32_t off = __float_as_uint(lr.x), cnt = __float_as_uint(lr.y);
+            for (uint32_t j = off; j < off + cnt; j++) {
+                const float4* p = a.prims + 3 * (size_t)j;
+                float4 v0 = p[0];
+                uint32_t id = __float_as_uint(v0.w);
+                float t;
+                float b3[3] = {0.0f, 0.0f, 0.0f};
+                if (id < a.g.n_spheres) {
+                    t = sphere_hit(a.g, id, o, d);
+                } else {
+                    t = tri_hit(v0, p[1], p[2], o, d, kx, ky, kz, sx, sy, sz, b3);
+                }
+                if (t != 0.0f && t < tBest) {
+                    tBest = t;
+                    best = id;
+                    bb[0] = b3[0]; bb[1] = b3[1]; bb[2] = b3[2];
+                }
+            }
+            node = pop();
+        }
+    }
+    a.hit_shape[i] = best;
+    a.hit_t[i] = tBest;
+    if (a.bary) {
+        a.bary[i] = bb[0];
+        a.bary[(size_t)n + i] = bb[1];
+        a.bary[2 * (size_t)n + i] = bb[2];
+    }
+}
+
+template <int BLOCK>
+static void launch_sd(const TraceArgs& a, int sd, hipStream_t st) {
+    dim3 grid((a.n + BLOCK - 1) / BLOCK);
+    switch (sd) {
+        case 8: hipLaunchKernelGGL((k_trace<BLOCK, 8>), grid, dim3(BLOCK), 0, st, a); break;
+        case 16: hipLaunchKernelGGL((k_trace<BLOCK, 16>), grid, dim3(BLOCK), 0, st, a); break;
+        case 24: hipLaunchKernelGGL((k_trace<BLOCK, 24>), grid, dim3(BLOCK), 0, st, a); break;
+        case 32: hipLaunchKernelGGL((k_trace<BLOCK, 32>), grid, dim3(BLOCK), 0, st, a); break;
+        case 48: hipLaunchKernelGGL((k_trace<BLOCK, 48>), grid, dim3(BLOCK), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_trace<BLOCK, 64>), grid, dim3(BLOCK), 0, st, a); break;
+    }
+}
+
+wpbr_status trace_launch(const TraceArgs& a, int block, int sd, hipStream_t st) {
+    if (a.n == 0) return WPBR_OK;
+    switch (block) {
+        case 64: launch_sd<64>(a, sd, st); break;
+        case 128: launch_sd<128>(a, sd, st); break;
+        case 512: launch_sd<512>(a, sd, st); break;
+        default: launch_sd<256>(a, sd, st); break;
+    }
+    return hipGetLastError() == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+}
+
+}  // namespace wpbr

@@ -1,0 +1,48 @@
+// Host-side context and object definitions behind the opaque C handles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include "wpbr_internal.hpp"
+
+struct wpbr_render_state;
+
+struct wpbr_ctx {
+    int device = 0;
+    int num_cus = 256;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last_ms = 0.0f;
+    std::string last_error;
+    // traversal
+    int trace_block = 128;
+    int trace_stack = 32;
+    uint32_t* spill = nullptr;
+    size_t spill_words = 0;
+    uint32_t spill_depth = 0;
+    uint32_t* err_flag = nullptr;
+    wpbr_render_state* render = nullptr;
+};
+
+struct wpbr_lbvh {
+    static constexpr int NEV = 12;
+    wpbr_ctx* ctx = nullptr;
+    void* mem = nullptr;
+    uint32_t cap = 0;
+    uint32_t n = 0, n_nodes = 0, n_treelets = 0;
+    bool root_is_leaf = false;
+    bool built = false;
+    wpbr::LbvhScratch s{};
+    wpbr::GeomView geom{};
+    hipEvent_t ev[NEV] = {};
+};
+
+namespace wpbr {
+wpbr_status set_err(wpbr_ctx* ctx, wpbr_status s, const std::string& msg);
+wpbr_status hip_check(wpbr_ctx* ctx, hipError_t e, const char* what);
+GeomView to_view(const wpbr_geometry* g);
+wpbr_status ctx_ensure_trace(wpbr_ctx* ctx, uint32_t n);
+wpbr_status ctx_finish_trace(wpbr_ctx* ctx);
+void ctx_release_render(wpbr_ctx* ctx);
+wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroid, uint32_t* cb_ord,
+                                hipStream_t st);
+}  // namespace wpbr

@@ -1,0 +1,78 @@
+// Internal types shared by the HIP translation units of libwpbr_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <algorithm>
+#include <string>
+#include <vector>
+#include "../../include/wpbr.h"
+
+namespace wpbr {
+
+struct GeomView {
+    uint32_t n_spheres;
+    const float* sphere_xform;
+    const float* sphere_radius;
+    uint32_t n_triangles;
+    const float* tri_verts;
+};
+
+// Device buffers of one LBVH build (all pointers device memory).
+struct LbvhScratch {
+    // a1-a3
+    float* bounds;        // n x 6 (shape order)
+    float* centroid;      // n x 3
+    uint32_t* cb_ord;     // 6: centroid bounds as order-preserving uints
+    uint32_t* morton;     // n (shape order)
+    // a4 radix sort
+    uint32_t* keys_a; uint32_t* vals_a;  // final sorted (code, shape index)
+    uint32_t* keys_b; uint32_t* vals_b;
+    uint32_t* hist;       // 4 x 256
+    uint32_t* status0; uint32_t* status1;
+    int status_words;
+    uint32_t* tile_counter;  // 8
+    uint32_t* err;
+    // treelets (<= 4096)
+    uint32_t* bin_start;  // 4096
+    uint32_t* presence;   // 128 words
+    uint32_t* t_start; uint32_t* t_count; uint32_t* t_slot; uint32_t* t_nodes;
+    uint32_t* t_final; float* t_box;
+    // emit records, indexed by the treelet's reserved slot range (2n)
+    uint32_t* r_start; uint32_t* r_n; uint32_t* r_c0; uint32_t* r_meta;
+    uint32_t* r_size; uint32_t* r_off; float* r_box;
+    // upper SAH
+    uint32_t* perm; uint8_t* bucket; uint32_t* list_a; uint32_t* list_b;
+    uint32_t* task_i0; uint32_t* task_i1; uint32_t* task_parent; uint32_t* task_depth;
+    uint32_t* u_c0; uint32_t* u_c1; uint32_t* u_axis; uint32_t* u_depth; uint32_t* u_size;
+    uint32_t* u_off; float* u_box;
+    uint32_t* upper_count; uint32_t* upper_root;
+    uint32_t* n_nodes; uint32_t* n_treelets;
+    // final CSceneTreeLBVH arrays (capacity 2n)
+    uint32_t* node_offset; uint32_t* node_nshapes; uint8_t* node_axis; float* node_bounds;
+    // traversal layout
+    void* trav;           // 64 B per node
+    void* prims;          // 48 B per shape, leaf order
+};
+
+wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st,
+                        hipEvent_t* ev, int nev);
+
+struct TraceArgs {
+    GeomView g;
+    const float4* trav;
+    const float4* prims;
+    const float* root_box;
+    uint32_t root_is_leaf;
+    uint32_t n;
+    const float* rays;
+    uint32_t* hit_shape;
+    float* hit_t;
+    float* bary;
+    uint32_t* spill;      // per-thread overflow stack
+    uint32_t spill_depth;
+    uint32_t* err;        // bit 0: stack overflow
+};
+
+wpbr_status trace_launch(const TraceArgs& a, int block, int stack_depth, hipStream_t st);
+
+}  // namespace wpbr

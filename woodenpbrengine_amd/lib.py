@@ -1,0 +1,235 @@
+"""ctypes binding of libwpbr_hip.so (the C ABI declared in include/wpbr.h).
+
+The library is the product: every call here goes to HIP kernels on the GPU.
+There is no CPU fallback; if the shared object is missing or no GPU is visible
+the calls raise.
+"""
+import ctypes as C
+import numpy as np
+
+from . import LIB_PATH
+
+WPBR_OK = 0
+STATUS = {0: "ok", 1: "invalid argument", 2: "out of device memory", 3: "HIP runtime error",
+          4: "traversal stack overflow", 5: "device-side wait timed out",
+          6: "acceleration structure not built"}
+
+# Exported symbols (kept in step with include/wpbr.h; tests check both).
+EXPORTS = [
+    "wpbr_open", "wpbr_close", "wpbr_status_string", "wpbr_last_error", "wpbr_malloc",
+    "wpbr_free", "wpbr_upload", "wpbr_download", "wpbr_sync", "wpbr_last_time_ms",
+    "wpbr_shape_bounds", "wpbr_lbvh_create", "wpbr_lbvh_destroy", "wpbr_lbvh_build",
+    "wpbr_lbvh_counts", "wpbr_lbvh_download", "wpbr_lbvh_kernel_times", "wpbr_collide",
+    "wpbr_collide_async", "wpbr_set_trace_config",
+]
+
+
+class WpbrError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__("wpbr status %d (%s)%s" % (status, STATUS.get(status, "?"),
+                                                    (": " + msg) if msg else ""))
+        self.status = status
+
+
+class Geometry(C.Structure):
+    _fields_ = [("n_spheres", C.c_uint32), ("d_sphere_xform", C.c_void_p),
+                ("d_sphere_radius", C.c_void_p), ("n_triangles", C.c_uint32),
+                ("d_tri_verts", C.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    """Load libwpbr_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise WpbrError(3, "%s is missing: run __graft_entry__.build()" % LIB_PATH)
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u32, i32, sz = C.c_void_p, C.c_uint32, C.c_int32, C.c_size_t
+    sig = {
+        "wpbr_open": (i32, [C.c_int, C.POINTER(vp)]),
+        "wpbr_close": (None, [vp]),
+        "wpbr_status_string": (C.c_char_p, [i32]),
+        "wpbr_last_error": (C.c_char_p, [vp]),
+        "wpbr_malloc": (i32, [vp, sz, C.POINTER(vp)]),
+        "wpbr_free": (i32, [vp, vp]),
+        "wpbr_upload": (i32, [vp, vp, vp, sz]),
+        "wpbr_download": (i32, [vp, vp, vp, sz]),
+        "wpbr_sync": (i32, [vp]),
+        "wpbr_last_time_ms": (C.c_float, [vp]),
+        "wpbr_shape_bounds": (i32, [vp, C.POINTER(Geometry), vp, vp]),
+        "wpbr_lbvh_create": (i32, [vp, C.POINTER(vp)]),
+        "wpbr_lbvh_destroy": (None, [vp]),
+        "wpbr_lbvh_build": (i32, [vp, C.POINTER(Geometry), vp]),
+        "wpbr_lbvh_counts": (i32, [vp, C.POINTER(u32)]),
+        "wpbr_lbvh_download": (i32, [vp, vp, C.c_int, vp, sz]),
+        "wpbr_lbvh_kernel_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
+        "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
+        "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
+        "wpbr_set_trace_config": (i32, [vp, C.c_int, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceArray:
+    """A device allocation holding a numpy-typed array."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if isinstance(shape, (tuple, list)) else (int(shape),)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = ctx.malloc(self.nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes == self.nbytes, (arr.nbytes, self.nbytes)
+        self.ctx.check(self.ctx.lib.wpbr_upload(self.ctx.h, self.ptr, _ptr(arr), arr.nbytes))
+        return self
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        self.ctx.check(self.ctx.lib.wpbr_download(self.ctx.h, _ptr(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.lib.wpbr_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        st = self.lib.wpbr_open(device, C.byref(h))
+        if st != WPBR_OK:
+            raise WpbrError(st, "wpbr_open(%d)" % device)
+        self.h = h
+
+    def check(self, st):
+        if st != WPBR_OK:
+            raise WpbrError(st, self.lib.wpbr_last_error(self.h).decode(errors="replace"))
+
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.lib.wpbr_malloc(self.h, max(1, int(nbytes)), C.byref(p)))
+        return p
+
+    def array(self, arr, dtype=None):
+        arr = np.ascontiguousarray(arr, dtype=dtype)
+        return DeviceArray(self, arr.shape, arr.dtype).upload(arr)
+
+    def empty(self, shape, dtype):
+        return DeviceArray(self, shape, dtype)
+
+    def sync(self):
+        self.check(self.lib.wpbr_sync(self.h))
+
+    def last_ms(self):
+        return float(self.lib.wpbr_last_time_ms(self.h))
+
+    def set_trace_config(self, block=0, stack_depth=0):
+        self.check(self.lib.wpbr_set_trace_config(self.h, block, stack_depth))
+
+    def close(self):
+        if self.h:
+            self.lib.wpbr_close(self.h)
+            self.h = None
+
+
+class Scene:
+    """Device-resident geometry: spheres (world + inverse 3x4) then triangles."""
+
+    def __init__(self, ctx, sphere_xform, sphere_radius, tri_verts):
+        self.ctx = ctx
+        self.sphere_xform = np.ascontiguousarray(sphere_xform, np.float32).reshape(-1, 24)
+        self.sphere_radius = np.ascontiguousarray(sphere_radius, np.float32).reshape(-1)
+        self.tri_verts = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+        ns, nt = len(self.sphere_radius), len(self.tri_verts)
+        self.d_sx = ctx.array(self.sphere_xform if ns else np.zeros((1, 24), np.float32))
+        self.d_sr = ctx.array(self.sphere_radius if ns else np.zeros(1, np.float32))
+        self.d_tv = ctx.array(self.tri_verts if nt else np.zeros((1, 9), np.float32))
+        self.geom = Geometry(ns, self.d_sx.ptr, self.d_sr.ptr, nt, self.d_tv.ptr)
+
+    @property
+    def n_shapes(self):
+        return self.geom.n_spheres + self.geom.n_triangles
+
+
+class Lbvh:
+    FIELDS = {"morton": (0, np.uint32), "sorted_code": (1, np.uint32),
+              "sorted_index": (2, np.uint32), "node_offset": (3, np.uint32),
+              "node_nshapes": (4, np.uint32), "node_axis": (5, np.uint8),
+              "node_bounds": (6, np.float32), "centroid_bounds": (7, np.float32),
+              "treelet_start": (8, np.uint32)}
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wpbr_lbvh_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def build(self, scene):
+        self.ctx.check(self.ctx.lib.wpbr_lbvh_build(self.ctx.h, C.byref(scene.geom), self.h))
+        return self.ctx.last_ms()
+
+    def counts(self):
+        c = (C.c_uint32 * 4)()
+        self.ctx.check(self.ctx.lib.wpbr_lbvh_counts(self.h, c))
+        return {"shapes": c[0], "nodes": c[1], "sah_nodes": c[2], "treelets": c[3]}
+
+    def kernel_times(self):
+        buf = (C.c_float * 16)()
+        k = self.ctx.lib.wpbr_lbvh_kernel_times(self.h, buf, 16)
+        return [float(buf[i]) for i in range(k)]
+
+    def download(self, name):
+        field, dt = self.FIELDS[name]
+        c = self.counts()
+        count = {"morton": c["shapes"], "sorted_code": c["shapes"], "sorted_index": c["shapes"],
+                 "node_offset": c["nodes"], "node_nshapes": c["nodes"], "node_axis": c["nodes"],
+                 "node_bounds": 6 * c["nodes"], "centroid_bounds": 6,
+                 "treelet_start": c["treelets"]}[name]
+        out = np.empty(count, dtype=dt)
+        self.ctx.check(self.ctx.lib.wpbr_lbvh_download(self.ctx.h, self.h, field, _ptr(out),
+                                                       out.nbytes))
+        if name in ("node_bounds", "centroid_bounds"):
+            out = out.reshape(-1, 6)
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.wpbr_lbvh_destroy(self.h)
+            self.h = None
+
+
+def collide(ctx, scene, bvh, d_rays, n, want_bary=False, wait=True):
+    """Closest hit of n SoA rays (device array of 6*n floats).  Returns device
+    arrays (shape id, t, barycentrics or None)."""
+    d_shape = ctx.empty(n, np.uint32)
+    d_t = ctx.empty(n, np.float32)
+    d_b = ctx.empty((3, n), np.float32) if want_bary else None
+    fn = ctx.lib.wpbr_collide if wait else ctx.lib.wpbr_collide_async
+    ctx.check(fn(ctx.h, C.byref(scene.geom), bvh.h, n, d_rays.ptr, d_shape.ptr, d_t.ptr,
+                 d_b.ptr if d_b is not None else None))
+    return d_shape, d_t, d_b
