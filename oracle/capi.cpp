@@ -171,3 +171,126 @@ void orc_partition_check(uint32_t n, const uint8_t* buckets, int best, uint32_t*
 }
 
 }  // extern "C"
+
+// ---- render -------------------------------------------------------------------
+// Scene description; the layout equals wpbr_scene_desc in include/wpbr.h.
+struct orc_scene_desc {
+    uint32_t n_spheres;
+    const float* sphere_xform;
+    const float* sphere_radius;
+    uint32_t n_triangles;
+    const float* tri_verts;
+    const uint8_t* tri_flags;
+    const float* tri_uv;
+    const float* tri_normal;
+    const float* tri_tangent;
+    const uint32_t* shape_material;
+    uint32_t n_materials;
+    const float* materials;  // 9 floats each: type(bits), alpha, eta, eta_rgb[3], k_rgb[3]
+    uint32_t n_sphere_lights;
+    const uint32_t* light_shape;
+    const float* light_emit;
+    uint32_t has_env;
+    const float* env_emit;
+    const float* env_rgb;
+    uint32_t env_w, env_h;
+    const float* env_xform;
+    const float* raster_camera;
+    const float* camera_world;
+    uint32_t width, height, spp, seed;
+};
+
+extern "C" {
+
+void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, uint32_t n_tiles) {
+    RenderScene sc;
+    sc.geom.n_spheres = d->n_spheres;
+    sc.geom.sphere_xform = d->sphere_xform;
+    sc.geom.sphere_radius = d->sphere_radius;
+    sc.geom.n_triangles = d->n_triangles;
+    sc.geom.tri_verts = d->tri_verts;
+    sc.tri_flags = d->tri_flags;
+    sc.tri_uv = d->tri_uv;
+    sc.tri_normal = d->tri_normal;
+    sc.tri_tangent = d->tri_tangent;
+    sc.shape_material = d->shape_material;
+    std::vector<Material> mats(d->n_materials);
+    for (uint32_t i = 0; i < d->n_materials; i++) {
+        const float* m = d->materials + 9 * (size_t)i;
+        std::memcpy(&mats[i].type, &m[0], 4);
+        mats[i].alpha = m[1];
+        mats[i].eta = m[2];
+        for (int c = 0; c < 3; c++) { mats[i].eta_rgb[c] = m[3 + c]; mats[i].k_rgb[c] = m[6 + c]; }
+    }
+    sc.n_materials = d->n_materials;
+    sc.materials = mats.data();
+    sc.n_sphere_lights = d->n_sphere_lights;
+    sc.light_shape = d->light_shape;
+    sc.light_emit = d->light_emit;
+    sc.has_env = (int)d->has_env;
+    sc.env_emit = d->env_emit;
+    sc.env_rgb = d->env_rgb;
+    sc.env_w = d->env_w;
+    sc.env_h = d->env_h;
+    sc.env_xform = d->env_xform;
+    sc.raster_camera = d->raster_camera;
+    sc.camera_world = d->camera_world;
+    sc.width = d->width;
+    sc.height = d->height;
+    sc.spp = d->spp;
+    sc.seed = d->seed;
+    uint32_t n = d->n_spheres + d->n_triangles;
+    std::vector<float> b(6 * (size_t)n), c(3 * (size_t)n);
+    sphere_bounds(d->n_spheres, d->sphere_xform, d->sphere_radius, b.data(), c.data());
+    triangle_bounds(d->n_triangles, d->tri_verts, b.data() + 6 * (size_t)d->n_spheres,
+                    c.data() + 3 * (size_t)d->n_spheres);
+    LbvhResult bvh;
+    lbvh_build(n, b.data(), c.data(), nullptr, bvh);
+    RenderOut* out = new RenderOut();
+    render(sc, bvh, *out, nthreads, tiles, n_tiles);
+    return out;
+}
+
+// field: 0 sample_xyz, 1 sample_pfilm, 2 film_rgb, 3 film_xyz, 4 film_w, 5 counters (u64 x 8)
+int orc_render_copy(void* h, int field, void* dst) {
+    RenderOut* o = (RenderOut*)h;
+    switch (field) {
+        case 0: std::memcpy(dst, o->sample_xyz.data(), 4 * o->sample_xyz.size()); return 0;
+        case 1: std::memcpy(dst, o->sample_pfilm.data(), 4 * o->sample_pfilm.size()); return 0;
+        case 2: std::memcpy(dst, o->film_rgb.data(), 4 * o->film_rgb.size()); return 0;
+        case 3: std::memcpy(dst, o->film_xyz.data(), 4 * o->film_xyz.size()); return 0;
+        case 4: std::memcpy(dst, o->film_w.data(), 4 * o->film_w.size()); return 0;
+        case 5: std::memcpy(dst, o->counters, 64); return 0;
+    }
+    return -1;
+}
+
+void orc_render_free(void* h) { delete (RenderOut*)h; }
+
+// Spectrum tables and conversions (for the pinning tests).
+void orc_spectrum_tables(float* out) {  // 3 x 64 XYZ, 7 x 64 refl, 7 x 64 illum
+    const Tables& T = tables();
+    std::memcpy(out, T.X, 256);
+    std::memcpy(out + 64, T.Y, 256);
+    std::memcpy(out + 128, T.Z, 256);
+    std::memcpy(out + 192, T.refl, 7 * 256);
+    std::memcpy(out + 192 + 448, T.illum, 7 * 256);
+}
+
+void orc_from_rgb(const float* rgb, int illuminant, float* out64) {
+    Spec s = from_rgb(rgb, illuminant != 0);
+    std::memcpy(out64, s.data(), 256);
+}
+
+void orc_to_xyz(const float* s64, float* xyz) {
+    Spec s;
+    std::memcpy(s.data(), s64, 256);
+    to_xyz(s, xyz);
+}
+
+void orc_samples(uint32_t seed, uint32_t pixel, uint32_t n1, uint32_t n2, float* u1, float* u2) {
+    for (uint32_t i = 0; i < n1; i++) u1[i] = sample_1d(seed, pixel, i);
+    for (uint32_t i = 0; i < n2; i++) sample_2d(seed, pixel, i, u2 + 2 * i);
+}
+
+}  // extern "C"

@@ -21,6 +21,7 @@
 //   * shading          - closed forms listed in shade.cpp.
 #pragma once
 #include "common.h"
+#include <array>
 #include <vector>
 
 namespace orc {
@@ -80,5 +81,68 @@ float tri_intersect(const float* v, V3 o, V3 d, float* bary);
 float sphere_intersect(const float* xform, float radius, V3 o, V3 d, V3* oL, V3* dL);
 int64_t trace_ray(const Scene& sc, const LbvhResult& bvh, V3 o, V3 d, float* tOut,
                   uint64_t* nCandidates, int* stackOverflow);
+
+// ---- spectrum (spectrum.cpp) ----------------------------------------------
+constexpr int NSPEC = 64;
+using Spec = std::array<float, NSPEC>;
+struct Tables {
+    float X[NSPEC], Y[NSPEC], Z[NSPEC];
+    float refl[7][NSPEC], illum[7][NSPEC];  // white, cyan, magenta, yellow, red, green, blue
+};
+const Tables& tables();
+float average_spectrum_samples(const float* lambda, const float* vals, int n, float l0, float l1);
+Spec from_rgb(const float* rgb, bool illuminant);
+void to_xyz(const Spec& s, float* xyz);
+void xyz_to_rgb(const float* xyz, float* rgb);
+
+// ---- render (render.cpp) ----------------------------------------------------
+enum MaterialType : uint32_t { MAT_CONDUCTOR = 0, MAT_DIELECTRIC = 1, MAT_SPECULAR = 2 };
+struct Material {
+    uint32_t type;
+    float alpha;       // alphax == alphay
+    float eta;         // dielectric etaT (etaI = 1)
+    float eta_rgb[3];  // conductor etaT as RGB (fromRGB Reflectance)
+    float k_rgb[3];    // conductor k as RGB (fromRGB Reflectance)
+};
+
+struct RenderScene {
+    Scene geom;
+    // per-triangle shading data (optional, may be null)
+    const uint8_t* tri_flags = nullptr;   // bit0: uvs, bit1: normals + tangents
+    const float* tri_uv = nullptr;        // nt x 6
+    const float* tri_normal = nullptr;    // nt x 9
+    const float* tri_tangent = nullptr;   // nt x 9
+    const uint32_t* shape_material = nullptr;  // per shape; 0xFFFFFFFF = none
+    uint32_t n_materials = 0;
+    const Material* materials = nullptr;
+    // lights: sphere lights first, then the env light (if any)
+    uint32_t n_sphere_lights = 0;
+    const uint32_t* light_shape = nullptr;     // shape id of each sphere light
+    const float* light_emit = nullptr;         // n_sphere_lights x 64
+    int has_env = 0;
+    const float* env_emit = nullptr;           // 64
+    const float* env_rgb = nullptr;            // env_h x env_w x 3
+    uint32_t env_w = 0, env_h = 0;
+    const float* env_xform = nullptr;          // world 3x4 then inverse 3x4
+    // camera
+    const float* raster_camera = nullptr;      // 4x4 row-major
+    const float* camera_world = nullptr;       // 3x4 row-major
+    uint32_t width = 0, height = 0, spp = 0;
+    uint32_t seed = 0;
+};
+
+struct RenderOut {
+    std::vector<float> sample_xyz;   // width*height*spp x 3, index (pixel*spp + s)
+    std::vector<float> sample_pfilm; // x 2
+    std::vector<float> film_rgb;     // width*height x 3 (toRGB(sum L w) / sum w)
+    std::vector<float> film_xyz;     // width*height x 3 (sum of XYZ(L) w)
+    std::vector<float> film_w;
+    uint64_t counters[8] = {0};      // primary hits, misses, shadow rays, bsdf rays, ...
+};
+
+float sample_1d(uint32_t seed, uint32_t pixel, uint32_t index);
+void sample_2d(uint32_t seed, uint32_t pixel, uint32_t index, float* u);
+void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
+            const uint32_t* tile_subset, uint32_t n_tiles);
 
 }  // namespace orc

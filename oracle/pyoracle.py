@@ -176,3 +176,93 @@ def partition_check(buckets, best):
     mids = np.empty(2, np.uint32)
     lib.orc_partition_check(n, _p(b), int(best), _p(a1), _p(a2), _p(mids))
     return a1, a2, mids
+
+
+# ---- render -------------------------------------------------------------------
+class SceneDesc(C.Structure):
+    _fields_ = [("n_spheres", C.c_uint32), ("sphere_xform", C.c_void_p),
+                ("sphere_radius", C.c_void_p), ("n_triangles", C.c_uint32),
+                ("tri_verts", C.c_void_p), ("tri_flags", C.c_void_p), ("tri_uv", C.c_void_p),
+                ("tri_normal", C.c_void_p), ("tri_tangent", C.c_void_p),
+                ("shape_material", C.c_void_p), ("n_materials", C.c_uint32),
+                ("materials", C.c_void_p), ("n_sphere_lights", C.c_uint32),
+                ("light_shape", C.c_void_p), ("light_emit", C.c_void_p),
+                ("has_env", C.c_uint32), ("env_emit", C.c_void_p), ("env_rgb", C.c_void_p),
+                ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env_xform", C.c_void_p),
+                ("raster_camera", C.c_void_p), ("camera_world", C.c_void_p),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("seed", C.c_uint32)]
+
+
+def scene_desc(s):
+    """Host-pointer scene description of a woodenpbrengine_amd.scenes.RenderScene.
+    Returns (desc, keepalive)."""
+    keep = [np.ascontiguousarray(a) for a in (
+        s.sphere_xform, s.sphere_radius, s.tri_verts, s.tri_flags, s.tri_uv, s.tri_normal,
+        s.tri_tangent, s.shape_material, s.material_table, s.light_shape, s.light_emit,
+        s.env_emit, s.env_rgb, s.env_xform, s.raster_camera, s.camera_world)]
+    (sx, sr, tv, tf, tuv, tn, tt, sm, mt, ls, le, ee, er, ex, rc, cw) = keep
+    d = SceneDesc(len(sr), _p(sx), _p(sr), len(tv), _p(tv), _p(tf), _p(tuv), _p(tn), _p(tt),
+                  _p(sm), len(mt), _p(mt), len(ls), _p(ls), _p(le),
+                  1 if s.env is not None else 0, _p(ee), _p(er), er.shape[1], er.shape[0],
+                  _p(ex), _p(rc), _p(cw), s.width, s.height, s.spp, s.seed)
+    return d, keep
+
+
+def render(s, nthreads=None, tiles=None):
+    """Render a scene on the CPU; returns a dict of numpy arrays."""
+    lib = load()
+    lib.orc_render.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_uint32]
+    lib.orc_render.restype = C.c_void_p
+    lib.orc_render_copy.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    lib.orc_render_free.argtypes = [C.c_void_p]
+    d, keep = scene_desc(s)
+    t = None if tiles is None else np.ascontiguousarray(tiles, np.uint32)
+    h = lib.orc_render(C.byref(d), nthreads or default_threads(), _p(t), 0 if t is None else len(t))
+    W, H, spp = s.width, s.height, s.spp
+    out = {"sample_xyz": np.empty((H * W * spp, 3), np.float32),
+           "sample_pfilm": np.empty((H * W * spp, 2), np.float32),
+           "film_rgb": np.empty((H, W, 3), np.float32),
+           "film_xyz": np.empty((H, W, 3), np.float32),
+           "film_w": np.empty((H, W), np.float32),
+           "counters": np.empty(8, np.uint64)}
+    for i, k in enumerate(["sample_xyz", "sample_pfilm", "film_rgb", "film_xyz", "film_w",
+                           "counters"]):
+        lib.orc_render_copy(h, i, _p(out[k]))
+    lib.orc_render_free(h)
+    del keep
+    return out
+
+
+def spectrum_tables():
+    lib = load()
+    lib.orc_spectrum_tables.argtypes = [C.c_void_p]
+    out = np.empty(192 + 2 * 448, np.float32)
+    lib.orc_spectrum_tables(_p(out))
+    return {"X": out[:64], "Y": out[64:128], "Z": out[128:192],
+            "refl": out[192:640].reshape(7, 64), "illum": out[640:].reshape(7, 64)}
+
+
+def from_rgb(rgb, illuminant=False):
+    lib = load()
+    lib.orc_from_rgb.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    out = np.empty(64, np.float32)
+    lib.orc_from_rgb(_p(_f32(rgb)), 1 if illuminant else 0, _p(out))
+    return out
+
+
+def to_xyz(spec):
+    lib = load()
+    lib.orc_to_xyz.argtypes = [C.c_void_p, C.c_void_p]
+    out = np.empty(3, np.float32)
+    lib.orc_to_xyz(_p(_f32(spec)), _p(out))
+    return out
+
+
+def samples(seed, pixel, n1, n2):
+    lib = load()
+    lib.orc_samples.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    u1 = np.empty(n1, np.float32)
+    u2 = np.empty((n2, 2), np.float32)
+    lib.orc_samples(seed, pixel, n1, n2, _p(u1), _p(u2))
+    return u1, u2

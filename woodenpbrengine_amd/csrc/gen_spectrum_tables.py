@@ -45,18 +45,25 @@ def parse(text):
     return out
 
 
+def lit(v):
+    t = "%.9g" % v
+    if not any(ch in t for ch in ".eEn"):
+        t += ".0"
+    return t + "f"
+
+
 def emit(tables):
     lines = ["// Generated by gen_spectrum_tables.py from the reference's SpectrumData.h",
              "// (CIE 1931 2-degree observer, 1 nm; Smits RGB->spectrum bases). Data only.",
              "#pragma once",
              "#define WPBR_N_CIE 471",
              "#define WPBR_N_RGB2SPECT 32",
-             "static const float WPBR_CIE_Y_INTEGRAL = %.9gf;" % tables["CIE_Y_integral"]]
+             "static const float WPBR_CIE_Y_INTEGRAL = %s;" % lit(tables["CIE_Y_integral"])]
     for name in NAMES:
         vals = tables[name]
         lines.append("static const float WPBR_%s[%d] = {" % (name, len(vals)))
         for i in range(0, len(vals), 6):
-            lines.append("  " + ", ".join("%.9gf" % v for v in vals[i:i + 6]) + ",")
+            lines.append("  " + ", ".join(lit(v) for v in vals[i:i + 6]) + ",")
         lines.append("};")
     return "\n".join(lines) + "\n"
 

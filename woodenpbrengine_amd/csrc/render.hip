@@ -1,0 +1,645 @@
+// Wavefront renderer for gfx950: MEngine::render (MEngine.cpp:380-448), direct
+// lighting with MIS, all film tiles in one launch per sample pass.
+//
+// One pass per sample index j (the per-pixel sample cursors of CSampler.h make
+// pass j depend on pass j-1), each pass five kernels over all pixels:
+//   k_camera  JobSamplerUpdateCameraSamples + JobCameraPerspGenerateRaysDifferential
+//             (CSSampler.cpp:7-24, CSCameraPerspective.cpp:9-52)
+//   trace     runCollisionSystem for primary rays (collide.hip)
+//   k_shade1  emitted light of misses (SScattering.cpp:20-80), surface interaction
+//             (CSTriangle.cpp:223-393 / CSSphere.cpp:397-461), BSDF request + frame
+//             (CMaterial.h:30-38, CBSDF.cpp:139-151), light pick and sample
+//             (SScattering.cpp:214-228, CSSphere.h:187-230), BSDF Compute toward the
+//             light and BSDF Sample (CBSDF*.cpp); shadow and BSDF rays go to one
+//             queue compacted with wave ballots (one atomic per wave)
+//   trace     both secondary rays of every sample in one launch
+//   k_shade2  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:206-237), both
+//             MIS integrations (SScattering.cpp:259-386) over the 64 wavelengths,
+//             XYZ and the Mitchell splat into the sample's 32x32 tile (CSFilm.cpp:199-251)
+// The reference's light sample for the env light stores a black Li
+// (CLightInfiniteArea.cpp:192-199): that strategy adds nothing, so its shadow
+// ray is not cast (the 2D sample is still consumed).
+#include "wpbr_device.hpp"
+#include "render_types.hpp"
+
+namespace wpbr {
+
+// ---- sample streams (counter-based, lowbias32 chain) ------------------------
+WPBR_D uint32_t mix32(uint32_t x) {
+    x ^= x >> 16; x *= 0x7feb352dU; x ^= x >> 15; x *= 0x846ca68bU; x ^= x >> 16;
+    return x;
+}
+WPBR_D float hash_unit(uint32_t seed, uint32_t pixel, uint32_t key) {
+    uint32_t h = mix32(seed ^ 0x9E3779B9u);
+    h = mix32(h ^ pixel);
+    h = mix32(h ^ key);
+    return (float)(h >> 8) * (1.0f / 16777216.0f);
+}
+WPBR_D float next1d(uint32_t seed, uint32_t pixel, uint32_t& c1) { return hash_unit(seed, pixel, 3u * c1++); }
+WPBR_D void next2d(uint32_t seed, uint32_t pixel, uint32_t& c2, float* u) {
+    u[0] = hash_unit(seed, pixel, 3u * c2 + 1u);
+    u[1] = hash_unit(seed, pixel, 3u * c2 + 2u);
+    c2++;
+}
+
+// ---- math helpers ---------------------------------------------------------------
+WPBR_D float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
+WPBR_D f3 xf_point4(const float* m, f3 p) {
+    float x = ((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3];
+    float y = ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7];
+    float z = ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11];
+    float w = ((m[12] * p.x + m[13] * p.y) + m[14] * p.z) + m[15];
+    if (w == 1.0f) return mk3(x, y, z);
+    return mk3(x / w, y / w, z / w);
+}
+WPBR_D float spherical_theta(f3 v) { return cr_acos(clampf(v.z, -1.0f, 1.0f)); }
+WPBR_D float spherical_phi(f3 v) {
+    float p = cr_atan2(v.y, v.x);
+    return (p < 0) ? (float)((double)p + 2.0 * 3.14159265358979323846) : p;
+}
+WPBR_D void coordinate_system(f3 v1, f3* v2, f3* v3o) {
+    if (fabsf(v1.x) > fabsf(v1.y)) *v2 = div3(mk3(-v1.z, 0, v1.x), sqrtf(v1.x * v1.x + v1.z * v1.z));
+    else *v2 = div3(mk3(0, v1.z, -v1.y), sqrtf(v1.y * v1.y + v1.z * v1.z));
+    *v3o = cross3(v1, *v2);
+}
+WPBR_D float cone_pdf(float cosThetaMax) { return 1 / (2 * PI_TRUE_F * (1 - cosThetaMax)); }
+WPBR_D bool is_zero(f3 v) { return v.x == 0.0f && v.y == 0.0f && v.z == 0.0f; }
+
+// ---- env light -------------------------------------------------------------------
+WPBR_D void env_bilerp(const SceneView& sv, float s_, float t_, float* rgb) {
+    int W = (int)sv.env_w, H = (int)sv.env_h;
+    float s = s_ * (float)W - 0.5f;
+    float t = t_ * (float)H - 0.5f;
+    int s0 = (int)floorf(s), t0 = (int)floorf(t);
+    s0 = max(s0, 0); t0 = max(t0, 0);
+    s0 = min(s0, W - 1); t0 = min(t0, H - 1);
+    float ds = s - (float)s0, dt = t - (float)t0;
+    int s0n = max(min(s0 + 1, W - 1), 0);
+    int t0n = max(min(t0 + 1, H - 1), 0);
+    const float* a = sv.env_rgb + ((size_t)t0 * W + s0) * 3;
+    const float* b = sv.env_rgb + ((size_t)t0n * W + s0) * 3;
+    const float* c = sv.env_rgb + ((size_t)t0 * W + s0n) * 3;
+    const float* d = sv.env_rgb + ((size_t)t0n * W + s0n) * 3;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float v = ((a[k] * (1 - ds)) * (1 - dt) + (b[k] * (1 - ds)) * dt) + (c[k] * ds) * (1 - dt);
+        rgb[k] = v + (d[k] * ds) * dt;
+    }
+}
+WPBR_D float env_pdf(const SceneView& sv, float u, float v) {
+    int nu = (int)sv.env_w, nv = (int)sv.env_h;
+    int iu = (int)(u * (float)nu), iv = (int)(v * (float)nv);
+    iu = iu < 0 ? 0 : (iu > nu - 1 ? nu - 1 : iu);
+    iv = iv < 0 ? 0 : (iv > nv - 1 ? nv - 1 : iv);
+    return sv.env_cond_func[(size_t)iv * nu + iu] / sv.env_marg_int[0];
+}
+// Smits illuminant spectrum of rgb (CSpectrumSampled.h:200-247): three bases and weights.
+WPBR_D void rgb_basis(const float* rgb, int* b, float* w) {
+    float r = rgb[0], g = rgb[1], bl = rgb[2];
+    if (r <= g && r <= bl) {
+        b[0] = 0; w[0] = r;
+        if (g <= bl) { b[1] = 1; w[1] = g - r; b[2] = 6; w[2] = bl - g; }
+        else { b[1] = 1; w[1] = bl - r; b[2] = 5; w[2] = g - bl; }
+    } else if (g <= r && g <= bl) {
+        b[0] = 0; w[0] = g;
+        if (r <= bl) { b[1] = 2; w[1] = r - g; b[2] = 6; w[2] = bl - r; }
+        else { b[1] = 2; w[1] = bl - g; b[2] = 4; w[2] = r - bl; }
+    } else {
+        b[0] = 0; w[0] = bl;
+        if (r <= g) { b[1] = 3; w[1] = r - bl; b[2] = 5; w[2] = g - r; }
+        else { b[1] = 3; w[1] = g - bl; b[2] = 4; w[2] = r - g; }
+    }
+}
+// fromRGB(rgb, Illuminant)[i]: ((0 + B0*w0) + B1*w1) + B2*w2
+WPBR_D float illum_at(const SceneView& sv, const int* b, const float* w, int i) {
+    float s = 0.0f;
+    s += sv.illum[b[0] * NSPEC + i] * w[0];
+    s += sv.illum[b[1] * NSPEC + i] * w[1];
+    s += sv.illum[b[2] * NSPEC + i] * w[2];
+    return s;
+}
+// Env direction -> (u, v, theta).
+WPBR_D void env_uv(const SceneView& sv, f3 dirW, float* u, float* v, float* theta) {
+    affine inv = load_affine(sv.env_xform + 12);
+    f3 dL = xf_vector(inv, dirW);
+    float th = spherical_theta(dL), ph = spherical_phi(dL);
+    *u = (float)((double)ph / (2 * PI_REF));
+    *v = (float)((double)th / PI_REF);
+    *theta = th;
+}
+
+// ---- microfacet + Fresnel ---------------------------------------------------------
+WPBR_D float cos2theta(f3 w) { return w.z * w.z; }
+WPBR_D float sin2theta(f3 w) { return smax(0.0f, 1.0f - cos2theta(w)); }
+WPBR_D float sintheta(f3 w) { return sqrtf(sin2theta(w)); }
+WPBR_D float tan2theta(f3 w) { return sin2theta(w) / cos2theta(w); }
+WPBR_D float cosphi(f3 w) { float s = sintheta(w); return (s == 0) ? 1 : clampf(w.x / s, -1, 1); }
+WPBR_D float sinphi(f3 w) { float s = sintheta(w); return (s == 0) ? 0 : clampf(w.y / s, -1, 1); }
+WPBR_D float abscos(f3 w) { return fabsf(w.z); }
+WPBR_D bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0; }
+
+WPBR_D float ggx_d(f3 wh, float ax, float ay) {
+    float t2 = tan2theta(wh);
+    if (isinf(t2)) return 0.0f;
+    const float cos4 = cos2theta(wh) * cos2theta(wh);
+    float c2p = cosphi(wh) * cosphi(wh), s2p = sinphi(wh) * sinphi(wh);
+    float e = (c2p / (ax * ax) + s2p / (ay * ay)) * t2;
+    float one_e = 1 + e;
+    double den = PI_REF * (double)ax * (double)ay * (double)cos4 * (double)one_e * (double)one_e;
+    return (float)(1 / den);
+}
+
+WPBR_D float fresnel_dielectric(float cosI, float etaI, float etaT) {
+    cosI = clampf(cosI, -1.0f, 1.0f);
+    bool entering = cosI > 0.0f;
+    if (!entering) { float t = etaI; etaI = etaT; etaT = t; cosI = fabsf(cosI); }
+    float sinI = sqrtf(smax(0.0f, (float)(1.0 - (double)(cosI * cosI))));
+    float sinT = etaI / etaT * sinI;
+    if (sinT >= 1.0f) return 1.0f;
+    float cosT = sqrtf(smax(0.0f, (float)(1.0 - (double)(sinT * sinT))));
+    float Rparl = ((etaT * cosI) - (etaI * cosT)) / ((etaT * cosI) + (etaI * cosT));
+    float Rperp = ((etaI * cosI) - (etaT * cosT)) / ((etaI * cosI) + (etaT * cosT));
+    return (Rparl * Rparl + Rperp * Rperp) / 2;
+}
+
+WPBR_D float fresnel_conductor1(float cosI, float etaT, float k) {
+    cosI = clampf(cosI, -1.0f, 1.0f);
+    float eta = etaT / 1.0f, etak = k / 1.0f;
+    float c2 = cosI * cosI;
+    float s2 = (float)(1. - (double)c2);
+    float eta2 = eta * eta, etak2 = etak * etak;
+    float t0 = eta2 - etak2 - s2;
+    float a2pb22 = t0 * t0 + eta2 * etak2 * 4;
+    float a2pb2 = sqrtf(a2pb22);
+    float t1 = a2pb2 + c2;
+    float a = sqrtf((a2pb2 + t0) * 0.5f);
+    float t2 = a * cosI * 2.0f;
+    float Rs = (t1 - t2) / (t1 + t2);
+    float t3 = a2pb2 * c2 + s2 * s2;
+    float t4 = t2 * s2;
+    float Rp = Rs * (t3 - t4) / (t3 + t4);
+    return (Rp + Rs) * 0.5f;
+}
+
+// Microfacet BSDF terms for (wo, wi) in the local frame: pdf and the scalar parts
+// of f*|cos wi| = Fr(cos) * sc * ac (conductor) or (1 * sc) * ac (dielectric).
+struct BsdfTerms { float cos_fr, sc, ac, pdf; uint32_t zero; };
+
+WPBR_D BsdfTerms microfacet_terms(const DevMaterial& m, f3 wo, f3 wi, f3 wh, float whD) {
+    BsdfTerms r;
+    r.pdf = (whD * abscos(wh)) / (4 * dot3(wo, wh));
+    float cO = abscos(wo), cI = abscos(wi);
+    r.ac = abscos(wi);
+    r.cos_fr = dot3(wi, wh);
+    r.zero = (cI == 0 || cO == 0 || (wh.x == 0 && wh.y == 0 && wh.z == 0)) ? 1u : 0u;
+    if (m.type == MAT_CONDUCTOR) r.sc = whD / (4 * cI * cO);
+    else {
+        float fr = fresnel_dielectric(r.cos_fr, 1.0f, m.eta);
+        r.sc = fr * whD / (4 * cI * cO);
+    }
+    return r;
+}
+
+// val(lambda) of the BSDF*|cos| for one wavelength.
+WPBR_D float bsdf_val(const SceneView& sv, const DevMaterial& m, float cos_fr, float sc, float ac,
+                      uint32_t zero, uint32_t type_spec_fr, int i) {
+    if (zero) return 0.0f * ac;
+    if (m.type == MAT_CONDUCTOR) {
+        const float* es = sv.mat_spec + (size_t)m.spec * 2 * NSPEC;
+        return ((fresnel_conductor1(cos_fr, es[i], es[NSPEC + i]) * 1.0f) * sc) * ac;
+    }
+    (void)type_spec_fr;
+    return (1.0f * sc) * ac;
+}
+
+// ---- surface interactions -----------------------------------------------------------
+struct SI { f3 p, n, ns, dpdu, wo; };
+
+WPBR_D SI si_triangle(const SceneView& sv, uint32_t tri, f3 rd, const float* b) {
+    const float* v = sv.g.tri_verts + 9 * (size_t)tri;
+    f3 P0 = mk3(v[0], v[1], v[2]), P1 = mk3(v[3], v[4], v[5]), P2 = mk3(v[6], v[7], v[8]);
+    uint8_t flags = sv.tri_flags[tri];
+    float uv[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+    if (flags & 1) {
+        const float* u = sv.tri_uv + 6 * (size_t)tri;
+        for (int i = 0; i < 3; i++) { uv[i][0] = u[2 * i]; uv[i][1] = u[2 * i + 1]; }
+    }
+    f3 dp02 = sub3(P0, P2), dp12 = sub3(P1, P2);
+    float duv02[2] = {uv[0][0] - uv[2][0], uv[0][1] - uv[2][1]};
+    float duv12[2] = {uv[1][0] - uv[2][0], uv[1][1] - uv[2][1]};
+    float det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+    if (det == 0) {
+        const float k = (float)(1.0 * 0.001);
+        uv[0][0] = 0; uv[0][1] = 0; uv[1][0] = k; uv[1][1] = 0; uv[2][0] = k; uv[2][1] = k;
+        duv02[0] = uv[0][0] - uv[2][0]; duv02[1] = uv[0][1] - uv[2][1];
+        duv12[0] = uv[1][0] - uv[2][0]; duv12[1] = uv[1][1] - uv[2][1];
+        det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+    }
+    float detInv = 1 / det;
+    f3 dpdu = mul3(sub3(mul3(dp02, duv12[1]), mul3(dp12, duv02[1])), detInv);
+    SI s;
+    s.p = add3(add3(mul3(P0, b[0]), mul3(P1, b[1])), mul3(P2, b[2]));
+    s.wo = neg3(rd);
+    s.dpdu = dpdu;
+    f3 ng = normalize3(cross3(dp02, dp12));
+    s.n = ng;
+    s.ns = ng;
+    if (flags & 2) {
+        const float* nn = sv.tri_normal + 9 * (size_t)tri;
+        const float* tt = sv.tri_tangent + 9 * (size_t)tri;
+        f3 N0 = mk3(nn[0], nn[1], nn[2]), N1 = mk3(nn[3], nn[4], nn[5]), N2 = mk3(nn[6], nn[7], nn[8]);
+        f3 T0 = mk3(tt[0], tt[1], tt[2]), T1 = mk3(tt[3], tt[4], tt[5]), T2 = mk3(tt[6], tt[7], tt[8]);
+        f3 ns = add3(add3(mul3(N0, b[0]), mul3(N1, b[1])), mul3(N2, b[2]));
+        ns = is_zero(ns) ? s.n : normalize3(ns);
+        f3 ss = add3(add3(mul3(T0, b[0]), mul3(T1, b[1])), mul3(T2, b[2]));
+        ss = is_zero(ss) ? normalize3(dpdu) : normalize3(ss);
+        f3 ts = cross3(ss, ns);
+        if (len2(ts) > 0) {
+            ts = normalize3(ts);
+            ss = cross3(ts, ns);
+        }
+        s.ns = normalize3(cross3(ss, ts));
+        s.dpdu = ss;
+        s.n = dot3(s.n, s.ns) < 0.0f ? neg3(s.n) : s.n;
+    }
+    return s;
+}
+
+WPBR_D SI si_sphere(const SceneView& sv, uint32_t sid, f3 o, f3 d, float t) {
+    const float* xf = sv.g.sphere_xform + 24 * (size_t)sid;
+    affine W = load_affine(xf), Inv = load_affine(xf + 12);
+    float radius = sv.g.sphere_radius[sid];
+    f3 oL = xf_point(Inv, o), dL = xf_vector(Inv, d);
+    f3 pH = add3(oL, mul3(dL, t));
+    float theta = cr_acos(pH.z / radius);
+    float invZR = (float)(1.0 / (double)sqrtf(pH.x * pH.x + pH.y * pH.y));
+    float cphi = pH.x * invZR, sphi = pH.y * invZR;
+    f3 dpdu = mul3(mk3(-pH.y, pH.x, 0.0f), (float)(2 * PI_REF));
+    f3 dpdv = mul3(mk3(pH.z * cphi, pH.z * sphi, -radius * cr_sin(theta)), (float)PI_REF);
+    f3 n = normalize3(cross3(dpdu, dpdv));
+    SI s;
+    s.p = xf_point(W, pH);
+    s.n = normalize3(xf_normal(Inv, n));
+    s.ns = s.n;
+    s.dpdu = xf_vector(W, dpdu);
+    s.wo = normalize3(xf_vector(W, neg3(dL)));
+    return s;
+}
+
+// ---- sphere lights ---------------------------------------------------------------------
+WPBR_D f3 sphere_light_sample(const SceneView& sv, uint32_t sid, f3 p, const float* u, float* pdf) {
+    const float* xf = sv.g.sphere_xform + 24 * (size_t)sid;
+    affine W = load_affine(xf);
+    float radius = sv.g.sphere_radius[sid];
+    f3 pc = xf_point(W, mk3(0, 0, 0));
+    f3 wc = normalize3(sub3(pc, p));
+    f3 wcX, wcY;
+    coordinate_system(wc, &wcX, &wcY);
+    float sinThetaMax2 = radius * radius / len2(sub3(p, pc));
+    float cosThetaMax = sqrtf(smax(0.0f, (float)(1.0 - (double)sinThetaMax2)));
+    float cosTheta = (float)((double)u[0] * (1.0 - (double)cosThetaMax) + (double)cosThetaMax);
+    float sinTheta = sqrtf(smax(0.0f, (float)(1.0 - (double)(cosTheta * cosTheta))));
+    float phi = (float)((double)(u[1] * 2) * PI_REF);
+    float dc = sqrtf(len2(sub3(p, pc)));
+    float ds = dc * cosTheta - sqrtf(smax(0.0f, radius * radius - dc * dc * sinTheta * sinTheta));
+    float cosAlpha = (dc * dc + radius * radius - ds * ds) / (2 * dc * radius);
+    float sinAlpha = sqrtf(smax(0.0f, 1 - cosAlpha * cosAlpha));
+    f3 nL = add3(add3(mul3(neg3(wcX), sinAlpha * cr_cos(phi)), mul3(neg3(wcY), sinAlpha * cr_sin(phi))),
+                 mul3(neg3(wc), cosAlpha));
+    f3 pL = mul3(nL, radius);
+    *pdf = cone_pdf(cosThetaMax);
+    return xf_point(W, pL);
+}
+WPBR_D float sphere_light_pdf(const SceneView& sv, uint32_t sid) {
+    const float* xf = sv.g.sphere_xform + 24 * (size_t)sid;
+    affine W = load_affine(xf);
+    float radius = sv.g.sphere_radius[sid];
+    f3 pc = xf_point(W, mk3(0, 0, 0));
+    float sinThetaMax2 = radius * radius / len2(pc);
+    float cosThetaMax = sqrtf(smax(0.0f, 1 - sinThetaMax2));
+    return cone_pdf(cosThetaMax);
+}
+
+// Wave-aggregated append to the secondary ray queue.
+WPBR_D uint32_t queue_append(bool want, uint32_t* count) {
+    uint64_t m = ballot64(want);
+    uint32_t total = popc64(m);
+    uint32_t base = 0;
+    uint32_t leader = m ? (uint32_t)__ffsll((long long)m) - 1 : 0;
+    if (total && lane_id() == leader) base = atomicAdd(count, total);
+    base = (uint32_t)__shfl((int)base, (int)leader);
+    return base + popc64(m & lanemask_lt());
+}
+
+WPBR_D void put_ray(float* q, uint32_t cap, uint32_t slot, f3 o, f3 d) {
+    q[slot] = o.x; q[cap + slot] = o.y; q[2 * (size_t)cap + slot] = o.z;
+    q[3 * (size_t)cap + slot] = d.x; q[4 * (size_t)cap + slot] = d.y; q[5 * (size_t)cap + slot] = d.z;
+}
+
+// ---- kernels ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_camera(RenderLaunch L) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const SceneView& sv = L.sv;
+    uint32_t px = i % sv.width, py = i / sv.width;
+    uint32_t c2 = L.spp_index == 0 ? 0u : L.ws.c2[i];
+    if (L.spp_index == 0) L.ws.c1[i] = 0;
+    float u[2];
+    next2d(L.seed, i, c2, u);
+    L.ws.c2[i] = c2;
+    float fx = (float)px + u[0], fy = (float)py + u[1];
+    L.ws.pfilm[i] = fx;
+    L.ws.pfilm[L.n + i] = fy;
+    f3 pCam = xf_point4(sv.raster_camera, mk3(fx, fy, 0.0f));
+    f3 dC = normalize3(pCam);
+    affine CW = load_affine(sv.camera_world);
+    f3 o = xf_point(CW, mk3(0, 0, 0));
+    f3 d = normalize3(xf_vector(CW, dC));
+    const uint32_t n = L.n;
+    L.ws.rays[i] = o.x; L.ws.rays[n + i] = o.y; L.ws.rays[2 * (size_t)n + i] = o.z;
+    L.ws.rays[3 * (size_t)n + i] = d.x; L.ws.rays[4 * (size_t)n + i] = d.y; L.ws.rays[5 * (size_t)n + i] = d.z;
+    if (i == 0) *L.ws.qcount = 0;
+}
+
+constexpr uint32_t F_LIGHT = 1u, F_BSDF = 2u, F_ENV = 4u, F_DONE = 8u, F_MISS = 16u;
+
+__global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = i < L.n;
+    const SceneView& sv = L.sv;
+    WaveState& ws = L.ws;
+    const uint32_t n = L.n, cap = 2 * L.n;
+    uint32_t flags = F_DONE;
+    bool wantL = false, wantB = false;
+    f3 lo = mk3(0, 0, 0), ld = lo, bo = lo, bd = lo;
+    if (active) {
+        f3 o = mk3(ws.rays[i], ws.rays[n + i], ws.rays[2 * (size_t)n + i]);
+        f3 d = mk3(ws.rays[3 * (size_t)n + i], ws.rays[4 * (size_t)n + i], ws.rays[5 * (size_t)n + i]);
+        uint32_t shape = ws.hit[i];
+        uint32_t c1 = ws.c1[i], c2 = ws.c2[i];
+        if (shape == 0xFFFFFFFFu) {
+            flags = F_DONE | F_MISS;
+            float X = 0, Y = 0, Z = 0;
+            if (sv.has_env) {
+                float u, v, th, rgb[3];
+                env_uv(sv, d, &u, &v, &th);
+                env_bilerp(sv, u, v, rgb);
+                int b[3];
+                float w[3];
+                rgb_basis(rgb, b, w);
+                for (int k = 0; k < NSPEC; k++) {
+                    float Lk = illum_at(sv, b, w, k) * sv.env_emit[k];
+                    X += Lk * sv.xyz[k];
+                    Y += Lk * sv.xyz[NSPEC + k];
+                    Z += Lk * sv.xyz[2 * NSPEC + k];
+                }
+            }
+            const float dl = 300.0f / 64.0f;
+            ws.le[i] = X * dl; ws.le[n + i] = Y * dl; ws.le[2 * (size_t)n + i] = Z * dl;
+        } else {
+            float t = ws.hit_t[i];
+            SI si;
+            if (shape < sv.g.n_spheres) {
+                si = si_sphere(sv, shape, o, d, t);
+            } else {
+                float b[3] = {ws.bary[i], ws.bary[n + i], ws.bary[2 * (size_t)n + i]};
+                si = si_triangle(sv, shape - sv.g.n_spheres, d, b);
+            }
+            const uint32_t nLights = sv.n_sphere_lights + (sv.has_env ? 1u : 0u);
+            float u1 = next1d(L.seed, i, c1);
+            uint32_t li = (uint32_t)((u1 + 0.5f) * (float)(nLights - 1));
+            if (li > nLights - 1) li = nLights - 1;
+            bool envPick = sv.has_env && li == sv.n_sphere_lights;
+            float ul[2];
+            next2d(L.seed, i, c2, ul);
+            f3 wiL = mk3(0, 0, 0);
+            float pdfL = 0;
+            if (!envPick) {
+                f3 sp = sphere_light_sample(sv, sv.light_shape[li], si.p, ul, &pdfL);
+                wiL = normalize3(sub3(sp, si.p));
+            }
+            uint32_t mid = sv.shape_material[shape];
+            flags = envPick ? F_ENV : 0u;
+            ws.light[i] = li;
+            if (mid == 0xFFFFFFFFu) {
+                flags |= F_DONE;
+            } else {
+                const DevMaterial m = sv.materials[mid];
+                ws.mat[i] = mid;
+                f3 ss = normalize3(si.dpdu);
+                f3 ns = si.ns;
+                f3 ts = cross3(ns, ss);
+                f3 woL = mk3(dot3(ss, si.wo), dot3(ts, si.wo), dot3(ns, si.wo));
+                // light strategy: BSDF Compute toward the light
+                if (!envPick && m.type != MAT_SPECULAR) {
+                    f3 wl = mk3(dot3(ss, wiL), dot3(ts, wiL), dot3(ns, wiL));
+                    f3 wh = normalize3(add3(woL, wl));
+                    float whD = ggx_d(wh, m.alpha, m.alpha);
+                    BsdfTerms bt = microfacet_terms(m, woL, wl, wh, whD);
+                    ws.l_cos[i] = bt.cos_fr; ws.l_sc[i] = bt.sc; ws.l_ac[i] = bt.ac;
+                    ws.l_bpdf[i] = bt.pdf; ws.l_zero[i] = bt.zero; ws.l_pdf[i] = pdfL;
+                    if (bt.pdf != 0.0f) {
+                        wantL = true;
+                        lo = add3(si.p, mul3(wiL, 0.1f));
+                        ld = wiL;
+                    }
+                }
+                // BSDF strategy: sample
+                f3 wiBL;
+                bool ok = true;
+                if (m.type == MAT_SPECULAR) {
+                    wiBL = mk3(-woL.x, -woL.y, woL.z);
+                    float c = abscos(wiBL);
+                    ws.b_pdf[i] = 1.0f;
+                    ws.b_cos[i] = 0.0f;
+                    ws.b_zero[i] = c == 0 ? 1u : 0u;
+                    ws.b_sc[i] = (c == 0) ? 0.0f : (fresnel_dielectric(wiBL.z, 1.0f, m.eta) / c);
+                    ws.b_ac[i] = c;
+                } else {
+                    float ub[2];
+                    next2d(L.seed, i, c2, ub);
+                    float phi = (float)((2 * PI_REF) * (double)ub[1]);
+                    float tan2 = m.alpha * m.alpha * ub[0] / (1.0f - ub[0]);
+                    float cosT = 1 / sqrtf(1 + tan2);
+                    float sinT = sqrtf(smax(0.f, 1.f - cosT * cosT));
+                    f3 wh = mk3(sinT * cr_cos(phi), sinT * cr_sin(phi), cosT);
+                    if (!same_hemi(woL, wh)) wh = neg3(wh);
+                    float whD = ggx_d(wh, m.alpha, m.alpha);
+                    wiBL = add3(neg3(woL), mul3(wh, 2 * dot3(woL, wh)));
+                    if (!same_hemi(wiBL, woL)) {
+                        ok = false;
+                    } else {
+                        BsdfTerms bt = microfacet_terms(m, woL, wiBL, wh, whD);
+                        ws.b_cos[i] = bt.cos_fr; ws.b_sc[i] = bt.sc; ws.b_ac[i] = bt.ac;
+                        ws.b_pdf[i] = bt.pdf; ws.b_zero[i] = bt.zero;
+                    }
+                }
+                if (ok) {
+                    f3 wiB = mk3((ss.x * wiBL.x + ts.x * wiBL.y) + ns.x * wiBL.z,
+                                 (ss.y * wiBL.x + ts.y * wiBL.y) + ns.y * wiBL.z,
+                                 (ss.z * wiBL.x + ts.z * wiBL.y) + ns.z * wiBL.z);
+                    ws.b_wi[i] = wiB.x; ws.b_wi[n + i] = wiB.y; ws.b_wi[2 * (size_t)n + i] = wiB.z;
+                    wantB = true;
+                    bo = add3(si.p, mul3(wiB, 0.01f));
+                    bd = wiB;
+                }
+            }
+            ws.c1[i] = c1;
+            ws.c2[i] = c2;
+        }
+        if (wantL) flags |= F_LIGHT;
+        if (wantB) flags |= F_BSDF;
+        ws.flags[i] = flags;
+    }
+    uint32_t sl = queue_append(wantL, ws.qcount);
+    uint32_t sb = queue_append(wantB, ws.qcount);
+    if (wantL) { put_ray(ws.qrays, cap, sl, lo, ld); ws.l_ray[i] = sl; }
+    if (wantB) { put_ray(ws.qrays, cap, sb, bo, bd); ws.b_ray[i] = sb; }
+}
+
+WPBR_D float mitchell1(float x) {
+    const float B = (float)(1.0 / 3.0f), Cc = (float)(1.0 / 3.0f);
+    x = fabsf(2 * x);
+    if (x > 1)
+        return ((-B - 6 * Cc) * x * x * x + (6 * B + 30 * Cc) * x * x + (-12 * B - 48 * Cc) * x +
+                (8 * B + 24 * Cc)) * (1.f / 6.f);
+    return ((12 - 9 * B - 6 * Cc) * x * x * x + (-18 + 12 * B + 6 * Cc) * x * x + (6 - 2 * B)) *
+           (1.f / 6.f);
+}
+
+__global__ __launch_bounds__(256) void k_shade2(RenderLaunch L) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const SceneView& sv = L.sv;
+    WaveState& ws = L.ws;
+    const uint32_t n = L.n;
+    uint32_t flags = ws.flags[i];
+    float X = 0, Y = 0, Z = 0;
+    bool black = true;
+    if (flags & F_MISS) {
+        X = ws.le[i]; Y = ws.le[n + i]; Z = ws.le[2 * (size_t)n + i];
+        black = false;  // the splat tests L itself; an exactly black Le adds zeros
+    } else if (flags & (F_LIGHT | F_BSDF)) {
+        const uint32_t li = ws.light[i];
+        const DevMaterial m = sv.materials[ws.mat[i]];
+        bool envPick = flags & F_ENV;
+        // light strategy
+        bool useL = false;
+        float sL = 0, lcos = 0, lsc = 0, lac = 0;
+        uint32_t lzero = 0;
+        const float* Le = nullptr;
+        if (flags & F_LIGHT) {
+            uint32_t h = ws.qhit[ws.l_ray[i]];
+            if (h == 0xFFFFFFFFu || h == sv.light_shape[li]) {
+                Le = sv.light_emit + (size_t)li * NSPEC;
+                bool lb = true;
+                for (int k = 0; k < NSPEC; k++) if (Le[k] != 0.0f) { lb = false; break; }
+                if (!lb) {
+                    float pdfL = ws.l_pdf[i], pdfB = ws.l_bpdf[i];
+                    float w = (pdfL * pdfL) / (pdfL * pdfL + pdfB * pdfB);
+                    sL = w / pdfL;
+                    lcos = ws.l_cos[i]; lsc = ws.l_sc[i]; lac = ws.l_ac[i]; lzero = ws.l_zero[i];
+                    useL = true;
+                }
+            }
+        }
+        // BSDF strategy
+        bool useB = false;
+        float sB = 0, bcos = 0, bsc = 0, bac = 0;
+        uint32_t bzero = 0;
+        int eb[3] = {0, 0, 0};
+        float ew[3] = {0, 0, 0};
+        const float* Lb = nullptr;
+        if (flags & F_BSDF) {
+            uint32_t h = ws.qhit[ws.b_ray[i]];
+            bool reach = h == 0xFFFFFFFFu || (!envPick && h == sv.light_shape[li]);
+            if (reach) {
+                float pdfB = ws.b_pdf[i];
+                float pdfL2 = 0;
+                bool bb = true;
+                if (envPick) {
+                    f3 wi = mk3(ws.b_wi[i], ws.b_wi[n + i], ws.b_wi[2 * (size_t)n + i]);
+                    float u, v, th;
+                    env_uv(sv, wi, &u, &v, &th);
+                    float sinT = cr_sin(th);
+                    if (sinT != 0.0f) {
+                        pdfL2 = (float)((double)env_pdf(sv, u, v) / (2 * PI_REF * PI_REF * (double)sinT));
+                        float rgb[3];
+                        env_bilerp(sv, u, v, rgb);
+                        rgb_basis(rgb, eb, ew);
+                        for (int k = 0; k < NSPEC; k++)
+                            if (illum_at(sv, eb, ew, k) * sv.env_emit[k] != 0.0f) { bb = false; break; }
+                    }
+                } else {
+                    pdfL2 = sphere_light_pdf(sv, sv.light_shape[li]);
+                    Lb = sv.light_emit + (size_t)li * NSPEC;
+                    for (int k = 0; k < NSPEC; k++) if (Lb[k] != 0.0f) { bb = false; break; }
+                }
+                if (!bb) {
+                    float w = (pdfB == 1.0f) ? 1.0f : (pdfB * pdfB) / (pdfB * pdfB + pdfL2 * pdfL2);
+                    sB = w / pdfB;
+                    bcos = ws.b_cos[i]; bsc = ws.b_sc[i]; bac = ws.b_ac[i]; bzero = ws.b_zero[i];
+                    useB = true;
+                }
+            }
+        }
+        if (useL || useB) {
+            for (int k = 0; k < NSPEC; k++) {
+                float Lk = 0.0f;
+                if (useL) Lk += (Le[k] * bsdf_val(sv, m, lcos, lsc, lac, lzero, 0, k)) * sL;
+                if (useB) {
+                    float li_k = envPick ? illum_at(sv, eb, ew, k) * sv.env_emit[k] : Lb[k];
+                    float v = (m.type == MAT_SPECULAR) ? (bzero ? 0.0f * bac : (1.0f * bsc) * bac)
+                                                       : bsdf_val(sv, m, bcos, bsc, bac, bzero, 0, k);
+                    Lk += (li_k * v) * sB;
+                }
+                if (Lk != 0.0f) black = false;
+                X += Lk * sv.xyz[k];
+                Y += Lk * sv.xyz[NSPEC + k];
+                Z += Lk * sv.xyz[2 * NSPEC + k];
+            }
+            const float dl = 300.0f / 64.0f;
+            X *= dl; Y *= dl; Z *= dl;
+        }
+    }
+    if (L.sample_xyz) {
+        size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
+        L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
+    }
+    // Mitchell splat clipped to the 32x32 tile of the pixel.
+    const uint32_t W = sv.width;
+    uint32_t px = i % W, py = i / W;
+    int tx0 = (int)(px & ~31u), ty0 = (int)(py & ~31u);
+    int tx1 = tx0 + 31, ty1 = ty0 + 31;
+    float fx = ws.pfilm[i], fy = ws.pfilm[n + i];
+    float dx = fx - 0.5f, dy = fy - 0.5f;
+    int x0 = (int)ceilf(dx - 2.0f), x1 = (int)floorf(dx + 2.0f);
+    int y0 = (int)ceilf(dy - 2.0f), y1 = (int)floorf(dy + 2.0f);
+    x0 = max(x0, tx0); y0 = max(y0, ty0);
+    x1 = min(x1, tx1); y1 = min(y1, ty1);
+    for (int y = y0; y <= y1; y++) {
+        float wy = mitchell1(((float)y - fy) / 2.0f);
+        for (int x = x0; x <= x1; x++) {
+            float w = mitchell1(((float)x - fx) / 2.0f) * wy;
+            float* f = L.film + 4 * ((size_t)y * W + x);
+            if (!black) {
+                atomicAdd(f, X * w);
+                atomicAdd(f + 1, Y * w);
+                atomicAdd(f + 2, Z * w);
+            }
+            atomicAdd(f + 3, w);
+        }
+    }
+}
+
+wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {
+    dim3 grid((L.n + 255) / 256), block(256);
+    switch (stage) {
+        case 0: hipLaunchKernelGGL(k_camera, grid, block, 0, st, L); break;
+        case 1: hipLaunchKernelGGL(k_shade1, grid, block, 0, st, L); break;
+        case 2: hipLaunchKernelGGL(k_shade2, grid, block, 0, st, L); break;
+    }
+    return hipGetLastError() == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+}
+
+}  // namespace wpbr

@@ -1,0 +1,92 @@
+// Device-side scene and per-sample wavefront state of the renderer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "wpbr_internal.hpp"
+
+namespace wpbr {
+
+constexpr int NSPEC = 64;
+enum : uint32_t { MAT_CONDUCTOR = 0, MAT_DIELECTRIC = 1, MAT_SPECULAR = 2 };
+
+struct DevMaterial {
+    uint32_t type;
+    float alpha;
+    float eta;      // dielectric etaT (etaI = 1)
+    uint32_t spec;  // conductor: index of its eta/k spectra in mat_spec (2 x 64 floats)
+};
+
+struct SceneView {
+    GeomView g;
+    const uint8_t* tri_flags;
+    const float* tri_uv;
+    const float* tri_normal;
+    const float* tri_tangent;
+    const uint32_t* shape_material;
+    const DevMaterial* materials;
+    const float* mat_spec;       // per conductor: eta[64], k[64]
+    uint32_t n_sphere_lights;
+    const uint32_t* light_shape;
+    const float* light_emit;     // n_sphere_lights x 64
+    uint32_t has_env;
+    const float* env_emit;       // 64
+    const float* env_rgb;        // h x w x 3
+    uint32_t env_w, env_h;
+    const float* env_xform;      // world 3x4, inverse 3x4
+    const float* env_cond_func;  // h x w
+    const float* env_cond_int;   // h
+    const float* env_marg_int;   // 1
+    const float* xyz;            // 3 x 64 (X, Y, Z)
+    const float* illum;          // 7 x 64 Smits illuminant bases
+    const float* raster_camera;  // 16
+    const float* camera_world;   // 12
+    uint32_t width, height;
+};
+
+// Per-sample state between the wavefront stages (SoA, one entry per pixel).
+struct WaveState {
+    uint32_t* c1;         // 1D cursor
+    uint32_t* c2;         // 2D cursor
+    float* pfilm;         // 2 x N
+    float* rays;          // primary rays, 6 x N
+    uint32_t* hit;        // primary hit shape
+    float* hit_t;
+    float* bary;          // 3 x N
+    // shade1 -> shade2
+    uint32_t* flags;      // bit0 light ray, bit1 bsdf ray, bit2 env pick, bit3 done (miss/no bsdf)
+    uint32_t* light;      // picked light index
+    uint32_t* mat;        // material index
+    float* l_cos;         // light strategy: dot(wi, wh) for the Fresnel term
+    float* l_sc;          // scalar factor before |cos|
+    float* l_ac;          // |cos theta_i|
+    float* l_pdf;         // light pdf
+    float* l_bpdf;        // BSDF pdf at the light direction
+    uint32_t* l_zero;     // BSDF value is exactly 0 (degenerate geometry)
+    float* b_cos;
+    float* b_sc;
+    float* b_ac;
+    float* b_pdf;
+    uint32_t* b_zero;
+    float* b_wi;          // 3 x N world direction
+    float* le;            // XYZ of the emitted light for primary misses (3 x N)
+    // compacted secondary ray queue
+    float* qrays;         // 6 x 2N
+    uint32_t* qhit;       // 2N
+    float* qt;            // 2N
+    uint32_t* qcount;     // 1
+    uint32_t* l_ray;      // queue slot of the light ray per sample
+    uint32_t* b_ray;      // queue slot of the BSDF ray per sample
+};
+
+struct RenderLaunch {
+    SceneView sv;
+    WaveState ws;
+    uint32_t n;           // pixels
+    uint32_t spp_index;
+    uint32_t spp;
+    uint32_t seed;
+    float* film;          // W*H*4: sum X w, sum Y w, sum Z w, sum w
+    float* sample_xyz;    // optional: (pixel*spp + j) x 3
+};
+
+}  // namespace wpbr

@@ -135,7 +135,6 @@ def icosphere(freq, radius=1.0, center=(0.0, 0.0, 0.0), displace=None):
     out = []
     for a, b, c in F:
         corners = np.array([a, b, c])
-        w = np.stack([I, J, K], axis=1).astype(np.float64)  # weights of b, c, a below
         # point = (a*K + b*I + c*J) / f, summed in ascending corner-index order
         wc = np.stack([K, I, J], axis=1).astype(np.float64)
         order = np.argsort(corners)
@@ -147,7 +146,6 @@ def icosphere(freq, radius=1.0, center=(0.0, 0.0, 0.0), displace=None):
         r = radius * (displace(P) if displace is not None else 1.0)
         P = P * (np.asarray(r)[:, None] if np.ndim(r) else r) + np.asarray(center)[None, :]
         out.append(P[tri].reshape(-1, 9))
-        del w
     return np.concatenate(out, axis=0).astype(np.float32)
 
 
@@ -184,3 +182,219 @@ def value_noise(seed, octaves=3, base_freq=4.0):
         return total / norm
 
     return noise
+
+
+# ---------------------------------------------------------------------------
+# rendered scenes
+N_SPECTRAL = 64
+MAT_CONDUCTOR, MAT_DIELECTRIC, MAT_SPECULAR = 0, 1, 2
+CU_ETA = (0.2004, 0.9240, 1.1022)   # copper-like eta (RGB)
+CU_K = (3.9129, 2.4528, 2.1421)     # copper-like k (RGB)
+
+
+def bin_centers():
+    return 400.0 + (np.arange(N_SPECTRAL) + 0.5) * (300.0 / N_SPECTRAL)
+
+
+def blackbody_spectrum(T, scale=1.0):
+    """Planck's law at the 64 bin centres, normalised to its peak, times scale."""
+    lam = bin_centers() * 1e-9
+    c, h, kb = 299792458.0, 6.62606957e-34, 1.3806488e-23
+    le = (2 * h * c * c) / (lam ** 5 * (np.exp((h * c) / (lam * kb * T)) - 1))
+    lam_max = 2.8977721e-3 / T
+    le_max = (2 * h * c * c) / (lam_max ** 5 * (np.exp((h * c) / (lam_max * kb * T)) - 1))
+    return (le / le_max * scale).astype(np.float32)
+
+
+def env_map(width, height, seed):
+    """Log-normal RGB texels, mu = 0, sigma = 1 (height x width x 3)."""
+    g = rng(seed)
+    return np.exp(g.standard_normal((height, width, 3))).astype(np.float32)
+
+
+def identity_xform():
+    return translate_xform((0.0, 0.0, 0.0))
+
+
+def camera(width, height, eye, target, up=(0.0, 1.0, 0.0), fov_deg=45.0, near=0.1, far=10000.0):
+    """(raster->camera 4x4, camera->world 3x4) for a perspective camera.
+
+    Raster space follows the reference's screen->raster transform: translate by
+    -pMin, scale by 1/(pMax - pMin), then by (res - 1) (CSCameraProjective.h:35-43).
+    The screen window is [-a, a] x [-1, 1] with a = width / height."""
+    inv_tan = 1.0 / np.tan(np.deg2rad(fov_deg) / 2)
+    persp = np.array([[1, 0, 0, 0], [0, 1, 0, 0],
+                      [0, 0, far / (far - near), -far * near / (far - near)], [0, 0, 1, 0]], float)
+    cam_to_screen = np.diag([inv_tan, inv_tan, 1.0, 1.0]) @ persp
+    a = width / height
+    pmin, pmax = np.array([-a, -1.0]), np.array([a, 1.0])
+    T = np.eye(4)
+    T[0, 3], T[1, 3] = -pmin[0], -pmin[1]
+    S1 = np.diag([1 / (pmax[0] - pmin[0]), 1 / (pmax[1] - pmin[1]), 1, 1])
+    S2 = np.diag([width - 1.0, height - 1.0, 1, 1])
+    screen_to_raster = S2 @ S1 @ T
+    raster_to_camera = np.linalg.inv(cam_to_screen) @ np.linalg.inv(screen_to_raster)
+    eye, target, up = (np.asarray(v, float) for v in (eye, target, up))
+    fwd = target - eye
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(up / np.linalg.norm(up), fwd)
+    right /= np.linalg.norm(right)
+    nup = np.cross(fwd, right)
+    cam_to_world = np.stack([right, nup, fwd, eye], axis=1)  # 3 x 4
+    return raster_to_camera.astype(np.float32).reshape(-1), cam_to_world.astype(np.float32).reshape(-1)
+
+
+def ground_quad(width, depth, y):
+    """createQuad (MeshTools.h:101-155) placed at height y: 2 triangles with the
+    quad's uvs, +y normals and +x tangents."""
+    w, d = width / 2.0, depth / 2.0
+    P = np.array([[-w, y, d], [w, y, d], [-w, y, -d], [w, y, -d]], float)
+    uv = np.array([[0, 1], [1, 1], [0, 0], [1, 0]], float)
+    idx = np.array([[0, 1, 2], [1, 2, 3]])
+    verts = P[idx].reshape(2, 9).astype(np.float32)
+    uvs = uv[idx].reshape(2, 6).astype(np.float32)
+    normals = np.tile(np.array([0, 1, 0], np.float32), (2, 3))
+    tangents = np.tile(np.array([1, 0, 0], np.float32), (2, 3))
+    return verts, uvs, normals, tangents
+
+
+class RenderScene:
+    """Arrays describing one rendered scene (spheres first, then triangles)."""
+
+    def __init__(self):
+        self.spheres = []        # (center, radius, material or None)
+        self.meshes = []         # (verts n x 9, material, uvs, normals, tangents)
+        self.materials = []      # 9 floats each
+        self.lights = []         # (sphere index, emit spectrum)
+        self.env = None          # (rgb h x w x 3, emit spectrum, xform 24)
+        self.cam = None
+        self.width = self.height = self.spp = 0
+        self.seed = 0
+
+    def material(self, mtype, alpha=0.1, eta=1.5, eta_rgb=(1.5, 1.5, 1.5), k_rgb=(0, 0, 0)):
+        m = np.zeros(9, np.float32)
+        m[0] = np.array([mtype], np.uint32).view(np.float32)[0]
+        m[1], m[2] = alpha, eta
+        m[3:6], m[6:9] = eta_rgb, k_rgb
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_sphere(self, center, radius, material=None):
+        self.spheres.append((np.asarray(center, float), float(radius), material))
+        return len(self.spheres) - 1
+
+    def add_light_sphere(self, center, radius, emit):
+        s = self.add_sphere(center, radius, None)
+        self.lights.append((s, np.asarray(emit, np.float32)))
+        return s
+
+    def add_mesh(self, verts, material, uvs=None, normals=None, tangents=None):
+        self.meshes.append((np.asarray(verts, np.float32).reshape(-1, 9), material, uvs, normals,
+                            tangents))
+
+    def finalize(self):
+        ns = len(self.spheres)
+        self.sphere_xform = (np.stack([translate_xform(c) for c, _, _ in self.spheres])
+                             if ns else np.zeros((0, 24), np.float32))
+        self.sphere_radius = np.array([r for _, r, _ in self.spheres], np.float32)
+        verts, flags, uvs, nrm, tan, mats = [], [], [], [], [], []
+        for v, m, uv, n, t in self.meshes:
+            k = len(v)
+            verts.append(v)
+            f = (1 if uv is not None else 0) | (2 if (n is not None and t is not None) else 0)
+            flags.append(np.full(k, f, np.uint8))
+            uvs.append(uv if uv is not None else np.zeros((k, 6), np.float32))
+            nrm.append(n if n is not None else np.zeros((k, 9), np.float32))
+            tan.append(t if t is not None else np.zeros((k, 9), np.float32))
+            mats.append(np.asarray(m, np.uint32) if np.ndim(m) else np.full(k, m, np.uint32))
+        self.tri_verts = np.concatenate(verts) if verts else np.zeros((0, 9), np.float32)
+        self.tri_flags = np.concatenate(flags) if flags else np.zeros(0, np.uint8)
+        self.tri_uv = np.concatenate(uvs).astype(np.float32) if uvs else np.zeros((0, 6), np.float32)
+        self.tri_normal = np.concatenate(nrm).astype(np.float32) if nrm else np.zeros((0, 9), np.float32)
+        self.tri_tangent = np.concatenate(tan).astype(np.float32) if tan else np.zeros((0, 9), np.float32)
+        sph_mat = np.array([0xFFFFFFFF if m is None else m for _, _, m in self.spheres], np.uint32)
+        self.shape_material = np.concatenate([sph_mat] + mats).astype(np.uint32)
+        self.material_table = (np.stack(self.materials) if self.materials
+                               else np.zeros((0, 9), np.float32))
+        self.light_shape = np.array([s for s, _ in self.lights], np.uint32)
+        self.light_emit = (np.stack([e for _, e in self.lights]).astype(np.float32)
+                           if self.lights else np.zeros((0, N_SPECTRAL), np.float32))
+        if self.env is not None:
+            self.env_rgb, self.env_emit, self.env_xform = self.env
+        else:
+            self.env_rgb = np.zeros((1, 1, 3), np.float32)
+            self.env_emit = np.zeros(N_SPECTRAL, np.float32)
+            self.env_xform = identity_xform()
+        self.raster_camera, self.camera_world = self.cam
+        return self
+
+    @property
+    def n_shapes(self):
+        return len(self.sphere_radius) + len(self.tri_verts)
+
+
+def c0_scene(width=128, height=128, spp=4, seed=SEED_BASE + 0):
+    """Plain, CPU-runnable workload: 128x128, 4 spp, direct lighting with MIS."""
+    g = rng(seed)
+    s = RenderScene()
+    cond = s.material(MAT_CONDUCTOR, alpha=0.3, eta_rgb=CU_ETA, k_rgb=CU_K)
+    diel = s.material(MAT_DIELECTRIC, alpha=0.1, eta=1.5)
+    s.add_sphere((-1.6, -0.5, 0.4), 0.5, diel)
+    s.add_sphere((1.6, -0.5, -0.3), 0.5, cond)
+    s.add_light_sphere((0.6, 2.0, -1.2), 0.3, blackbody_spectrum(6500.0, 40.0))
+    ico = icosphere(32)  # 20 480 triangles, radius 1 at the origin
+    rot = random_rotations(g, 1)[0]
+    v = ico.reshape(-1, 3).astype(np.float64)
+    # element-wise (not BLAS) so shared vertices stay bit-identical
+    v = np.stack([rot[r, 0] * v[:, 0] + rot[r, 1] * v[:, 1] + rot[r, 2] * v[:, 2]
+                  for r in range(3)], axis=1)
+    ico = v.reshape(-1, 9).astype(np.float32)
+    s.add_mesh(ico, cond)
+    qv, quv, qn, qt = ground_quad(12.0, 12.0, -1.0)
+    s.add_mesh(qv, diel, quv, qn, qt)
+    s.env = (env_map(256, 128, seed + 100), np.ones(N_SPECTRAL, np.float32), identity_xform())
+    s.cam = camera(width, height, (0.0, 0.9, -4.6), (0.0, -0.2, 0.0))
+    s.width, s.height, s.spp, s.seed = width, height, spp, seed
+    return s.finalize()
+
+
+def c2_scene(width=1024, height=1024, spp=16, seed=SEED_BASE + 2, freq=317, n_spheres=64,
+             env_size=(1024, 512)):
+    """Full wavefront render: displaced 2M-triangle icosphere, 64 spheres, ground,
+    4 sphere lights and an env map; materials by seed per triangle group."""
+    g = rng(seed)
+    s = RenderScene()
+
+    def rand_material():
+        x = g.random()
+        if x < 0.4:
+            return s.material(MAT_CONDUCTOR, alpha=float(g.uniform(0.05, 0.6)), eta_rgb=CU_ETA,
+                              k_rgb=CU_K)
+        if x < 0.8:
+            return s.material(MAT_DIELECTRIC, alpha=float(g.uniform(0.01, 0.3)), eta=1.5)
+        return s.material(MAT_SPECULAR, eta=1.66)
+
+    # 64 spheres on a ring around the icosphere
+    for i in range(n_spheres):
+        r = float(g.uniform(0.05, 0.2))
+        ang = 2 * np.pi * i / n_spheres + g.uniform(-0.03, 0.03)
+        rad = g.uniform(1.6, 3.2)
+        c = (rad * np.cos(ang), -1.1 + r + g.uniform(0, 1.5), rad * np.sin(ang))
+        s.add_sphere(c, r, rand_material())
+    light_pos = [(1.5, 2.5, -1.5), (-2.0, 2.2, -0.5), (0.5, 3.0, 1.5), (-0.8, 2.8, -2.2)]
+    for p in light_pos:
+        s.add_light_sphere(p, 0.15, blackbody_spectrum(6500.0, 60.0))
+    noise = value_noise(seed + 7)
+    ico = icosphere(freq, displace=lambda P: 1.0 + 0.05 * noise(P))
+    groups = np.array_split(np.arange(len(ico)), 64)
+    mat = np.empty(len(ico), np.uint32)
+    for grp in groups:
+        mat[grp] = rand_material()
+    s.add_mesh(ico, mat)
+    qv, quv, qn, qt = ground_quad(16.0, 16.0, -1.1)
+    s.add_mesh(qv, s.material(MAT_DIELECTRIC, alpha=0.2, eta=1.5), quv, qn, qt)
+    s.env = (env_map(env_size[0], env_size[1], seed + 100), np.ones(N_SPECTRAL, np.float32),
+             identity_xform())
+    s.cam = camera(width, height, (0.0, 1.2, -5.2), (0.0, -0.3, 0.0))
+    s.width, s.height, s.spp, s.seed = width, height, spp, seed
+    return s.finalize()
