@@ -1,0 +1,171 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X ray-tracing hot path; prints ONE JSON line.
+
+Default workload: the full wavefront render (1024x1024 film, 16 spp stratified
+streams, MIS direct lighting, a 2 009 780-triangle displaced icosphere, 64
+spheres, a ground quad, 4 sphere lights and a 1024x512 env map, all 1024 film
+tiles per pass).  One step = one whole render (16 passes).  value = Msamples/s.
+The LBVH build (1 048 576 triangles) and closest-hit traversal (4 194 304 rays)
+of the build+traversal microbenchmark are measured in the same run and reported
+as extra fields.  Inputs are resident in HBM before the timed region.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Msamples/s path-traced + LBVH Mprims/s build; fwd+adjoint vs fwd time"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes: traversal 28 B ray in + 8 B hit out
+TRACE_BYTES_PER_RAY = 36.0
+LBVH_BYTES_PER_PRIM = 196.0
+LBVH_BYTES_PER_NODE = 36.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, max_threads=16):
+    """The oracle (CPU restatement of the reference path) on every 16th film tile."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    ntx = scene.width // 32
+    ntiles = ntx * (scene.height // 32)
+    tiles = np.arange(0, ntiles, 16, dtype=np.uint32)
+    threads = max(1, min(max_threads, os.cpu_count() or 1))
+    out = po.render(scene, nthreads=threads, tiles=tiles)
+    secs = float(out["counters"][7]) * 1e-9
+    samples = len(tiles) * 32 * 32 * scene.spp
+    return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads,
+            "kind": "port",
+            "sample": "every 16th 32x32 tile (%d of %d tiles, %d samples), %.2f s of %d threads"
+                      % (len(tiles), ntiles, samples, secs, threads)}
+
+
+def bench_microbench(lib, scenes, ctx, steps):
+    tris, rays = scenes.microbench()
+    sc = lib.Scene(ctx, np.zeros((0, 24), np.float32), np.zeros(0, np.float32), tris)
+    bvh = lib.Lbvh(ctx)
+    bvh.build(sc)
+    build_ms = []
+    for _ in range(max(steps, 3)):
+        build_ms.append(bvh.build(sc))
+    ktimes = bvh.kernel_times()
+    nodes = bvh.counts()["nodes"]
+    d_rays = ctx.array(rays)
+    n = rays.shape[1]
+    d_s, d_t, _ = lib.collide(ctx, sc, bvh, d_rays, n)
+    trace_ms = []
+    for _ in range(max(steps, 3)):
+        lib.collide(ctx, sc, bvh, d_rays, n)
+        trace_ms.append(ctx.last_ms())
+    b = float(np.median(build_ms))
+    t = float(np.median(trace_ms))
+    lbvh_bytes = LBVH_BYTES_PER_PRIM * len(tris) + LBVH_BYTES_PER_NODE * nodes
+    res = {
+        "lbvh_build": {"prims": len(tris), "nodes": nodes, "ms": b,
+                       "mprims_per_s": len(tris) / b / 1e3,
+                       "hbm_frac": lbvh_bytes / (b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                       "kernel_ms": [round(x, 4) for x in ktimes]},
+        "traversal": {"rays": n, "ms": t, "mrays_per_s": n / t / 1e3},
+    }
+    bvh.destroy()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["render", "c0"], default="render")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-microbench", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box here has one MI355X; the multi-GPU "
+              "sharding of the render is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    from woodenpbrengine_amd import lib, scenes
+
+    t0 = time.time()
+    if args.workload == "render":
+        scene = scenes.c2_scene()
+        wl = "full wavefront render: 1024x1024 film, 16 spp, displaced icosphere (2009780 tris)"
+    else:
+        scene = scenes.c0_scene()
+        wl = "plain render: 128x128 film, 4 spp, icosphere (20480 tris)"
+    log("scene generated in %.1f s (%d shapes)" % (time.time() - t0, scene.n_shapes))
+    ctx = lib.Context(0)
+    r = lib.Renderer(ctx, scene)
+    log("scene uploaded + LBVH built")
+    for _ in range(args.warmup):
+        r.render()
+    ctx.sync()
+    dev_ms, trace_ms, shade_ms = [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        dev_ms.append(r.render())
+        st = r.stats()
+        trace_ms.append(st["trace_ms"])
+        shade_ms.append(st["shade_ms"])
+    ctx.sync()
+    wall = time.perf_counter() - t_start
+    ms_per_step = wall * 1e3 / args.steps
+    samples = scene.width * scene.height * scene.spp
+    value = samples / (ms_per_step * 1e-3) / 1e6
+    # rays traced per render: primary + compacted secondary (queue length)
+    log("render: %.2f ms/step wall, %.2f ms device, trace %.2f ms, shade %.2f ms"
+        % (ms_per_step, np.mean(dev_ms), np.mean(trace_ms), np.mean(shade_ms)))
+    img = r.film_rgb()
+    extra = {"image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
+             "device_ms_per_step": float(np.mean(dev_ms)),
+             "trace_ms_per_step": float(np.mean(trace_ms)),
+             "shade_ms_per_step": float(np.mean(shade_ms))}
+    r.destroy()
+    # Dominant kernel: traversal.  Rays per render = primary (W*H*spp) + secondary.
+    rays_per_step = None
+    mb = {}
+    if not args.no_microbench:
+        mb = bench_microbench(lib, scenes, ctx, args.steps)
+        log("microbench:", json.dumps(mb))
+    roofline = None
+    if mb:
+        tr = mb["traversal"]
+        achieved = TRACE_BYTES_PER_RAY * tr["rays"] / (tr["ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                    "kernel": "k_trace (closest hit, 4194304 rays x 36 B)"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        t1 = time.time()
+        cpu = cpu_baseline(scene)
+        log("cpu baseline in %.1f s: %s" % (time.time() - t1, cpu))
+    out = {
+        "metric": METRIC, "value": value, "unit": "Msamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded scene generators, counter-based sample streams)",
+        "config": {"workload": wl, "width": scene.width, "height": scene.height,
+                   "spp": scene.spp, "shapes": scene.n_shapes,
+                   "triangles": int(len(scene.tri_verts)), "spheres": int(len(scene.sphere_radius))},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    out.update(mb)
+    out.update(extra)
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

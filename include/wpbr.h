@@ -118,6 +118,62 @@ wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const w
  * depth (8..64).  0 keeps the current value. */
 wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth);
 
+/* ---- rendering (a14-a31) ------------------------------------------------- *
+ * Scene description: host pointers, copied to the device by wpbr_scene_create.
+ * Shape order as in wpbr_geometry.  Materials are 9 floats each: type (u32
+ * bits: 0 conductor microfacet, 1 dielectric microfacet, 2 specular reflection
+ * "perfect glass"), alpha, eta (dielectric etaT, etaI = 1), eta RGB[3] and k RGB[3]
+ * (conductor; converted with fromRGB(Reflectance) as CMaterialMetal.cpp:123-127).
+ * Lights: sphere lights (their shape ids and 64-bin emission), then the
+ * infinite env light if has_env.  The camera is a raster->camera 4x4 and a
+ * camera->world 3x4 (CSCameraProjective.h:25-52). */
+typedef struct {
+    uint32_t n_spheres;
+    const float* sphere_xform;
+    const float* sphere_radius;
+    uint32_t n_triangles;
+    const float* tri_verts;
+    const uint8_t* tri_flags;     /* bit0 per-vertex uvs, bit1 normals + tangents */
+    const float* tri_uv;          /* n_triangles x 6 */
+    const float* tri_normal;      /* n_triangles x 9 */
+    const float* tri_tangent;     /* n_triangles x 9 */
+    const uint32_t* shape_material; /* per shape, 0xFFFFFFFF = none (light sphere) */
+    uint32_t n_materials;
+    const float* materials;
+    uint32_t n_sphere_lights;
+    const uint32_t* light_shape;
+    const float* light_emit;      /* n_sphere_lights x 64 */
+    uint32_t has_env;
+    const float* env_emit;        /* 64 */
+    const float* env_rgb;         /* env_h x env_w x 3 */
+    uint32_t env_w, env_h;
+    const float* env_xform;       /* world 3x4 then inverse 3x4 */
+    const float* raster_camera;   /* 16 */
+    const float* camera_world;    /* 12 */
+    uint32_t width, height, spp, seed;
+} wpbr_scene_desc;
+
+typedef struct wpbr_scene wpbr_scene;
+
+/* Uploads the scene, runs the load-time preprocessing (spectral tables,
+ * SampledSpectrum::init CSpectrumSampled.h:236-281; env-light distribution,
+ * JobLightInfiniteAreaPreprocessSampling CLightInfiniteArea.cpp:113-139) and
+ * builds its LBVH (wpbr_lbvh_build). */
+wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* desc, wpbr_scene** out);
+void wpbr_scene_destroy(wpbr_scene* scene);
+/* Rebuilds the scene's LBVH from its device geometry. */
+wpbr_status wpbr_scene_rebuild(wpbr_ctx* ctx, wpbr_scene* scene);
+/* Renders spp samples per pixel (all film tiles per pass).  d_film receives
+ * width*height*4 floats: the Mitchell-weighted sums of X, Y, Z and of the
+ * weights (toRGB of the sum / weight is the reference's film value before its
+ * tonemap).  d_sample_xyz (optional) receives width*height*spp*3 floats, the
+ * XYZ of each sample's radiance at index (pixel * spp + j). */
+wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* d_sample_xyz);
+/* Statistics of the last wpbr_render: [0] total ms, [1] trace ms (all trace
+ * launches), [2] number of trace launches, [3] rays traced, [4] shade ms,
+ * [5] camera ms, [6] secondary rays. Returns the number written. */
+int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap);
+
 #ifdef __cplusplus
 }
 #endif

@@ -3,6 +3,7 @@
 #include "oracle.h"
 #include <thread>
 #include <atomic>
+#include <chrono>
 
 using namespace orc;
 
@@ -247,7 +248,11 @@ void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, u
     LbvhResult bvh;
     lbvh_build(n, b.data(), c.data(), nullptr, bvh);
     RenderOut* out = new RenderOut();
+    auto t0 = std::chrono::steady_clock::now();
     render(sc, bvh, *out, nthreads, tiles, n_tiles);
+    auto t1 = std::chrono::steady_clock::now();
+    // counters[7]: wall time of the render loop alone (ns), excluding the build
+    out->counters[7] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
     return out;
 }
 

@@ -43,7 +43,7 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     const size_t tiles = (N + 3839) / 3840;
     s.bounds = c.take<float>(6 * N);
     s.centroid = c.take<float>(3 * N);
-    s.cb_ord = c.take<uint32_t>(8);
+    s.cb_ord = c.take<uint32_t>(8 + 6 * 512);
     s.morton = c.take<uint32_t>(N);
     s.keys_a = c.take<uint32_t>(N);
     s.vals_a = c.take<uint32_t>(N);
@@ -85,6 +85,10 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.u_size = c.take<uint32_t>(4096);
     s.u_off = c.take<uint32_t>(4096);
     s.u_box = c.take<float>(6 * 4096);
+    s.u_parent = c.take<uint32_t>(4096);
+    s.u_arrive = c.take<uint32_t>(4096);
+    s.t_parent = c.take<uint32_t>(4096);
+    s.t_cen = c.take<float>(3 * 4096);
     s.upper_count = c.take<uint32_t>(1);
     s.upper_root = c.take<uint32_t>(1);
     s.n_nodes = c.take<uint32_t>(1);
@@ -93,8 +97,10 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.node_nshapes = c.take<uint32_t>(2 * N);
     s.node_axis = c.take<uint8_t>(2 * N);
     s.node_bounds = c.take<float>(12 * N);
-    s.trav = c.take<float4>(4 * 2 * N);
+    s.trav = c.take<float4>(4 * 3 * N);
     s.prims = c.take<float4>(3 * N);
+    s.sub_perm = c.take<uint32_t>(N);
+    s.sub_count = c.take<uint32_t>(1);
     return c.off + 256;
 }
 
@@ -203,7 +209,9 @@ wpbr_status wpbr_shape_bounds(wpbr_ctx* ctx, const wpbr_geometry* geom, float* d
     if (!ctx || !geom || !d_bounds || !d_centroid) return WPBR_ERR_INVALID_ARG;
     hipSetDevice(ctx->device);
     uint32_t* cb = nullptr;
-    wpbr_status s = hip_check(ctx, hipMalloc(&cb, 32), "hipMalloc");
+    const size_t nshape = (size_t)geom->n_spheres + geom->n_triangles;
+    (void)nshape;
+    wpbr_status s = hip_check(ctx, hipMalloc(&cb, 4 * (8 + 6 * 512)), "hipMalloc");
     if (s) return s;
     s = shape_bounds_launch(to_view(geom), d_bounds, d_centroid, cb, ctx->stream);
     if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "shape bounds");
@@ -234,12 +242,13 @@ void wpbr_lbvh_destroy(wpbr_lbvh* b) {
 wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b) {
     if (!ctx || !geom || !b) return WPBR_ERR_INVALID_ARG;
     uint64_t n64 = (uint64_t)geom->n_spheres + geom->n_triangles;
-    if (n64 == 0 || n64 >= (1ull << 30)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count");
+    if (n64 == 0 || n64 >= (1ull << 28)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count (1 .. 2^28-1)");
     if ((geom->n_spheres && (!geom->d_sphere_xform || !geom->d_sphere_radius)) ||
         (geom->n_triangles && !geom->d_tri_verts))
         return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
     uint32_t n = (uint32_t)n64;
     hipSetDevice(ctx->device);
+    (void)hipGetLastError();  // launch checks below must not see stale errors
     if (n > b->cap) {
         if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
         LbvhScratch tmp;
@@ -256,14 +265,17 @@ wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh*
     uint32_t info[2] = {0, 0};
     s = hip_check(ctx, hipMemcpyAsync(&info[0], b->s.n_nodes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (!s) s = hip_check(ctx, hipMemcpyAsync(&info[1], b->s.n_treelets, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    uint32_t err = 0, ns0 = 0;
+    uint32_t err = 0, ns0 = 0, off0 = 0;
     if (!s) s = hip_check(ctx, hipMemcpyAsync(&err, b->s.err, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (!s) s = hip_check(ctx, hipMemcpyAsync(&ns0, b->s.node_nshapes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipMemcpyAsync(&off0, b->s.node_offset, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "lbvh build");
     if (s) return s;
     b->n_nodes = info[0];
     b->n_treelets = info[1];
     b->root_is_leaf = ns0 > 0;
+    // the root record, or a primitive range when the root is a leaf of <= 4 shapes
+    b->root_word = (ns0 > 0 && ns0 <= 4) ? (0x80000000u | ((ns0 - 1) << 28) | off0) : 0u;
     float ms = 0;
     hipEventElapsedTime(&ms, b->ev[0], b->ev[wpbr_lbvh::NEV - 1]);
     ctx->last_ms = ms;
@@ -333,6 +345,7 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     if ((uint64_t)geom->n_spheres + geom->n_triangles != b->n)
         return set_err(ctx, WPBR_ERR_INVALID_ARG, "geometry does not match the lbvh");
     hipSetDevice(ctx->device);
+    (void)hipGetLastError();  // launch checks below must not see stale errors
     wpbr_status s = ctx_ensure_trace(ctx, n);
     if (s) return s;
     TraceArgs a;
@@ -340,8 +353,10 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     a.trav = (const float4*)b->s.trav;
     a.prims = (const float4*)b->s.prims;
     a.root_box = b->s.node_bounds;
-    a.root_is_leaf = b->root_is_leaf;
+    a.root_word = b->root_word;
     a.n = n;
+    a.n_dev = nullptr;
+    a.stride = 0;
     a.rays = d_rays;
     a.hit_shape = d_hit_shape;
     a.hit_t = d_hit_t;
@@ -354,7 +369,9 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     hipEventRecord(ctx->ev1, ctx->stream);
     if (s) return set_err(ctx, s, "trace launch failed");
     if (!wait) return WPBR_OK;
-    return ctx_finish_trace(ctx);
+    s = ctx_finish_trace(ctx);
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
 }
 
 wpbr_status wpbr_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* b, uint32_t n,
@@ -414,7 +431,6 @@ wpbr_status ctx_finish_trace(wpbr_ctx* ctx) {
     wpbr_status s = hip_check(ctx, hipMemcpyAsync(&err, ctx->err_flag, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "collide");
     if (s) return s;
-    hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1);
     if (err) {
         hipMemsetAsync(ctx->err_flag, 0, 16, ctx->stream);
         hipStreamSynchronize(ctx->stream);

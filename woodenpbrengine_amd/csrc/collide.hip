@@ -94,10 +94,11 @@ template <int BLOCK, int SD>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ uint32_t stack[SD * BLOCK];
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= a.n) return;
-    const uint32_t n = a.n;
-    f3 o = mk3(a.rays[i], a.rays[n + i], a.rays[2 * (size_t)n + i]);
-    f3 d = mk3(a.rays[3 * (size_t)n + i], a.rays[4 * (size_t)n + i], a.rays[5 * (size_t)n + i]);
+    const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+    if (i >= n) return;
+    const size_t rs = a.stride ? a.stride : n;
+    f3 o = mk3(a.rays[i], a.rays[rs + i], a.rays[2 * rs + i]);
+    f3 d = mk3(a.rays[3 * rs + i], a.rays[4 * rs + i], a.rays[5 * rs + i]);
     f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
     const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
     // Triangle setup shared by every candidate (pbrt-v3 MaxDimension, shear).
@@ -109,17 +110,17 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     float sx = -dkx / dkz, sy = -dky / dkz, sz = 1.0f / dkz;
 
     float tBest = INF;
-    uint32_t best = 0xFFFFFFFFu;
+    uint32_t best = 0xFFFFFFFFu, bestSlot = 0xFFFFFFFFu, bestLeaf = 0xFFFFFFFFu;
     float bb[3] = {0.0f, 0.0f, 0.0f};
     const float* rb = a.root_box;
     uint32_t node = 0xFFFFFFFFu;  // 0xFFFFFFFF = done
     if (slab_hit(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, inv, nx, ny, nz, INF))
-        node = a.root_is_leaf ? 0x80000000u : 0u;
+        node = a.root_word;
     uint32_t sp = 0;
     const uint32_t spill_base = i;
     auto push = [&](uint32_t v) {
         if (sp < (uint32_t)SD) stack[sp * BLOCK + threadIdx.x] = v;
-        else if (sp - SD < a.spill_depth) a.spill[(size_t)(sp - SD) * n + spill_base] = v;
+        else if (sp - SD < a.spill_depth) a.spill[(size_t)(sp - SD) * a.n + spill_base] = v;
         else { atomicOr(a.err, 1u); return; }
         sp++;
     };
@@ -127,10 +128,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
         if (sp == 0) return 0xFFFFFFFFu;
         sp--;
         return sp < (uint32_t)SD ? stack[sp * BLOCK + threadIdx.x]
-                                 : a.spill[(size_t)(sp - SD) * n + spill_base];
+                                 : a.spill[(size_t)(sp - SD) * a.n + spill_base];
     };
     while (node != 0xFFFFFFFFu) {
-        // Interior nodes.
+        // Records: test both children, near first by dirIsNeg[axis].
         while (!(node & 0x80000000u)) {
             const float4* r = a.trav + 4 * (size_t)node;
             float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
@@ -147,24 +148,28 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
             else if (hf) node = farW;
             else { node = pop(); if (node == 0xFFFFFFFFu) break; }
         }
-        // Leaves.
+        // Primitive ranges.
         while (node != 0xFFFFFFFFu && (node & 0x80000000u)) {
-            const float4 lr = a.trav[4 * (size_t)(node & 0x7FFFFFFFu) + 3];
-            uint32_t off = __float_as_uint(lr.x), cnt = __float_as_uint(lr.y);
-            for (uint32_t j = off; j < off + cnt; j++) {
+            uint32_t first = node & 0x0FFFFFFFu, cnt = ((node >> 28) & 7u) + 1u;
+            for (uint32_t j = first; j < first + cnt; j++) {
                 const float4* p = a.prims + 3 * (size_t)j;
-                float4 v0 = p[0];
+                float4 v0 = p[0], v1 = p[1], v2 = p[2];
                 uint32_t id = __float_as_uint(v0.w);
+                uint32_t slot = __float_as_uint(v1.w), leaf = __float_as_uint(v2.w);
                 float t;
                 float b3[3] = {0.0f, 0.0f, 0.0f};
                 if (id < a.g.n_spheres) {
                     t = sphere_hit(a.g, id, o, d);
                 } else {
-                    t = tri_hit(v0, p[1], p[2], o, d, kx, ky, kz, sx, sy, sz, b3);
+                    t = tri_hit(v0, v1, v2, o, d, kx, ky, kz, sx, sy, sz, b3);
                 }
-                if (t != 0.0f && t < tBest) {
+                // first strict minimum in candidate order: leaves in depth-first
+                // order, slots in order inside a leaf
+                if (t != 0.0f && (t < tBest || (t == tBest && leaf == bestLeaf && slot < bestSlot))) {
                     tBest = t;
                     best = id;
+                    bestSlot = slot;
+                    bestLeaf = leaf;
                     bb[0] = b3[0]; bb[1] = b3[1]; bb[2] = b3[2];
                 }
             }
@@ -175,8 +180,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     a.hit_t[i] = tBest;
     if (a.bary) {
         a.bary[i] = bb[0];
-        a.bary[(size_t)n + i] = bb[1];
-        a.bary[2 * (size_t)n + i] = bb[2];
+        a.bary[rs + i] = bb[1];
+        a.bary[2 * rs + i] = bb[2];
     }
 }
 
@@ -201,7 +206,7 @@ wpbr_status trace_launch(const TraceArgs& a, int block, int sd, hipStream_t st) 
         case 512: launch_sd<512>(a, sd, st); break;
         default: launch_sd<256>(a, sd, st); break;
     }
-    return hipGetLastError() == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+    return launch_status();
 }
 
 }  // namespace wpbr

@@ -45,8 +45,7 @@ __global__ void k_init(LbvhScratch s) {
     for (uint32_t j = i; j < (uint32_t)s.status_words; j += stride) s.status0[j] = 0;
     for (uint32_t j = i; j < 128; j += stride) s.presence[j] = 0;
     if (i < 8) s.tile_counter[i] = 0;
-    if (i < 3) { s.cb_ord[i] = f2ord(FLT_MAX_); s.cb_ord[3 + i] = f2ord(FLT_LOWEST_); }
-    if (i == 0) { *s.err = 0; *s.upper_count = 0; }
+    if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; }
 }
 
 // ---------------------------------------------------------------------------
@@ -69,11 +68,18 @@ WPBR_D void shape_box(const GeomView& g, uint32_t i, bbox& b) {
     }
 }
 
+// Centroid bounds in two stages without atomics or device-wide fences: each
+// k_bounds block (a grid-stride loop over a fixed number of blocks) writes its
+// partial min/max (order-preserving uints) to cb_ord[8 + 6*b]; the next kernel
+// reduces the partials (a kernel boundary orders the writes).  cb_ord[0..5]
+// receive the final bounds, cb_ord[7] the number of partials.
+constexpr int BOUNDS_BLOCKS = 512;
+
 __global__ __launch_bounds__(256) void k_bounds(GeomView g, uint32_t n, float* bounds,
                                                 float* centroid, uint32_t* cb_ord) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t mn[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, mx[3] = {0, 0, 0};
-    if (i < n) {
+    __shared__ uint32_t s_red[4][6];
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0};  // min xyz, max xyz
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         bbox b;
         shape_box(g, i, b);
         store_box(bounds + 6 * (size_t)i, b);
@@ -81,25 +87,59 @@ __global__ __launch_bounds__(256) void k_bounds(GeomView g, uint32_t n, float* b
         for (int k = 0; k < 3; k++) {
             float c = (float)((double)b.mx[k] * 0.5 + (double)b.mn[k] * 0.5);
             centroid[3 * (size_t)i + k] = c;
-            mn[k] = f2ord(c);
-            mx[k] = mn[k];
+            uint32_t o = f2ord(c);
+            v[k] = min(v[k], o);
+            v[3 + k] = max(v[3 + k], o);
         }
     }
 #pragma unroll
     for (int k = 0; k < 3; k++) {
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) {
-            mn[k] = min(mn[k], (uint32_t)__shfl_xor((int)mn[k], off));
-            mx[k] = max(mx[k], (uint32_t)__shfl_xor((int)mx[k], off));
+            v[k] = min(v[k], (uint32_t)__shfl_xor((int)v[k], off));
+            v[3 + k] = max(v[3 + k], (uint32_t)__shfl_xor((int)v[3 + k], off));
         }
     }
-    if (lane_id() == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            if (mn[k] != 0xFFFFFFFFu) atomicMin(&cb_ord[k], mn[k]);
-            if (mx[k] != 0) atomicMax(&cb_ord[3 + k], mx[k]);
+    const int w = threadIdx.x >> 6;
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; k++) s_red[w][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        uint32_t r = s_red[0][k];
+        for (int ww = 1; ww < 4; ww++) r = (k < 3) ? min(r, s_red[ww][k]) : max(r, s_red[ww][k]);
+        cb_ord[8 + 6 * (size_t)blockIdx.x + k] = r;
+        if (blockIdx.x == 0 && k == 0) cb_ord[7] = gridDim.x;
+    }
+}
+
+// Reduce the k_bounds partials (called by every thread of a consumer block;
+// blockDim.x must be a multiple of 64 and at most 1024).
+WPBR_D void reduce_cb(const uint32_t* cb_ord, uint32_t* s_cb /* LDS[6] */, uint32_t* out_or_null) {
+    __shared__ uint32_t s_w[16][6];
+    const uint32_t nb = cb_ord[7];
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < nb; b += blockDim.x) {
+        const uint32_t* p = cb_ord + 8 + 6 * (size_t)b;
+        for (int k = 0; k < 3; k++) { v[k] = min(v[k], p[k]); v[3 + k] = max(v[3 + k], p[3 + k]); }
+    }
+    for (int k = 0; k < 3; k++) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            v[k] = min(v[k], (uint32_t)__shfl_xor((int)v[k], off));
+            v[3 + k] = max(v[3 + k], (uint32_t)__shfl_xor((int)v[3 + k], off));
         }
     }
+    if (lane_id() == 0)
+        for (int k = 0; k < 6; k++) s_w[threadIdx.x >> 6][k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        int k = threadIdx.x;
+        uint32_t r = s_w[0][k];
+        for (uint32_t w = 1; w < blockDim.x / 64; w++) r = (k < 3) ? min(r, s_w[w][k]) : max(r, s_w[w][k]);
+        s_cb[k] = r;
+        if (out_or_null) out_or_null[k] = r;
+    }
+    __syncthreads();
 }
 
 // ---------------------------------------------------------------------------
@@ -119,11 +159,12 @@ __global__ __launch_bounds__(MORTON_BLOCK) void k_morton(const float* centroid, 
                                                          const uint32_t* cb_ord, uint32_t* codes,
                                                          uint32_t* hist) {
     __shared__ uint32_t h[4 * 256];
+    __shared__ uint32_t s_cb[6];
     for (int j = threadIdx.x; j < 4 * 256; j += MORTON_BLOCK) h[j] = 0;
-    __syncthreads();
+    reduce_cb(cb_ord, s_cb, blockIdx.x == 0 ? const_cast<uint32_t*>(cb_ord) : nullptr);
     float mn[3], mx[3];
 #pragma unroll
-    for (int k = 0; k < 3; k++) { mn[k] = ord2f(cb_ord[k]); mx[k] = ord2f(cb_ord[3 + k]); }
+    for (int k = 0; k < 3; k++) { mn[k] = ord2f(s_cb[k]); mx[k] = ord2f(s_cb[3 + k]); }
     uint32_t base = blockIdx.x * (MORTON_BLOCK * MORTON_ITEMS);
 #pragma unroll 4
     for (int it = 0; it < MORTON_ITEMS; it++) {
@@ -440,12 +481,18 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32
 }
 
 // ---------------------------------------------------------------------------
-// Upper SAH over treelet roots: one workgroup, level-synchronous, one wave per
-// task.  A task is a range [i0, i1] of the position array `perm` (treelet ids)
-// together with the slot in its parent where the result is linked.
+// Upper SAH over treelet roots (buildUpperSAH, SLBVH.cpp:457-563): one
+// workgroup.  The position array `perm` (treelet ids, permuted in place as
+// std::partition does) and the per-position bucket live in LDS.  Tasks (ranges of
+// perm) larger than LANE_TASK_MAX are processed level by level, one wave each;
+// smaller ranges are finished by a single lane, depth first, with the oracle's
+// sequential algorithm.  Node sizes and bounds are then formed bottom-up with
+// arrival counters, and preorder offsets top-down by walking parent links.
 constexpr int SAH_BLOCK = 1024;
 constexpr int SAH_WAVES = SAH_BLOCK / 64;
 constexpr int NB = 12;
+constexpr uint32_t LANE_TASK_MAX = 32;
+constexpr uint32_t NONE = 0xFFFFFFFFu;
 
 WPBR_D float wave_min_f(float v) {
     for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
@@ -455,194 +502,256 @@ WPBR_D float wave_max_f(float v) {
     for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
     return v;
 }
-WPBR_D uint32_t wave_sum_u(uint32_t v) {
-    for (int off = 32; off >= 1; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
-    return v;
+
+WPBR_D float c_at(const LbvhScratch& s, uint32_t t, int dim) { return s.t_cen[3 * (size_t)t + dim]; }
+
+// Link a child reference (bit 31: treelet) under parent reference (node << 1 | side).
+WPBR_D void link_child(LbvhScratch& s, uint32_t parent, uint32_t ref) {
+    if (parent == NONE) *s.upper_root = ref;
+    else if (parent & 1u) s.u_c1[parent >> 1] = ref;
+    else s.u_c0[parent >> 1] = ref;
+    if (ref & 0x80000000u) s.t_parent[ref & 0x7FFFFFFFu] = parent;
+    else s.u_parent[ref] = parent;
+}
+
+// SAH split of perm[i0..i1] by one lane (the oracle's loop, SLBVH.cpp:465-553).
+WPBR_D uint32_t lane_split(LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint32_t i0, uint32_t i1,
+                           int* dimOut) {
+    bbox bounds = box_empty(), cb = box_empty();
+    for (uint32_t i = i0; i <= i1; i++) {
+        uint32_t t = perm[i];
+        bounds = box_union(bounds, load_box(s.t_box + 6 * (size_t)t));
+        cb = box_union_pt(cb, mk3(c_at(s, t, 0), c_at(s, t, 1), c_at(s, t, 2)));
+    }
+    int dim = box_max_extent(cb);
+    float cmin = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
+    float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cmin;
+    bbox bb[NB];
+    uint32_t bc[NB];
+    for (int b = 0; b < NB; b++) { bb[b] = box_empty(); bc[b] = 0; }
+    for (uint32_t i = i0; i <= i1; i++) {
+        uint32_t t = perm[i];
+        uint32_t ib = 0;
+        if (dist > 0.0f) {
+            float tt = (c_at(s, t, dim) - cmin) / dist;
+            ib = (uint32_t)(uint8_t)(uint32_t)((float)NB * tt);
+            if (ib == NB) ib = NB - 1;
+        }
+        bucket[i] = (uint8_t)ib;
+        bc[ib]++;
+        bb[ib] = box_union(bb[ib], load_box(s.t_box + 6 * (size_t)t));
+    }
+    float area = box_area(bounds);
+    float cbest = 0.0f;
+    int best = 0;
+    for (int i = 0; i < NB - 1; i++) {
+        bbox b0 = box_empty(), b1 = box_empty();
+        uint32_t c0 = 0, c1 = 0;
+        for (int j = 0; j <= i; j++) { b0 = box_union(b0, bb[j]); c0 += bc[j]; }
+        for (int j = i + 1; j < NB; j++) { b1 = box_union(b1, bb[j]); c1 += bc[j]; }
+        float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+        float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+        float cost = (float)(0.125 + (double)((l + r) / area));
+        if (i == 0) cbest = cost;
+        else if (cost < cbest) { best = i; cbest = cost; }
+    }
+    // libstdc++ two-pointer std::partition on perm[i0..i1]
+    uint32_t first = i0, last = i1 + 1;
+    while (true) {
+        while (first != last && bucket[first] <= best) ++first;
+        if (first == last) break;
+        --last;
+        while (first != last && !(bucket[last] <= best)) --last;
+        if (first == last) break;
+        uint16_t tp = perm[first]; perm[first] = perm[last]; perm[last] = tp;
+        uint8_t tb = bucket[first]; bucket[first] = bucket[last]; bucket[last] = tb;
+        ++first;
+    }
+    uint32_t mid = first;
+    if (mid == i1 + 1) mid = (i1 - i0 + 1) / 2 + i0;
+    *dimOut = dim;
+    return mid;
 }
 
 __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t n_total_shapes) {
+    __shared__ uint16_t s_perm[4096];
+    __shared__ uint8_t s_bucket[4096];
     __shared__ uint32_t s_k;
     __shared__ uint32_t s_task_next;
-    __shared__ uint32_t s_level_end;
     __shared__ uint32_t s_bcount[SAH_WAVES][NB];
     __shared__ uint32_t s_bmin[SAH_WAVES][NB][3];
     __shared__ uint32_t s_bmax[SAH_WAVES][NB][3];
-    __shared__ uint32_t s_depth_max;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     if (tid == 0) {
         uint32_t k = 0;
         for (int w = 0; w < 128; w++) k += __popc(s.presence[w]);
         s_k = k;
-        s_depth_max = 0;
-    }
-    __syncthreads();
-    const uint32_t k = s_k;
-    for (uint32_t i = tid; i < k; i += SAH_BLOCK) s.perm[i] = i;
-    // Task arrays in global scratch: tasks of level L are [lvl_begin, lvl_end).
-    if (tid == 0) {
         s.task_i0[0] = 0;
         s.task_i1[0] = k - 1;
-        s.task_parent[0] = 0xFFFFFFFFu;  // root
-        s.task_depth[0] = 0;
+        s.task_parent[0] = NONE;
         *s.upper_count = 0;
         s_task_next = 1;
     }
     __syncthreads();
+    const uint32_t k = s_k;
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK) {
+        s_perm[i] = (uint16_t)i;
+        bbox rb = load_box(s.t_box + 6 * (size_t)i);
+#pragma unroll
+        for (int c = 0; c < 3; c++) s.t_cen[3 * (size_t)i + c] = (float)((double)(rb.mn[c] + rb.mx[c]) * 0.5);
+        s.u_arrive[i] = 0;
+    }
+    __threadfence_block();
+    __syncthreads();
     uint32_t lb = 0, le = 1;
-    uint32_t num_upper = 0;
     while (lb < le) {
-        // Each wave takes tasks lb + wave, lb + wave + SAH_WAVES, ...
+        // ---- large ranges: one wave each
         for (uint32_t t = lb + wave; t < le; t += SAH_WAVES) {
             uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
+            if (i1 - i0 + 1 <= LANE_TASK_MAX) continue;
             uint32_t parent = s.task_parent[t];
-            uint32_t depth = s.task_depth[t];
-            uint32_t ref;
-            if (i0 == i1) {
-                ref = 0x80000000u | s.perm[i0];  // treelet root
-            } else {
-                // ---- bounds and centroid bounds over [i0, i1]
-                float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-                float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-                for (uint32_t i = i0 + lane; i <= i1; i += 64) {
-                    bbox rb = load_box(s.t_box + 6 * (size_t)s.perm[i]);
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        bmn[c] = smin(bmn[c], rb.mn[c]);
-                        bmx[c] = smax(bmx[c], rb.mx[c]);
-                        float cc = (float)((double)(rb.mn[c] + rb.mx[c]) * 0.5);
-                        cmn[c] = smin(cmn[c], cc);
-                        cmx[c] = smax(cmx[c], cc);
-                    }
-                }
-                bbox bounds, cb;
+            float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
+            float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
+            for (uint32_t i = i0 + lane; i <= i1; i += 64) {
+                uint32_t tr = s_perm[i];
+                bbox rb = load_box(s.t_box + 6 * (size_t)tr);
 #pragma unroll
                 for (int c = 0; c < 3; c++) {
-                    bounds.mn[c] = wave_min_f(bmn[c]);
-                    bounds.mx[c] = wave_max_f(bmx[c]);
-                    cb.mn[c] = wave_min_f(cmn[c]);
-                    cb.mx[c] = wave_max_f(cmx[c]);
+                    bmn[c] = smin(bmn[c], rb.mn[c]);
+                    bmx[c] = smax(bmx[c], rb.mx[c]);
+                    float cc = c_at(s, tr, c);
+                    cmn[c] = smin(cmn[c], cc);
+                    cmx[c] = smax(cmx[c], cc);
                 }
-                int dim = box_max_extent(cb);
-                float dist = (dim == 0 ? cb.mx[0] - cb.mn[0] : (dim == 1 ? cb.mx[1] - cb.mn[1] : cb.mx[2] - cb.mn[2]));
-                float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
-                // ---- buckets
-                if (lane < NB) {
-                    s_bcount[wave][lane] = 0;
-                    for (int c = 0; c < 3; c++) {
-                        s_bmin[wave][lane][c] = f2ord(FLT_MAX_);
-                        s_bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
-                    }
-                }
-                __builtin_amdgcn_wave_barrier();
-                for (uint32_t i = i0 + lane; i <= i1; i += 64) {
-                    bbox rb = load_box(s.t_box + 6 * (size_t)s.perm[i]);
-                    float rmn = dim == 0 ? rb.mn[0] : (dim == 1 ? rb.mn[1] : rb.mn[2]);
-                    float rmx = dim == 0 ? rb.mx[0] : (dim == 1 ? rb.mx[1] : rb.mx[2]);
-                    float centroid = (float)((double)(rmn + rmx) * 0.5);
-                    uint32_t ib = 0;
-                    if (dist > 0.0f) {
-                        float tt = (centroid - cbmn) / dist;
-                        ib = (uint32_t)(uint8_t)(uint32_t)((float)NB * tt);
-                        if (ib == NB) ib = NB - 1;
-                    }
-                    s.bucket[i] = (uint8_t)ib;
-                    atomicAdd(&s_bcount[wave][ib], 1u);
+            }
+            bbox bounds, cb;
 #pragma unroll
+            for (int c = 0; c < 3; c++) {
+                bounds.mn[c] = wave_min_f(bmn[c]);
+                bounds.mx[c] = wave_max_f(bmx[c]);
+                cb.mn[c] = wave_min_f(cmn[c]);
+                cb.mx[c] = wave_max_f(cmx[c]);
+            }
+            int dim = box_max_extent(cb);
+            float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
+            float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
+            if (lane < NB) {
+                s_bcount[wave][lane] = 0;
+                for (int c = 0; c < 3; c++) {
+                    s_bmin[wave][lane][c] = f2ord(FLT_MAX_);
+                    s_bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t i = i0 + lane; i <= i1; i += 64) {
+                uint32_t tr = s_perm[i];
+                bbox rb = load_box(s.t_box + 6 * (size_t)tr);
+                uint32_t ib = 0;
+                if (dist > 0.0f) {
+                    float tt = (c_at(s, tr, dim) - cbmn) / dist;
+                    ib = (uint32_t)(uint8_t)(uint32_t)((float)NB * tt);
+                    if (ib == NB) ib = NB - 1;
+                }
+                s_bucket[i] = (uint8_t)ib;
+                atomicAdd(&s_bcount[wave][ib], 1u);
+#pragma unroll
+                for (int c = 0; c < 3; c++) {
+                    atomicMin(&s_bmin[wave][ib][c], f2ord(rb.mn[c]));
+                    atomicMax(&s_bmax[wave][ib][c], f2ord(rb.mx[c]));
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            float cost = INF;
+            uint32_t cnt_le = 0;
+            if (lane < NB - 1) {
+                bbox b0 = box_empty(), b1 = box_empty();
+                uint32_t c0 = 0, c1 = 0;
+                for (int j = 0; j < NB; j++) {
+                    bbox bj;
                     for (int c = 0; c < 3; c++) {
-                        atomicMin(&s_bmin[wave][ib][c], f2ord(rb.mn[c]));
-                        atomicMax(&s_bmax[wave][ib][c], f2ord(rb.mx[c]));
+                        bj.mn[c] = ord2f(s_bmin[wave][j][c]);
+                        bj.mx[c] = ord2f(s_bmax[wave][j][c]);
                     }
+                    if (j <= lane) { b0 = box_union(b0, bj); c0 += s_bcount[wave][j]; }
+                    else { b1 = box_union(b1, bj); c1 += s_bcount[wave][j]; }
+                }
+                float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+                float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+                cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
+                cnt_le = c0;
+            }
+            int best = 0;
+            float cmin = __shfl(cost, 0);
+            for (int i = 1; i < NB - 1; i++) {
+                float ci = __shfl(cost, i);
+                if (ci < cmin) { best = i; cmin = ci; }
+            }
+            uint32_t T = (uint32_t)__shfl((int)cnt_le, best);
+            uint32_t size = i1 - i0 + 1;
+            uint32_t mid;
+            if (T == size) {
+                mid = size / 2 + i0;
+            } else {
+                // two-pointer partition == swap the r-th misplaced element of the left
+                // region [i0, i0+T) with the r-th (from the right) of the right region
+                uint32_t rl = 0;
+                for (uint32_t c = 0; c * 64 < T; c++) {
+                    uint32_t i = i0 + c * 64 + lane;
+                    bool bad = (c * 64 + lane < T) && (s_bucket[i] > (uint32_t)best);
+                    uint64_t m = ballot64(bad);
+                    if (bad) s.list_a[i0 + rl + popc64(m & lanemask_lt())] = i;
+                    rl += popc64(m);
+                }
+                uint32_t rr = 0, rsz = size - T;
+                for (uint32_t c = 0; c * 64 < rsz; c++) {
+                    uint32_t i = i1 - (c * 64 + lane);
+                    bool bad = (c * 64 + lane < rsz) && (s_bucket[i] <= (uint32_t)best);
+                    uint64_t m = ballot64(bad);
+                    if (bad) s.list_b[i0 + rr + popc64(m & lanemask_lt())] = i;
+                    rr += popc64(m);
                 }
                 __threadfence_block();
                 __builtin_amdgcn_wave_barrier();
-                // ---- costs (lane i computes cost i), first minimum
-                float cost = INF;
-                uint32_t cnt_le = 0;
-                if (lane < NB - 1) {
-                    bbox b0 = box_empty(), b1 = box_empty();
-                    uint32_t c0 = 0, c1 = 0;
-                    for (int j = 0; j < NB; j++) {
-                        bbox bj;
-                        for (int c = 0; c < 3; c++) {
-                            bj.mn[c] = ord2f(s_bmin[wave][j][c]);
-                            bj.mx[c] = ord2f(s_bmax[wave][j][c]);
-                        }
-                        if (j <= lane) { b0 = box_union(b0, bj); c0 += s_bcount[wave][j]; }
-                        else { b1 = box_union(b1, bj); c1 += s_bcount[wave][j]; }
-                    }
-                    float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
-                    float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
-                    cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
-                    cnt_le = c0;
+                for (uint32_t r = lane; r < rl; r += 64) {
+                    uint32_t a = s.list_a[i0 + r], b = s.list_b[i0 + r];
+                    uint16_t pa = s_perm[a], pb = s_perm[b];
+                    s_perm[a] = pb;
+                    s_perm[b] = pa;
                 }
-                int best = 0;
-                float cmin = __shfl(cost, 0);
-                for (int i = 1; i < NB - 1; i++) {
-                    float ci = __shfl(cost, i);
-                    if (ci < cmin) { best = i; cmin = ci; }
-                }
-                uint32_t T = __shfl((int)cnt_le, best);
-                uint32_t size = i1 - i0 + 1;
-                uint32_t mid;
-                if (T == size) {
-                    mid = size / 2 + i0;  // no element moves; midpoint fallback (SLBVH.cpp:550-553)
-                } else {
-                    // libstdc++ two-pointer partition == swap the r-th misplaced element
-                    // of the left region [i0, i0+T) with the r-th (from the right)
-                    // misplaced element of the right region.
-                    uint32_t rl = 0;
-                    for (uint32_t c = 0; c * 64 < T; c++) {
-                        uint32_t i = i0 + c * 64 + lane;
-                        bool bad = (c * 64 + lane < T) && (s.bucket[i] > (uint32_t)best);
-                        uint64_t m = ballot64(bad);
-                        if (bad) s.list_a[i0 + rl + popc64(m & lanemask_lt())] = i;
-                        rl += popc64(m);
-                    }
-                    uint32_t rr = 0;
-                    uint32_t rsz = size - T;
-                    for (uint32_t c = 0; c * 64 < rsz; c++) {
-                        uint32_t i = i1 - (c * 64 + lane);
-                        bool bad = (c * 64 + lane < rsz) && (s.bucket[i] <= (uint32_t)best);
-                        uint64_t m = ballot64(bad);
-                        if (bad) s.list_b[i0 + rr + popc64(m & lanemask_lt())] = i;
-                        rr += popc64(m);
-                    }
-                    __threadfence_block();
-                    __builtin_amdgcn_wave_barrier();
-                    for (uint32_t r = lane; r < rl; r += 64) {
-                        uint32_t a = s.list_a[i0 + r], b = s.list_b[i0 + r];
-                        uint32_t pa = s.perm[a], pb = s.perm[b];
-                        s.perm[a] = pb;
-                        s.perm[b] = pa;
-                    }
-                    __threadfence_block();
-                    __builtin_amdgcn_wave_barrier();
-                    mid = i0 + T;
-                }
-                // ---- create the node and two child tasks
-                uint32_t node = 0;
-                if (lane == 0) {
-                    node = atomicAdd(s.upper_count, 1u);
-                    uint32_t ct = atomicAdd(&s_task_next, 2u);
-                    s.u_axis[node] = (uint32_t)dim;
-                    s.u_depth[node] = depth;
-                    s.task_i0[ct] = i0;
-                    s.task_i1[ct] = mid - 1;
-                    s.task_parent[ct] = node << 1;
-                    s.task_depth[ct] = depth + 1;
-                    s.task_i0[ct + 1] = mid;
-                    s.task_i1[ct + 1] = i1;
-                    s.task_parent[ct + 1] = (node << 1) | 1u;
-                    s.task_depth[ct + 1] = depth + 1;
-                    atomicMax(&s_depth_max, depth);
-                }
-                node = (uint32_t)__shfl((int)node, 0);
-                ref = node;
+                __builtin_amdgcn_wave_barrier();
+                mid = i0 + T;
             }
             if (lane == 0) {
-                if (parent == 0xFFFFFFFFu) *s.upper_root = ref;
-                else if (parent & 1u) s.u_c1[parent >> 1] = ref;
-                else s.u_c0[parent >> 1] = ref;
+                uint32_t node = atomicAdd(s.upper_count, 1u);
+                uint32_t ct = atomicAdd(&s_task_next, 2u);
+                s.u_axis[node] = (uint32_t)dim;
+                link_child(s, parent, node);
+                s.task_i0[ct] = i0;
+                s.task_i1[ct] = mid - 1;
+                s.task_parent[ct] = node << 1;
+                s.task_i0[ct + 1] = mid;
+                s.task_i1[ct + 1] = i1;
+                s.task_parent[ct + 1] = (node << 1) | 1u;
+            }
+        }
+        // ---- small ranges: one lane each, to completion
+        for (uint32_t t = lb + tid; t < le; t += SAH_BLOCK) {
+            uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
+            if (i1 - i0 + 1 > LANE_TASK_MAX) continue;
+            uint32_t st0[LANE_TASK_MAX + 2], st1[LANE_TASK_MAX + 2], stp[LANE_TASK_MAX + 2];
+            int sp = 0;
+            st0[0] = i0; st1[0] = i1; stp[0] = s.task_parent[t]; sp = 1;
+            while (sp > 0) {
+                sp--;
+                uint32_t a = st0[sp], b = st1[sp], p = stp[sp];
+                if (a == b) { link_child(s, p, 0x80000000u | s_perm[a]); continue; }
+                int dim;
+                uint32_t mid = lane_split(s, s_perm, s_bucket, a, b, &dim);
+                uint32_t node = atomicAdd(s.upper_count, 1u);
+                s.u_axis[node] = (uint32_t)dim;
+                link_child(s, p, node);
+                st0[sp] = mid; st1[sp] = b; stp[sp] = (node << 1) | 1u; sp++;
+                st0[sp] = a; st1[sp] = mid - 1; stp[sp] = node << 1; sp++;
             }
         }
         __threadfence_block();
@@ -651,58 +760,69 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
         le = s_task_next;
         __syncthreads();
     }
-    num_upper = *s.upper_count;
+    const uint32_t num_upper = *s.upper_count;
+    for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) s.u_arrive[u] = 0;
+    __threadfence_block();
     __syncthreads();
-    const uint32_t dmax = s_depth_max;
-    // Bottom-up: sizes and bounds of upper nodes (children have larger depth).
-    for (int d = (int)dmax; d >= 0; d--) {
-        for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) {
-            if (s.u_depth[u] != (uint32_t)d) continue;
+    // Bottom-up: sizes and bounds; the second child to arrive computes its parent.
+    for (uint32_t t = tid; t < k; t += SAH_BLOCK) {
+        uint32_t ref = s.t_parent[t];
+        while (ref != NONE) {
+            uint32_t u = ref >> 1;
+            __threadfence();
+            if (atomicAdd(&s.u_arrive[u], 1u) == 0) break;
+            __threadfence();
             uint32_t sz = 1;
             bbox b[2];
             for (int c = 0; c < 2; c++) {
-                uint32_t ref = c ? s.u_c1[u] : s.u_c0[u];
-                if (ref & 0x80000000u) {
-                    uint32_t t = ref & 0x7FFFFFFFu;
-                    sz += s.t_nodes[t];
-                    b[c] = load_box(s.t_box + 6 * (size_t)t);
+                uint32_t cr = c ? s.u_c1[u] : s.u_c0[u];
+                if (cr & 0x80000000u) {
+                    uint32_t tt = cr & 0x7FFFFFFFu;
+                    sz += s.t_nodes[tt];
+                    b[c] = load_box(s.t_box + 6 * (size_t)tt);
                 } else {
-                    sz += s.u_size[ref];
-                    b[c] = load_box(s.u_box + 6 * (size_t)ref);
+                    sz += __hip_atomic_load(&s.u_size[cr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    const float* q = s.u_box + 6 * (size_t)cr;
+                    for (int e = 0; e < 3; e++) {
+                        b[c].mn[e] = __hip_atomic_load(q + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        b[c].mx[e] = __hip_atomic_load(q + 3 + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                 }
             }
+            bbox ub = box_union(b[0], b[1]);
             s.u_size[u] = sz;
-            store_box(s.u_box + 6 * (size_t)u, box_union(b[0], b[1]));
+            store_box(s.u_box + 6 * (size_t)u, ub);
+            ref = s.u_parent[u];
         }
-        __threadfence_block();
-        __syncthreads();
-    }
-    // Top-down preorder offsets.
-    uint32_t root = *s.upper_root;
-    if (tid == 0) {
-        if (root & 0x80000000u) s.t_final[root & 0x7FFFFFFFu] = 0;
-        else s.u_off[root] = 0;
     }
     __threadfence_block();
     __syncthreads();
-    for (uint32_t d = 0; d <= dmax && num_upper > 0; d++) {
-        for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) {
-            if (s.u_depth[u] != d) continue;
-            uint32_t off = s.u_off[u];
-            uint32_t r0 = s.u_c0[u], r1 = s.u_c1[u];
-            uint32_t size0 = (r0 & 0x80000000u) ? s.t_nodes[r0 & 0x7FFFFFFFu] : s.u_size[r0];
-            uint32_t o0 = off + 1, o1 = off + 1 + size0;
-            if (r0 & 0x80000000u) s.t_final[r0 & 0x7FFFFFFFu] = o0; else s.u_off[r0] = o0;
-            if (r1 & 0x80000000u) s.t_final[r1 & 0x7FFFFFFFu] = o1; else s.u_off[r1] = o1;
-            // Final node record for the upper node.
-            s.node_offset[off] = o1;
+    // Top-down: preorder offset = sum over the path of 1 + (right child ? size(left) : 0).
+    const uint32_t total_items = k + num_upper;
+    for (uint32_t x = tid; x < total_items; x += SAH_BLOCK) {
+        bool isT = x < k;
+        uint32_t ref = isT ? s.t_parent[x] : s.u_parent[x - k];
+        uint32_t off = 0;
+        while (ref != NONE) {
+            uint32_t u = ref >> 1;
+            off += 1;
+            if (ref & 1u) {
+                uint32_t c0 = s.u_c0[u];
+                off += (c0 & 0x80000000u) ? s.t_nodes[c0 & 0x7FFFFFFFu] : s.u_size[c0];
+            }
+            ref = s.u_parent[u];
+        }
+        if (isT) {
+            s.t_final[x] = off;
+        } else {
+            uint32_t u = x - k;
+            uint32_t c0 = s.u_c0[u];
+            uint32_t size0 = (c0 & 0x80000000u) ? s.t_nodes[c0 & 0x7FFFFFFFu] : s.u_size[c0];
+            s.node_offset[off] = off + 1 + size0;
             s.node_nshapes[off] = 0;
             s.node_axis[off] = (uint8_t)s.u_axis[u];
-            bbox b = load_box(s.u_box + 6 * (size_t)u);
-            store_box(s.node_bounds + 6 * (size_t)off, b);
+            store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
         }
-        __threadfence_block();
-        __syncthreads();
     }
     if (tid == 0) {
         uint32_t total = num_upper;
@@ -740,65 +860,173 @@ __global__ __launch_bounds__(256) void k_scatter(LbvhScratch s) {
 }
 
 // ---------------------------------------------------------------------------
-// Traversal layout.  Record i (64 bytes, one per reference node index):
-//   interior: [0..5] child0 box, [6..11] child1 box, [12] child0 word,
-//             [13] child1 word, [14] axis, [15] 0
-//   leaf:     [12] first primitive slot, [13] primitive count, [15] = 1
-// A child word is the child's node index, with bit 31 set when it is a leaf.
+// Traversal layout (derived from the reference node arrays; the reference arrays
+// themselves are unchanged).  Record r (64 bytes):
+//   [0..5] child0 box, [6..11] child1 box, [12] child0 word, [13] child1 word,
+//   [14] split axis (the near child is child1 when dirIsNeg[axis])
+// A child word is either a record index (bit 31 clear) or a primitive range:
+// 0x80000000 | (count - 1) << 28 | first primitive position (count <= 4).
+// Records 0 .. n_nodes-1 are the reference nodes: interior nodes hold their two
+// children; a reference leaf with more than 4 primitives holds the root of a
+// small binary tree over its primitives (median splits, ranges of <= 4), whose
+// other records are appended after n_nodes.  Primitives are stored in that
+// order; each keeps its slot (position in the reference's shapesEntities) and
+// its leaf's first slot, so ties inside one reference leaf still go to the
+// lowest slot, as the reference's candidate order does (SLBVH.cpp:110-126).
+constexpr uint32_t RANGE_MAX = 4;
+constexpr int SUB_STACK = 40;
+
+WPBR_D uint32_t range_word(uint32_t first, uint32_t count) {
+    return 0x80000000u | ((count - 1) << 28) | first;
+}
+
 __global__ void k_trav_layout(LbvhScratch s) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nn = *s.n_nodes;
-    if (i >= nn) return;
+    if (i >= nn || s.node_nshapes[i] > 0) return;
     float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)i;
-    uint32_t ns = s.node_nshapes[i];
-    if (ns > 0) {
-        rec[3] = make_float4(__uint_as_float(s.node_offset[i]), __uint_as_float(ns), 0.0f,
-                             __uint_as_float(1u));
-        return;
+    uint32_t c[2] = {i + 1, s.node_offset[i]};
+    uint32_t w[2];
+    for (int k = 0; k < 2; k++) {
+        uint32_t ns = s.node_nshapes[c[k]];
+        w[k] = (ns > 0 && ns <= RANGE_MAX) ? range_word(s.node_offset[c[k]], ns) : c[k];
     }
-    uint32_t c0 = i + 1, c1 = s.node_offset[i];
-    const float* b0 = s.node_bounds + 6 * (size_t)c0;
-    const float* b1 = s.node_bounds + 6 * (size_t)c1;
-    uint32_t w0 = c0 | (s.node_nshapes[c0] ? 0x80000000u : 0u);
-    uint32_t w1 = c1 | (s.node_nshapes[c1] ? 0x80000000u : 0u);
+    const float* b0 = s.node_bounds + 6 * (size_t)c[0];
+    const float* b1 = s.node_bounds + 6 * (size_t)c[1];
     rec[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
     rec[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
     rec[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
-    rec[3] = make_float4(__uint_as_float(w0), __uint_as_float(w1),
+    rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
                          __uint_as_float((uint32_t)s.node_axis[i]), 0.0f);
 }
 
-// Leaf-ordered primitive records (48 bytes): v0.xyz + shape id, v1.xyz, v2.xyz.
-__global__ void k_prim_gather(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index,
-                              uint32_t n) {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
-    uint32_t id = sorted_index[j];
-    float4* rec = reinterpret_cast<float4*>(s.prims) + 3 * (size_t)j;
+WPBR_D float cen_of(const LbvhScratch& s, const uint32_t* sorted_index, uint32_t slot, int axis) {
+    return s.centroid[3 * (size_t)sorted_index[slot] + axis];
+}
+
+// Writes the primitive record of position p (holding reference slot `slot`).
+WPBR_D void write_prim(const LbvhScratch& s, const GeomView& g, const uint32_t* sorted_index,
+                       uint32_t p, uint32_t slot, uint32_t leaf_off) {
+    uint32_t id = sorted_index[slot];
+    float4* rec = reinterpret_cast<float4*>(s.prims) + 3 * (size_t)p;
     if (id < g.n_spheres) {
         rec[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(id));
+        rec[1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(slot));
+        rec[2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(leaf_off));
         return;
     }
     const float* t = g.tri_verts + 9 * (size_t)(id - g.n_spheres);
     rec[0] = make_float4(t[0], t[1], t[2], __uint_as_float(id));
-    rec[1] = make_float4(t[3], t[4], t[5], 0.0f);
-    rec[2] = make_float4(t[6], t[7], t[8], 0.0f);
+    rec[1] = make_float4(t[3], t[4], t[5], __uint_as_float(slot));
+    rec[2] = make_float4(t[6], t[7], t[8], __uint_as_float(leaf_off));
+}
+
+// One thread per reference node; leaves build their sub-tree and write their
+// primitives.  sub_perm (one entry per slot) holds the working order.
+__global__ void k_subtree(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nn = *s.n_nodes;
+    if (i >= nn) return;
+    uint32_t n = s.node_nshapes[i];
+    if (n == 0) return;
+    const uint32_t off = s.node_offset[i];
+    uint32_t* perm = s.sub_perm;
+    for (uint32_t k = 0; k < n; k++) perm[off + k] = off + k;
+    if (n > RANGE_MAX) {
+        uint32_t sb[SUB_STACK], se[SUB_STACK], sr[SUB_STACK];
+        int sp = 1;
+        sb[0] = off; se[0] = off + n; sr[0] = i;
+        while (sp > 0) {
+            sp--;
+            uint32_t a = sb[sp], b = se[sp], r = sr[sp];
+            uint32_t m = b - a;
+            float lo[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, hi[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
+            for (uint32_t q = a; q < b; q++) {
+                for (int k = 0; k < 3; k++) {
+                    float c = cen_of(s, sorted_index, perm[q], k);
+                    lo[k] = smin(lo[k], c);
+                    hi[k] = smax(hi[k], c);
+                }
+            }
+            float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+            int axis = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
+            float ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+            uint32_t mid = a + m / 2;
+            if (ext > 0.0f) {
+                // insertion sort by centroid on the split axis (ranges are small:
+                // reference leaves hold fewer than 30 shapes unless all codes are equal)
+                if (m <= 64) {
+                    for (uint32_t x = a + 1; x < b; x++) {
+                        uint32_t v = perm[x];
+                        float cv = cen_of(s, sorted_index, v, axis);
+                        uint32_t y = x;
+                        while (y > a && cen_of(s, sorted_index, perm[y - 1], axis) > cv) {
+                            perm[y] = perm[y - 1];
+                            y--;
+                        }
+                        perm[y] = v;
+                    }
+                } else {
+                    // Lomuto partitions around the middle element (nth_element)
+                    uint32_t L = a, R = b - 1;
+                    while (L < R) {
+                        uint32_t pm = (L + R) / 2;
+                        uint32_t t = perm[pm]; perm[pm] = perm[R]; perm[R] = t;
+                        float pv = cen_of(s, sorted_index, perm[R], axis);
+                        uint32_t st = L;
+                        for (uint32_t x = L; x < R; x++) {
+                            if (cen_of(s, sorted_index, perm[x], axis) < pv) {
+                                uint32_t u = perm[x]; perm[x] = perm[st]; perm[st] = u;
+                                st++;
+                            }
+                        }
+                        t = perm[st]; perm[st] = perm[R]; perm[R] = t;
+                        if (st == mid) break;
+                        if (st < mid) L = st + 1;
+                        else R = st - 1;
+                    }
+                }
+            }
+            uint32_t cb[2] = {a, mid}, ce[2] = {mid, b}, w[2];
+            bbox box[2];
+            for (int c = 0; c < 2; c++) {
+                box[c] = box_empty();
+                for (uint32_t q = cb[c]; q < ce[c]; q++)
+                    box[c] = box_union(box[c], load_box(s.bounds + 6 * (size_t)sorted_index[perm[q]]));
+                uint32_t cm = ce[c] - cb[c];
+                if (cm <= RANGE_MAX) {
+                    w[c] = range_word(cb[c], cm);
+                } else {
+                    uint32_t nr = nn + atomicAdd(s.sub_count, 1u);
+                    w[c] = nr;
+                    sb[sp] = cb[c]; se[sp] = ce[c]; sr[sp] = nr; sp++;
+                }
+            }
+            float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)r;
+            rec[0] = make_float4(box[0].mn[0], box[0].mn[1], box[0].mn[2], box[0].mx[0]);
+            rec[1] = make_float4(box[0].mx[1], box[0].mx[2], box[1].mn[0], box[1].mn[1]);
+            rec[2] = make_float4(box[1].mn[2], box[1].mx[0], box[1].mx[1], box[1].mx[2]);
+            rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
+                                 __uint_as_float((uint32_t)axis), 0.0f);
+        }
+    }
+    for (uint32_t k = 0; k < n; k++) write_prim(s, g, sorted_index, off + k, perm[off + k], off);
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_cb_init(uint32_t* cb_ord) {
-    if (threadIdx.x < 3) {
-        cb_ord[threadIdx.x] = f2ord(FLT_MAX_);
-        cb_ord[3 + threadIdx.x] = f2ord(FLT_LOWEST_);
-    }
+__global__ void k_cb_final(uint32_t* cb_ord) {
+    __shared__ uint32_t s_cb[6];
+    reduce_cb(cb_ord, s_cb, cb_ord);
 }
 
 wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroid, uint32_t* cb_ord,
                                 hipStream_t st) {
     uint32_t n = g.n_spheres + g.n_triangles;
-    hipLaunchKernelGGL(k_cb_init, dim3(1), dim3(64), 0, st, cb_ord);
-    if (n) hipLaunchKernelGGL(k_bounds, dim3((n + 255) / 256), dim3(256), 0, st, g, n, bounds, centroid, cb_ord);
-    return hipGetLastError() == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+    if (!n) return WPBR_OK;
+    uint32_t nb = std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256);
+    hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(256), 0, st, g, n, bounds, centroid, cb_ord);
+    hipLaunchKernelGGL(k_cb_final, dim3(1), dim3(64), 0, st, cb_ord);
+    return launch_status();
 }
 
 // ---------------------------------------------------------------------------
@@ -809,7 +1037,7 @@ wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream
     const uint32_t num_tiles = (n + SORT_TILE - 1) / SORT_TILE;
     rec(0);
     hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, (s.status_words + 255) / 256)), dim3(256), 0, st, s);
-    hipLaunchKernelGGL(k_bounds, dim3((n + 255) / 256), dim3(256), 0, st, g, n, s.bounds,
+    hipLaunchKernelGGL(k_bounds, dim3(std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256)), dim3(256), 0, st, g, n, s.bounds,
                        s.centroid, s.cb_ord);
     rec(1);
     hipLaunchKernelGGL(k_morton, dim3((n + MORTON_BLOCK * MORTON_ITEMS - 1) / (MORTON_BLOCK * MORTON_ITEMS)),
@@ -840,9 +1068,9 @@ wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream
     rec(10);
     uint32_t max_nodes = 2 * n;
     hipLaunchKernelGGL(k_trav_layout, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s);
-    hipLaunchKernelGGL(k_prim_gather, dim3((n + 255) / 256), dim3(256), 0, st, s, g, s.vals_a, n);
+    hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, g, s.vals_a);
     rec(11);
-    return hipGetLastError() == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+    return launch_status();
 }
 
 }  // namespace wpbr

@@ -507,15 +507,15 @@ WPBR_D float mitchell1(float x) {
            (1.f / 6.f);
 }
 
-__global__ __launch_bounds__(256) void k_shade2(RenderLaunch L) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L.n) return;
+// Radiance of pixel i's sample in this pass, as XYZ; black = every wavelength is 0.
+WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& Y, float& Z,
+                            bool& black) {
     const SceneView& sv = L.sv;
-    WaveState& ws = L.ws;
+    const WaveState& ws = L.ws;
     const uint32_t n = L.n;
     uint32_t flags = ws.flags[i];
-    float X = 0, Y = 0, Z = 0;
-    bool black = true;
+    X = 0; Y = 0; Z = 0;
+    black = true;
     if (flags & F_MISS) {
         X = ws.le[i]; Y = ws.le[n + i]; Z = ws.le[2 * (size_t)n + i];
         black = false;  // the splat tests L itself; an exactly black Le adds zeros
@@ -602,33 +602,59 @@ __global__ __launch_bounds__(256) void k_shade2(RenderLaunch L) {
             X *= dl; Y *= dl; Z *= dl;
         }
     }
-    if (L.sample_xyz) {
-        size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
-        L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
-    }
-    // Mitchell splat clipped to the 32x32 tile of the pixel.
+}
+
+// One workgroup per 32x32 film tile: the reference clips the Mitchell splat to
+// the sample's own tile (CSFilm.cpp:224-225), so the tile's accumulation stays in
+// LDS and is added to the film once, without global atomics.
+constexpr int TILE_THREADS = 256;
+__global__ __launch_bounds__(TILE_THREADS) void k_shade2(RenderLaunch L) {
+    __shared__ float acc[1024 * 4];
+    const SceneView& sv = L.sv;
     const uint32_t W = sv.width;
-    uint32_t px = i % W, py = i / W;
-    int tx0 = (int)(px & ~31u), ty0 = (int)(py & ~31u);
-    int tx1 = tx0 + 31, ty1 = ty0 + 31;
-    float fx = ws.pfilm[i], fy = ws.pfilm[n + i];
-    float dx = fx - 0.5f, dy = fy - 0.5f;
-    int x0 = (int)ceilf(dx - 2.0f), x1 = (int)floorf(dx + 2.0f);
-    int y0 = (int)ceilf(dy - 2.0f), y1 = (int)floorf(dy + 2.0f);
-    x0 = max(x0, tx0); y0 = max(y0, ty0);
-    x1 = min(x1, tx1); y1 = min(y1, ty1);
-    for (int y = y0; y <= y1; y++) {
-        float wy = mitchell1(((float)y - fy) / 2.0f);
-        for (int x = x0; x <= x1; x++) {
-            float w = mitchell1(((float)x - fx) / 2.0f) * wy;
-            float* f = L.film + 4 * ((size_t)y * W + x);
-            if (!black) {
-                atomicAdd(f, X * w);
-                atomicAdd(f + 1, Y * w);
-                atomicAdd(f + 2, Z * w);
-            }
-            atomicAdd(f + 3, w);
+    const uint32_t ntx = W / 32;
+    const uint32_t tx0 = (blockIdx.x % ntx) * 32, ty0 = (blockIdx.x / ntx) * 32;
+    for (int k = threadIdx.x; k < 1024 * 4; k += TILE_THREADS) acc[k] = 0.0f;
+    __syncthreads();
+    const uint32_t n = L.n;
+    for (int r = 0; r < 4; r++) {
+        const uint32_t lp = threadIdx.x + r * TILE_THREADS;  // local pixel, row-major in the tile
+        const uint32_t px = tx0 + (lp & 31), py = ty0 + (lp >> 5);
+        const uint32_t i = py * W + px;
+        float X, Y, Z;
+        bool black;
+        sample_radiance(L, i, X, Y, Z, black);
+        if (L.sample_xyz) {
+            size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
+            L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
         }
+        float fx = L.ws.pfilm[i], fy = L.ws.pfilm[n + i];
+        float dx = fx - 0.5f, dy = fy - 0.5f;
+        int x0 = (int)ceilf(dx - 2.0f), x1 = (int)floorf(dx + 2.0f);
+        int y0 = (int)ceilf(dy - 2.0f), y1 = (int)floorf(dy + 2.0f);
+        x0 = max(x0, (int)tx0); y0 = max(y0, (int)ty0);
+        x1 = min(x1, (int)tx0 + 31); y1 = min(y1, (int)ty0 + 31);
+        for (int y = y0; y <= y1; y++) {
+            float wy = mitchell1(((float)y - fy) / 2.0f);
+            for (int x = x0; x <= x1; x++) {
+                float w = mitchell1(((float)x - fx) / 2.0f) * wy;
+                float* f = acc + 4 * ((y - (int)ty0) * 32 + (x - (int)tx0));
+                if (!black) {
+                    atomicAdd(f, X * w);
+                    atomicAdd(f + 1, Y * w);
+                    atomicAdd(f + 2, Z * w);
+                }
+                atomicAdd(f + 3, w);
+            }
+        }
+    }
+    __syncthreads();
+    for (int lp = threadIdx.x; lp < 1024; lp += TILE_THREADS) {
+        const uint32_t px = tx0 + (lp & 31), py = ty0 + (lp >> 5);
+        float4* f = reinterpret_cast<float4*>(L.film) + ((size_t)py * W + px);
+        float4 v = *f;
+        v.x += acc[4 * lp]; v.y += acc[4 * lp + 1]; v.z += acc[4 * lp + 2]; v.w += acc[4 * lp + 3];
+        *f = v;
     }
 }
 
@@ -637,9 +663,10 @@ wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {
     switch (stage) {
         case 0: hipLaunchKernelGGL(k_camera, grid, block, 0, st, L); break;
         case 1: hipLaunchKernelGGL(k_shade1, grid, block, 0, st, L); break;
-        case 2: hipLaunchKernelGGL(k_shade2, grid, block, 0, st, L); break;
+        case 2: hipLaunchKernelGGL(k_shade2, dim3((L.sv.width / 32) * (L.sv.height / 32)),
+                                   dim3(TILE_THREADS), 0, st, L); break;
     }
-    return hipGetLastError() == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+    return launch_status();
 }
 
 }  // namespace wpbr

@@ -1,0 +1,454 @@
+// Host side of the renderer: scene upload, load-time preprocessing and the
+// per-pass launch loop of wpbr_render.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <vector>
+#include "render_types.hpp"
+#include "wpbr_ctx.hpp"
+#include "spectrum_tables.inc"
+
+using namespace wpbr;
+
+namespace wpbr {
+wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
+}
+
+namespace {
+
+constexpr double PI_REF_H = 3.14;  // pch.h:30
+
+// ---- SampledSpectrum::init (CSpectrumSampled.h:87-130, 236-281) ---------------
+float lerp_h(float a, float b, float t) { return (1 - t) * a + t * b; }
+
+float average_samples(const float* lambda, const float* vals, int n, float l0, float l1) {
+    if (l1 <= lambda[0]) return vals[0];
+    if (l0 >= lambda[n - 1]) return vals[n - 1];
+    if (n == 1) return vals[0];
+    float sum = 0;
+    if (l0 < lambda[0]) sum += vals[0] * (lambda[0] - l0);
+    if (l1 > lambda[n - 1]) sum += vals[n - 1] * (l1 - lambda[n - 1]);
+    int j = 0;
+    while (l0 > lambda[j + 1]) j++;
+    for (; j + 1 < n && l1 >= lambda[j]; j++) {
+        float s0 = l0 > lambda[j] ? l0 : lambda[j];
+        float s1 = l1 < lambda[j + 1] ? l1 : lambda[j + 1];
+        float a = lerp_h(vals[j], vals[j + 1], (s0 - lambda[j]) / (lambda[j + 1] - lambda[j]));
+        float b = lerp_h(vals[j], vals[j + 1], (s1 - lambda[j]) / (lambda[j + 1] - lambda[j]));
+        float prod = (a + b) * (s1 - s0);
+        sum = (float)((double)sum + (double)prod * 0.5);
+    }
+    return sum / (l1 - l0);
+}
+
+struct HostTables {
+    float xyz[3 * NSPEC];
+    float refl[7 * NSPEC];
+    float illum[7 * NSPEC];
+};
+
+const HostTables& host_tables() {
+    static HostTables t = [] {
+        HostTables h;
+        const float* refl[7] = {WPBR_RGBRefl2SpectWhite, WPBR_RGBRefl2SpectCyan,
+                                WPBR_RGBRefl2SpectMagenta, WPBR_RGBRefl2SpectYellow,
+                                WPBR_RGBRefl2SpectRed, WPBR_RGBRefl2SpectGreen,
+                                WPBR_RGBRefl2SpectBlue};
+        const float* illum[7] = {WPBR_RGBIllum2SpectWhite, WPBR_RGBIllum2SpectCyan,
+                                 WPBR_RGBIllum2SpectMagenta, WPBR_RGBIllum2SpectYellow,
+                                 WPBR_RGBIllum2SpectRed, WPBR_RGBIllum2SpectGreen,
+                                 WPBR_RGBIllum2SpectBlue};
+        for (int i = 0; i < NSPEC; i++) {
+            float wl0 = lerp_h(400.0f, 700.0f, float(i) / float(NSPEC));
+            float wl1 = lerp_h(400.0f, 700.0f, float(i + 1) / float(NSPEC));
+            h.xyz[i] = average_samples(WPBR_CIE_lambda, WPBR_CIE_X, WPBR_N_CIE, wl0, wl1);
+            h.xyz[NSPEC + i] = average_samples(WPBR_CIE_lambda, WPBR_CIE_Y, WPBR_N_CIE, wl0, wl1);
+            h.xyz[2 * NSPEC + i] = average_samples(WPBR_CIE_lambda, WPBR_CIE_Z, WPBR_N_CIE, wl0, wl1);
+            for (int b = 0; b < 7; b++) {
+                h.refl[b * NSPEC + i] = average_samples(WPBR_RGB2SpectLambda, refl[b], WPBR_N_RGB2SPECT, wl0, wl1);
+                h.illum[b * NSPEC + i] = average_samples(WPBR_RGB2SpectLambda, illum[b], WPBR_N_RGB2SPECT, wl0, wl1);
+            }
+        }
+        return h;
+    }();
+    return t;
+}
+
+// fromRGB(rgb, Reflectance) (CSpectrumSampled.h:147-199).
+void from_rgb_refl(const float* rgb, float* s) {
+    const float* B = host_tables().refl;
+    for (int i = 0; i < NSPEC; i++) s[i] = 0.0f;
+    auto add = [&](int b, float k) { for (int i = 0; i < NSPEC; i++) s[i] += B[b * NSPEC + i] * k; };
+    float r = rgb[0], g = rgb[1], b = rgb[2];
+    if (r <= g && r <= b) {
+        add(0, r);
+        if (g <= b) { add(1, g - r); add(6, b - g); }
+        else { add(1, b - r); add(5, g - b); }
+    } else if (g <= r && g <= b) {
+        add(0, g);
+        if (r <= b) { add(2, r - g); add(6, b - r); }
+        else { add(2, b - g); add(4, r - b); }
+    } else {
+        add(0, b);
+        if (r <= g) { add(3, r - b); add(5, g - r); }
+        else { add(3, g - b); add(4, r - g); }
+    }
+}
+
+// Bilinear mip-0 lookup (CSTextureProcess.h:58-81), indices clamped to the texture.
+void env_bilerp_h(const float* rgbTex, int W, int H, float s_, float t_, float* out) {
+    float s = s_ * (float)W - 0.5f, t = t_ * (float)H - 0.5f;
+    int s0 = (int)std::floor(s), t0 = (int)std::floor(t);
+    s0 = s0 < 0 ? 0 : s0; t0 = t0 < 0 ? 0 : t0;
+    s0 = s0 > W - 1 ? W - 1 : s0; t0 = t0 > H - 1 ? H - 1 : t0;
+    float ds = s - (float)s0, dt = t - (float)t0;
+    int s0n = s0 + 1 > W - 1 ? W - 1 : s0 + 1;
+    int t0n = t0 + 1 > H - 1 ? H - 1 : t0 + 1;
+    s0n = s0n < 0 ? 0 : s0n; t0n = t0n < 0 ? 0 : t0n;
+    auto tex = [&](int x, int y, int c) { return rgbTex[((size_t)y * W + x) * 3 + c]; };
+    for (int c = 0; c < 3; c++) {
+        float v = ((tex(s0, t0, c) * (1 - ds)) * (1 - dt) + (tex(s0, t0n, c) * (1 - ds)) * dt) +
+                  (tex(s0n, t0, c) * ds) * (1 - dt);
+        out[c] = v + (tex(s0n, t0n, c) * ds) * dt;
+    }
+}
+
+// JobLightInfiniteAreaPreprocessSampling (CLightInfiniteArea.cpp:113-139) followed by
+// the pbrt-v3 Distribution2D constructor: returns the per-row functions (the
+// reference's data array, whose index is x + y * H) and the marginal integral.
+void env_distribution(const float* rgbTex, uint32_t W, uint32_t H, std::vector<float>& func,
+                      std::vector<float>& rowInt, float& margInt) {
+    func.assign((size_t)W * H, 0.0f);
+    for (uint32_t y = 0; y < H; y++) {
+        float yN = (float)y / (float)H;
+        float sinTheta = (float)std::sin(PI_REF_H * (double)float(y + 0.5f) / (double)float(H));
+        for (uint32_t x = 0; x < W; x++) {
+            float xN = (float)x / (float)W;
+            float rgb[3];
+            env_bilerp_h(rgbTex, (int)W, (int)H, xN, yN, rgb);
+            func[x + (size_t)y * H] = rgb[1];
+            func[x + (size_t)y * H] *= sinTheta;
+        }
+    }
+    rowInt.assign(H, 0.0f);
+    for (uint32_t v = 0; v < H; v++) {
+        float c = 0.0f;
+        for (uint32_t i = 1; i < W + 1; i++) c = c + func[(size_t)v * W + i - 1] / (int)W;
+        rowInt[v] = c;
+    }
+    float c = 0.0f;
+    for (uint32_t i = 1; i < H + 1; i++) c = c + rowInt[i - 1] / (int)H;
+    margInt = c;
+}
+
+}  // namespace
+
+struct wpbr_scene {
+    wpbr_ctx* ctx = nullptr;
+    std::vector<void*> bufs;
+    SceneView sv{};
+    wpbr_geometry geom{};
+    wpbr_lbvh* bvh = nullptr;
+    uint32_t spp = 1, seed = 0;
+};
+
+struct wpbr_render_state {
+    void* mem = nullptr;
+    uint32_t cap = 0;
+    WaveState ws{};
+    std::vector<hipEvent_t> evs;
+    double stats[8] = {0};
+};
+
+namespace wpbr {
+
+void ctx_release_render(wpbr_ctx* ctx) {
+    if (!ctx->render) return;
+    if (ctx->render->mem) hipFree(ctx->render->mem);
+    for (auto& e : ctx->render->evs) hipEventDestroy(e);
+    delete ctx->render;
+    ctx->render = nullptr;
+}
+
+static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
+    if (!ctx->render) ctx->render = new wpbr_render_state();
+    wpbr_render_state* r = ctx->render;
+    if (n <= r->cap) return WPBR_OK;
+    if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
+    // words per pixel: see the carve below
+    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2;
+    size_t bytes = (words * (size_t)n + 64) * 4 + 64 * 256;
+    wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
+    if (s) return s;
+    r->cap = n;
+    char* p = (char*)r->mem;
+    size_t off = 0;
+    auto take = [&](size_t count) -> void* {
+        off = (off + 255) & ~size_t(255);
+        void* q = p + off;
+        off += count * 4;
+        return q;
+    };
+    WaveState& w = r->ws;
+    w.c1 = (uint32_t*)take(n);
+    w.c2 = (uint32_t*)take(n);
+    w.pfilm = (float*)take(2 * (size_t)n);
+    w.rays = (float*)take(6 * (size_t)n);
+    w.hit = (uint32_t*)take(n);
+    w.hit_t = (float*)take(n);
+    w.bary = (float*)take(3 * (size_t)n);
+    w.flags = (uint32_t*)take(n);
+    w.light = (uint32_t*)take(n);
+    w.mat = (uint32_t*)take(n);
+    w.l_cos = (float*)take(n);
+    w.l_sc = (float*)take(n);
+    w.l_ac = (float*)take(n);
+    w.l_pdf = (float*)take(n);
+    w.l_bpdf = (float*)take(n);
+    w.l_zero = (uint32_t*)take(n);
+    w.b_cos = (float*)take(n);
+    w.b_sc = (float*)take(n);
+    w.b_ac = (float*)take(n);
+    w.b_pdf = (float*)take(n);
+    w.b_zero = (uint32_t*)take(n);
+    w.b_wi = (float*)take(3 * (size_t)n);
+    w.le = (float*)take(3 * (size_t)n);
+    w.qrays = (float*)take(12 * (size_t)n);
+    w.qhit = (uint32_t*)take(2 * (size_t)n);
+    w.qt = (float*)take(2 * (size_t)n);
+    w.qcount = (uint32_t*)take(64);
+    w.l_ray = (uint32_t*)take(n);
+    w.b_ray = (uint32_t*)take(n);
+    if (off > bytes) return set_err(ctx, WPBR_ERR_HIP, "render state carve overflow");
+    return WPBR_OK;
+}
+
+}  // namespace wpbr
+
+extern "C" {
+
+wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** out) {
+    if (!ctx || !d || !out) return WPBR_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (d->width == 0 || d->height == 0 || d->spp == 0 || (d->width % 32) || (d->height % 32))
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "film must be a non-empty multiple of 32x32 tiles");
+    uint32_t nLights = d->n_sphere_lights + (d->has_env ? 1u : 0u);
+    if (nLights == 0) return set_err(ctx, WPBR_ERR_INVALID_ARG, "scene has no light");
+    uint64_t n = (uint64_t)d->n_spheres + d->n_triangles;
+    if (n == 0) return set_err(ctx, WPBR_ERR_INVALID_ARG, "scene has no shapes");
+    for (uint32_t l = 0; l < d->n_sphere_lights; l++)
+        if (d->light_shape[l] >= d->n_spheres)
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "light shape is not a sphere");
+    for (uint64_t s = 0; s < n; s++)
+        if (d->shape_material[s] != 0xFFFFFFFFu && d->shape_material[s] >= d->n_materials)
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "material index out of range");
+    hipSetDevice(ctx->device);
+    wpbr_scene* sc = new wpbr_scene();
+    sc->ctx = ctx;
+    sc->spp = d->spp;
+    sc->seed = d->seed;
+    wpbr_status st = WPBR_OK;
+    auto up = [&](const void* src, size_t bytes) -> void* {
+        if (st) return nullptr;
+        void* p = nullptr;
+        st = hip_check(ctx, hipMalloc(&p, bytes ? bytes : 4), "hipMalloc scene");
+        if (st) return nullptr;
+        sc->bufs.push_back(p);
+        if (bytes && src) st = hip_check(ctx, hipMemcpy(p, src, bytes, hipMemcpyHostToDevice), "scene upload");
+        return p;
+    };
+    const uint32_t ns = d->n_spheres, nt = d->n_triangles;
+    SceneView& v = sc->sv;
+    v.g.n_spheres = ns;
+    v.g.sphere_xform = (const float*)up(d->sphere_xform, 96 * (size_t)ns);
+    v.g.sphere_radius = (const float*)up(d->sphere_radius, 4 * (size_t)ns);
+    v.g.n_triangles = nt;
+    v.g.tri_verts = (const float*)up(d->tri_verts, 36 * (size_t)nt);
+    std::vector<uint8_t> flags(nt, 0);
+    if (d->tri_flags) std::memcpy(flags.data(), d->tri_flags, nt);
+    v.tri_flags = (const uint8_t*)up(flags.data(), nt);
+    v.tri_uv = (const float*)up(d->tri_uv, d->tri_uv ? 24 * (size_t)nt : 0);
+    v.tri_normal = (const float*)up(d->tri_normal, d->tri_normal ? 36 * (size_t)nt : 0);
+    v.tri_tangent = (const float*)up(d->tri_tangent, d->tri_tangent ? 36 * (size_t)nt : 0);
+    v.shape_material = (const uint32_t*)up(d->shape_material, 4 * (size_t)n);
+    // materials: conductor spectra via fromRGB(Reflectance)
+    std::vector<DevMaterial> mats(d->n_materials);
+    std::vector<float> specs;
+    for (uint32_t m = 0; m < d->n_materials; m++) {
+        const float* src = d->materials + 9 * (size_t)m;
+        std::memcpy(&mats[m].type, &src[0], 4);
+        mats[m].alpha = src[1];
+        mats[m].eta = src[2];
+        mats[m].spec = 0;
+        if (mats[m].type == MAT_CONDUCTOR) {
+            mats[m].spec = (uint32_t)(specs.size() / (2 * NSPEC));
+            float e[NSPEC], k[NSPEC];
+            from_rgb_refl(src + 3, e);
+            from_rgb_refl(src + 6, k);
+            specs.insert(specs.end(), e, e + NSPEC);
+            specs.insert(specs.end(), k, k + NSPEC);
+        }
+    }
+    v.materials = (const DevMaterial*)up(mats.data(), sizeof(DevMaterial) * mats.size());
+    v.mat_spec = (const float*)up(specs.data(), 4 * specs.size());
+    v.n_sphere_lights = d->n_sphere_lights;
+    v.light_shape = (const uint32_t*)up(d->light_shape, 4 * (size_t)d->n_sphere_lights);
+    v.light_emit = (const float*)up(d->light_emit, 4 * NSPEC * (size_t)d->n_sphere_lights);
+    v.has_env = d->has_env;
+    v.env_w = d->has_env ? d->env_w : 1;
+    v.env_h = d->has_env ? d->env_h : 1;
+    if (d->has_env) {
+        v.env_emit = (const float*)up(d->env_emit, 4 * NSPEC);
+        v.env_rgb = (const float*)up(d->env_rgb, 12 * (size_t)d->env_w * d->env_h);
+        v.env_xform = (const float*)up(d->env_xform, 96);
+        std::vector<float> func, rowInt;
+        float margInt = 0;
+        env_distribution(d->env_rgb, d->env_w, d->env_h, func, rowInt, margInt);
+        v.env_cond_func = (const float*)up(func.data(), 4 * func.size());
+        v.env_cond_int = (const float*)up(rowInt.data(), 4 * rowInt.size());
+        v.env_marg_int = (const float*)up(&margInt, 4);
+    }
+    const HostTables& T = host_tables();
+    v.xyz = (const float*)up(T.xyz, sizeof(T.xyz));
+    v.illum = (const float*)up(T.illum, sizeof(T.illum));
+    v.raster_camera = (const float*)up(d->raster_camera, 64);
+    v.camera_world = (const float*)up(d->camera_world, 48);
+    v.width = d->width;
+    v.height = d->height;
+    if (st) { wpbr_scene_destroy(sc); return st; }
+    sc->geom.n_spheres = ns;
+    sc->geom.d_sphere_xform = v.g.sphere_xform;
+    sc->geom.d_sphere_radius = v.g.sphere_radius;
+    sc->geom.n_triangles = nt;
+    sc->geom.d_tri_verts = v.g.tri_verts;
+    st = wpbr_lbvh_create(ctx, &sc->bvh);
+    if (!st) st = wpbr_lbvh_build(ctx, &sc->geom, sc->bvh);
+    if (st) { wpbr_scene_destroy(sc); return st; }
+    *out = sc;
+    return WPBR_OK;
+}
+
+void wpbr_scene_destroy(wpbr_scene* sc) {
+    if (!sc) return;
+    hipSetDevice(sc->ctx->device);
+    hipStreamSynchronize(sc->ctx->stream);
+    if (sc->bvh) wpbr_lbvh_destroy(sc->bvh);
+    for (void* p : sc->bufs) hipFree(p);
+    delete sc;
+}
+
+wpbr_status wpbr_scene_rebuild(wpbr_ctx* ctx, wpbr_scene* sc) {
+    if (!ctx || !sc) return WPBR_ERR_INVALID_ARG;
+    return wpbr_lbvh_build(ctx, &sc->geom, sc->bvh);
+}
+
+wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_sample_xyz) {
+    if (!ctx || !sc || !d_film) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();  // launch checks below must not see stale errors
+    const uint32_t n = sc->sv.width * sc->sv.height;
+    wpbr_status s = ensure_wave(ctx, n);
+    if (!s) s = ctx_ensure_trace(ctx, 2 * n);
+    if (s) return s;
+    wpbr_render_state* r = ctx->render;
+    hipStream_t st = ctx->stream;
+    s = hip_check(ctx, hipMemsetAsync(d_film, 0, (size_t)n * 16, st), "film clear");
+    if (s) return s;
+    RenderLaunch L;
+    L.sv = sc->sv;
+    L.ws = r->ws;
+    L.n = n;
+    L.spp = sc->spp;
+    L.seed = sc->seed;
+    L.film = d_film;
+    L.sample_xyz = d_sample_xyz;
+    const wpbr_lbvh* b = sc->bvh;
+    TraceArgs ta;
+    ta.g = sc->sv.g;
+    ta.trav = (const float4*)b->s.trav;
+    ta.prims = (const float4*)b->s.prims;
+    ta.root_box = b->s.node_bounds;
+    ta.root_word = b->root_word;
+    ta.spill = ctx->spill;
+    ta.spill_depth = ctx->spill_depth;
+    ta.err = ctx->err_flag;
+    ta.n_dev = nullptr;
+    ta.stride = 0;
+    // events: start, then per pass 5 boundaries (camera | trace | shade1 | trace | shade2)
+    const size_t nev = 2 + 5 * (size_t)sc->spp;
+    while (r->evs.size() < nev) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return set_err(ctx, WPBR_ERR_HIP, "hipEventCreate");
+        r->evs.push_back(e);
+    }
+    hipEventRecord(r->evs[0], st);
+    for (uint32_t j = 0; j < sc->spp && !s; j++) {
+        hipEvent_t* e = &r->evs[1 + 5 * (size_t)j];
+        L.spp_index = j;
+        s = render_pass_launch(L, 0, st);
+        hipEventRecord(e[0], st);
+        ta.n = n;
+        ta.n_dev = nullptr;
+        ta.stride = 0;
+        ta.rays = r->ws.rays;
+        ta.hit_shape = r->ws.hit;
+        ta.hit_t = r->ws.hit_t;
+        ta.bary = r->ws.bary;
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st);
+        hipEventRecord(e[1], st);
+        if (!s) s = render_pass_launch(L, 1, st);
+        hipEventRecord(e[2], st);
+        // secondary rays: the queue holds *qcount rays with SoA stride 2n
+        ta.n = 2 * n;
+        ta.n_dev = r->ws.qcount;
+        ta.stride = 2 * n;
+        ta.rays = r->ws.qrays;
+        ta.hit_shape = r->ws.qhit;
+        ta.hit_t = r->ws.qt;
+        ta.bary = nullptr;
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st);
+        hipEventRecord(e[3], st);
+        if (!s) s = render_pass_launch(L, 2, st);
+        hipEventRecord(e[4], st);
+    }
+    if (s) {
+        return set_err(ctx, s, std::string("render launch failed: ") +
+                                   hipGetErrorString(last_launch_error));
+    }
+    s = ctx_finish_trace(ctx);
+    if (s) return s;
+    double trace_ms = 0, shade_ms = 0, cam_ms = 0;
+    for (uint32_t j = 0; j < sc->spp; j++) {
+        hipEvent_t* e = &r->evs[1 + 5 * (size_t)j];
+        hipEvent_t prev = j ? r->evs[5 * (size_t)j] : r->evs[0];
+        float t0, t1, t2, t3, t4;
+        hipEventElapsedTime(&t0, prev, e[0]);
+        hipEventElapsedTime(&t1, e[0], e[1]);
+        hipEventElapsedTime(&t2, e[1], e[2]);
+        hipEventElapsedTime(&t3, e[2], e[3]);
+        hipEventElapsedTime(&t4, e[3], e[4]);
+        cam_ms += t0;
+        trace_ms += t1 + t3;
+        shade_ms += t2 + t4;
+    }
+    float tot = 0;
+    hipEventElapsedTime(&tot, r->evs[0], r->evs[5 * (size_t)sc->spp]);
+    ctx->last_ms = tot;
+    r->stats[0] = tot;
+    r->stats[1] = trace_ms;
+    r->stats[2] = 2.0 * sc->spp;
+    r->stats[3] = 0;
+    r->stats[4] = shade_ms;
+    r->stats[5] = cam_ms;
+    r->stats[6] = 0;
+    return WPBR_OK;
+}
+
+int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap) {
+    if (!ctx || !out || !ctx->render) return 0;
+    int k = cap < 7 ? cap : 7;
+    for (int i = 0; i < k; i++) out[i] = ctx->render->stats[i];
+    return k;
+}
+
+}  // extern "C"

@@ -30,6 +30,7 @@ struct wpbr_lbvh {
     uint32_t cap = 0;
     uint32_t n = 0, n_nodes = 0, n_treelets = 0;
     bool root_is_leaf = false;
+    uint32_t root_word = 0;
     bool built = false;
     wpbr::LbvhScratch s{};
     wpbr::GeomView geom{};

@@ -9,6 +9,15 @@
 
 namespace wpbr {
 
+// Status of the launches since the last call: the HIP error (if any) is kept in
+// last_launch_error so the C ABI can report it.
+inline thread_local hipError_t last_launch_error = hipSuccess;
+inline wpbr_status launch_status() {
+    hipError_t e = hipGetLastError();
+    last_launch_error = e;
+    return e == hipSuccess ? WPBR_OK : WPBR_ERR_HIP;
+}
+
 struct GeomView {
     uint32_t n_spheres;
     const float* sphere_xform;
@@ -45,13 +54,16 @@ struct LbvhScratch {
     uint32_t* task_i0; uint32_t* task_i1; uint32_t* task_parent; uint32_t* task_depth;
     uint32_t* u_c0; uint32_t* u_c1; uint32_t* u_axis; uint32_t* u_depth; uint32_t* u_size;
     uint32_t* u_off; float* u_box;
+    uint32_t* u_parent; uint32_t* u_arrive; uint32_t* t_parent; float* t_cen;
     uint32_t* upper_count; uint32_t* upper_root;
     uint32_t* n_nodes; uint32_t* n_treelets;
     // final CSceneTreeLBVH arrays (capacity 2n)
     uint32_t* node_offset; uint32_t* node_nshapes; uint8_t* node_axis; float* node_bounds;
     // traversal layout
     void* trav;           // 64 B per node
-    void* prims;          // 48 B per shape, leaf order
+    void* prims;          // 48 B per shape, traversal order
+    uint32_t* sub_perm;   // per slot: working order of the leaf sub-trees
+    uint32_t* sub_count;  // sub-tree records appended after the reference nodes
 };
 
 wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st,
@@ -62,8 +74,10 @@ struct TraceArgs {
     const float4* trav;
     const float4* prims;
     const float* root_box;
-    uint32_t root_is_leaf;
-    uint32_t n;
+    uint32_t root_word;   // first traversal word: record 0 or a primitive range
+    uint32_t n;           // ray count (upper bound when n_dev is set)
+    const uint32_t* n_dev;  // optional device-side ray count (queue launches)
+    uint32_t stride;      // SoA stride of rays (0: n)
     const float* rays;
     uint32_t* hit_shape;
     float* hit_t;

@@ -20,7 +20,8 @@ EXPORTS = [
     "wpbr_free", "wpbr_upload", "wpbr_download", "wpbr_sync", "wpbr_last_time_ms",
     "wpbr_shape_bounds", "wpbr_lbvh_create", "wpbr_lbvh_destroy", "wpbr_lbvh_build",
     "wpbr_lbvh_counts", "wpbr_lbvh_download", "wpbr_lbvh_kernel_times", "wpbr_collide",
-    "wpbr_collide_async", "wpbr_set_trace_config",
+    "wpbr_collide_async", "wpbr_set_trace_config", "wpbr_scene_create", "wpbr_scene_destroy",
+    "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats",
 ]
 
 
@@ -35,6 +36,22 @@ class Geometry(C.Structure):
     _fields_ = [("n_spheres", C.c_uint32), ("d_sphere_xform", C.c_void_p),
                 ("d_sphere_radius", C.c_void_p), ("n_triangles", C.c_uint32),
                 ("d_tri_verts", C.c_void_p)]
+
+
+class SceneDesc(C.Structure):
+    """wpbr_scene_desc (include/wpbr.h): host pointers."""
+    _fields_ = [("n_spheres", C.c_uint32), ("sphere_xform", C.c_void_p),
+                ("sphere_radius", C.c_void_p), ("n_triangles", C.c_uint32),
+                ("tri_verts", C.c_void_p), ("tri_flags", C.c_void_p), ("tri_uv", C.c_void_p),
+                ("tri_normal", C.c_void_p), ("tri_tangent", C.c_void_p),
+                ("shape_material", C.c_void_p), ("n_materials", C.c_uint32),
+                ("materials", C.c_void_p), ("n_sphere_lights", C.c_uint32),
+                ("light_shape", C.c_void_p), ("light_emit", C.c_void_p),
+                ("has_env", C.c_uint32), ("env_emit", C.c_void_p), ("env_rgb", C.c_void_p),
+                ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env_xform", C.c_void_p),
+                ("raster_camera", C.c_void_p), ("camera_world", C.c_void_p),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("seed", C.c_uint32)]
 
 
 _lib = None
@@ -70,6 +87,11 @@ def load():
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_set_trace_config": (i32, [vp, C.c_int, C.c_int]),
+        "wpbr_scene_create": (i32, [vp, C.POINTER(SceneDesc), C.POINTER(vp)]),
+        "wpbr_scene_destroy": (None, [vp]),
+        "wpbr_scene_rebuild": (i32, [vp, vp]),
+        "wpbr_render": (i32, [vp, vp, vp, vp]),
+        "wpbr_render_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -233,3 +255,63 @@ def collide(ctx, scene, bvh, d_rays, n, want_bary=False, wait=True):
     ctx.check(fn(ctx.h, C.byref(scene.geom), bvh.h, n, d_rays.ptr, d_shape.ptr, d_t.ptr,
                  d_b.ptr if d_b is not None else None))
     return d_shape, d_t, d_b
+
+
+def scene_desc(s):
+    """wpbr_scene_desc of a woodenpbrengine_amd.scenes.RenderScene; returns
+    (desc, keepalive arrays)."""
+    keep = [np.ascontiguousarray(a) for a in (
+        s.sphere_xform, s.sphere_radius, s.tri_verts, s.tri_flags, s.tri_uv, s.tri_normal,
+        s.tri_tangent, s.shape_material, s.material_table, s.light_shape, s.light_emit,
+        s.env_emit, s.env_rgb, s.env_xform, s.raster_camera, s.camera_world)]
+    (sx, sr, tv, tf, tuv, tn, tt, sm, mt, ls, le, ee, er, ex, rc, cw) = keep
+    d = SceneDesc(len(sr), _ptr(sx), _ptr(sr), len(tv), _ptr(tv), _ptr(tf), _ptr(tuv), _ptr(tn),
+                  _ptr(tt), _ptr(sm), len(mt), _ptr(mt), len(ls), _ptr(ls), _ptr(le),
+                  1 if s.env is not None else 0, _ptr(ee), _ptr(er), er.shape[1], er.shape[0],
+                  _ptr(ex), _ptr(rc), _ptr(cw), s.width, s.height, s.spp, s.seed)
+    return d, keep
+
+
+class Renderer:
+    """A device-resident scene (wpbr_scene) and its film."""
+
+    STATS = ["total_ms", "trace_ms", "trace_launches", "rays", "shade_ms", "camera_ms",
+             "secondary_rays"]
+
+    def __init__(self, ctx, scene):
+        self.ctx = ctx
+        self.scene = scene
+        d, keep = scene_desc(scene)
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wpbr_scene_create(ctx.h, C.byref(d), C.byref(h)))
+        del keep
+        self.h = h
+        self.W, self.H, self.spp = scene.width, scene.height, scene.spp
+        self.d_film = ctx.empty((self.H, self.W, 4), np.float32)
+        self.d_samples = None
+
+    def render(self, keep_samples=False):
+        if keep_samples and self.d_samples is None:
+            self.d_samples = self.ctx.empty((self.H * self.W * self.spp, 3), np.float32)
+        self.ctx.check(self.ctx.lib.wpbr_render(
+            self.ctx.h, self.h, self.d_film.ptr,
+            self.d_samples.ptr if keep_samples else None))
+        return self.ctx.last_ms()
+
+    def stats(self):
+        buf = (C.c_double * 8)()
+        k = self.ctx.lib.wpbr_render_stats(self.ctx.h, buf, 8)
+        return {self.STATS[i]: float(buf[i]) for i in range(k)}
+
+    def film_rgb(self):
+        """toRGB(sum XYZ w) / sum w per pixel (H x W x 3)."""
+        f = self.d_film.numpy().astype(np.float64)
+        xyz, w = f[..., :3], f[..., 3:4]
+        M = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556],
+                      [0.041556, -0.204043, 1.057311]])
+        return (xyz @ M.T) / w
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.wpbr_scene_destroy(self.h)
+            self.h = None
