@@ -769,9 +769,11 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
         uint32_t ref = s.t_parent[t];
         while (ref != NONE) {
             uint32_t u = ref >> 1;
-            __threadfence();
-            if (atomicAdd(&s.u_arrive[u], 1u) == 0) break;
-            __threadfence();
+            // acquire-release at workgroup scope: every thread of the build is in
+            // this workgroup, so no device-wide (L2 write-back) fence is needed
+            if (__hip_atomic_fetch_add(&s.u_arrive[u], 1u, __ATOMIC_ACQ_REL,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
+                break;
             uint32_t sz = 1;
             bbox b[2];
             for (int c = 0; c < 2; c++) {
@@ -921,6 +923,71 @@ WPBR_D void write_prim(const LbvhScratch& s, const GeomView& g, const uint32_t* 
     rec[2] = make_float4(t[6], t[7], t[8], __uint_as_float(leaf_off));
 }
 
+// Sub-tree of a leaf with at most SMALL_LEAF shapes, built from a private copy
+// of its centroids and boxes (same splits as the general path below).
+constexpr uint32_t SMALL_LEAF = 32;
+
+WPBR_D void subtree_small(LbvhScratch& s, const GeomView& g, const uint32_t* sorted_index,
+                          uint32_t i, uint32_t off, uint32_t n, uint32_t nn) {
+    float cen[SMALL_LEAF][3];
+    uint8_t perm[SMALL_LEAF];
+    for (uint32_t k = 0; k < n; k++) {
+        uint32_t id = sorted_index[off + k];
+        for (int c = 0; c < 3; c++) cen[k][c] = s.centroid[3 * (size_t)id + c];
+        perm[k] = (uint8_t)k;
+    }
+    uint8_t sb[SUB_STACK], se[SUB_STACK];
+    uint32_t sr[SUB_STACK];
+    int sp = 1;
+    sb[0] = 0; se[0] = (uint8_t)n; sr[0] = i;
+    while (sp > 0) {
+        sp--;
+        uint32_t a = sb[sp], b = se[sp], r = sr[sp];
+        uint32_t m = b - a;
+        float lo[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, hi[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
+        for (uint32_t q = a; q < b; q++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = smin(lo[k], cen[perm[q]][k]);
+                hi[k] = smax(hi[k], cen[perm[q]][k]);
+            }
+        float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+        int axis = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
+        float ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
+        uint32_t mid = a + m / 2;
+        if (ext > 0.0f) {
+            for (uint32_t x = a + 1; x < b; x++) {
+                uint8_t v = perm[x];
+                float cv = cen[v][axis];
+                uint32_t y = x;
+                while (y > a && cen[perm[y - 1]][axis] > cv) { perm[y] = perm[y - 1]; y--; }
+                perm[y] = v;
+            }
+        }
+        uint32_t cb[2] = {a, mid}, ce[2] = {mid, b}, w[2];
+        bbox box[2];
+        for (int c = 0; c < 2; c++) {
+            box[c] = box_empty();
+            for (uint32_t q = cb[c]; q < ce[c]; q++)
+                box[c] = box_union(box[c], load_box(s.bounds + 6 * (size_t)sorted_index[off + perm[q]]));
+            uint32_t cm = ce[c] - cb[c];
+            if (cm <= RANGE_MAX) {
+                w[c] = range_word(off + cb[c], cm);
+            } else {
+                uint32_t nr = nn + atomicAdd(s.sub_count, 1u);
+                w[c] = nr;
+                sb[sp] = (uint8_t)cb[c]; se[sp] = (uint8_t)ce[c]; sr[sp] = nr; sp++;
+            }
+        }
+        float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)r;
+        rec[0] = make_float4(box[0].mn[0], box[0].mn[1], box[0].mn[2], box[0].mx[0]);
+        rec[1] = make_float4(box[0].mx[1], box[0].mx[2], box[1].mn[0], box[1].mn[1]);
+        rec[2] = make_float4(box[1].mn[2], box[1].mx[0], box[1].mx[1], box[1].mx[2]);
+        rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
+                             __uint_as_float((uint32_t)axis), 0.0f);
+    }
+    for (uint32_t k = 0; k < n; k++) write_prim(s, g, sorted_index, off + k, off + perm[k], off);
+}
+
 // One thread per reference node; leaves build their sub-tree and write their
 // primitives.  sub_perm (one entry per slot) holds the working order.
 __global__ void k_subtree(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index) {
@@ -930,6 +997,10 @@ __global__ void k_subtree(LbvhScratch s, GeomView g, const uint32_t* __restrict_
     uint32_t n = s.node_nshapes[i];
     if (n == 0) return;
     const uint32_t off = s.node_offset[i];
+    if (n > RANGE_MAX && n <= SMALL_LEAF) {
+        subtree_small(s, g, sorted_index, i, off, n, nn);
+        return;
+    }
     uint32_t* perm = s.sub_perm;
     for (uint32_t k = 0; k < n; k++) perm[off + k] = off + k;
     if (n > RANGE_MAX) {
