@@ -99,8 +99,10 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.node_bounds = c.take<float>(12 * N);
     s.trav = c.take<float4>(4 * 3 * N);
     s.prims = c.take<float4>(3 * N);
-    s.sub_perm = c.take<uint32_t>(N);
+    s.slot_leaf = c.take<uint32_t>(N);
     s.sub_count = c.take<uint32_t>(1);
+    s.big_list = c.take<uint32_t>(N / 64 + 1);
+    s.big_count = c.take<uint32_t>(1);
     return c.off + 256;
 }
 
@@ -280,7 +282,8 @@ wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh*
     hipEventElapsedTime(&ms, b->ev[0], b->ev[wpbr_lbvh::NEV - 1]);
     ctx->last_ms = ms;
     b->built = err == 0;
-    if (err) return set_err(ctx, WPBR_ERR_TIMEOUT, "radix sort look-back timed out");
+    if (err & 1u) return set_err(ctx, WPBR_ERR_TIMEOUT, "radix sort look-back timed out");
+    if (err) return set_err(ctx, WPBR_ERR_HIP, "upper SAH build: internal stack exhausted");
     return WPBR_OK;
 }
 

@@ -20,8 +20,10 @@
 //                   libstdc++ std::partition order, and lays it out in preorder
 //   k_scatter       a6: flatten (SLBVH.cpp:432-455): treelet nodes to their final
 //                   preorder slots
-//   k_trav_layout   derived traversal layout: per interior node both child boxes
-//                   in one 64-byte record; leaf triangles gathered in leaf order
+//   k_subtree(_big) derived traversal layout: per interior node both child boxes
+//                   in one 64-byte record; leaves of more than 4 shapes get a
+//                   sub-tree over their (Morton-ordered) slots
+//   k_prims         primitive records at their slots
 #include "wpbr_device.hpp"
 #include "wpbr_internal.hpp"
 
@@ -45,7 +47,7 @@ __global__ void k_init(LbvhScratch s) {
     for (uint32_t j = i; j < (uint32_t)s.status_words; j += stride) s.status0[j] = 0;
     for (uint32_t j = i; j < 128; j += stride) s.presence[j] = 0;
     if (i < 8) s.tile_counter[i] = 0;
-    if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; }
+    if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; *s.big_count = 0; }
 }
 
 // ---------------------------------------------------------------------------
@@ -492,6 +494,7 @@ constexpr int SAH_BLOCK = 1024;
 constexpr int SAH_WAVES = SAH_BLOCK / 64;
 constexpr int NB = 12;
 constexpr uint32_t LANE_TASK_MAX = 32;
+constexpr int LANE_STACK = 6;  // smaller child first: depth <= log2(LANE_TASK_MAX) + 1
 constexpr uint32_t NONE = 0xFFFFFFFFu;
 
 WPBR_D float wave_min_f(float v) {
@@ -515,7 +518,11 @@ WPBR_D void link_child(LbvhScratch& s, uint32_t parent, uint32_t ref) {
 }
 
 // SAH split of perm[i0..i1] by one lane (the oracle's loop, SLBVH.cpp:465-553).
-WPBR_D uint32_t lane_split(LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint32_t i0, uint32_t i1,
+// Bucket statistics stay in registers (statically indexed).  The split costs use
+// one prefix and one suffix sweep of unions: min/max are exact, so these boxes
+// equal the reference's per-split unions up to the sign of a zero coordinate,
+// which no extent (and hence no area or cost) can see.
+WPBR_D uint32_t lane_split(const LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint32_t i0, uint32_t i1,
                            int* dimOut) {
     bbox bounds = box_empty(), cb = box_empty();
     for (uint32_t i = i0; i <= i1; i++) {
@@ -528,6 +535,7 @@ WPBR_D uint32_t lane_split(LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint
     float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cmin;
     bbox bb[NB];
     uint32_t bc[NB];
+#pragma unroll
     for (int b = 0; b < NB; b++) { bb[b] = box_empty(); bc[b] = 0; }
     for (uint32_t i = i0; i <= i1; i++) {
         uint32_t t = perm[i];
@@ -538,20 +546,40 @@ WPBR_D uint32_t lane_split(LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint
             if (ib == NB) ib = NB - 1;
         }
         bucket[i] = (uint8_t)ib;
-        bc[ib]++;
-        bb[ib] = box_union(bb[ib], load_box(s.t_box + 6 * (size_t)t));
+        bbox tb = load_box(s.t_box + 6 * (size_t)t);
+#pragma unroll
+        for (int b = 0; b < NB; b++) {
+            if (ib == (uint32_t)b) {
+                bc[b]++;
+                bb[b] = box_union(bb[b], tb);
+            }
+        }
     }
     float area = box_area(bounds);
+    float la[NB - 1], ra[NB - 1];
+    {
+        bbox acc = box_empty();
+        uint32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < NB - 1; i++) {
+            acc = box_union(acc, bb[i]);
+            c += bc[i];
+            la[i] = (c == 0) ? 0.0f : (float)c * box_area(acc);
+        }
+        acc = box_empty();
+        c = 0;
+#pragma unroll
+        for (int i = NB - 1; i >= 1; i--) {
+            acc = box_union(acc, bb[i]);
+            c += bc[i];
+            ra[i - 1] = (c == 0) ? 0.0f : (float)c * box_area(acc);
+        }
+    }
     float cbest = 0.0f;
     int best = 0;
+#pragma unroll
     for (int i = 0; i < NB - 1; i++) {
-        bbox b0 = box_empty(), b1 = box_empty();
-        uint32_t c0 = 0, c1 = 0;
-        for (int j = 0; j <= i; j++) { b0 = box_union(b0, bb[j]); c0 += bc[j]; }
-        for (int j = i + 1; j < NB; j++) { b1 = box_union(b1, bb[j]); c1 += bc[j]; }
-        float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
-        float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
-        float cost = (float)(0.125 + (double)((l + r) / area));
+        float cost = (float)(0.125 + (double)((la[i] + ra[i]) / area));
         if (i == 0) cbest = cost;
         else if (cost < cbest) { best = i; cbest = cost; }
     }
@@ -576,23 +604,26 @@ WPBR_D uint32_t lane_split(LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint
 __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t n_total_shapes) {
     __shared__ uint16_t s_perm[4096];
     __shared__ uint8_t s_bucket[4096];
+    __shared__ uint16_t s_list_a[4096];
+    __shared__ uint16_t s_list_b[4096];
+    __shared__ uint32_t s_lstack[LANE_STACK][SAH_BLOCK];
     __shared__ uint32_t s_k;
     __shared__ uint32_t s_task_next;
+    __shared__ uint32_t s_upper;
+    __shared__ uint32_t s_total;
     __shared__ uint32_t s_bcount[SAH_WAVES][NB];
     __shared__ uint32_t s_bmin[SAH_WAVES][NB][3];
     __shared__ uint32_t s_bmax[SAH_WAVES][NB][3];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    if (tid == 0) {
-        uint32_t k = 0;
-        for (int w = 0; w < 128; w++) k += __popc(s.presence[w]);
-        s_k = k;
-        s.task_i0[0] = 0;
-        s.task_i1[0] = k - 1;
-        s.task_parent[0] = NONE;
-        *s.upper_count = 0;
-        s_task_next = 1;
-    }
+    if (tid == 0) { s_k = 0; s_upper = 0; s_total = 0; s_task_next = 1; }
     __syncthreads();
+    if (tid < 128) atomicAdd(&s_k, (uint32_t)__popc(s.presence[tid]));
+    __syncthreads();
+    if (tid == 0) {
+        s.task_i0[0] = 0;
+        s.task_i1[0] = s_k - 1;
+        s.task_parent[0] = NONE;
+    }
     const uint32_t k = s_k;
     for (uint32_t i = tid; i < k; i += SAH_BLOCK) {
         s_perm[i] = (uint16_t)i;
@@ -699,7 +730,7 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
                     uint32_t i = i0 + c * 64 + lane;
                     bool bad = (c * 64 + lane < T) && (s_bucket[i] > (uint32_t)best);
                     uint64_t m = ballot64(bad);
-                    if (bad) s.list_a[i0 + rl + popc64(m & lanemask_lt())] = i;
+                    if (bad) s_list_a[i0 + rl + popc64(m & lanemask_lt())] = (uint16_t)i;
                     rl += popc64(m);
                 }
                 uint32_t rr = 0, rsz = size - T;
@@ -707,13 +738,12 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
                     uint32_t i = i1 - (c * 64 + lane);
                     bool bad = (c * 64 + lane < rsz) && (s_bucket[i] <= (uint32_t)best);
                     uint64_t m = ballot64(bad);
-                    if (bad) s.list_b[i0 + rr + popc64(m & lanemask_lt())] = i;
+                    if (bad) s_list_b[i0 + rr + popc64(m & lanemask_lt())] = (uint16_t)i;
                     rr += popc64(m);
                 }
-                __threadfence_block();
                 __builtin_amdgcn_wave_barrier();
                 for (uint32_t r = lane; r < rl; r += 64) {
-                    uint32_t a = s.list_a[i0 + r], b = s.list_b[i0 + r];
+                    uint32_t a = s_list_a[i0 + r], b = s_list_b[i0 + r];
                     uint16_t pa = s_perm[a], pb = s_perm[b];
                     s_perm[a] = pb;
                     s_perm[b] = pa;
@@ -722,7 +752,7 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
                 mid = i0 + T;
             }
             if (lane == 0) {
-                uint32_t node = atomicAdd(s.upper_count, 1u);
+                uint32_t node = atomicAdd(&s_upper, 1u);
                 uint32_t ct = atomicAdd(&s_task_next, 2u);
                 s.u_axis[node] = (uint32_t)dim;
                 link_child(s, parent, node);
@@ -734,24 +764,39 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
                 s.task_parent[ct + 1] = (node << 1) | 1u;
             }
         }
-        // ---- small ranges: one lane each, to completion
+        // ---- small ranges: one lane each, to completion; the larger child of a
+        // split waits on the lane's LDS stack (offsets relative to the task)
         for (uint32_t t = lb + tid; t < le; t += SAH_BLOCK) {
-            uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
-            if (i1 - i0 + 1 > LANE_TASK_MAX) continue;
-            uint32_t st0[LANE_TASK_MAX + 2], st1[LANE_TASK_MAX + 2], stp[LANE_TASK_MAX + 2];
+            const uint32_t t0 = s.task_i0[t], t1 = s.task_i1[t];
+            if (t1 - t0 + 1 > LANE_TASK_MAX) continue;
+            uint32_t a = t0, b = t1, p = s.task_parent[t];
             int sp = 0;
-            st0[0] = i0; st1[0] = i1; stp[0] = s.task_parent[t]; sp = 1;
-            while (sp > 0) {
-                sp--;
-                uint32_t a = st0[sp], b = st1[sp], p = stp[sp];
-                if (a == b) { link_child(s, p, 0x80000000u | s_perm[a]); continue; }
+            while (true) {
+                if (a == b) {
+                    link_child(s, p, 0x80000000u | s_perm[a]);
+                    if (sp == 0) break;
+                    uint32_t e = s_lstack[--sp][tid];
+                    a = t0 + (e & 31u);
+                    b = t0 + ((e >> 5) & 31u);
+                    p = e >> 10;
+                    continue;
+                }
                 int dim;
                 uint32_t mid = lane_split(s, s_perm, s_bucket, a, b, &dim);
-                uint32_t node = atomicAdd(s.upper_count, 1u);
+                uint32_t node = atomicAdd(&s_upper, 1u);
                 s.u_axis[node] = (uint32_t)dim;
                 link_child(s, p, node);
-                st0[sp] = mid; st1[sp] = b; stp[sp] = (node << 1) | 1u; sp++;
-                st0[sp] = a; st1[sp] = mid - 1; stp[sp] = node << 1; sp++;
+                uint32_t nl = mid - a, nr = b + 1 - mid;
+                uint32_t pa, pb, pp;  // the child that waits
+                if (nl <= nr) {
+                    pa = mid; pb = b; pp = (node << 1) | 1u;
+                    b = mid - 1; p = node << 1;
+                } else {
+                    pa = a; pb = mid - 1; pp = node << 1;
+                    a = mid; p = (node << 1) | 1u;
+                }
+                if (sp < LANE_STACK) s_lstack[sp++][tid] = (pa - t0) | ((pb - t0) << 5) | (pp << 10);
+                else atomicOr(s.err, 4u);
             }
         }
         __threadfence_block();
@@ -760,7 +805,8 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
         le = s_task_next;
         __syncthreads();
     }
-    const uint32_t num_upper = *s.upper_count;
+    const uint32_t num_upper = s_upper;
+    if (tid == 0) *s.upper_count = num_upper;
     for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) s.u_arrive[u] = 0;
     __threadfence_block();
     __syncthreads();
@@ -826,10 +872,13 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t
             store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
         }
     }
+    uint32_t part = 0;
+    for (uint32_t t = tid; t < k; t += SAH_BLOCK) part += s.t_nodes[t];
+    for (int off = 32; off >= 1; off >>= 1) part += (uint32_t)__shfl_xor((int)part, off);
+    if (lane == 0) atomicAdd(&s_total, part);
+    __syncthreads();
     if (tid == 0) {
-        uint32_t total = num_upper;
-        for (uint32_t t = 0; t < k; t++) total += s.t_nodes[t];
-        *s.n_nodes = total;
+        *s.n_nodes = num_upper + s_total;
         *s.n_treelets = k;
     }
     (void)n_total_shapes;
@@ -865,223 +914,232 @@ __global__ __launch_bounds__(256) void k_scatter(LbvhScratch s) {
 // Traversal layout (derived from the reference node arrays; the reference arrays
 // themselves are unchanged).  Record r (64 bytes):
 //   [0..5] child0 box, [6..11] child1 box, [12] child0 word, [13] child1 word,
-//   [14] split axis (the near child is child1 when dirIsNeg[axis])
+//   [14] split axis (child0 lies below child1 on it; the near child is child1
+//   when dirIsNeg[axis])
 // A child word is either a record index (bit 31 clear) or a primitive range:
-// 0x80000000 | (count - 1) << 28 | first primitive position (count <= 4).
+// 0x80000000 | (count - 1) << 28 | first slot (count <= 4).
 // Records 0 .. n_nodes-1 are the reference nodes: interior nodes hold their two
-// children; a reference leaf with more than 4 primitives holds the root of a
-// small binary tree over its primitives (median splits, ranges of <= 4), whose
-// other records are appended after n_nodes.  Primitives are stored in that
-// order; each keeps its slot (position in the reference's shapesEntities) and
-// its leaf's first slot, so ties inside one reference leaf still go to the
-// lowest slot, as the reference's candidate order does (SLBVH.cpp:110-126).
+// children; a reference leaf with more than 4 shapes holds the root of a binary
+// tree over its slots, whose other records are appended after n_nodes.
+// The slots of a leaf are in Morton order, so halving the slot range is a
+// spatial split: node (L, j) of a leaf (off, n) covers slots
+// [off + j*n/2^L, off + (j+1)*n/2^L) and has children (L+1, 2j), (L+1, 2j+1).
+// Primitive records sit at their slot (position in the reference's
+// shapesEntities) and keep their leaf's first slot, so ties inside one
+// reference leaf still go to the lowest slot, as the reference's candidate
+// order does (SLBVH.cpp:110-126).
 constexpr uint32_t RANGE_MAX = 4;
-constexpr int SUB_STACK = 40;
+constexpr uint32_t BIG_LEAF = 64;  // larger leaves (all codes equal) get a workgroup
 
 WPBR_D uint32_t range_word(uint32_t first, uint32_t count) {
     return 0x80000000u | ((count - 1) << 28) | first;
 }
 
-__global__ void k_trav_layout(LbvhScratch s) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t nn = *s.n_nodes;
-    if (i >= nn || s.node_nshapes[i] > 0) return;
-    float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)i;
-    uint32_t c[2] = {i + 1, s.node_offset[i]};
-    uint32_t w[2];
-    for (int k = 0; k < 2; k++) {
-        uint32_t ns = s.node_nshapes[c[k]];
-        w[k] = (ns > 0 && ns <= RANGE_MAX) ? range_word(s.node_offset[c[k]], ns) : c[k];
-    }
-    const float* b0 = s.node_bounds + 6 * (size_t)c[0];
-    const float* b1 = s.node_bounds + 6 * (size_t)c[1];
-    rec[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
-    rec[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
-    rec[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
-    rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
-                         __uint_as_float((uint32_t)s.node_axis[i]), 0.0f);
+WPBR_D uint32_t sub_lo(uint32_t n, uint32_t L, uint32_t j) {
+    return (uint32_t)(((uint64_t)j * n) >> L);
 }
 
-WPBR_D float cen_of(const LbvhScratch& s, const uint32_t* sorted_index, uint32_t slot, int axis) {
-    return s.centroid[3 * (size_t)sorted_index[slot] + axis];
+// Deepest level of a leaf's sub-tree that holds an interior node.
+WPBR_D uint32_t sub_levels(uint32_t n) {
+    uint32_t L = 0;
+    while (((n + (2u << L) - 1) >> (L + 1)) > RANGE_MAX) L++;
+    return L;
 }
 
-// Writes the primitive record of position p (holding reference slot `slot`).
-WPBR_D void write_prim(const LbvhScratch& s, const GeomView& g, const uint32_t* sorted_index,
-                       uint32_t p, uint32_t slot, uint32_t leaf_off) {
-    uint32_t id = sorted_index[slot];
-    float4* rec = reinterpret_cast<float4*>(s.prims) + 3 * (size_t)p;
-    if (id < g.n_spheres) {
-        rec[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(id));
-        rec[1] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(slot));
-        rec[2] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(leaf_off));
-        return;
-    }
-    const float* t = g.tri_verts + 9 * (size_t)(id - g.n_spheres);
-    rec[0] = make_float4(t[0], t[1], t[2], __uint_as_float(id));
-    rec[1] = make_float4(t[3], t[4], t[5], __uint_as_float(slot));
-    rec[2] = make_float4(t[6], t[7], t[8], __uint_as_float(leaf_off));
+WPBR_D uint32_t sub_record(uint32_t root, uint32_t base, uint32_t L, uint32_t j) {
+    return L == 0 ? root : base + (1u << L) - 2 + j;
 }
 
-// Sub-tree of a leaf with at most SMALL_LEAF shapes, built from a private copy
-// of its centroids and boxes (same splits as the general path below).
-constexpr uint32_t SMALL_LEAF = 32;
-
-WPBR_D void subtree_small(LbvhScratch& s, const GeomView& g, const uint32_t* sorted_index,
-                          uint32_t i, uint32_t off, uint32_t n, uint32_t nn) {
-    float cen[SMALL_LEAF][3];
-    uint8_t perm[SMALL_LEAF];
-    for (uint32_t k = 0; k < n; k++) {
-        uint32_t id = sorted_index[off + k];
-        for (int c = 0; c < 3; c++) cen[k][c] = s.centroid[3 * (size_t)id + c];
-        perm[k] = (uint8_t)k;
-    }
-    uint8_t sb[SUB_STACK], se[SUB_STACK];
-    uint32_t sr[SUB_STACK];
-    int sp = 1;
-    sb[0] = 0; se[0] = (uint8_t)n; sr[0] = i;
-    while (sp > 0) {
-        sp--;
-        uint32_t a = sb[sp], b = se[sp], r = sr[sp];
-        uint32_t m = b - a;
-        float lo[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, hi[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-        for (uint32_t q = a; q < b; q++)
-            for (int k = 0; k < 3; k++) {
-                lo[k] = smin(lo[k], cen[perm[q]][k]);
-                hi[k] = smax(hi[k], cen[perm[q]][k]);
-            }
-        float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
-        int axis = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
-        float ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
-        uint32_t mid = a + m / 2;
-        if (ext > 0.0f) {
-            for (uint32_t x = a + 1; x < b; x++) {
-                uint8_t v = perm[x];
-                float cv = cen[v][axis];
-                uint32_t y = x;
-                while (y > a && cen[perm[y - 1]][axis] > cv) { perm[y] = perm[y - 1]; y--; }
-                perm[y] = v;
-            }
-        }
-        uint32_t cb[2] = {a, mid}, ce[2] = {mid, b}, w[2];
-        bbox box[2];
-        for (int c = 0; c < 2; c++) {
-            box[c] = box_empty();
-            for (uint32_t q = cb[c]; q < ce[c]; q++)
-                box[c] = box_union(box[c], load_box(s.bounds + 6 * (size_t)sorted_index[off + perm[q]]));
-            uint32_t cm = ce[c] - cb[c];
-            if (cm <= RANGE_MAX) {
-                w[c] = range_word(off + cb[c], cm);
-            } else {
-                uint32_t nr = nn + atomicAdd(s.sub_count, 1u);
-                w[c] = nr;
-                sb[sp] = (uint8_t)cb[c]; se[sp] = (uint8_t)ce[c]; sr[sp] = nr; sp++;
-            }
-        }
-        float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)r;
-        rec[0] = make_float4(box[0].mn[0], box[0].mn[1], box[0].mn[2], box[0].mx[0]);
-        rec[1] = make_float4(box[0].mx[1], box[0].mx[2], box[1].mn[0], box[1].mn[1]);
-        rec[2] = make_float4(box[1].mn[2], box[1].mx[0], box[1].mx[1], box[1].mx[2]);
-        rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
-                             __uint_as_float((uint32_t)axis), 0.0f);
-    }
-    for (uint32_t k = 0; k < n; k++) write_prim(s, g, sorted_index, off + k, off + perm[k], off);
+WPBR_D bbox slots_box(const LbvhScratch& s, const uint32_t* sorted_index, uint32_t a, uint32_t b) {
+    bbox r = box_empty();
+    for (uint32_t q = a; q < b; q++) r = box_union(r, load_box(s.bounds + 6 * (size_t)sorted_index[q]));
+    return r;
 }
 
-// One thread per reference node; leaves build their sub-tree and write their
-// primitives.  sub_perm (one entry per slot) holds the working order.
-__global__ void k_subtree(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t nn = *s.n_nodes;
+WPBR_D void write_rec(const LbvhScratch& s, uint32_t r, const bbox& b0, const bbox& b1, uint32_t w0,
+                      uint32_t w1, uint32_t axis) {
+    float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)r;
+    rec[0] = make_float4(b0.mn[0], b0.mn[1], b0.mn[2], b0.mx[0]);
+    rec[1] = make_float4(b0.mx[1], b0.mx[2], b1.mn[0], b1.mn[1]);
+    rec[2] = make_float4(b1.mn[2], b1.mx[0], b1.mx[1], b1.mx[2]);
+    rec[3] = make_float4(__uint_as_float(w0), __uint_as_float(w1), __uint_as_float(axis), 0.0f);
+}
+
+// Record r from two child boxes and words when the split has no axis of its
+// own: children ordered along the axis on which their centres differ most.
+WPBR_D void write_pair(const LbvhScratch& s, uint32_t r, bbox b0, bbox b1, uint32_t w0, uint32_t w1) {
+    float d[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) d[c] = (b1.mn[c] + b1.mx[c]) - (b0.mn[c] + b0.mx[c]);
+    float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+    uint32_t axis = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    float da = axis == 0 ? d[0] : (axis == 1 ? d[1] : d[2]);
+    if (da < 0.0f) write_rec(s, r, b1, b0, w1, w0, axis);
+    else write_rec(s, r, b0, b1, w0, w1, axis);
+}
+
+constexpr int SUB_STACK = 8;  // smaller child first: depth <= log2(BIG_LEAF / 5) + 1
+
+// One thread per reference node: interior nodes get their record; leaves of at
+// most BIG_LEAF shapes get a sub-tree split like emitLBVH below its leaf size
+// (SLBVH.cpp:374-401): at the highest Morton bit that differs inside the range
+// (child 0 below child 1 on axis bit % 3), or in halves when all codes are
+// equal.  Larger leaves (all codes equal) are queued for k_subtree_big.
+__global__ __launch_bounds__(256) void k_subtree(LbvhScratch s, const uint32_t* __restrict__ codes,
+                                                 const uint32_t* __restrict__ sorted_index) {
+    __shared__ uint16_t s_ab[SUB_STACK][256];
+    __shared__ uint32_t s_rec[SUB_STACK][256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nn = *s.n_nodes;
+    const uint32_t n = (i < nn) ? s.node_nshapes[i] : 0;
+    const uint32_t off = (n > 0) ? s.node_offset[i] : 0;
+    // sub-tree records (root excluded): at most n - 5 interior nodes of > 4 shapes
+    uint32_t need = (n > RANGE_MAX && n <= BIG_LEAF) ? n - RANGE_MAX - 1 : 0;
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t incl = need;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t v = (uint32_t)__shfl_up((int)incl, d);
+        if (lane >= (uint32_t)d) incl += v;
+    }
+    const uint32_t wtotal = (uint32_t)__shfl((int)incl, 63);
+    uint32_t wbase = 0;
+    if (lane == 63 && wtotal) wbase = atomicAdd(s.sub_count, wtotal);
+    wbase = (uint32_t)__shfl((int)wbase, 63);
     if (i >= nn) return;
-    uint32_t n = s.node_nshapes[i];
-    if (n == 0) return;
-    const uint32_t off = s.node_offset[i];
-    if (n > RANGE_MAX && n <= SMALL_LEAF) {
-        subtree_small(s, g, sorted_index, i, off, n, nn);
+    if (n == 0) {
+        uint32_t c[2] = {i + 1, s.node_offset[i]};
+        uint32_t w[2];
+        for (int k = 0; k < 2; k++) {
+            uint32_t ns = s.node_nshapes[c[k]];
+            w[k] = (ns > 0 && ns <= RANGE_MAX) ? range_word(s.node_offset[c[k]], ns) : c[k];
+        }
+        const float* b0 = s.node_bounds + 6 * (size_t)c[0];
+        const float* b1 = s.node_bounds + 6 * (size_t)c[1];
+        float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)i;
+        rec[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
+        rec[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
+        rec[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
+        rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
+                             __uint_as_float((uint32_t)s.node_axis[i]), 0.0f);
         return;
     }
-    uint32_t* perm = s.sub_perm;
-    for (uint32_t k = 0; k < n; k++) perm[off + k] = off + k;
-    if (n > RANGE_MAX) {
-        uint32_t sb[SUB_STACK], se[SUB_STACK], sr[SUB_STACK];
-        int sp = 1;
-        sb[0] = off; se[0] = off + n; sr[0] = i;
-        while (sp > 0) {
+    if (n > BIG_LEAF) {
+        uint32_t e = atomicAdd(s.big_count, 1u);
+        s.big_list[e] = i;
+        return;
+    }
+    for (uint32_t k = 0; k < n; k++) s.slot_leaf[off + k] = off;
+    if (n <= RANGE_MAX) return;
+    uint32_t next = nn + wbase + incl - need;
+    uint32_t a = 0, b = n, r = i;
+    int sp = 0;
+    while (true) {
+        const uint32_t ca = codes[off + a], cl = codes[off + b - 1];
+        uint32_t m;
+        int axis = -1;
+        if (ca != cl) {
+            int bit = 31 - __clz(ca ^ cl);
+            m = a + 1;
+            while (!((codes[off + m] >> bit) & 1u)) m++;
+            axis = bit % 3;
+        } else {
+            m = a + (b - a) / 2;
+        }
+        const bool in0 = m - a > RANGE_MAX, in1 = b - m > RANGE_MAX;
+        const uint32_t w0 = in0 ? next++ : range_word(off + a, m - a);
+        const uint32_t w1 = in1 ? next++ : range_word(off + m, b - m);
+        bbox b0 = slots_box(s, sorted_index, off + a, off + m);
+        bbox b1 = slots_box(s, sorted_index, off + m, off + b);
+        if (axis >= 0) write_rec(s, r, b0, b1, w0, w1, (uint32_t)axis);
+        else write_pair(s, r, b0, b1, w0, w1);
+        if (in0 && in1) {
+            uint32_t pa, pb, pr;
+            if (m - a <= b - m) { pa = m; pb = b; pr = w1; b = m; r = w0; }
+            else { pa = a; pb = m; pr = w0; a = m; r = w1; }
+            if (sp < SUB_STACK) { s_ab[sp][threadIdx.x] = (uint16_t)(pa | (pb << 7)); s_rec[sp][threadIdx.x] = pr; sp++; }
+            else atomicOr(s.err, 4u);
+        } else if (in0) {
+            b = m; r = w0;
+        } else if (in1) {
+            a = m; r = w1;
+        } else {
+            if (sp == 0) break;
             sp--;
-            uint32_t a = sb[sp], b = se[sp], r = sr[sp];
-            uint32_t m = b - a;
-            float lo[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, hi[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-            for (uint32_t q = a; q < b; q++) {
-                for (int k = 0; k < 3; k++) {
-                    float c = cen_of(s, sorted_index, perm[q], k);
-                    lo[k] = smin(lo[k], c);
-                    hi[k] = smax(hi[k], c);
-                }
-            }
-            float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
-            int axis = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
-            float ext = axis == 0 ? ex : (axis == 1 ? ey : ez);
-            uint32_t mid = a + m / 2;
-            if (ext > 0.0f) {
-                // insertion sort by centroid on the split axis (ranges are small:
-                // reference leaves hold fewer than 30 shapes unless all codes are equal)
-                if (m <= 64) {
-                    for (uint32_t x = a + 1; x < b; x++) {
-                        uint32_t v = perm[x];
-                        float cv = cen_of(s, sorted_index, v, axis);
-                        uint32_t y = x;
-                        while (y > a && cen_of(s, sorted_index, perm[y - 1], axis) > cv) {
-                            perm[y] = perm[y - 1];
-                            y--;
-                        }
-                        perm[y] = v;
-                    }
-                } else {
-                    // Lomuto partitions around the middle element (nth_element)
-                    uint32_t L = a, R = b - 1;
-                    while (L < R) {
-                        uint32_t pm = (L + R) / 2;
-                        uint32_t t = perm[pm]; perm[pm] = perm[R]; perm[R] = t;
-                        float pv = cen_of(s, sorted_index, perm[R], axis);
-                        uint32_t st = L;
-                        for (uint32_t x = L; x < R; x++) {
-                            if (cen_of(s, sorted_index, perm[x], axis) < pv) {
-                                uint32_t u = perm[x]; perm[x] = perm[st]; perm[st] = u;
-                                st++;
-                            }
-                        }
-                        t = perm[st]; perm[st] = perm[R]; perm[R] = t;
-                        if (st == mid) break;
-                        if (st < mid) L = st + 1;
-                        else R = st - 1;
-                    }
-                }
-            }
-            uint32_t cb[2] = {a, mid}, ce[2] = {mid, b}, w[2];
-            bbox box[2];
-            for (int c = 0; c < 2; c++) {
-                box[c] = box_empty();
-                for (uint32_t q = cb[c]; q < ce[c]; q++)
-                    box[c] = box_union(box[c], load_box(s.bounds + 6 * (size_t)sorted_index[perm[q]]));
-                uint32_t cm = ce[c] - cb[c];
-                if (cm <= RANGE_MAX) {
-                    w[c] = range_word(cb[c], cm);
-                } else {
-                    uint32_t nr = nn + atomicAdd(s.sub_count, 1u);
-                    w[c] = nr;
-                    sb[sp] = cb[c]; se[sp] = ce[c]; sr[sp] = nr; sp++;
-                }
-            }
-            float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)r;
-            rec[0] = make_float4(box[0].mn[0], box[0].mn[1], box[0].mn[2], box[0].mx[0]);
-            rec[1] = make_float4(box[0].mx[1], box[0].mx[2], box[1].mn[0], box[1].mn[1]);
-            rec[2] = make_float4(box[1].mn[2], box[1].mx[0], box[1].mx[1], box[1].mx[2]);
-            rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
-                                 __uint_as_float((uint32_t)axis), 0.0f);
+            uint32_t e = s_ab[sp][threadIdx.x];
+            a = e & 127u; b = e >> 7; r = s_rec[sp][threadIdx.x];
         }
     }
-    for (uint32_t k = 0; k < n; k++) write_prim(s, g, sorted_index, off + k, perm[off + k], off);
+}
+
+WPBR_D bbox record_union(const LbvhScratch& s, uint32_t r) {
+    const float4* rec = reinterpret_cast<const float4*>(s.trav) + 4 * (size_t)r;
+    float4 r0 = rec[0], r1 = rec[1], r2 = rec[2];
+    bbox a, b;
+    a.mn[0] = r0.x; a.mn[1] = r0.y; a.mn[2] = r0.z; a.mx[0] = r0.w; a.mx[1] = r1.x; a.mx[2] = r1.y;
+    b.mn[0] = r1.z; b.mn[1] = r1.w; b.mn[2] = r2.x; b.mx[0] = r2.y; b.mx[1] = r2.z; b.mx[2] = r2.w;
+    return box_union(a, b);
+}
+
+// Leaves of more than BIG_LEAF shapes (only when every Morton code in the leaf
+// is equal): one workgroup each, levels bottom-up.
+__global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32_t* __restrict__ sorted_index) {
+    const uint32_t count = *s.big_count;
+    const uint32_t nn = *s.n_nodes;
+    __shared__ uint32_t s_base;
+    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+        const uint32_t i = s.big_list[e];
+        const uint32_t n = s.node_nshapes[i], off = s.node_offset[i];
+        const uint32_t Lmax = sub_levels(n);
+        if (threadIdx.x == 0) s_base = nn + atomicAdd(s.sub_count, (2u << Lmax) - 2);
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s.slot_leaf[off + k] = off;
+        __syncthreads();
+        const uint32_t base = s_base;
+        for (int L = (int)Lmax; L >= 0; L--) {
+            for (uint32_t j = threadIdx.x; j < (1u << L); j += blockDim.x) {
+                uint32_t a = sub_lo(n, L, j), b = sub_lo(n, L, j + 1);
+                if (b - a <= RANGE_MAX) continue;
+                uint32_t m = sub_lo(n, L + 1, 2 * j + 1);
+                uint32_t w0, w1;
+                bbox b0, b1;
+                if (m - a <= RANGE_MAX) {
+                    w0 = range_word(off + a, m - a);
+                    b0 = slots_box(s, sorted_index, off + a, off + m);
+                } else {
+                    w0 = sub_record(i, base, L + 1, 2 * j);
+                    b0 = record_union(s, w0);
+                }
+                if (b - m <= RANGE_MAX) {
+                    w1 = range_word(off + m, b - m);
+                    b1 = slots_box(s, sorted_index, off + m, off + b);
+                } else {
+                    w1 = sub_record(i, base, L + 1, 2 * j + 1);
+                    b1 = record_union(s, w1);
+                }
+                write_pair(s, sub_record(i, base, L, j), b0, b1, w0, w1);
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
+// Primitive records at their slots: v0.xyz + shape id, v1.xyz + slot,
+// v2.xyz + first slot of the reference leaf.  Spheres keep only the words.
+__global__ void k_prims(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index, uint32_t n) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    uint32_t id = sorted_index[p];
+    uint32_t leaf = s.slot_leaf[p];
+    float4* rec = reinterpret_cast<float4*>(s.prims) + 3 * (size_t)p;
+    float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (id >= g.n_spheres) {
+        const float* t = g.tri_verts + 9 * (size_t)(id - g.n_spheres);
+#pragma unroll
+        for (int k = 0; k < 9; k++) v[k] = t[k];
+    }
+    rec[0] = make_float4(v[0], v[1], v[2], __uint_as_float(id));
+    rec[1] = make_float4(v[3], v[4], v[5], __uint_as_float(p));
+    rec[2] = make_float4(v[6], v[7], v[8], __uint_as_float(leaf));
 }
 
 // ---------------------------------------------------------------------------
@@ -1138,8 +1196,9 @@ wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream
     hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
     rec(10);
     uint32_t max_nodes = 2 * n;
-    hipLaunchKernelGGL(k_trav_layout, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s);
-    hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, g, s.vals_a);
+    hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, s.keys_a, s.vals_a);
+    hipLaunchKernelGGL(k_subtree_big, dim3(64), dim3(256), 0, st, s, s.vals_a);
+    hipLaunchKernelGGL(k_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, s.vals_a, n);
     rec(11);
     return launch_status();
 }

@@ -61,9 +61,11 @@ struct LbvhScratch {
     uint32_t* node_offset; uint32_t* node_nshapes; uint8_t* node_axis; float* node_bounds;
     // traversal layout
     void* trav;           // 64 B per node
-    void* prims;          // 48 B per shape, traversal order
-    uint32_t* sub_perm;   // per slot: working order of the leaf sub-trees
+    void* prims;          // 48 B per shape, at its slot
+    uint32_t* slot_leaf;  // per slot: first slot of its reference leaf
     uint32_t* sub_count;  // sub-tree records appended after the reference nodes
+    uint32_t* big_list;   // reference leaves of more than 64 shapes
+    uint32_t* big_count;
 };
 
 wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st,
