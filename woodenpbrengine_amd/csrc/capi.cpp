@@ -70,27 +70,18 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.r_size = c.take<uint32_t>(2 * N);
     s.r_off = c.take<uint32_t>(2 * N);
     s.r_box = c.take<float>(12 * N);
-    s.perm = c.take<uint32_t>(4096);
-    s.bucket = c.take<uint8_t>(4096);
-    s.list_a = c.take<uint32_t>(4096);
-    s.list_b = c.take<uint32_t>(4096);
     s.task_i0 = c.take<uint32_t>(8192);
     s.task_i1 = c.take<uint32_t>(8192);
-    s.task_parent = c.take<uint32_t>(8192);
-    s.task_depth = c.take<uint32_t>(8192);
-    s.u_c0 = c.take<uint32_t>(4096);
-    s.u_c1 = c.take<uint32_t>(4096);
+    s.task_dep = c.take<uint32_t>(8192);
+    s.task_rt = c.take<uint32_t>(8192);
+    s.u_i0 = c.take<uint32_t>(4096);
+    s.u_i1 = c.take<uint32_t>(4096);
+    s.u_mid = c.take<uint32_t>(4096);
+    s.u_dep = c.take<uint32_t>(4096);
+    s.u_rt = c.take<uint32_t>(4096);
     s.u_axis = c.take<uint32_t>(4096);
-    s.u_depth = c.take<uint32_t>(4096);
-    s.u_size = c.take<uint32_t>(4096);
-    s.u_off = c.take<uint32_t>(4096);
     s.u_box = c.take<float>(6 * 4096);
-    s.u_parent = c.take<uint32_t>(4096);
-    s.u_arrive = c.take<uint32_t>(4096);
-    s.t_parent = c.take<uint32_t>(4096);
-    s.t_cen = c.take<float>(3 * 4096);
     s.upper_count = c.take<uint32_t>(1);
-    s.upper_root = c.take<uint32_t>(1);
     s.n_nodes = c.take<uint32_t>(1);
     s.n_treelets = c.take<uint32_t>(1);
     s.node_offset = c.take<uint32_t>(2 * N);

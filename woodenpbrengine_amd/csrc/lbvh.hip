@@ -484,18 +484,44 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32
 
 // ---------------------------------------------------------------------------
 // Upper SAH over treelet roots (buildUpperSAH, SLBVH.cpp:457-563): one
-// workgroup.  The position array `perm` (treelet ids, permuted in place as
-// std::partition does) and the per-position bucket live in LDS.  Tasks (ranges of
-// perm) larger than LANE_TASK_MAX are processed level by level, one wave each;
-// smaller ranges are finished by a single lane, depth first, with the oracle's
-// sequential algorithm.  Node sizes and bounds are then formed bottom-up with
-// arrival counters, and preorder offsets top-down by walking parent links.
-constexpr int SAH_BLOCK = 1024;
+// workgroup.  The treelet root boxes, the position array `perm` (treelet ids,
+// permuted in place as std::partition does) and the per-position bucket live in
+// LDS.  Ranges of at least BLOCK_TASK_MIN positions are split by the whole
+// workgroup, ranges above LANE_TASK_MAX by one wave, smaller ones by one lane
+// (depth first, the larger child waiting on the lane's LDS stack).
+//
+// No parent/child links are kept: every node covers a contiguous range of final
+// positions [i0, i1], so with P = exclusive prefix sum of the treelet node
+// counts over final positions, a node of depth d whose root path turns right r
+// times sits at preorder offset d + (i0 - r) + P[i0] (the nodes before it are
+// its d ancestors, and the r subtrees hanging left of its path, which hold the
+// i0 treelets before it and i0 - r upper nodes).  The size of the subtree over
+// [i0, m) is (m - 1 - i0) + P[m] - P[i0].  Node bounds are the union of the
+// range's treelet boxes, computed when the range is split (the same value as
+// the union of the two child boxes; min/max are exact).
+constexpr int SAH_BLOCK = 512;   // 256 VGPRs per lane for the register-resident buckets
 constexpr int SAH_WAVES = SAH_BLOCK / 64;
 constexpr int NB = 12;
 constexpr uint32_t LANE_TASK_MAX = 32;
-constexpr int LANE_STACK = 6;  // smaller child first: depth <= log2(LANE_TASK_MAX) + 1
-constexpr uint32_t NONE = 0xFFFFFFFFu;
+constexpr int LANE_STACK = 6;               // smaller child first: depth <= log2(32) + 1
+constexpr uint32_t BLOCK_TASK_MIN = 1024;   // ranges this large use the whole workgroup
+static_assert(LANE_STACK * SAH_BLOCK <= 4100, "lane stacks share the prefix-sum array");
+
+struct SahLds {
+    float tbox[6 * 4096];       // treelet root boxes
+    uint16_t perm[4096];
+    uint8_t bucket[4096];
+    uint16_t la[4096], lb[4096];            // partition lists; then leaf depth / right turns
+    uint32_t aux[4100];         // lane stacks (LANE_STACK x SAH_BLOCK); then the prefix sums P[0..k]
+    uint32_t bcount[SAH_WAVES][NB];
+    uint32_t bmin[SAH_WAVES][NB][3];
+    uint32_t bmax[SAH_WAVES][NB][3];
+    float red[SAH_WAVES][12];
+    float fin[12];
+    uint32_t wcnt[SAH_WAVES];
+    uint32_t split[2];
+    uint32_t k, task_next, upper, nbig, nbig_next, total;
+};
 
 WPBR_D float wave_min_f(float v) {
     for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off));
@@ -506,29 +532,107 @@ WPBR_D float wave_max_f(float v) {
     return v;
 }
 
-WPBR_D float c_at(const LbvhScratch& s, uint32_t t, int dim) { return s.t_cen[3 * (size_t)t + dim]; }
-
-// Link a child reference (bit 31: treelet) under parent reference (node << 1 | side).
-WPBR_D void link_child(LbvhScratch& s, uint32_t parent, uint32_t ref) {
-    if (parent == NONE) *s.upper_root = ref;
-    else if (parent & 1u) s.u_c1[parent >> 1] = ref;
-    else s.u_c0[parent >> 1] = ref;
-    if (ref & 0x80000000u) s.t_parent[ref & 0x7FFFFFFFu] = parent;
-    else s.u_parent[ref] = parent;
+WPBR_D bbox tb_box(const SahLds& L, uint32_t t) {
+    const float* p = L.tbox + 6 * t;
+    bbox b;
+    b.mn[0] = p[0]; b.mn[1] = p[1]; b.mn[2] = p[2];
+    b.mx[0] = p[3]; b.mx[1] = p[4]; b.mx[2] = p[5];
+    return b;
 }
 
-// SAH split of perm[i0..i1] by one lane (the oracle's loop, SLBVH.cpp:465-553).
+// Treelet root centroid on one axis: (pMin + pMax) * 0.5 (SLBVH.cpp:472, 487).
+WPBR_D float tb_cen(const SahLds& L, uint32_t t, int dim) {
+    const float* p = L.tbox + 6 * t;
+    return (float)((double)(p[dim] + p[3 + dim]) * 0.5);
+}
+
+WPBR_D uint32_t sah_bucket(float c, float cmin, float dist) {
+    uint32_t ib = 0;
+    if (dist > 0.0f) {
+        float tt = (c - cmin) / dist;
+        ib = (uint32_t)(uint8_t)(uint32_t)((float)NB * tt);
+        if (ib == NB) ib = NB - 1;
+    }
+    return ib;
+}
+
+// Records upper node [i0, i1] split at mid; returns its index.
+WPBR_D uint32_t sah_node(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t mid, int dim,
+                         const bbox& bounds, uint32_t dep, uint32_t rt) {
+    uint32_t node = atomicAdd(&L.upper, 1u);
+    s.u_i0[node] = i0;
+    s.u_i1[node] = i1;
+    s.u_mid[node] = mid;
+    s.u_dep[node] = dep;
+    s.u_rt[node] = rt;
+    s.u_axis[node] = (uint32_t)dim;
+    store_box(s.u_box + 6 * (size_t)node, bounds);
+    return node;
+}
+
+// Queues the two children of a split made by a wave or the workgroup.
+WPBR_D void sah_children(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t mid, uint32_t dep,
+                         uint32_t rt) {
+    uint32_t ct = atomicAdd(&L.task_next, 2u);
+    s.task_i0[ct] = i0;
+    s.task_i1[ct] = mid - 1;
+    s.task_dep[ct] = dep + 1;
+    s.task_rt[ct] = rt;
+    s.task_i0[ct + 1] = mid;
+    s.task_i1[ct + 1] = i1;
+    s.task_dep[ct + 1] = dep + 1;
+    s.task_rt[ct + 1] = rt + 1;
+    uint32_t nb = (mid - i0 >= BLOCK_TASK_MIN) + (i1 + 1 - mid >= BLOCK_TASK_MIN);
+    if (nb) atomicAdd(&L.nbig_next, nb);
+}
+
+// The reference's split cost (SLBVH.cpp:504-538) for split `lane` (< NB-1) from
+// merged bucket statistics row 0; lanes >= NB-1 return +inf.
+WPBR_D float sah_cost(const SahLds& L, int lane, const bbox& bounds, uint32_t* cnt_le) {
+    float cost = INF;
+    *cnt_le = 0;
+    if (lane < NB - 1) {
+        bbox b0 = box_empty(), b1 = box_empty();
+        uint32_t c0 = 0, c1 = 0;
+        for (int j = 0; j < NB; j++) {
+            bbox bj;
+            for (int c = 0; c < 3; c++) {
+                bj.mn[c] = ord2f(L.bmin[0][j][c]);
+                bj.mx[c] = ord2f(L.bmax[0][j][c]);
+            }
+            if (j <= lane) { b0 = box_union(b0, bj); c0 += L.bcount[0][j]; }
+            else { b1 = box_union(b1, bj); c1 += L.bcount[0][j]; }
+        }
+        float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+        float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+        cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
+        *cnt_le = c0;
+    }
+    return cost;
+}
+
+// First minimum over the 11 split costs held by lanes 0..10 of one wave.
+WPBR_D int sah_best(float cost) {
+    int best = 0;
+    float cmin = __shfl(cost, 0);
+    for (int i = 1; i < NB - 1; i++) {
+        float ci = __shfl(cost, i);
+        if (ci < cmin) { best = i; cmin = ci; }
+    }
+    return best;
+}
+
+// SAH split of perm[i0..i1] by one lane (the reference's loop, SLBVH.cpp:465-553).
 // Bucket statistics stay in registers (statically indexed).  The split costs use
 // one prefix and one suffix sweep of unions: min/max are exact, so these boxes
 // equal the reference's per-split unions up to the sign of a zero coordinate,
 // which no extent (and hence no area or cost) can see.
-WPBR_D uint32_t lane_split(const LbvhScratch& s, uint16_t* perm, uint8_t* bucket, uint32_t i0, uint32_t i1,
-                           int* dimOut) {
+WPBR_D uint32_t lane_split(SahLds& L, uint32_t i0, uint32_t i1, int* dimOut, bbox* boundsOut) {
     bbox bounds = box_empty(), cb = box_empty();
     for (uint32_t i = i0; i <= i1; i++) {
-        uint32_t t = perm[i];
-        bounds = box_union(bounds, load_box(s.t_box + 6 * (size_t)t));
-        cb = box_union_pt(cb, mk3(c_at(s, t, 0), c_at(s, t, 1), c_at(s, t, 2)));
+        uint32_t t = L.perm[i];
+        bounds = box_union(bounds, tb_box(L, t));
+        cb = box_union_pt(cb, mk3(tb_cen(L, t, 0), tb_cen(L, t, 1), tb_cen(L, t, 2)));
     }
     int dim = box_max_extent(cb);
     float cmin = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
@@ -538,15 +642,10 @@ WPBR_D uint32_t lane_split(const LbvhScratch& s, uint16_t* perm, uint8_t* bucket
 #pragma unroll
     for (int b = 0; b < NB; b++) { bb[b] = box_empty(); bc[b] = 0; }
     for (uint32_t i = i0; i <= i1; i++) {
-        uint32_t t = perm[i];
-        uint32_t ib = 0;
-        if (dist > 0.0f) {
-            float tt = (c_at(s, t, dim) - cmin) / dist;
-            ib = (uint32_t)(uint8_t)(uint32_t)((float)NB * tt);
-            if (ib == NB) ib = NB - 1;
-        }
-        bucket[i] = (uint8_t)ib;
-        bbox tb = load_box(s.t_box + 6 * (size_t)t);
+        uint32_t t = L.perm[i];
+        uint32_t ib = sah_bucket(tb_cen(L, t, dim), cmin, dist);
+        L.bucket[i] = (uint8_t)ib;
+        bbox tb = tb_box(L, t);
 #pragma unroll
         for (int b = 0; b < NB; b++) {
             if (ib == (uint32_t)b) {
@@ -586,302 +685,401 @@ WPBR_D uint32_t lane_split(const LbvhScratch& s, uint16_t* perm, uint8_t* bucket
     // libstdc++ two-pointer std::partition on perm[i0..i1]
     uint32_t first = i0, last = i1 + 1;
     while (true) {
-        while (first != last && bucket[first] <= best) ++first;
+        while (first != last && L.bucket[first] <= best) ++first;
         if (first == last) break;
         --last;
-        while (first != last && !(bucket[last] <= best)) --last;
+        while (first != last && !(L.bucket[last] <= best)) --last;
         if (first == last) break;
-        uint16_t tp = perm[first]; perm[first] = perm[last]; perm[last] = tp;
-        uint8_t tb = bucket[first]; bucket[first] = bucket[last]; bucket[last] = tb;
+        uint16_t tp = L.perm[first]; L.perm[first] = L.perm[last]; L.perm[last] = tp;
+        uint8_t tb = L.bucket[first]; L.bucket[first] = L.bucket[last]; L.bucket[last] = tb;
         ++first;
     }
     uint32_t mid = first;
     if (mid == i1 + 1) mid = (i1 - i0 + 1) / 2 + i0;
     *dimOut = dim;
+    *boundsOut = bounds;
     return mid;
 }
 
-__global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s, uint32_t n_total_shapes) {
-    __shared__ uint16_t s_perm[4096];
-    __shared__ uint8_t s_bucket[4096];
-    __shared__ uint16_t s_list_a[4096];
-    __shared__ uint16_t s_list_b[4096];
-    __shared__ uint32_t s_lstack[LANE_STACK][SAH_BLOCK];
-    __shared__ uint32_t s_k;
-    __shared__ uint32_t s_task_next;
-    __shared__ uint32_t s_upper;
-    __shared__ uint32_t s_total;
-    __shared__ uint32_t s_bcount[SAH_WAVES][NB];
-    __shared__ uint32_t s_bmin[SAH_WAVES][NB][3];
-    __shared__ uint32_t s_bmax[SAH_WAVES][NB][3];
+// One SAH split of perm[i0..i1] by the whole workgroup (every thread calls it
+// with the same arguments): the same bucketing, costs and two-pointer
+// partition, with block-wide reductions and ranks.
+WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t dep, uint32_t rt) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    if (tid == 0) { s_k = 0; s_upper = 0; s_total = 0; s_task_next = 1; }
+    float v[12];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v[c] = FLT_MAX_; v[3 + c] = FLT_LOWEST_; v[6 + c] = FLT_MAX_; v[9 + c] = FLT_LOWEST_;
+    }
+    for (uint32_t i = i0 + tid; i <= i1; i += SAH_BLOCK) {
+        uint32_t tr = L.perm[i];
+        bbox rb = tb_box(L, tr);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c] = smin(v[c], rb.mn[c]);
+            v[3 + c] = smax(v[3 + c], rb.mx[c]);
+            float cc = tb_cen(L, tr, c);
+            v[6 + c] = smin(v[6 + c], cc);
+            v[9 + c] = smax(v[9 + c], cc);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        v[c] = wave_min_f(v[c]); v[3 + c] = wave_max_f(v[3 + c]);
+        v[6 + c] = wave_min_f(v[6 + c]); v[9 + c] = wave_max_f(v[9 + c]);
+    }
+    if (lane == 0)
+#pragma unroll
+        for (int q = 0; q < 12; q++) L.red[wave][q] = v[q];
+    if (lane < NB) {
+        L.bcount[wave][lane] = 0;
+        for (int c = 0; c < 3; c++) {
+            L.bmin[wave][lane][c] = f2ord(FLT_MAX_);
+            L.bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
+        }
+    }
     __syncthreads();
-    if (tid < 128) atomicAdd(&s_k, (uint32_t)__popc(s.presence[tid]));
+    if (tid < 12) {
+        bool mx = (tid % 6) >= 3;
+        float r = L.red[0][tid];
+        for (int w = 1; w < SAH_WAVES; w++) r = mx ? smax(r, L.red[w][tid]) : smin(r, L.red[w][tid]);
+        L.fin[tid] = r;
+    }
     __syncthreads();
+    bbox bounds, cb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        bounds.mn[c] = L.fin[c]; bounds.mx[c] = L.fin[3 + c];
+        cb.mn[c] = L.fin[6 + c]; cb.mx[c] = L.fin[9 + c];
+    }
+    const int dim = box_max_extent(cb);
+    const float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
+    const float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
+    for (uint32_t i = i0 + tid; i <= i1; i += SAH_BLOCK) {
+        uint32_t tr = L.perm[i];
+        bbox rb = tb_box(L, tr);
+        uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
+        L.bucket[i] = (uint8_t)ib;
+        atomicAdd(&L.bcount[wave][ib], 1u);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            atomicMin(&L.bmin[wave][ib][c], f2ord(rb.mn[c]));
+            atomicMax(&L.bmax[wave][ib][c], f2ord(rb.mx[c]));
+        }
+    }
+    __syncthreads();
+    if (tid < NB) {
+        uint32_t cnt = L.bcount[0][tid];
+        uint32_t mn[3], mx[3];
+        for (int c = 0; c < 3; c++) { mn[c] = L.bmin[0][tid][c]; mx[c] = L.bmax[0][tid][c]; }
+        for (int w = 1; w < SAH_WAVES; w++) {
+            cnt += L.bcount[w][tid];
+            for (int c = 0; c < 3; c++) { mn[c] = min(mn[c], L.bmin[w][tid][c]); mx[c] = max(mx[c], L.bmax[w][tid][c]); }
+        }
+        L.bcount[0][tid] = cnt;
+        for (int c = 0; c < 3; c++) { L.bmin[0][tid][c] = mn[c]; L.bmax[0][tid][c] = mx[c]; }
+    }
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t cnt_le;
+        float cost = sah_cost(L, lane, bounds, &cnt_le);
+        int best = sah_best(cost);
+        uint32_t T = (uint32_t)__shfl((int)cnt_le, best);
+        if (lane == 0) { L.split[0] = (uint32_t)best; L.split[1] = T; }
+    }
+    __syncthreads();
+    const uint32_t best = L.split[0], T = L.split[1], size = i1 - i0 + 1;
+    uint32_t mid;
+    if (T == size) {
+        mid = size / 2 + i0;
+    } else {
+        // two-pointer partition: the r-th misplaced element of [i0, i0+T) swaps
+        // with the r-th misplaced element of [i0+T, i1] counted from the right
+        uint32_t run = 0;
+        for (uint32_t c = 0; c < T; c += SAH_BLOCK) {
+            uint32_t i = i0 + c + tid;
+            bool bad = (c + tid < T) && (L.bucket[i] > best);
+            uint64_t m = ballot64(bad);
+            if (lane == 0) L.wcnt[wave] = popc64(m);
+            __syncthreads();
+            uint32_t before = 0, tot = 0;
+            for (int w = 0; w < SAH_WAVES; w++) { uint32_t x = L.wcnt[w]; before += (w < wave) ? x : 0; tot += x; }
+            if (bad) L.la[i0 + run + before + popc64(m & lanemask_lt())] = (uint16_t)i;
+            run += tot;
+            __syncthreads();
+        }
+        const uint32_t rl = run;
+        run = 0;
+        const uint32_t rsz = size - T;
+        for (uint32_t c = 0; c < rsz; c += SAH_BLOCK) {
+            uint32_t i = i1 - (c + tid);
+            bool bad = (c + tid < rsz) && (L.bucket[i] <= best);
+            uint64_t m = ballot64(bad);
+            if (lane == 0) L.wcnt[wave] = popc64(m);
+            __syncthreads();
+            uint32_t before = 0, tot = 0;
+            for (int w = 0; w < SAH_WAVES; w++) { uint32_t x = L.wcnt[w]; before += (w < wave) ? x : 0; tot += x; }
+            if (bad) L.lb[i0 + run + before + popc64(m & lanemask_lt())] = (uint16_t)i;
+            run += tot;
+            __syncthreads();
+        }
+        for (uint32_t r = tid; r < rl; r += SAH_BLOCK) {
+            uint32_t a = L.la[i0 + r], b = L.lb[i0 + r];
+            uint16_t pa = L.perm[a], pb = L.perm[b];
+            L.perm[a] = pb;
+            L.perm[b] = pa;
+        }
+        mid = i0 + T;
+    }
+    if (tid == 0) {
+        sah_node(s, L, i0, i1, mid, dim, bounds, dep, rt);
+        sah_children(s, L, i0, i1, mid, dep, rt);
+    }
+    __syncthreads();
+}
+
+// One SAH split of perm[i0..i1] by one wave.
+WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t dep, uint32_t rt) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
+    float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
+    for (uint32_t i = i0 + lane; i <= i1; i += 64) {
+        uint32_t tr = L.perm[i];
+        bbox rb = tb_box(L, tr);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            bmn[c] = smin(bmn[c], rb.mn[c]);
+            bmx[c] = smax(bmx[c], rb.mx[c]);
+            float cc = tb_cen(L, tr, c);
+            cmn[c] = smin(cmn[c], cc);
+            cmx[c] = smax(cmx[c], cc);
+        }
+    }
+    bbox bounds, cb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        bounds.mn[c] = wave_min_f(bmn[c]);
+        bounds.mx[c] = wave_max_f(bmx[c]);
+        cb.mn[c] = wave_min_f(cmn[c]);
+        cb.mx[c] = wave_max_f(cmx[c]);
+    }
+    int dim = box_max_extent(cb);
+    float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
+    float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
+    // per-wave statistics in row `wave`, merged into a private row 0 copy below
+    if (lane < NB) {
+        L.bcount[wave][lane] = 0;
+        for (int c = 0; c < 3; c++) {
+            L.bmin[wave][lane][c] = f2ord(FLT_MAX_);
+            L.bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t i = i0 + lane; i <= i1; i += 64) {
+        uint32_t tr = L.perm[i];
+        bbox rb = tb_box(L, tr);
+        uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
+        L.bucket[i] = (uint8_t)ib;
+        atomicAdd(&L.bcount[wave][ib], 1u);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            atomicMin(&L.bmin[wave][ib][c], f2ord(rb.mn[c]));
+            atomicMax(&L.bmax[wave][ib][c], f2ord(rb.mx[c]));
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    float cost = INF;
+    uint32_t cnt_le = 0;
+    if (lane < NB - 1) {
+        bbox b0 = box_empty(), b1 = box_empty();
+        uint32_t c0 = 0, c1 = 0;
+        for (int j = 0; j < NB; j++) {
+            bbox bj;
+            for (int c = 0; c < 3; c++) {
+                bj.mn[c] = ord2f(L.bmin[wave][j][c]);
+                bj.mx[c] = ord2f(L.bmax[wave][j][c]);
+            }
+            if (j <= lane) { b0 = box_union(b0, bj); c0 += L.bcount[wave][j]; }
+            else { b1 = box_union(b1, bj); c1 += L.bcount[wave][j]; }
+        }
+        float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+        float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+        cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
+        cnt_le = c0;
+    }
+    int best = sah_best(cost);
+    uint32_t T = (uint32_t)__shfl((int)cnt_le, best);
+    uint32_t size = i1 - i0 + 1;
+    uint32_t mid;
+    if (T == size) {
+        mid = size / 2 + i0;
+    } else {
+        uint32_t rl = 0;
+        for (uint32_t c = 0; c * 64 < T; c++) {
+            uint32_t i = i0 + c * 64 + lane;
+            bool bad = (c * 64 + lane < T) && (L.bucket[i] > (uint32_t)best);
+            uint64_t m = ballot64(bad);
+            if (bad) L.la[i0 + rl + popc64(m & lanemask_lt())] = (uint16_t)i;
+            rl += popc64(m);
+        }
+        uint32_t rr = 0, rsz = size - T;
+        for (uint32_t c = 0; c * 64 < rsz; c++) {
+            uint32_t i = i1 - (c * 64 + lane);
+            bool bad = (c * 64 + lane < rsz) && (L.bucket[i] <= (uint32_t)best);
+            uint64_t m = ballot64(bad);
+            if (bad) L.lb[i0 + rr + popc64(m & lanemask_lt())] = (uint16_t)i;
+            rr += popc64(m);
+        }
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t r = lane; r < rl; r += 64) {
+            uint32_t a = L.la[i0 + r], b = L.lb[i0 + r];
+            uint16_t pa = L.perm[a], pb = L.perm[b];
+            L.perm[a] = pb;
+            L.perm[b] = pa;
+        }
+        __builtin_amdgcn_wave_barrier();
+        mid = i0 + T;
+    }
+    if (lane == 0) {
+        sah_node(s, L, i0, i1, mid, dim, bounds, dep, rt);
+        sah_children(s, L, i0, i1, mid, dep, rt);
+    }
+}
+
+__global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s) {
+    __shared__ SahLds L;
+    const int tid = threadIdx.x, lane = tid & 63;
+    if (tid == 0) { L.k = 0; L.upper = 0; L.total = 0; L.task_next = 1; L.nbig = 0; L.nbig_next = 0; }
+    __syncthreads();
+    if (tid < 128) atomicAdd(&L.k, (uint32_t)__popc(s.presence[tid]));
+    __syncthreads();
+    const uint32_t k = L.k;
     if (tid == 0) {
         s.task_i0[0] = 0;
-        s.task_i1[0] = s_k - 1;
-        s.task_parent[0] = NONE;
+        s.task_i1[0] = k - 1;
+        s.task_dep[0] = 0;
+        s.task_rt[0] = 0;
+        L.nbig = k >= BLOCK_TASK_MIN;
     }
-    const uint32_t k = s_k;
-    for (uint32_t i = tid; i < k; i += SAH_BLOCK) {
-        s_perm[i] = (uint16_t)i;
-        bbox rb = load_box(s.t_box + 6 * (size_t)i);
-#pragma unroll
-        for (int c = 0; c < 3; c++) s.t_cen[3 * (size_t)i + c] = (float)((double)(rb.mn[c] + rb.mx[c]) * 0.5);
-        s.u_arrive[i] = 0;
-    }
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK) L.perm[i] = (uint16_t)i;
+    for (uint32_t i = tid; i < 6 * k; i += SAH_BLOCK) L.tbox[i] = s.t_box[i];
     __threadfence_block();
     __syncthreads();
     uint32_t lb = 0, le = 1;
     while (lb < le) {
-        // ---- large ranges: one wave each
-        for (uint32_t t = lb + wave; t < le; t += SAH_WAVES) {
-            uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
-            if (i1 - i0 + 1 <= LANE_TASK_MAX) continue;
-            uint32_t parent = s.task_parent[t];
-            float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-            float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-            for (uint32_t i = i0 + lane; i <= i1; i += 64) {
-                uint32_t tr = s_perm[i];
-                bbox rb = load_box(s.t_box + 6 * (size_t)tr);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    bmn[c] = smin(bmn[c], rb.mn[c]);
-                    bmx[c] = smax(bmx[c], rb.mx[c]);
-                    float cc = c_at(s, tr, c);
-                    cmn[c] = smin(cmn[c], cc);
-                    cmx[c] = smax(cmx[c], cc);
-                }
-            }
-            bbox bounds, cb;
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                bounds.mn[c] = wave_min_f(bmn[c]);
-                bounds.mx[c] = wave_max_f(bmx[c]);
-                cb.mn[c] = wave_min_f(cmn[c]);
-                cb.mx[c] = wave_max_f(cmx[c]);
-            }
-            int dim = box_max_extent(cb);
-            float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
-            float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
-            if (lane < NB) {
-                s_bcount[wave][lane] = 0;
-                for (int c = 0; c < 3; c++) {
-                    s_bmin[wave][lane][c] = f2ord(FLT_MAX_);
-                    s_bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (uint32_t i = i0 + lane; i <= i1; i += 64) {
-                uint32_t tr = s_perm[i];
-                bbox rb = load_box(s.t_box + 6 * (size_t)tr);
-                uint32_t ib = 0;
-                if (dist > 0.0f) {
-                    float tt = (c_at(s, tr, dim) - cbmn) / dist;
-                    ib = (uint32_t)(uint8_t)(uint32_t)((float)NB * tt);
-                    if (ib == NB) ib = NB - 1;
-                }
-                s_bucket[i] = (uint8_t)ib;
-                atomicAdd(&s_bcount[wave][ib], 1u);
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    atomicMin(&s_bmin[wave][ib][c], f2ord(rb.mn[c]));
-                    atomicMax(&s_bmax[wave][ib][c], f2ord(rb.mx[c]));
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            float cost = INF;
-            uint32_t cnt_le = 0;
-            if (lane < NB - 1) {
-                bbox b0 = box_empty(), b1 = box_empty();
-                uint32_t c0 = 0, c1 = 0;
-                for (int j = 0; j < NB; j++) {
-                    bbox bj;
-                    for (int c = 0; c < 3; c++) {
-                        bj.mn[c] = ord2f(s_bmin[wave][j][c]);
-                        bj.mx[c] = ord2f(s_bmax[wave][j][c]);
-                    }
-                    if (j <= lane) { b0 = box_union(b0, bj); c0 += s_bcount[wave][j]; }
-                    else { b1 = box_union(b1, bj); c1 += s_bcount[wave][j]; }
-                }
-                float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
-                float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
-                cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
-                cnt_le = c0;
-            }
-            int best = 0;
-            float cmin = __shfl(cost, 0);
-            for (int i = 1; i < NB - 1; i++) {
-                float ci = __shfl(cost, i);
-                if (ci < cmin) { best = i; cmin = ci; }
-            }
-            uint32_t T = (uint32_t)__shfl((int)cnt_le, best);
-            uint32_t size = i1 - i0 + 1;
-            uint32_t mid;
-            if (T == size) {
-                mid = size / 2 + i0;
-            } else {
-                // two-pointer partition == swap the r-th misplaced element of the left
-                // region [i0, i0+T) with the r-th (from the right) of the right region
-                uint32_t rl = 0;
-                for (uint32_t c = 0; c * 64 < T; c++) {
-                    uint32_t i = i0 + c * 64 + lane;
-                    bool bad = (c * 64 + lane < T) && (s_bucket[i] > (uint32_t)best);
-                    uint64_t m = ballot64(bad);
-                    if (bad) s_list_a[i0 + rl + popc64(m & lanemask_lt())] = (uint16_t)i;
-                    rl += popc64(m);
-                }
-                uint32_t rr = 0, rsz = size - T;
-                for (uint32_t c = 0; c * 64 < rsz; c++) {
-                    uint32_t i = i1 - (c * 64 + lane);
-                    bool bad = (c * 64 + lane < rsz) && (s_bucket[i] <= (uint32_t)best);
-                    uint64_t m = ballot64(bad);
-                    if (bad) s_list_b[i0 + rr + popc64(m & lanemask_lt())] = (uint16_t)i;
-                    rr += popc64(m);
-                }
-                __builtin_amdgcn_wave_barrier();
-                for (uint32_t r = lane; r < rl; r += 64) {
-                    uint32_t a = s_list_a[i0 + r], b = s_list_b[i0 + r];
-                    uint16_t pa = s_perm[a], pb = s_perm[b];
-                    s_perm[a] = pb;
-                    s_perm[b] = pa;
-                }
-                __builtin_amdgcn_wave_barrier();
-                mid = i0 + T;
-            }
-            if (lane == 0) {
-                uint32_t node = atomicAdd(&s_upper, 1u);
-                uint32_t ct = atomicAdd(&s_task_next, 2u);
-                s.u_axis[node] = (uint32_t)dim;
-                link_child(s, parent, node);
-                s.task_i0[ct] = i0;
-                s.task_i1[ct] = mid - 1;
-                s.task_parent[ct] = node << 1;
-                s.task_i0[ct + 1] = mid;
-                s.task_i1[ct + 1] = i1;
-                s.task_parent[ct + 1] = (node << 1) | 1u;
+        // ---- very large ranges: the whole workgroup, one range at a time
+        if (L.nbig) {
+            for (uint32_t t = lb; t < le; t++) {
+                const uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
+                if (i1 - i0 + 1 < BLOCK_TASK_MIN) continue;
+                sah_block_split(s, L, i0, i1, s.task_dep[t], s.task_rt[t]);
             }
         }
+        // ---- large ranges: one wave each
+        for (uint32_t t = lb + (tid >> 6); t < le; t += SAH_WAVES) {
+            const uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
+            const uint32_t m = i1 - i0 + 1;
+            if (m <= LANE_TASK_MAX || m >= BLOCK_TASK_MIN) continue;
+            sah_wave_split(s, L, i0, i1, s.task_dep[t], s.task_rt[t]);
+        }
         // ---- small ranges: one lane each, to completion; the larger child of a
-        // split waits on the lane's LDS stack (offsets relative to the task)
+        // split waits on the lane's LDS stack (a, b, depth and right turns
+        // relative to the task)
         for (uint32_t t = lb + tid; t < le; t += SAH_BLOCK) {
             const uint32_t t0 = s.task_i0[t], t1 = s.task_i1[t];
             if (t1 - t0 + 1 > LANE_TASK_MAX) continue;
-            uint32_t a = t0, b = t1, p = s.task_parent[t];
+            const uint32_t d0 = s.task_dep[t], r0 = s.task_rt[t];
+            uint32_t a = t0, b = t1, dd = 0, dr = 0;
             int sp = 0;
             while (true) {
                 if (a == b) {
-                    link_child(s, p, 0x80000000u | s_perm[a]);
+                    L.la[a] = (uint16_t)(d0 + dd);
+                    L.lb[a] = (uint16_t)(r0 + dr);
                     if (sp == 0) break;
-                    uint32_t e = s_lstack[--sp][tid];
+                    uint32_t e = L.aux[(--sp) * SAH_BLOCK + tid];
                     a = t0 + (e & 31u);
                     b = t0 + ((e >> 5) & 31u);
-                    p = e >> 10;
+                    dd = (e >> 10) & 31u;
+                    dr = (e >> 15) & 31u;
                     continue;
                 }
                 int dim;
-                uint32_t mid = lane_split(s, s_perm, s_bucket, a, b, &dim);
-                uint32_t node = atomicAdd(&s_upper, 1u);
-                s.u_axis[node] = (uint32_t)dim;
-                link_child(s, p, node);
+                bbox bounds;
+                uint32_t mid = lane_split(L, a, b, &dim, &bounds);
+                sah_node(s, L, a, b, mid, dim, bounds, d0 + dd, r0 + dr);
                 uint32_t nl = mid - a, nr = b + 1 - mid;
-                uint32_t pa, pb, pp;  // the child that waits
+                uint32_t pa, pb, pr;  // the child that waits (depth dd + 1)
                 if (nl <= nr) {
-                    pa = mid; pb = b; pp = (node << 1) | 1u;
-                    b = mid - 1; p = node << 1;
+                    pa = mid; pb = b; pr = dr + 1;
+                    b = mid - 1;
                 } else {
-                    pa = a; pb = mid - 1; pp = node << 1;
-                    a = mid; p = (node << 1) | 1u;
+                    pa = a; pb = mid - 1; pr = dr;
+                    a = mid; dr = dr + 1;
                 }
-                if (sp < LANE_STACK) s_lstack[sp++][tid] = (pa - t0) | ((pb - t0) << 5) | (pp << 10);
-                else atomicOr(s.err, 4u);
+                dd++;
+                if (sp < LANE_STACK)
+                    L.aux[(sp++) * SAH_BLOCK + tid] = (pa - t0) | ((pb - t0) << 5) | (dd << 10) | (pr << 15);
+                else
+                    atomicOr(s.err, 4u);
             }
         }
         __threadfence_block();
         __syncthreads();
+        if (tid == 0) { L.nbig = L.nbig_next; L.nbig_next = 0; }
         lb = le;
-        le = s_task_next;
+        le = L.task_next;
         __syncthreads();
     }
-    const uint32_t num_upper = s_upper;
-    if (tid == 0) *s.upper_count = num_upper;
-    for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) s.u_arrive[u] = 0;
-    __threadfence_block();
-    __syncthreads();
-    // Bottom-up: sizes and bounds; the second child to arrive computes its parent.
-    for (uint32_t t = tid; t < k; t += SAH_BLOCK) {
-        uint32_t ref = s.t_parent[t];
-        while (ref != NONE) {
-            uint32_t u = ref >> 1;
-            // acquire-release at workgroup scope: every thread of the build is in
-            // this workgroup, so no device-wide (L2 write-back) fence is needed
-            if (__hip_atomic_fetch_add(&s.u_arrive[u], 1u, __ATOMIC_ACQ_REL,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP) == 0)
-                break;
-            uint32_t sz = 1;
-            bbox b[2];
-            for (int c = 0; c < 2; c++) {
-                uint32_t cr = c ? s.u_c1[u] : s.u_c0[u];
-                if (cr & 0x80000000u) {
-                    uint32_t tt = cr & 0x7FFFFFFFu;
-                    sz += s.t_nodes[tt];
-                    b[c] = load_box(s.t_box + 6 * (size_t)tt);
-                } else {
-                    sz += __hip_atomic_load(&s.u_size[cr], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    const float* q = s.u_box + 6 * (size_t)cr;
-                    for (int e = 0; e < 3; e++) {
-                        b[c].mn[e] = __hip_atomic_load(q + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        b[c].mx[e] = __hip_atomic_load(q + 3 + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                    }
-                }
-            }
-            bbox ub = box_union(b[0], b[1]);
-            s.u_size[u] = sz;
-            store_box(s.u_box + 6 * (size_t)u, ub);
-            ref = s.u_parent[u];
-        }
+    // P: exclusive prefix of treelet node counts over final positions
+    // (PER consecutive positions per thread), in aux[0..k]
+    constexpr int PER = 4096 / SAH_BLOCK;
+    const uint32_t num_upper = L.upper;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        uint32_t i = PER * tid + q;
+        v[q] = (i < k) ? s.t_nodes[L.perm[i]] : 0u;
+        sum += v[q];
     }
-    __threadfence_block();
-    __syncthreads();
-    // Top-down: preorder offset = sum over the path of 1 + (right child ? size(left) : 0).
-    const uint32_t total_items = k + num_upper;
-    for (uint32_t x = tid; x < total_items; x += SAH_BLOCK) {
-        bool isT = x < k;
-        uint32_t ref = isT ? s.t_parent[x] : s.u_parent[x - k];
-        uint32_t off = 0;
-        while (ref != NONE) {
-            uint32_t u = ref >> 1;
-            off += 1;
-            if (ref & 1u) {
-                uint32_t c0 = s.u_c0[u];
-                off += (c0 & 0x80000000u) ? s.t_nodes[c0 & 0x7FFFFFFFu] : s.u_size[c0];
-            }
-            ref = s.u_parent[u];
-        }
-        if (isT) {
-            s.t_final[x] = off;
-        } else {
-            uint32_t u = x - k;
-            uint32_t c0 = s.u_c0[u];
-            uint32_t size0 = (c0 & 0x80000000u) ? s.t_nodes[c0 & 0x7FFFFFFFu] : s.u_size[c0];
-            s.node_offset[off] = off + 1 + size0;
-            s.node_nshapes[off] = 0;
-            s.node_axis[off] = (uint8_t)s.u_axis[u];
-            store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
-        }
+    uint32_t incl = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t x = (uint32_t)__shfl_up((int)incl, d);
+        if (lane >= d) incl += x;
     }
-    uint32_t part = 0;
-    for (uint32_t t = tid; t < k; t += SAH_BLOCK) part += s.t_nodes[t];
-    for (int off = 32; off >= 1; off >>= 1) part += (uint32_t)__shfl_xor((int)part, off);
-    if (lane == 0) atomicAdd(&s_total, part);
+    if (lane == 63) L.wcnt[tid >> 6] = incl;
     __syncthreads();
+    uint32_t before = 0;
+    for (int w = 0; w < (tid >> 6); w++) before += L.wcnt[w];
+    uint32_t run = before + incl - sum;
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        uint32_t i = PER * tid + q;
+        if (i < k) L.aux[i] = run;
+        run += v[q];
+    }
     if (tid == 0) {
-        *s.n_nodes = num_upper + s_total;
+        uint32_t total = 0;
+        for (int w = 0; w < SAH_WAVES; w++) total += L.wcnt[w];
+        L.aux[k] = total;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK)
+        s.t_final[L.perm[i]] = (uint32_t)L.la[i] + (i - (uint32_t)L.lb[i]) + L.aux[i];
+    for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) {
+        const uint32_t i0 = s.u_i0[u], mid = s.u_mid[u];
+        const uint32_t off = s.u_dep[u] + (i0 - s.u_rt[u]) + L.aux[i0];
+        const uint32_t size0 = (mid - 1 - i0) + (L.aux[mid] - L.aux[i0]);
+        s.node_offset[off] = off + 1 + size0;
+        s.node_nshapes[off] = 0;
+        s.node_axis[off] = (uint8_t)s.u_axis[u];
+        store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
+    }
+    if (tid == 0) {
+        *s.upper_count = num_upper;
+        *s.n_nodes = num_upper + L.aux[k];
         *s.n_treelets = k;
     }
-    (void)n_total_shapes;
 }
 
 // ---------------------------------------------------------------------------
@@ -1191,7 +1389,7 @@ wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream
     hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, s.vals_a,
                        s.bounds, n);
     rec(8);
-    hipLaunchKernelGGL(k_upper_sah, dim3(1), dim3(SAH_BLOCK), 0, st, s, n);
+    hipLaunchKernelGGL(k_upper_sah, dim3(1), dim3(SAH_BLOCK), 0, st, s);
     rec(9);
     hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
     rec(10);

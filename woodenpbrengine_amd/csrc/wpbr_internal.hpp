@@ -49,13 +49,11 @@ struct LbvhScratch {
     // emit records, indexed by the treelet's reserved slot range (2n)
     uint32_t* r_start; uint32_t* r_n; uint32_t* r_c0; uint32_t* r_meta;
     uint32_t* r_size; uint32_t* r_off; float* r_box;
-    // upper SAH
-    uint32_t* perm; uint8_t* bucket; uint32_t* list_a; uint32_t* list_b;
-    uint32_t* task_i0; uint32_t* task_i1; uint32_t* task_parent; uint32_t* task_depth;
-    uint32_t* u_c0; uint32_t* u_c1; uint32_t* u_axis; uint32_t* u_depth; uint32_t* u_size;
-    uint32_t* u_off; float* u_box;
-    uint32_t* u_parent; uint32_t* u_arrive; uint32_t* t_parent; float* t_cen;
-    uint32_t* upper_count; uint32_t* upper_root;
+    // upper SAH: tasks (ranges of final positions) and nodes
+    uint32_t* task_i0; uint32_t* task_i1; uint32_t* task_dep; uint32_t* task_rt;
+    uint32_t* u_i0; uint32_t* u_i1; uint32_t* u_mid; uint32_t* u_dep; uint32_t* u_rt;
+    uint32_t* u_axis; float* u_box;
+    uint32_t* upper_count;
     uint32_t* n_nodes; uint32_t* n_treelets;
     // final CSceneTreeLBVH arrays (capacity 2n)
     uint32_t* node_offset; uint32_t* node_nshapes; uint8_t* node_axis; float* node_bounds;
