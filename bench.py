@@ -33,21 +33,38 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "k_trace_render_c2_pmc.txt"
+CPU_TILE_STRIDE = 4
+
+
+def measured_traffic():
+    """HBM bytes per k_trace launch of the render, from the committed rocprofv3
+    --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(scene, max_threads=16):
-    """The oracle (CPU restatement of the reference path) on every 16th film tile."""
+    """The oracle (CPU restatement of the reference path) on every 4th film tile."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
     ntx = scene.width // 32
     ntiles = ntx * (scene.height // 32)
-    tiles = np.arange(0, ntiles, 16, dtype=np.uint32)
+    tiles = np.arange(0, ntiles, CPU_TILE_STRIDE, dtype=np.uint32)
     threads = max(1, min(max_threads, os.cpu_count() or 1))
     out = po.render(scene, nthreads=threads, tiles=tiles)
     secs = float(out["counters"][7]) * 1e-9
     samples = len(tiles) * 32 * 32 * scene.spp
     return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads,
             "kind": "port",
-            "sample": "every 16th 32x32 tile (%d of %d tiles, %d samples), %.2f s of %d threads"
-                      % (len(tiles), ntiles, samples, secs, threads)}
+            "sample": "every %dth 32x32 tile of the render (%d of %d tiles, %d samples), "
+                      "%.2f s wall on %d threads" % (CPU_TILE_STRIDE, len(tiles), ntiles, samples, secs,
+                                                     threads)}
 
 
 def bench_microbench(lib, scenes, ctx, steps):
@@ -89,11 +106,24 @@ def main():
     ap.add_argument("--workload", choices=["render", "c0"], default="render")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run one plain 512 MiB device copy (reference kernel for the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE scale factor)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box here has one MI355X; the multi-GPU "
               "sharding of the render is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27
+        a = torch.ones(n, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
     from woodenpbrengine_amd import lib, scenes
 
     t0 = time.time()
@@ -110,40 +140,48 @@ def main():
     for _ in range(args.warmup):
         r.render()
     ctx.sync()
-    dev_ms, trace_ms, shade_ms = [], [], []
+    dev_ms, trace_ms, shade_ms, rays, launches = [], [], [], [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         dev_ms.append(r.render())
         st = r.stats()
         trace_ms.append(st["trace_ms"])
         shade_ms.append(st["shade_ms"])
+        rays.append(st["rays"])
+        launches.append(st["trace_launches"])
     ctx.sync()
     wall = time.perf_counter() - t_start
     ms_per_step = wall * 1e3 / args.steps
     samples = scene.width * scene.height * scene.spp
     value = samples / (ms_per_step * 1e-3) / 1e6
-    # rays traced per render: primary + compacted secondary (queue length)
-    log("render: %.2f ms/step wall, %.2f ms device, trace %.2f ms, shade %.2f ms"
-        % (ms_per_step, np.mean(dev_ms), np.mean(trace_ms), np.mean(shade_ms)))
+    # rays traced per render: primary (W*H*spp) + compacted secondary (queue lengths)
+    rays_per_step = float(np.mean(rays))
+    t_ms = float(np.mean(trace_ms))
+    n_launch = float(np.mean(launches))
+    log("render: %.2f ms/step wall, %.2f ms device, trace %.2f ms (%d launches, %.0f rays), shade %.2f ms"
+        % (ms_per_step, np.mean(dev_ms), t_ms, n_launch, rays_per_step, np.mean(shade_ms)))
     img = r.film_rgb()
     extra = {"image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
              "device_ms_per_step": float(np.mean(dev_ms)),
-             "trace_ms_per_step": float(np.mean(trace_ms)),
-             "shade_ms_per_step": float(np.mean(shade_ms))}
+             "trace_ms_per_step": t_ms,
+             "shade_ms_per_step": float(np.mean(shade_ms)),
+             "rays_per_step": rays_per_step,
+             "render_mrays_per_s": rays_per_step / (t_ms * 1e-3) / 1e6}
     r.destroy()
-    # Dominant kernel: traversal.  Rays per render = primary (W*H*spp) + secondary.
-    rays_per_step = None
+    # Dominant kernel: k_trace.  Its launches in one render (a primary and a
+    # secondary-queue launch per pass) timed with HIP events on the render stream.
+    avg_launch_ms = t_ms / n_launch
+    bytes_per_launch = TRACE_BYTES_PER_RAY * rays_per_step / n_launch
+    achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                "kernel": "k_trace in the render (36 B of ray I/O per ray; %.0f rays per launch)"
+                          % (rays_per_step / n_launch),
+                "avg_launch_us": avg_launch_ms * 1e3}
     mb = {}
     if not args.no_microbench:
         mb = bench_microbench(lib, scenes, ctx, args.steps)
         log("microbench:", json.dumps(mb))
-    roofline = None
-    if mb:
-        tr = mb["traversal"]
-        achieved = TRACE_BYTES_PER_RAY * tr["rays"] / (tr["ms"] * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                    "kernel": "k_trace (closest hit, 4194304 rays x 36 B)"}
     cpu = None
     if not args.no_cpu_baseline:
         t1 = time.time()

@@ -615,6 +615,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_shade2(RenderLaunch L) {
     const uint32_t ntx = W / 32;
     const uint32_t tx0 = (blockIdx.x % ntx) * 32, ty0 = (blockIdx.x / ntx) * 32;
     for (int k = threadIdx.x; k < 1024 * 4; k += TILE_THREADS) acc[k] = 0.0f;
+    // secondary rays traced in this pass (statistics)
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(L.ws.qcount + 2), (unsigned long long)*L.ws.qcount);
     __syncthreads();
     const uint32_t n = L.n;
     for (int r = 0; r < 4; r++) {

@@ -353,6 +353,8 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     wpbr_render_state* r = ctx->render;
     hipStream_t st = ctx->stream;
     s = hip_check(ctx, hipMemsetAsync(d_film, 0, (size_t)n * 16, st), "film clear");
+    // qcount[0]: queue length of the current pass; qcount[2..3]: secondary rays traced
+    if (!s) s = hip_check(ctx, hipMemsetAsync(r->ws.qcount, 0, 64, st), "queue counters");
     if (s) return s;
     RenderLaunch L;
     L.sv = sc->sv;
@@ -415,7 +417,9 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         return set_err(ctx, s, std::string("render launch failed: ") +
                                    hipGetErrorString(last_launch_error));
     }
-    s = ctx_finish_trace(ctx);
+    unsigned long long qtotal = 0;
+    s = hip_check(ctx, hipMemcpyAsync(&qtotal, r->ws.qcount + 2, 8, hipMemcpyDeviceToHost, st), "d2h");
+    if (!s) s = ctx_finish_trace(ctx);
     if (s) return s;
     double trace_ms = 0, shade_ms = 0, cam_ms = 0;
     for (uint32_t j = 0; j < sc->spp; j++) {
@@ -437,10 +441,10 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     r->stats[0] = tot;
     r->stats[1] = trace_ms;
     r->stats[2] = 2.0 * sc->spp;
-    r->stats[3] = 0;
+    r->stats[3] = (double)n * sc->spp + (double)qtotal;
     r->stats[4] = shade_ms;
     r->stats[5] = cam_ms;
-    r->stats[6] = 0;
+    r->stats[6] = (double)qtotal;
     return WPBR_OK;
 }
 
