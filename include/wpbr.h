@@ -114,9 +114,13 @@ wpbr_status wpbr_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lb
 wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* bvh,
                                uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
                                float* d_hit_t, float* d_bary);
-/* Traversal launch shape: block size (64..1024, multiple of 64) and LDS stack
- * depth (8..64).  0 keeps the current value. */
-wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth);
+/* Traversal launch shape: block size (64, 128, 256 or 512), LDS stack depth
+ * (8, 16, 24, 32, 48 or 64), ray-to-workgroup mapping (1: ray i to workgroup
+ * i / block; 2: the ray range split into one contiguous part per XCD, so each
+ * XCD's L2 sees one part of the film) and node width (2: binary records, 4:
+ * records of four grandchildren).  0 keeps the current value.  Results do not
+ * depend on the configuration. */
+wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth, int mapping, int width);
 
 /* ---- rendering (a14-a31) ------------------------------------------------- *
  * Scene description: host pointers, copied to the device by wpbr_scene_create.

@@ -44,6 +44,9 @@ def _case(name):
 
 
 CASES = ["tiny", "small", "medium", "icosphere", "clustered"]
+# (block, LDS stack depth, mapping, node width); stack depth 8 forces spills
+TRACE_CONFIGS = [(128, 32, 1, 2), (64, 16, 1, 4), (256, 8, 2, 4), (64, 8, 1, 2)]
+DEFAULT_TRACE = (64, 16, 1, 4)
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -87,10 +90,17 @@ def test_collide_matches_oracle(ctx, name):
     b = lib.Lbvh(ctx)
     b.build(sc)
     d_rays = ctx.array(rays)
-    d_s, d_t, _ = lib.collide(ctx, sc, b, d_rays, rays.shape[1])
-    gs, gt = d_s.numpy(), d_t.numpy()
-    np.testing.assert_array_equal(gs, os_)
-    np.testing.assert_array_equal(gt.view(np.uint32), ot.view(np.uint32))
+    try:
+        # every traversal configuration gives the same hits
+        for cfg in TRACE_CONFIGS:
+            ctx.set_trace_config(*cfg)
+            d_s, d_t, _ = lib.collide(ctx, sc, b, d_rays, rays.shape[1])
+            gs, gt = d_s.numpy(), d_t.numpy()
+            np.testing.assert_array_equal(gs, os_, err_msg="config %s" % (cfg,))
+            np.testing.assert_array_equal(gt.view(np.uint32), ot.view(np.uint32),
+                                          err_msg="config %s" % (cfg,))
+    finally:
+        ctx.set_trace_config(*DEFAULT_TRACE)
     b.destroy()
 
 

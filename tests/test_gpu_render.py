@@ -34,9 +34,14 @@ def _compare(scene, ctx):
     return frac_bad, rmse / scale, exact
 
 
-def test_render_c0_small(ctx):
+@pytest.mark.parametrize("width", [2, 4])
+def test_render_c0_small(ctx, width):
     s = scenes.c0_scene(width=64, height=64, spp=2)
-    frac_bad, rel_rmse, exact = _compare(s, ctx)
+    ctx.set_trace_config(width=width)
+    try:
+        frac_bad, rel_rmse, exact = _compare(s, ctx)
+    finally:
+        ctx.set_trace_config(width=4)
     assert frac_bad == 0.0
     assert rel_rmse < 1e-5
 

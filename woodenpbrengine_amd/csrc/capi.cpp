@@ -89,6 +89,7 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.node_axis = c.take<uint8_t>(2 * N);
     s.node_bounds = c.take<float>(12 * N);
     s.trav = c.take<float4>(4 * 3 * N);
+    s.trav4 = c.take<float4>(8 * 3 * N);
     s.prims = c.take<float4>(3 * N);
     s.slot_leaf = c.take<uint32_t>(N);
     s.sub_count = c.take<uint32_t>(1);
@@ -345,12 +346,15 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     TraceArgs a;
     a.g = to_view(geom);
     a.trav = (const float4*)b->s.trav;
+    a.trav4 = (const float4*)b->s.trav4;
+    a.width = (uint32_t)ctx->trace_width;
     a.prims = (const float4*)b->s.prims;
     a.root_box = b->s.node_bounds;
     a.root_word = b->root_word;
     a.n = n;
     a.n_dev = nullptr;
     a.stride = 0;
+    a.parts = (uint32_t)ctx->trace_parts;
     a.rays = d_rays;
     a.hit_shape = d_hit_shape;
     a.hit_t = d_hit_t;
@@ -379,8 +383,16 @@ wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const w
     return collide_impl(ctx, geom, b, n, d_rays, d_hit_shape, d_hit_t, d_bary, false);
 }
 
-wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth) {
+wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth, int mapping, int width) {
     if (!ctx) return WPBR_ERR_INVALID_ARG;
+    if (width) {
+        if (width != 2 && width != 4) return WPBR_ERR_INVALID_ARG;
+        ctx->trace_width = width;
+    }
+    if (mapping) {
+        if (mapping != 1 && mapping != 2) return WPBR_ERR_INVALID_ARG;
+        ctx->trace_parts = mapping == 2 ? ctx->num_xcd : 1;
+    }
     if (block) {
         if (block != 64 && block != 128 && block != 256 && block != 512) return WPBR_ERR_INVALID_ARG;
         ctx->trace_block = block;

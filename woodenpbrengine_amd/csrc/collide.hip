@@ -90,12 +90,22 @@ WPBR_D float sphere_hit(const GeomView& g, uint32_t id, f3 o, f3 d) {
     return (t0 <= 0) ? t1 : t0;
 }
 
-template <int BLOCK, int SD>
+template <int BLOCK, int SD, int WIDTH>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ uint32_t stack[SD * BLOCK];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t n = a.n_dev ? *a.n_dev : a.n;
-    if (i >= n) return;
+    uint32_t i;
+    if (a.parts > 1) {
+        // workgroup b runs on XCD b % parts: that XCD takes one contiguous part
+        const uint32_t x = blockIdx.x % a.parts, j = blockIdx.x / a.parts;
+        const uint32_t lo = (uint32_t)(((uint64_t)n * x) / a.parts);
+        const uint32_t hi = (uint32_t)(((uint64_t)n * (x + 1)) / a.parts);
+        i = lo + j * BLOCK + threadIdx.x;
+        if (i >= hi) return;
+    } else {
+        i = blockIdx.x * BLOCK + threadIdx.x;
+        if (i >= n) return;
+    }
     const size_t rs = a.stride ? a.stride : n;
     f3 o = mk3(a.rays[i], a.rays[rs + i], a.rays[2 * rs + i]);
     f3 d = mk3(a.rays[3 * rs + i], a.rays[4 * rs + i], a.rays[5 * rs + i]);
@@ -131,22 +141,67 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                                  : a.spill[(size_t)(sp - SD) * a.n + spill_base];
     };
     while (node != 0xFFFFFFFFu) {
-        // Records: test both children, near first by dirIsNeg[axis].
-        while (!(node & 0x80000000u)) {
-            const float4* r = a.trav + 4 * (size_t)node;
-            float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-            float lim = tBest * PRUNE_SLACK;
-            bool h0 = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
-            bool h1 = slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
-            uint32_t w0 = __float_as_uint(r3.x), w1 = __float_as_uint(r3.y);
-            uint32_t axis = __float_as_uint(r3.z);
-            bool negAxis = (axis == 0) ? nx : ((axis == 1) ? ny : nz);
-            uint32_t nearW = negAxis ? w1 : w0, farW = negAxis ? w0 : w1;
-            bool hn = negAxis ? h1 : h0, hf = negAxis ? h0 : h1;
-            if (hn && hf) { push(farW); node = nearW; }
-            else if (hn) node = nearW;
-            else if (hf) node = farW;
-            else { node = pop(); if (node == 0xFFFFFFFFu) break; }
+        if (WIDTH == 2) {
+            // Binary records: test both children, near first by dirIsNeg[axis].
+            while (!(node & 0x80000000u)) {
+                const float4* r = a.trav + 4 * (size_t)node;
+                float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+                float lim = tBest * PRUNE_SLACK;
+                bool h0 = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
+                bool h1 = slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
+                uint32_t w0 = __float_as_uint(r3.x), w1 = __float_as_uint(r3.y);
+                uint32_t axis = __float_as_uint(r3.z);
+                bool negAxis = (axis == 0) ? nx : ((axis == 1) ? ny : nz);
+                uint32_t nearW = negAxis ? w1 : w0, farW = negAxis ? w0 : w1;
+                bool hn = negAxis ? h1 : h0, hf = negAxis ? h0 : h1;
+                if (hn && hf) { push(farW); node = nearW; }
+                else if (hn) node = nearW;
+                else if (hf) node = farW;
+                else { node = pop(); if (node == 0xFFFFFFFFu) break; }
+            }
+        } else {
+            // 4-wide records (the grandchildren of a binary record): the pair of
+            // the near child first, the near grandchild first inside each pair;
+            // hits after the first are pushed in reverse visiting order.
+            while (!(node & 0x80000000u)) {
+                const float4* r = a.trav4 + 8 * (size_t)node;
+                float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6],
+                       r7 = r[7];
+                float lim = tBest * PRUNE_SLACK;
+                uint32_t w[4] = {__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z),
+                                 __float_as_uint(r6.w)};
+                bool h[4];
+                h[0] = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
+                h[1] = w[1] != 0xFFFFFFFFu &&
+                       slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
+                h[2] = slab_hit(r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, o, inv, nx, ny, nz, lim);
+                h[3] = w[3] != 0xFFFFFFFFu &&
+                       slab_hit(r4.z, r4.w, r5.x, r5.y, r5.z, r5.w, o, inv, nx, ny, nz, lim);
+                const uint32_t axes = __float_as_uint(r7.x);
+                auto neg = [&](uint32_t ax) { return ax == 0 ? nx : (ax == 1 ? ny : nz); };
+                const uint32_t pf = neg(axes & 3u) ? 1u : 0u;         // pair visited first
+                const uint32_t i0 = neg((axes >> 2) & 3u) ? 1u : 0u;  // first inside pair 0
+                const uint32_t i1 = neg((axes >> 4) & 3u) ? 1u : 0u;  // first inside pair 1
+                // visiting order: slots s0 s1 (pair pf), s2 s3 (the other pair)
+                const uint32_t f0 = pf ? 2 + i1 : i0;
+                const uint32_t s0 = f0, s1 = f0 ^ 1u;
+                const uint32_t f1 = pf ? i0 : 2 + i1;
+                const uint32_t s2 = f1, s3 = f1 ^ 1u;
+                uint32_t next = 0xFFFFFFFFu;
+                uint32_t seq[4] = {s0, s1, s2, s3};
+#pragma unroll
+                for (int q = 3; q >= 0; q--) {
+                    uint32_t sl = seq[q];
+                    bool hq = sl == 0 ? h[0] : (sl == 1 ? h[1] : (sl == 2 ? h[2] : h[3]));
+                    uint32_t wq = sl == 0 ? w[0] : (sl == 1 ? w[1] : (sl == 2 ? w[2] : w[3]));
+                    if (hq) {
+                        if (next != 0xFFFFFFFFu) push(next);
+                        next = wq;
+                    }
+                }
+                if (next != 0xFFFFFFFFu) node = next;
+                else { node = pop(); if (node == 0xFFFFFFFFu) break; }
+            }
         }
         // Primitive ranges.
         while (node != 0xFFFFFFFFu && (node & 0x80000000u)) {
@@ -185,26 +240,34 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, int WIDTH>
 static void launch_sd(const TraceArgs& a, int sd, hipStream_t st) {
-    dim3 grid((a.n + BLOCK - 1) / BLOCK);
+    const uint32_t P = a.parts > 1 ? a.parts : 1;
+    const uint32_t per_part = (a.n + P - 1) / P;
+    dim3 grid(P * ((per_part + BLOCK - 1) / BLOCK));
     switch (sd) {
-        case 8: hipLaunchKernelGGL((k_trace<BLOCK, 8>), grid, dim3(BLOCK), 0, st, a); break;
-        case 16: hipLaunchKernelGGL((k_trace<BLOCK, 16>), grid, dim3(BLOCK), 0, st, a); break;
-        case 24: hipLaunchKernelGGL((k_trace<BLOCK, 24>), grid, dim3(BLOCK), 0, st, a); break;
-        case 32: hipLaunchKernelGGL((k_trace<BLOCK, 32>), grid, dim3(BLOCK), 0, st, a); break;
-        case 48: hipLaunchKernelGGL((k_trace<BLOCK, 48>), grid, dim3(BLOCK), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_trace<BLOCK, 64>), grid, dim3(BLOCK), 0, st, a); break;
+        case 8: hipLaunchKernelGGL((k_trace<BLOCK, 8, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
+        case 16: hipLaunchKernelGGL((k_trace<BLOCK, 16, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
+        case 24: hipLaunchKernelGGL((k_trace<BLOCK, 24, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
+        case 32: hipLaunchKernelGGL((k_trace<BLOCK, 32, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
+        case 48: hipLaunchKernelGGL((k_trace<BLOCK, 48, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
+        default: hipLaunchKernelGGL((k_trace<BLOCK, 64, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
     }
+}
+
+template <int BLOCK>
+static void launch_w(const TraceArgs& a, int sd, hipStream_t st) {
+    if (a.width == 4) launch_sd<BLOCK, 4>(a, sd, st);
+    else launch_sd<BLOCK, 2>(a, sd, st);
 }
 
 wpbr_status trace_launch(const TraceArgs& a, int block, int sd, hipStream_t st) {
     if (a.n == 0) return WPBR_OK;
     switch (block) {
-        case 64: launch_sd<64>(a, sd, st); break;
-        case 128: launch_sd<128>(a, sd, st); break;
-        case 512: launch_sd<512>(a, sd, st); break;
-        default: launch_sd<256>(a, sd, st); break;
+        case 64: launch_w<64>(a, sd, st); break;
+        case 128: launch_w<128>(a, sd, st); break;
+        case 512: launch_w<512>(a, sd, st); break;
+        default: launch_w<256>(a, sd, st); break;
     }
     return launch_status();
 }

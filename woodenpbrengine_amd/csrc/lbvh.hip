@@ -1321,6 +1321,64 @@ __global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32
     }
 }
 
+// 4-wide records: record r holds the grandchildren of binary record r (a child
+// that is a primitive range stays one child).  Slots 0,1 come from child 0 and
+// 2,3 from child 1; w[3].x packs the axes (bits 0-1 record r, 2-3 child 0,
+// 4-5 child 1).  Visiting the pair of the near child first and, inside each
+// pair, its near child first is the binary depth-first order.  Empty slots have
+// the word 0xFFFFFFFF.
+// Only written binary records are read: reference leaves of at most 4 shapes
+// have no record, and sub-tree records allocated but not used by a leaf are
+// never written, so a child word read from such a record is not followed
+// unless it indexes an existing record (those nodes are unreachable anyway).
+__global__ void k_bvh4(LbvhScratch s) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t nn = *s.n_nodes, total = nn + *s.sub_count;
+    if (r >= total) return;
+    if (r < nn) {
+        const uint32_t ns = s.node_nshapes[r];
+        if (ns > 0 && ns <= RANGE_MAX) return;
+    }
+    const float4* bin = reinterpret_cast<const float4*>(s.trav);
+    float4 q0 = bin[4 * (size_t)r], q1 = bin[4 * (size_t)r + 1], q2 = bin[4 * (size_t)r + 2],
+           q3 = bin[4 * (size_t)r + 3];
+    // binary record r as boxes + words
+    float cb[2][6] = {{q0.x, q0.y, q0.z, q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w}};
+    uint32_t cw[2] = {__float_as_uint(q3.x), __float_as_uint(q3.y)};
+    uint32_t axes = __float_as_uint(q3.z) & 3u;
+    float box[4][6];
+    uint32_t w[4];
+    for (int c = 0; c < 2; c++) {
+        if (!(cw[c] & 0x80000000u) && cw[c] >= total) cw[c] = 0xFFFFFFFFu;
+        if (!(cw[c] & 0x80000000u)) {
+            const float4* g = bin + 4 * (size_t)cw[c];
+            float4 g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+            box[2 * c][0] = g0.x; box[2 * c][1] = g0.y; box[2 * c][2] = g0.z;
+            box[2 * c][3] = g0.w; box[2 * c][4] = g1.x; box[2 * c][5] = g1.y;
+            box[2 * c + 1][0] = g1.z; box[2 * c + 1][1] = g1.w; box[2 * c + 1][2] = g2.x;
+            box[2 * c + 1][3] = g2.y; box[2 * c + 1][4] = g2.z; box[2 * c + 1][5] = g2.w;
+            w[2 * c] = __float_as_uint(g3.x);
+            w[2 * c + 1] = __float_as_uint(g3.y);
+            axes |= (__float_as_uint(g3.z) & 3u) << (2 + 2 * c);
+        } else {
+            for (int k = 0; k < 6; k++) { box[2 * c][k] = cb[c][k]; box[2 * c + 1][k] = 0.0f; }
+            w[2 * c] = cw[c];
+            w[2 * c + 1] = 0xFFFFFFFFu;
+        }
+    }
+    float4* o = reinterpret_cast<float4*>(s.trav4) + 8 * (size_t)r;
+    for (int p = 0; p < 2; p++) {
+        const float* a = box[2 * p];
+        const float* b = box[2 * p + 1];
+        o[3 * p] = make_float4(a[0], a[1], a[2], a[3]);
+        o[3 * p + 1] = make_float4(a[4], a[5], b[0], b[1]);
+        o[3 * p + 2] = make_float4(b[2], b[3], b[4], b[5]);
+    }
+    o[6] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
+                       __uint_as_float(w[3]));
+    o[7] = make_float4(__uint_as_float(axes), 0.0f, 0.0f, 0.0f);
+}
+
 // Primitive records at their slots: v0.xyz + shape id, v1.xyz + slot,
 // v2.xyz + first slot of the reference leaf.  Spheres keep only the words.
 __global__ void k_prims(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index, uint32_t n) {
@@ -1397,6 +1455,8 @@ wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream
     hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, s.keys_a, s.vals_a);
     hipLaunchKernelGGL(k_subtree_big, dim3(64), dim3(256), 0, st, s, s.vals_a);
     hipLaunchKernelGGL(k_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, s.vals_a, n);
+    const uint32_t max_records = 3 * std::max<uint32_t>(n, 1);
+    hipLaunchKernelGGL(k_bvh4, dim3((max_records + 255) / 256), dim3(256), 0, st, s);
     rec(11);
     return launch_status();
 }

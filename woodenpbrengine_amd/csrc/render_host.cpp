@@ -368,6 +368,8 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     TraceArgs ta;
     ta.g = sc->sv.g;
     ta.trav = (const float4*)b->s.trav;
+    ta.trav4 = (const float4*)b->s.trav4;
+    ta.width = (uint32_t)ctx->trace_width;
     ta.prims = (const float4*)b->s.prims;
     ta.root_box = b->s.node_bounds;
     ta.root_word = b->root_word;
@@ -376,6 +378,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     ta.err = ctx->err_flag;
     ta.n_dev = nullptr;
     ta.stride = 0;
+    ta.parts = (uint32_t)ctx->trace_parts;
     // events: start, then per pass 5 boundaries (camera | trace | shade1 | trace | shade2)
     const size_t nev = 2 + 5 * (size_t)sc->spp;
     while (r->evs.size() < nev) {

@@ -14,8 +14,11 @@ struct wpbr_ctx {
     float last_ms = 0.0f;
     std::string last_error;
     // traversal
-    int trace_block = 128;
-    int trace_stack = 32;
+    int trace_block = 64;
+    int trace_stack = 16;
+    int num_xcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin over them
+    int trace_parts = 1;      // ray range parts (1, or num_xcd for the XCD-aware mapping)
+    int trace_width = 4;      // 2: binary records, 4: 4-wide records
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
     uint32_t spill_depth = 0;

@@ -58,7 +58,8 @@ struct LbvhScratch {
     // final CSceneTreeLBVH arrays (capacity 2n)
     uint32_t* node_offset; uint32_t* node_nshapes; uint8_t* node_axis; float* node_bounds;
     // traversal layout
-    void* trav;           // 64 B per node
+    void* trav;           // 64 B per binary record
+    void* trav4;          // 128 B per record: the record's grandchildren (4-wide)
     void* prims;          // 48 B per shape, at its slot
     uint32_t* slot_leaf;  // per slot: first slot of its reference leaf
     uint32_t* sub_count;  // sub-tree records appended after the reference nodes
@@ -72,12 +73,15 @@ wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream
 struct TraceArgs {
     GeomView g;
     const float4* trav;
+    const float4* trav4;
+    uint32_t width;       // 2: binary records, 4: 4-wide records
     const float4* prims;
     const float* root_box;
     uint32_t root_word;   // first traversal word: record 0 or a primitive range
     uint32_t n;           // ray count (upper bound when n_dev is set)
     const uint32_t* n_dev;  // optional device-side ray count (queue launches)
     uint32_t stride;      // SoA stride of rays (0: n)
+    uint32_t parts;       // 1: linear; P: part x of the ray range to workgroups b with b % P == x
     const float* rays;
     uint32_t* hit_shape;
     float* hit_t;

@@ -86,7 +86,7 @@ def load():
         "wpbr_lbvh_kernel_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
-        "wpbr_set_trace_config": (i32, [vp, C.c_int, C.c_int]),
+        "wpbr_set_trace_config": (i32, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
         "wpbr_scene_create": (i32, [vp, C.POINTER(SceneDesc), C.POINTER(vp)]),
         "wpbr_scene_destroy": (None, [vp]),
         "wpbr_scene_rebuild": (i32, [vp, vp]),
@@ -170,8 +170,8 @@ class Context:
     def last_ms(self):
         return float(self.lib.wpbr_last_time_ms(self.h))
 
-    def set_trace_config(self, block=0, stack_depth=0):
-        self.check(self.lib.wpbr_set_trace_config(self.h, block, stack_depth))
+    def set_trace_config(self, block=0, stack_depth=0, mapping=0, width=0):
+        self.check(self.lib.wpbr_set_trace_config(self.h, block, stack_depth, mapping, width))
 
     def close(self):
         if self.h:
