@@ -79,6 +79,8 @@ def bench_microbench(lib, scenes, ctx, steps):
     nodes = bvh.counts()["nodes"]
     d_rays = ctx.array(rays)
     n = rays.shape[1]
+    saved = ctx.trace_config()
+    tuned = lib.autotune_collide(ctx, sc, bvh, d_rays, n)
     d_s, d_t, _ = lib.collide(ctx, sc, bvh, d_rays, n)
     trace_ms = []
     for _ in range(max(steps, 3)):
@@ -92,8 +94,9 @@ def bench_microbench(lib, scenes, ctx, steps):
                        "mprims_per_s": len(tris) / b / 1e3,
                        "hbm_frac": lbvh_bytes / (b * 1e-3) / 1e9 / HBM_PEAK_GBS,
                        "kernel_ms": [round(x, 4) for x in ktimes]},
-        "traversal": {"rays": n, "ms": t, "mrays_per_s": n / t / 1e3},
+        "traversal": {"rays": n, "ms": t, "mrays_per_s": n / t / 1e3, "config": tuned},
     }
+    ctx.set_trace_config(**saved)
     bvh.destroy()
     return res
 
@@ -137,6 +140,8 @@ def main():
     ctx = lib.Context(0)
     r = lib.Renderer(ctx, scene)
     log("scene uploaded + LBVH built")
+    tuned = r.autotune()
+    log("traversal configuration:", tuned)
     for _ in range(args.warmup):
         r.render()
     ctx.sync()
@@ -166,6 +171,7 @@ def main():
              "trace_ms_per_step": t_ms,
              "shade_ms_per_step": float(np.mean(shade_ms)),
              "rays_per_step": rays_per_step,
+             "trace_config": tuned,
              "render_mrays_per_s": rays_per_step / (t_ms * 1e-3) / 1e6}
     r.destroy()
     # Dominant kernel: k_trace.  Its launches in one render (a primary and a

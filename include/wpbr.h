@@ -114,13 +114,34 @@ wpbr_status wpbr_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lb
 wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* bvh,
                                uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
                                float* d_hit_t, float* d_bary);
-/* Traversal launch shape: block size (64, 128, 256 or 512), LDS stack depth
- * (8, 16, 24, 32, 48 or 64), ray-to-workgroup mapping (1: ray i to workgroup
- * i / block; 2: the ray range split into one contiguous part per XCD, so each
- * XCD's L2 sees one part of the film) and node width (2: binary records, 4:
- * records of four grandchildren).  0 keeps the current value.  Results do not
- * depend on the configuration. */
-wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth, int mapping, int width);
+/* Traversal launch configuration (no counterpart in the reference, whose
+ * traversal is a CPU job).  Results do not depend on it.
+ *   block        workgroup size: 64, 128, 256 or 512
+ *   stack_depth  LDS stack entries per lane: 8, 16, 24, 32, 48 or 64 (deeper
+ *                entries spill to global memory, 256 in all)
+ *   mapping      1: ray i to workgroup i / block; 2: one contiguous part of the
+ *                rays per XCD; 3: persistent lanes that fetch rays dynamically
+ *   width        2: binary records; 4: records of four grandchildren
+ *   refill       mapping 3: idle lanes of a wave that trigger a fetch (1..64)
+ *   chunk        mapping 3: rays a wave claims per atomic (64..65536)
+ * wpbr_set_trace_config: a 0 field keeps the current value. */
+typedef struct {
+    int32_t block;
+    int32_t stack_depth;
+    int32_t mapping;
+    int32_t width;
+    int32_t refill;
+    int32_t chunk;
+} wpbr_trace_config;
+wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, const wpbr_trace_config* cfg);
+wpbr_status wpbr_get_trace_config(wpbr_ctx* ctx, wpbr_trace_config* cfg);
+/* Autotuner: times wpbr_collide over the given rays for each candidate launch
+ * configuration (block, stack depth, mapping, width, refill), keeps the
+ * fastest in the context and returns it in *best (optional).  The hit buffers
+ * receive the (configuration-independent) result. */
+wpbr_status wpbr_autotune_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* bvh,
+                                  uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
+                                  float* d_hit_t, wpbr_trace_config* best);
 
 /* ---- rendering (a14-a31) ------------------------------------------------- *
  * Scene description: host pointers, copied to the device by wpbr_scene_create.
@@ -177,6 +198,10 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* 
  * launches), [2] number of trace launches, [3] rays traced, [4] shade ms,
  * [5] camera ms, [6] secondary rays. Returns the number written. */
 int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap);
+/* Autotuner for the render: renders the scene once per candidate traversal
+ * configuration, keeps the one with the least traversal time in the context
+ * and returns it in *best (optional).  d_film receives the last render. */
+wpbr_status wpbr_autotune_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, wpbr_trace_config* best);
 
 #ifdef __cplusplus
 }

@@ -41,4 +41,4 @@ def test_status_strings_without_gpu():
 def test_invalid_arguments_rejected_without_gpu():
     so = lib.load()
     assert so.wpbr_open(0, None) == 1
-    assert so.wpbr_set_trace_config(None, 128, 32, 2, 4) == 1
+    assert so.wpbr_set_trace_config(None, None) == 1

@@ -44,9 +44,12 @@ def _case(name):
 
 
 CASES = ["tiny", "small", "medium", "icosphere", "clustered"]
-# (block, LDS stack depth, mapping, node width); stack depth 8 forces spills
-TRACE_CONFIGS = [(128, 32, 1, 2), (64, 16, 1, 4), (256, 8, 2, 4), (64, 8, 1, 2)]
-DEFAULT_TRACE = (64, 16, 1, 4)
+# (block, LDS stack depth, mapping, node width, refill, chunk); stack depth 8
+# forces spills; mapping 3 is the persistent dynamic-fetch kernel
+TRACE_CONFIGS = [(128, 32, 1, 2, 16, 256), (64, 16, 1, 4, 16, 256), (256, 8, 2, 4, 16, 256),
+                 (64, 8, 1, 2, 16, 256), (64, 16, 3, 4, 16, 256), (128, 8, 3, 2, 1, 64),
+                 (64, 16, 3, 4, 64, 4096)]
+DEFAULT_TRACE = (64, 16, 1, 4, 16, 256)
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -115,4 +118,27 @@ def test_rebuild_reuses_buffers(ctx):
         ob = po.build_scene_bvh(po.OScene(sx, sr, tris))
         np.testing.assert_array_equal(b.download("node_offset"), ob.get("node_offset"))
         np.testing.assert_array_equal(b.download("node_bounds"), ob.get("node_bounds"))
+    b.destroy()
+
+
+def test_autotune_collide(ctx):
+    """The autotuner keeps one of its candidates and the hits stay the oracle's."""
+    sx, sr, tris = _case("medium")
+    osc = po.OScene(sx, sr, tris)
+    ob = po.build_scene_bvh(osc)
+    rays = scenes.random_rays(20000, 23)
+    os_, ot, _, ovf = po.collide(osc, ob, rays)
+    sc = lib.Scene(ctx, sx, sr, tris)
+    b = lib.Lbvh(ctx)
+    b.build(sc)
+    d_rays = ctx.array(rays)
+    try:
+        cfg = lib.autotune_collide(ctx, sc, b, d_rays, rays.shape[1])
+        assert cfg == ctx.trace_config()
+        assert cfg["mapping"] in (1, 3) and cfg["width"] in (2, 4) and cfg["block"] in (64, 128)
+        d_s, d_t, _ = lib.collide(ctx, sc, b, d_rays, rays.shape[1])
+        np.testing.assert_array_equal(d_s.numpy(), os_)
+        np.testing.assert_array_equal(d_t.numpy().view(np.uint32), ot.view(np.uint32))
+    finally:
+        ctx.set_trace_config(*DEFAULT_TRACE)
     b.destroy()

@@ -34,14 +34,15 @@ def _compare(scene, ctx):
     return frac_bad, rmse / scale, exact
 
 
-@pytest.mark.parametrize("width", [2, 4])
-def test_render_c0_small(ctx, width):
+@pytest.mark.parametrize("width,mapping", [(2, 1), (4, 1), (4, 3)])
+def test_render_c0_small(ctx, width, mapping):
     s = scenes.c0_scene(width=64, height=64, spp=2)
-    ctx.set_trace_config(width=width)
+    saved = ctx.trace_config()
+    ctx.set_trace_config(width=width, mapping=mapping)
     try:
         frac_bad, rel_rmse, exact = _compare(s, ctx)
     finally:
-        ctx.set_trace_config(width=4)
+        ctx.set_trace_config(**saved)
     assert frac_bad == 0.0
     assert rel_rmse < 1e-5
 

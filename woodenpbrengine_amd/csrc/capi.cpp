@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -347,14 +348,12 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     a.g = to_view(geom);
     a.trav = (const float4*)b->s.trav;
     a.trav4 = (const float4*)b->s.trav4;
-    a.width = (uint32_t)ctx->trace_width;
     a.prims = (const float4*)b->s.prims;
     a.root_box = b->s.node_bounds;
     a.root_word = b->root_word;
     a.n = n;
     a.n_dev = nullptr;
     a.stride = 0;
-    a.parts = (uint32_t)ctx->trace_parts;
     a.rays = d_rays;
     a.hit_shape = d_hit_shape;
     a.hit_t = d_hit_t;
@@ -362,8 +361,9 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     a.spill = ctx->spill;
     a.spill_depth = ctx->spill_depth;
     a.err = ctx->err_flag;
+    const uint32_t dyn_blocks = ctx_trace_shape(ctx, a);
     hipEventRecord(ctx->ev0, ctx->stream);
-    s = trace_launch(a, ctx->trace_block, ctx->trace_stack, ctx->stream);
+    s = trace_launch(a, ctx->trace_block, ctx->trace_stack, ctx->stream, dyn_blocks);
     hipEventRecord(ctx->ev1, ctx->stream);
     if (s) return set_err(ctx, s, "trace launch failed");
     if (!wait) return WPBR_OK;
@@ -383,26 +383,33 @@ wpbr_status wpbr_collide_async(wpbr_ctx* ctx, const wpbr_geometry* geom, const w
     return collide_impl(ctx, geom, b, n, d_rays, d_hit_shape, d_hit_t, d_bary, false);
 }
 
-wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth, int mapping, int width) {
-    if (!ctx) return WPBR_ERR_INVALID_ARG;
-    if (width) {
-        if (width != 2 && width != 4) return WPBR_ERR_INVALID_ARG;
-        ctx->trace_width = width;
-    }
-    if (mapping) {
-        if (mapping != 1 && mapping != 2) return WPBR_ERR_INVALID_ARG;
-        ctx->trace_parts = mapping == 2 ? ctx->num_xcd : 1;
-    }
-    if (block) {
-        if (block != 64 && block != 128 && block != 256 && block != 512) return WPBR_ERR_INVALID_ARG;
-        ctx->trace_block = block;
-    }
-    if (stack_depth) {
-        if (stack_depth != 8 && stack_depth != 16 && stack_depth != 24 && stack_depth != 32 &&
-            stack_depth != 48 && stack_depth != 64)
-            return WPBR_ERR_INVALID_ARG;
-        ctx->trace_stack = stack_depth;
-    }
+wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, const wpbr_trace_config* cfg) {
+    if (!ctx || !cfg) return WPBR_ERR_INVALID_ARG;
+    const int sd = cfg->stack_depth;
+    if ((cfg->block && cfg->block != 64 && cfg->block != 128 && cfg->block != 256 && cfg->block != 512) ||
+        (sd && sd != 8 && sd != 16 && sd != 24 && sd != 32 && sd != 48 && sd != 64) ||
+        (cfg->mapping && (cfg->mapping < 1 || cfg->mapping > 3)) ||
+        (cfg->width && cfg->width != 2 && cfg->width != 4) ||
+        (cfg->refill && (cfg->refill < 1 || cfg->refill > 64)) ||
+        (cfg->chunk && (cfg->chunk < 64 || cfg->chunk > 65536)))
+        return WPBR_ERR_INVALID_ARG;
+    if (cfg->chunk) ctx->trace_chunk = cfg->chunk;
+    if (cfg->block) ctx->trace_block = cfg->block;
+    if (sd) ctx->trace_stack = sd;
+    if (cfg->mapping) ctx->trace_mapping = cfg->mapping;
+    if (cfg->width) ctx->trace_width = cfg->width;
+    if (cfg->refill) ctx->trace_refill = cfg->refill;
+    return WPBR_OK;
+}
+
+wpbr_status wpbr_get_trace_config(wpbr_ctx* ctx, wpbr_trace_config* cfg) {
+    if (!ctx || !cfg) return WPBR_ERR_INVALID_ARG;
+    cfg->block = ctx->trace_block;
+    cfg->stack_depth = ctx->trace_stack;
+    cfg->mapping = ctx->trace_mapping;
+    cfg->width = ctx->trace_width;
+    cfg->refill = ctx->trace_refill;
+    cfg->chunk = ctx->trace_chunk;
     return WPBR_OK;
 }
 
@@ -410,10 +417,23 @@ wpbr_status wpbr_set_trace_config(wpbr_ctx* ctx, int block, int stack_depth, int
 
 namespace wpbr {
 
+// Persistent grid: 32 waves per CU (8 per SIMD).
+static uint32_t dyn_threads(const wpbr_ctx* ctx) { return (uint32_t)ctx->num_cus * 32u * 64u; }
+
+uint32_t ctx_trace_shape(const wpbr_ctx* ctx, TraceArgs& a) {
+    a.width = (uint32_t)ctx->trace_width;
+    a.parts = ctx->trace_mapping == 2 ? (uint32_t)ctx->num_xcd : 1u;
+    a.refill = (uint32_t)ctx->trace_refill;
+    a.chunk = (uint32_t)ctx->trace_chunk;
+    a.claim = ctx->err_flag + 2;
+    return ctx->trace_mapping == 3 ? dyn_threads(ctx) / (uint32_t)ctx->trace_block : 0u;
+}
+
 wpbr_status ctx_ensure_trace(wpbr_ctx* ctx, uint32_t n) {
-    // Spill capacity so that LDS + spill = 256 entries, the reference's stack.
+    // Spill capacity so that LDS + spill = 256 entries, the reference's stack:
+    // one slot per ray (static kernel) or per persistent lane.
     uint32_t depth = 256 - (uint32_t)ctx->trace_stack;
-    size_t need = (size_t)n * depth;
+    size_t need = (size_t)std::max(n, dyn_threads(ctx)) * depth;
     if (!ctx->err_flag) {
         wpbr_status s = hip_check(ctx, hipMalloc(&ctx->err_flag, 16), "hipMalloc");
         if (s) return s;
@@ -446,3 +466,64 @@ wpbr_status ctx_finish_trace(wpbr_ctx* ctx) {
 }
 
 }  // namespace wpbr
+
+// ---- autotuner -------------------------------------------------------------
+namespace wpbr {
+
+std::vector<wpbr_trace_config> trace_candidates(bool small) {
+    std::vector<wpbr_trace_config> c;
+    const int blocks[2] = {64, 128};
+    const int sds[3] = {8, 16, 24};
+    for (int width : {4, 2})
+        for (int b : blocks)
+            for (int sd : sds) {
+                if (small && sd != 16) continue;
+                c.push_back({b, sd, 1, width, 16, 64});
+                for (int rf : {8, 16}) {
+                    if (small && rf != 16) continue;
+                    c.push_back({b, sd, 3, width, rf, 64});
+                }
+            }
+    return c;
+}
+
+wpbr_status autotune(wpbr_ctx* ctx, const std::vector<wpbr_trace_config>& cands,
+                     const std::function<wpbr_status(float*)>& run, wpbr_trace_config* best) {
+    wpbr_trace_config saved;
+    wpbr_get_trace_config(ctx, &saved);
+    float best_ms = 3.4e38f;
+    wpbr_trace_config best_cfg = saved;
+    for (const wpbr_trace_config& cfg : cands) {
+        wpbr_status s = wpbr_set_trace_config(ctx, &cfg);
+        float ms = 0, t = 0;
+        if (!s) s = run(&t);  // warm-up (allocations, caches)
+        for (int r = 0; r < 2 && !s; r++) {
+            s = run(&t);
+            ms += t;
+        }
+        if (s) {
+            wpbr_set_trace_config(ctx, &saved);
+            return s;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_cfg = cfg;
+        }
+    }
+    wpbr_set_trace_config(ctx, &best_cfg);
+    if (best) *best = best_cfg;
+    return WPBR_OK;
+}
+
+}  // namespace wpbr
+
+extern "C" wpbr_status wpbr_autotune_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_lbvh* b,
+                                             uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
+                                             float* d_hit_t, wpbr_trace_config* best) {
+    if (!ctx || !geom || !b) return WPBR_ERR_INVALID_ARG;
+    return autotune(ctx, trace_candidates(false), [&](float* ms) {
+        wpbr_status s = collide_impl(ctx, geom, b, n, d_rays, d_hit_shape, d_hit_t, nullptr, true);
+        *ms = ctx->last_ms;
+        return s;
+    }, best);
+}

@@ -90,6 +90,164 @@ WPBR_D float sphere_hit(const GeomView& g, uint32_t id, f3 o, f3 d) {
     return (t0 <= 0) ? t1 : t0;
 }
 
+// Per-lane traversal state of one ray.
+struct RayState {
+    f3 o, d, inv;
+    bool nx, ny, nz;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    float tBest;
+    uint32_t best, bestSlot, bestLeaf;
+    float bb0, bb1, bb2;
+    uint32_t node;  // next record or primitive range; NODE_DONE when finished
+    uint32_t sp;
+};
+constexpr uint32_t NODE_DONE = 0xFFFFFFFFu;
+
+// Lane stack: SD entries in LDS (lane-interleaved), the rest in a global spill
+// array (entry e of slot `base` at spill[e * stride + base]).
+template <int BLOCK, int SD>
+struct LaneStack {
+    uint32_t* lds;
+    uint32_t* spill;
+    uint32_t depth;
+    size_t stride;
+    uint32_t base;
+    uint32_t* err;
+    WPBR_D void push(uint32_t& sp, uint32_t v) {
+        if (sp < (uint32_t)SD) lds[sp * BLOCK + threadIdx.x] = v;
+        else if (sp - SD < depth) spill[(size_t)(sp - SD) * stride + base] = v;
+        else { atomicOr(err, 1u); return; }
+        sp++;
+    }
+    WPBR_D uint32_t pop(uint32_t& sp) {
+        if (sp == 0) return NODE_DONE;
+        sp--;
+        return sp < (uint32_t)SD ? lds[sp * BLOCK + threadIdx.x] : spill[(size_t)(sp - SD) * stride + base];
+    }
+};
+
+WPBR_D void ray_begin(RayState& R, const TraceArgs& a, uint32_t i, size_t rs) {
+    R.o = mk3(a.rays[i], a.rays[rs + i], a.rays[2 * rs + i]);
+    R.d = mk3(a.rays[3 * rs + i], a.rays[4 * rs + i], a.rays[5 * rs + i]);
+    R.inv = mk3(1.0f / R.d.x, 1.0f / R.d.y, 1.0f / R.d.z);
+    R.nx = R.inv.x < 0; R.ny = R.inv.y < 0; R.nz = R.inv.z < 0;
+    // Triangle setup shared by every candidate (pbrt-v3 MaxDimension, shear).
+    float adx = fabsf(R.d.x), ady = fabsf(R.d.y), adz = fabsf(R.d.z);
+    R.kz = (adx > ady) ? ((adx > adz) ? 0 : 2) : ((ady > adz) ? 1 : 2);
+    R.kx = R.kz + 1; if (R.kx == 3) R.kx = 0;
+    R.ky = R.kx + 1; if (R.ky == 3) R.ky = 0;
+    float dkx = comp(R.d, R.kx), dky = comp(R.d, R.ky), dkz = comp(R.d, R.kz);
+    R.sx = -dkx / dkz; R.sy = -dky / dkz; R.sz = 1.0f / dkz;
+    R.tBest = INF;
+    R.best = R.bestSlot = R.bestLeaf = 0xFFFFFFFFu;
+    R.bb0 = R.bb1 = R.bb2 = 0.0f;
+    const float* rb = a.root_box;
+    R.node = slab_hit(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], R.o, R.inv, R.nx, R.ny, R.nz, INF)
+                 ? a.root_word : NODE_DONE;
+    R.sp = 0;
+}
+
+WPBR_D void ray_end(const RayState& R, const TraceArgs& a, uint32_t i, size_t rs) {
+    a.hit_shape[i] = R.best;
+    a.hit_t[i] = R.tBest;
+    if (a.bary) {
+        a.bary[i] = R.bb0;
+        a.bary[rs + i] = R.bb1;
+        a.bary[2 * rs + i] = R.bb2;
+    }
+}
+
+// One step of the while-while loop: records until a primitive range (or the
+// end), then primitive ranges until a record (or the end).
+template <int BLOCK, int SD, int WIDTH>
+WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
+    uint32_t node = R.node;
+    const f3 o = R.o, inv = R.inv;
+    const bool nx = R.nx, ny = R.ny, nz = R.nz;
+    if (WIDTH == 2) {
+        // Binary records: test both children, near first by dirIsNeg[axis].
+        while (node != NODE_DONE && !(node & 0x80000000u)) {
+            const float4* r = a.trav + 4 * (size_t)node;
+            float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+            float lim = R.tBest * PRUNE_SLACK;
+            bool h0 = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
+            bool h1 = slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
+            uint32_t w0 = __float_as_uint(r3.x), w1 = __float_as_uint(r3.y);
+            uint32_t axis = __float_as_uint(r3.z);
+            bool negAxis = (axis == 0) ? nx : ((axis == 1) ? ny : nz);
+            uint32_t nearW = negAxis ? w1 : w0, farW = negAxis ? w0 : w1;
+            bool hn = negAxis ? h1 : h0, hf = negAxis ? h0 : h1;
+            if (hn && hf) { S.push(R.sp, farW); node = nearW; }
+            else if (hn) node = nearW;
+            else if (hf) node = farW;
+            else node = S.pop(R.sp);
+        }
+    } else {
+        // 4-wide records (the grandchildren of a binary record): the pair of the
+        // near child first, the near grandchild first inside each pair; hits
+        // after the first are pushed in reverse visiting order.
+        while (node != NODE_DONE && !(node & 0x80000000u)) {
+            const float4* r = a.trav4 + 8 * (size_t)node;
+            float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
+            float lim = R.tBest * PRUNE_SLACK;
+            uint32_t w[4] = {__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z),
+                             __float_as_uint(r6.w)};
+            bool h[4];
+            h[0] = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
+            h[1] = w[1] != 0xFFFFFFFFu && slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
+            h[2] = slab_hit(r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, o, inv, nx, ny, nz, lim);
+            h[3] = w[3] != 0xFFFFFFFFu && slab_hit(r4.z, r4.w, r5.x, r5.y, r5.z, r5.w, o, inv, nx, ny, nz, lim);
+            const uint32_t axes = __float_as_uint(r7.x);
+            auto neg = [&](uint32_t ax) { return ax == 0 ? nx : (ax == 1 ? ny : nz); };
+            const uint32_t pf = neg(axes & 3u) ? 1u : 0u;         // pair visited first
+            const uint32_t i0 = neg((axes >> 2) & 3u) ? 1u : 0u;  // first inside pair 0
+            const uint32_t i1 = neg((axes >> 4) & 3u) ? 1u : 0u;  // first inside pair 1
+            const uint32_t f0 = pf ? 2 + i1 : i0;
+            const uint32_t f1 = pf ? i0 : 2 + i1;
+            const uint32_t seq[4] = {f0, f0 ^ 1u, f1, f1 ^ 1u};
+            uint32_t next = NODE_DONE;
+#pragma unroll
+            for (int q = 3; q >= 0; q--) {
+                const uint32_t sl = seq[q];
+                bool hq = sl == 0 ? h[0] : (sl == 1 ? h[1] : (sl == 2 ? h[2] : h[3]));
+                uint32_t wq = sl == 0 ? w[0] : (sl == 1 ? w[1] : (sl == 2 ? w[2] : w[3]));
+                if (hq) {
+                    if (next != NODE_DONE) S.push(R.sp, next);
+                    next = wq;
+                }
+            }
+            node = (next != NODE_DONE) ? next : S.pop(R.sp);
+        }
+    }
+    // Primitive ranges.
+    while (node != NODE_DONE && (node & 0x80000000u)) {
+        uint32_t first = node & 0x0FFFFFFFu, cnt = ((node >> 28) & 7u) + 1u;
+        for (uint32_t j = first; j < first + cnt; j++) {
+            const float4* p = a.prims + 3 * (size_t)j;
+            float4 v0 = p[0], v1 = p[1], v2 = p[2];
+            uint32_t id = __float_as_uint(v0.w);
+            uint32_t slot = __float_as_uint(v1.w), leaf = __float_as_uint(v2.w);
+            float t;
+            float b3[3] = {0.0f, 0.0f, 0.0f};
+            if (id < a.g.n_spheres) t = sphere_hit(a.g, id, R.o, R.d);
+            else t = tri_hit(v0, v1, v2, R.o, R.d, R.kx, R.ky, R.kz, R.sx, R.sy, R.sz, b3);
+            // first strict minimum in candidate order: leaves in depth-first
+            // order, slots in order inside a leaf
+            if (t != 0.0f && (t < R.tBest || (t == R.tBest && leaf == R.bestLeaf && slot < R.bestSlot))) {
+                R.tBest = t;
+                R.best = id;
+                R.bestSlot = slot;
+                R.bestLeaf = leaf;
+                R.bb0 = b3[0]; R.bb1 = b3[1]; R.bb2 = b3[2];
+            }
+        }
+        node = S.pop(R.sp);
+    }
+    R.node = node;
+}
+
+// One lane per ray, rays assigned to workgroups statically.
 template <int BLOCK, int SD, int WIDTH>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     __shared__ uint32_t stack[SD * BLOCK];
@@ -107,167 +265,135 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
         if (i >= n) return;
     }
     const size_t rs = a.stride ? a.stride : n;
-    f3 o = mk3(a.rays[i], a.rays[rs + i], a.rays[2 * rs + i]);
-    f3 d = mk3(a.rays[3 * rs + i], a.rays[4 * rs + i], a.rays[5 * rs + i]);
-    f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    const bool nx = inv.x < 0, ny = inv.y < 0, nz = inv.z < 0;
-    // Triangle setup shared by every candidate (pbrt-v3 MaxDimension, shear).
-    float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
-    int kz = (adx > ady) ? ((adx > adz) ? 0 : 2) : ((ady > adz) ? 1 : 2);
-    int kx = kz + 1; if (kx == 3) kx = 0;
-    int ky = kx + 1; if (ky == 3) ky = 0;
-    float dkx = comp(d, kx), dky = comp(d, ky), dkz = comp(d, kz);
-    float sx = -dkx / dkz, sy = -dky / dkz, sz = 1.0f / dkz;
+    LaneStack<BLOCK, SD> S{stack, a.spill, a.spill_depth, (size_t)a.n, i, a.err};
+    RayState R;
+    ray_begin(R, a, i, rs);
+    while (R.node != NODE_DONE) ray_step<BLOCK, SD, WIDTH>(R, a, S);
+    ray_end(R, a, i, rs);
+}
 
-    float tBest = INF;
-    uint32_t best = 0xFFFFFFFFu, bestSlot = 0xFFFFFFFFu, bestLeaf = 0xFFFFFFFFu;
-    float bb[3] = {0.0f, 0.0f, 0.0f};
-    const float* rb = a.root_box;
-    uint32_t node = 0xFFFFFFFFu;  // 0xFFFFFFFF = done
-    if (slab_hit(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, inv, nx, ny, nz, INF))
-        node = a.root_word;
-    uint32_t sp = 0;
-    const uint32_t spill_base = i;
-    auto push = [&](uint32_t v) {
-        if (sp < (uint32_t)SD) stack[sp * BLOCK + threadIdx.x] = v;
-        else if (sp - SD < a.spill_depth) a.spill[(size_t)(sp - SD) * a.n + spill_base] = v;
-        else { atomicOr(a.err, 1u); return; }
-        sp++;
-    };
-    auto pop = [&]() -> uint32_t {
-        if (sp == 0) return 0xFFFFFFFFu;
-        sp--;
-        return sp < (uint32_t)SD ? stack[sp * BLOCK + threadIdx.x]
-                                 : a.spill[(size_t)(sp - SD) * a.n + spill_base];
-    };
-    while (node != 0xFFFFFFFFu) {
-        if (WIDTH == 2) {
-            // Binary records: test both children, near first by dirIsNeg[axis].
-            while (!(node & 0x80000000u)) {
-                const float4* r = a.trav + 4 * (size_t)node;
-                float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-                float lim = tBest * PRUNE_SLACK;
-                bool h0 = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
-                bool h1 = slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
-                uint32_t w0 = __float_as_uint(r3.x), w1 = __float_as_uint(r3.y);
-                uint32_t axis = __float_as_uint(r3.z);
-                bool negAxis = (axis == 0) ? nx : ((axis == 1) ? ny : nz);
-                uint32_t nearW = negAxis ? w1 : w0, farW = negAxis ? w0 : w1;
-                bool hn = negAxis ? h1 : h0, hf = negAxis ? h0 : h1;
-                if (hn && hf) { push(farW); node = nearW; }
-                else if (hn) node = nearW;
-                else if (hf) node = farW;
-                else { node = pop(); if (node == 0xFFFFFFFFu) break; }
+// Persistent lanes with dynamic ray fetch: a lane whose ray is finished takes
+// the next ray of its wave's pool once at least `refill` lanes of the wave are
+// idle; the pool is refilled with `chunk` consecutive rays by one atomic on the
+// global claim counter.  Every wave leaves when the rays are exhausted and its
+// lanes are idle; the last wave out resets the claim counters for the next
+// launch.
+template <int BLOCK, int SD, int WIDTH>
+__global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a) {
+    __shared__ uint32_t stack[SD * BLOCK];
+    const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+    const size_t rs = a.stride ? a.stride : n;
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    LaneStack<BLOCK, SD> S{stack, a.spill, a.spill_depth, (size_t)gridDim.x * BLOCK, gtid, a.err};
+    const uint32_t lane = __lane_id();
+    RayState R;
+    R.node = NODE_DONE;
+    R.sp = 0;
+    bool active = false;
+    bool exhausted = false;                // wave-uniform: the claim counter has passed n
+    uint32_t pool_next = 0, pool_end = 0;  // wave-uniform: unassigned rays of the wave
+    uint32_t i = 0;
+    while (true) {
+        const uint64_t idle = ballot64(!active);
+        const uint32_t nidle = popc64(idle);
+        if ((!exhausted || pool_next < pool_end) && nidle >= a.refill) {
+            // ranks below the pool's remainder take pool rays, the rest a new chunk
+            const uint32_t rem = pool_end - pool_next;
+            uint32_t nbase = 0, nend = 0;
+            if (rem < nidle && !exhausted) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.claim, a.chunk);
+                base = (uint32_t)__shfl((int)base, 0);
+                if (base >= n) exhausted = true;
+                else { nbase = base; nend = min(base + a.chunk, n); if (nend == n) exhausted = true; }
             }
-        } else {
-            // 4-wide records (the grandchildren of a binary record): the pair of
-            // the near child first, the near grandchild first inside each pair;
-            // hits after the first are pushed in reverse visiting order.
-            while (!(node & 0x80000000u)) {
-                const float4* r = a.trav4 + 8 * (size_t)node;
-                float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6],
-                       r7 = r[7];
-                float lim = tBest * PRUNE_SLACK;
-                uint32_t w[4] = {__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z),
-                                 __float_as_uint(r6.w)};
-                bool h[4];
-                h[0] = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
-                h[1] = w[1] != 0xFFFFFFFFu &&
-                       slab_hit(r1.z, r1.w, r2.x, r2.y, r2.z, r2.w, o, inv, nx, ny, nz, lim);
-                h[2] = slab_hit(r3.x, r3.y, r3.z, r3.w, r4.x, r4.y, o, inv, nx, ny, nz, lim);
-                h[3] = w[3] != 0xFFFFFFFFu &&
-                       slab_hit(r4.z, r4.w, r5.x, r5.y, r5.z, r5.w, o, inv, nx, ny, nz, lim);
-                const uint32_t axes = __float_as_uint(r7.x);
-                auto neg = [&](uint32_t ax) { return ax == 0 ? nx : (ax == 1 ? ny : nz); };
-                const uint32_t pf = neg(axes & 3u) ? 1u : 0u;         // pair visited first
-                const uint32_t i0 = neg((axes >> 2) & 3u) ? 1u : 0u;  // first inside pair 0
-                const uint32_t i1 = neg((axes >> 4) & 3u) ? 1u : 0u;  // first inside pair 1
-                // visiting order: slots s0 s1 (pair pf), s2 s3 (the other pair)
-                const uint32_t f0 = pf ? 2 + i1 : i0;
-                const uint32_t s0 = f0, s1 = f0 ^ 1u;
-                const uint32_t f1 = pf ? i0 : 2 + i1;
-                const uint32_t s2 = f1, s3 = f1 ^ 1u;
-                uint32_t next = 0xFFFFFFFFu;
-                uint32_t seq[4] = {s0, s1, s2, s3};
-#pragma unroll
-                for (int q = 3; q >= 0; q--) {
-                    uint32_t sl = seq[q];
-                    bool hq = sl == 0 ? h[0] : (sl == 1 ? h[1] : (sl == 2 ? h[2] : h[3]));
-                    uint32_t wq = sl == 0 ? w[0] : (sl == 1 ? w[1] : (sl == 2 ? w[2] : w[3]));
-                    if (hq) {
-                        if (next != 0xFFFFFFFFu) push(next);
-                        next = wq;
-                    }
+            const uint32_t fresh = nend - nbase;
+            const uint32_t take = min(nidle, rem + fresh);
+            if (!active) {
+                const uint32_t r = popc64(idle & lanemask_lt());
+                if (r < take) {
+                    i = (r < rem) ? pool_next + r : nbase + (r - rem);
+                    ray_begin(R, a, i, rs);
+                    active = true;
                 }
-                if (next != 0xFFFFFFFFu) node = next;
-                else { node = pop(); if (node == 0xFFFFFFFFu) break; }
+            }
+            if (take <= rem) {
+                pool_next += take;
+            } else {
+                pool_next = nbase + (take - rem);
+                pool_end = nend;
             }
         }
-        // Primitive ranges.
-        while (node != 0xFFFFFFFFu && (node & 0x80000000u)) {
-            uint32_t first = node & 0x0FFFFFFFu, cnt = ((node >> 28) & 7u) + 1u;
-            for (uint32_t j = first; j < first + cnt; j++) {
-                const float4* p = a.prims + 3 * (size_t)j;
-                float4 v0 = p[0], v1 = p[1], v2 = p[2];
-                uint32_t id = __float_as_uint(v0.w);
-                uint32_t slot = __float_as_uint(v1.w), leaf = __float_as_uint(v2.w);
-                float t;
-                float b3[3] = {0.0f, 0.0f, 0.0f};
-                if (id < a.g.n_spheres) {
-                    t = sphere_hit(a.g, id, o, d);
-                } else {
-                    t = tri_hit(v0, v1, v2, o, d, kx, ky, kz, sx, sy, sz, b3);
-                }
-                // first strict minimum in candidate order: leaves in depth-first
-                // order, slots in order inside a leaf
-                if (t != 0.0f && (t < tBest || (t == tBest && leaf == bestLeaf && slot < bestSlot))) {
-                    tBest = t;
-                    best = id;
-                    bestSlot = slot;
-                    bestLeaf = leaf;
-                    bb[0] = b3[0]; bb[1] = b3[1]; bb[2] = b3[2];
-                }
+        if (ballot64(active) == 0) {
+            if (exhausted && pool_next >= pool_end) break;
+            continue;
+        }
+        if (active) {
+            if (R.node != NODE_DONE) ray_step<BLOCK, SD, WIDTH>(R, a, S);
+            if (R.node == NODE_DONE) {
+                ray_end(R, a, i, rs);
+                active = false;
             }
-            node = pop();
         }
     }
-    a.hit_shape[i] = best;
-    a.hit_t[i] = tBest;
-    if (a.bary) {
-        a.bary[i] = bb[0];
-        a.bary[rs + i] = bb[1];
-        a.bary[2 * rs + i] = bb[2];
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t waves = gridDim.x * (BLOCK / 64);
+        if (atomicAdd(a.claim + 1, 1u) == waves - 1) {
+            a.claim[0] = 0;
+            a.claim[1] = 0;
+        }
     }
 }
 
-template <int BLOCK, int WIDTH>
-static void launch_sd(const TraceArgs& a, int sd, hipStream_t st) {
-    const uint32_t P = a.parts > 1 ? a.parts : 1;
-    const uint32_t per_part = (a.n + P - 1) / P;
-    dim3 grid(P * ((per_part + BLOCK - 1) / BLOCK));
-    switch (sd) {
-        case 8: hipLaunchKernelGGL((k_trace<BLOCK, 8, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
-        case 16: hipLaunchKernelGGL((k_trace<BLOCK, 16, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
-        case 24: hipLaunchKernelGGL((k_trace<BLOCK, 24, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
-        case 32: hipLaunchKernelGGL((k_trace<BLOCK, 32, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
-        case 48: hipLaunchKernelGGL((k_trace<BLOCK, 48, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
-        default: hipLaunchKernelGGL((k_trace<BLOCK, 64, WIDTH>), grid, dim3(BLOCK), 0, st, a); break;
+template <int BLOCK, int WIDTH, bool DYN>
+static void launch_sd(const TraceArgs& a, int sd, hipStream_t st, uint32_t dyn_blocks) {
+    dim3 grid;
+    if (DYN) {
+        grid = dim3(dyn_blocks);
+    } else {
+        const uint32_t P = a.parts > 1 ? a.parts : 1;
+        const uint32_t per_part = (a.n + P - 1) / P;
+        grid = dim3(P * ((per_part + BLOCK - 1) / BLOCK));
     }
+#define WPBR_TRACE_CASE(SDV)                                                                         \
+    case SDV:                                                                                        \
+        if constexpr (DYN) hipLaunchKernelGGL((k_trace_dyn<BLOCK, SDV, WIDTH>), grid, dim3(BLOCK), 0, st, a); \
+        else hipLaunchKernelGGL((k_trace<BLOCK, SDV, WIDTH>), grid, dim3(BLOCK), 0, st, a);             \
+        break;
+    switch (sd) {
+        WPBR_TRACE_CASE(8)
+        WPBR_TRACE_CASE(16)
+        WPBR_TRACE_CASE(24)
+        WPBR_TRACE_CASE(32)
+        WPBR_TRACE_CASE(48)
+        default:
+        WPBR_TRACE_CASE(64)
+    }
+#undef WPBR_TRACE_CASE
 }
 
 template <int BLOCK>
-static void launch_w(const TraceArgs& a, int sd, hipStream_t st) {
-    if (a.width == 4) launch_sd<BLOCK, 4>(a, sd, st);
-    else launch_sd<BLOCK, 2>(a, sd, st);
+static void launch_w(const TraceArgs& a, int sd, hipStream_t st, uint32_t dyn_blocks) {
+    const bool dyn = dyn_blocks > 0;
+    if (a.width == 4) {
+        if (dyn) launch_sd<BLOCK, 4, true>(a, sd, st, dyn_blocks);
+        else launch_sd<BLOCK, 4, false>(a, sd, st, 0);
+    } else {
+        if (dyn) launch_sd<BLOCK, 2, true>(a, sd, st, dyn_blocks);
+        else launch_sd<BLOCK, 2, false>(a, sd, st, 0);
+    }
 }
 
-wpbr_status trace_launch(const TraceArgs& a, int block, int sd, hipStream_t st) {
+// dyn_blocks > 0 selects the persistent dynamic-fetch kernel with that many
+// workgroups (the spill array then has dyn_blocks * block slots).
+wpbr_status trace_launch(const TraceArgs& a, int block, int sd, hipStream_t st, uint32_t dyn_blocks) {
     if (a.n == 0) return WPBR_OK;
     switch (block) {
-        case 64: launch_w<64>(a, sd, st); break;
-        case 128: launch_w<128>(a, sd, st); break;
-        case 512: launch_w<512>(a, sd, st); break;
-        default: launch_w<256>(a, sd, st); break;
+        case 64: launch_w<64>(a, sd, st, dyn_blocks); break;
+        case 128: launch_w<128>(a, sd, st, dyn_blocks); break;
+        case 512: launch_w<512>(a, sd, st, dyn_blocks); break;
+        default: launch_w<256>(a, sd, st, dyn_blocks); break;
     }
     return launch_status();
 }

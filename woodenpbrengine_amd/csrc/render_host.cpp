@@ -369,7 +369,6 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     ta.g = sc->sv.g;
     ta.trav = (const float4*)b->s.trav;
     ta.trav4 = (const float4*)b->s.trav4;
-    ta.width = (uint32_t)ctx->trace_width;
     ta.prims = (const float4*)b->s.prims;
     ta.root_box = b->s.node_bounds;
     ta.root_word = b->root_word;
@@ -378,7 +377,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     ta.err = ctx->err_flag;
     ta.n_dev = nullptr;
     ta.stride = 0;
-    ta.parts = (uint32_t)ctx->trace_parts;
+    const uint32_t dyn_blocks = ctx_trace_shape(ctx, ta);
     // events: start, then per pass 5 boundaries (camera | trace | shade1 | trace | shade2)
     const size_t nev = 2 + 5 * (size_t)sc->spp;
     while (r->evs.size() < nev) {
@@ -399,7 +398,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         ta.hit_shape = r->ws.hit;
         ta.hit_t = r->ws.hit_t;
         ta.bary = r->ws.bary;
-        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st);
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
         hipEventRecord(e[1], st);
         if (!s) s = render_pass_launch(L, 1, st);
         hipEventRecord(e[2], st);
@@ -411,7 +410,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         ta.hit_shape = r->ws.qhit;
         ta.hit_t = r->ws.qt;
         ta.bary = nullptr;
-        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st);
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
         hipEventRecord(e[3], st);
         if (!s) s = render_pass_launch(L, 2, st);
         hipEventRecord(e[4], st);
@@ -459,3 +458,13 @@ int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap) {
 }
 
 }  // extern "C"
+
+extern "C" wpbr_status wpbr_autotune_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film,
+                                            wpbr_trace_config* best) {
+    if (!ctx || !sc || !d_film) return WPBR_ERR_INVALID_ARG;
+    return wpbr::autotune(ctx, wpbr::trace_candidates(true), [&](float* ms) {
+        wpbr_status s = wpbr_render(ctx, sc, d_film, nullptr);
+        *ms = (float)ctx->render->stats[1];
+        return s;
+    }, best);
+}

@@ -1,7 +1,9 @@
 // Host-side context and object definitions behind the opaque C handles.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <functional>
 #include <string>
+#include <vector>
 #include "wpbr_internal.hpp"
 
 struct wpbr_render_state;
@@ -17,8 +19,10 @@ struct wpbr_ctx {
     int trace_block = 64;
     int trace_stack = 16;
     int num_xcd = 8;          // MI355X: 8 XCDs, workgroups dispatched round-robin over them
-    int trace_parts = 1;      // ray range parts (1, or num_xcd for the XCD-aware mapping)
+    int trace_mapping = 1;    // 1 linear, 2 one ray part per XCD, 3 persistent lanes + dynamic fetch
     int trace_width = 4;      // 2: binary records, 4: 4-wide records
+    int trace_refill = 16;    // persistent kernel: idle lanes per wave that trigger a fetch
+    int trace_chunk = 256;    // persistent kernel: rays claimed per atomic
     uint32_t* spill = nullptr;
     size_t spill_words = 0;
     uint32_t spill_depth = 0;
@@ -45,6 +49,14 @@ wpbr_status set_err(wpbr_ctx* ctx, wpbr_status s, const std::string& msg);
 wpbr_status hip_check(wpbr_ctx* ctx, hipError_t e, const char* what);
 GeomView to_view(const wpbr_geometry* g);
 wpbr_status ctx_ensure_trace(wpbr_ctx* ctx, uint32_t n);
+// Fills the launch-shape fields of a TraceArgs from the context's configuration
+// and returns the persistent grid size (0: static kernel).
+uint32_t ctx_trace_shape(const wpbr_ctx* ctx, TraceArgs& a);
+// Autotuner: candidate configurations (small: a reduced set for expensive runs)
+// and the timing loop (run() performs one measured call and returns its ms).
+std::vector<wpbr_trace_config> trace_candidates(bool small);
+wpbr_status autotune(wpbr_ctx* ctx, const std::vector<wpbr_trace_config>& cands,
+                     const std::function<wpbr_status(float*)>& run, wpbr_trace_config* best);
 wpbr_status ctx_finish_trace(wpbr_ctx* ctx);
 void ctx_release_render(wpbr_ctx* ctx);
 wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroid, uint32_t* cb_ord,

@@ -82,6 +82,9 @@ struct TraceArgs {
     const uint32_t* n_dev;  // optional device-side ray count (queue launches)
     uint32_t stride;      // SoA stride of rays (0: n)
     uint32_t parts;       // 1: linear; P: part x of the ray range to workgroups b with b % P == x
+    uint32_t* claim;      // persistent kernel: [0] rays claimed, [1] waves finished (zero between launches)
+    uint32_t refill;      // persistent kernel: idle lanes that trigger a fetch (1..64)
+    uint32_t chunk;       // persistent kernel: rays claimed per atomic
     const float* rays;
     uint32_t* hit_shape;
     float* hit_t;
@@ -91,6 +94,6 @@ struct TraceArgs {
     uint32_t* err;        // bit 0: stack overflow
 };
 
-wpbr_status trace_launch(const TraceArgs& a, int block, int stack_depth, hipStream_t st);
+wpbr_status trace_launch(const TraceArgs& a, int block, int stack_depth, hipStream_t st, uint32_t dyn_blocks);
 
 }  // namespace wpbr

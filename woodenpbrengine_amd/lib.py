@@ -20,8 +20,10 @@ EXPORTS = [
     "wpbr_free", "wpbr_upload", "wpbr_download", "wpbr_sync", "wpbr_last_time_ms",
     "wpbr_shape_bounds", "wpbr_lbvh_create", "wpbr_lbvh_destroy", "wpbr_lbvh_build",
     "wpbr_lbvh_counts", "wpbr_lbvh_download", "wpbr_lbvh_kernel_times", "wpbr_collide",
-    "wpbr_collide_async", "wpbr_set_trace_config", "wpbr_scene_create", "wpbr_scene_destroy",
-    "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats",
+    "wpbr_collide_async", "wpbr_set_trace_config", "wpbr_get_trace_config", "wpbr_scene_create",
+    "wpbr_scene_destroy",
+    "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats", "wpbr_autotune_collide",
+    "wpbr_autotune_render",
 ]
 
 
@@ -36,6 +38,12 @@ class Geometry(C.Structure):
     _fields_ = [("n_spheres", C.c_uint32), ("d_sphere_xform", C.c_void_p),
                 ("d_sphere_radius", C.c_void_p), ("n_triangles", C.c_uint32),
                 ("d_tri_verts", C.c_void_p)]
+
+
+class TraceConfig(C.Structure):
+    """wpbr_trace_config (include/wpbr.h)."""
+    _fields_ = [("block", C.c_int32), ("stack_depth", C.c_int32), ("mapping", C.c_int32),
+                ("width", C.c_int32), ("refill", C.c_int32), ("chunk", C.c_int32)]
 
 
 class SceneDesc(C.Structure):
@@ -86,12 +94,16 @@ def load():
         "wpbr_lbvh_kernel_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
-        "wpbr_set_trace_config": (i32, [vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "wpbr_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
+        "wpbr_get_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
         "wpbr_scene_create": (i32, [vp, C.POINTER(SceneDesc), C.POINTER(vp)]),
         "wpbr_scene_destroy": (None, [vp]),
         "wpbr_scene_rebuild": (i32, [vp, vp]),
         "wpbr_render": (i32, [vp, vp, vp, vp]),
         "wpbr_render_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.c_int]),
+        "wpbr_autotune_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp,
+                                        C.POINTER(TraceConfig)]),
+        "wpbr_autotune_render": (i32, [vp, vp, vp, C.POINTER(TraceConfig)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -170,8 +182,14 @@ class Context:
     def last_ms(self):
         return float(self.lib.wpbr_last_time_ms(self.h))
 
-    def set_trace_config(self, block=0, stack_depth=0, mapping=0, width=0):
-        self.check(self.lib.wpbr_set_trace_config(self.h, block, stack_depth, mapping, width))
+    def set_trace_config(self, block=0, stack_depth=0, mapping=0, width=0, refill=0, chunk=0):
+        cfg = TraceConfig(block, stack_depth, mapping, width, refill, chunk)
+        self.check(self.lib.wpbr_set_trace_config(self.h, C.byref(cfg)))
+
+    def trace_config(self):
+        cfg = TraceConfig()
+        self.check(self.lib.wpbr_get_trace_config(self.h, C.byref(cfg)))
+        return {f: getattr(cfg, f) for f, _ in TraceConfig._fields_}
 
     def close(self):
         if self.h:
@@ -257,6 +275,16 @@ def collide(ctx, scene, bvh, d_rays, n, want_bary=False, wait=True):
     return d_shape, d_t, d_b
 
 
+def autotune_collide(ctx, scene, bvh, d_rays, n):
+    """Picks the fastest traversal configuration for these rays (kept in ctx)."""
+    d_shape = ctx.empty(n, np.uint32)
+    d_t = ctx.empty(n, np.float32)
+    cfg = TraceConfig()
+    ctx.check(ctx.lib.wpbr_autotune_collide(ctx.h, C.byref(scene.geom), bvh.h, n, d_rays.ptr,
+                                            d_shape.ptr, d_t.ptr, C.byref(cfg)))
+    return {f: getattr(cfg, f) for f, _ in TraceConfig._fields_}
+
+
 def scene_desc(s):
     """wpbr_scene_desc of a woodenpbrengine_amd.scenes.RenderScene; returns
     (desc, keepalive arrays)."""
@@ -289,6 +317,13 @@ class Renderer:
         self.W, self.H, self.spp = scene.width, scene.height, scene.spp
         self.d_film = ctx.empty((self.H, self.W, 4), np.float32)
         self.d_samples = None
+
+    def autotune(self):
+        """Picks the traversal configuration with the least render trace time."""
+        cfg = TraceConfig()
+        self.ctx.check(self.ctx.lib.wpbr_autotune_render(self.ctx.h, self.h, self.d_film.ptr,
+                                                         C.byref(cfg)))
+        return {f: getattr(cfg, f) for f, _ in TraceConfig._fields_}
 
     def render(self, keep_samples=False):
         if keep_samples and self.d_samples is None:
