@@ -2,7 +2,7 @@
 // lighting with MIS, all film tiles in one launch per sample pass.
 //
 // One pass per sample index j (the per-pixel sample cursors of CSampler.h make
-// pass j depend on pass j-1), each pass five kernels over all pixels:
+// pass j depend on pass j-1), each pass six kernels over all pixels:
 //   k_camera  JobSamplerUpdateCameraSamples + JobCameraPerspGenerateRaysDifferential
 //             (CSSampler.cpp:7-24, CSCameraPerspective.cpp:9-52)
 //   trace     runCollisionSystem for primary rays (collide.hip)
@@ -13,9 +13,10 @@
 //             light and BSDF Sample (CBSDF*.cpp); shadow and BSDF rays go to one
 //             queue compacted with wave ballots (one atomic per wave)
 //   trace     both secondary rays of every sample in one launch
-//   k_shade2  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:206-237), both
-//             MIS integrations (SScattering.cpp:259-386) over the 64 wavelengths,
-//             XYZ and the Mitchell splat into the sample's 32x32 tile (CSFilm.cpp:199-251)
+//   k_radiance  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:206-237),
+//             both MIS integrations (SScattering.cpp:259-386) over the 64
+//             wavelengths, XYZ of the sample
+//   k_splat   the Mitchell splat into the sample's 32x32 tile (CSFilm.cpp:199-251)
 // The reference's light sample for the env light stores a black Li
 // (CLightInfiniteArea.cpp:192-199): that strategy adds nothing, so its shadow
 // ray is not cast (the 2D sample is still consumed).
@@ -162,6 +163,11 @@ WPBR_D float fresnel_dielectric(float cosI, float etaI, float etaT) {
     return (Rparl * Rparl + Rperp * Rperp) / 2;
 }
 
+// FrConductor (CBXDF.h:200-238, pbrt-v3) for one wavelength.  This is the
+// innermost spectral loop of the radiance kernel (64 wavelengths x 2 MIS
+// strategies per sample); its two square roots and two quotients use the
+// hardware approximations (v_sqrt_f32, v_rcp_f32: about 1 ulp), which keeps
+// the per-sample radiance far inside the 1e-4 relative bound.
 WPBR_D float fresnel_conductor1(float cosI, float etaT, float k) {
     cosI = clampf(cosI, -1.0f, 1.0f);
     float eta = etaT / 1.0f, etak = k / 1.0f;
@@ -170,14 +176,14 @@ WPBR_D float fresnel_conductor1(float cosI, float etaT, float k) {
     float eta2 = eta * eta, etak2 = etak * etak;
     float t0 = eta2 - etak2 - s2;
     float a2pb22 = t0 * t0 + eta2 * etak2 * 4;
-    float a2pb2 = sqrtf(a2pb22);
+    float a2pb2 = __builtin_amdgcn_sqrtf(a2pb22);
     float t1 = a2pb2 + c2;
-    float a = sqrtf((a2pb2 + t0) * 0.5f);
+    float a = __builtin_amdgcn_sqrtf((a2pb2 + t0) * 0.5f);
     float t2 = a * cosI * 2.0f;
-    float Rs = (t1 - t2) / (t1 + t2);
+    float Rs = (t1 - t2) * __builtin_amdgcn_rcpf(t1 + t2);
     float t3 = a2pb2 * c2 + s2 * s2;
     float t4 = t2 * s2;
-    float Rp = Rs * (t3 - t4) / (t3 + t4);
+    float Rp = Rs * (t3 - t4) * __builtin_amdgcn_rcpf(t3 + t4);
     return (Rp + Rs) * 0.5f;
 }
 
@@ -604,33 +610,45 @@ WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& 
     }
 }
 
+// Radiance of every pixel's sample of this pass (one lane per pixel).
+__global__ __launch_bounds__(256) void k_radiance(RenderLaunch L) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    // secondary rays traced in this pass (statistics)
+    if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(L.ws.qcount + 2), (unsigned long long)*L.ws.qcount);
+    if (i >= L.n) return;
+    float X, Y, Z;
+    bool black;
+    sample_radiance(L, i, X, Y, Z, black);
+    const uint32_t n = L.n;
+    L.ws.rad[i] = X;
+    L.ws.rad[n + i] = Y;
+    L.ws.rad[2 * (size_t)n + i] = Z;
+    L.ws.rad[3 * (size_t)n + i] = black ? 1.0f : 0.0f;
+    if (L.sample_xyz) {
+        size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
+        L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
+    }
+}
+
 // One workgroup per 32x32 film tile: the reference clips the Mitchell splat to
 // the sample's own tile (CSFilm.cpp:224-225), so the tile's accumulation stays in
 // LDS and is added to the film once, without global atomics.
 constexpr int TILE_THREADS = 256;
-__global__ __launch_bounds__(TILE_THREADS) void k_shade2(RenderLaunch L) {
+__global__ __launch_bounds__(TILE_THREADS) void k_splat(RenderLaunch L) {
     __shared__ float acc[1024 * 4];
     const SceneView& sv = L.sv;
     const uint32_t W = sv.width;
     const uint32_t ntx = W / 32;
     const uint32_t tx0 = (blockIdx.x % ntx) * 32, ty0 = (blockIdx.x / ntx) * 32;
     for (int k = threadIdx.x; k < 1024 * 4; k += TILE_THREADS) acc[k] = 0.0f;
-    // secondary rays traced in this pass (statistics)
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        atomicAdd(reinterpret_cast<unsigned long long*>(L.ws.qcount + 2), (unsigned long long)*L.ws.qcount);
     __syncthreads();
     const uint32_t n = L.n;
     for (int r = 0; r < 4; r++) {
         const uint32_t lp = threadIdx.x + r * TILE_THREADS;  // local pixel, row-major in the tile
         const uint32_t px = tx0 + (lp & 31), py = ty0 + (lp >> 5);
         const uint32_t i = py * W + px;
-        float X, Y, Z;
-        bool black;
-        sample_radiance(L, i, X, Y, Z, black);
-        if (L.sample_xyz) {
-            size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
-            L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
-        }
+        const float X = L.ws.rad[i], Y = L.ws.rad[n + i], Z = L.ws.rad[2 * (size_t)n + i];
+        const bool black = L.ws.rad[3 * (size_t)n + i] != 0.0f;
         float fx = L.ws.pfilm[i], fy = L.ws.pfilm[n + i];
         float dx = fx - 0.5f, dy = fy - 0.5f;
         int x0 = (int)ceilf(dx - 2.0f), x1 = (int)floorf(dx + 2.0f);
@@ -666,8 +684,11 @@ wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {
     switch (stage) {
         case 0: hipLaunchKernelGGL(k_camera, grid, block, 0, st, L); break;
         case 1: hipLaunchKernelGGL(k_shade1, grid, block, 0, st, L); break;
-        case 2: hipLaunchKernelGGL(k_shade2, dim3((L.sv.width / 32) * (L.sv.height / 32)),
-                                   dim3(TILE_THREADS), 0, st, L); break;
+        case 2:
+            hipLaunchKernelGGL(k_radiance, grid, block, 0, st, L);
+            hipLaunchKernelGGL(k_splat, dim3((L.sv.width / 32) * (L.sv.height / 32)), dim3(TILE_THREADS), 0,
+                               st, L);
+            break;
     }
     return launch_status();
 }
