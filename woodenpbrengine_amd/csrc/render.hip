@@ -2,7 +2,7 @@
 // lighting with MIS, all film tiles in one launch per sample pass.
 //
 // One pass per sample index j (the per-pixel sample cursors of CSampler.h make
-// pass j depend on pass j-1), each pass six kernels over all pixels:
+// pass j depend on pass j-1), each pass five kernels over all pixels:
 //   k_camera  JobSamplerUpdateCameraSamples + JobCameraPerspGenerateRaysDifferential
 //             (CSSampler.cpp:7-24, CSCameraPerspective.cpp:9-52)
 //   trace     runCollisionSystem for primary rays (collide.hip)
@@ -13,10 +13,10 @@
 //             light and BSDF Sample (CBSDF*.cpp); shadow and BSDF rays go to one
 //             queue compacted with wave ballots (one atomic per wave)
 //   trace     both secondary rays of every sample in one launch
-//   k_radiance  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:206-237),
+//   k_radiance_splat  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:206-237),
 //             both MIS integrations (SScattering.cpp:259-386) over the 64
-//             wavelengths, XYZ of the sample
-//   k_splat   the Mitchell splat into the sample's 32x32 tile (CSFilm.cpp:199-251)
+//             wavelengths, XYZ, and the Mitchell splat into the sample's 32x32
+//             tile (CSFilm.cpp:199-251), gathered per film pixel
 // The reference's light sample for the env light stores a black Li
 // (CLightInfiniteArea.cpp:192-199): that strategy adds nothing, so its shadow
 // ray is not cast (the 2D sample is still consumed).
@@ -610,73 +610,67 @@ WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& 
     }
 }
 
-// Radiance of every pixel's sample of this pass (one lane per pixel).
-__global__ __launch_bounds__(256) void k_radiance(RenderLaunch L) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    // secondary rays traced in this pass (statistics)
-    if (i == 0) atomicAdd(reinterpret_cast<unsigned long long*>(L.ws.qcount + 2), (unsigned long long)*L.ws.qcount);
-    if (i >= L.n) return;
-    float X, Y, Z;
-    bool black;
-    sample_radiance(L, i, X, Y, Z, black);
-    const uint32_t n = L.n;
-    L.ws.rad[i] = X;
-    L.ws.rad[n + i] = Y;
-    L.ws.rad[2 * (size_t)n + i] = Z;
-    L.ws.rad[3 * (size_t)n + i] = black ? 1.0f : 0.0f;
-    if (L.sample_xyz) {
-        size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
-        L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
-    }
-}
-
-// One workgroup per 32x32 film tile: the reference clips the Mitchell splat to
-// the sample's own tile (CSFilm.cpp:224-225), so the tile's accumulation stays in
-// LDS and is added to the film once, without global atomics.
-constexpr int TILE_THREADS = 256;
-__global__ __launch_bounds__(TILE_THREADS) void k_splat(RenderLaunch L) {
-    __shared__ float acc[1024 * 4];
+// One workgroup per 32x32 film tile, one lane per pixel: the radiance of the
+// pixel's sample of this pass, then the Mitchell splat (CSFilm.cpp:199-251).
+// The reference clips a sample's splat to its own tile (CSFilm.cpp:224-225), so
+// every film pixel of the tile receives only the samples of the tile pixels
+// within two pixels of it; each lane gathers those from LDS in a fixed order
+// (deterministic sums, no atomics) and adds them to the film once.
+constexpr int TILE_THREADS = 1024;
+__global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L) {
+    __shared__ float s_fx[1024], s_fy[1024], s_x[1024], s_y[1024], s_z[1024];
+    __shared__ uint32_t s_black[1024];
     const SceneView& sv = L.sv;
     const uint32_t W = sv.width;
     const uint32_t ntx = W / 32;
     const uint32_t tx0 = (blockIdx.x % ntx) * 32, ty0 = (blockIdx.x / ntx) * 32;
-    for (int k = threadIdx.x; k < 1024 * 4; k += TILE_THREADS) acc[k] = 0.0f;
-    __syncthreads();
     const uint32_t n = L.n;
-    for (int r = 0; r < 4; r++) {
-        const uint32_t lp = threadIdx.x + r * TILE_THREADS;  // local pixel, row-major in the tile
-        const uint32_t px = tx0 + (lp & 31), py = ty0 + (lp >> 5);
-        const uint32_t i = py * W + px;
-        const float X = L.ws.rad[i], Y = L.ws.rad[n + i], Z = L.ws.rad[2 * (size_t)n + i];
-        const bool black = L.ws.rad[3 * (size_t)n + i] != 0.0f;
-        float fx = L.ws.pfilm[i], fy = L.ws.pfilm[n + i];
-        float dx = fx - 0.5f, dy = fy - 0.5f;
-        int x0 = (int)ceilf(dx - 2.0f), x1 = (int)floorf(dx + 2.0f);
-        int y0 = (int)ceilf(dy - 2.0f), y1 = (int)floorf(dy + 2.0f);
-        x0 = max(x0, (int)tx0); y0 = max(y0, (int)ty0);
-        x1 = min(x1, (int)tx0 + 31); y1 = min(y1, (int)ty0 + 31);
-        for (int y = y0; y <= y1; y++) {
-            float wy = mitchell1(((float)y - fy) / 2.0f);
-            for (int x = x0; x <= x1; x++) {
-                float w = mitchell1(((float)x - fx) / 2.0f) * wy;
-                float* f = acc + 4 * ((y - (int)ty0) * 32 + (x - (int)tx0));
-                if (!black) {
-                    atomicAdd(f, X * w);
-                    atomicAdd(f + 1, Y * w);
-                    atomicAdd(f + 2, Z * w);
-                }
-                atomicAdd(f + 3, w);
+    // secondary rays traced in this pass (statistics)
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        atomicAdd(reinterpret_cast<unsigned long long*>(L.ws.qcount + 2), (unsigned long long)*L.ws.qcount);
+    const uint32_t lp = threadIdx.x;  // local pixel, row-major in the tile
+    const int lx = (int)(lp & 31), ly = (int)(lp >> 5);
+    const uint32_t px = tx0 + lx, py = ty0 + ly;
+    const uint32_t i = py * W + px;
+    {
+        float X, Y, Z;
+        bool black;
+        sample_radiance(L, i, X, Y, Z, black);
+        if (L.sample_xyz) {
+            size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
+            L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
+        }
+        s_fx[lp] = L.ws.pfilm[i];
+        s_fy[lp] = L.ws.pfilm[n + i];
+        s_x[lp] = X; s_y[lp] = Y; s_z[lp] = Z;
+        s_black[lp] = black ? 1u : 0u;
+    }
+    __syncthreads();
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, aw = 0.0f;
+    const float fpx = (float)(int)px, fpy = (float)(int)py;
+    for (int sy = max(ly - 2, 0); sy <= min(ly + 2, 31); sy++) {
+        for (int sx = max(lx - 2, 0); sx <= min(lx + 2, 31); sx++) {
+            const int q = sy * 32 + sx;
+            const float fx = s_fx[q], fy = s_fy[q];
+            // the sample's footprint (before the tile clip, which keeps this pixel)
+            const float dx = fx - 0.5f, dy = fy - 0.5f;
+            const int x0 = (int)ceilf(dx - 2.0f), x1 = (int)floorf(dx + 2.0f);
+            const int y0 = (int)ceilf(dy - 2.0f), y1 = (int)floorf(dy + 2.0f);
+            if ((int)px < x0 || (int)px > x1 || (int)py < y0 || (int)py > y1) continue;
+            const float wy = mitchell1((fpy - fy) / 2.0f);
+            const float w = mitchell1((fpx - fx) / 2.0f) * wy;
+            if (!s_black[q]) {
+                ax += s_x[q] * w;
+                ay += s_y[q] * w;
+                az += s_z[q] * w;
             }
+            aw += w;
         }
     }
-    __syncthreads();
-    for (int lp = threadIdx.x; lp < 1024; lp += TILE_THREADS) {
-        const uint32_t px = tx0 + (lp & 31), py = ty0 + (lp >> 5);
-        float4* f = reinterpret_cast<float4*>(L.film) + ((size_t)py * W + px);
-        float4 v = *f;
-        v.x += acc[4 * lp]; v.y += acc[4 * lp + 1]; v.z += acc[4 * lp + 2]; v.w += acc[4 * lp + 3];
-        *f = v;
-    }
+    float4* f = reinterpret_cast<float4*>(L.film) + i;
+    float4 v = *f;
+    v.x += ax; v.y += ay; v.z += az; v.w += aw;
+    *f = v;
 }
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {
@@ -685,9 +679,8 @@ wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {
         case 0: hipLaunchKernelGGL(k_camera, grid, block, 0, st, L); break;
         case 1: hipLaunchKernelGGL(k_shade1, grid, block, 0, st, L); break;
         case 2:
-            hipLaunchKernelGGL(k_radiance, grid, block, 0, st, L);
-            hipLaunchKernelGGL(k_splat, dim3((L.sv.width / 32) * (L.sv.height / 32)), dim3(TILE_THREADS), 0,
-                               st, L);
+            hipLaunchKernelGGL(k_radiance_splat, dim3((L.sv.width / 32) * (L.sv.height / 32)),
+                               dim3(TILE_THREADS), 0, st, L);
             break;
     }
     return launch_status();

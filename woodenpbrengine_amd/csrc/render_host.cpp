@@ -176,7 +176,7 @@ static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     if (n <= r->cap) return WPBR_OK;
     if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
     // words per pixel: see the carve below
-    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2 + 4;
+    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2;
     size_t bytes = (words * (size_t)n + 64) * 4 + 64 * 256;
     wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
     if (s) return s;
@@ -219,7 +219,6 @@ static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     w.qcount = (uint32_t*)take(64);
     w.l_ray = (uint32_t*)take(n);
     w.b_ray = (uint32_t*)take(n);
-    w.rad = (float*)take(4 * (size_t)n);
     if (off > bytes) return set_err(ctx, WPBR_ERR_HIP, "render state carve overflow");
     return WPBR_OK;
 }

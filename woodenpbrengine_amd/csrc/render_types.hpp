@@ -76,7 +76,6 @@ struct WaveState {
     uint32_t* qcount;     // 1
     uint32_t* l_ray;      // queue slot of the light ray per sample
     uint32_t* b_ray;      // queue slot of the BSDF ray per sample
-    float* rad;           // this pass's sample: X, Y, Z and 1 if black (4 x N)
 };
 
 struct RenderLaunch {
