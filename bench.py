@@ -145,7 +145,7 @@ def main():
     for _ in range(args.warmup):
         r.render()
     ctx.sync()
-    dev_ms, trace_ms, shade_ms, rays, launches = [], [], [], [], []
+    dev_ms, trace_ms, shade_ms, rays, launches, prim_ms = [], [], [], [], [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         dev_ms.append(r.render())
@@ -153,6 +153,7 @@ def main():
         trace_ms.append(st["trace_ms"])
         shade_ms.append(st["shade_ms"])
         rays.append(st["rays"])
+        prim_ms.append(st["primary_trace_ms"])
         launches.append(st["trace_launches"])
     ctx.sync()
     wall = time.perf_counter() - t_start
@@ -163,13 +164,15 @@ def main():
     rays_per_step = float(np.mean(rays))
     t_ms = float(np.mean(trace_ms))
     n_launch = float(np.mean(launches))
-    log("render: %.2f ms/step wall, %.2f ms device, trace %.2f ms (%d launches, %.0f rays), shade %.2f ms"
-        % (ms_per_step, np.mean(dev_ms), t_ms, n_launch, rays_per_step, np.mean(shade_ms)))
+    log("render: %.2f ms/step wall, %.2f ms device, trace %.2f ms (primary %.2f; %d launches, %.0f rays), "
+        "shade %.2f ms" % (ms_per_step, np.mean(dev_ms), t_ms, np.mean(prim_ms), n_launch, rays_per_step,
+                           np.mean(shade_ms)))
     img = r.film_rgb()
     extra = {"image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
              "device_ms_per_step": float(np.mean(dev_ms)),
              "trace_ms_per_step": t_ms,
              "shade_ms_per_step": float(np.mean(shade_ms)),
+             "primary_trace_ms_per_step": float(np.mean(prim_ms)),
              "rays_per_step": rays_per_step,
              "trace_config": tuned,
              "render_mrays_per_s": rays_per_step / (t_ms * 1e-3) / 1e6}

@@ -196,7 +196,8 @@ wpbr_status wpbr_scene_rebuild(wpbr_ctx* ctx, wpbr_scene* scene);
 wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* d_sample_xyz);
 /* Statistics of the last wpbr_render: [0] total ms, [1] trace ms (all trace
  * launches), [2] number of trace launches, [3] rays traced, [4] shade ms,
- * [5] camera ms, [6] secondary rays. Returns the number written. */
+ * [5] camera ms, [6] secondary rays, [7] trace ms of the primary rays.
+ * Returns the number written. */
 int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap);
 /* Autotuner for the render: renders the scene once per candidate traversal
  * configuration, keeps the one with the least traversal time in the context

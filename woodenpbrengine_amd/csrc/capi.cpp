@@ -358,6 +358,7 @@ static wpbr_status collide_impl(wpbr_ctx* ctx, const wpbr_geometry* geom, const 
     a.hit_shape = d_hit_shape;
     a.hit_t = d_hit_t;
     a.bary = d_bary;
+    a.target = nullptr;
     a.spill = ctx->spill;
     a.spill_depth = ctx->spill_depth;
     a.err = ctx->err_flag;

@@ -101,6 +101,10 @@ struct RayState {
     float bb0, bb1, bb2;
     uint32_t node;  // next record or primitive range; NODE_DONE when finished
     uint32_t sp;
+    // visibility mode (a.target set): is the first hit `target` (or nothing)?
+    bool occ;
+    uint32_t target;
+    float tLim;     // t of the target along the ray (INF if it is missed)
 };
 constexpr uint32_t NODE_DONE = 0xFFFFFFFFu;
 
@@ -142,6 +146,27 @@ WPBR_D void ray_begin(RayState& R, const TraceArgs& a, uint32_t i, size_t rs) {
     R.tBest = INF;
     R.best = R.bestSlot = R.bestLeaf = 0xFFFFFFFFu;
     R.bb0 = R.bb1 = R.bb2 = 0.0f;
+    R.occ = a.target != nullptr;
+    R.target = 0xFFFFFFFFu;
+    R.tLim = INF;
+    if (R.occ) {
+        R.target = a.target[i];
+        if (R.target != 0xFFFFFFFFu) {
+            float t;
+            if (R.target < a.g.n_spheres) {
+                t = sphere_hit(a.g, R.target, R.o, R.d);
+            } else {
+                const float* v = a.g.tri_verts + 9 * (size_t)(R.target - a.g.n_spheres);
+                float b3[3];
+                t = tri_hit(make_float4(v[0], v[1], v[2], 0.f), make_float4(v[3], v[4], v[5], 0.f),
+                            make_float4(v[6], v[7], v[8], 0.f), R.o, R.d, R.kx, R.ky, R.kz, R.sx, R.sy,
+                            R.sz, b3);
+            }
+            if (t != 0.0f) R.tLim = t;
+        }
+        // prune beyond the target: only closer candidates can occlude it
+        R.tBest = R.tLim;
+    }
     const float* rb = a.root_box;
     R.node = slab_hit(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], R.o, R.inv, R.nx, R.ny, R.nz, INF)
                  ? a.root_word : NODE_DONE;
@@ -149,6 +174,13 @@ WPBR_D void ray_begin(RayState& R, const TraceArgs& a, uint32_t i, size_t rs) {
 }
 
 WPBR_D void ray_end(const RayState& R, const TraceArgs& a, uint32_t i, size_t rs) {
+    if (R.occ) {
+        // occluded: the occluder; else the target when the ray reaches it, or a miss
+        const bool occluded = R.best != 0xFFFFFFFFu;
+        a.hit_shape[i] = occluded ? R.best : (R.tLim != INF ? R.target : 0xFFFFFFFFu);
+        a.hit_t[i] = R.tBest;
+        return;
+    }
     a.hit_shape[i] = R.best;
     a.hit_t[i] = R.tBest;
     if (a.bary) {
@@ -228,10 +260,31 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
             float4 v0 = p[0], v1 = p[1], v2 = p[2];
             uint32_t id = __float_as_uint(v0.w);
             uint32_t slot = __float_as_uint(v1.w), leaf = __float_as_uint(v2.w);
+            if (R.occ && id == R.target) continue;  // the target never hides itself
             float t;
             float b3[3] = {0.0f, 0.0f, 0.0f};
             if (id < a.g.n_spheres) t = sphere_hit(a.g, id, R.o, R.d);
             else t = tri_hit(v0, v1, v2, R.o, R.d, R.kx, R.ky, R.kz, R.sx, R.sy, R.sz, b3);
+            if (R.occ) {
+                if (t != 0.0f && t <= R.tLim) {
+                    if (t < R.tLim) {
+                        // a candidate before the target: the first hit is not the target
+                        R.best = id;
+                        R.tBest = t;
+                        R.node = NODE_DONE;
+                        R.sp = 0;
+                        return;
+                    }
+                    // equal t: the candidate order decides; redo as a closest-hit query
+                    R.occ = false;
+                    R.tBest = INF;
+                    R.best = R.bestSlot = R.bestLeaf = 0xFFFFFFFFu;
+                    R.sp = 0;
+                    R.node = a.root_word;
+                    return;
+                }
+                continue;
+            }
             // first strict minimum in candidate order: leaves in depth-first
             // order, slots in order inside a leaf
             if (t != 0.0f && (t < R.tBest || (t == R.tBest && leaf == R.bestLeaf && slot < R.bestSlot))) {

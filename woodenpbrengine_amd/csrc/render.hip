@@ -377,6 +377,7 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
     const uint32_t n = L.n, cap = 2 * L.n;
     uint32_t flags = F_DONE;
     bool wantL = false, wantB = false;
+    uint32_t tgtL = 0xFFFFFFFFu, tgtB = 0xFFFFFFFFu;
     f3 lo = mk3(0, 0, 0), ld = lo, bo = lo, bd = lo;
     if (active) {
         f3 o = mk3(ws.rays[i], ws.rays[n + i], ws.rays[2 * (size_t)n + i]);
@@ -427,6 +428,7 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
             uint32_t mid = sv.shape_material[shape];
             flags = envPick ? F_ENV : 0u;
             ws.light[i] = li;
+            if (!envPick) tgtL = tgtB = sv.light_shape[li];
             if (mid == 0xFFFFFFFFu) {
                 flags |= F_DONE;
             } else {
@@ -499,8 +501,9 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
     }
     uint32_t sl = queue_append(wantL, ws.qcount);
     uint32_t sb = queue_append(wantB, ws.qcount);
-    if (wantL) { put_ray(ws.qrays, cap, sl, lo, ld); ws.l_ray[i] = sl; }
-    if (wantB) { put_ray(ws.qrays, cap, sb, bo, bd); ws.b_ray[i] = sb; }
+    // each queue ray asks whether its first hit is the sampled light (or nothing)
+    if (wantL) { put_ray(ws.qrays, cap, sl, lo, ld); ws.l_ray[i] = sl; ws.qtarget[sl] = tgtL; }
+    if (wantB) { put_ray(ws.qrays, cap, sb, bo, bd); ws.b_ray[i] = sb; ws.qtarget[sb] = tgtB; }
 }
 
 WPBR_D float mitchell1(float x) {

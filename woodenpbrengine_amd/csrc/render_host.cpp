@@ -176,7 +176,7 @@ static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     if (n <= r->cap) return WPBR_OK;
     if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
     // words per pixel: see the carve below
-    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2;
+    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2 + 2;
     size_t bytes = (words * (size_t)n + 64) * 4 + 64 * 256;
     wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
     if (s) return s;
@@ -219,6 +219,7 @@ static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     w.qcount = (uint32_t*)take(64);
     w.l_ray = (uint32_t*)take(n);
     w.b_ray = (uint32_t*)take(n);
+    w.qtarget = (uint32_t*)take(2 * (size_t)n);
     if (off > bytes) return set_err(ctx, WPBR_ERR_HIP, "render state carve overflow");
     return WPBR_OK;
 }
@@ -398,6 +399,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         ta.hit_shape = r->ws.hit;
         ta.hit_t = r->ws.hit_t;
         ta.bary = r->ws.bary;
+        ta.target = nullptr;
         if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
         hipEventRecord(e[1], st);
         if (!s) s = render_pass_launch(L, 1, st);
@@ -410,6 +412,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         ta.hit_shape = r->ws.qhit;
         ta.hit_t = r->ws.qt;
         ta.bary = nullptr;
+        ta.target = r->ws.qtarget;
         if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
         hipEventRecord(e[3], st);
         if (!s) s = render_pass_launch(L, 2, st);
@@ -423,7 +426,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     s = hip_check(ctx, hipMemcpyAsync(&qtotal, r->ws.qcount + 2, 8, hipMemcpyDeviceToHost, st), "d2h");
     if (!s) s = ctx_finish_trace(ctx);
     if (s) return s;
-    double trace_ms = 0, shade_ms = 0, cam_ms = 0;
+    double trace_ms = 0, shade_ms = 0, cam_ms = 0, prim_ms = 0;
     for (uint32_t j = 0; j < sc->spp; j++) {
         hipEvent_t* e = &r->evs[1 + 5 * (size_t)j];
         hipEvent_t prev = j ? r->evs[5 * (size_t)j] : r->evs[0];
@@ -435,6 +438,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         hipEventElapsedTime(&t4, e[3], e[4]);
         cam_ms += t0;
         trace_ms += t1 + t3;
+        prim_ms += t1;
         shade_ms += t2 + t4;
     }
     float tot = 0;
@@ -447,12 +451,13 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     r->stats[4] = shade_ms;
     r->stats[5] = cam_ms;
     r->stats[6] = (double)qtotal;
+    r->stats[7] = prim_ms;
     return WPBR_OK;
 }
 
 int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap) {
     if (!ctx || !out || !ctx->render) return 0;
-    int k = cap < 7 ? cap : 7;
+    int k = cap < 8 ? cap : 8;
     for (int i = 0; i < k; i++) out[i] = ctx->render->stats[i];
     return k;
 }

@@ -76,6 +76,7 @@ struct WaveState {
     uint32_t* qcount;     // 1
     uint32_t* l_ray;      // queue slot of the light ray per sample
     uint32_t* b_ray;      // queue slot of the BSDF ray per sample
+    uint32_t* qtarget;    // 2N: light shape a queue ray must reach (0xFFFFFFFF: escape to the env)
 };
 
 struct RenderLaunch {

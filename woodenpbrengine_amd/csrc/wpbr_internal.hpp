@@ -83,6 +83,8 @@ struct TraceArgs {
     uint32_t stride;      // SoA stride of rays (0: n)
     uint32_t parts;       // 1: linear; P: part x of the ray range to workgroups b with b % P == x
     uint32_t* claim;      // persistent kernel: [0] rays claimed, [1] waves finished (zero between launches)
+    const uint32_t* target;  // visibility queries: per ray, the shape whose visibility is asked
+                             // (0xFFFFFFFF: none); the result is the target, a miss or an occluder
     uint32_t refill;      // persistent kernel: idle lanes that trigger a fetch (1..64)
     uint32_t chunk;       // persistent kernel: rays claimed per atomic
     const float* rays;

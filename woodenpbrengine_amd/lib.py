@@ -304,7 +304,7 @@ class Renderer:
     """A device-resident scene (wpbr_scene) and its film."""
 
     STATS = ["total_ms", "trace_ms", "trace_launches", "rays", "shade_ms", "camera_ms",
-             "secondary_rays"]
+             "secondary_rays", "primary_trace_ms"]
 
     def __init__(self, ctx, scene):
         self.ctx = ctx
