@@ -293,6 +293,8 @@ void orc_to_xyz(const float* s64, float* xyz) {
     to_xyz(s, xyz);
 }
 
+void orc_shade_probe(int fn, const float* in, float* out) { shade_probe(fn, in, out); }
+
 void orc_samples(uint32_t seed, uint32_t pixel, uint32_t n1, uint32_t n2, float* u1, float* u2) {
     for (uint32_t i = 0; i < n1; i++) u1[i] = sample_1d(seed, pixel, i);
     for (uint32_t i = 0; i < n2; i++) sample_2d(seed, pixel, i, u2 + 2 * i);

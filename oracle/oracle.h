@@ -144,5 +144,6 @@ float sample_1d(uint32_t seed, uint32_t pixel, uint32_t index);
 void sample_2d(uint32_t seed, uint32_t pixel, uint32_t index, float* u);
 void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
             const uint32_t* tile_subset, uint32_t n_tiles);
+void shade_probe(int fn, const float* in, float* out);
 
 }  // namespace orc

@@ -266,3 +266,19 @@ def samples(seed, pixel, n1, n2):
     u2 = np.empty((n2, 2), np.float32)
     lib.orc_samples(seed, pixel, n1, n2, _p(u1), _p(u2))
     return u1, u2
+
+
+PROBE = {"fresnel_dielectric": (0, 1), "fresnel_conductor": (1, 1), "ggx_d": (2, 1),
+         "ggx_sample": (3, 4), "mitchell": (4, 1), "cone_pdf": (5, 1), "coordinate_system": (6, 6),
+         "spherical": (7, 2), "dist1d": (8, 4)}
+
+
+def probe(name, *args):
+    """One oracle shading function (render.cpp shade_probe) on float32 inputs."""
+    lib = load()
+    lib.orc_shade_probe.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+    fn, nout = PROBE[name]
+    inp = _f32(np.concatenate([np.atleast_1d(np.asarray(a, np.float64)).ravel() for a in args]))
+    out = np.zeros(max(nout, 1), np.float32)
+    lib.orc_shade_probe(fn, _p(inp), _p(out))
+    return out if nout > 1 else float(out[0])

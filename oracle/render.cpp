@@ -720,3 +720,53 @@ void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nt
 }
 
 }  // namespace orc
+
+namespace orc {
+
+// One shading function at a time, for the pinning tests (tests/test_oracle_shade.py):
+//   0 fresnel_dielectric(cosI, etaI, etaT)       1 fresnel_conductor1(cosI, eta, k)
+//   2 ggx_d(wh.xyz, ax, ay)                      3 ggx_sample(wo.xyz, u0, u1, ax, ay) -> wh.xyz, D
+//   4 mitchell1(x)                               5 cone_pdf(cosThetaMax)
+//   6 coordinate_system(v.xyz) -> v2.xyz, v3.xyz 7 spherical theta, phi of v.xyz
+//   8 Dist1D over in[2..2+n) (n = in[0]): sample_continuous(in[1]) -> x, pdf, offset, funcInt
+void shade_probe(int fn, const float* in, float* out) {
+    switch (fn) {
+        case 0: out[0] = fresnel_dielectric(in[0], in[1], in[2]); break;
+        case 1: out[0] = fresnel_conductor1(in[0], in[1], in[2]); break;
+        case 2: out[0] = ggx_d(v3(in[0], in[1], in[2]), in[3], in[4]); break;
+        case 3: {
+            float whD;
+            V3 wh = ggx_sample(v3(in[0], in[1], in[2]), in + 3, in[5], in[6], &whD);
+            out[0] = wh.x; out[1] = wh.y; out[2] = wh.z; out[3] = whD;
+            break;
+        }
+        case 4: out[0] = mitchell1(in[0]); break;
+        case 5: out[0] = cone_pdf(in[0]); break;
+        case 6: {
+            V3 a, b;
+            coordinate_system(v3(in[0], in[1], in[2]), &a, &b);
+            out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = b.x; out[4] = b.y; out[5] = b.z;
+            break;
+        }
+        case 7: {
+            V3 v = v3(in[0], in[1], in[2]);
+            out[0] = spherical_theta(v);
+            out[1] = spherical_phi(v);
+            break;
+        }
+        case 8: {
+            Dist1D d;
+            d.build(in + 2, (int)in[0]);
+            float pdf;
+            int off;
+            out[0] = d.sample_continuous(in[1], &pdf, &off);
+            out[1] = pdf;
+            out[2] = (float)off;
+            out[3] = d.funcInt;
+            break;
+        }
+        default: out[0] = 0.0f;
+    }
+}
+
+}  // namespace orc

@@ -43,6 +43,20 @@ WPBR_D void next2d(uint32_t seed, uint32_t pixel, uint32_t& c2, float* u) {
     c2++;
 }
 
+// ---- per-pixel state order ----------------------------------------------------------
+// Per-pixel state (rays, hits, shading terms) is kept in tile order: 32x32 film
+// tiles row-major, 8x8 blocks row-major inside a tile, pixels row-major inside a
+// block, so one wavefront's 64 lanes trace one 8x8 block of the film.  Sample
+// streams and outputs use the film's row-major pixel index py * W + px.
+WPBR_D void state_pixel(uint32_t i, uint32_t W, uint32_t& px, uint32_t& py) {
+    const uint32_t t = i >> 10, r = i & 1023u, b = r >> 6, q = r & 63u, ntx = W >> 5;
+    px = ((t % ntx) << 5) + ((b & 3u) << 3) + (q & 7u);
+    py = ((t / ntx) << 5) + ((b >> 2) << 3) + (q >> 3);
+}
+WPBR_D uint32_t state_index(uint32_t tile, uint32_t lx, uint32_t ly) {
+    return (tile << 10) + ((((ly >> 3) << 2) + (lx >> 3)) << 6) + ((ly & 7u) << 3) + (lx & 7u);
+}
+
 // ---- math helpers ---------------------------------------------------------------
 WPBR_D float clampf(float x, float lo, float hi) { return x < lo ? lo : (x > hi ? hi : x); }
 WPBR_D f3 xf_point4(const float* m, f3 p) {
@@ -347,11 +361,12 @@ __global__ __launch_bounds__(256) void k_camera(RenderLaunch L) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= L.n) return;
     const SceneView& sv = L.sv;
-    uint32_t px = i % sv.width, py = i / sv.width;
+    uint32_t px, py;
+    state_pixel(i, sv.width, px, py);
     uint32_t c2 = L.spp_index == 0 ? 0u : L.ws.c2[i];
     if (L.spp_index == 0) L.ws.c1[i] = 0;
     float u[2];
-    next2d(L.seed, i, c2, u);
+    next2d(L.seed, py * sv.width + px, c2, u);
     L.ws.c2[i] = c2;
     float fx = (float)px + u[0], fy = (float)py + u[1];
     L.ws.pfilm[i] = fx;
@@ -413,12 +428,15 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
                 si = si_triangle(sv, shape - sv.g.n_spheres, d, b);
             }
             const uint32_t nLights = sv.n_sphere_lights + (sv.has_env ? 1u : 0u);
-            float u1 = next1d(L.seed, i, c1);
+            uint32_t px, py;
+            state_pixel(i, sv.width, px, py);
+            const uint32_t pix = py * sv.width + px;  // sample-stream id
+            float u1 = next1d(L.seed, pix, c1);
             uint32_t li = (uint32_t)((u1 + 0.5f) * (float)(nLights - 1));
             if (li > nLights - 1) li = nLights - 1;
             bool envPick = sv.has_env && li == sv.n_sphere_lights;
             float ul[2];
-            next2d(L.seed, i, c2, ul);
+            next2d(L.seed, pix, c2, ul);
             f3 wiL = mk3(0, 0, 0);
             float pdfL = 0;
             if (!envPick) {
@@ -465,7 +483,7 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
                     ws.b_ac[i] = c;
                 } else {
                     float ub[2];
-                    next2d(L.seed, i, c2, ub);
+                    next2d(L.seed, pix, c2, ub);
                     float phi = (float)((2 * PI_REF) * (double)ub[1]);
                     float tan2 = m.alpha * m.alpha * ub[0] / (1.0f - ub[0]);
                     float cosT = 1 / sqrtf(1 + tan2);
@@ -634,13 +652,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L)
     const uint32_t lp = threadIdx.x;  // local pixel, row-major in the tile
     const int lx = (int)(lp & 31), ly = (int)(lp >> 5);
     const uint32_t px = tx0 + lx, py = ty0 + ly;
-    const uint32_t i = py * W + px;
+    const uint32_t i = state_index(blockIdx.x, lx, ly);
+    const uint32_t pix = py * W + px;
     {
         float X, Y, Z;
         bool black;
         sample_radiance(L, i, X, Y, Z, black);
         if (L.sample_xyz) {
-            size_t o = ((size_t)i * L.spp + L.spp_index) * 3;
+            size_t o = ((size_t)pix * L.spp + L.spp_index) * 3;
             L.sample_xyz[o] = X; L.sample_xyz[o + 1] = Y; L.sample_xyz[o + 2] = Z;
         }
         s_fx[lp] = L.ws.pfilm[i];
@@ -670,7 +689,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L)
             aw += w;
         }
     }
-    float4* f = reinterpret_cast<float4*>(L.film) + i;
+    float4* f = reinterpret_cast<float4*>(L.film) + pix;
     float4 v = *f;
     v.x += ax; v.y += ay; v.z += az; v.w += aw;
     *f = v;
