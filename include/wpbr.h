@@ -87,6 +87,16 @@ void wpbr_lbvh_destroy(wpbr_lbvh* bvh);
  * codes, radix sort, treelets, upper SAH, flatten, traversal layout).  Device
  * buffers are owned by the lbvh object and reused across builds. */
 wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* bvh);
+/* The same build one job at a time, in order (MEngine::buildLBVH,
+ * MEngine.cpp:341-350), so each of the reference's JOB_RUN calls maps to one
+ * call: 0 JobUpdateBoundsAndCentroidSphere/Triangle (CSTriangle.cpp:116-142,
+ * CSSphere.h:126-153), 1 JobGenerateShapesCentroidBound (SLBVH.cpp:164),
+ * 2 JobGenerateShapesMortonCode (SLBVH.cpp:186), 3 JobSortsShapesByMortonCode
+ * (SLBVH.cpp:231), 4 JobEmitLBVH (SLBVH.cpp:328), 5 JobBuildUpperSAH
+ * (SLBVH.cpp:410).  Stages 0-4 return once queued on the context's stream;
+ * stage 5 waits for the build.  A stage out of order, or a different geometry
+ * than stage 0's, is WPBR_ERR_INVALID_ARG. */
+wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* bvh, int stage);
 /* Counts: [0] shapes, [1] nodes, [2] upper SAH nodes, [3] treelets. */
 wpbr_status wpbr_lbvh_counts(wpbr_lbvh* bvh, uint32_t counts[4]);
 /* Downloads one array of CSceneTreeLBVH (SLBVH.h:17-40) or build state to host

@@ -142,3 +142,22 @@ def test_autotune_collide(ctx):
     finally:
         ctx.set_trace_config(*DEFAULT_TRACE)
     b.destroy()
+
+
+def test_staged_build_matches_and_checks_order(ctx):
+    """wpbr_lbvh_build_stage: the six jobs in order equal wpbr_lbvh_build; a
+    stage out of order is rejected."""
+    sx, sr, tris = _case("small")
+    sc = lib.Scene(ctx, sx, sr, tris)
+    b = lib.Lbvh(ctx)
+    with pytest.raises(lib.WpbrError):
+        b.build_stage(sc, 3)
+    for stage in range(6):
+        b.build_stage(sc, stage)
+    ob = po.build_scene_bvh(po.OScene(sx, sr, tris))
+    np.testing.assert_array_equal(b.download("node_offset"), ob.get("node_offset"))
+    np.testing.assert_array_equal(b.download("node_bounds"), ob.get("node_bounds"))
+    b.build_stage(sc, 0)
+    with pytest.raises(lib.WpbrError):
+        b.build_stage(sc, 2)
+    b.destroy()

@@ -234,29 +234,46 @@ void wpbr_lbvh_destroy(wpbr_lbvh* b) {
     delete b;
 }
 
-wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b) {
+wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b, int stage) {
     if (!ctx || !geom || !b) return WPBR_ERR_INVALID_ARG;
-    uint64_t n64 = (uint64_t)geom->n_spheres + geom->n_triangles;
-    if (n64 == 0 || n64 >= (1ull << 28)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count (1 .. 2^28-1)");
-    if ((geom->n_spheres && (!geom->d_sphere_xform || !geom->d_sphere_radius)) ||
-        (geom->n_triangles && !geom->d_tri_verts))
-        return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
-    uint32_t n = (uint32_t)n64;
+    if (stage < 0 || stage >= LBVH_STAGES) return set_err(ctx, WPBR_ERR_INVALID_ARG, "stage (0 .. 5)");
     hipSetDevice(ctx->device);
     (void)hipGetLastError();  // launch checks below must not see stale errors
-    if (n > b->cap) {
-        if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
-        LbvhScratch tmp;
-        size_t bytes = lbvh_layout(tmp, nullptr, n);
-        wpbr_status s = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc lbvh");
-        if (s) { b->cap = 0; return s; }
-        b->cap = n;
+    if (stage == 0) {
+        uint64_t n64 = (uint64_t)geom->n_spheres + geom->n_triangles;
+        if (n64 == 0 || n64 >= (1ull << 28)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count (1 .. 2^28-1)");
+        if ((geom->n_spheres && (!geom->d_sphere_xform || !geom->d_sphere_radius)) ||
+            (geom->n_triangles && !geom->d_tri_verts))
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
+        uint32_t n = (uint32_t)n64;
+        if (n > b->cap) {
+            if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
+            LbvhScratch tmp;
+            size_t bytes = lbvh_layout(tmp, nullptr, n);
+            wpbr_status s = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc lbvh");
+            if (s) { b->cap = 0; return s; }
+            b->cap = n;
+        }
+        lbvh_layout(b->s, (char*)b->mem, n);
+        b->n = n;
+        b->geom = to_view(geom);
+        b->built = false;
+    } else {
+        if (b->next_stage != stage)
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "lbvh stages run in order 0 .. 5");
+        GeomView g = to_view(geom);
+        if (g.n_spheres != b->geom.n_spheres || g.n_triangles != b->geom.n_triangles ||
+            g.sphere_xform != b->geom.sphere_xform || g.tri_verts != b->geom.tri_verts)
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "geometry changed between lbvh stages");
     }
-    lbvh_layout(b->s, (char*)b->mem, n);
-    b->n = n;
-    b->geom = to_view(geom);
-    wpbr_status s = lbvh_launch(b->s, b->geom, n, ctx->stream, b->ev, wpbr_lbvh::NEV);
-    if (s) return set_err(ctx, s, "lbvh launch failed");
+    wpbr_status s = lbvh_launch_stage(b->s, b->geom, b->n, ctx->stream, stage, b->ev, wpbr_lbvh::NEV);
+    if (s) {
+        b->next_stage = 0;
+        return set_err(ctx, s, "lbvh launch failed");
+    }
+    b->next_stage = stage + 1;
+    if (stage < LBVH_STAGES - 1) return WPBR_OK;
+    b->next_stage = 0;
     uint32_t info[2] = {0, 0};
     s = hip_check(ctx, hipMemcpyAsync(&info[0], b->s.n_nodes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (!s) s = hip_check(ctx, hipMemcpyAsync(&info[1], b->s.n_treelets, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
@@ -277,6 +294,14 @@ wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh*
     b->built = err == 0;
     if (err & 1u) return set_err(ctx, WPBR_ERR_TIMEOUT, "radix sort look-back timed out");
     if (err) return set_err(ctx, WPBR_ERR_HIP, "upper SAH build: internal stack exhausted");
+    return WPBR_OK;
+}
+
+wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b) {
+    for (int stage = 0; stage < LBVH_STAGES; stage++) {
+        wpbr_status s = wpbr_lbvh_build_stage(ctx, geom, b, stage);
+        if (s) return s;
+    }
     return WPBR_OK;
 }
 

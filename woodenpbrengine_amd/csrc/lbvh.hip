@@ -1416,48 +1416,74 @@ wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroi
 
 // ---------------------------------------------------------------------------
 // Host-side launch sequence.
-wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st,
-                        hipEvent_t* ev, int nev) {
+// One stage of the build (the reference's job boundaries, MEngine.cpp:341-350):
+//   0 bounds + centroids      JobUpdateBoundsAndCentroidSphere / ...Triangle
+//   1 centroid bounds         JobGenerateShapesCentroidBound
+//   2 Morton codes            JobGenerateShapesMortonCode
+//   3 sort + treelets         JobSortsShapesByMortonCode
+//   4 treelet trees           JobEmitLBVH
+//   5 upper SAH + flatten     JobBuildUpperSAH (+ the derived traversal layout)
+// ev[0..11] bracket the kernels (the per-kernel times of wpbr_lbvh_kernel_times).
+wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
+                              hipEvent_t* ev, int nev) {
     auto rec = [&](int i) { if (ev && i < nev) hipEventRecord(ev[i], st); };
-    const uint32_t num_tiles = (n + SORT_TILE - 1) / SORT_TILE;
-    rec(0);
-    hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, (s.status_words + 255) / 256)), dim3(256), 0, st, s);
-    hipLaunchKernelGGL(k_bounds, dim3(std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256)), dim3(256), 0, st, g, n, s.bounds,
-                       s.centroid, s.cb_ord);
-    rec(1);
-    hipLaunchKernelGGL(k_morton, dim3((n + MORTON_BLOCK * MORTON_ITEMS - 1) / (MORTON_BLOCK * MORTON_ITEMS)),
-                       dim3(MORTON_BLOCK), 0, st, s.centroid, n, s.cb_ord, s.morton, s.hist);
-    rec(2);
-    // ping-pong: morton -> B -> A -> B -> A
-    const uint32_t* kin[4] = {s.morton, s.keys_b, s.keys_a, s.keys_b};
-    const uint32_t* vin[4] = {nullptr, s.vals_b, s.vals_a, s.vals_b};
-    uint32_t* kout[4] = {s.keys_b, s.keys_a, s.keys_b, s.keys_a};
-    uint32_t* vout[4] = {s.vals_b, s.vals_a, s.vals_b, s.vals_a};
-    for (int p = 0; p < 4; p++) {
-        uint32_t* st_cur = (p & 1) ? s.status1 : s.status0;
-        uint32_t* st_next = (p == 3) ? nullptr : ((p & 1) ? s.status0 : s.status1);
-        hipLaunchKernelGGL(k_onesweep, dim3(num_tiles), dim3(SORT_BLOCK), 0, st, kin[p], vin[p],
-                           kout[p], vout[p], n, p, s.hist, st_cur, st_next, s.tile_counter, s.err,
-                           num_tiles);
-        rec(3 + p);
+    switch (stage) {
+        case 0:
+            rec(0);
+            hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, (s.status_words + 255) / 256)), dim3(256), 0,
+                               st, s);
+            hipLaunchKernelGGL(k_bounds, dim3(std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256)), dim3(256), 0,
+                               st, g, n, s.bounds, s.centroid, s.cb_ord);
+            rec(1);
+            break;
+        case 1:
+            hipLaunchKernelGGL(k_cb_final, dim3(1), dim3(64), 0, st, s.cb_ord);
+            break;
+        case 2:
+            hipLaunchKernelGGL(k_morton, dim3((n + MORTON_BLOCK * MORTON_ITEMS - 1) / (MORTON_BLOCK * MORTON_ITEMS)),
+                               dim3(MORTON_BLOCK), 0, st, s.centroid, n, s.cb_ord, s.morton, s.hist);
+            rec(2);
+            break;
+        case 3: {
+            const uint32_t num_tiles = (n + SORT_TILE - 1) / SORT_TILE;
+            // ping-pong: morton -> B -> A -> B -> A
+            const uint32_t* kin[4] = {s.morton, s.keys_b, s.keys_a, s.keys_b};
+            const uint32_t* vin[4] = {nullptr, s.vals_b, s.vals_a, s.vals_b};
+            uint32_t* kout[4] = {s.keys_b, s.keys_a, s.keys_b, s.keys_a};
+            uint32_t* vout[4] = {s.vals_b, s.vals_a, s.vals_b, s.vals_a};
+            for (int p = 0; p < 4; p++) {
+                uint32_t* st_cur = (p & 1) ? s.status1 : s.status0;
+                uint32_t* st_next = (p == 3) ? nullptr : ((p & 1) ? s.status0 : s.status1);
+                hipLaunchKernelGGL(k_onesweep, dim3(num_tiles), dim3(SORT_BLOCK), 0, st, kin[p], vin[p], kout[p],
+                                   vout[p], n, p, s.hist, st_cur, st_next, s.tile_counter, s.err, num_tiles);
+                rec(3 + p);
+            }
+            hipLaunchKernelGGL(k_treelet_mark, dim3((n + 255) / 256), dim3(256), 0, st, s.keys_a, n, s.bin_start,
+                               s.presence);
+            rec(7);
+            break;
+        }
+        case 4:
+            hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, s.vals_a, s.bounds, n);
+            rec(8);
+            break;
+        case 5: {
+            hipLaunchKernelGGL(k_upper_sah, dim3(1), dim3(SAH_BLOCK), 0, st, s);
+            rec(9);
+            hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
+            rec(10);
+            const uint32_t max_nodes = 2 * n;
+            hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, s.keys_a, s.vals_a);
+            hipLaunchKernelGGL(k_subtree_big, dim3(64), dim3(256), 0, st, s, s.vals_a);
+            hipLaunchKernelGGL(k_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, s.vals_a, n);
+            const uint32_t max_records = 3 * std::max<uint32_t>(n, 1);
+            hipLaunchKernelGGL(k_bvh4, dim3((max_records + 255) / 256), dim3(256), 0, st, s);
+            rec(11);
+            break;
+        }
+        default:
+            return WPBR_ERR_INVALID_ARG;
     }
-    hipLaunchKernelGGL(k_treelet_mark, dim3((n + 255) / 256), dim3(256), 0, st, s.keys_a, n,
-                       s.bin_start, s.presence);
-    rec(7);
-    hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, s.vals_a,
-                       s.bounds, n);
-    rec(8);
-    hipLaunchKernelGGL(k_upper_sah, dim3(1), dim3(SAH_BLOCK), 0, st, s);
-    rec(9);
-    hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
-    rec(10);
-    uint32_t max_nodes = 2 * n;
-    hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, s.keys_a, s.vals_a);
-    hipLaunchKernelGGL(k_subtree_big, dim3(64), dim3(256), 0, st, s, s.vals_a);
-    hipLaunchKernelGGL(k_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, s.vals_a, n);
-    const uint32_t max_records = 3 * std::max<uint32_t>(n, 1);
-    hipLaunchKernelGGL(k_bvh4, dim3((max_records + 255) / 256), dim3(256), 0, st, s);
-    rec(11);
     return launch_status();
 }
 

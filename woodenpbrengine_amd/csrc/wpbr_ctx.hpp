@@ -39,6 +39,7 @@ struct wpbr_lbvh {
     bool root_is_leaf = false;
     uint32_t root_word = 0;
     bool built = false;
+    int next_stage = 0;  // staged build: the stage wpbr_lbvh_build_stage accepts next
     wpbr::LbvhScratch s{};
     wpbr::GeomView geom{};
     hipEvent_t ev[NEV] = {};

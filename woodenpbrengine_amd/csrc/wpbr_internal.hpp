@@ -67,8 +67,9 @@ struct LbvhScratch {
     uint32_t* big_count;
 };
 
-wpbr_status lbvh_launch(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st,
-                        hipEvent_t* ev, int nev);
+constexpr int LBVH_STAGES = 6;
+wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
+                              hipEvent_t* ev, int nev);
 
 struct TraceArgs {
     GeomView g;

@@ -1,0 +1,273 @@
+// C++ host mirror of WoodenPBREngine's job-driven hot path over libwpbr_hip.so.
+//
+// The reference runs the path as fixed sequences of ECS jobs, each started by
+// JOB_RUN (construct, set the temp allocator, run, reset; MEngine.cpp:44-50):
+// MEngine::buildLBVH (MEngine.cpp:341-350) and MEngine::runCollisionSystem
+// (MEngine.cpp:352-361).  This header keeps those job names, their order and the
+// component layouts for a C++ host: every Job's run() issues the device work of
+// the reference job through the C ABI in wpbr.h, on the engine's context.
+// Jobs whose work the GPU fuses into a neighbour (the traversal intersects
+// candidates while it walks the tree) are kept as empty steps, so a caller's
+// JOB_RUN sequence stays exactly the reference's.
+//
+// Header-only C++17; link with -lwpbr_hip.  Errors throw wpbr_host::Error
+// (the C ABI itself never throws).
+#pragma once
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "wpbr.h"
+
+namespace wpbr_host {
+
+// ---- component layouts (field order as in the reference) -------------------
+struct DVector3f {
+    float x, y, z;
+};
+using DPoint3f = DVector3f;
+struct DBounds3f {  // WoodenMath bounds: pMin, pMax
+    DPoint3f pMin, pMax;
+};
+struct DRayf {  // CRay.h:8-20 (origin, direction, t)
+    DPoint3f origin;
+    DVector3f dir;
+    float t;
+};
+struct CRayCast {  // CSufraceInteraction.h:36-42
+    DRayf ray;
+    bool bSurfInteraction;
+};
+struct CInteraction {  // CSufraceInteraction.h:16-34
+    DPoint3f p;
+    DVector3f wo;
+    uint32_t hCollision;  // shape entity, INVALID_HANDLE on a miss
+    float time;
+};
+struct LinearBVHNode {  // SLBVH.h:17-27
+    uint32_t offset;    // shapes offset (leaf) or second child offset (interior)
+    uint32_t nShapes;   // 0 for interior nodes
+    uint8_t axis;
+};
+struct CSceneTreeLBVH {  // SLBVH.h:28-40
+    std::vector<LinearBVHNode> nodes;
+    std::vector<DBounds3f> nodesBounds;
+    std::vector<uint32_t> shapesEntities;
+};
+constexpr uint32_t INVALID_HANDLE = 0xFFFFFFFFu;
+
+class Error : public std::runtime_error {
+public:
+    Error(wpbr_status s, const std::string& what) : std::runtime_error(what), status(s) {}
+    wpbr_status status;
+};
+
+class MEngine;
+
+// ---- MEngine::buildLBVH jobs (SLBVH.cpp, CSTriangle.cpp, CSSphere.h) -------
+struct JobUpdateBoundsAndCentroidSphere { void run(MEngine* e); };    // CSSphere.h:126-153
+struct JobUpdateBoundsAndCentroidTriangle { void run(MEngine* e); };  // CSTriangle.cpp:116-142
+struct JobGenerateShapesCentroidBound { void run(MEngine* e); };      // SLBVH.cpp:164-181
+struct JobGenerateShapesMortonCode { void run(MEngine* e); };         // SLBVH.cpp:186-227
+struct JobSortsShapesByMortonCode { void run(MEngine* e); };          // SLBVH.cpp:231-324
+struct JobEmitLBVH { void run(MEngine* e); };                         // SLBVH.cpp:328-406
+struct JobBuildUpperSAH { void run(MEngine* e); };                    // SLBVH.cpp:410-563
+
+// ---- MEngine::runCollisionSystem jobs ----------------------------------------
+struct JobProcessRayCasts { void run(MEngine* e); };                       // SLBVH.cpp:85-162
+struct JobProcessSphereSurfInteractionRequests { void run(MEngine*) {} };   // in the traversal
+struct JobProcessTriangleInteractionRequests { void run(MEngine*) {} };     // in the traversal
+struct JobProcessRayCastsResults { void run(MEngine* e); };                // SLBVH.cpp:9-66
+struct JobProcessSphereFullInteractionRequests { void run(MEngine*) {} };   // render path only
+struct JobProcessTriangleFullInteractionRequests { void run(MEngine*) {} }; // render path only
+struct JobCollisionFinish { void run(MEngine* e); };                       // SLBVH.cpp:72-82
+
+#define WPBR_JOB_RUN(JobT) \
+    {                      \
+        JobT j;            \
+        j.run(this);       \
+    }
+
+class MEngine {
+public:
+    explicit MEngine(int device = 0) {
+        check(wpbr_open(device, &ctx_), "wpbr_open");
+        check(wpbr_lbvh_create(ctx_, &bvh_), "wpbr_lbvh_create");
+    }
+    ~MEngine() {
+        release_shapes();
+        release_rays();
+        if (bvh_) wpbr_lbvh_destroy(bvh_);
+        if (ctx_) wpbr_close(ctx_);
+    }
+    MEngine(const MEngine&) = delete;
+    MEngine& operator=(const MEngine&) = delete;
+
+    // Shapes in the reference's order: spheres (world 3x4 then inverse 3x4,
+    // radius), then triangles (world-space vertices, 9 floats).
+    void setShapes(const std::vector<float>& sphereXform24, const std::vector<float>& sphereRadius,
+                   const std::vector<float>& triVerts9) {
+        release_shapes();
+        geom_.n_spheres = (uint32_t)sphereRadius.size();
+        geom_.n_triangles = (uint32_t)(triVerts9.size() / 9);
+        geom_.d_sphere_xform = upload(sphereXform24, &d_xform_);
+        geom_.d_sphere_radius = upload(sphereRadius, &d_radius_);
+        geom_.d_tri_verts = upload(triVerts9, &d_tris_);
+    }
+
+    // MEngine.cpp:341-350
+    void buildLBVH() {
+        WPBR_JOB_RUN(JobUpdateBoundsAndCentroidSphere)
+        WPBR_JOB_RUN(JobUpdateBoundsAndCentroidTriangle)
+        WPBR_JOB_RUN(JobGenerateShapesCentroidBound)
+        WPBR_JOB_RUN(JobGenerateShapesMortonCode)
+        WPBR_JOB_RUN(JobSortsShapesByMortonCode)
+        WPBR_JOB_RUN(JobEmitLBVH)
+        WPBR_JOB_RUN(JobBuildUpperSAH)
+    }
+
+    // MEngine.cpp:352-361: rayCasts in, interactions out (one per ray cast).
+    void runCollisionSystem() {
+        WPBR_JOB_RUN(JobProcessRayCasts)
+        WPBR_JOB_RUN(JobProcessSphereSurfInteractionRequests)
+        WPBR_JOB_RUN(JobProcessTriangleInteractionRequests)
+        WPBR_JOB_RUN(JobProcessRayCastsResults)
+        WPBR_JOB_RUN(JobProcessSphereFullInteractionRequests)
+        WPBR_JOB_RUN(JobProcessTriangleFullInteractionRequests)
+        WPBR_JOB_RUN(JobCollisionFinish)
+    }
+
+    // The built tree as the reference's CSceneTreeLBVH.
+    CSceneTreeLBVH tree() {
+        uint32_t c[4];
+        check(wpbr_lbvh_counts(bvh_, c), "wpbr_lbvh_counts");
+        const uint32_t n = c[0], nn = c[1];
+        std::vector<uint32_t> off(nn), ns(nn);
+        std::vector<uint8_t> axis(nn);
+        CSceneTreeLBVH t;
+        t.nodesBounds.resize(nn);
+        t.shapesEntities.resize(n);
+        check(wpbr_lbvh_download(ctx_, bvh_, 3, off.data(), 4 * (size_t)nn), "download");
+        check(wpbr_lbvh_download(ctx_, bvh_, 4, ns.data(), 4 * (size_t)nn), "download");
+        check(wpbr_lbvh_download(ctx_, bvh_, 5, axis.data(), nn), "download");
+        check(wpbr_lbvh_download(ctx_, bvh_, 6, t.nodesBounds.data(), 24 * (size_t)nn), "download");
+        check(wpbr_lbvh_download(ctx_, bvh_, 2, t.shapesEntities.data(), 4 * (size_t)n), "download");
+        t.nodes.resize(nn);
+        for (uint32_t i = 0; i < nn; i++) t.nodes[i] = {off[i], ns[i], axis[i]};
+        return t;
+    }
+
+    std::vector<CRayCast> rayCasts;          // input of runCollisionSystem
+    std::vector<CInteraction> interactions;  // its output, one per ray cast
+
+    wpbr_ctx* ctx() const { return ctx_; }
+    wpbr_lbvh* lbvh() const { return bvh_; }
+    const wpbr_geometry& geometry() const { return geom_; }
+
+    // used by the jobs
+    void build_stage(int stage) { check(wpbr_lbvh_build_stage(ctx_, &geom_, bvh_, stage), "wpbr_lbvh_build_stage"); }
+    void trace_rays() {
+        const uint32_t n = (uint32_t)rayCasts.size();
+        std::vector<float> soa(6 * (size_t)n);
+        for (uint32_t i = 0; i < n; i++) {
+            const DRayf& r = rayCasts[i].ray;
+            soa[i] = r.origin.x;
+            soa[n + i] = r.origin.y;
+            soa[2 * (size_t)n + i] = r.origin.z;
+            soa[3 * (size_t)n + i] = r.dir.x;
+            soa[4 * (size_t)n + i] = r.dir.y;
+            soa[5 * (size_t)n + i] = r.dir.z;
+        }
+        ensure_rays(n);
+        if (n) check(wpbr_upload(ctx_, d_rays_, soa.data(), soa.size() * 4), "upload rays");
+        check(wpbr_collide_async(ctx_, &geom_, bvh_, n, (const float*)d_rays_, (uint32_t*)d_hit_,
+                                 (float*)d_t_, nullptr),
+              "wpbr_collide_async");
+        n_traced_ = n;
+    }
+    void fetch_hits() {
+        hit_.resize(n_traced_);
+        t_.resize(n_traced_);
+        check(wpbr_sync(ctx_), "wpbr_sync");
+        if (n_traced_) {
+            check(wpbr_download(ctx_, hit_.data(), d_hit_, 4 * (size_t)n_traced_), "download hits");
+            check(wpbr_download(ctx_, t_.data(), d_t_, 4 * (size_t)n_traced_), "download t");
+        }
+    }
+    // SLBVH.cpp:50-62: a miss keeps INVALID_HANDLE; a hit records p = o + d t,
+    // wo = -d, the shape and t.
+    void finish_interactions() {
+        interactions.resize(n_traced_);
+        for (uint32_t i = 0; i < n_traced_; i++) {
+            const DRayf& r = rayCasts[i].ray;
+            CInteraction& it = interactions[i];
+            it.hCollision = hit_[i];
+            it.time = t_[i];
+            if (hit_[i] == INVALID_HANDLE) {
+                it.p = {0, 0, 0};
+                it.wo = {0, 0, 0};
+                continue;
+            }
+            it.p = {r.origin.x + r.dir.x * t_[i], r.origin.y + r.dir.y * t_[i], r.origin.z + r.dir.z * t_[i]};
+            it.wo = {-r.dir.x, -r.dir.y, -r.dir.z};
+        }
+    }
+
+private:
+    void check(wpbr_status s, const char* what) {
+        if (s != WPBR_OK)
+            throw Error(s, std::string(what) + ": " + wpbr_status_string(s) + " (" +
+                               (ctx_ ? wpbr_last_error(ctx_) : "") + ")");
+    }
+    const float* upload(const std::vector<float>& v, void** d) {
+        if (v.empty()) return nullptr;
+        check(wpbr_malloc(ctx_, v.size() * 4, d), "wpbr_malloc");
+        check(wpbr_upload(ctx_, *d, v.data(), v.size() * 4), "wpbr_upload");
+        return (const float*)*d;
+    }
+    void release_shapes() {
+        for (void** p : {&d_xform_, &d_radius_, &d_tris_})
+            if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
+    }
+    void ensure_rays(uint32_t n) {
+        if (n <= ray_cap_) return;
+        release_rays();
+        check(wpbr_malloc(ctx_, 24 * (size_t)n, &d_rays_), "wpbr_malloc");
+        check(wpbr_malloc(ctx_, 4 * (size_t)n, &d_hit_), "wpbr_malloc");
+        check(wpbr_malloc(ctx_, 4 * (size_t)n, &d_t_), "wpbr_malloc");
+        ray_cap_ = n;
+    }
+    void release_rays() {
+        for (void** p : {&d_rays_, &d_hit_, &d_t_})
+            if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
+        ray_cap_ = 0;
+    }
+
+    wpbr_ctx* ctx_ = nullptr;
+    wpbr_lbvh* bvh_ = nullptr;
+    wpbr_geometry geom_{};
+    void *d_xform_ = nullptr, *d_radius_ = nullptr, *d_tris_ = nullptr;
+    void *d_rays_ = nullptr, *d_hit_ = nullptr, *d_t_ = nullptr;
+    uint32_t ray_cap_ = 0, n_traced_ = 0;
+    std::vector<uint32_t> hit_;
+    std::vector<float> t_;
+};
+
+// Spheres and triangles are bounded by one kernel (stage 0); the sphere job
+// is the empty step before it.
+inline void JobUpdateBoundsAndCentroidSphere::run(MEngine*) {}
+inline void JobUpdateBoundsAndCentroidTriangle::run(MEngine* e) { e->build_stage(0); }
+inline void JobGenerateShapesCentroidBound::run(MEngine* e) { e->build_stage(1); }
+inline void JobGenerateShapesMortonCode::run(MEngine* e) { e->build_stage(2); }
+inline void JobSortsShapesByMortonCode::run(MEngine* e) { e->build_stage(3); }
+inline void JobEmitLBVH::run(MEngine* e) { e->build_stage(4); }
+inline void JobBuildUpperSAH::run(MEngine* e) { e->build_stage(5); }
+inline void JobProcessRayCasts::run(MEngine* e) { e->trace_rays(); }
+inline void JobProcessRayCastsResults::run(MEngine* e) { e->fetch_hits(); }
+inline void JobCollisionFinish::run(MEngine* e) { e->finish_interactions(); }
+
+#undef WPBR_JOB_RUN
+
+}  // namespace wpbr_host

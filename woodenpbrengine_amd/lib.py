@@ -19,6 +19,7 @@ EXPORTS = [
     "wpbr_open", "wpbr_close", "wpbr_status_string", "wpbr_last_error", "wpbr_malloc",
     "wpbr_free", "wpbr_upload", "wpbr_download", "wpbr_sync", "wpbr_last_time_ms",
     "wpbr_shape_bounds", "wpbr_lbvh_create", "wpbr_lbvh_destroy", "wpbr_lbvh_build",
+    "wpbr_lbvh_build_stage",
     "wpbr_lbvh_counts", "wpbr_lbvh_download", "wpbr_lbvh_kernel_times", "wpbr_collide",
     "wpbr_collide_async", "wpbr_set_trace_config", "wpbr_get_trace_config", "wpbr_scene_create",
     "wpbr_scene_destroy",
@@ -92,6 +93,7 @@ def load():
         "wpbr_lbvh_counts": (i32, [vp, C.POINTER(u32)]),
         "wpbr_lbvh_download": (i32, [vp, vp, C.c_int, vp, sz]),
         "wpbr_lbvh_kernel_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
+        "wpbr_lbvh_build_stage": (i32, [vp, C.POINTER(Geometry), vp, C.c_int]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
@@ -232,6 +234,10 @@ class Lbvh:
     def build(self, scene):
         self.ctx.check(self.ctx.lib.wpbr_lbvh_build(self.ctx.h, C.byref(scene.geom), self.h))
         return self.ctx.last_ms()
+
+    def build_stage(self, scene, stage):
+        """One job of MEngine::buildLBVH (0 bounds .. 5 upper SAH)."""
+        self.ctx.check(self.ctx.lib.wpbr_lbvh_build_stage(self.ctx.h, C.byref(scene.geom), self.h, stage))
 
     def counts(self):
         c = (C.c_uint32 * 4)()
