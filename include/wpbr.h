@@ -214,6 +214,32 @@ int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap);
  * and returns it in *best (optional).  d_film receives the last render. */
 wpbr_status wpbr_autotune_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, wpbr_trace_config* best);
 
+/* ---- differentiable render (no counterpart in the reference) -------------- *
+ * An autograd.Function-style pair over the render: wpbr_render is the forward
+ * (film of XYZ-weight sums), wpbr_render_backward the adjoint.
+ * Parameters: per material its alpha (roughness) and its conductor eta and k
+ * as RGB (fromRGB(Reflectance), CMaterialMetal.cpp:46), and the env map's RGB
+ * texels (env_h x env_w x 3).  Gradient layout (floats): alpha[n_mat],
+ * eta_k[n_mat][6] (eta RGB, k RGB), env[env_h * env_w * 3].
+ * counts: [0] n_mat, [1] 6 n_mat, [2] env floats (0 without an env map). */
+wpbr_status wpbr_scene_param_counts(wpbr_scene* scene, uint32_t counts[3]);
+/* Replaces parameters (host arrays; a null array keeps its values). */
+wpbr_status wpbr_scene_set_params(wpbr_ctx* ctx, wpbr_scene* scene, const float* alpha,
+                                  const float* eta_k, const float* env_rgb);
+/* The film's image, rgb = xyzToRGB(sum XYZ w) / sum w (CoefficientSpectrum.h:14-24,
+ * CSFilm.cpp:129-170 before the tonemap), its L2 loss against d_target_rgb
+ * (W*H*3) in *loss and the loss gradient 2 (rgb - target) in d_dL_drgb.  Any
+ * output may be null. */
+wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
+                           const float* d_target_rgb, float* d_dL_drgb, float* d_rgb, double* loss);
+/* Gradient of the loss with respect to every parameter, given the forward's
+ * film and d loss / d rgb per pixel (W*H*3): the sample passes are replayed from
+ * their sample streams (nothing is kept from the forward but the film) and a
+ * hand-written adjoint kernel differentiates each sample's radiance with the
+ * forward's discrete decisions held fixed (visibility, light pick, env cells). */
+wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
+                                 const float* d_dL_drgb, float* d_grad);
+
 #ifdef __cplusplus
 }
 #endif

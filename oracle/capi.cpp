@@ -203,8 +203,17 @@ struct orc_scene_desc {
 
 extern "C" {
 
-void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, uint32_t n_tiles) {
+}  // extern "C"
+
+struct SceneHold {
     RenderScene sc;
+    std::vector<Material> mats;
+    LbvhResult bvh;
+};
+
+static SceneHold* make_scene(const orc_scene_desc* d) {
+    SceneHold* H = new SceneHold();
+    RenderScene& sc = H->sc;
     sc.geom.n_spheres = d->n_spheres;
     sc.geom.sphere_xform = d->sphere_xform;
     sc.geom.sphere_radius = d->sphere_radius;
@@ -215,7 +224,8 @@ void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, u
     sc.tri_normal = d->tri_normal;
     sc.tri_tangent = d->tri_tangent;
     sc.shape_material = d->shape_material;
-    std::vector<Material> mats(d->n_materials);
+    std::vector<Material>& mats = H->mats;
+    mats.resize(d->n_materials);
     for (uint32_t i = 0; i < d->n_materials; i++) {
         const float* m = d->materials + 9 * (size_t)i;
         std::memcpy(&mats[i].type, &m[0], 4);
@@ -245,16 +255,40 @@ void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, u
     sphere_bounds(d->n_spheres, d->sphere_xform, d->sphere_radius, b.data(), c.data());
     triangle_bounds(d->n_triangles, d->tri_verts, b.data() + 6 * (size_t)d->n_spheres,
                     c.data() + 3 * (size_t)d->n_spheres);
-    LbvhResult bvh;
-    lbvh_build(n, b.data(), c.data(), nullptr, bvh);
+    lbvh_build(n, b.data(), c.data(), nullptr, H->bvh);
+    return H;
+}
+
+
+extern "C" {
+
+void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, uint32_t n_tiles) {
+    SceneHold* H = make_scene(d);
     RenderOut* out = new RenderOut();
     auto t0 = std::chrono::steady_clock::now();
-    render(sc, bvh, *out, nthreads, tiles, n_tiles);
+    render(H->sc, H->bvh, *out, nthreads, tiles, n_tiles);
     auto t1 = std::chrono::steady_clock::now();
     // counters[7]: wall time of the render loop alone (ns), excluding the build
     out->counters[7] = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(t1 - t0).count();
+    delete H;
     return out;
 }
+
+// Finite-difference reference: record the base render's decisions, then
+// evaluate the L2 image loss at other parameters (see diff.cpp).
+void* orc_diff_record(const orc_scene_desc* d, int nthreads) {
+    SceneHold* H = make_scene(d);
+    DiffState* D = diff_record(H->sc, H->bvh, nthreads);
+    delete H;
+    return D;
+}
+
+double orc_diff_loss(void* h, const double* alpha, const double* etak, const double* env_rgb,
+                     const float* target_rgb, float* rgb_out, int nthreads) {
+    return diff_loss(*(DiffState*)h, alpha, etak, env_rgb, target_rgb, rgb_out, nthreads);
+}
+
+void orc_diff_free(void* h) { diff_free((DiffState*)h); }
 
 // field: 0 sample_xyz, 1 sample_pfilm, 2 film_rgb, 3 film_xyz, 4 film_w, 5 counters (u64 x 8)
 int orc_render_copy(void* h, int field, void* dst) {

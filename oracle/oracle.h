@@ -140,10 +140,43 @@ struct RenderOut {
     uint64_t counters[8] = {0};      // primary hits, misses, shadow rays, bsdf rays, ...
 };
 
+// Every discrete decision of one sample (the finite-difference reference of
+// diff.cpp re-evaluates the estimator with them frozen).
+struct EnvCell {
+    int32_t s0, t0, s0n, t0n;  // bilinear texels (clamped)
+    int32_t rgb_case;          // fromRGB ordering case (0..5) of the looked-up rgb
+    int32_t iu, iv;            // distribution cell of the pdf
+};
+struct SampleRec {
+    uint8_t kind = 0;       // 0 miss, 1 hit without a contribution, 2 hit with a material
+    uint8_t env_pick = 0;   // the env light was picked
+    uint8_t light_on = 0;   // the light strategy contributes (reached, pdf != 0)
+    uint8_t bsdf_on = 0;    // the BSDF ray reached the picked light (sample not rejected)
+    uint32_t mid = 0, li = 0;
+    float dir[3] = {0, 0, 0};  // primary ray direction (miss)
+    float wo[3], ss[3], ts[3], ns[3];
+    float wl[3];            // light direction (local frame)
+    float pdfL = 0;         // light sample pdf
+    float ub[2] = {0, 0};   // BSDF 2D sample
+    float pdfL2s = 0;       // sphere light pdf in the BSDF strategy
+    float pfilm[2];
+    EnvCell env;            // env lookup of the miss or of the BSDF direction
+};
+
 float sample_1d(uint32_t seed, uint32_t pixel, uint32_t index);
 void sample_2d(uint32_t seed, uint32_t pixel, uint32_t index, float* u);
 void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
-            const uint32_t* tile_subset, uint32_t n_tiles);
+            const uint32_t* tile_subset, uint32_t n_tiles, SampleRec* recs = nullptr);
+// Smits case index (0..5) of rgb, in the order of from_rgb's branches.
+int rgb_case(const float* rgb);
+float mitchell_weight(float x);  // the film's Mitchell filter (CSFilm.h:62-86)
+
+// Finite-difference reference (diff.cpp).
+struct DiffState;
+DiffState* diff_record(const RenderScene& sc, const LbvhResult& bvh, int nthreads);
+void diff_free(DiffState* d);
+double diff_loss(const DiffState& d, const double* alpha, const double* etak, const double* env_rgb,
+                 const float* target_rgb, float* rgb_out, int nthreads);
 void shade_probe(int fn, const float* in, float* out);
 
 }  // namespace orc

@@ -282,3 +282,40 @@ def probe(name, *args):
     out = np.zeros(max(nout, 1), np.float32)
     lib.orc_shade_probe(fn, _p(inp), _p(out))
     return out if nout > 1 else float(out[0])
+
+
+class DiffOracle:
+    """Finite-difference reference (diff.cpp): records the base render's
+    discrete decisions, then evaluates the L2 image loss in double precision at
+    other parameters (alpha per material, eta/k RGB per material, env texels)."""
+
+    def __init__(self, s, nthreads=None):
+        lib = load()
+        lib.orc_diff_record.argtypes = [C.c_void_p, C.c_int]
+        lib.orc_diff_record.restype = C.c_void_p
+        lib.orc_diff_loss.argtypes = [C.c_void_p] + [C.c_void_p] * 5 + [C.c_int]
+        lib.orc_diff_loss.restype = C.c_double
+        lib.orc_diff_free.argtypes = [C.c_void_p]
+        self.lib = lib
+        self.scene = s
+        self.nthreads = nthreads or default_threads()
+        d, keep = scene_desc(s)
+        self.h = lib.orc_diff_record(C.byref(d), self.nthreads)
+        del keep
+
+    def loss(self, alpha, etak, env, target=None, want_rgb=False):
+        a = np.ascontiguousarray(alpha, np.float64)
+        ek = np.ascontiguousarray(etak, np.float64)
+        ev = np.ascontiguousarray(env, np.float64)
+        tg = None if target is None else np.ascontiguousarray(target, np.float32)
+        rgb = np.empty((self.scene.height, self.scene.width, 3), np.float32) if want_rgb else None
+        val = self.lib.orc_diff_loss(self.h, _p(a), _p(ek), _p(ev), _p(tg), _p(rgb), self.nthreads)
+        return (val, rgb) if want_rgb else val
+
+    def close(self):
+        if self.h:
+            self.lib.orc_diff_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

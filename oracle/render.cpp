@@ -431,12 +431,28 @@ static bool trace(const Ctx& C, V3 o, V3 d, uint32_t* shape, float* t) {
     return true;
 }
 
-static Spec env_le(const Ctx& C, V3 dirW, bool* black, float* pdfOut, bool wantPdf) {
+static Spec env_le(const Ctx& C, V3 dirW, bool* black, float* pdfOut, bool wantPdf,
+                   EnvCell* cell = nullptr) {
     const RenderScene& sc = C.sc;
     Affine Inv = load_affine(sc.env_xform + 12);
     V3 dL = xf_vector(Inv, dirW);
     float theta = spherical_theta(dL), phi = spherical_phi(dL);
     float u = (float)((double)phi / (2 * PI_REF)), v = (float)((double)theta / PI_REF);
+    if (cell) {
+        const int W = (int)sc.env_w, H = (int)sc.env_h;
+        int s0 = (int)std::floor(u * (float)W - 0.5f), t0 = (int)std::floor(v * (float)H - 0.5f);
+        s0 = std::min(std::max(s0, 0), W - 1);
+        t0 = std::min(std::max(t0, 0), H - 1);
+        cell->s0 = s0; cell->t0 = t0;
+        cell->s0n = std::max(std::min(s0 + 1, W - 1), 0);
+        cell->t0n = std::max(std::min(t0 + 1, H - 1), 0);
+        int iu = (int)(u * (float)W), iv = (int)(v * (float)H);
+        cell->iu = iu < 0 ? 0 : (iu > W - 1 ? W - 1 : iu);
+        cell->iv = iv < 0 ? 0 : (iv > H - 1 ? H - 1 : iv);
+        float rgbc[3];
+        env_bilerp(sc, u, v, rgbc);
+        cell->rgb_case = rgb_case(rgbc);
+    }
     Spec L;
     L.fill(0.0f);
     *black = true;
@@ -457,7 +473,7 @@ static Spec env_le(const Ctx& C, V3 dirW, bool* black, float* pdfOut, bool wantP
 static float power_heuristic(float f, float g) { return (f * f) / (f * f + g * g); }
 
 static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uint32_t& c2,
-                       float* pFilm, uint64_t* counters) {
+                       float* pFilm, uint64_t* counters, SampleRec* rec) {
     const RenderScene& sc = C.sc;
     const uint32_t pixel = py * sc.width + px;
     Spec L;
@@ -466,6 +482,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     sample_2d(sc.seed, pixel, c2++, u2);
     pFilm[0] = (float)px + u2[0];
     pFilm[1] = (float)py + u2[1];
+    if (rec) { rec->pfilm[0] = pFilm[0]; rec->pfilm[1] = pFilm[1]; }
     V3 pCam = xf_point4(sc.raster_camera, v3(pFilm[0], pFilm[1], 0.0f));
     V3 dC = normalize3(pCam);
     Affine CW = load_affine(sc.camera_world);
@@ -475,12 +492,14 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     float t;
     if (!trace(C, o, d, &shape, &t)) {
         counters[1]++;
+        if (rec) { rec->kind = 0; rec->dir[0] = d.x; rec->dir[1] = d.y; rec->dir[2] = d.z; }
         if (sc.has_env) {
             bool black;
-            L = env_le(C, d, &black, nullptr, false);
+            L = env_le(C, d, &black, nullptr, false, rec ? &rec->env : nullptr);
         }
         return L;
     }
+    if (rec) rec->kind = 1;
     counters[0]++;
     SI si;
     if (shape < sc.geom.n_spheres) {
@@ -511,6 +530,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     }
     uint32_t mid = sc.shape_material ? sc.shape_material[shape] : 0xFFFFFFFFu;
     if (mid == 0xFFFFFFFFu) return L;
+    if (rec) { rec->kind = 2; rec->mid = mid; rec->li = li; rec->env_pick = envPick; rec->pdfL = pdfL; }
     const Material& m = sc.materials[mid];
     const MatSpec& ms = C.mats[mid];
     // BSDF frame (CBSDF.cpp:139-151)
@@ -523,6 +543,13 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
                   (ss.z * w.x + ts.z * w.y) + ns.z * w.z);
     };
     V3 woL = toLocal(si.wo);
+    if (rec) {
+        V3 wl = toLocal(wiL);
+        for (int c = 0; c < 3; c++) {
+            rec->wo[c] = (&woL.x)[c]; rec->ss[c] = (&ss.x)[c]; rec->ts[c] = (&ts.x)[c];
+            rec->ns[c] = (&ns.x)[c]; rec->wl[c] = (&wl.x)[c];
+        }
+    }
     // ---- light strategy (sphere lights only: env Li is black)
     if (!envPick) {
         counters[2]++;
@@ -543,6 +570,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
             bool black = true;
             for (int i = 0; i < NSPEC; i++) if (Le[i] != 0.0f) { black = false; break; }
             if (!black && pdfB != 0.0f) {
+                if (rec) rec->light_on = 1;
                 float w = power_heuristic(pdfL, pdfB);
                 float s = w / pdfL;
                 for (int i = 0; i < NSPEC; i++) L[i] += (Le[i] * val[i]) * s;
@@ -565,6 +593,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     } else {
         float ub[2];
         sample_2d(sc.seed, pixel, c2++, ub);
+        if (rec) { rec->ub[0] = ub[0]; rec->ub[1] = ub[1]; }
         float whD;
         V3 wh = ggx_sample(woL, ub, m.alpha, m.alpha, &whD);
         wiBL = (-woL) + wh * (2 * dot3(woL, wh));
@@ -596,13 +625,14 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     float pdfL2 = 0;
     bool black = true;
     if (envPick) {
-        Li = env_le(C, wiB, &black, &pdfL2, true);
+        Li = env_le(C, wiB, &black, &pdfL2, true, rec ? &rec->env : nullptr);
     } else {
         pdfL2 = sphere_light_pdf(sc, sc.light_shape[li]);
         for (int i = 0; i < NSPEC; i++) Li[i] = C.light_emit[li][i];
         for (int i = 0; i < NSPEC; i++) if (Li[i] != 0.0f) { black = false; break; }
     }
     if (black) return L;
+    if (rec) { rec->bsdf_on = 1; rec->pdfL2s = envPick ? 0.0f : pdfL2; }
     float w = (pdfB == 1.0f) ? 1.0f : power_heuristic(pdfB, pdfL2);
     float s = w / pdfB;
     for (int i = 0; i < NSPEC; i++) L[i] += (Li[i] * valB[i]) * s;
@@ -620,8 +650,10 @@ static float mitchell1(float x) {
            (1.f / 6.f);
 }
 
+float mitchell_weight(float x) { return mitchell1(x); }
+
 void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
-            const uint32_t* tile_subset, uint32_t n_tiles_subset) {
+            const uint32_t* tile_subset, uint32_t n_tiles_subset, SampleRec* recs) {
     Ctx C{sc, bvh, {}, {}, {}, {}};
     if (sc.has_env) {
         build_env_dist(sc, C.env);
@@ -669,8 +701,8 @@ void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nt
                     for (uint32_t px = tx0; px <= tx1; px++) {
                         size_t lp = (size_t)(py - ty0) * tw + (px - tx0);
                         float pf[2];
-                        Spec L = run_sample(C, px, py, c1[lp], c2[lp], pf, cnt);
                         size_t si_ = ((size_t)(py * W + px) * spp + j);
+                        Spec L = run_sample(C, px, py, c1[lp], c2[lp], pf, cnt, recs ? recs + si_ : nullptr);
                         to_xyz(L, &out.sample_xyz[3 * si_]);
                         out.sample_pfilm[2 * si_] = pf[0];
                         out.sample_pfilm[2 * si_ + 1] = pf[1];

@@ -66,6 +66,13 @@ const Tables& tables() {
     return t;
 }
 
+int rgb_case(const float* rgb) {
+    float r = rgb[0], g = rgb[1], b = rgb[2];
+    if (r <= g && r <= b) return g <= b ? 0 : 1;
+    if (g <= r && g <= b) return r <= b ? 2 : 3;
+    return r <= g ? 4 : 5;
+}
+
 // Basis order: 0 white, 1 cyan, 2 magenta, 3 yellow, 4 red, 5 green, 6 blue.
 Spec from_rgb(const float* rgb, bool illuminant) {
     const Tables& T = tables();

@@ -452,6 +452,20 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
                 f3 ns = si.ns;
                 f3 ts = cross3(ns, ss);
                 f3 woL = mk3(dot3(ss, si.wo), dot3(ts, si.wo), dot3(ns, si.wo));
+                if (L.record) {
+                    // the adjoint replays this sample from its frame and samples
+                    f3 wlr = mk3(dot3(ss, wiL), dot3(ts, wiL), dot3(ns, wiL));
+                    const float rec3[5][3] = {{woL.x, woL.y, woL.z}, {ss.x, ss.y, ss.z}, {ts.x, ts.y, ts.z},
+                                              {ns.x, ns.y, ns.z}, {wlr.x, wlr.y, wlr.z}};
+                    for (int c = 0; c < 3; c++) {
+                        ws.r_wo[(size_t)c * n + i] = rec3[0][c];
+                        ws.r_frame[(size_t)c * n + i] = rec3[1][c];
+                        ws.r_frame[(size_t)(3 + c) * n + i] = rec3[2][c];
+                        ws.r_frame[(size_t)(6 + c) * n + i] = rec3[3][c];
+                        ws.r_wl[(size_t)c * n + i] = rec3[4][c];
+                    }
+                    ws.r_pdfl[i] = pdfL;
+                }
                 // light strategy: BSDF Compute toward the light
                 if (!envPick && m.type != MAT_SPECULAR) {
                     f3 wl = mk3(dot3(ss, wiL), dot3(ts, wiL), dot3(ns, wiL));
@@ -480,6 +494,7 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
                 } else {
                     float ub[2];
                     next2d(L.seed, pix, c2, ub);
+                    if (L.record) { ws.r_ub[i] = ub[0]; ws.r_ub[n + i] = ub[1]; }
                     float phi = (float)((2 * PI_REF) * (double)ub[1]);
                     float tan2 = m.alpha * m.alpha * ub[0] / (1.0f - ub[0]);
                     float cosT = 1 / sqrtf(1 + tan2);

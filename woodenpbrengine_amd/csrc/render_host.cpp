@@ -4,17 +4,12 @@
 #include <cmath>
 #include <cstring>
 #include <vector>
-#include "render_types.hpp"
-#include "wpbr_ctx.hpp"
+#include "render_host.hpp"
 #include "spectrum_tables.inc"
 
 using namespace wpbr;
 
 namespace wpbr {
-wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
-}
-
-namespace {
 
 constexpr double PI_REF_H = 3.14;  // pch.h:30
 
@@ -40,12 +35,6 @@ float average_samples(const float* lambda, const float* vals, int n, float l0, f
     }
     return sum / (l1 - l0);
 }
-
-struct HostTables {
-    float xyz[3 * NSPEC];
-    float refl[7 * NSPEC];
-    float illum[7 * NSPEC];
-};
 
 const HostTables& host_tables() {
     static HostTables t = [] {
@@ -141,24 +130,7 @@ void env_distribution(const float* rgbTex, uint32_t W, uint32_t H, std::vector<f
     margInt = c;
 }
 
-}  // namespace
-
-struct wpbr_scene {
-    wpbr_ctx* ctx = nullptr;
-    std::vector<void*> bufs;
-    SceneView sv{};
-    wpbr_geometry geom{};
-    wpbr_lbvh* bvh = nullptr;
-    uint32_t spp = 1, seed = 0;
-};
-
-struct wpbr_render_state {
-    void* mem = nullptr;
-    uint32_t cap = 0;
-    WaveState ws{};
-    std::vector<hipEvent_t> evs;
-    double stats[8] = {0};
-};
+}  // namespace wpbr
 
 namespace wpbr {
 
@@ -170,13 +142,13 @@ void ctx_release_render(wpbr_ctx* ctx) {
     ctx->render = nullptr;
 }
 
-static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
+wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     if (!ctx->render) ctx->render = new wpbr_render_state();
     wpbr_render_state* r = ctx->render;
     if (n <= r->cap) return WPBR_OK;
     if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
     // words per pixel: see the carve below
-    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2 + 2;
+    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2 + 2 + 18;
     size_t bytes = (words * (size_t)n + 64) * 4 + 64 * 256;
     wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
     if (s) return s;
@@ -220,6 +192,11 @@ static wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     w.l_ray = (uint32_t*)take(n);
     w.b_ray = (uint32_t*)take(n);
     w.qtarget = (uint32_t*)take(2 * (size_t)n);
+    w.r_wo = (float*)take(3 * (size_t)n);
+    w.r_frame = (float*)take(9 * (size_t)n);
+    w.r_wl = (float*)take(3 * (size_t)n);
+    w.r_pdfl = (float*)take(n);
+    w.r_ub = (float*)take(2 * (size_t)n);
     if (off > bytes) return set_err(ctx, WPBR_ERR_HIP, "render state carve overflow");
     return WPBR_OK;
 }
@@ -292,6 +269,9 @@ wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scen
     }
     v.materials = (const DevMaterial*)up(mats.data(), sizeof(DevMaterial) * mats.size());
     v.mat_spec = (const float*)up(specs.data(), 4 * specs.size());
+    sc->d_materials = const_cast<DevMaterial*>(v.materials);
+    sc->d_mat_spec = const_cast<float*>(v.mat_spec);
+    sc->h_materials.assign(d->materials, d->materials + 9 * (size_t)d->n_materials);
     v.n_sphere_lights = d->n_sphere_lights;
     v.light_shape = (const uint32_t*)up(d->light_shape, 4 * (size_t)d->n_sphere_lights);
     v.light_emit = (const float*)up(d->light_emit, 4 * NSPEC * (size_t)d->n_sphere_lights);
@@ -308,6 +288,11 @@ wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scen
         v.env_cond_func = (const float*)up(func.data(), 4 * func.size());
         v.env_cond_int = (const float*)up(rowInt.data(), 4 * rowInt.size());
         v.env_marg_int = (const float*)up(&margInt, 4);
+        sc->d_env_rgb = const_cast<float*>(v.env_rgb);
+        sc->d_env_func = const_cast<float*>(v.env_cond_func);
+        sc->d_env_rowint = const_cast<float*>(v.env_cond_int);
+        sc->d_env_margint = const_cast<float*>(v.env_marg_int);
+        sc->h_env.assign(d->env_rgb, d->env_rgb + 3 * (size_t)d->env_w * d->env_h);
     }
     const HostTables& T = host_tables();
     v.xyz = (const float*)up(T.xyz, sizeof(T.xyz));
@@ -365,6 +350,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     L.seed = sc->seed;
     L.film = d_film;
     L.sample_xyz = d_sample_xyz;
+    L.record = 0;
     const wpbr_lbvh* b = sc->bvh;
     TraceArgs ta;
     ta.g = sc->sv.g;

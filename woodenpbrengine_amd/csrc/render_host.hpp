@@ -1,0 +1,57 @@
+// Host-side scene and render state shared by the forward renderer
+// (render_host.cpp) and the adjoint driver (adjoint_host.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "render_types.hpp"
+#include "wpbr_ctx.hpp"
+
+namespace wpbr {
+
+wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
+wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n);
+
+struct HostTables {
+    float xyz[3 * NSPEC];
+    float refl[7 * NSPEC];
+    float illum[7 * NSPEC];
+};
+const HostTables& host_tables();
+// fromRGB(rgb, Reflectance) (CSpectrumSampled.h:147-199).
+void from_rgb_refl(const float* rgb, float* s);
+// Bilinear mip-0 lookup (CSTextureProcess.h:58-81), indices clamped to the texture.
+void env_bilerp_h(const float* rgbTex, int W, int H, float s_, float t_, float* out);
+// The env light's sampling distribution (CLightInfiniteArea.cpp:113-139).
+void env_distribution(const float* rgbTex, uint32_t W, uint32_t H, std::vector<float>& func,
+                      std::vector<float>& rowInt, float& margInt);
+
+}  // namespace wpbr
+
+struct wpbr_scene {
+    wpbr_ctx* ctx = nullptr;
+    std::vector<void*> bufs;
+    wpbr::SceneView sv{};
+    wpbr_geometry geom{};
+    wpbr_lbvh* bvh = nullptr;
+    uint32_t spp = 1, seed = 0;
+    // parameters (host copies) and the writable buffers behind sv's pointers
+    std::vector<float> h_materials;  // n_materials x 9, the descriptor's layout
+    std::vector<float> h_env;        // env_h x env_w x 3
+    wpbr::DevMaterial* d_materials = nullptr;
+    float* d_mat_spec = nullptr;
+    float* d_env_rgb = nullptr;
+    float* d_env_func = nullptr;
+    float* d_env_rowint = nullptr;
+    float* d_env_margint = nullptr;
+    wpbr::AdjView adj{};
+    bool adj_ready = false;
+};
+
+struct wpbr_render_state {
+    void* mem = nullptr;
+    uint32_t cap = 0;
+    wpbr::WaveState ws{};
+    std::vector<hipEvent_t> evs;
+    double stats[8] = {0};
+};
+

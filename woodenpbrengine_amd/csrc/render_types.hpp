@@ -77,6 +77,24 @@ struct WaveState {
     uint32_t* l_ray;      // queue slot of the light ray per sample
     uint32_t* b_ray;      // queue slot of the BSDF ray per sample
     uint32_t* qtarget;    // 2N: light shape a queue ray must reach (0xFFFFFFFF: escape to the env)
+    // shade1 record for the adjoint's path replay (written when RenderLaunch::record)
+    float* r_wo;          // 3 x N: wo in the shading frame
+    float* r_frame;       // 9 x N: ss, ts, ns (world)
+    float* r_wl;          // 3 x N: light sample direction in the shading frame
+    float* r_pdfl;        // N: light sample pdf
+    float* r_ub;          // 2 x N: the BSDF sample's 2D sample
+};
+
+// Device tables of the adjoint (derivatives with respect to the parameters).
+struct AdjView {
+    uint32_t n_mat;
+    const float* refl;      // 7 x 64 Smits reflectance bases
+    const float* mat_rgb;   // n_mat x 6: conductor eta RGB, k RGB
+    const int* mat_case;    // n_mat x 2: fromRGB case of eta and of k
+    const int* env_src;     // env cells x 4: texels of the bilinear lookup behind a
+                            // distribution cell (-1: the cell is never written)
+    const float* env_srcw;  // env cells x 4: their weights times sin(theta) of the row
+    const float* env_dI;    // texels: derivative of the marginal integral by the green channel
 };
 
 struct RenderLaunch {
@@ -88,6 +106,17 @@ struct RenderLaunch {
     uint32_t seed;
     float* film;          // W*H*4: sum X w, sum Y w, sum Z w, sum w
     float* sample_xyz;    // optional: (pixel*spp + j) x 3
+    uint32_t record;      // shade1 writes the adjoint's replay record
+};
+
+// Arguments of the adjoint kernel.
+struct AdjArgs {
+    RenderLaunch L;
+    AdjView A;
+    const float4* film;  // forward film: sum XYZ w, sum w
+    const float* dL;     // W*H*3: d loss / d rgb
+    float* part;         // per-block rows of 7 n_mat + 1: alpha, eta rgb, k rgb, d/dI
+    float* grad_env;     // env_h*env_w*3
 };
 
 }  // namespace wpbr

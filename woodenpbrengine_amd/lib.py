@@ -24,7 +24,8 @@ EXPORTS = [
     "wpbr_collide_async", "wpbr_set_trace_config", "wpbr_get_trace_config", "wpbr_scene_create",
     "wpbr_scene_destroy",
     "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats", "wpbr_autotune_collide",
-    "wpbr_autotune_render",
+    "wpbr_autotune_render", "wpbr_scene_param_counts", "wpbr_scene_set_params", "wpbr_film_loss",
+    "wpbr_render_backward",
 ]
 
 
@@ -106,6 +107,10 @@ def load():
         "wpbr_autotune_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp,
                                         C.POINTER(TraceConfig)]),
         "wpbr_autotune_render": (i32, [vp, vp, vp, C.POINTER(TraceConfig)]),
+        "wpbr_scene_param_counts": (i32, [vp, C.POINTER(C.c_uint32)]),
+        "wpbr_scene_set_params": (i32, [vp, vp, vp, vp, vp]),
+        "wpbr_film_loss": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]),
+        "wpbr_render_backward": (i32, [vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -356,3 +361,56 @@ class Renderer:
         if self.h:
             self.ctx.lib.wpbr_scene_destroy(self.h)
             self.h = None
+
+
+class DiffRenderer(Renderer):
+    """Differentiable render in the style of an autograd.Function: forward()
+    renders the film, loss() compares its image with a target, backward() returns
+    the gradient for every parameter (alpha per material, eta/k RGB per
+    material, env texels)."""
+
+    def __init__(self, ctx, scene):
+        super().__init__(ctx, scene)
+        c = (C.c_uint32 * 3)()
+        ctx.check(ctx.lib.wpbr_scene_param_counts(self.h, c))
+        self.n_mat, self.n_etak, self.n_env = int(c[0]), int(c[1]), int(c[2])
+        self.d_grad = ctx.empty(self.n_mat + self.n_etak + self.n_env, np.float32)
+        self.d_dL = ctx.empty((self.H, self.W, 3), np.float32)
+        self.d_target = None
+
+    def params(self):
+        """The scene description's parameters: alpha, eta_k (n x 6), env (h x w x 3)."""
+        m = np.asarray(self.scene.materials, np.float32).reshape(-1, 9)
+        env = np.asarray(self.scene.env_rgb, np.float32) if self.n_env else None
+        return m[:, 1].copy(), m[:, 3:9].copy(), env
+
+    def set_params(self, alpha=None, eta_k=None, env=None):
+        arr = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (alpha, eta_k, env)]
+        self.ctx.check(self.ctx.lib.wpbr_scene_set_params(self.ctx.h, self.h, *[_ptr(a) for a in arr]))
+
+    def forward(self):
+        return self.render()
+
+    def set_target(self, rgb):
+        if self.d_target is None:
+            self.d_target = self.ctx.empty((self.H, self.W, 3), np.float32)
+        self.d_target.upload(rgb)
+
+    def loss(self):
+        """L2 loss of the film's image against the target; also fills dL/drgb."""
+        out = C.c_double()
+        self.ctx.check(self.ctx.lib.wpbr_film_loss(self.ctx.h, self.h, self.d_film.ptr,
+                                                   self.d_target.ptr, self.d_dL.ptr, None,
+                                                   C.byref(out)))
+        return out.value
+
+    def backward(self):
+        self.ctx.check(self.ctx.lib.wpbr_render_backward(self.ctx.h, self.h, self.d_film.ptr,
+                                                         self.d_dL.ptr, self.d_grad.ptr))
+        return self.ctx.last_ms()
+
+    def gradients(self):
+        g = self.d_grad.numpy()
+        a, e = self.n_mat, self.n_etak
+        env = g[a + e:].reshape(np.asarray(self.scene.env_rgb).shape) if self.n_env else None
+        return g[:a], g[a:a + e].reshape(-1, 6), env

@@ -398,3 +398,9 @@ def c2_scene(width=1024, height=1024, spp=16, seed=SEED_BASE + 2, freq=317, n_sp
     s.cam = camera(width, height, (0.0, 1.2, -5.2), (0.0, -0.3, 0.0))
     s.width, s.height, s.spp, s.seed = width, height, spp, seed
     return s.finalize()
+
+
+def c4_scene(width=512, height=512, spp=8, seed=SEED_BASE + 2, freq=317):
+    """Differentiable render: the full render's scene at 512x512, 8 spp, with a
+    256x128 env map (its texels are parameters)."""
+    return c2_scene(width=width, height=height, spp=spp, seed=seed, freq=freq, env_size=(256, 128))
