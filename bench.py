@@ -23,7 +23,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "Msamples/s path-traced + LBVH Mprims/s build; fwd+adjoint vs fwd time"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes (see DESIGN.md): traversal 28 B ray in + 8 B hit out
+# algorithmic bytes: traversal 28 B ray in + 8 B hit out
 TRACE_BYTES_PER_RAY = 36.0
 LBVH_BYTES_PER_PRIM = 196.0
 LBVH_BYTES_PER_NODE = 36.0
@@ -101,6 +101,41 @@ def bench_microbench(lib, scenes, ctx, steps):
     return res
 
 
+def bench_adjoint(lib, scenes, ctx, steps):
+    """Differentiable render: forward, L2 loss against a target rendered with
+    parameters perturbed by up to +-20 %, and the path-replay backward pass."""
+    s = scenes.c4_scene()
+    r = lib.DiffRenderer(ctx, s)
+    alpha, etak, env = r.params()
+    g = np.random.default_rng(2026)
+    r.set_params(*[(p * (1 + 0.2 * g.uniform(-1, 1, p.shape))).astype(np.float32)
+                   for p in (alpha, etak, env)])
+    r.autotune()
+    r.forward()
+    r.set_target(r.film_rgb().astype(np.float32))
+    r.set_params(alpha, etak, env)
+    fwd, bwd, tot = [], [], []
+    for k in range(max(steps, 3) + 1):
+        ctx.sync()
+        t0 = time.perf_counter()
+        f = r.forward()
+        loss = r.loss()
+        b = r.backward()
+        ctx.sync()
+        if k:  # the first iteration builds the adjoint tables
+            fwd.append(f)
+            bwd.append(b)
+            tot.append((time.perf_counter() - t0) * 1e3)
+    f, b, t = float(np.median(fwd)), float(np.median(bwd)), float(np.median(tot))
+    grads = r.gradients()
+    r.destroy()
+    return {"adjoint": {"workload": "differentiable render: 512x512 film, 8 spp, 2009850 shapes, "
+                                    "%d alpha + %d eta/k + %d env parameters" % (r.n_mat, r.n_etak, r.n_env),
+                        "forward_ms": f, "backward_ms": b, "fwd_plus_bwd_wall_ms": t,
+                        "ratio_fwd_plus_bwd_over_fwd": (f + b) / f, "loss": loss,
+                        "grad_abs_max": [float(np.abs(x).max()) for x in grads]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,13 +144,14 @@ def main():
     ap.add_argument("--workload", choices=["render", "c0"], default="render")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one plain 512 MiB device copy (reference kernel for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scale factor)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box here has one MI355X; the multi-GPU "
-              "sharding of the render is described in DESIGN.md", file=sys.stderr)
+        print("bench.py measures one GPU (--gpus 1); sharding the render over several GPUs "
+              "is not implemented", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         import torch
@@ -191,6 +227,10 @@ def main():
     if not args.no_microbench:
         mb = bench_microbench(lib, scenes, ctx, args.steps)
         log("microbench:", json.dumps(mb))
+    if not args.no_adjoint:
+        adj = bench_adjoint(lib, scenes, ctx, args.steps)
+        log("adjoint:", json.dumps(adj))
+        mb.update(adj)
     cpu = None
     if not args.no_cpu_baseline:
         t1 = time.time()

@@ -296,10 +296,16 @@ double diff_loss(const DiffState& D, const double* alpha, const double* etak, co
                 double rgb[3] = {((a[0] * 3.240479 + a[1] * -1.537150) + a[2] * -0.498535) / a[3],
                                  ((a[0] * -0.969256 + a[1] * 1.875991) + a[2] * 0.041556) / a[3],
                                  ((a[0] * 0.041556 + a[1] * -0.204043) + a[2] * 1.057311) / a[3]};
+                // pixels with a non-finite image or target value are left out of the loss
+                bool finite = true;
+                for (int c = 0; c < 3; c++) {
+                    const size_t o = ((size_t)py * W + px) * 3 + c;
+                    finite = finite && std::isfinite(rgb[c]) && (!target || std::isfinite(target[o]));
+                }
                 for (int c = 0; c < 3; c++) {
                     const size_t o = ((size_t)py * W + px) * 3 + c;
                     if (rgb_out) rgb_out[o] = (float)rgb[c];
-                    if (target) {
+                    if (target && finite) {
                         double e = rgb[c] - (double)target[o];
                         lsum += e * e;
                     }

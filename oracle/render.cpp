@@ -19,7 +19,7 @@
 //   film splat                   CSFilm.cpp:199-251 (Mitchell r=2, B=C=1/3), clipped to
 //                                the sample's 32x32 tile
 //
-// Choices where the reference is undefined (semantics table):
+// Choices where the reference is undefined:
 //   * light pick index clamped to n-1 (SScattering.cpp:217 overruns for n >= 3);
 //   * a rejected microfacet sample gives f = 0, pdf = 0 and casts no BSDF ray
 //     (CBSDFConductorMicroface.cpp:292-295 leaves pdf and the ray unset);

@@ -3,7 +3,14 @@ oracle: the oracle re-evaluates the estimator in double precision with the
 base render's discrete decisions frozen (oracle/diff.cpp), central differences
 with h = 1e-4; gradients of the L2 image loss against a target rendered with
 parameters perturbed by up to +-20 % (seeded).  Bound: 2 % relative error per
-parameter (plus an absolute floor of 1e-3 of the family's largest gradient)."""
+parameter (plus an absolute floor of 1e-3 of the family's largest gradient).
+
+The estimator is piecewise smooth: a roughness change moves a sampled half
+vector, and a few samples per million sit within 1e-4 of a rejection boundary
+or of a grazing singularity, where the h = 1e-4 difference measures the jump
+rather than the slope.  Such a difference is recognised by disagreeing with the
+h = 1e-5 one by more than the bound; the check then uses h = 1e-5, which must
+itself agree with h = 1e-6 (converged), and the test reports it."""
 import numpy as np
 import pytest
 
@@ -31,8 +38,11 @@ def _setup(ctx, width=128, height=128, spp=2, freq=40):
     return s, r, (alpha, etak, env), target
 
 
-def test_adjoint_matches_finite_differences(ctx):
-    s, r, (alpha, etak, env), target = _setup(ctx)
+@pytest.mark.parametrize("size", ["128x128x2", "512x512x8"])
+def test_adjoint_matches_finite_differences(ctx, size):
+    # 512x512x8 is the differentiable-render workload itself (2009850 shapes)
+    w, h, spp = map(int, size.split("x"))
+    s, r, (alpha, etak, env), target = _setup(ctx, w, h, spp, 40 if w == 128 else 317)
     print("scene", s.n_shapes, "shapes", flush=True)
     r.forward()
     r.set_target(target)
@@ -43,7 +53,9 @@ def test_adjoint_matches_finite_differences(ctx):
     base = [alpha.astype(np.float64), etak.astype(np.float64), env.astype(np.float64)]
     loss_o = D.loss(*base, target)
     print("loss gpu %.6g oracle %.6g" % (loss, loss_o), flush=True)
-    assert loss == pytest.approx(loss_o, rel=1e-3)
+    assert np.isfinite(loss) and loss == pytest.approx(loss_o, rel=1e-3)
+    for x in (ga, ge, genv):
+        assert np.all(np.isfinite(x))
     types = np.asarray(s.materials, np.float32).reshape(-1, 9)[:, 0].view(np.uint32)
     # only conductors read eta/k as RGB
     assert np.all(ge[types != scenes.MAT_CONDUCTOR] == 0)
@@ -55,19 +67,59 @@ def test_adjoint_matches_finite_differences(ctx):
         for k in g.choice(live, size=min(n, len(live)), replace=False):
             picks.append((fam, int(k), float(grad[k]), float(np.abs(grad).max())))
     assert len(picks) >= 24
-    bad = []
-    for fam, k, adj, gmax in picks:
-        fd = []
+    def central(fam, k, h):
+        out = []
         for sign in (1, -1):
             p = [b.copy() for b in base]
             arr = {"alpha": p[0], "eta_k": p[1].reshape(-1), "env": p[2].reshape(-1)}[fam]
-            arr[k] += sign * H_FD
-            fd.append(D.loss(*p, target))
-        fd = (fd[0] - fd[1]) / (2 * H_FD)
-        err = abs(adj - fd)
-        ok = err <= REL_TOL * abs(fd) + 1e-3 * gmax
-        print("%-6s %6d adjoint %+.6e fd %+.6e rel %.2e %s" % (fam, k, adj, fd, err / max(abs(fd), 1e-30),
-                                                             "" if ok else "<-- out of bound"), flush=True)
+            arr[k] += sign * h
+            out.append(D.loss(*p, target))
+        return (out[0] - out[1]) / (2 * h)
+
+    bad, refined = [], []
+    for fam, k, adj, gmax in picks:
+        def close(a, b):
+            return abs(a - b) <= REL_TOL * abs(b) + 1e-3 * gmax
+        fd = central(fam, k, H_FD)
+        note = ""
+        if not close(adj, fd):
+            fd5 = central(fam, k, H_FD / 10)
+            if not close(fd, fd5) and close(fd5, central(fam, k, H_FD / 100)):
+                note = "(h=1e-4 gave %+.6e: jump inside the step)" % fd
+                refined.append((fam, k))
+                fd = fd5
+        ok = close(adj, fd)
+        print("%-6s %6d adjoint %+.6e fd %+.6e rel %.2e %s %s" % (
+            fam, k, adj, fd, abs(adj - fd) / max(abs(fd), 1e-30), note,
+            "" if ok else "<-- out of bound"), flush=True)
         if not ok:
             bad.append((fam, k, adj, fd))
+    print("checked %d parameters, %d against h=1e-5" % (len(picks), len(refined)), flush=True)
     assert not bad, bad
+    assert len(refined) <= len(picks) // 8
+
+
+@pytest.mark.parametrize("size", ["128x128x2", "512x512x8"])
+def test_env_gradient_matches_the_uniform_scale_derivative(ctx, size):
+    # Env radiance enters the estimator linearly and its sampling pdf is
+    # homogeneous of degree 0 in the texels, so with I(c) the image at env * c,
+    # I(c) = A + c B and dL/dc at c = 1 is sum 2 (I - T) B with B = I(2) - I(1),
+    # from two forward renders.  The adjoint gives the same derivative as
+    # sum env . dL/denv; the pdf terms of that sum cancel (Euler).
+    w, h, spp = map(int, size.split("x"))
+    s, r, (alpha, etak, env), target = _setup(ctx, w, h, spp, 40 if w == 128 else 317)
+    r.forward()
+    r.set_target(target)
+    loss = r.loss()
+    r.backward()
+    genv = r.gradients()[2]
+    i1 = r.film_rgb()
+    r.set_params(env=env * 2)
+    r.forward()
+    i2 = r.film_rgb()
+    r.set_params(env=env)
+    ok = np.isfinite(i1).all(-1) & np.isfinite(i2).all(-1) & np.isfinite(target).all(-1)
+    dldc = float(np.sum((2 * (i1 - target) * (i2 - i1))[ok]))
+    adj = float(np.sum(env.astype(np.float64) * genv))
+    print("loss %.6g dL/dc forward %.8e adjoint %.8e" % (loss, dldc, adj), flush=True)
+    assert adj == pytest.approx(dldc, rel=2e-3)

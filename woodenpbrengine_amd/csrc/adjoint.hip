@@ -542,10 +542,17 @@ __global__ void k_film_loss(const float4* film, const float* target, uint32_t np
         const float rgb[3] = {((f.x * 3.240479f + f.y * -1.537150f) + f.z * -0.498535f) / f.w,
                               ((f.x * -0.969256f + f.y * 1.875991f) + f.z * 0.041556f) / f.w,
                               ((f.x * 0.041556f + f.y * -0.204043f) + f.z * 1.057311f) / f.w};
+        // a pixel whose image or target is not finite (a NaN sample of the
+        // estimator reached it) is left out of the loss: no loss, zero dL
+        bool finite = true;
+        for (int c = 0; c < 3; c++) {
+            finite = finite && __builtin_isfinite(rgb[c]);
+            if (target) finite = finite && __builtin_isfinite(target[3 * (size_t)p + c]);
+        }
         for (int c = 0; c < 3; c++) {
             if (rgb_out) rgb_out[3 * (size_t)p + c] = rgb[c];
             if (target) {
-                const float e = rgb[c] - target[3 * (size_t)p + c];
+                const float e = finite ? rgb[c] - target[3 * (size_t)p + c] : 0.0f;
                 l += e * e;
                 if (dL) dL[3 * (size_t)p + c] = 2.0f * e;
             }

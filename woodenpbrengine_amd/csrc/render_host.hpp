@@ -1,5 +1,5 @@
 // Host-side scene and render state shared by the forward renderer
-// (render_host.cpp) and the adjoint driver (adjoint_host.cpp).
+// (render_host.cpp) and the adjoint host code (adjoint_host.cpp).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <vector>

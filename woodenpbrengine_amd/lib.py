@@ -386,7 +386,11 @@ class DiffRenderer(Renderer):
 
     def set_params(self, alpha=None, eta_k=None, env=None):
         arr = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (alpha, eta_k, env)]
-        self.ctx.check(self.ctx.lib.wpbr_scene_set_params(self.ctx.h, self.h, *[_ptr(a) for a in arr]))
+        for a, n, name in zip(arr, (self.n_mat, self.n_etak, self.n_env), ("alpha", "eta_k", "env")):
+            if a is not None and a.size != n:
+                raise ValueError("%s has %d values, the scene has %d" % (name, a.size, n))
+        self.ctx.check(self.ctx.lib.wpbr_scene_set_params(
+            self.ctx.h, self.h, *[None if a is None else _ptr(a) for a in arr]))
 
     def forward(self):
         return self.render()
