@@ -240,6 +240,17 @@ wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film
 wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
                                  const float* d_dL_drgb, float* d_grad);
 
+/* The backward pass's derivatives of the conductor microfacet BSDF
+ * (CBSDFConductorMicroface.cpp:11-82 Compute/Sample, TR distribution
+ * MicrofaceDistr.h:12-90, FrConductor CBXDF.h:200-238) for n requests of one
+ * wavelength, evaluated by the same device code as wpbr_render_backward.
+ * d_in: 11 x n floats SoA: wo xyz, wi xyz (shading frame), alpha, eta, k, u0, u1.
+ * d_out: 13 x n floats SoA: D, dD/dalpha, pdf, dpdf/dalpha, f, df/dalpha
+ * (f = Fr D / (4 |cos_i| |cos_o|) |cos_i|), Fr, dFr/deta, dFr/dk (Fr at wi.wh),
+ * then for the half vector sampled from (u0, u1): wh_z, dwh_z/dalpha, and its
+ * reflected direction: wi_z, dwi_z/dalpha.  No counterpart in the reference. */
+wpbr_status wpbr_bsdf_derivatives(wpbr_ctx* ctx, uint32_t n, const float* d_in, float* d_out);
+
 #ifdef __cplusplus
 }
 #endif

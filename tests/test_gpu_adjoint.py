@@ -123,3 +123,84 @@ def test_env_gradient_matches_the_uniform_scale_derivative(ctx, size):
     adj = float(np.sum(env.astype(np.float64) * genv))
     print("loss %.6g dL/dc forward %.8e adjoint %.8e" % (loss, dldc, adj), flush=True)
     assert adj == pytest.approx(dldc, rel=2e-3)
+
+
+def _fr_conductor_complex(c, eta, k):
+    """FrConductor from the complex index z = eta + i k and its partials by eta
+    and k (holomorphic in z: d/deta = d/dz, d/dk = i d/dz)."""
+    z = eta + 1j * k
+    s2 = 1 - c * c
+    w = np.sqrt(z * z - s2)
+    dw = z / w
+    g = (c - w) / (c + w)
+    dg = -2 * c * dw / (c + w) ** 2
+    p = (z * z * c - w) / (z * z * c + w)
+    dp = (4 * z * c * w - 2 * z * z * c * dw) / (z * z * c + w) ** 2
+    F = 0.5 * (abs(g) ** 2 + abs(p) ** 2)
+    dz = np.conj(g) * dg + np.conj(p) * dp
+    return F, np.real(dz), np.real(1j * dz)
+
+
+def test_bsdf_derivatives_match_the_analytic_forms(ctx):
+    g = np.random.default_rng(11)
+    n = 4096
+
+    def hemi(cmin):
+        v = g.normal(size=(n, 3))
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        v[:, 2] = np.abs(v[:, 2])
+        v = v[v[:, 2] > cmin]
+        return v
+
+    wo, wi = hemi(0.15), hemi(0.15)
+    m = min(len(wo), len(wi))
+    wo, wi = wo[:m].astype(np.float32), wi[:m].astype(np.float32)
+    alpha = g.uniform(0.05, 0.6, m).astype(np.float32)
+    eta = g.uniform(0.2, 2.0, m).astype(np.float32)
+    k = g.uniform(1.0, 5.0, m).astype(np.float32)
+    u = g.uniform(0.05, 0.95, (m, 2)).astype(np.float32)
+    out = lib.bsdf_derivatives(ctx, wo, wi, alpha, eta, k, u)
+    # analytic forms in float64 from the float32 inputs
+    wo64, wi64, a = wo.astype(np.float64), wi.astype(np.float64), alpha.astype(np.float64)
+    h = wo64 + wi64
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    c = h[:, 2]
+    t2 = (1 - c * c) / (c * c)
+    e = t2 / a ** 2
+    D = 1 / (3.14 * a ** 2 * c ** 4 * (1 + e) ** 2)  # TR with PI = 3.14
+    dD = D * (-2 / a + 4 * t2 / (a ** 3 * (1 + e)))
+    woh = np.sum(wo64 * h, axis=1)
+    cI, cO = np.abs(wi64[:, 2]), np.abs(wo64[:, 2])
+    e64, k64 = eta.astype(np.float64), k.astype(np.float64)
+    F, dFe, dFk = _fr_conductor_complex(np.sum(wi64 * h, axis=1), e64, k64)
+    exp = {"D": D, "dD_dalpha": dD, "pdf": D * np.abs(c) / (4 * woh), "dpdf_dalpha": dD * np.abs(c) / (4 * woh),
+           "f": F * D / (4 * cO), "df_dalpha": F * dD / (4 * cO), "Fr": F, "dFr_deta": dFe, "dFr_dk": dFk}
+    r = u[:, 0].astype(np.float64) / (1 - u[:, 0].astype(np.float64))
+    cs = 1 / np.sqrt(1 + a * a * r)
+    dcs = -a * r * (1 + a * a * r) ** -1.5
+    sn = np.sqrt(1 - cs * cs)
+    dsn = -cs * dcs / sn
+    phi = 2 * 3.14 * u[:, 1].astype(np.float64)
+    hs = np.stack([sn * np.cos(phi), sn * np.sin(phi), cs], 1)
+    dhs = np.stack([dsn * np.cos(phi), dsn * np.sin(phi), dcs], 1)
+    wdh = np.sum(wo64 * hs, 1)
+    exp["wh_z"], exp["dwh_z_dalpha"] = cs, dcs
+    exp["wi_z"] = 2 * wdh * cs - wo64[:, 2]
+    exp["dwi_z_dalpha"] = 2 * np.sum(wo64 * dhs, 1) * cs + 2 * wdh * dcs
+    # the derivative forms against a central difference of the closed forms
+    # themselves, so that a wrong closed form fails here rather than on the GPU
+    hh = 1e-6
+    for de, dk, ref in ((hh, 0, dFe), (0, hh, dFk)):
+        Fp = _fr_conductor_complex(np.sum(wi64 * h, axis=1), e64 + de, k64 + dk)[0]
+        Fm = _fr_conductor_complex(np.sum(wi64 * h, axis=1), e64 - de, k64 - dk)[0]
+        np.testing.assert_allclose((Fp - Fm) / (2 * hh), ref, rtol=1e-5, atol=1e-8)
+    Dp = 1 / (3.14 * (a + hh) ** 2 * c ** 4 * (1 + t2 / (a + hh) ** 2) ** 2)
+    Dm = 1 / (3.14 * (a - hh) ** 2 * c ** 4 * (1 + t2 / (a - hh) ** 2) ** 2)
+    np.testing.assert_allclose((Dp - Dm) / (2 * hh), dD, rtol=1e-5, atol=1e-6 * np.abs(dD).max())
+    csp, csm = 1 / np.sqrt(1 + (a + hh) ** 2 * r), 1 / np.sqrt(1 + (a - hh) ** 2 * r)
+    np.testing.assert_allclose((csp - csm) / (2 * hh), dcs, rtol=1e-5, atol=1e-9)
+    for f, v in exp.items():
+        # relative error, with a floor of 1e-4 of the field's largest magnitude
+        err = np.abs(out[f] - v) / np.maximum(np.abs(v), 1e-4 * np.abs(v).max())
+        print("%-14s max rel err %.2e" % (f, err.max()), flush=True)
+        assert np.all(err < 2e-3), (f, int(np.argmax(err)))

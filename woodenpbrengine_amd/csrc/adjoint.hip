@@ -567,6 +567,48 @@ __global__ void k_film_loss(const float4* film, const float* target, uint32_t np
     if (threadIdx.x == 0 && partial) partial[blockIdx.x] = s[0];
 }
 
+// The adjoint's conductor-microfacet derivatives for given local directions,
+// one wavelength each: input SoA (n each) wo xyz, wi xyz, alpha, eta, k, u0, u1;
+// output SoA D, dD/dalpha, pdf, dpdf/dalpha, f, df/dalpha (f = Fr D / (4 cos_i
+// cos_o) |cos_i|), Fr, dFr/deta, dFr/dk at cos = wi.wh, then the half vector
+// sampled from (u0, u1) and its reflected wi: wh_z, dwh_z/dalpha, wi_z,
+// dwi_z/dalpha (CBSDFConductorMicroface.cpp:11-82, MicrofaceDistr.h:12-90,
+// CBXDF.h:200-238).
+constexpr int BSDF_GRAD_IN = 11, BSDF_GRAD_OUT = 13;
+__global__ void k_bsdf_grad(const float* in, float* out, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x[BSDF_GRAD_IN];
+    for (int j = 0; j < BSDF_GRAD_IN; j++) x[j] = in[(size_t)j * n + i];
+    const f3 wo = mk3(x[0], x[1], x[2]), wi = mk3(x[3], x[4], x[5]);
+    const D1 al = dvar<1>(x[6], 0);
+    const f3 h = normalize3(add3(wo, wi));
+    const V1 wh = v1c(h);
+    const D1 Dv = ggx_d1(wh, al);
+    const D1 pdf = Dv * (fabsf(h.z) / (4 * dot3(wo, h)));
+    const D3 F = fresnel_conductor3(dot3(wi, h), x[7], x[8]);
+    const float cI = fabsf(wi.z), cO = fabsf(wo.z);
+    const D1 f = (Dv / (4 * cI * cO)) * (F.v * cI);
+    // sampled half vector and direction along alpha (as the adjoint's BSDF strategy)
+    const float u0 = x[9], u1 = x[10];
+    const float phi = (float)((2 * PI_REF) * (double)u1);
+    const D1 tan2 = al * al * (u0 / (1.0f - u0));
+    const D1 cosT = 1.0f / dsqrt(1.0f + tan2);
+    const D1 sinT = dsqrt(dmax0(1.0f - cosT * cosT));
+    V1 hs = {sinT * cr_cos(phi), sinT * cr_sin(phi), cosT};
+    if (!(wo.z * hs.z.v > 0)) hs = {-hs.x, -hs.y, -hs.z};
+    const D1 wdh = vdot(hs, wo);
+    const D1 wiz = hs.z * (2.0f * wdh) - wo.z;
+    const float o[BSDF_GRAD_OUT] = {Dv.v, Dv.d[0], pdf.v, pdf.d[0], f.v, f.d[0], F.v, F.d[1], F.d[2],
+                                    hs.z.v, hs.z.d[0], wiz.v, wiz.d[0]};
+    for (int j = 0; j < BSDF_GRAD_OUT; j++) out[(size_t)j * n + i] = o[j];
+}
+
+wpbr_status bsdf_grad_launch(const float* in, float* out, uint32_t n, hipStream_t st) {
+    if (n) hipLaunchKernelGGL(k_bsdf_grad, dim3((n + 255) / 256), dim3(256), 0, st, in, out, n);
+    return launch_status();
+}
+
 wpbr_status adjoint_launch(const AdjArgs& a, hipStream_t st) {
     const uint32_t blocks = (a.L.n + ADJ_BLOCK - 1) / ADJ_BLOCK;
     const size_t lds = sizeof(float) * (7 * (size_t)a.A.n_mat + 1);

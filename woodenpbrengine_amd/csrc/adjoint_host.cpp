@@ -16,6 +16,7 @@ wpbr_status adjoint_finish(const float* part, uint32_t rows, uint32_t n_mat, flo
                            const float* env_dI, uint32_t ntex, hipStream_t st);
 wpbr_status film_loss_launch(const float4* film, const float* target, uint32_t npix, float* rgb, float* dL,
                              float* partial, hipStream_t st);
+wpbr_status bsdf_grad_launch(const float* in, float* out, uint32_t n, hipStream_t st);
 
 namespace {
 
@@ -202,6 +203,15 @@ wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_film, c
     }
     if (partial) hipFreeAsync(partial, ctx->stream);
     return s;
+}
+
+wpbr_status wpbr_bsdf_derivatives(wpbr_ctx* ctx, uint32_t n, const float* d_in, float* d_out) {
+    if (!ctx || (n && (!d_in || !d_out))) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    wpbr_status s = bsdf_grad_launch(d_in, d_out, n, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("bsdf derivatives launch failed: ") + hipGetErrorString(last_launch_error));
+    return hip_check(ctx, hipStreamSynchronize(ctx->stream), "bsdf derivatives");
 }
 
 wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_film, const float* d_dL_drgb,

@@ -25,7 +25,7 @@ EXPORTS = [
     "wpbr_scene_destroy",
     "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats", "wpbr_autotune_collide",
     "wpbr_autotune_render", "wpbr_scene_param_counts", "wpbr_scene_set_params", "wpbr_film_loss",
-    "wpbr_render_backward",
+    "wpbr_render_backward", "wpbr_bsdf_derivatives",
 ]
 
 
@@ -111,6 +111,7 @@ def load():
         "wpbr_scene_set_params": (i32, [vp, vp, vp, vp, vp]),
         "wpbr_film_loss": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]),
         "wpbr_render_backward": (i32, [vp, vp, vp, vp, vp]),
+        "wpbr_bsdf_derivatives": (i32, [vp, C.c_uint32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -418,3 +419,21 @@ class DiffRenderer(Renderer):
         a, e = self.n_mat, self.n_etak
         env = g[a + e:].reshape(np.asarray(self.scene.env_rgb).shape) if self.n_env else None
         return g[:a], g[a:a + e].reshape(-1, 6), env
+
+
+BSDF_GRAD_FIELDS = ["D", "dD_dalpha", "pdf", "dpdf_dalpha", "f", "df_dalpha", "Fr", "dFr_deta", "dFr_dk",
+                    "wh_z", "dwh_z_dalpha", "wi_z", "dwi_z_dalpha"]
+
+
+def bsdf_derivatives(ctx, wo, wi, alpha, eta, k, u):
+    """The backward pass's conductor-microfacet derivatives (wpbr_bsdf_derivatives)
+    for n requests: wo, wi (n x 3, shading frame), alpha, eta, k (n), u (n x 2).
+    Returns a dict of n-vectors named by BSDF_GRAD_FIELDS."""
+    wo, wi, u = (np.asarray(a, np.float32).reshape(-1, m) for a, m in ((wo, 3), (wi, 3), (u, 2)))
+    n = wo.shape[0]
+    cols = [wo.T, wi.T] + [np.asarray(a, np.float32).reshape(1, n) for a in (alpha, eta, k)] + [u.T]
+    d_in = ctx.array(np.ascontiguousarray(np.concatenate(cols, 0), np.float32))
+    d_out = ctx.empty((len(BSDF_GRAD_FIELDS), n), np.float32)
+    ctx.check(ctx.lib.wpbr_bsdf_derivatives(ctx.h, n, d_in.ptr, d_out.ptr))
+    out = d_out.numpy()
+    return {f: out[j] for j, f in enumerate(BSDF_GRAD_FIELDS)}
