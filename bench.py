@@ -101,7 +101,7 @@ def bench_microbench(lib, scenes, ctx, steps):
     return res
 
 
-def bench_adjoint(lib, scenes, ctx, steps):
+def bench_adjoint(lib, scenes, ctx, steps, tune=True):
     """Differentiable render: forward, L2 loss against a target rendered with
     parameters perturbed by up to +-20 %, and the path-replay backward pass."""
     s = scenes.c4_scene()
@@ -110,7 +110,8 @@ def bench_adjoint(lib, scenes, ctx, steps):
     g = np.random.default_rng(2026)
     r.set_params(*[(p * (1 + 0.2 * g.uniform(-1, 1, p.shape))).astype(np.float32)
                    for p in (alpha, etak, env)])
-    r.autotune()
+    if tune:
+        r.autotune()
     r.forward()
     r.set_target(r.film_rgb().astype(np.float32))
     r.set_params(alpha, etak, env)
@@ -145,6 +146,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--trace-config", default=None,
+                    help="fixed traversal configuration block,stack_depth,mapping,width,refill,chunk "
+                         "instead of the autotuner (for profiling one configuration)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one plain 512 MiB device copy (reference kernel for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scale factor)")
@@ -176,7 +180,12 @@ def main():
     ctx = lib.Context(0)
     r = lib.Renderer(ctx, scene)
     log("scene uploaded + LBVH built")
-    tuned = r.autotune()
+    if args.trace_config:
+        vals = [int(x) for x in args.trace_config.split(",")]
+        ctx.set_trace_config(**dict(zip(["block", "stack_depth", "mapping", "width", "refill", "chunk"], vals)))
+        tuned = ctx.trace_config()
+    else:
+        tuned = r.autotune()
     log("traversal configuration:", tuned)
     for _ in range(args.warmup):
         r.render()
@@ -228,7 +237,7 @@ def main():
         mb = bench_microbench(lib, scenes, ctx, args.steps)
         log("microbench:", json.dumps(mb))
     if not args.no_adjoint:
-        adj = bench_adjoint(lib, scenes, ctx, args.steps)
+        adj = bench_adjoint(lib, scenes, ctx, args.steps, tune=not args.trace_config)
         log("adjoint:", json.dumps(adj))
         mb.update(adj)
     cpu = None

@@ -87,6 +87,14 @@ void wpbr_lbvh_destroy(wpbr_lbvh* bvh);
  * codes, radix sort, treelets, upper SAH, flatten, traversal layout).  Device
  * buffers are owned by the lbvh object and reused across builds. */
 wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* bvh);
+/* Segmented build: the trees of n_scenes independent scenes (geoms[i] into
+ * bvhs[i], each from wpbr_lbvh_create, all distinct), one kernel launch per
+ * build kernel for all scenes together; each tree equals what wpbr_lbvh_build
+ * gives for that scene alone.  Replaces MEngine::buildLBVH (MEngine.cpp:341-350)
+ * run once per scene; the segmented form itself has no counterpart in the
+ * reference.  At most 65535 scenes; wpbr_last_time_ms is the whole batch. */
+wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wpbr_geometry* geoms,
+                                      wpbr_lbvh* const* bvhs);
 /* The same build one job at a time, in order (MEngine::buildLBVH,
  * MEngine.cpp:341-350), so each of the reference's JOB_RUN calls maps to one
  * call: 0 JobUpdateBoundsAndCentroidSphere/Triangle (CSTriangle.cpp:116-142,

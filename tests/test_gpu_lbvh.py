@@ -161,3 +161,52 @@ def test_staged_build_matches_and_checks_order(ctx):
     with pytest.raises(lib.WpbrError):
         b.build_stage(sc, 2)
     b.destroy()
+
+
+def _oracle_compare(b, ob):
+    c = b.counts()
+    assert c["nodes"] == ob.n_nodes and c["treelets"] == ob.n_treelets
+    for f in ("morton", "sorted_code", "sorted_index", "treelet_start", "node_nshapes", "node_offset",
+              "node_bounds"):
+        np.testing.assert_array_equal(b.download(f), ob.get(f), err_msg=f)
+    ns = b.download("node_nshapes")
+    np.testing.assert_array_equal(b.download("node_axis")[ns == 0], ob.get("node_axis")[ns == 0])
+
+
+def test_segmented_build_matches_oracle(ctx):
+    """wpbr_lbvh_build_segmented: every scene's tree equals the oracle's (and so
+    the single-scene build's), for scenes from 1 to 100 000 shapes in one batch;
+    a second batch of other sizes reuses the same objects; collision on the
+    segmented trees gives the oracle's hits."""
+    cases = [_case(n) for n in CASES]
+    cases.append((np.zeros((0, 24), np.float32), np.zeros(0, np.float32), scenes.random_triangles(1, 41)))
+    sx1, sr1 = _spheres(1, 42)
+    cases.append((sx1, sr1, np.zeros((0, 9), np.float32)))
+    for k, n in enumerate((40, 900, 5000)):
+        cases.append((*_spheres(k, 43 + k), scenes.random_triangles(n, 50 + k, median_edge=0.03)))
+    scs = [lib.Scene(ctx, *c) for c in cases]
+    bvhs = [lib.Lbvh(ctx) for _ in cases]
+    print("segmented build of %d scenes" % len(scs), flush=True)
+    lib.build_segmented(ctx, scs, bvhs)
+    rays = np.concatenate([scenes.pinhole_rays(32, 32, (0.5, 0.5, -1.0)), scenes.random_rays(2048, 44)], axis=1)
+    d_rays = ctx.array(rays)
+    for i, (c, sc, b) in enumerate(zip(cases, scs, bvhs)):
+        print("scene", i, sc.n_shapes, flush=True)
+        osc = po.OScene(*c)
+        ob = po.build_scene_bvh(osc)
+        _oracle_compare(b, ob)
+        os_, ot, _, ovf = po.collide(osc, ob, rays)
+        d_s, d_t, _ = lib.collide(ctx, sc, b, d_rays, rays.shape[1])
+        np.testing.assert_array_equal(d_s.numpy(), os_)
+        np.testing.assert_array_equal(d_t.numpy().view(np.uint32), ot.view(np.uint32))
+    # second batch: other scenes (some larger, some smaller) into the same objects
+    cases2 = [(*_spheres(2, 60 + k), scenes.random_triangles(n, 70 + k, median_edge=0.02))
+              for k, n in enumerate((30000, 3, 700))]
+    scs2 = [lib.Scene(ctx, *c) for c in cases2]
+    lib.build_segmented(ctx, scs2, bvhs[:3])
+    for c, b in zip(cases2, bvhs[:3]):
+        _oracle_compare(b, po.build_scene_bvh(po.OScene(*c)))
+    with pytest.raises(lib.WpbrError):
+        lib.build_segmented(ctx, scs2[:2], [bvhs[0], bvhs[0]])
+    for b in bvhs:
+        b.destroy()

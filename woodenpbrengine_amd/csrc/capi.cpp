@@ -109,6 +109,45 @@ GeomView to_view(const wpbr_geometry* g) {
     return v;
 }
 
+// Checks a scene's geometry and sizes the LBVH's buffers for it (stage 0 of a build).
+static wpbr_status lbvh_prepare(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b) {
+    uint64_t n64 = (uint64_t)geom->n_spheres + geom->n_triangles;
+    if (n64 == 0 || n64 >= (1ull << 28)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count (1 .. 2^28-1)");
+    if ((geom->n_spheres && (!geom->d_sphere_xform || !geom->d_sphere_radius)) ||
+        (geom->n_triangles && !geom->d_tri_verts))
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
+    uint32_t n = (uint32_t)n64;
+    if (n > b->cap) {
+        if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
+        LbvhScratch tmp;
+        size_t bytes = lbvh_layout(tmp, nullptr, n);
+        wpbr_status s = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc lbvh");
+        if (s) { b->cap = 0; return s; }
+        b->cap = n;
+    }
+    lbvh_layout(b->s, (char*)b->mem, n);
+    b->n = n;
+    b->geom = to_view(geom);
+    b->built = false;
+    b->next_stage = 0;
+    return WPBR_OK;
+}
+
+// The host-visible results of a finished build: node and treelet counts, the
+// root's traversal word and the device error bits.
+static wpbr_status lbvh_finish(wpbr_ctx* ctx, wpbr_lbvh* b, const uint32_t* info) {
+    b->n_nodes = info[0];
+    b->n_treelets = info[1];
+    const uint32_t err = info[2], ns0 = info[3], off0 = info[4];
+    b->root_is_leaf = ns0 > 0;
+    // the root record, or a primitive range when the root is a leaf of <= 4 shapes
+    b->root_word = (ns0 > 0 && ns0 <= 4) ? (0x80000000u | ((ns0 - 1) << 28) | off0) : 0u;
+    b->built = err == 0;
+    if (err & 1u) return set_err(ctx, WPBR_ERR_TIMEOUT, "radix sort look-back timed out");
+    if (err) return set_err(ctx, WPBR_ERR_HIP, "upper SAH build: internal stack exhausted");
+    return WPBR_OK;
+}
+
 }  // namespace wpbr
 
 extern "C" {
@@ -139,6 +178,7 @@ void wpbr_close(wpbr_ctx* ctx) {
     hipStreamSynchronize(ctx->stream);
     if (ctx->spill) hipFree(ctx->spill);
     if (ctx->err_flag) hipFree(ctx->err_flag);
+    if (ctx->batch_mem) hipFree(ctx->batch_mem);
     ctx_release_render(ctx);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
@@ -240,24 +280,8 @@ wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr
     hipSetDevice(ctx->device);
     (void)hipGetLastError();  // launch checks below must not see stale errors
     if (stage == 0) {
-        uint64_t n64 = (uint64_t)geom->n_spheres + geom->n_triangles;
-        if (n64 == 0 || n64 >= (1ull << 28)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "shape count (1 .. 2^28-1)");
-        if ((geom->n_spheres && (!geom->d_sphere_xform || !geom->d_sphere_radius)) ||
-            (geom->n_triangles && !geom->d_tri_verts))
-            return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
-        uint32_t n = (uint32_t)n64;
-        if (n > b->cap) {
-            if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
-            LbvhScratch tmp;
-            size_t bytes = lbvh_layout(tmp, nullptr, n);
-            wpbr_status s = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc lbvh");
-            if (s) { b->cap = 0; return s; }
-            b->cap = n;
-        }
-        lbvh_layout(b->s, (char*)b->mem, n);
-        b->n = n;
-        b->geom = to_view(geom);
-        b->built = false;
+        wpbr_status s = lbvh_prepare(ctx, geom, b);
+        if (s) return s;
     } else {
         if (b->next_stage != stage)
             return set_err(ctx, WPBR_ERR_INVALID_ARG, "lbvh stages run in order 0 .. 5");
@@ -274,27 +298,113 @@ wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr
     b->next_stage = stage + 1;
     if (stage < LBVH_STAGES - 1) return WPBR_OK;
     b->next_stage = 0;
-    uint32_t info[2] = {0, 0};
-    s = hip_check(ctx, hipMemcpyAsync(&info[0], b->s.n_nodes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    if (!s) s = hip_check(ctx, hipMemcpyAsync(&info[1], b->s.n_treelets, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    uint32_t err = 0, ns0 = 0, off0 = 0;
-    if (!s) s = hip_check(ctx, hipMemcpyAsync(&err, b->s.err, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    if (!s) s = hip_check(ctx, hipMemcpyAsync(&ns0, b->s.node_nshapes, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
-    if (!s) s = hip_check(ctx, hipMemcpyAsync(&off0, b->s.node_offset, 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    uint32_t info[5] = {0, 0, 0, 0, 0};
+    uint32_t* src[5] = {b->s.n_nodes, b->s.n_treelets, b->s.err, b->s.node_nshapes, b->s.node_offset};
+    for (int k = 0; k < 5 && !s; k++)
+        s = hip_check(ctx, hipMemcpyAsync(&info[k], src[k], 4, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "lbvh build");
     if (s) return s;
-    b->n_nodes = info[0];
-    b->n_treelets = info[1];
-    b->root_is_leaf = ns0 > 0;
-    // the root record, or a primitive range when the root is a leaf of <= 4 shapes
-    b->root_word = (ns0 > 0 && ns0 <= 4) ? (0x80000000u | ((ns0 - 1) << 28) | off0) : 0u;
     float ms = 0;
     hipEventElapsedTime(&ms, b->ev[0], b->ev[wpbr_lbvh::NEV - 1]);
     ctx->last_ms = ms;
-    b->built = err == 0;
-    if (err & 1u) return set_err(ctx, WPBR_ERR_TIMEOUT, "radix sort look-back timed out");
-    if (err) return set_err(ctx, WPBR_ERR_HIP, "upper SAH build: internal stack exhausted");
-    return WPBR_OK;
+    return lbvh_finish(ctx, b, info);
+}
+
+wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wpbr_geometry* geoms,
+                                      wpbr_lbvh* const* bvhs) {
+    if (!ctx || !geoms || !bvhs || n_scenes == 0) return WPBR_ERR_INVALID_ARG;
+    if (n_scenes > 65535) return set_err(ctx, WPBR_ERR_INVALID_ARG, "at most 65535 scenes per segmented build");
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    for (uint32_t i = 0; i < n_scenes; i++) {
+        if (!bvhs[i]) return set_err(ctx, WPBR_ERR_INVALID_ARG, "null lbvh");
+        for (uint32_t j = 0; j < i; j++)
+            if (bvhs[j] == bvhs[i]) return set_err(ctx, WPBR_ERR_INVALID_ARG, "an lbvh appears twice");
+    }
+    for (uint32_t i = 0; i < n_scenes; i++) {
+        wpbr_status s = lbvh_prepare(ctx, &geoms[i], bvhs[i]);
+        if (s) return set_err(ctx, s, "scene " + std::to_string(i) + ": " + ctx->last_error);
+    }
+    // host image of the batch descriptors: scratch structs, geometry, sizes,
+    // then per launch kind the block -> scene map
+    std::vector<LbvhScratch> scr(n_scenes);
+    std::vector<GeomView> gv(n_scenes);
+    std::vector<uint32_t> nv(n_scenes);
+    std::vector<uint32_t> first[SEG_KINDS], count[SEG_KINDS];
+    std::vector<uint16_t> scene_of[SEG_KINDS];
+    uint32_t grid[SEG_KINDS] = {};
+    for (uint32_t i = 0; i < n_scenes; i++) {
+        scr[i] = bvhs[i]->s;
+        gv[i] = bvhs[i]->geom;
+        nv[i] = bvhs[i]->n;
+        uint32_t c[SEG_KINDS];
+        lbvh_seg_blocks(nv[i], scr[i].status_words, c);
+        for (int k = 0; k < SEG_KINDS; k++) {
+            first[k].push_back(grid[k]);
+            count[k].push_back(c[k]);
+            scene_of[k].insert(scene_of[k].end(), c[k], (uint16_t)i);
+            grid[k] += c[k];
+        }
+    }
+    size_t bytes = 0;
+    auto reserve = [&](size_t b) { size_t o = (bytes + 255) & ~size_t(255); bytes = o + b; return o; };
+    const size_t o_s = reserve(sizeof(LbvhScratch) * n_scenes), o_g = reserve(sizeof(GeomView) * n_scenes),
+                 o_n = reserve(4 * (size_t)n_scenes), o_work = reserve(4 * ((size_t)n_scenes + 1)),
+                 o_info = reserve(20 * (size_t)n_scenes);
+    size_t o_first[SEG_KINDS], o_count[SEG_KINDS], o_scene[SEG_KINDS];
+    for (int k = 0; k < SEG_KINDS; k++) {
+        o_first[k] = reserve(4 * (size_t)n_scenes);
+        o_count[k] = reserve(4 * (size_t)n_scenes);
+        o_scene[k] = reserve(2 * (size_t)grid[k]);
+    }
+    if (bytes > ctx->batch_cap) {
+        if (ctx->batch_mem) { hipFree(ctx->batch_mem); ctx->batch_mem = nullptr; ctx->batch_cap = 0; }
+        wpbr_status s = hip_check(ctx, hipMalloc(&ctx->batch_mem, bytes), "hipMalloc segmented build");
+        if (s) return s;
+        ctx->batch_cap = bytes;
+    }
+    std::vector<char> img(bytes, 0);
+    auto put = [&](size_t o, const void* p, size_t len) { if (len) std::memcpy(img.data() + o, p, len); };
+    put(o_s, scr.data(), sizeof(LbvhScratch) * n_scenes);
+    put(o_g, gv.data(), sizeof(GeomView) * n_scenes);
+    put(o_n, nv.data(), 4 * (size_t)n_scenes);
+    char* d = (char*)ctx->batch_mem;
+    LbvhBatch B;
+    B.s = (const LbvhScratch*)(d + o_s);
+    B.g = (const GeomView*)(d + o_g);
+    B.n = (const uint32_t*)(d + o_n);
+    B.n_scenes = n_scenes;
+    B.work = (uint32_t*)(d + o_work);
+    B.info = (uint32_t*)(d + o_info);
+    for (int k = 0; k < SEG_KINDS; k++) {
+        put(o_first[k], first[k].data(), 4 * (size_t)n_scenes);
+        put(o_count[k], count[k].data(), 4 * (size_t)n_scenes);
+        put(o_scene[k], scene_of[k].data(), 2 * (size_t)grid[k]);
+        B.map[k].first = (const uint32_t*)(d + o_first[k]);
+        B.map[k].count = (const uint32_t*)(d + o_count[k]);
+        B.map[k].scene = (const uint16_t*)(d + o_scene[k]);
+    }
+    wpbr_status s = hip_check(ctx, hipMemcpyAsync(ctx->batch_mem, img.data(), bytes, hipMemcpyHostToDevice,
+                                                  ctx->stream), "segmented build upload");
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "segmented build upload");
+    if (s) return s;
+    hipEventRecord(ctx->ev0, ctx->stream);
+    s = lbvh_launch_segmented(B, grid, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("segmented lbvh launch failed: ") + hipGetErrorString(last_launch_error));
+    std::vector<uint32_t> info(5 * (size_t)n_scenes);
+    s = hip_check(ctx, hipMemcpyAsync(info.data(), B.info, 4 * info.size(), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "segmented lbvh build");
+    if (s) return s;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) != hipSuccess) ms = 0;
+    ctx->last_ms = ms;
+    wpbr_status first_err = WPBR_OK;
+    for (uint32_t i = 0; i < n_scenes; i++) {
+        wpbr_status e = lbvh_finish(ctx, bvhs[i], &info[5 * (size_t)i]);
+        if (e && !first_err) first_err = set_err(ctx, e, "scene " + std::to_string(i) + ": " + ctx->last_error);
+    }
+    return first_err;
 }
 
 wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b) {

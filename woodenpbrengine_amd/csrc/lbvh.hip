@@ -40,15 +40,16 @@ constexpr uint32_t SPIN_LIMIT = 1u << 20;
 constexpr int MAX_PRIMS_IN_LEAF = 30;  // SLBVH.cpp:356
 
 // ---------------------------------------------------------------------------
-__global__ void k_init(LbvhScratch s) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
+WPBR_D void init_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
+    uint32_t i = bid * blockDim.x + threadIdx.x;
+    uint32_t stride = nblk * blockDim.x;
     for (uint32_t j = i; j < 4 * 256; j += stride) s.hist[j] = 0;
     for (uint32_t j = i; j < (uint32_t)s.status_words; j += stride) s.status0[j] = 0;
     for (uint32_t j = i; j < 128; j += stride) s.presence[j] = 0;
     if (i < 8) s.tile_counter[i] = 0;
     if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; *s.big_count = 0; }
 }
+__global__ void k_init(LbvhScratch s) { init_body(s, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 WPBR_D void shape_box(const GeomView& g, uint32_t i, bbox& b) {
@@ -77,11 +78,11 @@ WPBR_D void shape_box(const GeomView& g, uint32_t i, bbox& b) {
 // receive the final bounds, cb_ord[7] the number of partials.
 constexpr int BOUNDS_BLOCKS = 512;
 
-__global__ __launch_bounds__(256) void k_bounds(GeomView g, uint32_t n, float* bounds,
-                                                float* centroid, uint32_t* cb_ord) {
+WPBR_D void bounds_body(const GeomView& g, uint32_t n, float* bounds, float* centroid, uint32_t* cb_ord,
+                       uint32_t bid, uint32_t nblk) {
     __shared__ uint32_t s_red[4][6];
     uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0, 0, 0};  // min xyz, max xyz
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = bid * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
         bbox b;
         shape_box(g, i, b);
         store_box(bounds + 6 * (size_t)i, b);
@@ -110,9 +111,13 @@ __global__ __launch_bounds__(256) void k_bounds(GeomView g, uint32_t n, float* b
         int k = threadIdx.x;
         uint32_t r = s_red[0][k];
         for (int ww = 1; ww < 4; ww++) r = (k < 3) ? min(r, s_red[ww][k]) : max(r, s_red[ww][k]);
-        cb_ord[8 + 6 * (size_t)blockIdx.x + k] = r;
-        if (blockIdx.x == 0 && k == 0) cb_ord[7] = gridDim.x;
+        cb_ord[8 + 6 * (size_t)bid + k] = r;
+        if (bid == 0 && k == 0) cb_ord[7] = nblk;
     }
+}
+__global__ __launch_bounds__(256) void k_bounds(GeomView g, uint32_t n, float* bounds, float* centroid,
+                                                uint32_t* cb_ord) {
+    bounds_body(g, n, bounds, centroid, cb_ord, blockIdx.x, gridDim.x);
 }
 
 // Reduce the k_bounds partials (called by every thread of a consumer block;
@@ -157,17 +162,16 @@ WPBR_D uint32_t left_shift3(uint32_t x) {
 constexpr int MORTON_BLOCK = 256;
 constexpr int MORTON_ITEMS = 16;
 
-__global__ __launch_bounds__(MORTON_BLOCK) void k_morton(const float* centroid, uint32_t n,
-                                                         const uint32_t* cb_ord, uint32_t* codes,
-                                                         uint32_t* hist) {
+WPBR_D void morton_body(const float* centroid, uint32_t n, const uint32_t* cb_ord, uint32_t* codes,
+                       uint32_t* hist, uint32_t bid) {
     __shared__ uint32_t h[4 * 256];
     __shared__ uint32_t s_cb[6];
     for (int j = threadIdx.x; j < 4 * 256; j += MORTON_BLOCK) h[j] = 0;
-    reduce_cb(cb_ord, s_cb, blockIdx.x == 0 ? const_cast<uint32_t*>(cb_ord) : nullptr);
+    reduce_cb(cb_ord, s_cb, bid == 0 ? const_cast<uint32_t*>(cb_ord) : nullptr);
     float mn[3], mx[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) { mn[k] = ord2f(s_cb[k]); mx[k] = ord2f(s_cb[3 + k]); }
-    uint32_t base = blockIdx.x * (MORTON_BLOCK * MORTON_ITEMS);
+    uint32_t base = bid * (MORTON_BLOCK * MORTON_ITEMS);
 #pragma unroll 4
     for (int it = 0; it < MORTON_ITEMS; it++) {
         uint32_t i = base + it * MORTON_BLOCK + threadIdx.x;
@@ -193,15 +197,20 @@ __global__ __launch_bounds__(MORTON_BLOCK) void k_morton(const float* centroid, 
     for (int j = threadIdx.x; j < 4 * 256; j += MORTON_BLOCK)
         if (h[j]) atomicAdd(&hist[j], h[j]);
 }
+__global__ __launch_bounds__(MORTON_BLOCK) void k_morton(const float* centroid, uint32_t n,
+                                                         const uint32_t* cb_ord, uint32_t* codes,
+                                                         uint32_t* hist) {
+    morton_body(centroid, n, cb_ord, codes, hist, blockIdx.x);
+}
 
 // ---------------------------------------------------------------------------
 // One onesweep pass.  keys_in/vals_in -> keys_out/vals_out on digit `pass`.
 // vals_in == nullptr means the value is the key's index (first pass).
-__global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
+WPBR_D void onesweep_body(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
     uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int pass,
     const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* status_next,
-    uint32_t* tile_counter, uint32_t* err, uint32_t num_tiles) {
+    uint32_t* tile_counter, uint32_t* err) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_whist[SORT_WAVES][256];
     __shared__ uint32_t s_goff[256];       // global exclusive digit offset
@@ -326,19 +335,29 @@ __global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
         keys_out[dst] = k;
         vals_out[dst] = s_vals[p];
     }
-    (void)num_tiles;
+}
+__global__ __launch_bounds__(SORT_BLOCK) void k_onesweep(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in,
+    uint32_t* __restrict__ keys_out, uint32_t* __restrict__ vals_out, uint32_t n, int pass,
+    const uint32_t* __restrict__ hist, uint32_t* status, uint32_t* status_next,
+    uint32_t* tile_counter, uint32_t* err) {
+    onesweep_body(keys_in, vals_in, keys_out, vals_out, n, pass, hist, status, status_next, tile_counter, err);
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_treelet_mark(const uint32_t* __restrict__ sorted, uint32_t n,
-                               uint32_t* bin_start, uint32_t* presence) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+WPBR_D void treelet_mark_body(const uint32_t* __restrict__ sorted, uint32_t n, uint32_t* bin_start,
+                             uint32_t* presence, uint32_t bid) {
+    uint32_t i = bid * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t top = sorted[i] >> 18;
     if (i == 0 || top != (sorted[i - 1] >> 18)) {
         bin_start[top] = i;
         atomicOr(&presence[top >> 5], 1u << (top & 31));
     }
+}
+__global__ void k_treelet_mark(const uint32_t* __restrict__ sorted, uint32_t n, uint32_t* bin_start,
+                               uint32_t* presence) {
+    treelet_mark_body(sorted, n, bin_start, presence, blockIdx.x);
 }
 
 WPBR_D uint32_t presence_rank(const uint32_t* presence, uint32_t bin) {
@@ -354,14 +373,13 @@ WPBR_D uint32_t presence_rank(const uint32_t* presence, uint32_t bin) {
 constexpr int EMIT_BLOCK = 256;
 constexpr int EMIT_MAX_LEVELS = 64;
 
-__global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32_t* __restrict__ codes,
-                                                     const uint32_t* __restrict__ sorted_index,
-                                                     const float* __restrict__ bounds, uint32_t n) {
+WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
+                     const uint32_t* __restrict__ sorted_index, const float* __restrict__ bounds, uint32_t n,
+                     uint32_t bin) {
     __shared__ uint32_t s_q;
     __shared__ uint32_t s_lvl[EMIT_MAX_LEVELS + 2];
     __shared__ int s_nlvl;
     __shared__ uint32_t s_tidx, s_start, s_cnt;
-    const uint32_t bin = blockIdx.x;
     const int tid = threadIdx.x;
     if (!((s.presence[bin >> 5] >> (bin & 31)) & 1u)) return;
     if (tid == 0) {
@@ -480,6 +498,11 @@ __global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32
         bbox b = load_box(s.r_box + 6 * (size_t)base);
         store_box(s.t_box + 6 * (size_t)s_tidx, b);
     }
+}
+__global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32_t* __restrict__ codes,
+                                                     const uint32_t* __restrict__ sorted_index,
+                                                     const float* __restrict__ bounds, uint32_t n) {
+    emit_body(s, codes, sorted_index, bounds, n, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -946,7 +969,7 @@ WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, 
     }
 }
 
-__global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s) {
+WPBR_D void upper_sah_body(LbvhScratch& s) {
     __shared__ SahLds L;
     const int tid = threadIdx.x, lane = tid & 63;
     if (tid == 0) { L.k = 0; L.upper = 0; L.total = 0; L.task_next = 1; L.nbig = 0; L.nbig_next = 0; }
@@ -1081,11 +1104,11 @@ __global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s) {
         *s.n_treelets = k;
     }
 }
+__global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s) { upper_sah_body(s); }
 
 // ---------------------------------------------------------------------------
 // Flatten: treelet records to final preorder slots.  One workgroup per treelet.
-__global__ __launch_bounds__(256) void k_scatter(LbvhScratch s) {
-    const uint32_t t = blockIdx.x;
+WPBR_D void scatter_body(const LbvhScratch& s, uint32_t t) {
     if (t >= *s.n_treelets) return;
     const uint32_t base = s.t_slot[t];
     const uint32_t cnt = s.t_nodes[t];
@@ -1107,6 +1130,7 @@ __global__ __launch_bounds__(256) void k_scatter(LbvhScratch s) {
         }
     }
 }
+__global__ __launch_bounds__(256) void k_scatter(LbvhScratch s) { scatter_body(s, blockIdx.x); }
 
 // ---------------------------------------------------------------------------
 // Traversal layout (derived from the reference node arrays; the reference arrays
@@ -1183,11 +1207,11 @@ constexpr int SUB_STACK = 8;  // smaller child first: depth <= log2(BIG_LEAF / 5
 // (SLBVH.cpp:374-401): at the highest Morton bit that differs inside the range
 // (child 0 below child 1 on axis bit % 3), or in halves when all codes are
 // equal.  Larger leaves (all codes equal) are queued for k_subtree_big.
-__global__ __launch_bounds__(256) void k_subtree(LbvhScratch s, const uint32_t* __restrict__ codes,
-                                                 const uint32_t* __restrict__ sorted_index) {
+WPBR_D void subtree_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
+                        const uint32_t* __restrict__ sorted_index, uint32_t bid) {
     __shared__ uint16_t s_ab[SUB_STACK][256];
     __shared__ uint32_t s_rec[SUB_STACK][256];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     const uint32_t nn = *s.n_nodes;
     const uint32_t n = (i < nn) ? s.node_nshapes[i] : 0;
     const uint32_t off = (n > 0) ? s.node_offset[i] : 0;
@@ -1268,6 +1292,10 @@ __global__ __launch_bounds__(256) void k_subtree(LbvhScratch s, const uint32_t* 
         }
     }
 }
+__global__ __launch_bounds__(256) void k_subtree(LbvhScratch s, const uint32_t* __restrict__ codes,
+                                                 const uint32_t* __restrict__ sorted_index) {
+    subtree_body(s, codes, sorted_index, blockIdx.x);
+}
 
 WPBR_D bbox record_union(const LbvhScratch& s, uint32_t r) {
     const float4* rec = reinterpret_cast<const float4*>(s.trav) + 4 * (size_t)r;
@@ -1280,11 +1308,12 @@ WPBR_D bbox record_union(const LbvhScratch& s, uint32_t r) {
 
 // Leaves of more than BIG_LEAF shapes (only when every Morton code in the leaf
 // is equal): one workgroup each, levels bottom-up.
-__global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32_t* __restrict__ sorted_index) {
+WPBR_D void subtree_big_body(const LbvhScratch& s, const uint32_t* __restrict__ sorted_index, uint32_t bid,
+                            uint32_t nblk) {
     const uint32_t count = *s.big_count;
     const uint32_t nn = *s.n_nodes;
     __shared__ uint32_t s_base;
-    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+    for (uint32_t e = bid; e < count; e += nblk) {
         const uint32_t i = s.big_list[e];
         const uint32_t n = s.node_nshapes[i], off = s.node_offset[i];
         const uint32_t Lmax = sub_levels(n);
@@ -1320,6 +1349,9 @@ __global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32
         }
     }
 }
+__global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32_t* __restrict__ sorted_index) {
+    subtree_big_body(s, sorted_index, blockIdx.x, gridDim.x);
+}
 
 // 4-wide records: record r holds the grandchildren of binary record r (a child
 // that is a primitive range stays one child).  Slots 0,1 come from child 0 and
@@ -1331,8 +1363,8 @@ __global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32
 // have no record, and sub-tree records allocated but not used by a leaf are
 // never written, so a child word read from such a record is not followed
 // unless it indexes an existing record (those nodes are unreachable anyway).
-__global__ void k_bvh4(LbvhScratch s) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+WPBR_D void bvh4_body(const LbvhScratch& s, uint32_t bid) {
+    const uint32_t r = bid * blockDim.x + threadIdx.x;
     const uint32_t nn = *s.n_nodes, total = nn + *s.sub_count;
     if (r >= total) return;
     if (r < nn) {
@@ -1378,11 +1410,13 @@ __global__ void k_bvh4(LbvhScratch s) {
                        __uint_as_float(w[3]));
     o[7] = make_float4(__uint_as_float(axes), 0.0f, 0.0f, 0.0f);
 }
+__global__ void k_bvh4(LbvhScratch s) { bvh4_body(s, blockIdx.x); }
 
 // Primitive records at their slots: v0.xyz + shape id, v1.xyz + slot,
 // v2.xyz + first slot of the reference leaf.  Spheres keep only the words.
-__global__ void k_prims(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index, uint32_t n) {
-    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+WPBR_D void prims_body(const LbvhScratch& s, const GeomView& g, const uint32_t* __restrict__ sorted_index,
+                      uint32_t n, uint32_t bid) {
+    uint32_t p = bid * blockDim.x + threadIdx.x;
     if (p >= n) return;
     uint32_t id = sorted_index[p];
     uint32_t leaf = s.slot_leaf[p];
@@ -1396,6 +1430,9 @@ __global__ void k_prims(LbvhScratch s, GeomView g, const uint32_t* __restrict__ 
     rec[0] = make_float4(v[0], v[1], v[2], __uint_as_float(id));
     rec[1] = make_float4(v[3], v[4], v[5], __uint_as_float(p));
     rec[2] = make_float4(v[6], v[7], v[8], __uint_as_float(leaf));
+}
+__global__ void k_prims(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index, uint32_t n) {
+    prims_body(s, g, sorted_index, n, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1411,6 +1448,204 @@ wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroi
     uint32_t nb = std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256);
     hipLaunchKernelGGL(k_bounds, dim3(nb), dim3(256), 0, st, g, n, bounds, centroid, cb_ord);
     hipLaunchKernelGGL(k_cb_final, dim3(1), dim3(64), 0, st, cb_ord);
+    return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Segmented build: the LBVHs of many independent scenes, one launch per kernel
+// for all of them.  Every scene keeps its own scratch (its own wpbr_lbvh), so
+// each kernel body runs unchanged on its scene's data: block b of a launch
+// works for scene map.scene[b] as block b - map.first[scene] of
+// map.count[scene] (the grid the single-scene launch would use).  Per-treelet
+// kernels run over a work list of (scene, treelet) pairs built on the device.
+#define SEG_BLOCK(map)                                   \
+    const uint32_t sc = (map).scene[blockIdx.x];         \
+    const uint32_t bid = blockIdx.x - (map).first[sc];   \
+    const uint32_t nblk = (map).count[sc];               \
+    (void)nblk;
+
+__global__ void ks_init(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_INIT]);
+    init_body(B.s[sc], bid, nblk);
+}
+
+__global__ __launch_bounds__(256) void ks_bounds(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_BOUNDS]);
+    const LbvhScratch& s = B.s[sc];
+    bounds_body(B.g[sc], B.n[sc], s.bounds, s.centroid, s.cb_ord, bid, nblk);
+}
+
+__global__ void ks_cb_final(LbvhBatch B) {
+    __shared__ uint32_t s_cb[6];
+    uint32_t* cb = B.s[blockIdx.x].cb_ord;
+    reduce_cb(cb, s_cb, cb);
+}
+
+__global__ __launch_bounds__(MORTON_BLOCK) void ks_morton(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_MORTON]);
+    const LbvhScratch& s = B.s[sc];
+    morton_body(s.centroid, B.n[sc], s.cb_ord, s.morton, s.hist, bid);
+}
+
+// Pass p of the ping-pong: morton -> B -> A -> B -> A (as lbvh_launch_stage).
+__global__ __launch_bounds__(SORT_BLOCK) void ks_onesweep(LbvhBatch B, int p) {
+    SEG_BLOCK(B.map[SEG_SORT]);
+    const LbvhScratch& s = B.s[sc];
+    const SortPass q = sort_pass(s, p);
+    onesweep_body(q.kin, q.vin, q.kout, q.vout, B.n[sc], p, s.hist, q.status, q.status_next, s.tile_counter, s.err);
+}
+
+__global__ void ks_treelet_mark(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_MARK]);
+    const LbvhScratch& s = B.s[sc];
+    treelet_mark_body(s.keys_a, B.n[sc], s.bin_start, s.presence, bid);
+}
+
+// work[i] = number of treelets of scenes 0 .. i-1 (one workgroup).
+__global__ void ks_treelet_count(LbvhBatch B) {
+    __shared__ uint32_t s_sum[1024];
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < B.n_scenes; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t c = 0;
+        if (i < B.n_scenes)
+            for (int w = 0; w < 128; w++) c += __popc(B.s[i].presence[w]);
+        s_sum[threadIdx.x] = c;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+            uint32_t v = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_sum[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < B.n_scenes) B.work[i + 1] = run + s_sum[threadIdx.x];
+        if (base == 0 && threadIdx.x == 0) B.work[0] = 0;
+        run += s_sum[blockDim.x - 1];
+        __syncthreads();
+    }
+}
+
+// Scene of work item g: the last i with work[i] <= g.
+WPBR_D uint32_t work_scene(const uint32_t* work, uint32_t n_scenes, uint32_t g) {
+    uint32_t lo = 0, hi = n_scenes;  // work[lo] <= g < work[hi]
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (work[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The bin of the t-th occupied bin.
+WPBR_D uint32_t presence_select(const uint32_t* presence, uint32_t t) {
+    uint32_t w = 0;
+    for (; w < 127; w++) {
+        uint32_t c = __popc(presence[w]);
+        if (t < c) break;
+        t -= c;
+    }
+    uint32_t m = presence[w];
+    for (uint32_t k = 0; k < t; k++) m &= m - 1;
+    return 32 * w + (uint32_t)(__ffs(m) - 1);
+}
+
+__global__ __launch_bounds__(EMIT_BLOCK) void ks_emit(LbvhBatch B) {
+    __shared__ uint32_t s_sc, s_bin;
+    const uint32_t total = B.work[B.n_scenes];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        if (threadIdx.x == 0) {
+            const uint32_t sc = work_scene(B.work, B.n_scenes, g);
+            s_sc = sc;
+            s_bin = presence_select(B.s[sc].presence, g - B.work[sc]);
+        }
+        __syncthreads();
+        const uint32_t sc = s_sc, bin = s_bin;
+        const LbvhScratch& s = B.s[sc];
+        emit_body(s, s.keys_a, s.vals_a, s.bounds, B.n[sc], bin);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(SAH_BLOCK) void ks_upper_sah(LbvhBatch B) {
+    LbvhScratch s = B.s[blockIdx.x];
+    upper_sah_body(s);
+}
+
+__global__ __launch_bounds__(256) void ks_scatter(LbvhBatch B) {
+    const uint32_t total = B.work[B.n_scenes];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const uint32_t sc = work_scene(B.work, B.n_scenes, g);
+        scatter_body(B.s[sc], g - B.work[sc]);
+    }
+}
+
+__global__ __launch_bounds__(256) void ks_subtree(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_SUBTREE]);
+    const LbvhScratch& s = B.s[sc];
+    subtree_body(s, s.keys_a, s.vals_a, bid);
+}
+
+__global__ __launch_bounds__(256) void ks_subtree_big(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_SUBTREE_BIG]);
+    const LbvhScratch& s = B.s[sc];
+    subtree_big_body(s, s.vals_a, bid, nblk);
+}
+
+__global__ void ks_prims(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_PRIMS]);
+    const LbvhScratch& s = B.s[sc];
+    prims_body(s, B.g[sc], s.vals_a, B.n[sc], bid);
+}
+
+__global__ void ks_bvh4(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_BVH4]);
+    bvh4_body(B.s[sc], bid);
+}
+
+// Per scene: n_nodes, n_treelets, err, nshapes and offset of node 0.
+__global__ void ks_info(LbvhBatch B) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n_scenes) return;
+    const LbvhScratch& s = B.s[i];
+    uint32_t* o = B.info + 5 * (size_t)i;
+    o[0] = *s.n_nodes;
+    o[1] = *s.n_treelets;
+    o[2] = *s.err;
+    o[3] = s.node_nshapes[0];
+    o[4] = s.node_offset[0];
+}
+
+// Blocks per scene of each segmented launch: the single-scene grids.
+void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts) {
+    counts[SEG_INIT] = std::max<uint32_t>(1, ((uint32_t)status_words + 255) / 256);
+    counts[SEG_BOUNDS] = std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256);
+    counts[SEG_MORTON] = (n + MORTON_BLOCK * MORTON_ITEMS - 1) / (MORTON_BLOCK * MORTON_ITEMS);
+    counts[SEG_SORT] = (n + SORT_TILE - 1) / SORT_TILE;
+    counts[SEG_MARK] = (n + 255) / 256;
+    counts[SEG_SUBTREE] = (2 * n + 255) / 256;
+    counts[SEG_SUBTREE_BIG] = 4;
+    counts[SEG_PRIMS] = (n + 255) / 256;
+    counts[SEG_BVH4] = (3 * std::max<uint32_t>(n, 1) + 255) / 256;
+}
+
+wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st) {
+    const uint32_t S = B.n_scenes;
+    hipLaunchKernelGGL(ks_init, dim3(grid[SEG_INIT]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_bounds, dim3(grid[SEG_BOUNDS]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_cb_final, dim3(S), dim3(64), 0, st, B);
+    hipLaunchKernelGGL(ks_morton, dim3(grid[SEG_MORTON]), dim3(MORTON_BLOCK), 0, st, B);
+    for (int p = 0; p < 4; p++)
+        hipLaunchKernelGGL(ks_onesweep, dim3(grid[SEG_SORT]), dim3(SORT_BLOCK), 0, st, B, p);
+    hipLaunchKernelGGL(ks_treelet_mark, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
+    hipLaunchKernelGGL(ks_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_upper_sah, dim3(S), dim3(SAH_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_subtree, dim3(grid[SEG_SUBTREE]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_subtree_big, dim3(grid[SEG_SUBTREE_BIG]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_prims, dim3(grid[SEG_PRIMS]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_bvh4, dim3(grid[SEG_BVH4]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_info, dim3((S + 255) / 256), dim3(256), 0, st, B);
     return launch_status();
 }
 
@@ -1446,16 +1681,10 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
             break;
         case 3: {
             const uint32_t num_tiles = (n + SORT_TILE - 1) / SORT_TILE;
-            // ping-pong: morton -> B -> A -> B -> A
-            const uint32_t* kin[4] = {s.morton, s.keys_b, s.keys_a, s.keys_b};
-            const uint32_t* vin[4] = {nullptr, s.vals_b, s.vals_a, s.vals_b};
-            uint32_t* kout[4] = {s.keys_b, s.keys_a, s.keys_b, s.keys_a};
-            uint32_t* vout[4] = {s.vals_b, s.vals_a, s.vals_b, s.vals_a};
             for (int p = 0; p < 4; p++) {
-                uint32_t* st_cur = (p & 1) ? s.status1 : s.status0;
-                uint32_t* st_next = (p == 3) ? nullptr : ((p & 1) ? s.status0 : s.status1);
-                hipLaunchKernelGGL(k_onesweep, dim3(num_tiles), dim3(SORT_BLOCK), 0, st, kin[p], vin[p], kout[p],
-                                   vout[p], n, p, s.hist, st_cur, st_next, s.tile_counter, s.err, num_tiles);
+                const SortPass q = sort_pass(s, p);
+                hipLaunchKernelGGL(k_onesweep, dim3(num_tiles), dim3(SORT_BLOCK), 0, st, q.kin, q.vin, q.kout,
+                                   q.vout, n, p, s.hist, q.status, q.status_next, s.tile_counter, s.err);
                 rec(3 + p);
             }
             hipLaunchKernelGGL(k_treelet_mark, dim3((n + 255) / 256), dim3(256), 0, st, s.keys_a, n, s.bin_start,

@@ -28,6 +28,9 @@ struct wpbr_ctx {
     uint32_t spill_depth = 0;
     uint32_t* err_flag = nullptr;
     wpbr_render_state* render = nullptr;
+    // segmented LBVH builds: per-scene descriptors and block maps
+    void* batch_mem = nullptr;
+    size_t batch_cap = 0;
 };
 
 struct wpbr_lbvh {

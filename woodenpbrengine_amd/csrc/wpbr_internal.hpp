@@ -67,6 +67,46 @@ struct LbvhScratch {
     uint32_t* big_count;
 };
 
+// Buffers of radix-sort pass p (0..3): each pass reads what the previous one
+// wrote, keys {morton, B, A, B} -> {B, A, B, A} (values alike, none for pass 0
+// whose value is the key's index); look-back status alternates 0, 1, 0, 1 and
+// each pass clears the next pass's status.
+struct SortPass {
+    const uint32_t* kin; const uint32_t* vin; uint32_t* kout; uint32_t* vout;
+    uint32_t* status; uint32_t* status_next;
+};
+__host__ __device__ inline SortPass sort_pass(const LbvhScratch& s, int p) {
+    SortPass r;
+    r.kin = p == 0 ? s.morton : (p == 2 ? s.keys_a : s.keys_b);
+    r.vin = p == 0 ? nullptr : (p == 2 ? s.vals_a : s.vals_b);
+    r.kout = (p & 1) ? s.keys_a : s.keys_b;
+    r.vout = (p & 1) ? s.vals_a : s.vals_b;
+    r.status = (p & 1) ? s.status1 : s.status0;
+    r.status_next = p == 3 ? nullptr : ((p & 1) ? s.status0 : s.status1);
+    return r;
+}
+
+// Segmented build of several scenes (one launch per kernel for all of them).
+// Block b of a launch of kind k works for scene map[k].scene[b].
+enum SegKind { SEG_INIT, SEG_BOUNDS, SEG_MORTON, SEG_SORT, SEG_MARK, SEG_SUBTREE, SEG_SUBTREE_BIG, SEG_PRIMS,
+               SEG_BVH4, SEG_KINDS };
+struct SegMap {
+    const uint16_t* scene;  // per block
+    const uint32_t* first;  // per scene: its first block
+    const uint32_t* count;  // per scene: its number of blocks
+};
+struct LbvhBatch {
+    const LbvhScratch* s;   // per scene (device array)
+    const GeomView* g;
+    const uint32_t* n;
+    uint32_t n_scenes;
+    uint32_t* work;         // n_scenes + 1: treelets of the scenes before each scene
+    uint32_t* info;         // 5 per scene: n_nodes, n_treelets, err, nshapes[0], offset[0]
+    SegMap map[SEG_KINDS];
+};
+void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts);
+wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st);
+
 constexpr int LBVH_STAGES = 6;
 wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
                               hipEvent_t* ev, int nev);

@@ -25,7 +25,7 @@ EXPORTS = [
     "wpbr_scene_destroy",
     "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats", "wpbr_autotune_collide",
     "wpbr_autotune_render", "wpbr_scene_param_counts", "wpbr_scene_set_params", "wpbr_film_loss",
-    "wpbr_render_backward", "wpbr_bsdf_derivatives",
+    "wpbr_render_backward", "wpbr_bsdf_derivatives", "wpbr_lbvh_build_segmented",
 ]
 
 
@@ -112,6 +112,7 @@ def load():
         "wpbr_film_loss": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]),
         "wpbr_render_backward": (i32, [vp, vp, vp, vp, vp]),
         "wpbr_bsdf_derivatives": (i32, [vp, C.c_uint32, vp, vp]),
+        "wpbr_lbvh_build_segmented": (i32, [vp, u32, C.POINTER(Geometry), C.POINTER(vp)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -273,6 +274,17 @@ class Lbvh:
         if self.h:
             self.ctx.lib.wpbr_lbvh_destroy(self.h)
             self.h = None
+
+
+def build_segmented(ctx, scenes, bvhs):
+    """LBVHs of several scenes in one segmented build (wpbr_lbvh_build_segmented);
+    returns the batch's device time in ms."""
+    if len(scenes) != len(bvhs):
+        raise ValueError("one lbvh per scene")
+    geoms = (Geometry * len(scenes))(*[s.geom for s in scenes])
+    hs = (C.c_void_p * len(bvhs))(*[b.h.value for b in bvhs])
+    ctx.check(ctx.lib.wpbr_lbvh_build_segmented(ctx.h, len(scenes), geoms, hs))
+    return ctx.last_ms()
 
 
 def collide(ctx, scene, bvh, d_rays, n, want_bary=False, wait=True):
