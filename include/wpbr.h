@@ -222,6 +222,33 @@ int wpbr_render_stats(wpbr_ctx* ctx, double* out, int cap);
  * and returns it in *best (optional).  d_film receives the last render. */
 wpbr_status wpbr_autotune_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, wpbr_trace_config* best);
 
+/* ---- batched many-scene render ------------------------------------------- *
+ * The reference renders one scene, one 32x32 tile at a time (MEngine::render,
+ * MEngine.cpp:380-448); these entry points render many scenes together.
+ * wpbr_scenes_create uploads n_scenes descriptions and builds all their LBVHs
+ * with one segmented build (each scene as wpbr_scene_create would make it).
+ * A batch (wpbr_batch_create) holds the wavefront state of its scenes and
+ * renders them with one launch per stage of every sample pass (camera,
+ * primary trace, shade, secondary-ray trace, radiance + splat) over all
+ * scenes' pixels, tiles and rays; the pass loop is captured in a HIP graph at
+ * the first render (again after a traversal configuration change) and replayed.
+ * Every film equals what wpbr_render gives for its scene alone.  All scenes of a
+ * batch have the same spp; at most 65535 scenes; d_films[i] holds
+ * width*height*4 floats of scene i (as wpbr_render). */
+typedef struct wpbr_batch wpbr_batch;
+wpbr_status wpbr_scenes_create(wpbr_ctx* ctx, uint32_t n_scenes, const wpbr_scene_desc* descs,
+                               wpbr_scene** out);
+wpbr_status wpbr_batch_create(wpbr_ctx* ctx, uint32_t n_scenes, wpbr_scene* const* scenes,
+                              float* const* d_films, wpbr_batch** out);
+void wpbr_batch_destroy(wpbr_batch* batch);
+wpbr_status wpbr_batch_render(wpbr_ctx* ctx, wpbr_batch* batch);
+/* [0] ms of the last render (graph replay), [1] samples, [2] scenes,
+ * [3] launches per render, [4] secondary rays traced.  Returns the number written. */
+int wpbr_batch_stats(wpbr_ctx* ctx, wpbr_batch* batch, double* out, int cap);
+/* Autotuner for the batch: renders once per candidate traversal configuration
+ * (block size, LDS stack depth, node width), keeps the fastest. */
+wpbr_status wpbr_autotune_batch(wpbr_ctx* ctx, wpbr_batch* batch, wpbr_trace_config* best);
+
 /* ---- differentiable render (no counterpart in the reference) -------------- *
  * An autograd.Function-style pair over the render: wpbr_render is the forward
  * (film of XYZ-weight sums), wpbr_render_backward the adjoint.

@@ -399,6 +399,62 @@ __global__ __launch_bounds__(BLOCK) void k_trace_dyn(TraceArgs a) {
     }
 }
 
+// Batched form of k_trace: block b traces for scene map.scene[b] (its launch
+// record T[scene]: geometry, tree, rays, hits, spill slots) as its block
+// b - map.first[scene].
+template <int BLOCK, int SD, int WIDTH>
+__global__ __launch_bounds__(BLOCK) void k_trace_seg(const TraceArgs* __restrict__ T, SegMap map) {
+    __shared__ uint32_t stack[SD * BLOCK];
+    const uint32_t sc = map.scene[blockIdx.x];
+    const uint32_t bid = blockIdx.x - map.first[sc];
+    const TraceArgs a = T[sc];
+    const uint32_t n = a.n_dev ? *a.n_dev : a.n;
+    const uint32_t i = bid * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const size_t rs = a.stride ? a.stride : n;
+    LaneStack<BLOCK, SD> S{stack, a.spill, a.spill_depth, (size_t)a.n, i, a.err};
+    RayState R;
+    ray_begin(R, a, i, rs);
+    while (R.node != NODE_DONE) ray_step<BLOCK, SD, WIDTH>(R, a, S);
+    ray_end(R, a, i, rs);
+}
+
+template <int BLOCK, int WIDTH>
+static void launch_seg_sd(const TraceArgs* T, const SegMap& map, uint32_t grid, int sd, hipStream_t st) {
+#define WPBR_TRACE_SEG(SDV) \
+    case SDV: hipLaunchKernelGGL((k_trace_seg<BLOCK, SDV, WIDTH>), dim3(grid), dim3(BLOCK), 0, st, T, map); break;
+    switch (sd) {
+        WPBR_TRACE_SEG(8)
+        WPBR_TRACE_SEG(16)
+        WPBR_TRACE_SEG(24)
+        WPBR_TRACE_SEG(32)
+        WPBR_TRACE_SEG(48)
+        default:
+        WPBR_TRACE_SEG(64)
+    }
+#undef WPBR_TRACE_SEG
+}
+
+template <int BLOCK>
+static void launch_seg_w(const TraceArgs* T, const SegMap& map, uint32_t grid, int sd, int width, hipStream_t st) {
+    if (width == 4) launch_seg_sd<BLOCK, 4>(T, map, grid, sd, st);
+    else launch_seg_sd<BLOCK, 2>(T, map, grid, sd, st);
+}
+
+// Traces the rays of several launch records in one launch (grid: the map's
+// total blocks for this block size).
+wpbr_status trace_launch_seg(const TraceArgs* T, const SegMap& map, uint32_t grid, int block, int sd, int width,
+                             hipStream_t st) {
+    if (grid == 0) return WPBR_OK;
+    switch (block) {
+        case 64: launch_seg_w<64>(T, map, grid, sd, width, st); break;
+        case 128: launch_seg_w<128>(T, map, grid, sd, width, st); break;
+        case 512: launch_seg_w<512>(T, map, grid, sd, width, st); break;
+        default: launch_seg_w<256>(T, map, grid, sd, width, st); break;
+    }
+    return launch_status();
+}
+
 template <int BLOCK, int WIDTH, bool DYN>
 static void launch_sd(const TraceArgs& a, int sd, hipStream_t st, uint32_t dyn_blocks) {
     dim3 grid;

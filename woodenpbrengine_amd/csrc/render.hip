@@ -353,8 +353,7 @@ WPBR_D void put_ray(float* q, uint32_t cap, uint32_t slot, f3 o, f3 d) {
 }
 
 // ---- kernels ----------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_camera(RenderLaunch L) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+WPBR_D void camera_body(const RenderLaunch& L, uint32_t i) {
     if (i >= L.n) return;
     const SceneView& sv = L.sv;
     uint32_t px, py;
@@ -377,14 +376,14 @@ __global__ __launch_bounds__(256) void k_camera(RenderLaunch L) {
     L.ws.rays[3 * (size_t)n + i] = d.x; L.ws.rays[4 * (size_t)n + i] = d.y; L.ws.rays[5 * (size_t)n + i] = d.z;
     if (i == 0) *L.ws.qcount = 0;
 }
+__global__ __launch_bounds__(256) void k_camera(RenderLaunch L) { camera_body(L, blockIdx.x * blockDim.x + threadIdx.x); }
 
 constexpr uint32_t F_LIGHT = 1u, F_BSDF = 2u, F_ENV = 4u, F_DONE = 8u, F_MISS = 16u;
 
-__global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+WPBR_D void shade1_body(const RenderLaunch& L, uint32_t i) {
     bool active = i < L.n;
     const SceneView& sv = L.sv;
-    WaveState& ws = L.ws;
+    const WaveState& ws = L.ws;
     const uint32_t n = L.n, cap = 2 * L.n;
     uint32_t flags = F_DONE;
     bool wantL = false, wantB = false;
@@ -534,6 +533,7 @@ __global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) {
     if (wantL) { put_ray(ws.qrays, cap, sl, lo, ld); ws.l_ray[i] = sl; ws.qtarget[sl] = tgtL; }
     if (wantB) { put_ray(ws.qrays, cap, sb, bo, bd); ws.b_ray[i] = sb; ws.qtarget[sb] = tgtB; }
 }
+__global__ __launch_bounds__(256) void k_shade1(RenderLaunch L) { shade1_body(L, blockIdx.x * blockDim.x + threadIdx.x); }
 
 WPBR_D float mitchell1(float x) {
     const float B = (float)(1.0 / 3.0f), Cc = (float)(1.0 / 3.0f);
@@ -649,16 +649,16 @@ WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& 
 // within two pixels of it; each lane gathers those from LDS in a fixed order
 // (deterministic sums, no atomics) and adds them to the film once.
 constexpr int TILE_THREADS = 1024;
-__global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L) {
+WPBR_D void splat_body(const RenderLaunch& L, uint32_t tile) {
     __shared__ float s_fx[1024], s_fy[1024], s_x[1024], s_y[1024], s_z[1024];
     __shared__ uint32_t s_black[1024];
     const SceneView& sv = L.sv;
     const uint32_t W = sv.width;
     const uint32_t ntx = W / 32;
-    const uint32_t tx0 = (blockIdx.x % ntx) * 32, ty0 = (blockIdx.x / ntx) * 32;
+    const uint32_t tx0 = (tile % ntx) * 32, ty0 = (tile / ntx) * 32;
     const uint32_t n = L.n;
     // secondary rays traced in this pass (statistics)
-    if (blockIdx.x == 0 && threadIdx.x == 0)
+    if (tile == 0 && threadIdx.x == 0)
         atomicAdd(reinterpret_cast<unsigned long long*>(L.ws.qcount + 2), (unsigned long long)*L.ws.qcount);
     const uint32_t lp = threadIdx.x;  // local pixel, row-major in the tile
     const int lx = (int)(lp & 31), ly = (int)(lp >> 5);
@@ -704,6 +704,55 @@ __global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L)
     float4 v = *f;
     v.x += ax; v.y += ay; v.z += az; v.w += aw;
     *f = v;
+}
+__global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L) { splat_body(L, blockIdx.x); }
+
+// ---- batched render: the stages of one sample pass for many scenes per launch.
+// Block b works for scene map.scene[b] as its block b - map.first[scene]; the
+// scene's launch record (scene view, wave state, film) is in a device table, the
+// pass index a kernel argument.
+#define SEG_BLOCK(map)                                   \
+    const uint32_t sc = (map).scene[blockIdx.x];         \
+    const uint32_t bid = blockIdx.x - (map).first[sc];
+
+__global__ __launch_bounds__(256) void ks_camera(const RenderLaunch* __restrict__ T, SegMap map, uint32_t j) {
+    SEG_BLOCK(map);
+    RenderLaunch L = T[sc];
+    L.spp_index = j;
+    camera_body(L, bid * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(256) void ks_shade1(const RenderLaunch* __restrict__ T, SegMap map, uint32_t j) {
+    SEG_BLOCK(map);
+    RenderLaunch L = T[sc];
+    L.spp_index = j;
+    shade1_body(L, bid * 256 + threadIdx.x);
+}
+__global__ __launch_bounds__(TILE_THREADS) void ks_radiance_splat(const RenderLaunch* __restrict__ T, SegMap map,
+                                                                  uint32_t j) {
+    SEG_BLOCK(map);
+    RenderLaunch L = T[sc];
+    L.spp_index = j;
+    splat_body(L, bid);
+}
+// Clears the films and the queue statistics of every scene (a kernel, so that
+// the pass loop can be captured in a graph).
+__global__ void ks_clear(const RenderLaunch* __restrict__ T, SegMap map) {
+    SEG_BLOCK(map);
+    const RenderLaunch& L = T[sc];
+    const uint32_t i = bid * 256 + threadIdx.x;
+    if (i < L.n) reinterpret_cast<float4*>(L.film)[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (i < 16) L.ws.qcount[i] = 0;
+}
+
+wpbr_status render_batch_launch(const RenderLaunch* T, const SegMap& map, uint32_t grid, int stage, uint32_t j,
+                                hipStream_t st) {
+    switch (stage) {
+        case 0: hipLaunchKernelGGL(ks_camera, dim3(grid), dim3(256), 0, st, T, map, j); break;
+        case 1: hipLaunchKernelGGL(ks_shade1, dim3(grid), dim3(256), 0, st, T, map, j); break;
+        case 2: hipLaunchKernelGGL(ks_radiance_splat, dim3(grid), dim3(TILE_THREADS), 0, st, T, map, j); break;
+        case 3: hipLaunchKernelGGL(ks_clear, dim3(grid), dim3(256), 0, st, T, map); break;
+    }
+    return launch_status();
 }
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {

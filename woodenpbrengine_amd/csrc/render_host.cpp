@@ -142,26 +142,14 @@ void ctx_release_render(wpbr_ctx* ctx) {
     ctx->render = nullptr;
 }
 
-wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
-    if (!ctx->render) ctx->render = new wpbr_render_state();
-    wpbr_render_state* r = ctx->render;
-    if (n <= r->cap) return WPBR_OK;
-    if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
-    // words per pixel: see the carve below
-    const size_t words = 2 + 2 + 6 + 1 + 1 + 3 + 3 + 6 + 1 + 5 + 1 + 3 + 3 + 6 * 2 + 2 + 2 + 2 + 2 + 18;
-    size_t bytes = (words * (size_t)n + 64) * 4 + 64 * 256;
-    wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
-    if (s) return s;
-    r->cap = n;
-    char* p = (char*)r->mem;
+size_t carve_wave(char* p, uint32_t n, WaveState& w) {
     size_t off = 0;
     auto take = [&](size_t count) -> void* {
         off = (off + 255) & ~size_t(255);
-        void* q = p + off;
+        void* q = p ? p + off : nullptr;
         off += count * 4;
         return q;
     };
-    WaveState& w = r->ws;
     w.c1 = (uint32_t*)take(n);
     w.c2 = (uint32_t*)take(n);
     w.pfilm = (float*)take(2 * (size_t)n);
@@ -197,15 +185,29 @@ wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     w.r_wl = (float*)take(3 * (size_t)n);
     w.r_pdfl = (float*)take(n);
     w.r_ub = (float*)take(2 * (size_t)n);
-    if (off > bytes) return set_err(ctx, WPBR_ERR_HIP, "render state carve overflow");
+    return (off + 255) & ~size_t(255);
+}
+
+wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
+    if (!ctx->render) ctx->render = new wpbr_render_state();
+    wpbr_render_state* r = ctx->render;
+    if (n <= r->cap) return WPBR_OK;
+    if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
+    WaveState tmp;
+    const size_t bytes = carve_wave(nullptr, n, tmp);
+    wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
+    if (s) return s;
+    r->cap = n;
+    carve_wave((char*)r->mem, n, r->ws);
     return WPBR_OK;
 }
 
 }  // namespace wpbr
 
-extern "C" {
 
-wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** out) {
+namespace wpbr {
+// Uploads a scene description (everything but the LBVH build).
+wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** out) {
     if (!ctx || !d || !out) return WPBR_ERR_INVALID_ARG;
     *out = nullptr;
     if (d->width == 0 || d->height == 0 || d->spp == 0 || (d->width % 32) || (d->height % 32))
@@ -308,10 +310,39 @@ wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scen
     sc->geom.n_triangles = nt;
     sc->geom.d_tri_verts = v.g.tri_verts;
     st = wpbr_lbvh_create(ctx, &sc->bvh);
-    if (!st) st = wpbr_lbvh_build(ctx, &sc->geom, sc->bvh);
     if (st) { wpbr_scene_destroy(sc); return st; }
     *out = sc;
     return WPBR_OK;
+}
+}  // namespace wpbr
+
+extern "C" {
+
+wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** out) {
+    wpbr_status st = scene_upload(ctx, d, out);
+    if (st) return st;
+    st = wpbr_lbvh_build(ctx, &(*out)->geom, (*out)->bvh);
+    if (st) { wpbr_scene_destroy(*out); *out = nullptr; }
+    return st;
+}
+
+wpbr_status wpbr_scenes_create(wpbr_ctx* ctx, uint32_t n_scenes, const wpbr_scene_desc* descs, wpbr_scene** out) {
+    if (!ctx || !descs || !out || n_scenes == 0) return WPBR_ERR_INVALID_ARG;
+    for (uint32_t i = 0; i < n_scenes; i++) out[i] = nullptr;
+    std::vector<wpbr_geometry> geoms(n_scenes);
+    std::vector<wpbr_lbvh*> bvhs(n_scenes);
+    wpbr_status st = WPBR_OK;
+    for (uint32_t i = 0; i < n_scenes && !st; i++) {
+        st = scene_upload(ctx, &descs[i], &out[i]);
+        if (st) { st = set_err(ctx, st, "scene " + std::to_string(i) + ": " + ctx->last_error); break; }
+        geoms[i] = out[i]->geom;
+        bvhs[i] = out[i]->bvh;
+    }
+    if (!st) st = wpbr_lbvh_build_segmented(ctx, n_scenes, geoms.data(), bvhs.data());
+    if (st) {
+        for (uint32_t i = 0; i < n_scenes; i++) { wpbr_scene_destroy(out[i]); out[i] = nullptr; }
+    }
+    return st;
 }
 
 void wpbr_scene_destroy(wpbr_scene* sc) {

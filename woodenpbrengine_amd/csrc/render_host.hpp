@@ -9,6 +9,13 @@
 namespace wpbr {
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
+// Batched stages over a device table of launch records: 0 camera, 1 shade1,
+// 2 radiance + splat, 3 clear films and queue statistics.
+wpbr_status render_batch_launch(const RenderLaunch* T, const SegMap& map, uint32_t grid, int stage, uint32_t j,
+                                hipStream_t st);
+wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** out);
+// Carves one wave state for n pixels at p (p null: only the size); returns the bytes used.
+size_t carve_wave(char* p, uint32_t n, WaveState& w);
 wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n);
 
 struct HostTables {

@@ -138,5 +138,8 @@ struct TraceArgs {
 };
 
 wpbr_status trace_launch(const TraceArgs& a, int block, int stack_depth, hipStream_t st, uint32_t dyn_blocks);
+// Batched traversal: T holds one record per scene (device), map assigns blocks to scenes.
+wpbr_status trace_launch_seg(const TraceArgs* T, const SegMap& map, uint32_t grid, int block, int stack_depth,
+                             int width, hipStream_t st);
 
 }  // namespace wpbr

@@ -26,6 +26,8 @@ EXPORTS = [
     "wpbr_scene_rebuild", "wpbr_render", "wpbr_render_stats", "wpbr_autotune_collide",
     "wpbr_autotune_render", "wpbr_scene_param_counts", "wpbr_scene_set_params", "wpbr_film_loss",
     "wpbr_render_backward", "wpbr_bsdf_derivatives", "wpbr_lbvh_build_segmented",
+    "wpbr_scenes_create", "wpbr_batch_create", "wpbr_batch_destroy", "wpbr_batch_render",
+    "wpbr_batch_stats", "wpbr_autotune_batch",
 ]
 
 
@@ -113,6 +115,12 @@ def load():
         "wpbr_render_backward": (i32, [vp, vp, vp, vp, vp]),
         "wpbr_bsdf_derivatives": (i32, [vp, C.c_uint32, vp, vp]),
         "wpbr_lbvh_build_segmented": (i32, [vp, u32, C.POINTER(Geometry), C.POINTER(vp)]),
+        "wpbr_scenes_create": (i32, [vp, u32, C.POINTER(SceneDesc), C.POINTER(vp)]),
+        "wpbr_batch_create": (i32, [vp, u32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]),
+        "wpbr_batch_destroy": (None, [vp]),
+        "wpbr_batch_render": (i32, [vp, vp]),
+        "wpbr_batch_stats": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.c_int]),
+        "wpbr_autotune_batch": (i32, [vp, vp, C.POINTER(TraceConfig)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -449,3 +457,53 @@ def bsdf_derivatives(ctx, wo, wi, alpha, eta, k, u):
     ctx.check(ctx.lib.wpbr_bsdf_derivatives(ctx.h, n, d_in.ptr, d_out.ptr))
     out = d_out.numpy()
     return {f: out[j] for j, f in enumerate(BSDF_GRAD_FIELDS)}
+
+
+class BatchRenderer:
+    """Many scenes rendered together (wpbr_scenes_create + wpbr_batch_*): one
+    segmented LBVH build, then per render one graph replay of the batched pass
+    loop.  films()[i] is scene i's film (H x W x 4), as Renderer.d_film."""
+
+    STATS = ["ms", "samples", "scenes", "launches", "secondary_rays"]
+
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        S = len(self.scenes)
+        descs, keeps = zip(*[scene_desc(s) for s in self.scenes])
+        arr = (SceneDesc * S)(*descs)
+        hs = (C.c_void_p * S)()
+        ctx.check(ctx.lib.wpbr_scenes_create(ctx.h, S, arr, hs))
+        del keeps
+        self.hs = [C.c_void_p(h) for h in hs]
+        self.d_films = [ctx.empty((s.height, s.width, 4), np.float32) for s in self.scenes]
+        fp = (C.c_void_p * S)(*[f.ptr.value for f in self.d_films])
+        b = C.c_void_p()
+        ctx.check(ctx.lib.wpbr_batch_create(ctx.h, S, hs, fp, C.byref(b)))
+        self.b = b
+
+    def render(self):
+        self.ctx.check(self.ctx.lib.wpbr_batch_render(self.ctx.h, self.b))
+        return self.ctx.last_ms()
+
+    def autotune(self):
+        cfg = TraceConfig()
+        self.ctx.check(self.ctx.lib.wpbr_autotune_batch(self.ctx.h, self.b, C.byref(cfg)))
+        return {f: getattr(cfg, f) for f, _ in TraceConfig._fields_}
+
+    def stats(self):
+        buf = (C.c_double * 5)()
+        k = self.ctx.lib.wpbr_batch_stats(self.ctx.h, self.b, buf, 5)
+        return {self.STATS[i]: float(buf[i]) for i in range(k)}
+
+    def film(self, i):
+        return self.d_films[i].numpy()
+
+    def destroy(self):
+        if self.b:
+            self.ctx.lib.wpbr_batch_destroy(self.b)
+            self.b = None
+        for h in self.hs:
+            if h:
+                self.ctx.lib.wpbr_scene_destroy(h)
+        self.hs = []

@@ -404,3 +404,23 @@ def c4_scene(width=512, height=512, spp=8, seed=SEED_BASE + 2, freq=317):
     """Differentiable render: the full render's scene at 512x512, 8 spp, with a
     256x128 env map (its texels are parameters)."""
     return c2_scene(width=width, height=height, spp=spp, seed=seed, freq=freq, env_size=(256, 128))
+
+
+C3_FILMS = [(128, 128), (256, 256), (512, 512), (1024, 512)]
+
+
+def c3_scenes(n_scenes=64, spp=8, seed=SEED_BASE + 3, max_triangles=500_000):
+    """Batched many-scene render: n_scenes independent scenes of the full
+    render's kind (displaced icosphere from a per-scene seed, 64 spheres, ground,
+    4 sphere lights, a 256x128 env map, materials as the full render), triangle
+    counts log-uniform in [1 000, max_triangles], films drawn from
+    128^2 / 256^2 / 512^2 / 1024x512, spp samples per pixel."""
+    g = rng(seed)
+    out = []
+    for k in range(n_scenes):
+        tris = float(np.exp(g.uniform(np.log(1e3), np.log(max_triangles))))
+        freq = max(2, int(round(np.sqrt(tris / 20.0))))
+        w, h = C3_FILMS[int(g.integers(len(C3_FILMS)))]
+        out.append(c2_scene(width=w, height=h, spp=spp, seed=seed + 1000 * (k + 1), freq=freq,
+                            env_size=(256, 128)))
+    return out
