@@ -137,6 +137,53 @@ def bench_adjoint(lib, scenes, ctx, steps, tune=True):
                         "grad_abs_max": [float(np.abs(x).max()) for x in grads]}}
 
 
+def bench_batch(lib, scenes, ctx, steps, tune=True):
+    """Batched many-scene render: 64 scenes (1k-500k triangles, films 128^2 ..
+    1024x512, 8 spp): the segmented LBVH build of all scenes, then the batched
+    render (one graph replay per step); beside it the same scenes rendered one
+    at a time with the single-scene renderer."""
+    ss = scenes.c3_scenes()
+    samples = sum(s.width * s.height * s.spp for s in ss)
+    shapes = sum(s.n_shapes for s in ss)
+    br = lib.BatchRenderer(ctx, ss)
+    # segmented build of all scenes into the batch's trees, timed alone
+    geo = [lib.Scene(ctx, s.sphere_xform, s.sphere_radius, s.tri_verts) for s in ss]
+    bv = [lib.Lbvh(ctx) for _ in ss]
+    lib.build_segmented(ctx, geo, bv)
+    seg_ms = float(np.median([lib.build_segmented(ctx, geo, bv) for _ in range(max(steps, 3))]))
+    for b in bv:
+        b.destroy()
+    del geo
+    tuned = br.autotune() if tune else ctx.trace_config()
+    br.render()
+    ms = []
+    for _ in range(max(steps, 3)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        br.render()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    st = br.stats()
+    br.destroy()
+    t = float(np.median(ms))
+    # one scene at a time (wpbr_render per scene, same traversal configuration)
+    single = 0.0
+    for s in ss:
+        r = lib.Renderer(ctx, s)
+        r.render()
+        ctx.sync()
+        t0 = time.perf_counter()
+        r.render()
+        single += (time.perf_counter() - t0) * 1e3
+        r.destroy()
+    return {"batch": {"workload": "batched many-scene render: %d scenes, %d shapes, %d pixels, 8 spp"
+                                  % (len(ss), shapes, samples // 8),
+                      "segmented_build_ms": seg_ms, "segmented_build_mprims_per_s": shapes / seg_ms / 1e3,
+                      "render_ms": t, "msamples_per_s": samples / t / 1e3,
+                      "device_ms": st.get("ms"), "secondary_rays": st.get("secondary_rays"),
+                      "one_scene_at_a_time_ms": single, "batched_speedup": single / t,
+                      "trace_config": tuned}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +193,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--trace-config", default=None,
                     help="fixed traversal configuration block,stack_depth,mapping,width,refill,chunk "
                          "instead of the autotuner (for profiling one configuration)")
@@ -240,6 +288,10 @@ def main():
         adj = bench_adjoint(lib, scenes, ctx, args.steps, tune=not args.trace_config)
         log("adjoint:", json.dumps(adj))
         mb.update(adj)
+    if not args.no_batch:
+        bat = bench_batch(lib, scenes, ctx, args.steps, tune=not args.trace_config)
+        log("batch:", json.dumps(bat))
+        mb.update(bat)
     cpu = None
     if not args.no_cpu_baseline:
         t1 = time.time()
