@@ -10,7 +10,7 @@
  *
  * Reference interfaces each group replaces (WoodenPBREngine/...):
  *   wpbr_shape_bounds        JobUpdateBoundsAndCentroidTriangle / ...Sphere
- *                            (CSTriangle.cpp:190-216, CSSphere.h:126-153)
+ *                            (CSTriangle.cpp:116-142, CSSphere.h:126-153)
  *   wpbr_lbvh_build          MEngine::buildLBVH (MEngine.cpp:341-350): the jobs
  *                            JobGenerateShapesCentroidBound (SLBVH.cpp:164),
  *                            JobGenerateShapesMortonCode (:186),
@@ -166,7 +166,7 @@ wpbr_status wpbr_autotune_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, cons
  * Shape order as in wpbr_geometry.  Materials are 9 floats each: type (u32
  * bits: 0 conductor microfacet, 1 dielectric microfacet, 2 specular reflection
  * "perfect glass"), alpha, eta (dielectric etaT, etaI = 1), eta RGB[3] and k RGB[3]
- * (conductor; converted with fromRGB(Reflectance) as CMaterialMetal.cpp:123-127).
+ * (conductor; converted with fromRGB(Reflectance) as CMaterialMetal.cpp:46).
  * Lights: sphere lights (their shape ids and 64-bin emission), then the
  * infinite env light if has_env.  The camera is a raster->camera 4x4 and a
  * camera->world 3x4 (CSCameraProjective.h:25-52). */
@@ -200,7 +200,7 @@ typedef struct wpbr_scene wpbr_scene;
 
 /* Uploads the scene, runs the load-time preprocessing (spectral tables,
  * SampledSpectrum::init CSpectrumSampled.h:236-281; env-light distribution,
- * JobLightInfiniteAreaPreprocessSampling CLightInfiniteArea.cpp:113-139) and
+ * JobLightInfiniteAreaPreprocessSampling CLightInfiniteArea.cpp:44-70) and
  * builds its LBVH (wpbr_lbvh_build). */
 wpbr_status wpbr_scene_create(wpbr_ctx* ctx, const wpbr_scene_desc* desc, wpbr_scene** out);
 void wpbr_scene_destroy(wpbr_scene* scene);
@@ -262,7 +262,7 @@ wpbr_status wpbr_scene_param_counts(wpbr_scene* scene, uint32_t counts[3]);
 wpbr_status wpbr_scene_set_params(wpbr_ctx* ctx, wpbr_scene* scene, const float* alpha,
                                   const float* eta_k, const float* env_rgb);
 /* The film's image, rgb = xyzToRGB(sum XYZ w) / sum w (CoefficientSpectrum.h:14-24,
- * CSFilm.cpp:129-170 before the tonemap), its L2 loss against d_target_rgb
+ * CSFilm.cpp:141-170 before the tonemap), its L2 loss against d_target_rgb
  * (W*H*3) in *loss and the loss gradient 2 (rgb - target) in d_dL_drgb.  Any
  * output may be null. */
 wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,

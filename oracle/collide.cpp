@@ -58,7 +58,7 @@ float tri_intersect(const float* v, V3 o, V3 d, float* bary) {
         py[i] = q.y + sy * q.z;
         pz[i] = q.z + 0.0f * q.z;
     }
-    // res = X * rot(Y) - rot(X) * Y with rot = (1,2,0)  (CSTriangle.cpp:134-137)
+    // res = X * rot(Y) - rot(X) * Y with rot = (1,2,0)  (CSTriangle.cpp:60-63)
     float r0 = px[0] * py[1] - px[1] * py[0];
     float r1 = px[1] * py[2] - px[2] * py[1];
     float r2 = px[2] * py[0] - px[0] * py[2];

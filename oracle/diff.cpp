@@ -7,7 +7,7 @@
 // samples, the env-map cells and the fromRGB case of each lookup
 // (SampleRec).  diff_loss() re-evaluates the same estimator in double
 // precision at any parameter values with those decisions frozen, then splats
-// (Mitchell, clipped to the tile, CSFilm.cpp:199-251), converts to RGB
+// (Mitchell, clipped to the tile, CSFilm.cpp:70-122), converts to RGB
 // (CoefficientSpectrum.h:14-24) and returns the L2 loss against a target image.
 // Central differences of diff_loss are the derivative that a path-replay
 // adjoint of this estimator computes (visibility and cell changes excluded).
@@ -15,7 +15,7 @@
 // Parameters: per material alpha, conductor eta/k as RGB (fromRGB(Reflectance),
 // CMaterialMetal.cpp:46) and the env map texels (RGB), which enter the env
 // light's radiance (CLightInfiniteArea.cpp:72-92, 206-237) and its sampling
-// distribution (CLightInfiniteArea.cpp:113-139, pdf in the BSDF strategy's MIS
+// distribution (CLightInfiniteArea.cpp:44-70, pdf in the BSDF strategy's MIS
 // weight).
 #include <atomic>
 #include <cmath>
@@ -135,7 +135,7 @@ double diff_loss(const DiffState& D, const double* alpha, const double* etak, co
                  const float* target, float* rgb_out, int nthreads) {
     const Tables& T = tables();
     const uint32_t EW = D.env_w, EH = D.env_h;
-    // env distribution (CLightInfiniteArea.cpp:113-139) in double: the green
+    // env distribution (CLightInfiniteArea.cpp:44-70) in double: the green
     // channel of a bilinear lookup at (x/W, y/H) times sin(theta), stored at the
     // reference's index x + y * H
     std::vector<double> func((size_t)EW * EH, 0.0);

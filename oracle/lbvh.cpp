@@ -1,7 +1,7 @@
 // Oracle: LBVH build (MEngine::buildLBVH, MEngine.cpp:341-350).
 // TEST INFRASTRUCTURE ONLY - restates the reference's CPU algorithm, single thread.
 //
-//   shape bounds/centroids   CSTriangle.cpp:190-216, CSSphere.h:136-152, CCentroid.h:10-13
+//   shape bounds/centroids   CSTriangle.cpp:116-142, CSSphere.h:136-152, CCentroid.h:10-13
 //   centroid bounds          SLBVH.cpp:164-181
 //   Morton codes             SLBVH.cpp:186-227
 //   6-bit LSD radix sort     SLBVH.cpp:231-295 (stable; 5 passes over 30 bits)
@@ -18,7 +18,7 @@ namespace orc {
 void triangle_bounds(uint32_t n, const float* v, float* bounds, float* centroid) {
     for (uint32_t i = 0; i < n; i++) {
         const float* t = v + 9 * (size_t)i;
-        // DBounds3f(p0, p1) then union with p2 (CSTriangle.cpp:211-213).
+        // DBounds3f(p0, p1) then union with p2 (CSTriangle.cpp:137-139).
         Bounds b = bounds_empty();
         b = bounds_union_pt(b, v3(t[0], t[1], t[2]));
         b = bounds_union_pt(b, v3(t[3], t[4], t[5]));

@@ -169,7 +169,7 @@ void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nt
             const uint32_t* tile_subset, uint32_t n_tiles, SampleRec* recs = nullptr);
 // Smits case index (0..5) of rgb, in the order of from_rgb's branches.
 int rgb_case(const float* rgb);
-float mitchell_weight(float x);  // the film's Mitchell filter (CSFilm.h:62-86)
+float mitchell_weight(float x);  // the film's Mitchell filter (CSFilm.h:20-44)
 
 // Finite-difference reference (diff.cpp).
 struct DiffState;

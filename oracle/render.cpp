@@ -4,28 +4,28 @@
 // Per sample, in the reference's job order:
 //   camera sample + ray          CSSampler.cpp:7-24, CSCameraPerspective.cpp:9-52
 //   runCollisionSystem           collide.cpp (trace_ray)
-//   emitted light for misses     SScattering.cpp:20-80, CLightInfiniteArea.cpp:141-160
-//   surface interaction          CSTriangle.cpp:223-393, CSSphere.cpp:397-461, CTransform.h:41-66
-//   BSDF requests / transform    CMaterial.h:30-38, CMaterial*.cpp, CBSDF.cpp:139-151
-//   light pick                   SScattering.cpp:214-228
-//   light sample                 CSSphere.h:187-230 + CSSphere.cpp:523-556,
-//                                CLightInfiniteArea.cpp:163-204
-//   shadow ray + BSDF compute    SScattering.cpp:231-257, CBSDF*.cpp Compute jobs
-//   integrate (light strategy)   SScattering.cpp:259-299
-//   BSDF sample + ray            SScattering.cpp:303-338, CBSDF*.cpp Sample jobs
-//   light Li compute             CSSphere.h:232-267 + CSSphere.cpp:496-505,
-//                                CLightInfiniteArea.cpp:206-237
-//   integrate (BSDF strategy)    SScattering.cpp:341-386
-//   film splat                   CSFilm.cpp:199-251 (Mitchell r=2, B=C=1/3), clipped to
+//   emitted light for misses     SScattering.cpp:20-80, CLightInfiniteArea.cpp:72-92
+//   surface interaction          CSTriangle.cpp:149-319, CSSphere.cpp:122-186, CTransform.h:41-66
+//   BSDF requests / transform    CMaterial.h:30-38, CMaterial*.cpp, CBSDF.cpp:12-24
+//   light pick                   SScattering.cpp:82-88
+//   light sample                 CSSphere.h:187-230 + CSSphere.cpp:248-281,
+//                                CLightInfiniteArea.cpp:94-135
+//   shadow ray + BSDF compute    SScattering.cpp:99-125, CBSDF*.cpp Compute jobs
+//   integrate (light strategy)   SScattering.cpp:127-167
+//   BSDF sample + ray            SScattering.cpp:171-206, CBSDF*.cpp Sample jobs
+//   light Li compute             CSSphere.h:232-267 + CSSphere.cpp:221-246,
+//                                CLightInfiniteArea.cpp:137-168
+//   integrate (BSDF strategy)    SScattering.cpp:209-254
+//   film splat                   CSFilm.cpp:70-122 (Mitchell r=2, B=C=1/3), clipped to
 //                                the sample's 32x32 tile
 //
 // Choices where the reference is undefined:
-//   * light pick index clamped to n-1 (SScattering.cpp:217 overruns for n >= 3);
+//   * light pick index clamped to n-1 (SScattering.cpp:85 overruns for n >= 3);
 //   * a rejected microfacet sample gives f = 0, pdf = 0 and casts no BSDF ray
-//     (CBSDFConductorMicroface.cpp:292-295 leaves pdf and the ray unset);
+//     (CBSDFConductorMicroface.cpp:75, 99-115 leaves pdf and the ray unset);
 //   * a hit on a shape with no material (a light sphere) gets no BSDF and no
 //     emission (CMaterial.h:34 is undefined there; emission on hit is disabled);
-//   * the env-light sampling strategy stores a black Li (CLightInfiniteArea.cpp:192-199),
+//   * the env-light sampling strategy stores a black Li (CLightInfiniteArea.cpp:106, 126-131),
 //     so its shadow ray cannot contribute and is skipped;
 //   * bilinear texel indices are clamped to the texture (CSTextureProcess.h:64-74
 //     clamps only below).
@@ -168,7 +168,7 @@ struct EnvDist {
     }
 };
 
-// JobLightInfiniteAreaPreprocessSampling, CLightInfiniteArea.cpp:113-139: the
+// JobLightInfiniteAreaPreprocessSampling, CLightInfiniteArea.cpp:44-70: the
 // green channel of a bilinear lookup at (x/W, y/H) times sin(theta), written to
 // data[x + y * H] (the reference's index) and distributed over W x H.
 static void build_env_dist(const RenderScene& sc, EnvDist& d) {
@@ -265,7 +265,7 @@ struct MatSpec {
 };
 
 // BSDF value (already multiplied by |cos theta_i|) and pdf for a direction pair
-// in the local frame: the Compute jobs (CBSDFConductorMicroface.cpp:301-314,
+// in the local frame: the Compute jobs (CBSDFConductorMicroface.cpp:84-97,
 // CBSDFDielectricMicroface.cpp:84-97).  Returns false when f is exactly 0.
 static void bsdf_microfacet_f(const Material& m, const MatSpec& ms, V3 wo, V3 wi, Spec& val,
                               float* pdf) {
@@ -294,7 +294,7 @@ struct SI {
     V3 p, n, ns, dpdu, wo;
 };
 
-// JobProcessTriangleFullInteractionRequests, CSTriangle.cpp:223-393.
+// JobProcessTriangleFullInteractionRequests, CSTriangle.cpp:149-319.
 static SI si_triangle(const RenderScene& sc, uint32_t tri, V3 rd, const float* b) {
     const float* v = sc.geom.tri_verts + 9 * (size_t)tri;
     V3 P0 = v3(v[0], v[1], v[2]), P1 = v3(v[3], v[4], v[5]), P2 = v3(v[6], v[7], v[8]);
@@ -346,7 +346,7 @@ static SI si_triangle(const RenderScene& sc, uint32_t tri, V3 rd, const float* b
     return s;
 }
 
-// SSphere::computeSurfInteraction + CTransform::operator()(SI), CSSphere.cpp:397-461.
+// SSphere::computeSurfInteraction + CTransform::operator()(SI), CSSphere.cpp:122-186.
 // Vector-by-double-scalar products take the scalar rounded to float.
 static SI si_sphere(const RenderScene& sc, uint32_t sid, V3 o, V3 d, float t) {
     const float* xf = sc.geom.sphere_xform + 24 * (size_t)sid;
@@ -378,7 +378,7 @@ struct LightSample {
     bool black;  // Li is black: no light-strategy contribution
 };
 
-// SSphere::sample, CSSphere.cpp:523-556.
+// SSphere::sample, CSSphere.cpp:248-281.
 static V3 sphere_light_sample(const RenderScene& sc, uint32_t sid, V3 p, const float* u, float* pdf) {
     const float* xf = sc.geom.sphere_xform + 24 * (size_t)sid;
     Affine W = load_affine(xf);
@@ -403,7 +403,7 @@ static V3 sphere_light_sample(const RenderScene& sc, uint32_t sid, V3 p, const f
     return xf_point(W, pL);
 }
 
-// SSphere::pdf with the comma operator (CSSphere.cpp:502-505): distance from the origin.
+// SSphere::pdf with the comma operator (CSSphere.cpp:221-246): distance from the origin.
 static float sphere_light_pdf(const RenderScene& sc, uint32_t sid) {
     const float* xf = sc.geom.sphere_xform + 24 * (size_t)sid;
     Affine W = load_affine(xf);
@@ -533,7 +533,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     if (rec) { rec->kind = 2; rec->mid = mid; rec->li = li; rec->env_pick = envPick; rec->pdfL = pdfL; }
     const Material& m = sc.materials[mid];
     const MatSpec& ms = C.mats[mid];
-    // BSDF frame (CBSDF.cpp:139-151)
+    // BSDF frame (CBSDF.cpp:12-24)
     V3 ss = normalize3(si.dpdu);
     V3 ns = si.ns;
     V3 ts = cross3(ns, ss);
@@ -639,7 +639,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     return L;
 }
 
-// Mitchell filter, CSFilm.h:62-86 (radius 2, B = C = 1/3 passed as 1.0/3.0f).
+// Mitchell filter, CSFilm.h:20-44 (radius 2, B = C = 1/3 passed as 1.0/3.0f).
 static float mitchell1(float x) {
     const float B = (float)(1.0 / 3.0f), Cc = (float)(1.0 / 3.0f);
     x = std::fabs(2 * x);
@@ -706,7 +706,7 @@ void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nt
                         to_xyz(L, &out.sample_xyz[3 * si_]);
                         out.sample_pfilm[2 * si_] = pf[0];
                         out.sample_pfilm[2 * si_ + 1] = pf[1];
-                        // splat (CSFilm.cpp:221-246)
+                        // splat (CSFilm.cpp:92-117)
                         float dx = pf[0] - 0.5f, dy = pf[1] - 0.5f;
                         int x0 = (int)std::ceil(dx - 2.0f), x1 = (int)std::floor(dx + 2.0f);
                         int y0 = (int)std::ceil(dy - 2.0f), y1 = (int)std::floor(dy + 2.0f);

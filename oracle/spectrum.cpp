@@ -1,7 +1,7 @@
 // Oracle: 64-bin sampled spectrum (CSpectrumSampled.h, CSpectrumSampled.cpp,
 // CoefficientSpectrum.h).  TEST INFRASTRUCTURE ONLY.
 //
-//   averageSpectrumSamples  CSpectrumSampled.h:87-130
+//   averageSpectrumSamples  CSpectrumSampled.h:46-89
 //   SampledSpectrum::init   CSpectrumSampled.h:236-281 (64 bins over 400-700 nm)
 //   fromRGB                 CSpectrumSampled.h:106-209 (Smits, Reflectance/Illuminant)
 //   toXYZ / toRGB           CSpectrumSampled.cpp:27-45, xyzToRGB CoefficientSpectrum.h:14-24
@@ -31,7 +31,7 @@ float average_spectrum_samples(const float* lambda, const float* vals, int n, fl
     for (; j + 1 < n && l1 >= lambda[j]; j++) {
         float s0 = std::max(l0, lambda[j]);
         float s1 = std::min(l1, lambda[j + 1]);
-        // (a + b) * (s1 - s0) in float, then "* 0.5" and "+=" in double (CSpectrumSampled.h:126-127)
+        // (a + b) * (s1 - s0) in float, then "* 0.5" and "+=" in double (CSpectrumSampled.h:85-86)
         float prod = (interp(s0, j) + interp(s1, j)) * (s1 - s0);
         sum = (float)((double)sum + (double)prod * 0.5);
     }

@@ -205,7 +205,7 @@ def test_triangle_closed_form():
     tri_y = np.array([0, 0, 0, 1, 0, 0, 0, 0, 1], np.float32)
     t, _ = po.tri_intersect(tri_y, [0.2, 5, 0.2], [0, -1, 0])
     # det is summed before the (1,2,0) permutation and tScale after it
-    # (CSTriangle.cpp:145-154), so t is within a few ULP of 5, not exact
+    # (CSTriangle.cpp:71-95), so t is within a few ULP of 5, not exact
     assert abs(t - 5.0) <= 4 * np.spacing(np.float32(5.0))
     # oblique: t equals the plane distance to float precision
     d = np.array([0.3, 0.2, 1.0])

@@ -15,12 +15,12 @@
 //     strategy, the sampled direction (forward-mode dual numbers along the
 //     replayed sample: half vector, reflected direction, env lookup, sin theta);
 //   * conductor eta/k enter the 64-wavelength Fresnel term (CBXDF.h:200-238)
-//     through fromRGB(Reflectance) (CSpectrumSampled.h:147-199);
+//     through fromRGB(Reflectance) (CSpectrumSampled.h:110-158);
 //   * env texels enter the env radiance of misses and of BSDF rays
 //     (CLightInfiniteArea.cpp:72-92, 206-237) through the bilinear lookup and
 //     fromRGB(Illuminant), and the env light pdf of the MIS weight through the
 //     sampling distribution's cell value and its integral
-//     (CLightInfiniteArea.cpp:113-139).
+//     (CLightInfiniteArea.cpp:44-70).
 // Visibility, the light pick, rejected samples, env-map cells and fromRGB cases
 // are held at their forward values (their changes are measure-zero events).
 #include "wpbr_device.hpp"
@@ -127,7 +127,7 @@ WPBR_D D3 fresnel_conductor3(float cosI, float eta, float k) {
     return (Rp + Rs) * 0.5f;
 }
 
-// fromRGB ordering case (the branches of CSpectrumSampled.h:147-247): the
+// fromRGB ordering case (the branches of CSpectrumSampled.h:110-207): the
 // channels holding the min / mid / max value and the bases of the three terms
 // (basis 0 is white).  d spectrum / d rgb[ch_min] = B0 - B1, [ch_mid] = B1 - B2,
 // [ch_max] = B2.
@@ -410,7 +410,7 @@ WPBR_D void adjoint_sample(const AdjArgs& a, uint32_t i, float* s_part) {
                     Ienv = sv.env_marg_int[0];
                     pL2 = (fcell / Ienv) / (sinT * (float)(2 * PI_REF * PI_REF));
                 } else {
-                    // sphere light: constant pdf (CSSphere.cpp:502-505)
+                    // sphere light: constant pdf (CSSphere.cpp:221-246)
                     pL2 = dc<1>(0.0f);
                 }
                 if (!black) {

@@ -2,7 +2,7 @@
 //
 // Kernel chain (one stream, no host round trip until the final node count):
 //   k_init          zero histograms, tile counters, look-back status, treelet bits
-//   k_bounds        a1: per-shape bounds + centroid (CSTriangle.cpp:190-216,
+//   k_bounds        a1: per-shape bounds + centroid (CSTriangle.cpp:116-142,
 //                   CSSphere.h:136-152); a2: centroid bounds by wave reduction and
 //                   order-preserving-uint atomics (SLBVH.cpp:164-181)
 //   k_morton        a3: 30-bit Morton codes (SLBVH.cpp:186-227) + the 4x256 digit

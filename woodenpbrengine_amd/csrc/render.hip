@@ -7,18 +7,18 @@
 //             (CSSampler.cpp:7-24, CSCameraPerspective.cpp:9-52)
 //   trace     runCollisionSystem for primary rays (collide.hip)
 //   k_shade1  emitted light of misses (SScattering.cpp:20-80), surface interaction
-//             (CSTriangle.cpp:223-393 / CSSphere.cpp:397-461), BSDF request + frame
-//             (CMaterial.h:30-38, CBSDF.cpp:139-151), light pick and sample
-//             (SScattering.cpp:214-228, CSSphere.h:187-230), BSDF Compute toward the
+//             (CSTriangle.cpp:149-319 / CSSphere.cpp:122-186), BSDF request + frame
+//             (CMaterial.h:30-38, CBSDF.cpp:12-24), light pick and sample
+//             (SScattering.cpp:82-88, CSSphere.h:187-230), BSDF Compute toward the
 //             light and BSDF Sample (CBSDF*.cpp); shadow and BSDF rays go to one
 //             queue compacted with wave ballots (one atomic per wave)
 //   trace     both secondary rays of every sample in one launch
-//   k_radiance_splat  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:206-237),
-//             both MIS integrations (SScattering.cpp:259-386) over the 64
+//   k_radiance_splat  Li compute (CSSphere.h:232-267, CLightInfiniteArea.cpp:137-168),
+//             both MIS integrations (SScattering.cpp:127-254) over the 64
 //             wavelengths, XYZ, and the Mitchell splat into the sample's 32x32
-//             tile (CSFilm.cpp:199-251), gathered per film pixel
+//             tile (CSFilm.cpp:70-122), gathered per film pixel
 // The reference's light sample for the env light stores a black Li
-// (CLightInfiniteArea.cpp:192-199): that strategy adds nothing, so its shadow
+// (CLightInfiniteArea.cpp:106, 126-131): that strategy adds nothing, so its shadow
 // ray is not cast (the 2D sample is still consumed).
 #include "wpbr_device.hpp"
 #include "render_types.hpp"
@@ -104,7 +104,7 @@ WPBR_D float env_pdf(const SceneView& sv, float u, float v) {
     iv = iv < 0 ? 0 : (iv > nv - 1 ? nv - 1 : iv);
     return sv.env_cond_func[(size_t)iv * nu + iu] / sv.env_marg_int[0];
 }
-// Smits illuminant spectrum of rgb (CSpectrumSampled.h:200-247): three bases and weights.
+// Smits illuminant spectrum of rgb (CSpectrumSampled.h:159-207): three bases and weights.
 WPBR_D void rgb_basis(const float* rgb, int* b, float* w) {
     float r = rgb[0], g = rgb[1], bl = rgb[2];
     if (r <= g && r <= bl) {
@@ -643,8 +643,8 @@ WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& 
 }
 
 // One workgroup per 32x32 film tile, one lane per pixel: the radiance of the
-// pixel's sample of this pass, then the Mitchell splat (CSFilm.cpp:199-251).
-// The reference clips a sample's splat to its own tile (CSFilm.cpp:224-225), so
+// pixel's sample of this pass, then the Mitchell splat (CSFilm.cpp:70-122).
+// The reference clips a sample's splat to its own tile (CSFilm.cpp:95-96), so
 // every film pixel of the tile receives only the samples of the tile pixels
 // within two pixels of it; each lane gathers those from LDS in a fixed order
 // (deterministic sums, no atomics) and adds them to the film once.

@@ -13,7 +13,7 @@ namespace wpbr {
 
 constexpr double PI_REF_H = 3.14;  // pch.h:30
 
-// ---- SampledSpectrum::init (CSpectrumSampled.h:87-130, 236-281) ---------------
+// ---- SampledSpectrum::init (CSpectrumSampled.h:46-89, 236-281) ---------------
 float lerp_h(float a, float b, float t) { return (1 - t) * a + t * b; }
 
 float average_samples(const float* lambda, const float* vals, int n, float l0, float l1) {
@@ -63,7 +63,7 @@ const HostTables& host_tables() {
     return t;
 }
 
-// fromRGB(rgb, Reflectance) (CSpectrumSampled.h:147-199).
+// fromRGB(rgb, Reflectance) (CSpectrumSampled.h:110-158).
 void from_rgb_refl(const float* rgb, float* s) {
     const float* B = host_tables().refl;
     for (int i = 0; i < NSPEC; i++) s[i] = 0.0f;
@@ -102,7 +102,7 @@ void env_bilerp_h(const float* rgbTex, int W, int H, float s_, float t_, float* 
     }
 }
 
-// JobLightInfiniteAreaPreprocessSampling (CLightInfiniteArea.cpp:113-139) followed by
+// JobLightInfiniteAreaPreprocessSampling (CLightInfiniteArea.cpp:44-70) followed by
 // the pbrt-v3 Distribution2D constructor: returns the per-row functions (the
 // reference's data array, whose index is x + y * H) and the marginal integral.
 void env_distribution(const float* rgbTex, uint32_t W, uint32_t H, std::vector<float>& func,

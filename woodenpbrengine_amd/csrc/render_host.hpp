@@ -24,11 +24,11 @@ struct HostTables {
     float illum[7 * NSPEC];
 };
 const HostTables& host_tables();
-// fromRGB(rgb, Reflectance) (CSpectrumSampled.h:147-199).
+// fromRGB(rgb, Reflectance) (CSpectrumSampled.h:110-158).
 void from_rgb_refl(const float* rgb, float* s);
 // Bilinear mip-0 lookup (CSTextureProcess.h:58-81), indices clamped to the texture.
 void env_bilerp_h(const float* rgbTex, int W, int H, float s_, float t_, float* out);
-// The env light's sampling distribution (CLightInfiniteArea.cpp:113-139).
+// The env light's sampling distribution (CLightInfiniteArea.cpp:44-70).
 void env_distribution(const float* rgbTex, uint32_t W, uint32_t H, std::vector<float>& func,
                       std::vector<float>& rowInt, float& margInt);
 
