@@ -43,6 +43,7 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     const size_t N = std::max<uint32_t>(n, 1);
     const size_t tiles = (N + 3839) / 3840;
     s.bounds = c.take<float>(6 * N);
+    s.slot_bounds = c.take<float>(6 * N);
     s.centroid = c.take<float>(3 * N);
     s.cb_ord = c.take<uint32_t>(8 + 6 * 512);
     s.morton = c.take<uint32_t>(N);
@@ -71,10 +72,19 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.r_size = c.take<uint32_t>(2 * N);
     s.r_off = c.take<uint32_t>(2 * N);
     s.r_box = c.take<float>(12 * N);
-    s.task_i0 = c.take<uint32_t>(8192);
-    s.task_i1 = c.take<uint32_t>(8192);
-    s.task_dep = c.take<uint32_t>(8192);
-    s.task_rt = c.take<uint32_t>(8192);
+    s.task_i0 = c.take<uint32_t>(SAH_TASKS);
+    s.task_i1 = c.take<uint32_t>(SAH_TASKS);
+    s.task_dep = c.take<uint32_t>(SAH_TASKS);
+    s.task_rt = c.take<uint32_t>(SAH_TASKS);
+    s.g_perm = c.take<uint32_t>(4096);
+    s.g_la = c.take<uint32_t>(4096);
+    s.g_lb = c.take<uint32_t>(4096);
+    s.sub_i0 = c.take<uint32_t>(4096);
+    s.sub_i1 = c.take<uint32_t>(4096);
+    s.sub_dep = c.take<uint32_t>(4096);
+    s.sub_rt = c.take<uint32_t>(4096);
+    s.sub_n = c.take<uint32_t>(1);
+    s.sub_ub = c.take<uint32_t>(4096);
     s.u_i0 = c.take<uint32_t>(4096);
     s.u_i1 = c.take<uint32_t>(4096);
     s.u_mid = c.take<uint32_t>(4096);
@@ -350,6 +360,7 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
     auto reserve = [&](size_t b) { size_t o = (bytes + 255) & ~size_t(255); bytes = o + b; return o; };
     const size_t o_s = reserve(sizeof(LbvhScratch) * n_scenes), o_g = reserve(sizeof(GeomView) * n_scenes),
                  o_n = reserve(4 * (size_t)n_scenes), o_work = reserve(4 * ((size_t)n_scenes + 1)),
+                 o_subwork = reserve(4 * ((size_t)n_scenes + 1)),
                  o_info = reserve(20 * (size_t)n_scenes);
     size_t o_first[SEG_KINDS], o_count[SEG_KINDS], o_scene[SEG_KINDS];
     for (int k = 0; k < SEG_KINDS; k++) {
@@ -375,6 +386,7 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
     B.n = (const uint32_t*)(d + o_n);
     B.n_scenes = n_scenes;
     B.work = (uint32_t*)(d + o_work);
+    B.subwork = (uint32_t*)(d + o_subwork);
     B.info = (uint32_t*)(d + o_info);
     for (int k = 0; k < SEG_KINDS; k++) {
         put(o_first[k], first[k].data(), 4 * (size_t)n_scenes);

@@ -15,9 +15,11 @@
 //   k_emit          a5: one workgroup per treelet builds the emitLBVH tree
 //                   (SLBVH.cpp:353-406) breadth-first, then subtree sizes and
 //                   bounds bottom-up and DFS-preorder offsets top-down
-//   k_upper_sah     a6: one workgroup builds the 12-bucket SAH tree over the
-//                   treelet roots (SLBVH.cpp:457-563), reproducing the
-//                   libstdc++ std::partition order, and lays it out in preorder
+//   k_sah_top/sub/final  a6: the 12-bucket SAH tree over the treelet roots
+//                   (SLBVH.cpp:457-563), reproducing the libstdc++
+//                   std::partition order: one workgroup splits the large
+//                   ranges, many workgroups finish the small ones, one lays the
+//                   tree out in preorder
 //   k_scatter       a6: flatten (SLBVH.cpp:432-455): treelet nodes to their final
 //                   preorder slots
 //   k_subtree(_big) derived traversal layout: per interior node both child boxes
@@ -47,7 +49,7 @@ WPBR_D void init_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
     for (uint32_t j = i; j < (uint32_t)s.status_words; j += stride) s.status0[j] = 0;
     for (uint32_t j = i; j < 128; j += stride) s.presence[j] = 0;
     if (i < 8) s.tile_counter[i] = 0;
-    if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; *s.big_count = 0; }
+    if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; *s.big_count = 0; *s.sub_n = 0; }
 }
 __global__ void k_init(LbvhScratch s) { init_body(s, blockIdx.x, gridDim.x); }
 
@@ -360,6 +362,19 @@ __global__ void k_treelet_mark(const uint32_t* __restrict__ sorted, uint32_t n, 
     treelet_mark_body(sorted, n, bin_start, presence, blockIdx.x);
 }
 
+// Shape bounds gathered into slot (sorted) order, as the reference's sort job
+// gathers them with the entities (SLBVH.cpp:231-324): the treelet and leaf
+// kernels then read contiguous boxes.
+WPBR_D void slot_bounds_body(const LbvhScratch& s, uint32_t n, uint32_t bid) {
+    const uint32_t p = bid * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const float* src = s.bounds + 6 * (size_t)s.vals_a[p];
+    float* dst = s.slot_bounds + 6 * (size_t)p;
+#pragma unroll
+    for (int c = 0; c < 6; c++) dst[c] = src[c];
+}
+__global__ void k_slot_bounds(LbvhScratch s, uint32_t n) { slot_bounds_body(s, n, blockIdx.x); }
+
 WPBR_D uint32_t presence_rank(const uint32_t* presence, uint32_t bin) {
     uint32_t r = 0;
     for (uint32_t w = 0; w < (bin >> 5); w++) r += __popc(presence[w]);
@@ -464,7 +479,7 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
                 uint32_t rs = s.r_start[base + q], rn = s.r_n[base + q];
                 b = box_empty();
                 for (uint32_t j = 0; j < rn; j++)
-                    b = box_union(b, load_box(bounds + 6 * (size_t)sorted_index[rs + j]));
+                    b = box_union(b, load_box(s.slot_bounds + 6 * (size_t)(rs + j)));
                 size = 1;
             } else {
                 uint32_t c0 = s.r_c0[base + q];
@@ -528,6 +543,9 @@ constexpr int NB = 12;
 constexpr uint32_t LANE_TASK_MAX = 32;
 constexpr int LANE_STACK = 6;               // smaller child first: depth <= log2(32) + 1
 constexpr uint32_t BLOCK_TASK_MIN = 1024;   // ranges this large use the whole workgroup
+constexpr uint32_t SAH_CUT = 256;           // sah_top defers ranges below this size
+constexpr uint32_t LANE_TASK_MAX_SUB = 8;   // lane tasks of sah_sub
+constexpr uint32_t SAH_SUB_TASKS = 2 * SAH_CUT;  // a deferred range makes fewer tasks than this
 static_assert(LANE_STACK * SAH_BLOCK <= 4100, "lane stacks share the prefix-sum array");
 
 struct SahLds {
@@ -543,7 +561,7 @@ struct SahLds {
     float fin[12];
     uint32_t wcnt[SAH_WAVES];
     uint32_t split[2];
-    uint32_t k, task_next, upper, nbig, nbig_next, total;
+    uint32_t k, task_next, nbig, nbig_next;
 };
 
 WPBR_D float wave_min_f(float v) {
@@ -580,9 +598,17 @@ WPBR_D uint32_t sah_bucket(float c, float cmin, float dist) {
 }
 
 // Records upper node [i0, i1] split at mid; returns its index.
-WPBR_D uint32_t sah_node(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t mid, int dim,
+// A task list (ranges i0..i1 of final positions with their depth and right
+// turns) and the counter that numbers upper nodes: global memory for the top
+// kernel, LDS for a workgroup of the sub-range kernel.
+struct SahTasks {
+    uint32_t* i0; uint32_t* i1; uint32_t* dep; uint32_t* rt;
+    uint32_t* uctr;
+};
+
+WPBR_D uint32_t sah_node(LbvhScratch& s, const SahTasks& T, uint32_t i0, uint32_t i1, uint32_t mid, int dim,
                          const bbox& bounds, uint32_t dep, uint32_t rt) {
-    uint32_t node = atomicAdd(&L.upper, 1u);
+    uint32_t node = atomicAdd(T.uctr, 1u);
     s.u_i0[node] = i0;
     s.u_i1[node] = i1;
     s.u_mid[node] = mid;
@@ -594,17 +620,17 @@ WPBR_D uint32_t sah_node(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, ui
 }
 
 // Queues the two children of a split made by a wave or the workgroup.
-WPBR_D void sah_children(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t mid, uint32_t dep,
+WPBR_D void sah_children(const SahTasks& T, SahLds& L, uint32_t i0, uint32_t i1, uint32_t mid, uint32_t dep,
                          uint32_t rt) {
     uint32_t ct = atomicAdd(&L.task_next, 2u);
-    s.task_i0[ct] = i0;
-    s.task_i1[ct] = mid - 1;
-    s.task_dep[ct] = dep + 1;
-    s.task_rt[ct] = rt;
-    s.task_i0[ct + 1] = mid;
-    s.task_i1[ct + 1] = i1;
-    s.task_dep[ct + 1] = dep + 1;
-    s.task_rt[ct + 1] = rt + 1;
+    T.i0[ct] = i0;
+    T.i1[ct] = mid - 1;
+    T.dep[ct] = dep + 1;
+    T.rt[ct] = rt;
+    T.i0[ct + 1] = mid;
+    T.i1[ct + 1] = i1;
+    T.dep[ct + 1] = dep + 1;
+    T.rt[ct + 1] = rt + 1;
     uint32_t nb = (mid - i0 >= BLOCK_TASK_MIN) + (i1 + 1 - mid >= BLOCK_TASK_MIN);
     if (nb) atomicAdd(&L.nbig_next, nb);
 }
@@ -727,7 +753,8 @@ WPBR_D uint32_t lane_split(SahLds& L, uint32_t i0, uint32_t i1, int* dimOut, bbo
 // One SAH split of perm[i0..i1] by the whole workgroup (every thread calls it
 // with the same arguments): the same bucketing, costs and two-pointer
 // partition, with block-wide reductions and ranks.
-WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t dep, uint32_t rt) {
+WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint32_t i0, uint32_t i1, uint32_t dep,
+                            uint32_t rt) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     float v[12];
 #pragma unroll
@@ -855,14 +882,15 @@ WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1,
         mid = i0 + T;
     }
     if (tid == 0) {
-        sah_node(s, L, i0, i1, mid, dim, bounds, dep, rt);
-        sah_children(s, L, i0, i1, mid, dep, rt);
+        sah_node(s, TL, i0, i1, mid, dim, bounds, dep, rt);
+        sah_children(TL, L, i0, i1, mid, dep, rt);
     }
     __syncthreads();
 }
 
 // One SAH split of perm[i0..i1] by one wave.
-WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, uint32_t dep, uint32_t rt) {
+WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint32_t i0, uint32_t i1, uint32_t dep,
+                           uint32_t rt) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
     float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
@@ -964,54 +992,60 @@ WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, uint32_t i0, uint32_t i1, 
         mid = i0 + T;
     }
     if (lane == 0) {
-        sah_node(s, L, i0, i1, mid, dim, bounds, dep, rt);
-        sah_children(s, L, i0, i1, mid, dep, rt);
+        sah_node(s, TL, i0, i1, mid, dim, bounds, dep, rt);
+        sah_children(TL, L, i0, i1, mid, dep, rt);
     }
 }
 
-WPBR_D void upper_sah_body(LbvhScratch& s) {
-    __shared__ SahLds L;
-    const int tid = threadIdx.x, lane = tid & 63;
-    if (tid == 0) { L.k = 0; L.upper = 0; L.total = 0; L.task_next = 1; L.nbig = 0; L.nbig_next = 0; }
-    __syncthreads();
-    if (tid < 128) atomicAdd(&L.k, (uint32_t)__popc(s.presence[tid]));
-    __syncthreads();
-    const uint32_t k = L.k;
-    if (tid == 0) {
-        s.task_i0[0] = 0;
-        s.task_i1[0] = k - 1;
-        s.task_dep[0] = 0;
-        s.task_rt[0] = 0;
-        L.nbig = k >= BLOCK_TASK_MIN;
-    }
-    for (uint32_t i = tid; i < k; i += SAH_BLOCK) L.perm[i] = (uint16_t)i;
-    for (uint32_t i = tid; i < 6 * k; i += SAH_BLOCK) L.tbox[i] = s.t_box[i];
-    __threadfence_block();
-    __syncthreads();
-    uint32_t lb = 0, le = 1;
+// The SAH top runs as three kernels so that independent sub-ranges use many
+// CUs: sah_top (one workgroup) splits the ranges of at least SAH_CUT treelets
+// and defers the smaller ones to a list; sah_sub (many workgroups) finishes the
+// deferred ranges, each in its own workgroup; sah_final lays the nodes out in
+// preorder.  Every range is split by the same steps in any of them, and ranges
+// are disjoint, so the result does not depend on the order.
+// Generations of tasks lb .. le (in s.task_*): very large ranges by the whole
+// workgroup, then ranges above lane_max by one wave each, then ranges of at
+// most lane_max by one lane each, to completion; ranges below `cut` go to the
+// deferred list instead.
+WPBR_D void sah_run(LbvhScratch& s, SahLds& L, const SahTasks& T, uint32_t lb, uint32_t le, uint32_t lane_max,
+                    uint32_t cut) {
+    const int tid = threadIdx.x;
     while (lb < le) {
         // ---- very large ranges: the whole workgroup, one range at a time
         if (L.nbig) {
             for (uint32_t t = lb; t < le; t++) {
-                const uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
+                const uint32_t i0 = T.i0[t], i1 = T.i1[t];
                 if (i1 - i0 + 1 < BLOCK_TASK_MIN) continue;
-                sah_block_split(s, L, i0, i1, s.task_dep[t], s.task_rt[t]);
+                sah_block_split(s, L, T, i0, i1, T.dep[t], T.rt[t]);
             }
         }
+        // ---- deferred ranges
+        if (cut)
+            for (uint32_t t = lb + tid; t < le; t += SAH_BLOCK) {
+                const uint32_t i0 = T.i0[t], i1 = T.i1[t];
+                if (i1 - i0 + 1 >= cut) continue;
+                const uint32_t e = atomicAdd(s.sub_n, 1u);
+                s.sub_i0[e] = i0;
+                s.sub_i1[e] = i1;
+                s.sub_dep[e] = T.dep[t];
+                s.sub_rt[e] = T.rt[t];
+                // a range of m treelets makes m - 1 upper nodes: number them now
+                s.sub_ub[e] = atomicAdd(T.uctr, i1 - i0);
+            }
         // ---- large ranges: one wave each
         for (uint32_t t = lb + (tid >> 6); t < le; t += SAH_WAVES) {
-            const uint32_t i0 = s.task_i0[t], i1 = s.task_i1[t];
+            const uint32_t i0 = T.i0[t], i1 = T.i1[t];
             const uint32_t m = i1 - i0 + 1;
-            if (m <= LANE_TASK_MAX || m >= BLOCK_TASK_MIN) continue;
-            sah_wave_split(s, L, i0, i1, s.task_dep[t], s.task_rt[t]);
+            if (m <= lane_max || m >= BLOCK_TASK_MIN || m < cut) continue;
+            sah_wave_split(s, L, T, i0, i1, T.dep[t], T.rt[t]);
         }
         // ---- small ranges: one lane each, to completion; the larger child of a
         // split waits on the lane's LDS stack (a, b, depth and right turns
         // relative to the task)
         for (uint32_t t = lb + tid; t < le; t += SAH_BLOCK) {
-            const uint32_t t0 = s.task_i0[t], t1 = s.task_i1[t];
-            if (t1 - t0 + 1 > LANE_TASK_MAX) continue;
-            const uint32_t d0 = s.task_dep[t], r0 = s.task_rt[t];
+            const uint32_t t0 = T.i0[t], t1 = T.i1[t];
+            if (t1 - t0 + 1 > lane_max || t1 - t0 + 1 < cut) continue;
+            const uint32_t d0 = T.dep[t], r0 = T.rt[t];
             uint32_t a = t0, b = t1, dd = 0, dr = 0;
             int sp = 0;
             while (true) {
@@ -1029,7 +1063,7 @@ WPBR_D void upper_sah_body(LbvhScratch& s) {
                 int dim;
                 bbox bounds;
                 uint32_t mid = lane_split(L, a, b, &dim, &bounds);
-                sah_node(s, L, a, b, mid, dim, bounds, d0 + dd, r0 + dr);
+                sah_node(s, T, a, b, mid, dim, bounds, d0 + dd, r0 + dr);
                 uint32_t nl = mid - a, nr = b + 1 - mid;
                 uint32_t pa, pb, pr;  // the child that waits (depth dd + 1)
                 if (nl <= nr) {
@@ -1053,10 +1087,114 @@ WPBR_D void upper_sah_body(LbvhScratch& s) {
         le = L.task_next;
         __syncthreads();
     }
-    // P: exclusive prefix of treelet node counts over final positions
-    // (PER consecutive positions per thread), in aux[0..k]
+}
+
+// Number of treelets (occupied bins), into L.k.
+WPBR_D uint32_t sah_treelets(const LbvhScratch& s, SahLds& L) {
+    const int tid = threadIdx.x;
+    if (tid == 0) L.k = 0;
+    __syncthreads();
+    if (tid < 128) atomicAdd(&L.k, (uint32_t)__popc(s.presence[tid]));
+    __syncthreads();
+    return L.k;
+}
+
+WPBR_D void sah_top_body(LbvhScratch& s) {
+    __shared__ SahLds L;
+    const int tid = threadIdx.x;
+    const uint32_t k = sah_treelets(s, L);
+    if (tid == 0) {
+        L.task_next = 1;
+        L.nbig_next = 0;
+        s.task_i0[0] = 0;
+        s.task_i1[0] = k - 1;
+        s.task_dep[0] = 0;
+        s.task_rt[0] = 0;
+        L.nbig = k >= BLOCK_TASK_MIN;
+    }
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK) L.perm[i] = (uint16_t)i;
+    for (uint32_t i = tid; i < 6 * k; i += SAH_BLOCK) L.tbox[i] = s.t_box[i];
+    __threadfence_block();
+    __syncthreads();
+    const SahTasks T{s.task_i0, s.task_i1, s.task_dep, s.task_rt, s.upper_count};
+    sah_run(s, L, T, 0, 1, LANE_TASK_MAX, SAH_CUT);
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK) s.g_perm[i] = L.perm[i];
+}
+
+// Deferred range st of a scene, by the whole workgroup; its task list and node
+// counter live in LDS.
+struct SahSubLds {
+    SahLds L;
+    uint32_t r[4];
+    uint32_t ti0[SAH_SUB_TASKS], ti1[SAH_SUB_TASKS], tdep[SAH_SUB_TASKS], trt[SAH_SUB_TASKS];
+    uint32_t uctr;
+};
+WPBR_D void sah_sub_one(LbvhScratch& s, SahSubLds& S, uint32_t st) {
+    SahLds& L = S.L;
+    uint32_t* s_r = S.r;
+    uint32_t* t_i0 = S.ti0;
+    uint32_t* t_i1 = S.ti1;
+    uint32_t* t_dep = S.tdep;
+    uint32_t* t_rt = S.trt;
+    uint32_t& s_uctr = S.uctr;
+    const int tid = threadIdx.x;
+    const SahTasks T{t_i0, t_i1, t_dep, t_rt, &s_uctr};
+    const uint32_t base = 0;
+    {
+        if (tid == 0) {
+            s_r[0] = s.sub_i0[st];
+            s_r[1] = s.sub_i1[st];
+            t_i0[base] = s_r[0];
+            t_i1[base] = s_r[1];
+            t_dep[base] = s.sub_dep[st];
+            t_rt[base] = s.sub_rt[st];
+            s_uctr = s.sub_ub[st];
+            L.task_next = base + 1;
+            L.nbig = 0;
+            L.nbig_next = 0;
+        }
+        __syncthreads();
+        const uint32_t i0 = s_r[0], i1 = s_r[1];
+        for (uint32_t i = i0 + tid; i <= i1; i += SAH_BLOCK) {
+            const uint32_t t = s.g_perm[i];
+            L.perm[i] = (uint16_t)t;
+            for (int c = 0; c < 6; c++) L.tbox[6 * t + c] = s.t_box[6 * (size_t)t + c];
+        }
+        __threadfence_block();
+        __syncthreads();
+        sah_run(s, L, T, base, base + 1, LANE_TASK_MAX_SUB, 0);
+        for (uint32_t i = i0 + tid; i <= i1; i += SAH_BLOCK) {
+            s.g_perm[i] = L.perm[i];
+            s.g_la[i] = L.la[i];
+            s.g_lb[i] = L.lb[i];
+        }
+        __syncthreads();
+    }
+}
+
+// The deferred ranges of one scene, workgroup bid of nblk taking every nblk-th.
+WPBR_D void sah_sub_body(LbvhScratch& s, uint32_t bid, uint32_t nblk) {
+    __shared__ SahSubLds S;
+    const uint32_t nsub = *s.sub_n;
+    for (uint32_t st = bid; st < nsub; st += nblk) sah_sub_one(s, S, st);
+}
+
+// Preorder layout: P = exclusive prefix of treelet node counts over final
+// positions (PER consecutive positions per thread), in aux[0..k]; then every
+// treelet's first node and every upper node at their closed-form offsets.
+WPBR_D void sah_final_body(LbvhScratch& s) {
+    __shared__ SahLds L;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t k = sah_treelets(s, L);
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK) {
+        L.perm[i] = (uint16_t)s.g_perm[i];
+        L.la[i] = (uint16_t)s.g_la[i];
+        L.lb[i] = (uint16_t)s.g_lb[i];
+    }
+    __threadfence_block();
+    __syncthreads();
     constexpr int PER = 4096 / SAH_BLOCK;
-    const uint32_t num_upper = L.upper;
+    const uint32_t num_upper = *s.upper_count;
     uint32_t v[PER], sum = 0;
 #pragma unroll
     for (int q = 0; q < PER; q++) {
@@ -1099,12 +1237,14 @@ WPBR_D void upper_sah_body(LbvhScratch& s) {
         store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
     }
     if (tid == 0) {
-        *s.upper_count = num_upper;
         *s.n_nodes = num_upper + L.aux[k];
         *s.n_treelets = k;
     }
 }
-__global__ __launch_bounds__(SAH_BLOCK) void k_upper_sah(LbvhScratch s) { upper_sah_body(s); }
+
+__global__ __launch_bounds__(SAH_BLOCK) void k_sah_top(LbvhScratch s) { sah_top_body(s); }
+__global__ __launch_bounds__(SAH_BLOCK) void k_sah_sub(LbvhScratch s) { sah_sub_body(s, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(SAH_BLOCK) void k_sah_final(LbvhScratch s) { sah_final_body(s); }
 
 // ---------------------------------------------------------------------------
 // Flatten: treelet records to final preorder slots.  One workgroup per treelet.
@@ -1174,7 +1314,7 @@ WPBR_D uint32_t sub_record(uint32_t root, uint32_t base, uint32_t L, uint32_t j)
 
 WPBR_D bbox slots_box(const LbvhScratch& s, const uint32_t* sorted_index, uint32_t a, uint32_t b) {
     bbox r = box_empty();
-    for (uint32_t q = a; q < b; q++) r = box_union(r, load_box(s.bounds + 6 * (size_t)sorted_index[q]));
+    for (uint32_t q = a; q < b; q++) r = box_union(r, load_box(s.slot_bounds + 6 * (size_t)q));
     return r;
 }
 
@@ -1501,6 +1641,11 @@ __global__ void ks_treelet_mark(LbvhBatch B) {
     treelet_mark_body(s.keys_a, B.n[sc], s.bin_start, s.presence, bid);
 }
 
+__global__ void ks_slot_bounds(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_MARK]);
+    slot_bounds_body(B.s[sc], B.n[sc], bid);
+}
+
 // work[i] = number of treelets of scenes 0 .. i-1 (one workgroup).
 __global__ void ks_treelet_count(LbvhBatch B) {
     __shared__ uint32_t s_sum[1024];
@@ -1566,9 +1711,48 @@ __global__ __launch_bounds__(EMIT_BLOCK) void ks_emit(LbvhBatch B) {
     }
 }
 
-__global__ __launch_bounds__(SAH_BLOCK) void ks_upper_sah(LbvhBatch B) {
+__global__ __launch_bounds__(SAH_BLOCK) void ks_sah_top(LbvhBatch B) {
     LbvhScratch s = B.s[blockIdx.x];
-    upper_sah_body(s);
+    sah_top_body(s);
+}
+// subwork[i] = deferred SAH ranges of scenes 0 .. i-1 (one workgroup).
+__global__ void ks_sah_sub_count(LbvhBatch B) {
+    __shared__ uint32_t s_sum[1024];
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < B.n_scenes; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        s_sum[threadIdx.x] = i < B.n_scenes ? *B.s[i].sub_n : 0u;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+            uint32_t v = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_sum[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < B.n_scenes) B.subwork[i + 1] = run + s_sum[threadIdx.x];
+        if (base == 0 && threadIdx.x == 0) B.subwork[0] = 0;
+        run += s_sum[blockDim.x - 1];
+        __syncthreads();
+    }
+}
+
+// Every scene's deferred ranges from one work list over a fixed grid (few
+// workgroups: each holds a CU's LDS, and launching them is not free).
+__global__ __launch_bounds__(SAH_BLOCK) void ks_sah_sub(LbvhBatch B) {
+    __shared__ SahSubLds S;
+    __shared__ uint32_t s_sc;
+    const uint32_t total = B.subwork[B.n_scenes];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        if (threadIdx.x == 0) s_sc = work_scene(B.subwork, B.n_scenes, g);
+        __syncthreads();
+        const uint32_t sc = s_sc;
+        LbvhScratch s = B.s[sc];
+        sah_sub_one(s, S, g - B.subwork[sc]);
+    }
+}
+__global__ __launch_bounds__(SAH_BLOCK) void ks_sah_final(LbvhBatch B) {
+    LbvhScratch s = B.s[blockIdx.x];
+    sah_final_body(s);
 }
 
 __global__ __launch_bounds__(256) void ks_scatter(LbvhBatch B) {
@@ -1637,9 +1821,13 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipS
     for (int p = 0; p < 4; p++)
         hipLaunchKernelGGL(ks_onesweep, dim3(grid[SEG_SORT]), dim3(SORT_BLOCK), 0, st, B, p);
     hipLaunchKernelGGL(ks_treelet_mark, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_slot_bounds, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(ks_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, B);
-    hipLaunchKernelGGL(ks_upper_sah, dim3(S), dim3(SAH_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
+    hipLaunchKernelGGL(ks_sah_sub, dim3(SAH_SEG_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_sah_final, dim3(S), dim3(SAH_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_subtree, dim3(grid[SEG_SUBTREE]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_subtree_big, dim3(grid[SEG_SUBTREE_BIG]), dim3(256), 0, st, B);
@@ -1689,6 +1877,7 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
             }
             hipLaunchKernelGGL(k_treelet_mark, dim3((n + 255) / 256), dim3(256), 0, st, s.keys_a, n, s.bin_start,
                                s.presence);
+            hipLaunchKernelGGL(k_slot_bounds, dim3((n + 255) / 256), dim3(256), 0, st, s, n);
             rec(7);
             break;
         }
@@ -1697,7 +1886,9 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
             rec(8);
             break;
         case 5: {
-            hipLaunchKernelGGL(k_upper_sah, dim3(1), dim3(SAH_BLOCK), 0, st, s);
+            hipLaunchKernelGGL(k_sah_top, dim3(1), dim3(SAH_BLOCK), 0, st, s);
+            hipLaunchKernelGGL(k_sah_sub, dim3(SAH_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
+            hipLaunchKernelGGL(k_sah_final, dim3(1), dim3(SAH_BLOCK), 0, st, s);
             rec(9);
             hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
             rec(10);

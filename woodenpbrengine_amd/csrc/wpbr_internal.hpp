@@ -30,6 +30,7 @@ struct GeomView {
 struct LbvhScratch {
     // a1-a3
     float* bounds;        // n x 6 (shape order)
+    float* slot_bounds;   // n x 6 (slot order: bounds of shape vals_a[p] at p)
     float* centroid;      // n x 3
     uint32_t* cb_ord;     // 6: centroid bounds as order-preserving uints
     uint32_t* morton;     // n (shape order)
@@ -55,6 +56,11 @@ struct LbvhScratch {
     uint32_t* u_axis; float* u_box;
     uint32_t* upper_count;
     uint32_t* n_nodes; uint32_t* n_treelets;
+    // SAH top in three kernels: final positions' treelets, leaf depth and right
+    // turns, and the deferred ranges (i0, i1, depth, right turns)
+    uint32_t* g_perm; uint32_t* g_la; uint32_t* g_lb;
+    uint32_t* sub_i0; uint32_t* sub_i1; uint32_t* sub_dep; uint32_t* sub_rt; uint32_t* sub_n;
+    uint32_t* sub_ub;     // first upper-node index of each deferred range
     // final CSceneTreeLBVH arrays (capacity 2n)
     uint32_t* node_offset; uint32_t* node_nshapes; uint8_t* node_axis; float* node_bounds;
     // traversal layout
@@ -101,6 +107,7 @@ struct LbvhBatch {
     const uint32_t* n;
     uint32_t n_scenes;
     uint32_t* work;         // n_scenes + 1: treelets of the scenes before each scene
+    uint32_t* subwork;      // n_scenes + 1: deferred SAH ranges of the scenes before each scene
     uint32_t* info;         // 5 per scene: n_nodes, n_treelets, err, nshapes[0], offset[0]
     SegMap map[SEG_KINDS];
 };
@@ -108,6 +115,11 @@ void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts);
 wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st);
 
 constexpr int LBVH_STAGES = 6;
+// SAH top: task list of the one-workgroup top kernel; workgroups of the
+// sub-range kernel
+constexpr uint32_t SAH_TASKS = 8192;
+constexpr uint32_t SAH_SUB_BLOCKS = 32;       // single scene
+constexpr uint32_t SAH_SEG_SUB_BLOCKS = 256;  // segmented build, over all scenes' ranges
 wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
                               hipEvent_t* ev, int nev);
 
