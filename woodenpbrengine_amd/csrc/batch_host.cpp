@@ -163,13 +163,15 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
 // secondary trace, radiance + splat.
 wpbr_status launch_passes(wpbr_ctx* ctx, wpbr_batch* b, hipStream_t st) {
     const int block = ctx->trace_block, sd = ctx->trace_stack, width = ctx->trace_width;
+    bool tables = true;
+    for (const wpbr_scene* sc : b->scenes) tables = tables && splat_tables_fit_h(sc->sv);
     wpbr_status s = render_batch_launch(b->d_L, b->map_pix, b->grid_pix, 3, 0, st);
     for (uint32_t j = 0; j < b->spp && !s; j++) {
         s = render_batch_launch(b->d_L, b->map_pix, b->grid_pix, 0, j, st);
         if (!s) s = trace_launch_seg(b->d_Tp, b->map_tp, b->grid_tp, block, sd, width, st);
         if (!s) s = render_batch_launch(b->d_L, b->map_pix, b->grid_pix, 1, j, st);
         if (!s) s = trace_launch_seg(b->d_Tq, b->map_tq, b->grid_tq, block, sd, width, st);
-        if (!s) s = render_batch_launch(b->d_L, b->map_tile, b->grid_tile, 2, j, st);
+        if (!s) s = render_batch_launch(b->d_L, b->map_tile, b->grid_tile, tables ? 4 : 2, j, st);
     }
     return s;
 }

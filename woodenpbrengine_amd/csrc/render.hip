@@ -649,9 +649,31 @@ WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& 
 // within two pixels of it; each lane gathers those from LDS in a fixed order
 // (deterministic sums, no atomics) and adds them to the film once.
 constexpr int TILE_THREADS = 1024;
-WPBR_D void splat_body(const RenderLaunch& L, uint32_t tile) {
+// Spectral tables the radiance of a sample reads 64 times (per wavelength):
+// with TABLES, staged in LDS once per tile (the launch checks that they fit).
+constexpr uint32_t SPLAT_MAX_LIGHTS = 16, SPLAT_MAX_SPEC = 8192;
+template <bool TABLES>
+WPBR_D void splat_body(const RenderLaunch& Lg, uint32_t tile) {
     __shared__ float s_fx[1024], s_fy[1024], s_x[1024], s_y[1024], s_z[1024];
     __shared__ uint32_t s_black[1024];
+    __shared__ float t_illum[7 * NSPEC], t_xyz[3 * NSPEC], t_env[NSPEC], t_light[SPLAT_MAX_LIGHTS * NSPEC];
+    __shared__ float t_spec[SPLAT_MAX_SPEC];
+    RenderLaunch L = Lg;
+    if (TABLES) {
+        const SceneView& g = Lg.sv;
+        for (uint32_t q = threadIdx.x; q < 7 * NSPEC; q += TILE_THREADS) t_illum[q] = g.illum[q];
+        for (uint32_t q = threadIdx.x; q < 3 * NSPEC; q += TILE_THREADS) t_xyz[q] = g.xyz[q];
+        if (g.has_env)
+            for (uint32_t q = threadIdx.x; q < NSPEC; q += TILE_THREADS) t_env[q] = g.env_emit[q];
+        for (uint32_t q = threadIdx.x; q < g.n_sphere_lights * NSPEC; q += TILE_THREADS) t_light[q] = g.light_emit[q];
+        for (uint32_t q = threadIdx.x; q < g.n_spec_floats; q += TILE_THREADS) t_spec[q] = g.mat_spec[q];
+        L.sv.illum = t_illum;
+        L.sv.xyz = t_xyz;
+        L.sv.env_emit = t_env;
+        L.sv.light_emit = t_light;
+        L.sv.mat_spec = t_spec;
+        __syncthreads();
+    }
     const SceneView& sv = L.sv;
     const uint32_t W = sv.width;
     const uint32_t ntx = W / 32;
@@ -705,7 +727,13 @@ WPBR_D void splat_body(const RenderLaunch& L, uint32_t tile) {
     v.x += ax; v.y += ay; v.z += az; v.w += aw;
     *f = v;
 }
-__global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L) { splat_body(L, blockIdx.x); }
+template <bool TABLES>
+__global__ __launch_bounds__(TILE_THREADS) void k_radiance_splat(RenderLaunch L) { splat_body<TABLES>(L, blockIdx.x); }
+
+// The spectral tables of the scene fit the splat's LDS copy.
+bool splat_tables_fit_h(const SceneView& sv) {
+    return sv.n_sphere_lights <= SPLAT_MAX_LIGHTS && sv.n_spec_floats <= SPLAT_MAX_SPEC;
+}
 
 // ---- batched render: the stages of one sample pass for many scenes per launch.
 // Block b works for scene map.scene[b] as its block b - map.first[scene]; the
@@ -727,12 +755,13 @@ __global__ __launch_bounds__(256) void ks_shade1(const RenderLaunch* __restrict_
     L.spp_index = j;
     shade1_body(L, bid * 256 + threadIdx.x);
 }
+template <bool TABLES>
 __global__ __launch_bounds__(TILE_THREADS) void ks_radiance_splat(const RenderLaunch* __restrict__ T, SegMap map,
                                                                   uint32_t j) {
     SEG_BLOCK(map);
     RenderLaunch L = T[sc];
     L.spp_index = j;
-    splat_body(L, bid);
+    splat_body<TABLES>(L, bid);
 }
 // Clears the films and the queue statistics of every scene (a kernel, so that
 // the pass loop can be captured in a graph).
@@ -749,7 +778,8 @@ wpbr_status render_batch_launch(const RenderLaunch* T, const SegMap& map, uint32
     switch (stage) {
         case 0: hipLaunchKernelGGL(ks_camera, dim3(grid), dim3(256), 0, st, T, map, j); break;
         case 1: hipLaunchKernelGGL(ks_shade1, dim3(grid), dim3(256), 0, st, T, map, j); break;
-        case 2: hipLaunchKernelGGL(ks_radiance_splat, dim3(grid), dim3(TILE_THREADS), 0, st, T, map, j); break;
+        case 2: hipLaunchKernelGGL(ks_radiance_splat<false>, dim3(grid), dim3(TILE_THREADS), 0, st, T, map, j); break;
+        case 4: hipLaunchKernelGGL(ks_radiance_splat<true>, dim3(grid), dim3(TILE_THREADS), 0, st, T, map, j); break;
         case 3: hipLaunchKernelGGL(ks_clear, dim3(grid), dim3(256), 0, st, T, map); break;
     }
     return launch_status();
@@ -761,8 +791,12 @@ wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st) {
         case 0: hipLaunchKernelGGL(k_camera, grid, block, 0, st, L); break;
         case 1: hipLaunchKernelGGL(k_shade1, grid, block, 0, st, L); break;
         case 2:
-            hipLaunchKernelGGL(k_radiance_splat, dim3((L.sv.width / 32) * (L.sv.height / 32)),
-                               dim3(TILE_THREADS), 0, st, L);
+            if (splat_tables_fit_h(L.sv))
+                hipLaunchKernelGGL(k_radiance_splat<true>, dim3((L.sv.width / 32) * (L.sv.height / 32)),
+                                   dim3(TILE_THREADS), 0, st, L);
+            else
+                hipLaunchKernelGGL(k_radiance_splat<false>, dim3((L.sv.width / 32) * (L.sv.height / 32)),
+                                   dim3(TILE_THREADS), 0, st, L);
             break;
     }
     return launch_status();

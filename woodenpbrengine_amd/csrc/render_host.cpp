@@ -271,6 +271,7 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
     }
     v.materials = (const DevMaterial*)up(mats.data(), sizeof(DevMaterial) * mats.size());
     v.mat_spec = (const float*)up(specs.data(), 4 * specs.size());
+    v.n_spec_floats = (uint32_t)specs.size();
     sc->d_materials = const_cast<DevMaterial*>(v.materials);
     sc->d_mat_spec = const_cast<float*>(v.mat_spec);
     sc->h_materials.assign(d->materials, d->materials + 9 * (size_t)d->n_materials);

@@ -10,7 +10,9 @@ namespace wpbr {
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
 // Batched stages over a device table of launch records: 0 camera, 1 shade1,
-// 2 radiance + splat, 3 clear films and queue statistics.
+// 2 radiance + splat, 3 clear films and queue statistics, 4 radiance + splat
+// with the spectral tables in LDS (every scene's tables fit: splat_tables_fit_h).
+bool splat_tables_fit_h(const SceneView& sv);
 wpbr_status render_batch_launch(const RenderLaunch* T, const SegMap& map, uint32_t grid, int stage, uint32_t j,
                                 hipStream_t st);
 wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** out);

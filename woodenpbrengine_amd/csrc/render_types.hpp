@@ -25,6 +25,7 @@ struct SceneView {
     const uint32_t* shape_material;
     const DevMaterial* materials;
     const float* mat_spec;       // per conductor: eta[64], k[64]
+    uint32_t n_spec_floats;      // floats in mat_spec
     uint32_t n_sphere_lights;
     const uint32_t* light_shape;
     const float* light_emit;     // n_sphere_lights x 64
