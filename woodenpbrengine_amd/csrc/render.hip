@@ -337,6 +337,18 @@ WPBR_D float sphere_light_pdf(const SceneView& sv, uint32_t sid) {
 }
 
 // Wave-aggregated append to the secondary ray queue.
+// Queue slots for the wave's light rays and BSDF rays with one atomic: the
+// wave's light rays first, then its BSDF rays, each in lane order.
+WPBR_D void queue_append2(bool wantA, bool wantB, uint32_t* count, uint32_t& slotA, uint32_t& slotB) {
+    const uint64_t ma = ballot64(wantA), mb = ballot64(wantB);
+    const uint32_t na = popc64(ma), total = na + popc64(mb);
+    uint32_t base = 0;
+    if (total && lane_id() == 0) base = atomicAdd(count, total);
+    base = (uint32_t)__shfl((int)base, 0);
+    slotA = base + popc64(ma & lanemask_lt());
+    slotB = base + na + popc64(mb & lanemask_lt());
+}
+
 WPBR_D uint32_t queue_append(bool want, uint32_t* count) {
     uint64_t m = ballot64(want);
     uint32_t total = popc64(m);
@@ -527,8 +539,8 @@ WPBR_D void shade1_body(const RenderLaunch& L, uint32_t i) {
         if (wantB) flags |= F_BSDF;
         ws.flags[i] = flags;
     }
-    uint32_t sl = queue_append(wantL, ws.qcount);
-    uint32_t sb = queue_append(wantB, ws.qcount);
+    uint32_t sl, sb;
+    queue_append2(wantL, wantB, ws.qcount, sl, sb);
     // each queue ray asks whether its first hit is the sampled light (or nothing)
     if (wantL) { put_ray(ws.qrays, cap, sl, lo, ld); ws.l_ray[i] = sl; ws.qtarget[sl] = tgtL; }
     if (wantB) { put_ray(ws.qrays, cap, sb, bo, bd); ws.b_ray[i] = sb; ws.qtarget[sb] = tgtB; }
