@@ -42,7 +42,6 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     Carver c{base};
     const size_t N = std::max<uint32_t>(n, 1);
     const size_t tiles = (N + 3839) / 3840;
-    s.bounds = c.take<float>(6 * N);
     s.slot_bounds = c.take<float>(6 * N);
     s.centroid = c.take<float>(3 * N);
     s.cb_ord = c.take<uint32_t>(8 + 6 * 512);
@@ -102,8 +101,6 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.trav = c.take<float4>(4 * 3 * N);
     s.trav4 = c.take<float4>(8 * 3 * N);
     s.prims = c.take<float4>(3 * N);
-    s.slot_leaf = c.take<uint32_t>(N);
-    s.sub_count = c.take<uint32_t>(1);
     s.big_list = c.take<uint32_t>(N / 64 + 1);
     s.big_count = c.take<uint32_t>(1);
     return c.off + 256;

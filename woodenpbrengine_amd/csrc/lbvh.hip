@@ -12,6 +12,8 @@
 //                   A stable sort of (code, index) has a unique result, equal to
 //                   the reference's 5 x 6-bit passes (SLBVH.cpp:245-283)
 //   k_treelet_mark  a4: treelet = run of equal top-12 bits (SLBVH.cpp:299-319)
+//   k_slot_prims    a4: shape bounds gathered into slot order (SLBVH.cpp:231-324),
+//                   with the traversal layout's primitive records
 //   k_emit          a5: one workgroup per treelet builds the emitLBVH tree
 //                   (SLBVH.cpp:353-406) breadth-first, then subtree sizes and
 //                   bounds bottom-up and DFS-preorder offsets top-down
@@ -25,7 +27,7 @@
 //   k_subtree(_big) derived traversal layout: per interior node both child boxes
 //                   in one 64-byte record; leaves of more than 4 shapes get a
 //                   sub-tree over their (Morton-ordered) slots
-//   k_prims         primitive records at their slots
+//   k_bvh4          4-wide records (a binary record's grandchildren)
 #include "wpbr_device.hpp"
 #include "wpbr_internal.hpp"
 
@@ -49,7 +51,9 @@ WPBR_D void init_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
     for (uint32_t j = i; j < (uint32_t)s.status_words; j += stride) s.status0[j] = 0;
     for (uint32_t j = i; j < 128; j += stride) s.presence[j] = 0;
     if (i < 8) s.tile_counter[i] = 0;
-    if (i == 0) { *s.err = 0; *s.upper_count = 0; *s.sub_count = 0; *s.big_count = 0; *s.sub_n = 0; }
+    if (i == 0) {
+        *s.err = 0; *s.upper_count = 0; *s.big_count = 0; *s.sub_n = 0;
+    }
 }
 __global__ void k_init(LbvhScratch s) { init_body(s, blockIdx.x, gridDim.x); }
 
@@ -87,7 +91,7 @@ WPBR_D void bounds_body(const GeomView& g, uint32_t n, float* bounds, float* cen
     for (uint32_t i = bid * blockDim.x + threadIdx.x; i < n; i += nblk * blockDim.x) {
         bbox b;
         shape_box(g, i, b);
-        store_box(bounds + 6 * (size_t)i, b);
+        if (bounds) store_box(bounds + 6 * (size_t)i, b);
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             float c = (float)((double)b.mx[k] * 0.5 + (double)b.mn[k] * 0.5);
@@ -362,18 +366,75 @@ __global__ void k_treelet_mark(const uint32_t* __restrict__ sorted, uint32_t n, 
     treelet_mark_body(sorted, n, bin_start, presence, blockIdx.x);
 }
 
-// Shape bounds gathered into slot (sorted) order, as the reference's sort job
-// gathers them with the entities (SLBVH.cpp:231-324): the treelet and leaf
-// kernels then read contiguous boxes.
-WPBR_D void slot_bounds_body(const LbvhScratch& s, uint32_t n, uint32_t bid) {
+// Shape bounds in slot (sorted) order, as the reference's sort job gathers
+// them with the entities (SLBVH.cpp:231-324), so that the treelet and leaf
+// kernels read contiguous boxes; and the primitive records of the traversal
+// layout at their slots: v0.xyz + shape id, v1.xyz + slot, v2.xyz + first slot
+// of the reference leaf (written by the leaf kernels).  Spheres keep only the
+// words.  One gather of the shape serves both: the box is recomputed from it
+// (the same min/max sequence as k_bounds, so the same values).
+WPBR_D void slot_prims_body(const LbvhScratch& s, const GeomView& g, uint32_t n, uint32_t bid) {
     const uint32_t p = bid * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const float* src = s.bounds + 6 * (size_t)s.vals_a[p];
-    float* dst = s.slot_bounds + 6 * (size_t)p;
+    const uint32_t id = s.vals_a[p];
+    float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bbox b;
+    if (id >= g.n_spheres) {
+        const float* t = g.tri_verts + 9 * (size_t)(id - g.n_spheres);
 #pragma unroll
-    for (int c = 0; c < 6; c++) dst[c] = src[c];
+        for (int k = 0; k < 9; k++) v[k] = t[k];
+        b = box_empty();
+        b = box_union_pt(b, mk3(v[0], v[1], v[2]));
+        b = box_union_pt(b, mk3(v[3], v[4], v[5]));
+        b = box_union_pt(b, mk3(v[6], v[7], v[8]));
+    } else {
+        shape_box(g, id, b);
+    }
+    float2* sb = reinterpret_cast<float2*>(s.slot_bounds + 6 * (size_t)p);
+    sb[0] = make_float2(b.mn[0], b.mn[1]);
+    sb[1] = make_float2(b.mn[2], b.mx[0]);
+    sb[2] = make_float2(b.mx[1], b.mx[2]);
+    float4* rec = reinterpret_cast<float4*>(s.prims) + 3 * (size_t)p;
+    rec[0] = make_float4(v[0], v[1], v[2], __uint_as_float(id));
+    rec[1] = make_float4(v[3], v[4], v[5], __uint_as_float(p));
+    rec[2] = make_float4(v[6], v[7], v[8], 0.0f);
 }
-__global__ void k_slot_bounds(LbvhScratch s, uint32_t n) { slot_bounds_body(s, n, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_slot_prims(LbvhScratch s, GeomView g, uint32_t n) {
+    slot_prims_body(s, g, n, blockIdx.x);
+}
+
+// Box of a slot (slot bounds are 8-byte aligned: 24 bytes per slot).
+WPBR_D bbox slot_box(const float* sb, uint32_t q) {
+    const float2* p = reinterpret_cast<const float2*>(sb + 6 * (size_t)q);
+    const float2 x = p[0], y = p[1], z = p[2];
+    bbox b;
+    b.mn[0] = x.x; b.mn[1] = x.y; b.mn[2] = y.x;
+    b.mx[0] = y.y; b.mx[1] = z.x; b.mx[2] = z.y;
+    return b;
+}
+
+// Unions of the slot boxes of [a, m) and [m, e) (e > a), each in slot order;
+// eight loads in flight at a time.
+WPBR_D void slot_range_boxes(const float* sb, uint32_t a, uint32_t m, uint32_t e, bbox& b0, bbox& b1) {
+    b0 = box_empty();
+    b1 = box_empty();
+    for (uint32_t q = a; q < e; q += 8) {
+        bbox t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = slot_box(sb, min(q + u, e - 1));
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const uint32_t j = q + u;
+            if (j < m) b0 = box_union(b0, t[u]);
+            else if (j < e) b1 = box_union(b1, t[u]);
+        }
+    }
+}
+
+// The leaf word of the primitive record at `slot`: its reference leaf's first slot.
+WPBR_D void set_prim_leaf(const LbvhScratch& s, uint32_t slot, uint32_t leaf) {
+    reinterpret_cast<uint32_t*>(s.prims)[12 * (size_t)slot + 11] = leaf;
+}
 
 WPBR_D uint32_t presence_rank(const uint32_t* presence, uint32_t bin) {
     uint32_t r = 0;
@@ -385,12 +446,10 @@ WPBR_D uint32_t presence_rank(const uint32_t* presence, uint32_t bin) {
 
 // ---------------------------------------------------------------------------
 // Emit: one workgroup per occupied top-12-bit bin (= treelet).
-constexpr int EMIT_BLOCK = 256;
+constexpr int EMIT_BLOCK = 64;  // one wave: a treelet level rarely has more nodes
 constexpr int EMIT_MAX_LEVELS = 64;
 
-WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
-                     const uint32_t* __restrict__ sorted_index, const float* __restrict__ bounds, uint32_t n,
-                     uint32_t bin) {
+WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t bin) {
     __shared__ uint32_t s_q;
     __shared__ uint32_t s_lvl[EMIT_MAX_LEVELS + 2];
     __shared__ int s_nlvl;
@@ -477,9 +536,8 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
             uint32_t size;
             if (meta & 0x80000000u) {
                 uint32_t rs = s.r_start[base + q], rn = s.r_n[base + q];
-                b = box_empty();
-                for (uint32_t j = 0; j < rn; j++)
-                    b = box_union(b, load_box(s.slot_bounds + 6 * (size_t)(rs + j)));
+                bbox unused;
+                slot_range_boxes(s.slot_bounds, rs, rs + rn, rs + rn, b, unused);
                 size = 1;
             } else {
                 uint32_t c0 = s.r_c0[base + q];
@@ -514,10 +572,8 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
         store_box(s.t_box + 6 * (size_t)s_tidx, b);
     }
 }
-__global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32_t* __restrict__ codes,
-                                                     const uint32_t* __restrict__ sorted_index,
-                                                     const float* __restrict__ bounds, uint32_t n) {
-    emit_body(s, codes, sorted_index, bounds, n, blockIdx.x);
+__global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32_t* __restrict__ codes, uint32_t n) {
+    emit_body(s, codes, n, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -549,7 +605,7 @@ constexpr uint32_t SAH_SUB_TASKS = 2 * SAH_CUT;  // a deferred range makes fewer
 static_assert(LANE_STACK * SAH_BLOCK <= 4100, "lane stacks share the prefix-sum array");
 
 struct SahLds {
-    float tbox[6 * 4096];       // treelet root boxes
+    alignas(16) float tbox[6 * 4096];  // treelet root boxes
     uint16_t perm[4096];
     uint8_t bucket[4096];
     uint16_t la[4096], lb[4096];            // partition lists; then leaf depth / right turns
@@ -1113,7 +1169,13 @@ WPBR_D void sah_top_body(LbvhScratch& s) {
         L.nbig = k >= BLOCK_TASK_MIN;
     }
     for (uint32_t i = tid; i < k; i += SAH_BLOCK) L.perm[i] = (uint16_t)i;
-    for (uint32_t i = tid; i < 6 * k; i += SAH_BLOCK) L.tbox[i] = s.t_box[i];
+    {
+        // 6k floats as 3k float2 (t_box is 8-byte aligned), several loads in flight
+        const float2* src = reinterpret_cast<const float2*>(s.t_box);
+        float2* dst = reinterpret_cast<float2*>(L.tbox);
+#pragma unroll 8
+        for (uint32_t i = tid; i < 3 * k; i += SAH_BLOCK) dst[i] = src[i];
+    }
     __threadfence_block();
     __syncthreads();
     const SahTasks T{s.task_i0, s.task_i1, s.task_dep, s.task_rt, s.upper_count};
@@ -1186,6 +1248,7 @@ WPBR_D void sah_final_body(LbvhScratch& s) {
     __shared__ SahLds L;
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t k = sah_treelets(s, L);
+#pragma unroll 8
     for (uint32_t i = tid; i < k; i += SAH_BLOCK) {
         L.perm[i] = (uint16_t)s.g_perm[i];
         L.la[i] = (uint16_t)s.g_la[i];
@@ -1227,6 +1290,7 @@ WPBR_D void sah_final_body(LbvhScratch& s) {
     __syncthreads();
     for (uint32_t i = tid; i < k; i += SAH_BLOCK)
         s.t_final[L.perm[i]] = (uint32_t)L.la[i] + (i - (uint32_t)L.lb[i]) + L.aux[i];
+#pragma unroll 4
     for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) {
         const uint32_t i0 = s.u_i0[u], mid = s.u_mid[u];
         const uint32_t off = s.u_dep[u] + (i0 - s.u_rt[u]) + L.aux[i0];
@@ -1312,9 +1376,9 @@ WPBR_D uint32_t sub_record(uint32_t root, uint32_t base, uint32_t L, uint32_t j)
     return L == 0 ? root : base + (1u << L) - 2 + j;
 }
 
-WPBR_D bbox slots_box(const LbvhScratch& s, const uint32_t* sorted_index, uint32_t a, uint32_t b) {
-    bbox r = box_empty();
-    for (uint32_t q = a; q < b; q++) r = box_union(r, load_box(s.slot_bounds + 6 * (size_t)q));
+WPBR_D bbox slots_box(const LbvhScratch& s, uint32_t a, uint32_t b) {
+    bbox r, unused;
+    slot_range_boxes(s.slot_bounds, a, b, b, r, unused);
     return r;
 }
 
@@ -1340,101 +1404,366 @@ WPBR_D void write_pair(const LbvhScratch& s, uint32_t r, bbox b0, bbox b1, uint3
     else write_rec(s, r, b0, b1, w0, w1, axis);
 }
 
-constexpr int SUB_STACK = 8;  // smaller child first: depth <= log2(BIG_LEAF / 5) + 1
 
-// One thread per reference node: interior nodes get their record; leaves of at
-// most BIG_LEAF shapes get a sub-tree split like emitLBVH below its leaf size
-// (SLBVH.cpp:374-401): at the highest Morton bit that differs inside the range
-// (child 0 below child 1 on axis bit % 3), or in halves when all codes are
-// equal.  Larger leaves (all codes equal) are queued for k_subtree_big.
-WPBR_D void subtree_body(const LbvhScratch& s, const uint32_t* __restrict__ codes,
-                        const uint32_t* __restrict__ sorted_index, uint32_t bid) {
-    __shared__ uint16_t s_ab[SUB_STACK][256];
-    __shared__ uint32_t s_rec[SUB_STACK][256];
-    const uint32_t i = bid * blockDim.x + threadIdx.x;
+// Record numbers: 0 .. nn - 1 are the reference nodes; the sub-tree records of
+// a reference leaf at slots [off, off + n) are nn + off .. nn + off + n - 6
+// (a sub-tree whose records each hold more than 4 slots has at most n - 4
+// records, its root being the leaf's own record).  Leaves' slot ranges are
+// disjoint, so no counter is needed; numbers of slots of small leaves stay
+// unused.
+//
+// Leaves of at most 4 shapes are a primitive range of their parent's record;
+// leaves of 5 .. BIG_LEAF shapes get one wave each (k_subtree_leaf); larger
+// ones (only when all their codes are equal) a workgroup (k_subtree_big).
+constexpr uint32_t SUBTREE_BLOCKS = 1024;
+
+// One thread per reference node (grid-stride over the node count): interior
+// nodes get their record (both child boxes and words; a child leaf of at most
+// 4 shapes is a primitive range), small leaves write their first slot into
+// their primitive records, the largest leaves are queued.
+WPBR_D void subtree_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
     const uint32_t nn = *s.n_nodes;
-    const uint32_t n = (i < nn) ? s.node_nshapes[i] : 0;
-    const uint32_t off = (n > 0) ? s.node_offset[i] : 0;
-    // sub-tree records (root excluded): at most n - 5 interior nodes of > 4 shapes
-    uint32_t need = (n > RANGE_MAX && n <= BIG_LEAF) ? n - RANGE_MAX - 1 : 0;
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t incl = need;
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t v = (uint32_t)__shfl_up((int)incl, d);
-        if (lane >= (uint32_t)d) incl += v;
-    }
-    const uint32_t wtotal = (uint32_t)__shfl((int)incl, 63);
-    uint32_t wbase = 0;
-    if (lane == 63 && wtotal) wbase = atomicAdd(s.sub_count, wtotal);
-    wbase = (uint32_t)__shfl((int)wbase, 63);
-    if (i >= nn) return;
-    if (n == 0) {
-        uint32_t c[2] = {i + 1, s.node_offset[i]};
-        uint32_t w[2];
-        for (int k = 0; k < 2; k++) {
-            uint32_t ns = s.node_nshapes[c[k]];
-            w[k] = (ns > 0 && ns <= RANGE_MAX) ? range_word(s.node_offset[c[k]], ns) : c[k];
-        }
-        const float* b0 = s.node_bounds + 6 * (size_t)c[0];
-        const float* b1 = s.node_bounds + 6 * (size_t)c[1];
-        float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)i;
-        rec[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
-        rec[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
-        rec[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
-        rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
-                             __uint_as_float((uint32_t)s.node_axis[i]), 0.0f);
-        return;
-    }
-    if (n > BIG_LEAF) {
-        uint32_t e = atomicAdd(s.big_count, 1u);
-        s.big_list[e] = i;
-        return;
-    }
-    for (uint32_t k = 0; k < n; k++) s.slot_leaf[off + k] = off;
-    if (n <= RANGE_MAX) return;
-    uint32_t next = nn + wbase + incl - need;
-    uint32_t a = 0, b = n, r = i;
-    int sp = 0;
-    while (true) {
-        const uint32_t ca = codes[off + a], cl = codes[off + b - 1];
-        uint32_t m;
-        int axis = -1;
-        if (ca != cl) {
-            int bit = 31 - __clz(ca ^ cl);
-            m = a + 1;
-            while (!((codes[off + m] >> bit) & 1u)) m++;
-            axis = bit % 3;
-        } else {
-            m = a + (b - a) / 2;
-        }
-        const bool in0 = m - a > RANGE_MAX, in1 = b - m > RANGE_MAX;
-        const uint32_t w0 = in0 ? next++ : range_word(off + a, m - a);
-        const uint32_t w1 = in1 ? next++ : range_word(off + m, b - m);
-        bbox b0 = slots_box(s, sorted_index, off + a, off + m);
-        bbox b1 = slots_box(s, sorted_index, off + m, off + b);
-        if (axis >= 0) write_rec(s, r, b0, b1, w0, w1, (uint32_t)axis);
-        else write_pair(s, r, b0, b1, w0, w1);
-        if (in0 && in1) {
-            uint32_t pa, pb, pr;
-            if (m - a <= b - m) { pa = m; pb = b; pr = w1; b = m; r = w0; }
-            else { pa = a; pb = m; pr = w0; a = m; r = w1; }
-            if (sp < SUB_STACK) { s_ab[sp][threadIdx.x] = (uint16_t)(pa | (pb << 7)); s_rec[sp][threadIdx.x] = pr; sp++; }
-            else atomicOr(s.err, 4u);
-        } else if (in0) {
-            b = m; r = w0;
-        } else if (in1) {
-            a = m; r = w1;
-        } else {
-            if (sp == 0) break;
-            sp--;
-            uint32_t e = s_ab[sp][threadIdx.x];
-            a = e & 127u; b = e >> 7; r = s_rec[sp][threadIdx.x];
+    for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nn; i += nblk * blockDim.x) {
+        const uint32_t n = s.node_nshapes[i];
+        if (n == 0) {
+            uint32_t c[2] = {i + 1, s.node_offset[i]};
+            uint32_t w[2];
+            for (int k = 0; k < 2; k++) {
+                uint32_t ns = s.node_nshapes[c[k]];
+                w[k] = (ns > 0 && ns <= RANGE_MAX) ? range_word(s.node_offset[c[k]], ns) : c[k];
+            }
+            const float* b0 = s.node_bounds + 6 * (size_t)c[0];
+            const float* b1 = s.node_bounds + 6 * (size_t)c[1];
+            float4* rec = reinterpret_cast<float4*>(s.trav) + 4 * (size_t)i;
+            rec[0] = make_float4(b0[0], b0[1], b0[2], b0[3]);
+            rec[1] = make_float4(b0[4], b0[5], b1[0], b1[1]);
+            rec[2] = make_float4(b1[2], b1[3], b1[4], b1[5]);
+            rec[3] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]),
+                                 __uint_as_float((uint32_t)s.node_axis[i]), 0.0f);
+        } else if (n <= RANGE_MAX) {
+            const uint32_t off = s.node_offset[i];
+            for (uint32_t k = 0; k < n; k++) set_prim_leaf(s, off + k, off);
+        } else if (n > BIG_LEAF) {
+            s.big_list[atomicAdd(s.big_count, 1u)] = i;
         }
     }
 }
-__global__ __launch_bounds__(256) void k_subtree(LbvhScratch s, const uint32_t* __restrict__ codes,
-                                                 const uint32_t* __restrict__ sorted_index) {
-    subtree_body(s, codes, sorted_index, blockIdx.x);
+__global__ __launch_bounds__(256) void k_subtree(LbvhScratch s) { subtree_body(s, blockIdx.x, gridDim.x); }
+
+// Component-wise choice of a box (no aggregate copies in divergent code).
+WPBR_D bbox box_sel(bool c, const bbox& a, const bbox& b) {
+    bbox r;
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        r.mn[e] = c ? a.mn[e] : b.mn[e];
+        r.mx[e] = c ? a.mx[e] : b.mx[e];
+    }
+    return r;
+}
+
+// Contents of a binary record: child boxes, child words, split axis.
+struct BinRec {
+    bbox b0, b1;
+    uint32_t w0, w1, axis;
+};
+
+// Children of a split without an axis of its own, ordered along the axis on
+// which their centres differ most (as write_pair); `swapped` when child 1 of
+// the arguments comes first.
+WPBR_D BinRec pair_rec(const bbox& b0, const bbox& b1, uint32_t w0, uint32_t w1, bool& swapped) {
+    float d[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) d[c] = (b1.mn[c] + b1.mx[c]) - (b0.mn[c] + b0.mx[c]);
+    const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+    const uint32_t axis = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    const float da = axis == 0 ? d[0] : (axis == 1 ? d[1] : d[2]);
+    swapped = da < 0.0f;
+    BinRec q;
+    q.b0 = box_sel(swapped, b1, b0);
+    q.b1 = box_sel(swapped, b0, b1);
+    q.w0 = swapped ? w1 : w0;
+    q.w1 = swapped ? w0 : w1;
+    q.axis = axis;
+    return q;
+}
+
+WPBR_D BinRec load_rec(const LbvhScratch& s, uint32_t r) {
+    const float4* g = reinterpret_cast<const float4*>(s.trav) + 4 * (size_t)r;
+    const float4 g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
+    BinRec q;
+    q.b0.mn[0] = g0.x; q.b0.mn[1] = g0.y; q.b0.mn[2] = g0.z; q.b0.mx[0] = g0.w; q.b0.mx[1] = g1.x; q.b0.mx[2] = g1.y;
+    q.b1.mn[0] = g1.z; q.b1.mn[1] = g1.w; q.b1.mn[2] = g2.x; q.b1.mx[0] = g2.y; q.b1.mx[1] = g2.z; q.b1.mx[2] = g2.w;
+    q.w0 = __float_as_uint(g3.x);
+    q.w1 = __float_as_uint(g3.y);
+    q.axis = __float_as_uint(g3.z);
+    return q;
+}
+
+// 4-wide record r: the grandchildren of binary record q (a child that is a
+// primitive range stays one child; k0 / k1 are the binary records of q's
+// children that are records).  Slots 0,1 come from child 0 and 2,3 from child
+// 1; the last word packs the axes (bits 0-1 record r, 2-3 child 0, 4-5 child
+// 1).  Visiting the pair of the near child first and, inside each pair, its
+// near child first is the binary depth-first order.  Empty slots have the word
+// 0xFFFFFFFF.
+// One child's pair of slots: its two child boxes, words and axis when it is a
+// record, else the child itself and an empty slot.
+struct Rec4Pair {
+    bbox x, y;
+    uint32_t wx, wy, axis;
+};
+WPBR_D Rec4Pair rec4_pair(uint32_t cw, const bbox& cb, const BinRec& k) {
+    const bool rec = !(cw & 0x80000000u);
+    Rec4Pair p;
+    bbox zero;
+#pragma unroll
+    for (int e = 0; e < 3; e++) { zero.mn[e] = 0.0f; zero.mx[e] = 0.0f; }
+    p.x = box_sel(rec, k.b0, cb);
+    p.y = box_sel(rec, k.b1, zero);
+    p.wx = rec ? k.w0 : cw;
+    p.wy = rec ? k.w1 : 0xFFFFFFFFu;
+    p.axis = rec ? (k.axis & 3u) : 0u;
+    return p;
+}
+WPBR_D void put_rec4(const LbvhScratch& s, uint32_t r, const BinRec& q, const BinRec& k0, const BinRec& k1) {
+    const Rec4Pair p0 = rec4_pair(q.w0, q.b0, k0), p1 = rec4_pair(q.w1, q.b1, k1);
+    const uint32_t axes = (q.axis & 3u) | (p0.axis << 2) | (p1.axis << 4);
+    float4* o = reinterpret_cast<float4*>(s.trav4) + 8 * (size_t)r;
+    o[0] = make_float4(p0.x.mn[0], p0.x.mn[1], p0.x.mn[2], p0.x.mx[0]);
+    o[1] = make_float4(p0.x.mx[1], p0.x.mx[2], p0.y.mn[0], p0.y.mn[1]);
+    o[2] = make_float4(p0.y.mn[2], p0.y.mx[0], p0.y.mx[1], p0.y.mx[2]);
+    o[3] = make_float4(p1.x.mn[0], p1.x.mn[1], p1.x.mn[2], p1.x.mx[0]);
+    o[4] = make_float4(p1.x.mx[1], p1.x.mx[2], p1.y.mn[0], p1.y.mn[1]);
+    o[5] = make_float4(p1.y.mn[2], p1.y.mx[0], p1.y.mx[1], p1.y.mx[2]);
+    o[6] = make_float4(__uint_as_float(p0.wx), __uint_as_float(p0.wy), __uint_as_float(p1.wx),
+                       __uint_as_float(p1.wy));
+    o[7] = make_float4(__uint_as_float(axes), 0.0f, 0.0f, 0.0f);
+}
+
+// Per wave of k_subtree_leaf: a sparse table of the leaf's slot boxes (level
+// k, entry j = union of slots j .. j + 2^k - 1, clipped to the leaf) and, per
+// split, the lanes of its children that are splits too.
+struct LeafLds {
+    float2 st[6][64][3];
+    uint8_t kid[64][2];
+};
+
+// Union of slots [x, y) (0 < y - x < 64) from the sparse table: two
+// overlapping power-of-two blocks (min/max are idempotent, so the overlap is
+// exact).
+WPBR_D bbox leaf_range_box(const LeafLds& W, uint32_t x, uint32_t y) {
+    const uint32_t k = 31 - __clz(y - x);
+    const float2* p = W.st[k][x];
+    const float2* q = W.st[k][y - (1u << k)];
+    bbox a, b;
+    a.mn[0] = p[0].x; a.mn[1] = p[0].y; a.mn[2] = p[1].x; a.mx[0] = p[1].y; a.mx[1] = p[2].x; a.mx[2] = p[2].y;
+    b.mn[0] = q[0].x; b.mn[1] = q[0].y; b.mn[2] = q[1].x; b.mx[0] = q[1].y; b.mx[1] = q[2].x; b.mx[2] = q[2].y;
+    return box_union(a, b);
+}
+
+// A range of equal codes [a0, a0 + len) (len > 4) is split in halves level by
+// level: node (l, t) covers [a0 + lo(l, t), a0 + lo(l, t + 1)) with
+// lo(l, t) = floor(t len / 2^l); the root is record `root`, node (l, t) of
+// level l > 0 is record base + 2^l - 2 + t ((2 << levels) - 2 of them).
+WPBR_D BinRec run_node(const LeafLds& W, uint32_t off, uint32_t a0, uint32_t len, uint32_t l, uint32_t t,
+                       uint32_t root, uint32_t base, bool& swapped) {
+    const uint32_t a = sub_lo(len, l, t), b = sub_lo(len, l, t + 1), m = sub_lo(len, l + 1, 2 * t + 1);
+    const uint32_t w0 = (m - a <= RANGE_MAX) ? range_word(off + a0 + a, m - a) : sub_record(root, base, l + 1, 2 * t);
+    const uint32_t w1 =
+        (b - m <= RANGE_MAX) ? range_word(off + a0 + m, b - m) : sub_record(root, base, l + 1, 2 * t + 1);
+    return pair_rec(leaf_range_box(W, a0 + a, a0 + m), leaf_range_box(W, a0 + m, a0 + b), w0, w1, swapped);
+}
+
+WPBR_D uint32_t leaf_run_count(uint32_t len) { return (2u << sub_levels(len)) - 2; }
+
+// Binary and 4-wide records of a run's sub-tree (one lane).
+WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t off, uint32_t a0, uint32_t len,
+                             uint32_t root, uint32_t base) {
+    const uint32_t Lmax = sub_levels(len);
+    for (uint32_t l = 0; l <= Lmax; l++) {
+        for (uint32_t t = 0; t < (1u << l); t++) {
+            const uint32_t a = sub_lo(len, l, t), b = sub_lo(len, l, t + 1);
+            if (b - a <= RANGE_MAX) continue;
+            const uint32_t m = sub_lo(len, l + 1, 2 * t + 1);
+            bool sw, sw0, sw1;
+            const BinRec q = run_node(W, off, a0, len, l, t, root, base, sw);
+            const BinRec c0 = (m - a > RANGE_MAX) ? run_node(W, off, a0, len, l + 1, 2 * t, root, base, sw0) : q;
+            const BinRec c1 = (b - m > RANGE_MAX) ? run_node(W, off, a0, len, l + 1, 2 * t + 1, root, base, sw1) : q;
+            const uint32_t r = sub_record(root, base, l, t);
+            write_rec(s, r, q.b0, q.b1, q.w0, q.w1, q.axis);
+            put_rec4(s, r, q, sw ? c1 : c0, sw ? c0 : c1);
+        }
+    }
+}
+
+// The sub-tree of reference leaf i (5 .. 64 shapes at slots off .. off + n - 1,
+// Morton-ordered) by one wave, lane j holding slot j; binary and 4-wide records
+// of all its nodes.  It is the tree emitLBVH would continue below its leaf size
+// (SLBVH.cpp:374-401): a range whose codes differ splits at its highest
+// differing bit (child 0 below child 1 on axis bit % 3); a range of equal codes
+// splits in halves; ranges of at most 4 slots are primitive ranges.  The splits
+// at differing bits are found all at once: boundary j (between slots j - 1 and
+// j) with d_j = clz(code_{j-1} ^ code_j) < 32 is the split of range [a, b), a
+// and b the nearest boundaries on either side with a smaller d (or the leaf's
+// ends), and its parent is the one of those two with the larger d.  The root is
+// record i.  The sub-tree shape inside a reference leaf does not change any
+// hit: candidates of one leaf are ranked by slot.
+WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ codes, LeafLds& W, uint32_t i,
+                         uint32_t nn) {
+    const uint32_t j = lane_id();
+    const uint32_t n = s.node_nshapes[i], off = s.node_offset[i];
+    const bool valid = j < n;
+    const uint32_t c = valid ? codes[off + j] : 0u;
+    bbox B = valid ? slot_box(s.slot_bounds, off + j) : box_empty();
+    if (valid) set_prim_leaf(s, off + j, off);
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        W.st[k][j][0] = make_float2(B.mn[0], B.mn[1]);
+        W.st[k][j][1] = make_float2(B.mn[2], B.mx[0]);
+        W.st[k][j][2] = make_float2(B.mx[1], B.mx[2]);
+        if (k < 5) {
+            const int d = 1 << k;
+            bbox o;
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                o.mn[q] = __shfl_down(B.mn[q], d);
+                o.mx[q] = __shfl_down(B.mx[q], d);
+            }
+            if (j + d < 64) B = box_union(B, o);
+        }
+    }
+    const uint32_t cprev = (uint32_t)__shfl_up((int)c, 1);
+    const bool bnd = j >= 1 && j < n;
+    const int delta = bnd ? ((c == cprev) ? 32 : __clz(c ^ cprev)) : 64;
+    const bool kar = delta < 32;
+    // nearest boundaries with a smaller delta, one distinct delta at a time
+    uint32_t a = 0, b = n;
+    uint64_t pending = ballot64(kar);
+    const bool whole_run = pending == 0;
+    while (pending) {
+        const int v = __shfl(delta, (int)(__ffsll((unsigned long long)pending) - 1));
+        const uint64_t smaller = ballot64(kar && delta < v);
+        const uint64_t same = ballot64(kar && delta == v);
+        if (kar && delta == v) {
+            const uint64_t left = smaller & ((1ull << j) - 1ull);
+            const uint64_t right = smaller & ~((2ull << j) - 1ull);
+            a = left ? 63u - (uint32_t)__clzll((long long)left) : 0u;
+            b = right ? (uint32_t)(__ffsll((unsigned long long)right) - 1) : n;
+        }
+        pending &= ~same;
+    }
+    const bool rec = kar && b - a > RANGE_MAX;
+    const bool root = kar && a == 0 && b == n;
+    // children that are runs of equal codes longer than a primitive range
+    const uint32_t ca = (uint32_t)__shfl((int)c, (int)a);
+    const uint32_t cb1 = (uint32_t)__shfl((int)c, (int)(b - 1));
+    const bool runL = rec && j - a > RANGE_MAX && ca == cprev;
+    const bool runR = rec && b - j > RANGE_MAX && c == cb1;
+    uint32_t cnt = (rec && !root) ? 1u : 0u;
+    if (runL) cnt += 1 + leaf_run_count(j - a);
+    if (runR) cnt += 1 + leaf_run_count(b - j);
+    if (whole_run && j == 0) cnt = leaf_run_count(n);
+    // record numbers: wave prefix of cnt from the leaf's block nn + off
+    uint32_t incl = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t x = (uint32_t)__shfl_up((int)incl, d);
+        if (j >= (uint32_t)d) incl += x;
+    }
+    uint32_t next = nn + off + incl - cnt;
+    if (whole_run) {
+        if (j == 0) leaf_run_records(s, W, off, 0, n, i, next);
+        return;
+    }
+    const uint32_t r = root ? i : (rec ? next++ : 0u);
+    uint32_t rootL = 0, rootR = 0, baseL = 0, baseR = 0;
+    if (runL) { rootL = next++; baseL = next; next += leaf_run_count(j - a); }
+    if (runR) { rootR = next++; baseR = next; next += leaf_run_count(b - j); }
+    // a split tells its parent split its lane
+    const int sa = __shfl(delta, (int)(a & 63u)), sb = __shfl(delta, (int)(b & 63u));
+    W.kid[j][0] = (uint8_t)j;
+    W.kid[j][1] = (uint8_t)j;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (rec && !root) {
+        const int da = a == 0 ? -1 : sa;
+        const int db = b == n ? -1 : sb;
+        const bool parent_left = da > db;  // the parent splits at a: this is its child 1
+        W.kid[parent_left ? a : b][parent_left ? 1 : 0] = (uint8_t)j;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t k0 = W.kid[j][0], k1 = W.kid[j][1];
+    // the child splits' record numbers, ranges and axes
+    const uint32_t r0 = (uint32_t)__shfl((int)r, (int)k0), r1 = (uint32_t)__shfl((int)r, (int)k1);
+    const uint32_t a0 = (uint32_t)__shfl((int)a, (int)k0), b0 = (uint32_t)__shfl((int)b, (int)k0);
+    const uint32_t a1 = (uint32_t)__shfl((int)a, (int)k1), b1 = (uint32_t)__shfl((int)b, (int)k1);
+    const int d0 = __shfl(delta, (int)k0), d1 = __shfl(delta, (int)k1);
+    BinRec q;
+    q.w0 = (j - a <= RANGE_MAX) ? range_word(off + a, j - a) : (runL ? rootL : r0);
+    q.w1 = (b - j <= RANGE_MAX) ? range_word(off + j, b - j) : (runR ? rootR : r1);
+    q.axis = (uint32_t)((31 - delta) % 3);
+    // the child splits' words
+    const uint32_t w00 = (uint32_t)__shfl((int)q.w0, (int)k0), w01 = (uint32_t)__shfl((int)q.w1, (int)k0);
+    const uint32_t w10 = (uint32_t)__shfl((int)q.w0, (int)k1), w11 = (uint32_t)__shfl((int)q.w1, (int)k1);
+    if (!rec) return;
+    q.b0 = leaf_range_box(W, a, j);
+    q.b1 = leaf_range_box(W, j, b);
+    BinRec c0 = q, c1 = q;
+    if (!(q.w0 & 0x80000000u)) {
+        if (runL) {
+            bool sw;
+            c0 = run_node(W, off, a, j - a, 0, 0, rootL, baseL, sw);
+        } else {
+            c0.b0 = leaf_range_box(W, a0, k0);
+            c0.b1 = leaf_range_box(W, k0, b0);
+            c0.w0 = w00;
+            c0.w1 = w01;
+            c0.axis = (uint32_t)((31 - d0) % 3);
+        }
+    }
+    if (!(q.w1 & 0x80000000u)) {
+        if (runR) {
+            bool sw;
+            c1 = run_node(W, off, j, b - j, 0, 0, rootR, baseR, sw);
+        } else {
+            c1.b0 = leaf_range_box(W, a1, k1);
+            c1.b1 = leaf_range_box(W, k1, b1);
+            c1.w0 = w10;
+            c1.w1 = w11;
+            c1.axis = (uint32_t)((31 - d1) % 3);
+        }
+    }
+    write_rec(s, r, q.b0, q.b1, q.w0, q.w1, q.axis);
+    put_rec4(s, r, q, c0, c1);
+    if (runL) leaf_run_records(s, W, off, a, j - a, rootL, baseL);
+    if (runR) leaf_run_records(s, W, off, j, b - j, rootR, baseR);
+}
+
+// Reference leaves of 5 .. BIG_LEAF shapes, one wave each: every wave scans
+// LEAF_CHUNK consecutive nodes (grid-stride) and builds their leaves in turn.
+constexpr uint32_t LEAF_WAVES = 4;
+constexpr uint32_t LEAF_BLOCKS = 2048;
+constexpr uint32_t LEAF_CHUNK = 16;
+WPBR_D void subtree_leaf_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t bid, uint32_t nblk) {
+    __shared__ LeafLds W[LEAF_WAVES];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t nn = *s.n_nodes;
+    for (uint32_t c0 = (bid * LEAF_WAVES + wave) * LEAF_CHUNK; c0 < nn; c0 += nblk * LEAF_WAVES * LEAF_CHUNK) {
+        const uint32_t i = c0 + lane;
+        const uint32_t ns = (lane < LEAF_CHUNK && i < nn) ? s.node_nshapes[i] : 0u;
+        uint64_t m = ballot64(ns > RANGE_MAX && ns <= BIG_LEAF);
+        while (m) {
+            const uint32_t k = (uint32_t)(__ffsll((unsigned long long)m) - 1);
+            m &= m - 1;
+            subtree_leaf(s, codes, W[wave], c0 + k, nn);
+            __builtin_amdgcn_wave_barrier();
+        }
+    }
+}
+__global__ __launch_bounds__(64 * LEAF_WAVES) void k_subtree_leaf(LbvhScratch s, const uint32_t* __restrict__ codes) {
+    subtree_leaf_body(s, codes, blockIdx.x, gridDim.x);
 }
 
 WPBR_D bbox record_union(const LbvhScratch& s, uint32_t r) {
@@ -1448,7 +1777,7 @@ WPBR_D bbox record_union(const LbvhScratch& s, uint32_t r) {
 
 // Leaves of more than BIG_LEAF shapes (only when every Morton code in the leaf
 // is equal): one workgroup each, levels bottom-up.
-WPBR_D void subtree_big_body(const LbvhScratch& s, const uint32_t* __restrict__ sorted_index, uint32_t bid,
+WPBR_D void subtree_big_body(const LbvhScratch& s, uint32_t bid,
                             uint32_t nblk) {
     const uint32_t count = *s.big_count;
     const uint32_t nn = *s.n_nodes;
@@ -1457,8 +1786,8 @@ WPBR_D void subtree_big_body(const LbvhScratch& s, const uint32_t* __restrict__ 
         const uint32_t i = s.big_list[e];
         const uint32_t n = s.node_nshapes[i], off = s.node_offset[i];
         const uint32_t Lmax = sub_levels(n);
-        if (threadIdx.x == 0) s_base = nn + atomicAdd(s.sub_count, (2u << Lmax) - 2);
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s.slot_leaf[off + k] = off;
+        if (threadIdx.x == 0) s_base = nn + off;
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) set_prim_leaf(s, off + k, off);
         __syncthreads();
         const uint32_t base = s_base;
         for (int L = (int)Lmax; L >= 0; L--) {
@@ -1470,14 +1799,14 @@ WPBR_D void subtree_big_body(const LbvhScratch& s, const uint32_t* __restrict__ 
                 bbox b0, b1;
                 if (m - a <= RANGE_MAX) {
                     w0 = range_word(off + a, m - a);
-                    b0 = slots_box(s, sorted_index, off + a, off + m);
+                    b0 = slots_box(s, off + a, off + m);
                 } else {
                     w0 = sub_record(i, base, L + 1, 2 * j);
                     b0 = record_union(s, w0);
                 }
                 if (b - m <= RANGE_MAX) {
                     w1 = range_word(off + m, b - m);
-                    b1 = slots_box(s, sorted_index, off + m, off + b);
+                    b1 = slots_box(s, off + m, off + b);
                 } else {
                     w1 = sub_record(i, base, L + 1, 2 * j + 1);
                     b1 = record_union(s, w1);
@@ -1489,91 +1818,35 @@ WPBR_D void subtree_big_body(const LbvhScratch& s, const uint32_t* __restrict__ 
         }
     }
 }
-__global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s, const uint32_t* __restrict__ sorted_index) {
-    subtree_big_body(s, sorted_index, blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256) void k_subtree_big(LbvhScratch s) {
+    subtree_big_body(s, blockIdx.x, gridDim.x);
 }
 
-// 4-wide records: record r holds the grandchildren of binary record r (a child
-// that is a primitive range stays one child).  Slots 0,1 come from child 0 and
-// 2,3 from child 1; w[3].x packs the axes (bits 0-1 record r, 2-3 child 0,
-// 4-5 child 1).  Visiting the pair of the near child first and, inside each
-// pair, its near child first is the binary depth-first order.  Empty slots have
-// the word 0xFFFFFFFF.
-// Only written binary records are read: reference leaves of at most 4 shapes
-// have no record, and sub-tree records allocated but not used by a leaf are
-// never written, so a child word read from such a record is not followed
-// unless it indexes an existing record (those nodes are unreachable anyway).
-WPBR_D void bvh4_body(const LbvhScratch& s, uint32_t bid) {
-    const uint32_t r = bid * blockDim.x + threadIdx.x;
-    const uint32_t nn = *s.n_nodes, total = nn + *s.sub_count;
-    if (r >= total) return;
-    if (r < nn) {
+// 4-wide records of the records not made by k_subtree_leaf: the interior
+// reference nodes and the sub-trees of the largest leaves, from their binary
+// records (the layout of put_rec4).
+WPBR_D void bvh4_record(const LbvhScratch& s, uint32_t r) {
+    const BinRec q = load_rec(s, r);
+    BinRec k0 = q, k1 = q;
+    if (!(q.w0 & 0x80000000u)) k0 = load_rec(s, q.w0);
+    if (!(q.w1 & 0x80000000u)) k1 = load_rec(s, q.w1);
+    put_rec4(s, r, q, k0, k1);
+}
+constexpr uint32_t BVH4_BLOCKS = 1024;
+WPBR_D void bvh4_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
+    const uint32_t nn = *s.n_nodes;
+    for (uint32_t r = bid * blockDim.x + threadIdx.x; r < nn; r += nblk * blockDim.x) {
         const uint32_t ns = s.node_nshapes[r];
-        if (ns > 0 && ns <= RANGE_MAX) return;
+        if (ns == 0 || ns > BIG_LEAF) bvh4_record(s, r);
     }
-    const float4* bin = reinterpret_cast<const float4*>(s.trav);
-    float4 q0 = bin[4 * (size_t)r], q1 = bin[4 * (size_t)r + 1], q2 = bin[4 * (size_t)r + 2],
-           q3 = bin[4 * (size_t)r + 3];
-    // binary record r as boxes + words
-    float cb[2][6] = {{q0.x, q0.y, q0.z, q0.w, q1.x, q1.y}, {q1.z, q1.w, q2.x, q2.y, q2.z, q2.w}};
-    uint32_t cw[2] = {__float_as_uint(q3.x), __float_as_uint(q3.y)};
-    uint32_t axes = __float_as_uint(q3.z) & 3u;
-    float box[4][6];
-    uint32_t w[4];
-    for (int c = 0; c < 2; c++) {
-        if (!(cw[c] & 0x80000000u) && cw[c] >= total) cw[c] = 0xFFFFFFFFu;
-        if (!(cw[c] & 0x80000000u)) {
-            const float4* g = bin + 4 * (size_t)cw[c];
-            float4 g0 = g[0], g1 = g[1], g2 = g[2], g3 = g[3];
-            box[2 * c][0] = g0.x; box[2 * c][1] = g0.y; box[2 * c][2] = g0.z;
-            box[2 * c][3] = g0.w; box[2 * c][4] = g1.x; box[2 * c][5] = g1.y;
-            box[2 * c + 1][0] = g1.z; box[2 * c + 1][1] = g1.w; box[2 * c + 1][2] = g2.x;
-            box[2 * c + 1][3] = g2.y; box[2 * c + 1][4] = g2.z; box[2 * c + 1][5] = g2.w;
-            w[2 * c] = __float_as_uint(g3.x);
-            w[2 * c + 1] = __float_as_uint(g3.y);
-            axes |= (__float_as_uint(g3.z) & 3u) << (2 + 2 * c);
-        } else {
-            for (int k = 0; k < 6; k++) { box[2 * c][k] = cb[c][k]; box[2 * c + 1][k] = 0.0f; }
-            w[2 * c] = cw[c];
-            w[2 * c + 1] = 0xFFFFFFFFu;
-        }
+    const uint32_t nbig = *s.big_count;
+    for (uint32_t e = bid; e < nbig; e += nblk) {
+        const uint32_t i = s.big_list[e];
+        const uint32_t cnt = (2u << sub_levels(s.node_nshapes[i])) - 2, base = nn + s.node_offset[i];
+        for (uint32_t q = threadIdx.x; q < cnt; q += blockDim.x) bvh4_record(s, base + q);
     }
-    float4* o = reinterpret_cast<float4*>(s.trav4) + 8 * (size_t)r;
-    for (int p = 0; p < 2; p++) {
-        const float* a = box[2 * p];
-        const float* b = box[2 * p + 1];
-        o[3 * p] = make_float4(a[0], a[1], a[2], a[3]);
-        o[3 * p + 1] = make_float4(a[4], a[5], b[0], b[1]);
-        o[3 * p + 2] = make_float4(b[2], b[3], b[4], b[5]);
-    }
-    o[6] = make_float4(__uint_as_float(w[0]), __uint_as_float(w[1]), __uint_as_float(w[2]),
-                       __uint_as_float(w[3]));
-    o[7] = make_float4(__uint_as_float(axes), 0.0f, 0.0f, 0.0f);
 }
-__global__ void k_bvh4(LbvhScratch s) { bvh4_body(s, blockIdx.x); }
-
-// Primitive records at their slots: v0.xyz + shape id, v1.xyz + slot,
-// v2.xyz + first slot of the reference leaf.  Spheres keep only the words.
-WPBR_D void prims_body(const LbvhScratch& s, const GeomView& g, const uint32_t* __restrict__ sorted_index,
-                      uint32_t n, uint32_t bid) {
-    uint32_t p = bid * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint32_t id = sorted_index[p];
-    uint32_t leaf = s.slot_leaf[p];
-    float4* rec = reinterpret_cast<float4*>(s.prims) + 3 * (size_t)p;
-    float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (id >= g.n_spheres) {
-        const float* t = g.tri_verts + 9 * (size_t)(id - g.n_spheres);
-#pragma unroll
-        for (int k = 0; k < 9; k++) v[k] = t[k];
-    }
-    rec[0] = make_float4(v[0], v[1], v[2], __uint_as_float(id));
-    rec[1] = make_float4(v[3], v[4], v[5], __uint_as_float(p));
-    rec[2] = make_float4(v[6], v[7], v[8], __uint_as_float(leaf));
-}
-__global__ void k_prims(LbvhScratch s, GeomView g, const uint32_t* __restrict__ sorted_index, uint32_t n) {
-    prims_body(s, g, sorted_index, n, blockIdx.x);
-}
+__global__ __launch_bounds__(256) void k_bvh4(LbvhScratch s) { bvh4_body(s, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 __global__ void k_cb_final(uint32_t* cb_ord) {
@@ -1606,13 +1879,15 @@ wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroi
 
 __global__ void ks_init(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_INIT]);
-    init_body(B.s[sc], bid, nblk);
+    const LbvhScratch s = B.s[sc];
+    init_body(s, bid, nblk);
 }
 
 __global__ __launch_bounds__(256) void ks_bounds(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_BOUNDS]);
-    const LbvhScratch& s = B.s[sc];
-    bounds_body(B.g[sc], B.n[sc], s.bounds, s.centroid, s.cb_ord, bid, nblk);
+    const LbvhScratch s = B.s[sc];
+    const GeomView g = B.g[sc];
+    bounds_body(g, B.n[sc], nullptr, s.centroid, s.cb_ord, bid, nblk);
 }
 
 __global__ void ks_cb_final(LbvhBatch B) {
@@ -1623,7 +1898,7 @@ __global__ void ks_cb_final(LbvhBatch B) {
 
 __global__ __launch_bounds__(MORTON_BLOCK) void ks_morton(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_MORTON]);
-    const LbvhScratch& s = B.s[sc];
+    const LbvhScratch s = B.s[sc];
     morton_body(s.centroid, B.n[sc], s.cb_ord, s.morton, s.hist, bid);
 }
 
@@ -1637,13 +1912,15 @@ __global__ __launch_bounds__(SORT_BLOCK) void ks_onesweep(LbvhBatch B, int p) {
 
 __global__ void ks_treelet_mark(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_MARK]);
-    const LbvhScratch& s = B.s[sc];
+    const LbvhScratch s = B.s[sc];
     treelet_mark_body(s.keys_a, B.n[sc], s.bin_start, s.presence, bid);
 }
 
-__global__ void ks_slot_bounds(LbvhBatch B) {
+__global__ __launch_bounds__(256) void ks_slot_prims(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_MARK]);
-    slot_bounds_body(B.s[sc], B.n[sc], bid);
+    const LbvhScratch s = B.s[sc];
+    const GeomView g = B.g[sc];
+    slot_prims_body(s, g, B.n[sc], bid);
 }
 
 // work[i] = number of treelets of scenes 0 .. i-1 (one workgroup).
@@ -1694,6 +1971,7 @@ WPBR_D uint32_t presence_select(const uint32_t* presence, uint32_t t) {
     return 32 * w + (uint32_t)(__ffs(m) - 1);
 }
 
+constexpr uint32_t SEG_EMIT_BLOCKS = 8192;
 __global__ __launch_bounds__(EMIT_BLOCK) void ks_emit(LbvhBatch B) {
     __shared__ uint32_t s_sc, s_bin;
     const uint32_t total = B.work[B.n_scenes];
@@ -1704,9 +1982,9 @@ __global__ __launch_bounds__(EMIT_BLOCK) void ks_emit(LbvhBatch B) {
             s_bin = presence_select(B.s[sc].presence, g - B.work[sc]);
         }
         __syncthreads();
-        const uint32_t sc = s_sc, bin = s_bin;
-        const LbvhScratch& s = B.s[sc];
-        emit_body(s, s.keys_a, s.vals_a, s.bounds, B.n[sc], bin);
+        const uint32_t sc = __builtin_amdgcn_readfirstlane(s_sc), bin = __builtin_amdgcn_readfirstlane(s_bin);
+        const LbvhScratch s = B.s[sc];
+        emit_body(s, s.keys_a, B.n[sc], bin);
         __syncthreads();
     }
 }
@@ -1745,7 +2023,7 @@ __global__ __launch_bounds__(SAH_BLOCK) void ks_sah_sub(LbvhBatch B) {
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         if (threadIdx.x == 0) s_sc = work_scene(B.subwork, B.n_scenes, g);
         __syncthreads();
-        const uint32_t sc = s_sc;
+        const uint32_t sc = __builtin_amdgcn_readfirstlane(s_sc);
         LbvhScratch s = B.s[sc];
         sah_sub_one(s, S, g - B.subwork[sc]);
     }
@@ -1759,38 +2037,40 @@ __global__ __launch_bounds__(256) void ks_scatter(LbvhBatch B) {
     const uint32_t total = B.work[B.n_scenes];
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         const uint32_t sc = work_scene(B.work, B.n_scenes, g);
-        scatter_body(B.s[sc], g - B.work[sc]);
+        const LbvhScratch s = B.s[sc];
+        scatter_body(s, g - B.work[sc]);
     }
 }
 
 __global__ __launch_bounds__(256) void ks_subtree(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_SUBTREE]);
-    const LbvhScratch& s = B.s[sc];
-    subtree_body(s, s.keys_a, s.vals_a, bid);
+    const LbvhScratch s = B.s[sc];
+    subtree_body(s, bid, nblk);
+}
+
+__global__ __launch_bounds__(64 * LEAF_WAVES) void ks_subtree_leaf(LbvhBatch B) {
+    SEG_BLOCK(B.map[SEG_SUBTREE_LEAF]);
+    const LbvhScratch s = B.s[sc];
+    subtree_leaf_body(s, s.keys_a, bid, nblk);
 }
 
 __global__ __launch_bounds__(256) void ks_subtree_big(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_SUBTREE_BIG]);
-    const LbvhScratch& s = B.s[sc];
-    subtree_big_body(s, s.vals_a, bid, nblk);
+    const LbvhScratch s = B.s[sc];
+    subtree_big_body(s, bid, nblk);
 }
 
-__global__ void ks_prims(LbvhBatch B) {
-    SEG_BLOCK(B.map[SEG_PRIMS]);
-    const LbvhScratch& s = B.s[sc];
-    prims_body(s, B.g[sc], s.vals_a, B.n[sc], bid);
-}
-
-__global__ void ks_bvh4(LbvhBatch B) {
+__global__ __launch_bounds__(256) void ks_bvh4(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_BVH4]);
-    bvh4_body(B.s[sc], bid);
+    const LbvhScratch s = B.s[sc];
+    bvh4_body(s, bid, nblk);
 }
 
 // Per scene: n_nodes, n_treelets, err, nshapes and offset of node 0.
 __global__ void ks_info(LbvhBatch B) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n_scenes) return;
-    const LbvhScratch& s = B.s[i];
+    const LbvhScratch s = B.s[i];
     uint32_t* o = B.info + 5 * (size_t)i;
     o[0] = *s.n_nodes;
     o[1] = *s.n_treelets;
@@ -1806,10 +2086,11 @@ void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts) {
     counts[SEG_MORTON] = (n + MORTON_BLOCK * MORTON_ITEMS - 1) / (MORTON_BLOCK * MORTON_ITEMS);
     counts[SEG_SORT] = (n + SORT_TILE - 1) / SORT_TILE;
     counts[SEG_MARK] = (n + 255) / 256;
-    counts[SEG_SUBTREE] = (2 * n + 255) / 256;
+    // the node-level kernels: about one node per thread (nodes ~ n / 10)
+    counts[SEG_SUBTREE] = std::min<uint32_t>(SUBTREE_BLOCKS, std::max<uint32_t>(1, (n + 2047) / 2048));
+    counts[SEG_SUBTREE_LEAF] = std::min<uint32_t>(LEAF_BLOCKS, std::max<uint32_t>(1, (n + 511) / 512));
     counts[SEG_SUBTREE_BIG] = 4;
-    counts[SEG_PRIMS] = (n + 255) / 256;
-    counts[SEG_BVH4] = (3 * std::max<uint32_t>(n, 1) + 255) / 256;
+    counts[SEG_BVH4] = std::min<uint32_t>(BVH4_BLOCKS, std::max<uint32_t>(1, (n + 511) / 512));
 }
 
 wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st) {
@@ -1821,17 +2102,17 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipS
     for (int p = 0; p < 4; p++)
         hipLaunchKernelGGL(ks_onesweep, dim3(grid[SEG_SORT]), dim3(SORT_BLOCK), 0, st, B, p);
     hipLaunchKernelGGL(ks_treelet_mark, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
-    hipLaunchKernelGGL(ks_slot_bounds, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_slot_prims, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
-    hipLaunchKernelGGL(ks_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(ks_sah_sub, dim3(SAH_SEG_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_sah_final, dim3(S), dim3(SAH_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_subtree, dim3(grid[SEG_SUBTREE]), dim3(256), 0, st, B);
+    hipLaunchKernelGGL(ks_subtree_leaf, dim3(grid[SEG_SUBTREE_LEAF]), dim3(64 * LEAF_WAVES), 0, st, B);
     hipLaunchKernelGGL(ks_subtree_big, dim3(grid[SEG_SUBTREE_BIG]), dim3(256), 0, st, B);
-    hipLaunchKernelGGL(ks_prims, dim3(grid[SEG_PRIMS]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_bvh4, dim3(grid[SEG_BVH4]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_info, dim3((S + 255) / 256), dim3(256), 0, st, B);
     return launch_status();
@@ -1849,14 +2130,14 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipS
 // ev[0..11] bracket the kernels (the per-kernel times of wpbr_lbvh_kernel_times).
 wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
                               hipEvent_t* ev, int nev) {
-    auto rec = [&](int i) { if (ev && i < nev) hipEventRecord(ev[i], st); };
+    auto rec = [&](int i) { if (ev && i < nev) (void)hipEventRecord(ev[i], st); };
     switch (stage) {
         case 0:
             rec(0);
             hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, (s.status_words + 255) / 256)), dim3(256), 0,
                                st, s);
             hipLaunchKernelGGL(k_bounds, dim3(std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256)), dim3(256), 0,
-                               st, g, n, s.bounds, s.centroid, s.cb_ord);
+                               st, g, n, nullptr, s.centroid, s.cb_ord);
             rec(1);
             break;
         case 1:
@@ -1877,12 +2158,12 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
             }
             hipLaunchKernelGGL(k_treelet_mark, dim3((n + 255) / 256), dim3(256), 0, st, s.keys_a, n, s.bin_start,
                                s.presence);
-            hipLaunchKernelGGL(k_slot_bounds, dim3((n + 255) / 256), dim3(256), 0, st, s, n);
+            hipLaunchKernelGGL(k_slot_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, n);
             rec(7);
             break;
         }
         case 4:
-            hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, s.vals_a, s.bounds, n);
+            hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, n);
             rec(8);
             break;
         case 5: {
@@ -1893,11 +2174,14 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
             hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
             rec(10);
             const uint32_t max_nodes = 2 * n;
-            hipLaunchKernelGGL(k_subtree, dim3((max_nodes + 255) / 256), dim3(256), 0, st, s, s.keys_a, s.vals_a);
-            hipLaunchKernelGGL(k_subtree_big, dim3(64), dim3(256), 0, st, s, s.vals_a);
-            hipLaunchKernelGGL(k_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, s.vals_a, n);
+            hipLaunchKernelGGL(k_subtree, dim3(std::min<uint32_t>(SUBTREE_BLOCKS, (max_nodes + 255) / 256)), dim3(256),
+                               0, st, s);
+            hipLaunchKernelGGL(k_subtree_leaf, dim3(std::min<uint32_t>(LEAF_BLOCKS, std::max<uint32_t>(1, (n + 511) / 512))),
+                               dim3(64 * LEAF_WAVES), 0, st, s, s.keys_a);
+            hipLaunchKernelGGL(k_subtree_big, dim3(64), dim3(256), 0, st, s);
             const uint32_t max_records = 3 * std::max<uint32_t>(n, 1);
-            hipLaunchKernelGGL(k_bvh4, dim3((max_records + 255) / 256), dim3(256), 0, st, s);
+            hipLaunchKernelGGL(k_bvh4, dim3(std::min<uint32_t>(BVH4_BLOCKS, (max_records + 255) / 256)), dim3(256), 0,
+                               st, s);
             rec(11);
             break;
         }

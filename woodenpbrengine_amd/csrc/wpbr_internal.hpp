@@ -29,7 +29,6 @@ struct GeomView {
 // Device buffers of one LBVH build (all pointers device memory).
 struct LbvhScratch {
     // a1-a3
-    float* bounds;        // n x 6 (shape order)
     float* slot_bounds;   // n x 6 (slot order: bounds of shape vals_a[p] at p)
     float* centroid;      // n x 3
     uint32_t* cb_ord;     // 6: centroid bounds as order-preserving uints
@@ -67,8 +66,6 @@ struct LbvhScratch {
     void* trav;           // 64 B per binary record
     void* trav4;          // 128 B per record: the record's grandchildren (4-wide)
     void* prims;          // 48 B per shape, at its slot
-    uint32_t* slot_leaf;  // per slot: first slot of its reference leaf
-    uint32_t* sub_count;  // sub-tree records appended after the reference nodes
     uint32_t* big_list;   // reference leaves of more than 64 shapes
     uint32_t* big_count;
 };
@@ -94,8 +91,8 @@ __host__ __device__ inline SortPass sort_pass(const LbvhScratch& s, int p) {
 
 // Segmented build of several scenes (one launch per kernel for all of them).
 // Block b of a launch of kind k works for scene map[k].scene[b].
-enum SegKind { SEG_INIT, SEG_BOUNDS, SEG_MORTON, SEG_SORT, SEG_MARK, SEG_SUBTREE, SEG_SUBTREE_BIG, SEG_PRIMS,
-               SEG_BVH4, SEG_KINDS };
+enum SegKind { SEG_INIT, SEG_BOUNDS, SEG_MORTON, SEG_SORT, SEG_MARK, SEG_SUBTREE, SEG_SUBTREE_LEAF, SEG_SUBTREE_BIG, SEG_BVH4,
+               SEG_KINDS };
 struct SegMap {
     const uint16_t* scene;  // per block
     const uint32_t* first;  // per scene: its first block
