@@ -436,13 +436,6 @@ WPBR_D void set_prim_leaf(const LbvhScratch& s, uint32_t slot, uint32_t leaf) {
     reinterpret_cast<uint32_t*>(s.prims)[12 * (size_t)slot + 11] = leaf;
 }
 
-WPBR_D uint32_t presence_rank(const uint32_t* presence, uint32_t bin) {
-    uint32_t r = 0;
-    for (uint32_t w = 0; w < (bin >> 5); w++) r += __popc(presence[w]);
-    uint32_t m = bin & 31;
-    if (m) r += __popc(presence[bin >> 5] & ((1u << m) - 1));
-    return r;
-}
 
 // ---------------------------------------------------------------------------
 // Emit: one workgroup per occupied top-12-bit bin (= treelet).
@@ -456,13 +449,23 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
     __shared__ uint32_t s_tidx, s_start, s_cnt;
     const int tid = threadIdx.x;
     if (!((s.presence[bin >> 5] >> (bin & 31)) & 1u)) return;
+    // the treelet's rank (occupied bins before it) and the next occupied bin,
+    // from the 128 presence words, two per lane (EMIT_BLOCK is one wave)
+    const uint32_t bw = bin >> 5, below = (1u << (bin & 31)) - 1u, above = ~((2u << (bin & 31)) - 1u);
+    const uint32_t q0 = 2 * (uint32_t)tid, q1 = q0 + 1;
+    const uint32_t p0 = s.presence[q0], p1 = s.presence[q1];
+    uint32_t cnt = (q0 < bw ? __popc(p0) : (q0 == bw ? __popc(p0 & below) : 0u)) +
+                   (q1 < bw ? __popc(p1) : (q1 == bw ? __popc(p1 & below) : 0u));
+    for (int d = 32; d >= 1; d >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, d);
+    const uint32_t m0 = q0 > bw ? p0 : (q0 == bw ? p0 & above : 0u);
+    const uint32_t m1 = q1 > bw ? p1 : (q1 == bw ? p1 & above : 0u);
+    const uint64_t has = ballot64((m0 | m1) != 0u);
+    const uint32_t nb_lane = m0 ? 32 * q0 + (uint32_t)(__ffs(m0) - 1) : 32 * q1 + (uint32_t)(__ffs(m1) - 1);
+    const uint32_t next_bin = has ? (uint32_t)__shfl((int)nb_lane, (int)(__ffsll((unsigned long long)has) - 1)) : 0u;
     if (tid == 0) {
-        uint32_t t = presence_rank(s.presence, bin);
+        uint32_t t = cnt;
         uint32_t start = s.bin_start[bin];
-        uint32_t end = n;
-        for (uint32_t b = bin + 1; b < 4096; b++) {
-            if ((s.presence[b >> 5] >> (b & 31)) & 1u) { end = s.bin_start[b]; break; }
-        }
+        uint32_t end = has ? s.bin_start[next_bin] : n;
         s_tidx = t;
         s_start = start;
         s_cnt = end - start;
@@ -480,7 +483,7 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
         s_nlvl = 1;
     }
     __syncthreads();
-    const uint32_t base = s.t_slot[s_tidx];
+    const uint32_t base = 2 * s_start - s_tidx;
     // Breadth-first construction.
     for (int L = 0; L < EMIT_MAX_LEVELS; L++) {
         uint32_t lb = s_lvl[L], le = s_lvl[L + 1];
@@ -1540,21 +1543,22 @@ WPBR_D void put_rec4(const LbvhScratch& s, uint32_t r, const BinRec& q, const Bi
     o[7] = make_float4(__uint_as_float(axes), 0.0f, 0.0f, 0.0f);
 }
 
-// Per wave of k_subtree_leaf: a sparse table of the leaf's slot boxes (level
-// k, entry j = union of slots j .. j + 2^k - 1, clipped to the leaf) and, per
-// split, the lanes of its children that are splits too.
+// Per wave of k_subtree_leaf: a sparse table of the slot boxes of the wave's
+// leaf (or of its two leaves, one per 32-lane half, each half using its own
+// lanes' entries): level k, entry j = union of slots j .. j + 2^k - 1 of the
+// leaf, clipped to it; and per split the lanes of its children that are splits.
 struct LeafLds {
     float2 st[6][64][3];
     uint8_t kid[64][2];
 };
 
-// Union of slots [x, y) (0 < y - x < 64) from the sparse table: two
-// overlapping power-of-two blocks (min/max are idempotent, so the overlap is
-// exact).
-WPBR_D bbox leaf_range_box(const LeafLds& W, uint32_t x, uint32_t y) {
+// Union of slots [x, y) (0 < y - x < 64) of the leaf whose table entries start
+// at lane `base`: two overlapping power-of-two blocks (min/max are idempotent,
+// so the overlap is exact).
+WPBR_D bbox leaf_range_box(const LeafLds& W, uint32_t base, uint32_t x, uint32_t y) {
     const uint32_t k = 31 - __clz(y - x);
-    const float2* p = W.st[k][x];
-    const float2* q = W.st[k][y - (1u << k)];
+    const float2* p = W.st[k][base + x];
+    const float2* q = W.st[k][base + y - (1u << k)];
     bbox a, b;
     a.mn[0] = p[0].x; a.mn[1] = p[0].y; a.mn[2] = p[1].x; a.mx[0] = p[1].y; a.mx[1] = p[2].x; a.mx[2] = p[2].y;
     b.mn[0] = q[0].x; b.mn[1] = q[0].y; b.mn[2] = q[1].x; b.mx[0] = q[1].y; b.mx[1] = q[2].x; b.mx[2] = q[2].y;
@@ -1565,20 +1569,20 @@ WPBR_D bbox leaf_range_box(const LeafLds& W, uint32_t x, uint32_t y) {
 // level: node (l, t) covers [a0 + lo(l, t), a0 + lo(l, t + 1)) with
 // lo(l, t) = floor(t len / 2^l); the root is record `root`, node (l, t) of
 // level l > 0 is record base + 2^l - 2 + t ((2 << levels) - 2 of them).
-WPBR_D BinRec run_node(const LeafLds& W, uint32_t off, uint32_t a0, uint32_t len, uint32_t l, uint32_t t,
-                       uint32_t root, uint32_t base, bool& swapped) {
+WPBR_D BinRec run_node(const LeafLds& W, uint32_t lb, uint32_t off, uint32_t a0, uint32_t len, uint32_t l,
+                       uint32_t t, uint32_t root, uint32_t base, bool& swapped) {
     const uint32_t a = sub_lo(len, l, t), b = sub_lo(len, l, t + 1), m = sub_lo(len, l + 1, 2 * t + 1);
     const uint32_t w0 = (m - a <= RANGE_MAX) ? range_word(off + a0 + a, m - a) : sub_record(root, base, l + 1, 2 * t);
     const uint32_t w1 =
         (b - m <= RANGE_MAX) ? range_word(off + a0 + m, b - m) : sub_record(root, base, l + 1, 2 * t + 1);
-    return pair_rec(leaf_range_box(W, a0 + a, a0 + m), leaf_range_box(W, a0 + m, a0 + b), w0, w1, swapped);
+    return pair_rec(leaf_range_box(W, lb, a0 + a, a0 + m), leaf_range_box(W, lb, a0 + m, a0 + b), w0, w1, swapped);
 }
 
 WPBR_D uint32_t leaf_run_count(uint32_t len) { return (2u << sub_levels(len)) - 2; }
 
 // Binary and 4-wide records of a run's sub-tree (one lane).
-WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t off, uint32_t a0, uint32_t len,
-                             uint32_t root, uint32_t base) {
+WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t lb, uint32_t off, uint32_t a0,
+                             uint32_t len, uint32_t root, uint32_t base) {
     const uint32_t Lmax = sub_levels(len);
     for (uint32_t l = 0; l <= Lmax; l++) {
         for (uint32_t t = 0; t < (1u << l); t++) {
@@ -1586,9 +1590,10 @@ WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t of
             if (b - a <= RANGE_MAX) continue;
             const uint32_t m = sub_lo(len, l + 1, 2 * t + 1);
             bool sw, sw0, sw1;
-            const BinRec q = run_node(W, off, a0, len, l, t, root, base, sw);
-            const BinRec c0 = (m - a > RANGE_MAX) ? run_node(W, off, a0, len, l + 1, 2 * t, root, base, sw0) : q;
-            const BinRec c1 = (b - m > RANGE_MAX) ? run_node(W, off, a0, len, l + 1, 2 * t + 1, root, base, sw1) : q;
+            const BinRec q = run_node(W, lb, off, a0, len, l, t, root, base, sw);
+            BinRec c0 = q, c1 = q;
+            if (m - a > RANGE_MAX) c0 = run_node(W, lb, off, a0, len, l + 1, 2 * t, root, base, sw0);
+            if (b - m > RANGE_MAX) c1 = run_node(W, lb, off, a0, len, l + 1, 2 * t + 1, root, base, sw1);
             const uint32_t r = sub_record(root, base, l, t);
             write_rec(s, r, q.b0, q.b1, q.w0, q.w1, q.axis);
             put_rec4(s, r, q, sw ? c1 : c0, sw ? c0 : c1);
@@ -1596,10 +1601,11 @@ WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t of
     }
 }
 
-// The sub-tree of reference leaf i (5 .. 64 shapes at slots off .. off + n - 1,
-// Morton-ordered) by one wave, lane j holding slot j; binary and 4-wide records
-// of all its nodes.  It is the tree emitLBVH would continue below its leaf size
-// (SLBVH.cpp:374-401): a range whose codes differ splits at its highest
+// The sub-tree of a reference leaf of 5 .. WD shapes (slots off .. off + n - 1,
+// Morton-ordered) by WD lanes, lane j holding slot j: binary and 4-wide records
+// of all its nodes.  WD = 32 runs two leaves per wave, one per half (`i` is the
+// half's leaf, or none).  It is the tree emitLBVH would continue below its leaf
+// size (SLBVH.cpp:374-401): a range whose codes differ splits at its highest
 // differing bit (child 0 below child 1 on axis bit % 3); a range of equal codes
 // splits in halves; ranges of at most 4 slots are primitive ranges.  The splits
 // at differing bits are found all at once: boundary j (between slots j - 1 and
@@ -1608,80 +1614,89 @@ WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t of
 // ends), and its parent is the one of those two with the larger d.  The root is
 // record i.  The sub-tree shape inside a reference leaf does not change any
 // hit: candidates of one leaf are ranked by slot.
+constexpr uint32_t NO_LEAF = 0xFFFFFFFFu;
+template <int WD>
 WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ codes, LeafLds& W, uint32_t i,
                          uint32_t nn) {
-    const uint32_t j = lane_id();
-    const uint32_t n = s.node_nshapes[i], off = s.node_offset[i];
+    constexpr uint64_t HMASK = WD == 64 ? ~0ull : 0xFFFFFFFFull;
+    const uint32_t lane = lane_id(), j = lane % WD, lb = lane - j, sh = lb;  // lb: the half's first lane
+    const bool has = i != NO_LEAF;
+    const uint32_t n = has ? s.node_nshapes[i] : 0u, off = has ? s.node_offset[i] : 0u;
     const bool valid = j < n;
     const uint32_t c = valid ? codes[off + j] : 0u;
     bbox B = valid ? slot_box(s.slot_bounds, off + j) : box_empty();
     if (valid) set_prim_leaf(s, off + j, off);
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-        W.st[k][j][0] = make_float2(B.mn[0], B.mn[1]);
-        W.st[k][j][1] = make_float2(B.mn[2], B.mx[0]);
-        W.st[k][j][2] = make_float2(B.mx[1], B.mx[2]);
-        if (k < 5) {
+        if ((1 << k) >= 2 * WD) break;
+        W.st[k][lane][0] = make_float2(B.mn[0], B.mn[1]);
+        W.st[k][lane][1] = make_float2(B.mn[2], B.mx[0]);
+        W.st[k][lane][2] = make_float2(B.mx[1], B.mx[2]);
+        if ((2 << k) < WD) {
             const int d = 1 << k;
             bbox o;
 #pragma unroll
             for (int q = 0; q < 3; q++) {
-                o.mn[q] = __shfl_down(B.mn[q], d);
-                o.mx[q] = __shfl_down(B.mx[q], d);
+                o.mn[q] = __shfl_down(B.mn[q], d, WD);
+                o.mx[q] = __shfl_down(B.mx[q], d, WD);
             }
-            if (j + d < 64) B = box_union(B, o);
+            if (j + d < (uint32_t)WD) B = box_union(B, o);
         }
     }
-    const uint32_t cprev = (uint32_t)__shfl_up((int)c, 1);
+    const uint32_t cprev = (uint32_t)__shfl_up((int)c, 1, WD);
     const bool bnd = j >= 1 && j < n;
     const int delta = bnd ? ((c == cprev) ? 32 : __clz(c ^ cprev)) : 64;
     const bool kar = delta < 32;
-    // nearest boundaries with a smaller delta, one distinct delta at a time
+    // nearest boundaries with a smaller delta: one ballot per delta value
+    // present (values below the smallest one present have no lanes)
     uint32_t a = 0, b = n;
-    uint64_t pending = ballot64(kar);
-    const bool whole_run = pending == 0;
-    while (pending) {
-        const int v = __shfl(delta, (int)(__ffsll((unsigned long long)pending) - 1));
-        const uint64_t smaller = ballot64(kar && delta < v);
+    int vmin = kar ? delta : 32;
+    for (int d = 32; d >= 1; d >>= 1) vmin = min(vmin, __shfl_xor(vmin, d));
+    for (int v = vmin; v < 32; v++) {
         const uint64_t same = ballot64(kar && delta == v);
+        if (!same) continue;
+        const uint64_t smaller = (ballot64(kar && delta < v) >> sh) & HMASK;
         if (kar && delta == v) {
             const uint64_t left = smaller & ((1ull << j) - 1ull);
             const uint64_t right = smaller & ~((2ull << j) - 1ull);
             a = left ? 63u - (uint32_t)__clzll((long long)left) : 0u;
             b = right ? (uint32_t)(__ffsll((unsigned long long)right) - 1) : n;
         }
-        pending &= ~same;
     }
+    const bool whole_run = has && n > RANGE_MAX && ((ballot64(kar) >> sh) & HMASK) == 0;
     const bool rec = kar && b - a > RANGE_MAX;
     const bool root = kar && a == 0 && b == n;
     // children that are runs of equal codes longer than a primitive range
-    const uint32_t ca = (uint32_t)__shfl((int)c, (int)a);
-    const uint32_t cb1 = (uint32_t)__shfl((int)c, (int)(b - 1));
+    const uint32_t ca = (uint32_t)__shfl((int)c, (int)a, WD);
+    const uint32_t cb1 = (uint32_t)__shfl((int)c, (int)((b - 1) & (WD - 1)), WD);
     const bool runL = rec && j - a > RANGE_MAX && ca == cprev;
     const bool runR = rec && b - j > RANGE_MAX && c == cb1;
     uint32_t cnt = (rec && !root) ? 1u : 0u;
     if (runL) cnt += 1 + leaf_run_count(j - a);
     if (runR) cnt += 1 + leaf_run_count(b - j);
     if (whole_run && j == 0) cnt = leaf_run_count(n);
-    // record numbers: wave prefix of cnt from the leaf's block nn + off
-    uint32_t incl = cnt;
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t x = (uint32_t)__shfl_up((int)incl, d);
-        if (j >= (uint32_t)d) incl += x;
+    // record numbers: prefix of cnt over the leaf's lanes, from its block nn + off
+    uint32_t excl;
+    if (ballot64(cnt > 1u) == 0) {
+        excl = (uint32_t)__popcll((long long)((ballot64(cnt != 0u) >> sh) & HMASK & ((1ull << j) - 1ull)));
+    } else {
+        uint32_t incl = cnt;
+        for (int d = 1; d < WD; d <<= 1) {
+            const uint32_t x = (uint32_t)__shfl_up((int)incl, d, WD);
+            if (j >= (uint32_t)d) incl += x;
+        }
+        excl = incl - cnt;
     }
-    uint32_t next = nn + off + incl - cnt;
-    if (whole_run) {
-        if (j == 0) leaf_run_records(s, W, off, 0, n, i, next);
-        return;
-    }
+    uint32_t next = nn + off + excl;
+    if (whole_run && j == 0) leaf_run_records(s, W, lb, off, 0, n, i, next);
     const uint32_t r = root ? i : (rec ? next++ : 0u);
     uint32_t rootL = 0, rootR = 0, baseL = 0, baseR = 0;
     if (runL) { rootL = next++; baseL = next; next += leaf_run_count(j - a); }
     if (runR) { rootR = next++; baseR = next; next += leaf_run_count(b - j); }
     // a split tells its parent split its lane
-    const int sa = __shfl(delta, (int)(a & 63u)), sb = __shfl(delta, (int)(b & 63u));
-    W.kid[j][0] = (uint8_t)j;
-    W.kid[j][1] = (uint8_t)j;
+    const int sa = __shfl(delta, (int)(a & (WD - 1)), WD), sb = __shfl(delta, (int)(b & (WD - 1)), WD);
+    W.kid[lane][0] = (uint8_t)j;
+    W.kid[lane][1] = (uint8_t)j;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1689,35 +1704,35 @@ WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ code
         const int da = a == 0 ? -1 : sa;
         const int db = b == n ? -1 : sb;
         const bool parent_left = da > db;  // the parent splits at a: this is its child 1
-        W.kid[parent_left ? a : b][parent_left ? 1 : 0] = (uint8_t)j;
+        W.kid[lb + (parent_left ? a : b)][parent_left ? 1 : 0] = (uint8_t)j;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const uint32_t k0 = W.kid[j][0], k1 = W.kid[j][1];
+    const uint32_t k0 = W.kid[lane][0], k1 = W.kid[lane][1];
     // the child splits' record numbers, ranges and axes
-    const uint32_t r0 = (uint32_t)__shfl((int)r, (int)k0), r1 = (uint32_t)__shfl((int)r, (int)k1);
-    const uint32_t a0 = (uint32_t)__shfl((int)a, (int)k0), b0 = (uint32_t)__shfl((int)b, (int)k0);
-    const uint32_t a1 = (uint32_t)__shfl((int)a, (int)k1), b1 = (uint32_t)__shfl((int)b, (int)k1);
-    const int d0 = __shfl(delta, (int)k0), d1 = __shfl(delta, (int)k1);
+    const uint32_t r0 = (uint32_t)__shfl((int)r, (int)k0, WD), r1 = (uint32_t)__shfl((int)r, (int)k1, WD);
+    const uint32_t a0 = (uint32_t)__shfl((int)a, (int)k0, WD), b0 = (uint32_t)__shfl((int)b, (int)k0, WD);
+    const uint32_t a1 = (uint32_t)__shfl((int)a, (int)k1, WD), b1 = (uint32_t)__shfl((int)b, (int)k1, WD);
+    const int d0 = __shfl(delta, (int)k0, WD), d1 = __shfl(delta, (int)k1, WD);
     BinRec q;
     q.w0 = (j - a <= RANGE_MAX) ? range_word(off + a, j - a) : (runL ? rootL : r0);
     q.w1 = (b - j <= RANGE_MAX) ? range_word(off + j, b - j) : (runR ? rootR : r1);
     q.axis = (uint32_t)((31 - delta) % 3);
     // the child splits' words
-    const uint32_t w00 = (uint32_t)__shfl((int)q.w0, (int)k0), w01 = (uint32_t)__shfl((int)q.w1, (int)k0);
-    const uint32_t w10 = (uint32_t)__shfl((int)q.w0, (int)k1), w11 = (uint32_t)__shfl((int)q.w1, (int)k1);
+    const uint32_t w00 = (uint32_t)__shfl((int)q.w0, (int)k0, WD), w01 = (uint32_t)__shfl((int)q.w1, (int)k0, WD);
+    const uint32_t w10 = (uint32_t)__shfl((int)q.w0, (int)k1, WD), w11 = (uint32_t)__shfl((int)q.w1, (int)k1, WD);
     if (!rec) return;
-    q.b0 = leaf_range_box(W, a, j);
-    q.b1 = leaf_range_box(W, j, b);
+    q.b0 = leaf_range_box(W, lb, a, j);
+    q.b1 = leaf_range_box(W, lb, j, b);
     BinRec c0 = q, c1 = q;
     if (!(q.w0 & 0x80000000u)) {
         if (runL) {
             bool sw;
-            c0 = run_node(W, off, a, j - a, 0, 0, rootL, baseL, sw);
+            c0 = run_node(W, lb, off, a, j - a, 0, 0, rootL, baseL, sw);
         } else {
-            c0.b0 = leaf_range_box(W, a0, k0);
-            c0.b1 = leaf_range_box(W, k0, b0);
+            c0.b0 = leaf_range_box(W, lb, a0, k0);
+            c0.b1 = leaf_range_box(W, lb, k0, b0);
             c0.w0 = w00;
             c0.w1 = w01;
             c0.axis = (uint32_t)((31 - d0) % 3);
@@ -1726,10 +1741,10 @@ WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ code
     if (!(q.w1 & 0x80000000u)) {
         if (runR) {
             bool sw;
-            c1 = run_node(W, off, j, b - j, 0, 0, rootR, baseR, sw);
+            c1 = run_node(W, lb, off, j, b - j, 0, 0, rootR, baseR, sw);
         } else {
-            c1.b0 = leaf_range_box(W, a1, k1);
-            c1.b1 = leaf_range_box(W, k1, b1);
+            c1.b0 = leaf_range_box(W, lb, a1, k1);
+            c1.b1 = leaf_range_box(W, lb, k1, b1);
             c1.w0 = w10;
             c1.w1 = w11;
             c1.axis = (uint32_t)((31 - d1) % 3);
@@ -1737,15 +1752,16 @@ WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ code
     }
     write_rec(s, r, q.b0, q.b1, q.w0, q.w1, q.axis);
     put_rec4(s, r, q, c0, c1);
-    if (runL) leaf_run_records(s, W, off, a, j - a, rootL, baseL);
-    if (runR) leaf_run_records(s, W, off, j, b - j, rootR, baseR);
+    if (runL) leaf_run_records(s, W, lb, off, a, j - a, rootL, baseL);
+    if (runR) leaf_run_records(s, W, lb, off, j, b - j, rootR, baseR);
 }
 
-// Reference leaves of 5 .. BIG_LEAF shapes, one wave each: every wave scans
-// LEAF_CHUNK consecutive nodes (grid-stride) and builds their leaves in turn.
+// Reference leaves of 5 .. BIG_LEAF shapes: every wave scans LEAF_CHUNK
+// consecutive nodes (grid-stride); leaves of at most 32 shapes go two per wave,
+// larger ones one per wave.
 constexpr uint32_t LEAF_WAVES = 4;
 constexpr uint32_t LEAF_BLOCKS = 2048;
-constexpr uint32_t LEAF_CHUNK = 16;
+constexpr uint32_t LEAF_CHUNK = 32;
 WPBR_D void subtree_leaf_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t bid, uint32_t nblk) {
     __shared__ LeafLds W[LEAF_WAVES];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1753,11 +1769,24 @@ WPBR_D void subtree_leaf_body(const LbvhScratch& s, const uint32_t* __restrict__
     for (uint32_t c0 = (bid * LEAF_WAVES + wave) * LEAF_CHUNK; c0 < nn; c0 += nblk * LEAF_WAVES * LEAF_CHUNK) {
         const uint32_t i = c0 + lane;
         const uint32_t ns = (lane < LEAF_CHUNK && i < nn) ? s.node_nshapes[i] : 0u;
-        uint64_t m = ballot64(ns > RANGE_MAX && ns <= BIG_LEAF);
-        while (m) {
-            const uint32_t k = (uint32_t)(__ffsll((unsigned long long)m) - 1);
-            m &= m - 1;
-            subtree_leaf(s, codes, W[wave], c0 + k, nn);
+        uint64_t m2 = ballot64(ns > RANGE_MAX && ns <= 32);
+        uint64_t m1 = ballot64(ns > 32 && ns <= BIG_LEAF);
+        while (m2) {
+            const uint32_t ka = (uint32_t)(__ffsll((unsigned long long)m2) - 1);
+            m2 &= m2 - 1;
+            uint32_t kb = NO_LEAF;
+            if (m2) {
+                kb = (uint32_t)(__ffsll((unsigned long long)m2) - 1);
+                m2 &= m2 - 1;
+            }
+            const uint32_t mine = lane < 32 ? c0 + ka : (kb == NO_LEAF ? NO_LEAF : c0 + kb);
+            subtree_leaf<32>(s, codes, W[wave], mine, nn);
+            __builtin_amdgcn_wave_barrier();
+        }
+        while (m1) {
+            const uint32_t k = (uint32_t)(__ffsll((unsigned long long)m1) - 1);
+            m1 &= m1 - 1;
+            subtree_leaf<64>(s, codes, W[wave], c0 + k, nn);
             __builtin_amdgcn_wave_barrier();
         }
     }
@@ -1958,31 +1987,34 @@ WPBR_D uint32_t work_scene(const uint32_t* work, uint32_t n_scenes, uint32_t g) 
     return lo;
 }
 
-// The bin of the t-th occupied bin.
-WPBR_D uint32_t presence_select(const uint32_t* presence, uint32_t t) {
-    uint32_t w = 0;
-    for (; w < 127; w++) {
-        uint32_t c = __popc(presence[w]);
-        if (t < c) break;
-        t -= c;
-    }
-    uint32_t m = presence[w];
-    for (uint32_t k = 0; k < t; k++) m &= m - 1;
-    return 32 * w + (uint32_t)(__ffs(m) - 1);
-}
-
 constexpr uint32_t SEG_EMIT_BLOCKS = 8192;
 __global__ __launch_bounds__(EMIT_BLOCK) void ks_emit(LbvhBatch B) {
-    __shared__ uint32_t s_sc, s_bin;
     const uint32_t total = B.work[B.n_scenes];
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        if (threadIdx.x == 0) {
-            const uint32_t sc = work_scene(B.work, B.n_scenes, g);
-            s_sc = sc;
-            s_bin = presence_select(B.s[sc].presence, g - B.work[sc]);
+        // the (g - work[sc])-th occupied bin of scene sc, two presence words per lane
+        const uint32_t sc = __builtin_amdgcn_readfirstlane(work_scene(B.work, B.n_scenes, g));
+        const uint32_t t = g - B.work[sc];
+        const uint32_t* pr = B.s[sc].presence;
+        const uint32_t q0 = 2 * threadIdx.x;
+        const uint32_t p0 = pr[q0], p1 = pr[q0 + 1];
+        const uint32_t c = __popc(p0) + __popc(p1);
+        uint32_t incl = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t x = (uint32_t)__shfl_up((int)incl, d);
+            if (threadIdx.x >= (uint32_t)d) incl += x;
         }
-        __syncthreads();
-        const uint32_t sc = __builtin_amdgcn_readfirstlane(s_sc), bin = __builtin_amdgcn_readfirstlane(s_bin);
+        const bool mine = incl > t && incl - c <= t;
+        uint32_t bin_lane = 0;
+        if (mine) {
+            uint32_t u = t - (incl - c);
+            uint32_t w = p0, q = q0;
+            if (u >= (uint32_t)__popc(p0)) { u -= __popc(p0); w = p1; q = q0 + 1; }
+            for (uint32_t k = 0; k < u; k++) w &= w - 1;
+            bin_lane = 32 * q + (uint32_t)(__ffs(w) - 1);
+        }
+        const uint64_t who = ballot64(mine);
+        const uint32_t bin = __builtin_amdgcn_readfirstlane(
+            (uint32_t)__shfl((int)bin_lane, (int)(__ffsll((unsigned long long)who) - 1)));
         const LbvhScratch s = B.s[sc];
         emit_body(s, s.keys_a, B.n[sc], bin);
         __syncthreads();
