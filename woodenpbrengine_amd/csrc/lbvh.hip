@@ -601,7 +601,9 @@ constexpr int SAH_WAVES = SAH_BLOCK / 64;
 constexpr int NB = 12;
 constexpr uint32_t LANE_TASK_MAX = 32;
 constexpr int LANE_STACK = 6;               // smaller child first: depth <= log2(32) + 1
-constexpr uint32_t BLOCK_TASK_MIN = 1024;   // ranges this large use the whole workgroup
+constexpr uint32_t BLOCK_TASK_MIN = 4096;   // ranges this large use the whole workgroup: smaller ones
+                                            // run one per wave, in parallel (the LDS atomics of the
+                                            // bucket statistics bound both ways alike)
 constexpr uint32_t SAH_CUT = 256;           // sah_top defers ranges below this size
 constexpr uint32_t LANE_TASK_MAX_SUB = 8;   // lane tasks of sah_sub
 constexpr uint32_t SAH_SUB_TASKS = 2 * SAH_CUT;  // a deferred range makes fewer tasks than this
