@@ -200,6 +200,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run one plain 512 MiB device copy (reference kernel for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE scale factor)")
+    ap.add_argument("--lbvh-only", action="store_true",
+                    help="only the LBVH builds (1 048 576 triangles, then the 64-scene segmented "
+                         "build), for kernel profiles")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU (--gpus 1); sharding the render over several GPUs "
@@ -216,6 +219,27 @@ def main():
         print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
         return
     from woodenpbrengine_amd import lib, scenes
+
+    if args.lbvh_only:
+        # the LBVH builds alone (for kernel profiles): the 1 048 576-triangle build
+        # (k_* kernels) and the 64-scene segmented build (ks_* kernels)
+        ctx = lib.Context(0)
+        tris, _ = scenes.microbench()
+        sc = lib.Scene(ctx, np.zeros((0, 24), np.float32), np.zeros(0, np.float32), tris)
+        bvh = lib.Lbvh(ctx)
+        bvh.build(sc)
+        one = [bvh.build(sc) for _ in range(max(args.steps, 3))]
+        ss = scenes.c3_scenes()
+        geo = [lib.Scene(ctx, s.sphere_xform, s.sphere_radius, s.tri_verts) for s in ss]
+        bv = [lib.Lbvh(ctx) for _ in ss]
+        lib.build_segmented(ctx, geo, bv)
+        seg = [lib.build_segmented(ctx, geo, bv) for _ in range(max(args.steps, 3))]
+        print(json.dumps({"lbvh_build_ms": float(np.median(one)), "prims": len(tris),
+                          "kernel_ms": [round(x, 4) for x in bvh.kernel_times()],
+                          "segmented_build_ms": float(np.median(seg)),
+                          "segmented_shapes": int(sum(s.n_shapes for s in ss))}), flush=True)
+        ctx.close()
+        return
 
     t0 = time.time()
     if args.workload == "render":
