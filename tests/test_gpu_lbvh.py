@@ -38,12 +38,20 @@ def _case(name):
         base = scenes.random_triangles(50, 18, median_edge=0.05)
         tris = np.repeat(base, 80, axis=0)
         sx, sr = _spheres(0, 19)
+    elif name == "runs":
+        # groups of 1..60 coincident triangles among unique ones: leaves of 33..64
+        # equal codes, and runs of equal codes inside leaves whose codes differ
+        g = np.random.default_rng(31)
+        base = scenes.random_triangles(300, 32, median_edge=0.05)
+        tris = np.concatenate([np.repeat(base, g.integers(1, 61, len(base)), axis=0),
+                               scenes.random_triangles(2000, 33, median_edge=0.02)])
+        sx, sr = _spheres(2, 34)
     else:
         raise KeyError(name)
     return sx, sr, tris
 
 
-CASES = ["tiny", "small", "medium", "icosphere", "clustered"]
+CASES = ["tiny", "small", "medium", "icosphere", "clustered", "runs"]
 # (block, LDS stack depth, mapping, node width, refill, chunk); stack depth 8
 # forces spills; mapping 3 is the persistent dynamic-fetch kernel
 TRACE_CONFIGS = [(128, 32, 1, 2, 16, 256), (64, 16, 1, 4, 16, 256), (256, 8, 2, 4, 16, 256),
