@@ -41,7 +41,7 @@ struct Carver {
 static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     Carver c{base};
     const size_t N = std::max<uint32_t>(n, 1);
-    const size_t tiles = (N + 3839) / 3840;
+    const size_t tiles = (N + LBVH_SORT_TILE - 1) / LBVH_SORT_TILE;
     s.slot_bounds = c.take<float>(6 * N);
     s.centroid = c.take<float>(3 * N);
     s.cb_ord = c.take<uint32_t>(8 + 6 * 512);

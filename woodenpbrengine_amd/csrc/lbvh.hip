@@ -34,8 +34,9 @@
 namespace wpbr {
 
 constexpr int SORT_BLOCK = 256;
-constexpr int SORT_ITEMS = 15;
+constexpr int SORT_ITEMS = LBVH_SORT_ITEMS;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+static_assert(SORT_TILE == LBVH_SORT_TILE, "sort tile size");
 constexpr int SORT_WAVES = SORT_BLOCK / 64;
 constexpr uint32_t FLAG_AGG = 1u << 30;
 constexpr uint32_t FLAG_INC = 2u << 30;

@@ -27,6 +27,13 @@ struct GeomView {
 };
 
 // Device buffers of one LBVH build (all pointers device memory).
+// Radix sort tile: 256 threads x LBVH_SORT_ITEMS keys (one look-back status row
+// of 256 digits per tile).
+#ifndef LBVH_SORT_ITEMS
+#define LBVH_SORT_ITEMS 20
+#endif
+constexpr uint32_t LBVH_SORT_TILE = 256 * LBVH_SORT_ITEMS;
+
 struct LbvhScratch {
     // a1-a3
     float* slot_bounds;   // n x 6 (slot order: bounds of shape vals_a[p] at p)
