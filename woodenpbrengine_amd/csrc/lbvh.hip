@@ -443,7 +443,10 @@ WPBR_D void set_prim_leaf(const LbvhScratch& s, uint32_t slot, uint32_t leaf) {
 constexpr int EMIT_BLOCK = 64;  // one wave: a treelet level rarely has more nodes
 constexpr int EMIT_MAX_LEVELS = 64;
 
+constexpr uint32_t EMIT_LDS_CODES = 1024;  // a treelet's codes are staged in LDS up to this size
+
 WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t bin) {
+    __shared__ uint32_t s_codes[EMIT_LDS_CODES];
     __shared__ uint32_t s_q;
     __shared__ uint32_t s_lvl[EMIT_MAX_LEVELS + 2];
     __shared__ int s_nlvl;
@@ -485,6 +488,13 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
     }
     __syncthreads();
     const uint32_t base = 2 * s_start - s_tidx;
+    // the treelet's codes in LDS when they fit (the split searches read them)
+    const uint32_t tstart = s_start, tcnt = s_cnt;
+    const bool staged = tcnt <= EMIT_LDS_CODES;
+    if (staged)
+        for (uint32_t k = tid; k < tcnt; k += EMIT_BLOCK) s_codes[k] = codes[tstart + k];
+    __syncthreads();
+    auto code = [&](uint32_t i) -> uint32_t { return staged ? s_codes[i - tstart] : codes[i]; };
     // Breadth-first construction.
     for (int L = 0; L < EMIT_MAX_LEVELS; L++) {
         uint32_t lb = s_lvl[L], le = s_lvl[L + 1];
@@ -495,7 +505,7 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
             bool leaf = (bit < 0) || (rn < (uint32_t)MAX_PRIMS_IN_LEAF);
             uint32_t x = 0;
             if (!leaf) {
-                x = codes[rs] ^ codes[rs + rn - 1];
+                x = code(rs) ^ code(rs + rn - 1);
                 x &= (bit >= 31) ? 0xFFFFFFFFu : ((2u << bit) - 1u);
                 if (x == 0) leaf = true;  // all remaining bits equal: recursion reaches bit -1
             }
@@ -507,7 +517,7 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
                 uint32_t ss = rs, e = rs + rn - 1;
                 while (ss + 1 != e) {  // SLBVH.cpp:383-391
                     uint32_t mid = (ss + e) / 2;
-                    if ((codes[ss] & mask) == (codes[mid] & mask)) ss = mid;
+                    if ((code(ss) & mask) == (code(mid) & mask)) ss = mid;
                     else e = mid;
                 }
                 uint32_t split = e;
@@ -1250,7 +1260,10 @@ WPBR_D void sah_sub_body(LbvhScratch& s, uint32_t bid, uint32_t nblk) {
 // Preorder layout: P = exclusive prefix of treelet node counts over final
 // positions (PER consecutive positions per thread), in aux[0..k]; then every
 // treelet's first node and every upper node at their closed-form offsets.
-WPBR_D void sah_final_body(LbvhScratch& s) {
+// Every workgroup computes the prefix sums; workgroup bid of nblk writes every
+// nblk-th slice of the treelets and upper nodes.
+constexpr uint32_t SAH_FINAL_BLOCKS = 16;
+WPBR_D void sah_final_body(LbvhScratch& s, uint32_t bid, uint32_t nblk) {
     __shared__ SahLds L;
     const int tid = threadIdx.x, lane = tid & 63;
     const uint32_t k = sah_treelets(s, L);
@@ -1294,10 +1307,10 @@ WPBR_D void sah_final_body(LbvhScratch& s) {
     }
     __threadfence_block();
     __syncthreads();
-    for (uint32_t i = tid; i < k; i += SAH_BLOCK)
+    for (uint32_t i = bid * SAH_BLOCK + tid; i < k; i += nblk * SAH_BLOCK)
         s.t_final[L.perm[i]] = (uint32_t)L.la[i] + (i - (uint32_t)L.lb[i]) + L.aux[i];
-#pragma unroll 4
-    for (uint32_t u = tid; u < num_upper; u += SAH_BLOCK) {
+#pragma unroll 2
+    for (uint32_t u = bid * SAH_BLOCK + tid; u < num_upper; u += nblk * SAH_BLOCK) {
         const uint32_t i0 = s.u_i0[u], mid = s.u_mid[u];
         const uint32_t off = s.u_dep[u] + (i0 - s.u_rt[u]) + L.aux[i0];
         const uint32_t size0 = (mid - 1 - i0) + (L.aux[mid] - L.aux[i0]);
@@ -1306,7 +1319,7 @@ WPBR_D void sah_final_body(LbvhScratch& s) {
         s.node_axis[off] = (uint8_t)s.u_axis[u];
         store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
     }
-    if (tid == 0) {
+    if (tid == 0 && bid == 0) {
         *s.n_nodes = num_upper + L.aux[k];
         *s.n_treelets = k;
     }
@@ -1314,7 +1327,7 @@ WPBR_D void sah_final_body(LbvhScratch& s) {
 
 __global__ __launch_bounds__(SAH_BLOCK) void k_sah_top(LbvhScratch s) { sah_top_body(s); }
 __global__ __launch_bounds__(SAH_BLOCK) void k_sah_sub(LbvhScratch s) { sah_sub_body(s, blockIdx.x, gridDim.x); }
-__global__ __launch_bounds__(SAH_BLOCK) void k_sah_final(LbvhScratch s) { sah_final_body(s); }
+__global__ __launch_bounds__(SAH_BLOCK) void k_sah_final(LbvhScratch s) { sah_final_body(s, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 // Flatten: treelet records to final preorder slots.  One workgroup per treelet.
@@ -2063,9 +2076,10 @@ __global__ __launch_bounds__(SAH_BLOCK) void ks_sah_sub(LbvhBatch B) {
         sah_sub_one(s, S, g - B.subwork[sc]);
     }
 }
+constexpr uint32_t SEG_SAH_FINAL_BLOCKS = 4;  // per scene
 __global__ __launch_bounds__(SAH_BLOCK) void ks_sah_final(LbvhBatch B) {
-    LbvhScratch s = B.s[blockIdx.x];
-    sah_final_body(s);
+    LbvhScratch s = B.s[blockIdx.x / SEG_SAH_FINAL_BLOCKS];
+    sah_final_body(s, blockIdx.x % SEG_SAH_FINAL_BLOCKS, SEG_SAH_FINAL_BLOCKS);
 }
 
 __global__ __launch_bounds__(256) void ks_scatter(LbvhBatch B) {
@@ -2143,7 +2157,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipS
     hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(ks_sah_sub, dim3(SAH_SEG_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
-    hipLaunchKernelGGL(ks_sah_final, dim3(S), dim3(SAH_BLOCK), 0, st, B);
+    hipLaunchKernelGGL(ks_sah_final, dim3(S * SEG_SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
     hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_subtree, dim3(grid[SEG_SUBTREE]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_subtree_leaf, dim3(grid[SEG_SUBTREE_LEAF]), dim3(64 * LEAF_WAVES), 0, st, B);
@@ -2204,7 +2218,7 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
         case 5: {
             hipLaunchKernelGGL(k_sah_top, dim3(1), dim3(SAH_BLOCK), 0, st, s);
             hipLaunchKernelGGL(k_sah_sub, dim3(SAH_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
-            hipLaunchKernelGGL(k_sah_final, dim3(1), dim3(SAH_BLOCK), 0, st, s);
+            hipLaunchKernelGGL(k_sah_final, dim3(SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
             rec(9);
             hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
             rec(10);
