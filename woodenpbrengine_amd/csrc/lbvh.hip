@@ -1,8 +1,8 @@
 // LBVH build for gfx950 (MEngine::buildLBVH, MEngine.cpp:341-350).
 //
 // Kernel chain (one stream, no host round trip until the final node count):
-//   k_init          zero histograms, tile counters, look-back status, treelet bits
-//   k_bounds        a1: per-shape bounds + centroid (CSTriangle.cpp:116-142,
+//   k_init_bounds   zero histograms, tile counters, look-back status, treelet bits;
+//                   a1: per-shape bounds + centroid (CSTriangle.cpp:116-142,
 //                   CSSphere.h:136-152); a2: centroid bounds by wave reduction and
 //                   order-preserving-uint atomics (SLBVH.cpp:164-181)
 //   k_morton        a3: 30-bit Morton codes (SLBVH.cpp:186-227) + the 4x256 digit
@@ -11,8 +11,8 @@
 //                   decoupled look-back, wave-ballot ranking, LDS-staged scatter.
 //                   A stable sort of (code, index) has a unique result, equal to
 //                   the reference's 5 x 6-bit passes (SLBVH.cpp:245-283)
-//   k_treelet_mark  a4: treelet = run of equal top-12 bits (SLBVH.cpp:299-319)
-//   k_slot_prims    a4: shape bounds gathered into slot order (SLBVH.cpp:231-324),
+//   k_slot_prims    a4: treelet = run of equal top-12 bits (SLBVH.cpp:299-319);
+//                   shape bounds gathered into slot order (SLBVH.cpp:231-324),
 //                   with the traversal layout's primitive records
 //   k_emit          a5: one workgroup per treelet builds the emitLBVH tree
 //                   (SLBVH.cpp:353-406) breadth-first, then subtree sizes and
@@ -56,7 +56,6 @@ WPBR_D void init_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
         *s.err = 0; *s.upper_count = 0; *s.big_count = 0; *s.sub_n = 0;
     }
 }
-__global__ void k_init(LbvhScratch s) { init_body(s, blockIdx.x, gridDim.x); }
 
 // ---------------------------------------------------------------------------
 WPBR_D void shape_box(const GeomView& g, uint32_t i, bbox& b) {
@@ -125,6 +124,13 @@ WPBR_D void bounds_body(const GeomView& g, uint32_t n, float* bounds, float* cen
 __global__ __launch_bounds__(256) void k_bounds(GeomView g, uint32_t n, float* bounds, float* centroid,
                                                 uint32_t* cb_ord) {
     bounds_body(g, n, bounds, centroid, cb_ord, blockIdx.x, gridDim.x);
+}
+
+// The build's first kernel: zero the build's counters, histograms, look-back
+// status and treelet bits (grid-stride), then bounds and centroid partials.
+__global__ __launch_bounds__(256) void k_init_bounds(LbvhScratch s, GeomView g, uint32_t n) {
+    init_body(s, blockIdx.x, gridDim.x);
+    bounds_body(g, n, nullptr, s.centroid, s.cb_ord, blockIdx.x, gridDim.x);
 }
 
 // Reduce the k_bounds partials (called by every thread of a consumer block;
@@ -362,13 +368,10 @@ WPBR_D void treelet_mark_body(const uint32_t* __restrict__ sorted, uint32_t n, u
         atomicOr(&presence[top >> 5], 1u << (top & 31));
     }
 }
-__global__ void k_treelet_mark(const uint32_t* __restrict__ sorted, uint32_t n, uint32_t* bin_start,
-                               uint32_t* presence) {
-    treelet_mark_body(sorted, n, bin_start, presence, blockIdx.x);
-}
 
-// Shape bounds in slot (sorted) order, as the reference's sort job gathers
-// them with the entities (SLBVH.cpp:231-324), so that the treelet and leaf
+// Treelet starts (the first slot of each run of equal top-12 bits), and shape
+// bounds in slot (sorted) order, as the reference's sort job gathers them with
+// the entities (SLBVH.cpp:231-324), so that the treelet and leaf
 // kernels read contiguous boxes; and the primitive records of the traversal
 // layout at their slots: v0.xyz + shape id, v1.xyz + slot, v2.xyz + first slot
 // of the reference leaf (written by the leaf kernels).  Spheres keep only the
@@ -377,6 +380,7 @@ __global__ void k_treelet_mark(const uint32_t* __restrict__ sorted, uint32_t n, 
 WPBR_D void slot_prims_body(const LbvhScratch& s, const GeomView& g, uint32_t n, uint32_t bid) {
     const uint32_t p = bid * blockDim.x + threadIdx.x;
     if (p >= n) return;
+    treelet_mark_body(s.keys_a, n, s.bin_start, s.presence, bid);
     const uint32_t id = s.vals_a[p];
     float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bbox b;
@@ -1922,15 +1926,11 @@ wpbr_status shape_bounds_launch(const GeomView& g, float* bounds, float* centroi
     const uint32_t nblk = (map).count[sc];               \
     (void)nblk;
 
-__global__ void ks_init(LbvhBatch B) {
-    SEG_BLOCK(B.map[SEG_INIT]);
-    const LbvhScratch s = B.s[sc];
-    init_body(s, bid, nblk);
-}
 
 __global__ __launch_bounds__(256) void ks_bounds(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_BOUNDS]);
     const LbvhScratch s = B.s[sc];
+    init_body(s, bid, nblk);
     const GeomView g = B.g[sc];
     bounds_body(g, B.n[sc], nullptr, s.centroid, s.cb_ord, bid, nblk);
 }
@@ -1955,11 +1955,6 @@ __global__ __launch_bounds__(SORT_BLOCK) void ks_onesweep(LbvhBatch B, int p) {
     onesweep_body(q.kin, q.vin, q.kout, q.vout, B.n[sc], p, s.hist, q.status, q.status_next, s.tile_counter, s.err);
 }
 
-__global__ void ks_treelet_mark(LbvhBatch B) {
-    SEG_BLOCK(B.map[SEG_MARK]);
-    const LbvhScratch s = B.s[sc];
-    treelet_mark_body(s.keys_a, B.n[sc], s.bin_start, s.presence, bid);
-}
 
 __global__ __launch_bounds__(256) void ks_slot_prims(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_MARK]);
@@ -2130,7 +2125,7 @@ __global__ void ks_info(LbvhBatch B) {
 
 // Blocks per scene of each segmented launch: the single-scene grids.
 void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts) {
-    counts[SEG_INIT] = std::max<uint32_t>(1, ((uint32_t)status_words + 255) / 256);
+    (void)status_words;
     counts[SEG_BOUNDS] = std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256);
     counts[SEG_MORTON] = (n + MORTON_BLOCK * MORTON_ITEMS - 1) / (MORTON_BLOCK * MORTON_ITEMS);
     counts[SEG_SORT] = (n + SORT_TILE - 1) / SORT_TILE;
@@ -2144,13 +2139,11 @@ void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts) {
 
 wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st) {
     const uint32_t S = B.n_scenes;
-    hipLaunchKernelGGL(ks_init, dim3(grid[SEG_INIT]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_bounds, dim3(grid[SEG_BOUNDS]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_cb_final, dim3(S), dim3(64), 0, st, B);
     hipLaunchKernelGGL(ks_morton, dim3(grid[SEG_MORTON]), dim3(MORTON_BLOCK), 0, st, B);
     for (int p = 0; p < 4; p++)
         hipLaunchKernelGGL(ks_onesweep, dim3(grid[SEG_SORT]), dim3(SORT_BLOCK), 0, st, B, p);
-    hipLaunchKernelGGL(ks_treelet_mark, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_slot_prims, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
@@ -2183,10 +2176,8 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
     switch (stage) {
         case 0:
             rec(0);
-            hipLaunchKernelGGL(k_init, dim3(std::max<uint32_t>(1, (s.status_words + 255) / 256)), dim3(256), 0,
-                               st, s);
-            hipLaunchKernelGGL(k_bounds, dim3(std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256)), dim3(256), 0,
-                               st, g, n, nullptr, s.centroid, s.cb_ord);
+            hipLaunchKernelGGL(k_init_bounds, dim3(std::min<uint32_t>(BOUNDS_BLOCKS, (n + 255) / 256)), dim3(256), 0,
+                               st, s, g, n);
             rec(1);
             break;
         case 1:
@@ -2205,8 +2196,6 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
                                    q.vout, n, p, s.hist, q.status, q.status_next, s.tile_counter, s.err);
                 rec(3 + p);
             }
-            hipLaunchKernelGGL(k_treelet_mark, dim3((n + 255) / 256), dim3(256), 0, st, s.keys_a, n, s.bin_start,
-                               s.presence);
             hipLaunchKernelGGL(k_slot_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, n);
             rec(7);
             break;

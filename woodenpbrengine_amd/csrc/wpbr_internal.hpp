@@ -98,7 +98,7 @@ __host__ __device__ inline SortPass sort_pass(const LbvhScratch& s, int p) {
 
 // Segmented build of several scenes (one launch per kernel for all of them).
 // Block b of a launch of kind k works for scene map[k].scene[b].
-enum SegKind { SEG_INIT, SEG_BOUNDS, SEG_MORTON, SEG_SORT, SEG_MARK, SEG_SUBTREE, SEG_SUBTREE_LEAF, SEG_SUBTREE_BIG, SEG_BVH4,
+enum SegKind { SEG_BOUNDS, SEG_MORTON, SEG_SORT, SEG_MARK, SEG_SUBTREE, SEG_SUBTREE_LEAF, SEG_SUBTREE_BIG, SEG_BVH4,
                SEG_KINDS };
 struct SegMap {
     const uint16_t* scene;  // per block
