@@ -136,8 +136,12 @@ namespace wpbr {
 
 void ctx_release_render(wpbr_ctx* ctx) {
     if (!ctx->render) return;
-    if (ctx->render->mem) hipFree(ctx->render->mem);
-    for (auto& e : ctx->render->evs) hipEventDestroy(e);
+    wpbr_render_state* r = ctx->render;
+    if (r->mem) (void)hipFree(r->mem);
+    if (r->spill2) (void)hipFree(r->spill2);
+    if (r->claim2) (void)hipFree(r->claim2);
+    if (r->stream2) (void)hipStreamDestroy(r->stream2);
+    for (auto& e : ctx->render->evs) (void)hipEventDestroy(e);
     delete ctx->render;
     ctx->render = nullptr;
 }
@@ -195,10 +199,39 @@ wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n) {
     if (r->mem) { hipFree(r->mem); r->mem = nullptr; r->cap = 0; }
     WaveState tmp;
     const size_t bytes = carve_wave(nullptr, n, tmp);
-    wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, bytes), "hipMalloc render state");
+    wpbr_status s = hip_check(ctx, hipMalloc(&r->mem, 2 * bytes), "hipMalloc render state");
     if (s) return s;
     r->cap = n;
     carve_wave((char*)r->mem, n, r->ws);
+    carve_wave((char*)r->mem + bytes, n, r->ws2);
+    // one set of per-pixel sample-stream cursors: each pass continues them
+    r->ws2.c1 = r->ws.c1;
+    r->ws2.c2 = r->ws.c2;
+    return WPBR_OK;
+}
+
+// The second stream of the render loop and its traversal scratch (a spill area
+// like the context's, and claim counters, zeroed once; the kernels leave them
+// zero).
+wpbr_status ensure_stream2(wpbr_ctx* ctx) {
+    wpbr_render_state* r = ctx->render;
+    if (!r->stream2) {
+        wpbr_status s = hip_check(ctx, hipStreamCreateWithFlags(&r->stream2, hipStreamNonBlocking), "stream");
+        if (s) return s;
+    }
+    if (!r->claim2) {
+        wpbr_status s = hip_check(ctx, hipMalloc(&r->claim2, 16), "hipMalloc");
+        if (!s) s = hip_check(ctx, hipMemsetAsync(r->claim2, 0, 16, ctx->stream), "memset");
+        if (s) return s;
+    }
+    if (r->spill2_words < ctx->spill_words) {
+        if (r->spill2) (void)hipFree(r->spill2);
+        r->spill2 = nullptr;
+        r->spill2_words = 0;
+        wpbr_status s = hip_check(ctx, hipMalloc(&r->spill2, ctx->spill_words * 4), "hipMalloc spill");
+        if (s) return s;
+        r->spill2_words = ctx->spill_words;
+    }
     return WPBR_OK;
 }
 
@@ -367,16 +400,17 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     const uint32_t n = sc->sv.width * sc->sv.height;
     wpbr_status s = ensure_wave(ctx, n);
     if (!s) s = ctx_ensure_trace(ctx, 2 * n);
+    if (!s) s = ensure_stream2(ctx);
     if (s) return s;
     wpbr_render_state* r = ctx->render;
     hipStream_t st = ctx->stream;
     s = hip_check(ctx, hipMemsetAsync(d_film, 0, (size_t)n * 16, st), "film clear");
     // qcount[0]: queue length of the current pass; qcount[2..3]: secondary rays traced
     if (!s) s = hip_check(ctx, hipMemsetAsync(r->ws.qcount, 0, 64, st), "queue counters");
+    if (!s) s = hip_check(ctx, hipMemsetAsync(r->ws2.qcount, 0, 64, st), "queue counters");
     if (s) return s;
     RenderLaunch L;
     L.sv = sc->sv;
-    L.ws = r->ws;
     L.n = n;
     L.spp = sc->spp;
     L.seed = sc->seed;
@@ -391,76 +425,99 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     ta.prims = (const float4*)b->s.prims;
     ta.root_box = b->s.node_bounds;
     ta.root_word = b->root_word;
-    ta.spill = ctx->spill;
     ta.spill_depth = ctx->spill_depth;
     ta.err = ctx->err_flag;
     ta.n_dev = nullptr;
     ta.stride = 0;
     const uint32_t dyn_blocks = ctx_trace_shape(ctx, ta);
-    // events: start, then per pass 5 boundaries (camera | trace | shade1 | trace | shade2)
-    const size_t nev = 2 + 5 * (size_t)sc->spp;
+    // Pass j runs on stream j % 2 with wavefront state j % 2 (the per-pixel
+    // sample-stream cursors are shared): its camera waits for pass j - 1's shade1
+    // (the last to advance the cursors), so pass j's camera, primary trace and
+    // shading overlap pass j - 1's secondary trace and splat on the other
+    // stream; its splat waits for pass j - 1's splat, so the film accumulates
+    // the passes in order.
+    hipStream_t sts[2] = {st, r->stream2};
+    const WaveState* wss[2] = {&r->ws, &r->ws2};
+    uint32_t* spills[2] = {ctx->spill, r->spill2};
+    uint32_t* claims[2] = {ta.claim, r->claim2};
+    // events: start, then per pass 6 (pass start | camera | trace | shade1 | trace | shade2)
+    const size_t nev = 1 + 6 * (size_t)sc->spp;
     while (r->evs.size() < nev) {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return set_err(ctx, WPBR_ERR_HIP, "hipEventCreate");
         r->evs.push_back(e);
     }
     hipEventRecord(r->evs[0], st);
+    hipStreamWaitEvent(sts[1], r->evs[0], 0);
     for (uint32_t j = 0; j < sc->spp && !s; j++) {
-        hipEvent_t* e = &r->evs[1 + 5 * (size_t)j];
+        hipEvent_t* e = &r->evs[1 + 6 * (size_t)j];
+        const int k = (int)(j & 1u);
+        hipStream_t sj = sts[k];
+        const WaveState& w = *wss[k];
+        L.ws = w;
         L.spp_index = j;
-        s = render_pass_launch(L, 0, st);
-        hipEventRecord(e[0], st);
+        ta.spill = spills[k];
+        ta.claim = claims[k];
+        if (j) hipStreamWaitEvent(sj, r->evs[6 * (size_t)j - 2], 0);  // pass j - 1's shade1
+        hipEventRecord(e[0], sj);
+        s = render_pass_launch(L, 0, sj);
+        hipEventRecord(e[1], sj);
         ta.n = n;
         ta.n_dev = nullptr;
         ta.stride = 0;
-        ta.rays = r->ws.rays;
-        ta.hit_shape = r->ws.hit;
-        ta.hit_t = r->ws.hit_t;
-        ta.bary = r->ws.bary;
+        ta.rays = w.rays;
+        ta.hit_shape = w.hit;
+        ta.hit_t = w.hit_t;
+        ta.bary = w.bary;
         ta.target = nullptr;
-        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
-        hipEventRecord(e[1], st);
-        if (!s) s = render_pass_launch(L, 1, st);
-        hipEventRecord(e[2], st);
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, sj, dyn_blocks);
+        hipEventRecord(e[2], sj);
+        if (!s) s = render_pass_launch(L, 1, sj);
+        hipEventRecord(e[3], sj);
         // secondary rays: the queue holds *qcount rays with SoA stride 2n
         ta.n = 2 * n;
-        ta.n_dev = r->ws.qcount;
+        ta.n_dev = w.qcount;
         ta.stride = 2 * n;
-        ta.rays = r->ws.qrays;
-        ta.hit_shape = r->ws.qhit;
-        ta.hit_t = r->ws.qt;
+        ta.rays = w.qrays;
+        ta.hit_shape = w.qhit;
+        ta.hit_t = w.qt;
         ta.bary = nullptr;
-        ta.target = r->ws.qtarget;
-        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
-        hipEventRecord(e[3], st);
-        if (!s) s = render_pass_launch(L, 2, st);
-        hipEventRecord(e[4], st);
+        ta.target = w.qtarget;
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, sj, dyn_blocks);
+        hipEventRecord(e[4], sj);
+        if (j) hipStreamWaitEvent(sj, r->evs[6 * (size_t)j], 0);  // pass j - 1's splat
+        if (!s) s = render_pass_launch(L, 2, sj);
+        hipEventRecord(e[5], sj);
     }
+    const hipEvent_t last = r->evs[6 * (size_t)sc->spp];
+    if (sc->spp % 2 == 0) hipStreamWaitEvent(st, last, 0);
     if (s) {
         return set_err(ctx, s, std::string("render launch failed: ") +
                                    hipGetErrorString(last_launch_error));
     }
-    unsigned long long qtotal = 0;
-    s = hip_check(ctx, hipMemcpyAsync(&qtotal, r->ws.qcount + 2, 8, hipMemcpyDeviceToHost, st), "d2h");
+    unsigned long long q2[2] = {0, 0};
+    s = hip_check(ctx, hipMemcpyAsync(&q2[0], r->ws.qcount + 2, 8, hipMemcpyDeviceToHost, st), "d2h");
+    if (!s) s = hip_check(ctx, hipMemcpyAsync(&q2[1], r->ws2.qcount + 2, 8, hipMemcpyDeviceToHost, st), "d2h");
     if (!s) s = ctx_finish_trace(ctx);
     if (s) return s;
+    const unsigned long long qtotal = q2[0] + q2[1];
+    // per-pass stage times, each on its own stream (the two streams overlap)
     double trace_ms = 0, shade_ms = 0, cam_ms = 0, prim_ms = 0;
     for (uint32_t j = 0; j < sc->spp; j++) {
-        hipEvent_t* e = &r->evs[1 + 5 * (size_t)j];
-        hipEvent_t prev = j ? r->evs[5 * (size_t)j] : r->evs[0];
+        hipEvent_t* e = &r->evs[1 + 6 * (size_t)j];
         float t0, t1, t2, t3, t4;
-        hipEventElapsedTime(&t0, prev, e[0]);
-        hipEventElapsedTime(&t1, e[0], e[1]);
-        hipEventElapsedTime(&t2, e[1], e[2]);
-        hipEventElapsedTime(&t3, e[2], e[3]);
-        hipEventElapsedTime(&t4, e[3], e[4]);
+        hipEventElapsedTime(&t0, e[0], e[1]);
+        hipEventElapsedTime(&t1, e[1], e[2]);
+        hipEventElapsedTime(&t2, e[2], e[3]);
+        hipEventElapsedTime(&t3, e[3], e[4]);
+        hipEventElapsedTime(&t4, e[4], e[5]);
         cam_ms += t0;
         trace_ms += t1 + t3;
         prim_ms += t1;
         shade_ms += t2 + t4;
     }
     float tot = 0;
-    hipEventElapsedTime(&tot, r->evs[0], r->evs[5 * (size_t)sc->spp]);
+    hipEventElapsedTime(&tot, r->evs[0], last);
     ctx->last_ms = tot;
     r->stats[0] = tot;
     r->stats[1] = trace_ms;
@@ -487,7 +544,7 @@ extern "C" wpbr_status wpbr_autotune_render(wpbr_ctx* ctx, wpbr_scene* sc, float
     if (!ctx || !sc || !d_film) return WPBR_ERR_INVALID_ARG;
     return wpbr::autotune(ctx, wpbr::trace_candidates(true), [&](float* ms) {
         wpbr_status s = wpbr_render(ctx, sc, d_film, nullptr);
-        *ms = (float)ctx->render->stats[1];
+        *ms = (float)ctx->render->stats[0];
         return s;
     }, best);
 }

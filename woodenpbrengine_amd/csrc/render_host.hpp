@@ -59,7 +59,15 @@ struct wpbr_scene {
 struct wpbr_render_state {
     void* mem = nullptr;
     uint32_t cap = 0;
-    wpbr::WaveState ws{};
+    // two wavefront states (sharing the sample-stream cursors): consecutive
+    // sample passes alternate between them (and between the context's stream
+    // and stream2), so that one pass's secondary trace and splat overlap the
+    // next pass's camera, primary trace and shading
+    wpbr::WaveState ws{}, ws2{};
+    hipStream_t stream2 = nullptr;
+    uint32_t* spill2 = nullptr;  // traversal stack overflow of stream2's launches
+    size_t spill2_words = 0;
+    uint32_t* claim2 = nullptr;  // persistent-kernel claim counters of stream2's launches
     std::vector<hipEvent_t> evs;
     double stats[8] = {0};
 };
