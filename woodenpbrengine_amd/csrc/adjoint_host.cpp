@@ -222,6 +222,7 @@ wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_f
     const uint32_t n = sc->sv.width * sc->sv.height;
     wpbr_status s = ensure_wave(ctx, n);
     if (!s) s = ctx_ensure_trace(ctx, 2 * n);
+    if (!s) s = ensure_stream2(ctx);
     if (!s && !sc->adj_ready) s = build_adjoint_tables(ctx, sc);
     if (s) return s;
     wpbr_render_state* r = ctx->render;
@@ -238,9 +239,9 @@ wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_f
     s = hip_check(ctx, hipMemsetAsync(part, 0, 4 * ((size_t)rows * ncol + 1), st), "memset");
     if (!s) s = hip_check(ctx, hipMemsetAsync(d_grad, 0, 4 * ngrad, st), "memset");
     if (!s) s = hip_check(ctx, hipMemsetAsync(r->ws.qcount, 0, 64, st), "queue counters");
+    if (!s) s = hip_check(ctx, hipMemsetAsync(r->ws2.qcount, 0, 64, st), "queue counters");
     RenderLaunch L;
     L.sv = sc->sv;
-    L.ws = r->ws;
     L.n = n;
     L.spp = sc->spp;
     L.seed = sc->seed;
@@ -255,12 +256,27 @@ wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_f
     ta.prims = (const float4*)b->s.prims;
     ta.root_box = b->s.node_bounds;
     ta.root_word = b->root_word;
-    ta.spill = ctx->spill;
     ta.spill_depth = ctx->spill_depth;
     ta.err = ctx->err_flag;
     ta.n_dev = nullptr;
     ta.stride = 0;
     const uint32_t dyn_blocks = ctx_trace_shape(ctx, ta);
+    // Passes alternate between the context's stream and stream2 with their own
+    // wavefront state (as in wpbr_render): pass j's camera waits for pass j - 1's
+    // shade1 (the sample-stream cursors), its k_adjoint for pass j - 1's (the
+    // gradient sums accumulate the passes in order).
+    hipStream_t sts[2] = {st, r->stream2};
+    const WaveState* wss[2] = {&r->ws, &r->ws2};
+    uint32_t* spills[2] = {ctx->spill, r->spill2};
+    uint32_t* claims[2] = {ta.claim, r->claim2};
+    while (r->evs.size() < 2 * (size_t)sc->spp) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+            hipFreeAsync(part, st);
+            return set_err(ctx, WPBR_ERR_HIP, "hipEventCreate");
+        }
+        r->evs.push_back(e);
+    }
     AdjArgs aa;
     aa.A = sc->adj;
     aa.film = (const float4*)d_film;
@@ -268,32 +284,44 @@ wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_f
     aa.part = part;
     aa.grad_env = d_grad + 7 * (size_t)nm;
     hipEventRecord(ctx->ev0, st);
+    hipStreamWaitEvent(sts[1], ctx->ev0, 0);
     for (uint32_t j = 0; j < sc->spp && !s; j++) {
+        const int k = (int)(j & 1u);
+        hipStream_t sj = sts[k];
+        const WaveState& w = *wss[k];
+        L.ws = w;
         L.spp_index = j;
+        ta.spill = spills[k];
+        ta.claim = claims[k];
         // path replay: the forward's camera, primary trace, shade1 and secondary trace
-        s = render_pass_launch(L, 0, st);
+        if (j) hipStreamWaitEvent(sj, r->evs[2 * (size_t)(j - 1)], 0);  // pass j - 1's shade1
+        s = render_pass_launch(L, 0, sj);
         ta.n = n;
         ta.n_dev = nullptr;
         ta.stride = 0;
-        ta.rays = r->ws.rays;
-        ta.hit_shape = r->ws.hit;
-        ta.hit_t = r->ws.hit_t;
-        ta.bary = r->ws.bary;
+        ta.rays = w.rays;
+        ta.hit_shape = w.hit;
+        ta.hit_t = w.hit_t;
+        ta.bary = w.bary;
         ta.target = nullptr;
-        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
-        if (!s) s = render_pass_launch(L, 1, st);
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, sj, dyn_blocks);
+        if (!s) s = render_pass_launch(L, 1, sj);
+        hipEventRecord(r->evs[2 * (size_t)j], sj);
         ta.n = 2 * n;
-        ta.n_dev = r->ws.qcount;
+        ta.n_dev = w.qcount;
         ta.stride = 2 * n;
-        ta.rays = r->ws.qrays;
-        ta.hit_shape = r->ws.qhit;
-        ta.hit_t = r->ws.qt;
+        ta.rays = w.qrays;
+        ta.hit_shape = w.qhit;
+        ta.hit_t = w.qt;
         ta.bary = nullptr;
-        ta.target = r->ws.qtarget;
-        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
+        ta.target = w.qtarget;
+        if (!s) s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, sj, dyn_blocks);
+        if (j) hipStreamWaitEvent(sj, r->evs[2 * (size_t)j - 1], 0);  // pass j - 1's k_adjoint
         aa.L = L;
-        if (!s) s = adjoint_launch(aa, st);
+        if (!s) s = adjoint_launch(aa, sj);
+        hipEventRecord(r->evs[2 * (size_t)j + 1], sj);
     }
+    if (sc->spp % 2 == 0) hipStreamWaitEvent(st, r->evs[2 * (size_t)sc->spp - 1], 0);
     if (!s) s = adjoint_finish(part, rows, nm, d_grad, dI, sc->adj.env_dI, ntex, st);
     hipEventRecord(ctx->ev1, st);
     if (s) {

@@ -19,6 +19,7 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
 // Carves one wave state for n pixels at p (p null: only the size); returns the bytes used.
 size_t carve_wave(char* p, uint32_t n, WaveState& w);
 wpbr_status ensure_wave(wpbr_ctx* ctx, uint32_t n);
+wpbr_status ensure_stream2(wpbr_ctx* ctx);
 
 struct HostTables {
     float xyz[3 * NSPEC];
