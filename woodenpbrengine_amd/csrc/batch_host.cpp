@@ -13,16 +13,21 @@ struct wpbr_batch {
     wpbr_ctx* ctx = nullptr;
     std::vector<wpbr_scene*> scenes;
     std::vector<float*> films;
-    std::vector<WaveState> ws;       // per scene, inside `mem`
-    std::vector<size_t> spill_off;   // per scene: first spill word
+    // per scene two wave states (sharing the sample-stream cursors) and two
+    // spill areas, inside `mem`: consecutive passes alternate between them and
+    // between the context's stream and stream2 (as wpbr_render does)
+    std::vector<WaveState> ws[2];
+    std::vector<size_t> spill_off[2];  // per scene: first spill word
+    hipStream_t stream2 = nullptr;
+    std::vector<hipEvent_t> evs;     // per pass: shade1 done, splat done
     uint32_t spp = 0;
     uint64_t pixels = 0;
     void* mem = nullptr;             // wave states + spill
     void* tables = nullptr;          // launch records and block maps
     size_t tables_cap = 0;
-    RenderLaunch* d_L = nullptr;
-    TraceArgs* d_Tp = nullptr;       // primary rays
-    TraceArgs* d_Tq = nullptr;       // secondary-ray queues
+    RenderLaunch* d_L[2] = {nullptr, nullptr};
+    TraceArgs* d_Tp[2] = {nullptr, nullptr};  // primary rays
+    TraceArgs* d_Tq[2] = {nullptr, nullptr};  // secondary-ray queues
     SegMap map_pix{}, map_tile{}, map_tp{}, map_tq{};
     uint32_t grid_pix = 0, grid_tile = 0, grid_tp = 0, grid_tq = 0;
     hipGraph_t graph = nullptr;
@@ -62,15 +67,17 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
     const uint32_t S = (uint32_t)b->scenes.size();
     const uint32_t block = (uint32_t)ctx->trace_block;
     const uint32_t depth = 256 - (uint32_t)ctx->trace_stack;
-    std::vector<RenderLaunch> L(S);
-    std::vector<TraceArgs> Tp(S), Tq(S);
+    std::vector<RenderLaunch> L(2 * S);
+    std::vector<TraceArgs> Tp(2 * S), Tq(2 * S);
     MapImage pix, tile, tp, tq;
-    for (uint32_t s = 0; s < S; s++) {
+    for (uint32_t s2 = 0; s2 < 2 * S; s2++) {
+        const uint32_t s = s2 % S, k = s2 / S;
         const wpbr_scene* sc = b->scenes[s];
         const uint32_t n = sc->sv.width * sc->sv.height;
-        RenderLaunch& l = L[s];
+        const WaveState& w = b->ws[k][s];
+        RenderLaunch& l = L[s2];
         l.sv = sc->sv;
-        l.ws = b->ws[s];
+        l.ws = w;
         l.n = n;
         l.spp_index = 0;
         l.spp = sc->spp;
@@ -88,27 +95,28 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
         a.root_box = bvh->s.node_bounds;
         a.root_word = bvh->root_word;
         a.parts = 1;
-        a.spill = (uint32_t*)((char*)b->mem + 4 * b->spill_off[s]);
+        a.spill = (uint32_t*)((char*)b->mem + 4 * b->spill_off[k][s]);
         a.spill_depth = depth;
         a.err = ctx->err_flag;
-        Tp[s] = a;
-        Tp[s].n = n;
-        Tp[s].n_dev = nullptr;
-        Tp[s].stride = 0;
-        Tp[s].rays = b->ws[s].rays;
-        Tp[s].hit_shape = b->ws[s].hit;
-        Tp[s].hit_t = b->ws[s].hit_t;
-        Tp[s].bary = b->ws[s].bary;
-        Tp[s].target = nullptr;
-        Tq[s] = a;
-        Tq[s].n = 2 * n;
-        Tq[s].n_dev = b->ws[s].qcount;
-        Tq[s].stride = 2 * n;
-        Tq[s].rays = b->ws[s].qrays;
-        Tq[s].hit_shape = b->ws[s].qhit;
-        Tq[s].hit_t = b->ws[s].qt;
-        Tq[s].bary = nullptr;
-        Tq[s].target = b->ws[s].qtarget;
+        Tp[s2] = a;
+        Tp[s2].n = n;
+        Tp[s2].n_dev = nullptr;
+        Tp[s2].stride = 0;
+        Tp[s2].rays = w.rays;
+        Tp[s2].hit_shape = w.hit;
+        Tp[s2].hit_t = w.hit_t;
+        Tp[s2].bary = w.bary;
+        Tp[s2].target = nullptr;
+        Tq[s2] = a;
+        Tq[s2].n = 2 * n;
+        Tq[s2].n_dev = w.qcount;
+        Tq[s2].stride = 2 * n;
+        Tq[s2].rays = w.qrays;
+        Tq[s2].hit_shape = w.qhit;
+        Tq[s2].hit_t = w.qt;
+        Tq[s2].bary = nullptr;
+        Tq[s2].target = w.qtarget;
+        if (k) continue;  // the block maps are the same for both states
         pix.add(s, (n + 255) / 256);
         tile.add(s, n / 1024);
         tp.add(s, (n + block - 1) / block);
@@ -116,8 +124,8 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
     }
     size_t bytes = 0;
     auto reserve = [&](size_t len) { size_t o = (bytes + 255) & ~size_t(255); bytes = o + len; return o; };
-    const size_t oL = reserve(sizeof(RenderLaunch) * S), oTp = reserve(sizeof(TraceArgs) * S),
-                 oTq = reserve(sizeof(TraceArgs) * S);
+    const size_t oL = reserve(sizeof(RenderLaunch) * 2 * S), oTp = reserve(sizeof(TraceArgs) * 2 * S),
+                 oTq = reserve(sizeof(TraceArgs) * 2 * S);
     MapImage* maps[4] = {&pix, &tile, &tp, &tq};
     size_t om[4][3];
     for (int k = 0; k < 4; k++) {
@@ -135,9 +143,9 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
     }
     std::vector<char> img(bytes, 0);
     auto put = [&](size_t o, const void* p, size_t len) { if (len) std::memcpy(img.data() + o, p, len); };
-    put(oL, L.data(), sizeof(RenderLaunch) * S);
-    put(oTp, Tp.data(), sizeof(TraceArgs) * S);
-    put(oTq, Tq.data(), sizeof(TraceArgs) * S);
+    put(oL, L.data(), sizeof(RenderLaunch) * 2 * S);
+    put(oTp, Tp.data(), sizeof(TraceArgs) * 2 * S);
+    put(oTq, Tq.data(), sizeof(TraceArgs) * 2 * S);
     char* d = (char*)b->tables;
     SegMap* dm[4] = {&b->map_pix, &b->map_tile, &b->map_tp, &b->map_tq};
     uint32_t* grids[4] = {&b->grid_pix, &b->grid_tile, &b->grid_tp, &b->grid_tq};
@@ -150,9 +158,11 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
         dm[k]->count = (const uint32_t*)(d + om[k][2]);
         *grids[k] = maps[k]->total;
     }
-    b->d_L = (RenderLaunch*)(d + oL);
-    b->d_Tp = (TraceArgs*)(d + oTp);
-    b->d_Tq = (TraceArgs*)(d + oTq);
+    for (int k = 0; k < 2; k++) {
+        b->d_L[k] = (RenderLaunch*)(d + oL) + k * S;
+        b->d_Tp[k] = (TraceArgs*)(d + oTp) + k * S;
+        b->d_Tq[k] = (TraceArgs*)(d + oTq) + k * S;
+    }
     wpbr_status st = hip_check(ctx, hipMemcpyAsync(b->tables, img.data(), bytes, hipMemcpyHostToDevice, ctx->stream),
                                "batch tables upload");
     if (!st) st = hip_check(ctx, hipStreamSynchronize(ctx->stream), "batch tables upload");
@@ -160,19 +170,34 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
 }
 
 // The pass loop: clear, then per sample pass camera, primary trace, shade1,
-// secondary trace, radiance + splat.
+// secondary trace, radiance + splat.  Pass j runs on stream j % 2 with wave
+// states j % 2; its camera waits for pass j - 1's shade1 (the sample-stream
+// cursors), its splat for pass j - 1's splat (the films accumulate the passes
+// in order).  Captured, the two streams become two branches of the graph.
 wpbr_status launch_passes(wpbr_ctx* ctx, wpbr_batch* b, hipStream_t st) {
     const int block = ctx->trace_block, sd = ctx->trace_stack, width = ctx->trace_width;
     bool tables = true;
     for (const wpbr_scene* sc : b->scenes) tables = tables && splat_tables_fit_h(sc->sv);
-    wpbr_status s = render_batch_launch(b->d_L, b->map_pix, b->grid_pix, 3, 0, st);
+    hipStream_t sts[2] = {st, b->stream2};
+    hipEvent_t* ev = b->evs.data();  // [0]: start; pass j: [1 + 2j] shade1, [2 + 2j] splat
+    wpbr_status s = render_batch_launch(b->d_L[0], b->map_pix, b->grid_pix, 3, 0, st);
+    if (!s) s = render_batch_launch(b->d_L[1], b->map_pix, b->grid_pix, 3, 0, st);
+    hipEventRecord(ev[0], st);
+    hipStreamWaitEvent(sts[1], ev[0], 0);
     for (uint32_t j = 0; j < b->spp && !s; j++) {
-        s = render_batch_launch(b->d_L, b->map_pix, b->grid_pix, 0, j, st);
-        if (!s) s = trace_launch_seg(b->d_Tp, b->map_tp, b->grid_tp, block, sd, width, st);
-        if (!s) s = render_batch_launch(b->d_L, b->map_pix, b->grid_pix, 1, j, st);
-        if (!s) s = trace_launch_seg(b->d_Tq, b->map_tq, b->grid_tq, block, sd, width, st);
-        if (!s) s = render_batch_launch(b->d_L, b->map_tile, b->grid_tile, tables ? 4 : 2, j, st);
+        const int k = (int)(j & 1u);
+        hipStream_t sj = sts[k];
+        if (j) hipStreamWaitEvent(sj, ev[2 * j - 1], 0);  // pass j - 1's shade1
+        s = render_batch_launch(b->d_L[k], b->map_pix, b->grid_pix, 0, j, sj);
+        if (!s) s = trace_launch_seg(b->d_Tp[k], b->map_tp, b->grid_tp, block, sd, width, sj);
+        if (!s) s = render_batch_launch(b->d_L[k], b->map_pix, b->grid_pix, 1, j, sj);
+        hipEventRecord(ev[1 + 2 * j], sj);
+        if (!s) s = trace_launch_seg(b->d_Tq[k], b->map_tq, b->grid_tq, block, sd, width, sj);
+        if (j) hipStreamWaitEvent(sj, ev[2 * j], 0);  // pass j - 1's splat
+        if (!s) s = render_batch_launch(b->d_L[k], b->map_tile, b->grid_tile, tables ? 4 : 2, j, sj);
+        hipEventRecord(ev[2 + 2 * j], sj);
     }
+    if (b->spp % 2 == 0) hipStreamWaitEvent(st, ev[2 * b->spp], 0);
     return s;
 }
 
@@ -227,28 +252,45 @@ wpbr_status wpbr_batch_create(wpbr_ctx* ctx, uint32_t n_scenes, wpbr_scene* cons
     b->films.assign(d_films, d_films + n_scenes);
     b->spp = scenes[0]->spp;
     // one allocation: every scene's wave state, then its spill slots (2n rays)
-    std::vector<size_t> wave_off(n_scenes);
+    std::vector<size_t> wave_off[2];
     size_t bytes = 0;
-    for (uint32_t s = 0; s < n_scenes; s++) {
-        const uint32_t n = scenes[s]->sv.width * scenes[s]->sv.height;
-        WaveState tmp;
-        wave_off[s] = bytes;
-        bytes += carve_wave(nullptr, n, tmp);
-        b->pixels += n;
+    for (int k = 0; k < 2; k++) {
+        wave_off[k].resize(n_scenes);
+        for (uint32_t s = 0; s < n_scenes; s++) {
+            const uint32_t n = scenes[s]->sv.width * scenes[s]->sv.height;
+            WaveState tmp;
+            wave_off[k][s] = bytes;
+            bytes += carve_wave(nullptr, n, tmp);
+            if (!k) b->pixels += n;
+        }
     }
     size_t spill_words = 0;
-    b->spill_off.resize(n_scenes);
-    for (uint32_t s = 0; s < n_scenes; s++) {
-        const uint32_t n = scenes[s]->sv.width * scenes[s]->sv.height;
-        b->spill_off[s] = bytes / 4 + spill_words;
-        spill_words += (size_t)SPILL_DEPTH_MAX * 2 * n;
+    for (int k = 0; k < 2; k++) {
+        b->spill_off[k].resize(n_scenes);
+        for (uint32_t s = 0; s < n_scenes; s++) {
+            const uint32_t n = scenes[s]->sv.width * scenes[s]->sv.height;
+            b->spill_off[k][s] = bytes / 4 + spill_words;
+            spill_words += (size_t)SPILL_DEPTH_MAX * 2 * n;
+        }
     }
     bytes += 4 * spill_words;
     st = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc batch state");
-    if (st) { delete b; return st; }
-    b->ws.resize(n_scenes);
-    for (uint32_t s = 0; s < n_scenes; s++)
-        carve_wave((char*)b->mem + wave_off[s], scenes[s]->sv.width * scenes[s]->sv.height, b->ws[s]);
+    if (!st) st = hip_check(ctx, hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking), "stream");
+    for (uint32_t e = 0; !st && e < 1 + 2 * b->spp; e++) {
+        hipEvent_t ev;
+        st = hip_check(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        if (!st) b->evs.push_back(ev);
+    }
+    if (st) { wpbr_batch_destroy(b); return st; }
+    for (int k = 0; k < 2; k++) {
+        b->ws[k].resize(n_scenes);
+        for (uint32_t s = 0; s < n_scenes; s++) {
+            carve_wave((char*)b->mem + wave_off[k][s], scenes[s]->sv.width * scenes[s]->sv.height, b->ws[k][s]);
+            // one set of sample-stream cursors per scene
+            b->ws[k][s].c1 = b->ws[0][s].c1;
+            b->ws[k][s].c2 = b->ws[0][s].c2;
+        }
+    }
     *out = b;
     return WPBR_OK;
 }
@@ -258,8 +300,10 @@ void wpbr_batch_destroy(wpbr_batch* b) {
     hipSetDevice(b->ctx->device);
     hipStreamSynchronize(b->ctx->stream);
     release_graph(b);
-    if (b->tables) hipFree(b->tables);
-    if (b->mem) hipFree(b->mem);
+    if (b->tables) (void)hipFree(b->tables);
+    if (b->mem) (void)hipFree(b->mem);
+    for (hipEvent_t e : b->evs) (void)hipEventDestroy(e);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     delete b;
 }
 
@@ -294,11 +338,12 @@ int wpbr_batch_stats(wpbr_ctx* ctx, wpbr_batch* b, double* out, int cap) {
     if (!ctx || !b || !out) return 0;
     // [4]: secondary rays traced in the last render (sum over the scenes)
     unsigned long long total = 0;
-    for (const WaveState& w : b->ws) {
-        unsigned long long q = 0;
-        if (hipMemcpy(&q, w.qcount + 2, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
-        total += q;
-    }
+    for (int k = 0; k < 2; k++)
+        for (const WaveState& w : b->ws[k]) {
+            unsigned long long q = 0;
+            if (hipMemcpy(&q, w.qcount + 2, 8, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+            total += q;
+        }
     double v[5] = {b->stats[0], b->stats[1], b->stats[2], b->stats[3], (double)total};
     int k = cap < 5 ? cap : 5;
     for (int i = 0; i < k; i++) out[i] = v[i];
