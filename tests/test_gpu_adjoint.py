@@ -38,7 +38,7 @@ def _setup(ctx, width=128, height=128, spp=2, freq=40):
     return s, r, (alpha, etak, env), target
 
 
-@pytest.mark.parametrize("size", ["128x128x2", "512x512x8"])
+@pytest.mark.parametrize("size", ["128x128x2", "128x128x3", "512x512x8"])
 def test_adjoint_matches_finite_differences(ctx, size):
     # 512x512x8 is the differentiable-render workload itself (2009850 shapes)
     w, h, spp = map(int, size.split("x"))
@@ -99,7 +99,7 @@ def test_adjoint_matches_finite_differences(ctx, size):
     assert len(refined) <= len(picks) // 8
 
 
-@pytest.mark.parametrize("size", ["128x128x2", "512x512x8"])
+@pytest.mark.parametrize("size", ["128x128x2", "128x128x3", "512x512x8"])
 def test_env_gradient_matches_the_uniform_scale_derivative(ctx, size):
     # Env radiance enters the estimator linearly and its sampling pdf is
     # homogeneous of degree 0 in the texels, so with I(c) the image at env * c,

@@ -12,16 +12,18 @@ from woodenpbrengine_amd import lib, scenes
 pytestmark = pytest.mark.gpu
 
 
-def _small_batch():
+def _small_batch(spp=2):
     # scenes of the batched workload's kind, fewer triangles and samples
-    ss = scenes.c3_scenes(n_scenes=6, spp=2, max_triangles=20_000)
+    ss = scenes.c3_scenes(n_scenes=6, spp=spp, max_triangles=20_000)
     # plus the plain scene at another size
-    ss.append(scenes.c0_scene(width=64, height=96, spp=2))
+    ss.append(scenes.c0_scene(width=64, height=96, spp=spp))
     return ss
 
 
-def test_batch_films_equal_single_renders(ctx):
-    ss = _small_batch()
+# passes alternate between two streams: an odd count ends on the other one
+@pytest.mark.parametrize("spp", [2, 3])
+def test_batch_films_equal_single_renders(ctx, spp):
+    ss = _small_batch(spp)
     print("batch of", [(s.width, s.height, s.n_shapes) for s in ss], flush=True)
     br = lib.BatchRenderer(ctx, ss)
     br.render()
