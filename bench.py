@@ -285,7 +285,18 @@ def main():
         "shade %.2f ms" % (ms_per_step, np.mean(dev_ms), t_ms, np.mean(prim_ms), n_launch, rays_per_step,
                            np.mean(shade_ms)))
     img = r.film_rgb()
-    extra = {"image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
+    # film output (tonemap to 8-bit RGB): 16 B read + 3 B written per pixel
+    film_ms = []
+    for _ in range(max(3, args.steps)):
+        lib.film_rgb8(ctx, r.d_film, scene.width, scene.height)
+        film_ms.append(ctx.last_ms())
+    npix = scene.width * scene.height
+    f_ms = float(np.median(film_ms))
+    f_gbs = 19.0 * npix / (f_ms * 1e-3) / 1e9
+    log("film output: %.1f us per 8-bit image (%.0f GB/s)" % (f_ms * 1e3, f_gbs))
+    extra = {"film_output": {"us": f_ms * 1e3, "gbs": f_gbs, "hbm_frac": f_gbs / HBM_PEAK_GBS,
+                             "pixels": npix},
+             "image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
              "device_ms_per_step": float(np.mean(dev_ms)),
              "trace_ms_per_step": t_ms,
              "shade_ms_per_step": float(np.mean(shade_ms)),

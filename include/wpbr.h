@@ -267,6 +267,18 @@ wpbr_status wpbr_scene_set_params(wpbr_ctx* ctx, wpbr_scene* scene, const float*
  * output may be null. */
 wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
                            const float* d_target_rgb, float* d_dL_drgb, float* d_rgb, double* loss);
+/* The film as an 8-bit RGB image (3 bytes per pixel, row-major, d_rgb8 on the
+ * device): JobOutputFilmTitles (CSFilm.cpp:141-170) per pixel: black if the
+ * accumulated radiance is black, else xyzToRGB(sum XYZ w) / (sum w * 800000),
+ * clamped to [0, 1], gammaCorrect (CSFilm.cpp:129-140, slope 12.98 below
+ * 0.0031308), times 255, truncated. */
+wpbr_status wpbr_film_rgb8(wpbr_ctx* ctx, uint32_t width, uint32_t height, const float* d_film,
+                           uint8_t* d_rgb8);
+/* Writes an 8-bit RGB image (host memory, row-major) as a PNG file: the output
+ * of JobOutputFilm (CSFilm.cpp:183-226, which merges the tiles and encodes with
+ * lodepng).  Host code only; no context needed. */
+wpbr_status wpbr_write_png(const char* path, uint32_t width, uint32_t height, const uint8_t* rgb8);
+
 /* Gradient of the loss with respect to every parameter, given the forward's
  * film and d loss / d rgb per pixel (W*H*3): the sample passes are replayed from
  * their sample streams (nothing is kept from the forward but the film) and a

@@ -327,6 +327,10 @@ void orc_to_xyz(const float* s64, float* xyz) {
     to_xyz(s, xyz);
 }
 
+void orc_film_rgb8(uint32_t npix, const float* xyz, const float* w, uint8_t* out) {
+    film_rgb8(npix, xyz, w, out);
+}
+
 void orc_shade_probe(int fn, const float* in, float* out) { shade_probe(fn, in, out); }
 
 void orc_samples(uint32_t seed, uint32_t pixel, uint32_t n1, uint32_t n2, float* u1, float* u2) {

@@ -32,6 +32,7 @@ inline float cr_atan2(float y, float x) { return (float)std::atan2((double)y, (d
 inline float cr_acos(float x) { return (float)std::acos((double)x); }
 inline float cr_log(float x) { return (float)std::log((double)x); }
 inline float cr_exp(float x) { return (float)std::exp((double)x); }
+inline float cr_pow(float x, float y) { return (float)std::pow((double)x, (double)y); }
 
 struct V3 {
     float x, y, z;

@@ -94,6 +94,9 @@ float average_spectrum_samples(const float* lambda, const float* vals, int n, fl
 Spec from_rgb(const float* rgb, bool illuminant);
 void to_xyz(const Spec& s, float* xyz);
 void xyz_to_rgb(const float* xyz, float* rgb);
+// JobOutputFilmTitles (CSFilm.cpp:141-170) with gammaCorrect (CSFilm.cpp:129-140):
+// per pixel the accumulated XYZ (3 floats) and filter weight sum -> 3 bytes.
+void film_rgb8(uint32_t npix, const float* xyz, const float* w, uint8_t* out);
 
 // ---- render (render.cpp) ----------------------------------------------------
 enum MaterialType : uint32_t { MAT_CONDUCTOR = 0, MAT_DIELECTRIC = 1, MAT_SPECULAR = 2 };

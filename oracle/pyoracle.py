@@ -259,6 +259,18 @@ def to_xyz(spec):
     return out
 
 
+def film_rgb8(xyz, w):
+    """8-bit RGB of a film: xyz (npix, 3) accumulated XYZ, w (npix,) filter weight sums."""
+    lib = load()
+    lib.orc_film_rgb8.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    xyz = _f32(xyz).reshape(-1, 3)
+    w = _f32(w).reshape(-1)
+    assert len(w) == len(xyz)
+    out = np.empty((len(w), 3), np.uint8)
+    lib.orc_film_rgb8(len(w), _p(xyz), _p(w), _p(out))
+    return out
+
+
 def samples(seed, pixel, n1, n2):
     lib = load()
     lib.orc_samples.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]

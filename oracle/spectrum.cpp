@@ -114,4 +114,34 @@ void xyz_to_rgb(const float* xyz, float* rgb) {
     rgb[2] = (xyz[0] * 0.041556f + xyz[1] * -0.204043f) + xyz[2] * 1.057311f;
 }
 
+// gammaCorrect, CSFilm.cpp:129-140 (slope 12.98 as written there, not sRGB's 12.92).
+static float gamma_correct(float v) {
+    if (v <= 0.0031308f) return 12.98f * v;
+    return 1.055f * cr_pow(v, (float)(1.f / 2.4f)) - 0.055f;
+}
+
+// JobOutputFilmTitles::update, CSFilm.cpp:141-170.  The pixel is black when its
+// accumulated XYZ is zero (the reference tests the accumulated spectrum; the XYZ
+// of a non-zero spectrum is non-zero because the Y table is positive on every
+// bin).  rgb / (w * 800000) is a divide per component; the clamp maps NaN to 0.
+void film_rgb8(uint32_t npix, const float* xyz, const float* w, uint8_t* out) {
+    for (uint32_t i = 0; i < npix; i++) {
+        const float* p = xyz + 3 * (size_t)i;
+        uint8_t* o = out + 3 * (size_t)i;
+        if (p[0] == 0.0f && p[1] == 0.0f && p[2] == 0.0f) {
+            o[0] = o[1] = o[2] = 0;
+            continue;
+        }
+        float rgb[3];
+        xyz_to_rgb(p, rgb);
+        const float d = w[i] * 800000.f;
+        for (int c = 0; c < 3; c++) {
+            float v = rgb[c] / d;
+            v = std::fmin(std::fmax(v, 0.0f), 1.0f);
+            v = gamma_correct(v) * 255.0f;
+            o[c] = (uint8_t)v;
+        }
+    }
+}
+
 }  // namespace orc

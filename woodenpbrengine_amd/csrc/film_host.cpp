@@ -1,0 +1,114 @@
+// Film output (the data format right after the path): the tonemap of
+// JobOutputFilmTitles on the GPU and the PNG file of JobOutputFilm
+// (CSFilm.cpp:129-226).  The reference encodes with lodepng; this writer stores
+// the same 8-bit RGB image in a PNG with uncompressed deflate blocks (any PNG
+// reader decodes both to the same pixels).
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "render_host.hpp"
+
+using namespace wpbr;
+
+namespace {
+
+uint32_t crc32_of(const uint8_t* p, size_t n, uint32_t crc) {
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; i++) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; k++) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        ready = true;
+    }
+    crc = ~crc;
+    for (size_t i = 0; i < n; i++) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    return ~crc;
+}
+
+void put32(std::vector<uint8_t>& v, uint32_t x) {
+    v.push_back((uint8_t)(x >> 24));
+    v.push_back((uint8_t)(x >> 16));
+    v.push_back((uint8_t)(x >> 8));
+    v.push_back((uint8_t)x);
+}
+
+void chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_t>& data) {
+    put32(out, (uint32_t)data.size());
+    const size_t start = out.size();
+    out.insert(out.end(), type, type + 4);
+    out.insert(out.end(), data.begin(), data.end());
+    put32(out, crc32_of(out.data() + start, out.size() - start, 0));
+}
+
+}  // namespace
+
+extern "C" {
+
+wpbr_status wpbr_film_rgb8(wpbr_ctx* ctx, uint32_t width, uint32_t height, const float* d_film,
+                           uint8_t* d_rgb8) {
+    if (!ctx || !d_film || !d_rgb8) return WPBR_ERR_INVALID_ARG;
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    hipEventRecord(ctx->ev0, ctx->stream);
+    wpbr_status s = film_rgb8_launch(d_film, width * height, d_rgb8, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("film output launch failed: ") + hipGetErrorString(last_launch_error));
+    s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "film output");
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
+}
+
+wpbr_status wpbr_write_png(const char* path, uint32_t width, uint32_t height, const uint8_t* rgb8) {
+    if (!path || !rgb8 || width == 0 || height == 0) return WPBR_ERR_INVALID_ARG;
+    // scanlines with filter type 0
+    const size_t row = 3 * (size_t)width;
+    std::vector<uint8_t> raw;
+    raw.reserve((row + 1) * height);
+    for (uint32_t y = 0; y < height; y++) {
+        raw.push_back(0);
+        raw.insert(raw.end(), rgb8 + y * row, rgb8 + (y + 1) * row);
+    }
+    // zlib stream of stored deflate blocks, Adler-32 at the end
+    std::vector<uint8_t> z = {0x78, 0x01};
+    for (size_t off = 0; off < raw.size() || off == 0; ) {
+        const size_t len = std::min<size_t>(65535, raw.size() - off);
+        const bool last = off + len == raw.size();
+        z.push_back(last ? 1 : 0);
+        z.push_back((uint8_t)len);
+        z.push_back((uint8_t)(len >> 8));
+        z.push_back((uint8_t)~len);
+        z.push_back((uint8_t)(~len >> 8));
+        z.insert(z.end(), raw.begin() + off, raw.begin() + off + len);
+        off += len;
+        if (last) break;
+    }
+    uint32_t a = 1, b = 0;
+    for (uint8_t v : raw) {
+        a = (a + v) % 65521u;
+        b = (b + a) % 65521u;
+    }
+    put32(z, (b << 16) | a);
+    std::vector<uint8_t> png = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    std::vector<uint8_t> ihdr;
+    put32(ihdr, width);
+    put32(ihdr, height);
+    ihdr.push_back(8);  // bit depth
+    ihdr.push_back(2);  // colour type: RGB
+    ihdr.push_back(0);  // deflate
+    ihdr.push_back(0);  // adaptive filtering
+    ihdr.push_back(0);  // no interlace
+    chunk(png, "IHDR", ihdr);
+    chunk(png, "IDAT", z);
+    chunk(png, "IEND", {});
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return WPBR_ERR_INVALID_ARG;
+    const bool ok = std::fwrite(png.data(), 1, png.size(), f) == png.size();
+    return (std::fclose(f) == 0 && ok) ? WPBR_OK : WPBR_ERR_INVALID_ARG;
+}
+
+}  // extern "C"

@@ -785,6 +785,48 @@ __global__ void ks_clear(const RenderLaunch* __restrict__ T, SegMap map) {
     if (i < 16) L.ws.qcount[i] = 0;
 }
 
+// ---- film output -----------------------------------------------------------------------
+// gammaCorrect (CSFilm.cpp:129-140), the reference's 12.98 slope included.
+WPBR_D float gamma_correct(float v) {
+    if (v <= 0.0031308f) return 12.98f * v;
+    return 1.055f * cr_pow(v, (float)(1.f / 2.4f)) - 0.055f;
+}
+
+// JobOutputFilmTitles (CSFilm.cpp:141-170) on the film (sum XYZ w, sum w per
+// pixel, row-major): black pixels 0; else rgb = xyzToRGB(sum XYZ w) /
+// (sum w * 800000), clamped to [0, 1] (NaN to 0), gamma-corrected, * 255,
+// truncated to 8 bits; three bytes per pixel.  HBM-bound: 16 B read and 3 B
+// written per pixel.
+__global__ __launch_bounds__(256) void k_film_rgb8(const float4* __restrict__ film, uint32_t npix,
+                                                   uint8_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    const float4 f = film[i];
+    uint8_t* o = out + 3 * (size_t)i;
+    if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) {
+        o[0] = o[1] = o[2] = 0;
+        return;
+    }
+    float rgb[3];
+    rgb[0] = (f.x * 3.240479f + f.y * -1.537150f) + f.z * -0.498535f;
+    rgb[1] = (f.x * -0.969256f + f.y * 1.875991f) + f.z * 0.041556f;
+    rgb[2] = (f.x * 0.041556f + f.y * -0.204043f) + f.z * 1.057311f;
+    const float d = f.w * 800000.f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        float v = rgb[c] / d;
+        v = fminf(fmaxf(v, 0.0f), 1.0f);
+        v = gamma_correct(v) * 255.0f;
+        o[c] = (uint8_t)v;
+    }
+}
+
+wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st) {
+    if (npix) hipLaunchKernelGGL(k_film_rgb8, dim3((npix + 255) / 256), dim3(256), 0, st,
+                                 reinterpret_cast<const float4*>(film), npix, out);
+    return launch_status();
+}
+
 wpbr_status render_batch_launch(const RenderLaunch* T, const SegMap& map, uint32_t grid, int stage, uint32_t j,
                                 hipStream_t st) {
     switch (stage) {

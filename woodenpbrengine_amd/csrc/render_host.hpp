@@ -9,6 +9,7 @@
 namespace wpbr {
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
+wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st);
 // Batched stages over a device table of launch records: 0 camera, 1 shade1,
 // 2 radiance + splat, 3 clear films and queue statistics, 4 radiance + splat
 // with the spectral tables in LDS (every scene's tables fit: splat_tables_fit_h).

@@ -27,7 +27,7 @@ EXPORTS = [
     "wpbr_autotune_render", "wpbr_scene_param_counts", "wpbr_scene_set_params", "wpbr_film_loss",
     "wpbr_render_backward", "wpbr_bsdf_derivatives", "wpbr_lbvh_build_segmented",
     "wpbr_scenes_create", "wpbr_batch_create", "wpbr_batch_destroy", "wpbr_batch_render",
-    "wpbr_batch_stats", "wpbr_autotune_batch",
+    "wpbr_batch_stats", "wpbr_autotune_batch", "wpbr_film_rgb8", "wpbr_write_png",
 ]
 
 
@@ -121,6 +121,8 @@ def load():
         "wpbr_batch_render": (i32, [vp, vp]),
         "wpbr_batch_stats": (C.c_int, [vp, vp, C.POINTER(C.c_double), C.c_int]),
         "wpbr_autotune_batch": (i32, [vp, vp, C.POINTER(TraceConfig)]),
+        "wpbr_film_rgb8": (i32, [vp, u32, u32, vp, vp]),
+        "wpbr_write_png": (i32, [C.c_char_p, u32, u32, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -317,6 +319,26 @@ def autotune_collide(ctx, scene, bvh, d_rays, n):
     return {f: getattr(cfg, f) for f, _ in TraceConfig._fields_}
 
 
+def film_rgb8(ctx, d_film, width, height):
+    """8-bit RGB (height x width x 3) of a device film of float4 (sum XYZ w, sum w)
+    per pixel, computed on the device."""
+    d_rgb = ctx.empty((height, width, 3), np.uint8)
+    ctx.check(ctx.lib.wpbr_film_rgb8(ctx.h, width, height, d_film.ptr, d_rgb.ptr))
+    out = d_rgb.numpy()
+    d_rgb.free()
+    return out
+
+
+def write_png(path, rgb8):
+    """Writes an 8-bit RGB image (H x W x 3, host memory) as a PNG file."""
+    rgb8 = np.ascontiguousarray(rgb8, dtype=np.uint8)
+    if rgb8.ndim != 3 or rgb8.shape[2] != 3:
+        raise WpbrError(1, "write_png wants an H x W x 3 uint8 image")
+    st = load().wpbr_write_png(str(path).encode(), rgb8.shape[1], rgb8.shape[0], _ptr(rgb8))
+    if st:
+        raise WpbrError(st, "cannot write %s" % path)
+
+
 def scene_desc(s):
     """wpbr_scene_desc of a woodenpbrengine_amd.scenes.RenderScene; returns
     (desc, keepalive arrays)."""
@@ -377,6 +399,15 @@ class Renderer:
         M = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556],
                       [0.041556, -0.204043, 1.057311]])
         return (xyz @ M.T) / w
+
+    def film_rgb8(self):
+        """The film as 8-bit RGB (H x W x 3 uint8), tonemapped on the device
+        (JobOutputFilmTitles, CSFilm.cpp:141-170)."""
+        return film_rgb8(self.ctx, self.d_film, self.W, self.H)
+
+    def write_png(self, path):
+        """Writes the film as a PNG file (JobOutputFilm, CSFilm.cpp:183-226)."""
+        write_png(path, self.film_rgb8())
 
     def destroy(self):
         if self.h:
