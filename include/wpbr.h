@@ -194,7 +194,34 @@ typedef struct {
     const float* raster_camera;   /* 16 */
     const float* camera_world;    /* 12 */
     uint32_t width, height, spp, seed;
+    /* Sample values (WPBR_SAMPLER_*).  The stratified modes give each pixel the
+     * arrays of JobCreateCameraSamples (CSFilm.cpp:22-23, 57-64): 16 + spp 1D
+     * and 16 + 4 spp 2D values, consumed in order over the spp passes. */
+    uint32_t sampler;
 } wpbr_scene_desc;
+
+/* Sampler modes.  INDEPENDENT: a counter-based hash stream per (seed, pixel,
+ * index).  STRATIFIED: the arrays of JobSamplerStratifiedGenerateSampels1D/2D
+ * (CSSamplerStratified.cpp:8-90) with a defined shuffle; the 2D strata are
+ * scaled by the array size as the reference does (CSSamplerStratified.cpp:79).
+ * STRATIFIED_N: the same with the 2D strata scaled by n = sqrt(size), the
+ * stratification the reference's outer invNSamples intends. */
+enum { WPBR_SAMPLER_INDEPENDENT = 0, WPBR_SAMPLER_STRATIFIED = 1, WPBR_SAMPLER_STRATIFIED_N = 2 };
+
+/* The per-pixel sample arrays of pixels pixel0 .. pixel0 + npix - 1 (film
+ * row-major index) on the device: d_u1 receives npix x n1 floats, d_u2 npix x n2
+ * float pairs, each pixel's array in consumption order.  For the stratified
+ * modes these are the CSamples1D / CSamples2D data that
+ * JobSamplerStratifiedGenerateSampels1D/2D (CSSamplerStratified.cpp:8-90)
+ * write: 1D element k = min((k + xi) / n1, 1); 2D element (i, j) of the n x n
+ * grid (n = (int)sqrt((float)n2)) = min((i + xi, j + xi') * s, 1) with s = 1/n2
+ * (STRATIFIED) or 1/n (STRATIFIED_N), the n2 - n^2 tail (0, 0); then shuffled
+ * (the reference's std::shuffle from a random_device seed is replaced by a
+ * hash permutation of the array positions, defined per seed and pixel).  The
+ * render with the same seed and mode reads exactly these values.  Either
+ * output may be null. */
+wpbr_status wpbr_sample_arrays(wpbr_ctx* ctx, uint32_t seed, uint32_t sampler, uint32_t pixel0,
+                               uint32_t npix, uint32_t n1, uint32_t n2, float* d_u1, float* d_u2);
 
 typedef struct wpbr_scene wpbr_scene;
 

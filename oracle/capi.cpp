@@ -199,6 +199,7 @@ struct orc_scene_desc {
     const float* raster_camera;
     const float* camera_world;
     uint32_t width, height, spp, seed;
+    uint32_t sampler;
 };
 
 extern "C" {
@@ -250,6 +251,7 @@ static SceneHold* make_scene(const orc_scene_desc* d) {
     sc.height = d->height;
     sc.spp = d->spp;
     sc.seed = d->seed;
+    sc.smp = make_sampler(d->seed, d->sampler, 16u + d->spp, 16u + 4u * d->spp);
     uint32_t n = d->n_spheres + d->n_triangles;
     std::vector<float> b(6 * (size_t)n), c(3 * (size_t)n);
     sphere_bounds(d->n_spheres, d->sphere_xform, d->sphere_radius, b.data(), c.data());
@@ -334,8 +336,29 @@ void orc_film_rgb8(uint32_t npix, const float* xyz, const float* w, uint8_t* out
 void orc_shade_probe(int fn, const float* in, float* out) { shade_probe(fn, in, out); }
 
 void orc_samples(uint32_t seed, uint32_t pixel, uint32_t n1, uint32_t n2, float* u1, float* u2) {
-    for (uint32_t i = 0; i < n1; i++) u1[i] = sample_1d(seed, pixel, i);
-    for (uint32_t i = 0; i < n2; i++) sample_2d(seed, pixel, i, u2 + 2 * i);
+    const Sampler S = make_sampler(seed, 0, 1, 1);
+    for (uint32_t i = 0; i < n1; i++) u1[i] = sample_1d(S, pixel, i);
+    for (uint32_t i = 0; i < n2; i++) sample_2d(S, pixel, i, u2 + 2 * i);
 }
+
+// The sample arrays of pixels pixel0 .. pixel0 + npix - 1 (npix x n1, npix x n2 x 2):
+// by_element = 0 builds them as the reference's jobs do (stratified_arrays),
+// 1 reads them one position at a time as the render does (sample_1d / sample_2d).
+void orc_sample_arrays(uint32_t seed, uint32_t mode, uint32_t pixel0, uint32_t npix, uint32_t n1,
+                       uint32_t n2, int by_element, float* u1, float* u2) {
+    const Sampler S = make_sampler(seed, mode, n1, n2);
+    for (uint32_t p = 0; p < npix; p++) {
+        float* a = u1 + (size_t)p * n1;
+        float* b = u2 + 2 * (size_t)p * n2;
+        if (!by_element && mode != 0) {
+            stratified_arrays(S, pixel0 + p, a, b);
+            continue;
+        }
+        for (uint32_t c = 0; c < n1; c++) a[c] = sample_1d(S, pixel0 + p, c);
+        for (uint32_t c = 0; c < n2; c++) sample_2d(S, pixel0 + p, c, b + 2 * c);
+    }
+}
+
+uint32_t orc_permute_index(uint32_t c, uint32_t n, uint32_t key) { return permute_index(c, n, key); }
 
 }  // extern "C"

@@ -108,6 +108,19 @@ struct Material {
     float k_rgb[3];    // conductor k as RGB (fromRGB Reflectance)
 };
 
+// Sample values of a render: mode 0 independent hash streams per (seed, pixel,
+// index); modes 1, 2 the shuffled stratified arrays of
+// JobSamplerStratifiedGenerateSampels1D/2D (CSSamplerStratified.cpp:8-90), n1 1D
+// and n2 2D values per pixel (CSFilm.cpp:22-23 with spp passes: 16 + spp,
+// 16 + 4 spp), 2D grid root x root scaled by 1/n2 (mode 1, as the reference) or
+// 1/root (mode 2).
+struct Sampler {
+    uint32_t seed = 0, mode = 0;
+    uint32_t n1 = 1, n2 = 1, root = 1;
+    float inv1 = 1.0f, inv2 = 1.0f;
+};
+Sampler make_sampler(uint32_t seed, uint32_t mode, uint32_t n1, uint32_t n2);
+
 struct RenderScene {
     Scene geom;
     // per-triangle shading data (optional, may be null)
@@ -132,6 +145,7 @@ struct RenderScene {
     const float* camera_world = nullptr;       // 3x4 row-major
     uint32_t width = 0, height = 0, spp = 0;
     uint32_t seed = 0;
+    Sampler smp;
 };
 
 struct RenderOut {
@@ -166,8 +180,12 @@ struct SampleRec {
     EnvCell env;            // env lookup of the miss or of the BSDF direction
 };
 
-float sample_1d(uint32_t seed, uint32_t pixel, uint32_t index);
-void sample_2d(uint32_t seed, uint32_t pixel, uint32_t index, float* u);
+float sample_1d(const Sampler& S, uint32_t pixel, uint32_t index);
+void sample_2d(const Sampler& S, uint32_t pixel, uint32_t index, float* u);
+// The arrays of pixel `pixel` as the reference's jobs build them: stratified
+// elements in the reference's loop order, then shuffled (u1: n1, u2: n2 x 2).
+void stratified_arrays(const Sampler& S, uint32_t pixel, float* u1, float* u2);
+uint32_t permute_index(uint32_t c, uint32_t n, uint32_t key);
 void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
             const uint32_t* tile_subset, uint32_t n_tiles, SampleRec* recs = nullptr);
 // Smits case index (0..5) of rgb, in the order of from_rgb's branches.

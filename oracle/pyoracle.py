@@ -191,7 +191,7 @@ class SceneDesc(C.Structure):
                 ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env_xform", C.c_void_p),
                 ("raster_camera", C.c_void_p), ("camera_world", C.c_void_p),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("sampler", C.c_uint32)]
 
 
 def scene_desc(s):
@@ -205,7 +205,8 @@ def scene_desc(s):
     d = SceneDesc(len(sr), _p(sx), _p(sr), len(tv), _p(tv), _p(tf), _p(tuv), _p(tn), _p(tt),
                   _p(sm), len(mt), _p(mt), len(ls), _p(ls), _p(le),
                   1 if s.env is not None else 0, _p(ee), _p(er), er.shape[1], er.shape[0],
-                  _p(ex), _p(rc), _p(cw), s.width, s.height, s.spp, s.seed)
+                  _p(ex), _p(rc), _p(cw), s.width, s.height, s.spp, s.seed,
+                  getattr(s, "sampler", 0))
     return d, keep
 
 
@@ -269,6 +270,23 @@ def film_rgb8(xyz, w):
     out = np.empty((len(w), 3), np.uint8)
     lib.orc_film_rgb8(len(w), _p(xyz), _p(w), _p(out))
     return out
+
+
+def sample_arrays(seed, mode, pixel0, npix, n1, n2, by_element=False):
+    """Per-pixel sample arrays (npix x n1, npix x n2 x 2) of a sampler mode."""
+    lib = load()
+    lib.orc_sample_arrays.argtypes = [C.c_uint32] * 6 + [C.c_int, C.c_void_p, C.c_void_p]
+    u1 = np.empty((npix, n1), np.float32)
+    u2 = np.empty((npix, n2, 2), np.float32)
+    lib.orc_sample_arrays(seed, mode, pixel0, npix, n1, n2, 1 if by_element else 0, _p(u1), _p(u2))
+    return u1, u2
+
+
+def permute_index(c, n, key):
+    lib = load()
+    lib.orc_permute_index.argtypes = [C.c_uint32] * 3
+    lib.orc_permute_index.restype = C.c_uint32
+    return int(lib.orc_permute_index(c, n, key))
 
 
 def samples(seed, pixel, n1, n2):

@@ -48,12 +48,93 @@ static inline float hash_unit(uint32_t seed, uint32_t pixel, uint32_t key) {
     h = mix32(h ^ key);
     return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
-float sample_1d(uint32_t seed, uint32_t pixel, uint32_t index) {
-    return hash_unit(seed, pixel, 3u * index);
+
+Sampler make_sampler(uint32_t seed, uint32_t mode, uint32_t n1, uint32_t n2) {
+    Sampler S;
+    S.seed = seed;
+    S.mode = mode;
+    S.n1 = n1;
+    S.n2 = n2;
+    // CSSamplerStratified.cpp:63-80: `int n = std::sqrt((float)size)`;
+    // `float invNSamples = 1.0 / size` (1D, and the inner 2D one that is used),
+    // the outer `1.0 / n` for mode 2
+    S.root = (uint32_t)(int)std::sqrt((float)n2);
+    S.inv1 = (float)(1.0 / n1);
+    S.inv2 = mode == 2 ? (float)(1.0 / (int)S.root) : (float)(1.0 / n2);
+    return S;
 }
-void sample_2d(uint32_t seed, uint32_t pixel, uint32_t index, float* u) {
-    u[0] = hash_unit(seed, pixel, 3u * index + 1u);
-    u[1] = hash_unit(seed, pixel, 3u * index + 2u);
+
+// Shuffle: position c of a shuffled array holds element permute_index(c) of the
+// array in generation order.  permute_index is a bijection of [0, n) (this
+// build's definition; the reference's std::shuffle draws from a random_device):
+// a hash permutation of [0, 2^b), b = bits of n - 1 (key xor, odd multipliers and
+// right xor-shifts are invertible modulo 2^b), restricted to [0, n) by cycle
+// walking, after Kensler's permute() (Correlated Multi-Jittered Sampling, 2013).
+uint32_t permute_index(uint32_t c, uint32_t n, uint32_t key) {
+    if (n <= 1u) return 0u;
+    const uint32_t b = 32u - (uint32_t)__builtin_clz(n - 1u);
+    const uint32_t w = 0xFFFFFFFFu >> (32u - b);
+    const uint32_t s = (b + 1u) >> 1;
+    do {
+        c = ((c ^ key) * 0x9E3779B1u) & w;
+        c ^= c >> s;
+        c = ((c + (key >> 11)) * 0x2C1B3C6Du) & w;
+        c ^= c >> s;
+        c = ((c ^ (key >> 21)) * 0x297A2D39u) & w;
+        c ^= c >> s;
+    } while (c >= n);
+    return c;
+}
+static uint32_t shuffle_key(uint32_t seed, uint32_t pixel, uint32_t dim) {
+    return mix32(mix32(seed ^ (0x632BE5ABu + dim)) ^ pixel);
+}
+
+// JobSamplerStratifiedGenerateSampels1D / 2D (CSSamplerStratified.cpp:8-46,
+// 49-90) for one pixel, the jitter of element k being the hash stream's value
+// at index k (the reference draws uniform(gen) in the same order).
+void stratified_arrays(const Sampler& S, uint32_t pixel, float* u1, float* u2) {
+    std::vector<float> d1(S.n1);
+    for (uint32_t k = 0; k < S.n1; k++)  // (iData + u) * invNSamples, minv(.., 1)
+        d1[k] = std::min(((float)k + hash_unit(S.seed, pixel, 3u * k)) * S.inv1, 1.0f);
+    std::vector<float> d2(2 * (size_t)S.n2, 0.0f);  // resize(): value-initialised tail
+    for (uint32_t i = 0, k = 0; i < S.root; i++) {
+        for (uint32_t j = 0; j < S.root; j++, k++) {
+            const float dtX = hash_unit(S.seed, pixel, 3u * k + 1u);
+            const float dtY = hash_unit(S.seed, pixel, 3u * k + 2u);
+            d2[2 * k] = std::min(((float)i + dtX) * S.inv2, 1.0f);
+            d2[2 * k + 1] = std::min(((float)j + dtY) * S.inv2, 1.0f);
+        }
+    }
+    const uint32_t key1 = shuffle_key(S.seed, pixel, 1u), key2 = shuffle_key(S.seed, pixel, 2u);
+    for (uint32_t c = 0; c < S.n1; c++) u1[c] = d1[permute_index(c, S.n1, key1)];
+    for (uint32_t c = 0; c < S.n2; c++) {
+        const uint32_t k = permute_index(c, S.n2, key2);
+        u2[2 * c] = d2[2 * k];
+        u2[2 * c + 1] = d2[2 * k + 1];
+    }
+}
+
+// Value at position `index` of a pixel's 1D array, one element at a time
+// (positions past the end of a stratified array wrap).
+float sample_1d(const Sampler& S, uint32_t pixel, uint32_t index) {
+    if (S.mode == 0) return hash_unit(S.seed, pixel, 3u * index);
+    const uint32_t k = permute_index(index % S.n1, S.n1, shuffle_key(S.seed, pixel, 1u));
+    return std::min(((float)k + hash_unit(S.seed, pixel, 3u * k)) * S.inv1, 1.0f);
+}
+void sample_2d(const Sampler& S, uint32_t pixel, uint32_t index, float* u) {
+    if (S.mode == 0) {
+        u[0] = hash_unit(S.seed, pixel, 3u * index + 1u);
+        u[1] = hash_unit(S.seed, pixel, 3u * index + 2u);
+        return;
+    }
+    const uint32_t k = permute_index(index % S.n2, S.n2, shuffle_key(S.seed, pixel, 2u));
+    if (k >= S.root * S.root) {
+        u[0] = u[1] = 0.0f;
+        return;
+    }
+    const uint32_t i = k / S.root, j = k % S.root;
+    u[0] = std::min(((float)i + hash_unit(S.seed, pixel, 3u * k + 1u)) * S.inv2, 1.0f);
+    u[1] = std::min(((float)j + hash_unit(S.seed, pixel, 3u * k + 2u)) * S.inv2, 1.0f);
 }
 
 // ---- helpers -------------------------------------------------------------
@@ -479,7 +560,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     Spec L;
     L.fill(0.0f);
     float u2[2];
-    sample_2d(sc.seed, pixel, c2++, u2);
+    sample_2d(sc.smp, pixel, c2++, u2);
     pFilm[0] = (float)px + u2[0];
     pFilm[1] = (float)py + u2[1];
     if (rec) { rec->pfilm[0] = pFilm[0]; rec->pfilm[1] = pFilm[1]; }
@@ -512,13 +593,13 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     }
     // light pick (1D), clamped
     const uint32_t nLights = sc.n_sphere_lights + (sc.has_env ? 1u : 0u);
-    float u1 = sample_1d(sc.seed, pixel, c1++);
+    float u1 = sample_1d(sc.smp, pixel, c1++);
     uint32_t li = (uint32_t)((u1 + 0.5f) * (float)(nLights - 1));
     if (li > nLights - 1) li = nLights - 1;
     bool envPick = sc.has_env && li == sc.n_sphere_lights;
     // light sample (2D)
     float ul[2];
-    sample_2d(sc.seed, pixel, c2++, ul);
+    sample_2d(sc.smp, pixel, c2++, ul);
     V3 wiL = v3(0, 0, 0);
     float pdfL = 0;
     if (!envPick) {
@@ -592,7 +673,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
         }
     } else {
         float ub[2];
-        sample_2d(sc.seed, pixel, c2++, ub);
+        sample_2d(sc.smp, pixel, c2++, ub);
         if (rec) { rec->ub[0] = ub[0]; rec->ub[1] = ub[1]; }
         float whD;
         V3 wh = ggx_sample(woL, ub, m.alpha, m.alpha, &whD);

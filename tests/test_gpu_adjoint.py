@@ -23,8 +23,9 @@ H_FD = 1e-4
 REL_TOL = 0.02
 
 
-def _setup(ctx, width=128, height=128, spp=2, freq=40):
+def _setup(ctx, width=128, height=128, spp=2, freq=40, sampler=0):
     s = scenes.c4_scene(width=width, height=height, spp=spp, freq=freq)
+    s.sampler = sampler
     r = lib.DiffRenderer(ctx, s)
     alpha, etak, env = r.params()
     g = np.random.default_rng(2026)
@@ -38,11 +39,14 @@ def _setup(ctx, width=128, height=128, spp=2, freq=40):
     return s, r, (alpha, etak, env), target
 
 
-@pytest.mark.parametrize("size", ["128x128x2", "128x128x3", "512x512x8"])
+# "/s2": stratified sample arrays (the replay regenerates the same values)
+@pytest.mark.parametrize("size", ["128x128x2", "128x128x3", "128x128x3/s2", "512x512x8"])
 def test_adjoint_matches_finite_differences(ctx, size):
     # 512x512x8 is the differentiable-render workload itself (2009850 shapes)
+    size, _, smp = size.partition("/s")
     w, h, spp = map(int, size.split("x"))
-    s, r, (alpha, etak, env), target = _setup(ctx, w, h, spp, 40 if w == 128 else 317)
+    s, r, (alpha, etak, env), target = _setup(ctx, w, h, spp, 40 if w == 128 else 317,
+                                              int(smp) if smp else 0)
     print("scene", s.n_shapes, "shapes", flush=True)
     r.forward()
     r.set_target(target)

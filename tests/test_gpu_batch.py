@@ -17,6 +17,10 @@ def _small_batch(spp=2):
     ss = scenes.c3_scenes(n_scenes=6, spp=spp, max_triangles=20_000)
     # plus the plain scene at another size
     ss.append(scenes.c0_scene(width=64, height=96, spp=spp))
+    if spp % 2:
+        # scenes with stratified sample arrays in the same batch
+        ss[1].sampler = 1
+        ss[-1].sampler = 2
     return ss
 
 

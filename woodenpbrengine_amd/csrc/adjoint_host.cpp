@@ -244,7 +244,7 @@ wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_f
     L.sv = sc->sv;
     L.n = n;
     L.spp = sc->spp;
-    L.seed = sc->seed;
+    L.smp = sc->smp;
     L.film = nullptr;
     L.sample_xyz = nullptr;
     L.record = 1;

@@ -81,7 +81,7 @@ wpbr_status upload_tables(wpbr_ctx* ctx, wpbr_batch* b) {
         l.n = n;
         l.spp_index = 0;
         l.spp = sc->spp;
-        l.seed = sc->seed;
+        l.smp = sc->smp;
         l.film = b->films[s];
         l.sample_xyz = nullptr;
         l.record = 0;

@@ -1,9 +1,11 @@
-// Film output (the data format right after the path): the tonemap of
-// JobOutputFilmTitles on the GPU and the PNG file of JobOutputFilm
-// (CSFilm.cpp:129-226).  The reference encodes with lodepng; this writer stores
+// The data formats either side of the render: the per-pixel sample arrays
+// (JobSamplerStratifiedGenerateSampels1D/2D, CSSamplerStratified.cpp:8-90) and
+// the film output, the tonemap of JobOutputFilmTitles on the GPU and the PNG
+// file of JobOutputFilm (CSFilm.cpp:129-226).  The reference encodes with lodepng; this writer stores
 // the same 8-bit RGB image in a PNG with uncompressed deflate blocks (any PNG
 // reader decodes both to the same pixels).
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -47,7 +49,44 @@ void chunk(std::vector<uint8_t>& out, const char* type, const std::vector<uint8_
 
 }  // namespace
 
+namespace wpbr {
+
+SamplerDef make_sampler(uint32_t seed, uint32_t mode, uint32_t n1, uint32_t n2) {
+    SamplerDef S;
+    S.seed = seed;
+    S.mode = mode;
+    S.n1 = n1;
+    S.n2 = n2;
+    // CSSamplerStratified.cpp:63-80: int n = sqrt((float)size); the 1D scale and the
+    // reference's 2D scale are 1.0 / size rounded to float, the intended 2D one 1.0 / n
+    S.root = (uint32_t)(int)std::sqrt((float)n2);
+    S.inv1 = (float)(1.0 / n1);
+    S.inv2 = mode == WPBR_SAMPLER_STRATIFIED_N ? (float)(1.0 / (int)S.root) : (float)(1.0 / n2);
+    return S;
+}
+
+}  // namespace wpbr
+
 extern "C" {
+
+wpbr_status wpbr_sample_arrays(wpbr_ctx* ctx, uint32_t seed, uint32_t sampler, uint32_t pixel0, uint32_t npix,
+                               uint32_t n1, uint32_t n2, float* d_u1, float* d_u2) {
+    if (!ctx) return WPBR_ERR_INVALID_ARG;
+    if (sampler > WPBR_SAMPLER_STRATIFIED_N) return set_err(ctx, WPBR_ERR_INVALID_ARG, "unknown sampler mode");
+    if ((d_u1 && n1 == 0) || (d_u2 && n2 == 0))
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "empty sample array");
+    if ((uint64_t)pixel0 + npix > 0x100000000ull) return set_err(ctx, WPBR_ERR_INVALID_ARG, "pixel range");
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    const SamplerDef S = make_sampler(seed, sampler, n1 ? n1 : 1, n2 ? n2 : 1);
+    hipEventRecord(ctx->ev0, ctx->stream);
+    wpbr_status s = sample_arrays_launch(S, pixel0, npix, d_u1, d_u2, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("sample array launch failed: ") + hipGetErrorString(last_launch_error));
+    s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "sample arrays");
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
+}
 
 wpbr_status wpbr_film_rgb8(wpbr_ctx* ctx, uint32_t width, uint32_t height, const float* d_film,
                            uint8_t* d_rgb8) {

@@ -36,11 +36,98 @@ WPBR_D float hash_unit(uint32_t seed, uint32_t pixel, uint32_t key) {
     h = mix32(h ^ key);
     return (float)(h >> 8) * (1.0f / 16777216.0f);
 }
-WPBR_D float next1d(uint32_t seed, uint32_t pixel, uint32_t& c1) { return hash_unit(seed, pixel, 3u * c1++); }
-WPBR_D void next2d(uint32_t seed, uint32_t pixel, uint32_t& c2, float* u) {
-    u[0] = hash_unit(seed, pixel, 3u * c2 + 1u);
-    u[1] = hash_unit(seed, pixel, 3u * c2 + 2u);
-    c2++;
+
+// ---- stratified sample arrays ----------------------------------------------------
+// JobSamplerStratifiedGenerateSampels1D/2D (CSSamplerStratified.cpp:8-90) give
+// each pixel a 1D array (element k = min((k + xi_k) / n1, 1)) and a 2D array
+// (element k = i n + j of an n x n grid = min((i + xi, j + xi') * inv2, 1), the
+// n2 - n^2 tail left (0, 0)), then shuffle both.  Here no array is stored:
+// position c of the shuffled array holds element permute_index(c), and an
+// element's jitter is the hash stream's value at its index, so any position is
+// computed on its own where it is consumed.
+
+// A bijection of [0, n): a hash permutation of [0, 2^b), b = bits of n - 1
+// (xor with a key, odd multipliers and right xor-shifts are each invertible
+// modulo 2^b), restricted to [0, n) by cycle walking: the orbit of c under a
+// permutation returns to c, so the walk ends; the construction of Kensler's
+// permute() ("Correlated Multi-Jittered Sampling", Pixar 2013), with rounds of
+// its own.
+WPBR_D uint32_t permute_index(uint32_t c, uint32_t n, uint32_t key) {
+    if (n <= 1u) return 0u;
+    const uint32_t b = 32u - (uint32_t)__clz((int)(n - 1u));
+    const uint32_t w = 0xFFFFFFFFu >> (32u - b);
+    const uint32_t s = (b + 1u) >> 1;
+    do {
+        c = ((c ^ key) * 0x9E3779B1u) & w;
+        c ^= c >> s;
+        c = ((c + (key >> 11)) * 0x2C1B3C6Du) & w;
+        c ^= c >> s;
+        c = ((c ^ (key >> 21)) * 0x297A2D39u) & w;
+        c ^= c >> s;
+    } while (c >= n);
+    return c;
+}
+
+WPBR_D uint32_t shuffle_key(uint32_t seed, uint32_t pixel, uint32_t dim) {
+    return mix32(mix32(seed ^ (0x632BE5ABu + dim)) ^ pixel);
+}
+
+// Value at position c of a pixel's 1D array (positions past the end wrap).
+WPBR_D float sample1_at(const SamplerDef& S, uint32_t pixel, uint32_t c) {
+    if (S.mode == 0u) return hash_unit(S.seed, pixel, 3u * c);
+    if (c >= S.n1) c %= S.n1;
+    const uint32_t k = permute_index(c, S.n1, shuffle_key(S.seed, pixel, 1u));
+    return fminf(((float)k + hash_unit(S.seed, pixel, 3u * k)) * S.inv1, 1.0f);
+}
+
+// Value at position c of a pixel's 2D array.
+WPBR_D void sample2_at(const SamplerDef& S, uint32_t pixel, uint32_t c, float* u) {
+    if (S.mode == 0u) {
+        u[0] = hash_unit(S.seed, pixel, 3u * c + 1u);
+        u[1] = hash_unit(S.seed, pixel, 3u * c + 2u);
+        return;
+    }
+    if (c >= S.n2) c %= S.n2;
+    const uint32_t k = permute_index(c, S.n2, shuffle_key(S.seed, pixel, 2u));
+    if (k >= S.root * S.root) {
+        u[0] = u[1] = 0.0f;
+        return;
+    }
+    const uint32_t i = k / S.root, j = k - i * S.root;
+    u[0] = fminf(((float)i + hash_unit(S.seed, pixel, 3u * k + 1u)) * S.inv2, 1.0f);
+    u[1] = fminf(((float)j + hash_unit(S.seed, pixel, 3u * k + 2u)) * S.inv2, 1.0f);
+}
+
+WPBR_D float next1d(const SamplerDef& S, uint32_t pixel, uint32_t& c1) { return sample1_at(S, pixel, c1++); }
+WPBR_D void next2d(const SamplerDef& S, uint32_t pixel, uint32_t& c2, float* u) { sample2_at(S, pixel, c2++, u); }
+
+// The arrays themselves, one thread per element (wpbr_sample_arrays).
+__global__ __launch_bounds__(256) void k_sample_arrays(SamplerDef S, uint32_t pixel0, uint64_t n_el,
+                                                       uint32_t len, uint32_t dim, float* __restrict__ out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_el; t += stride) {
+        const uint32_t p = (uint32_t)(t / len), c = (uint32_t)(t - (uint64_t)p * len);
+        if (dim == 1u) {
+            out[t] = sample1_at(S, pixel0 + p, c);
+        } else {
+            float u[2];
+            sample2_at(S, pixel0 + p, c, u);
+            reinterpret_cast<float2*>(out)[t] = make_float2(u[0], u[1]);
+        }
+    }
+}
+
+wpbr_status sample_arrays_launch(const SamplerDef& S, uint32_t pixel0, uint32_t npix, float* u1, float* u2,
+                                 hipStream_t st) {
+    for (uint32_t dim = 1; dim <= 2; dim++) {
+        float* out = dim == 1 ? u1 : u2;
+        const uint64_t n_el = (uint64_t)npix * (dim == 1 ? S.n1 : S.n2);
+        if (!out || n_el == 0) continue;
+        const uint64_t blocks = std::min<uint64_t>((n_el + 255) / 256, 16384);
+        hipLaunchKernelGGL(k_sample_arrays, dim3((uint32_t)blocks), dim3(256), 0, st, S, pixel0, n_el,
+                           dim == 1 ? S.n1 : S.n2, dim, out);
+    }
+    return launch_status();
 }
 
 // ---- per-pixel state order ----------------------------------------------------------
@@ -373,7 +460,7 @@ WPBR_D void camera_body(const RenderLaunch& L, uint32_t i) {
     uint32_t c2 = L.spp_index == 0 ? 0u : L.ws.c2[i];
     if (L.spp_index == 0) L.ws.c1[i] = 0;
     float u[2];
-    next2d(L.seed, py * sv.width + px, c2, u);
+    next2d(L.smp, py * sv.width + px, c2, u);
     L.ws.c2[i] = c2;
     float fx = (float)px + u[0], fy = (float)py + u[1];
     L.ws.pfilm[i] = fx;
@@ -438,12 +525,12 @@ WPBR_D void shade1_body(const RenderLaunch& L, uint32_t i) {
             uint32_t px, py;
             state_pixel(i, sv.width, px, py);
             const uint32_t pix = py * sv.width + px;  // sample-stream id
-            float u1 = next1d(L.seed, pix, c1);
+            float u1 = next1d(L.smp, pix, c1);
             uint32_t li = (uint32_t)((u1 + 0.5f) * (float)(nLights - 1));
             if (li > nLights - 1) li = nLights - 1;
             bool envPick = sv.has_env && li == sv.n_sphere_lights;
             float ul[2];
-            next2d(L.seed, pix, c2, ul);
+            next2d(L.smp, pix, c2, ul);
             f3 wiL = mk3(0, 0, 0);
             float pdfL = 0;
             if (!envPick) {
@@ -504,7 +591,7 @@ WPBR_D void shade1_body(const RenderLaunch& L, uint32_t i) {
                     ws.b_ac[i] = c;
                 } else {
                     float ub[2];
-                    next2d(L.seed, pix, c2, ub);
+                    next2d(L.smp, pix, c2, ub);
                     if (L.record) { ws.r_ub[i] = ub[0]; ws.r_ub[n + i] = ub[1]; }
                     float phi = (float)((2 * PI_REF) * (double)ub[1]);
                     float tan2 = m.alpha * m.alpha * ub[0] / (1.0f - ub[0]);

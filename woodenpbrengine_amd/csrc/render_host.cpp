@@ -245,6 +245,7 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
     *out = nullptr;
     if (d->width == 0 || d->height == 0 || d->spp == 0 || (d->width % 32) || (d->height % 32))
         return set_err(ctx, WPBR_ERR_INVALID_ARG, "film must be a non-empty multiple of 32x32 tiles");
+    if (d->sampler > WPBR_SAMPLER_STRATIFIED_N) return set_err(ctx, WPBR_ERR_INVALID_ARG, "unknown sampler mode");
     uint32_t nLights = d->n_sphere_lights + (d->has_env ? 1u : 0u);
     if (nLights == 0) return set_err(ctx, WPBR_ERR_INVALID_ARG, "scene has no light");
     uint64_t n = (uint64_t)d->n_spheres + d->n_triangles;
@@ -260,6 +261,7 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
     sc->ctx = ctx;
     sc->spp = d->spp;
     sc->seed = d->seed;
+    sc->smp = make_sampler(d->seed, d->sampler, 16u + d->spp, 16u + 4u * d->spp);
     wpbr_status st = WPBR_OK;
     auto up = [&](const void* src, size_t bytes) -> void* {
         if (st) return nullptr;
@@ -413,7 +415,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     L.sv = sc->sv;
     L.n = n;
     L.spp = sc->spp;
-    L.seed = sc->seed;
+    L.smp = sc->smp;
     L.film = d_film;
     L.sample_xyz = d_sample_xyz;
     L.record = 0;

@@ -10,6 +10,9 @@ namespace wpbr {
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
 wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st);
+wpbr_status sample_arrays_launch(const SamplerDef& S, uint32_t pixel0, uint32_t npix, float* u1, float* u2,
+                                 hipStream_t st);
+SamplerDef make_sampler(uint32_t seed, uint32_t mode, uint32_t n1, uint32_t n2);
 // Batched stages over a device table of launch records: 0 camera, 1 shade1,
 // 2 radiance + splat, 3 clear films and queue statistics, 4 radiance + splat
 // with the spectral tables in LDS (every scene's tables fit: splat_tables_fit_h).
@@ -45,6 +48,7 @@ struct wpbr_scene {
     wpbr_geometry geom{};
     wpbr_lbvh* bvh = nullptr;
     uint32_t spp = 1, seed = 0;
+    wpbr::SamplerDef smp{};
     // parameters (host copies) and the writable buffers behind sv's pointers
     std::vector<float> h_materials;  // n_materials x 9, the descriptor's layout
     std::vector<float> h_env;        // env_h x env_w x 3

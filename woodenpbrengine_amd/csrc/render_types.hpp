@@ -98,13 +98,24 @@ struct AdjView {
     const float* env_dI;    // texels: derivative of the marginal integral by the green channel
 };
 
+// Where a render's sample values come from (wpbr_scene_desc::sampler): mode 0
+// independent hash streams; 1 and 2 the shuffled stratified arrays of n1 1D
+// and n2 2D values per pixel, 2D strata on a root x root grid scaled by inv2
+// (1/n2 for mode 1, 1/root for mode 2).
+struct SamplerDef {
+    uint32_t seed;
+    uint32_t mode;
+    uint32_t n1, n2, root;
+    float inv1, inv2;
+};
+
 struct RenderLaunch {
     SceneView sv;
     WaveState ws;
     uint32_t n;           // pixels
     uint32_t spp_index;
     uint32_t spp;
-    uint32_t seed;
+    SamplerDef smp;
     float* film;          // W*H*4: sum X w, sum Y w, sum Z w, sum w
     float* sample_xyz;    // optional: (pixel*spp + j) x 3
     uint32_t record;      // shade1 writes the adjoint's replay record

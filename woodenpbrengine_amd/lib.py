@@ -28,6 +28,7 @@ EXPORTS = [
     "wpbr_render_backward", "wpbr_bsdf_derivatives", "wpbr_lbvh_build_segmented",
     "wpbr_scenes_create", "wpbr_batch_create", "wpbr_batch_destroy", "wpbr_batch_render",
     "wpbr_batch_stats", "wpbr_autotune_batch", "wpbr_film_rgb8", "wpbr_write_png",
+    "wpbr_sample_arrays",
 ]
 
 
@@ -63,7 +64,7 @@ class SceneDesc(C.Structure):
                 ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env_xform", C.c_void_p),
                 ("raster_camera", C.c_void_p), ("camera_world", C.c_void_p),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("sampler", C.c_uint32)]
 
 
 _lib = None
@@ -123,6 +124,7 @@ def load():
         "wpbr_autotune_batch": (i32, [vp, vp, C.POINTER(TraceConfig)]),
         "wpbr_film_rgb8": (i32, [vp, u32, u32, vp, vp]),
         "wpbr_write_png": (i32, [C.c_char_p, u32, u32, vp]),
+        "wpbr_sample_arrays": (i32, [vp, u32, u32, u32, u32, u32, u32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -319,6 +321,21 @@ def autotune_collide(ctx, scene, bvh, d_rays, n):
     return {f: getattr(cfg, f) for f, _ in TraceConfig._fields_}
 
 
+SAMPLER_INDEPENDENT, SAMPLER_STRATIFIED, SAMPLER_STRATIFIED_N = 0, 1, 2
+
+
+def sample_arrays(ctx, seed, sampler, pixel0, npix, n1, n2):
+    """Per-pixel sample arrays generated on the device (wpbr_sample_arrays):
+    (npix x n1 floats, npix x n2 x 2 floats), each row in consumption order."""
+    d1 = ctx.empty((npix, n1), np.float32)
+    d2 = ctx.empty((npix, n2, 2), np.float32)
+    ctx.check(ctx.lib.wpbr_sample_arrays(ctx.h, seed, sampler, pixel0, npix, n1, n2, d1.ptr, d2.ptr))
+    out = d1.numpy(), d2.numpy()
+    d1.free()
+    d2.free()
+    return out
+
+
 def film_rgb8(ctx, d_film, width, height):
     """8-bit RGB (height x width x 3) of a device film of float4 (sum XYZ w, sum w)
     per pixel, computed on the device."""
@@ -350,7 +367,8 @@ def scene_desc(s):
     d = SceneDesc(len(sr), _ptr(sx), _ptr(sr), len(tv), _ptr(tv), _ptr(tf), _ptr(tuv), _ptr(tn),
                   _ptr(tt), _ptr(sm), len(mt), _ptr(mt), len(ls), _ptr(ls), _ptr(le),
                   1 if s.env is not None else 0, _ptr(ee), _ptr(er), er.shape[1], er.shape[0],
-                  _ptr(ex), _ptr(rc), _ptr(cw), s.width, s.height, s.spp, s.seed)
+                  _ptr(ex), _ptr(rc), _ptr(cw), s.width, s.height, s.spp, s.seed,
+                  getattr(s, "sampler", 0))
     return d, keep
 
 

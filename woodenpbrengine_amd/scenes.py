@@ -270,6 +270,7 @@ class RenderScene:
         self.cam = None
         self.width = self.height = self.spp = 0
         self.seed = 0
+        self.sampler = 0         # 0 independent streams, 1 / 2 stratified arrays
 
     def material(self, mtype, alpha=0.1, eta=1.5, eta_rgb=(1.5, 1.5, 1.5), k_rgb=(0, 0, 0)):
         m = np.zeros(9, np.float32)
