@@ -193,6 +193,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-microbench", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-stratified", action="store_true", help="skip the stratified-sampler render")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--trace-config", default=None,
                     help="fixed traversal configuration block,stack_depth,mapping,width,refill,chunk "
@@ -294,8 +295,23 @@ def main():
     f_ms = float(np.median(film_ms))
     f_gbs = 19.0 * npix / (f_ms * 1e-3) / 1e9
     log("film output: %.1f us per 8-bit image (%.0f GB/s)" % (f_ms * 1e3, f_gbs))
+    # the per-pixel sample arrays of this film (16 + spp 1D and 16 + 4 spp 2D values)
+    n1, n2 = 16 + scene.spp, 16 + 4 * scene.spp
+    d1, d2 = ctx.empty((npix, n1), np.float32), ctx.empty((npix, n2, 2), np.float32)
+    sa_ms = []
+    for _ in range(max(3, args.steps)):
+        ctx.check(ctx.lib.wpbr_sample_arrays(ctx.h, scene.seed, lib.SAMPLER_STRATIFIED_N, 0, npix, n1, n2,
+                                             d1.ptr, d2.ptr))
+        sa_ms.append(ctx.last_ms())
+    d1.free()
+    d2.free()
+    sa_ms = float(np.median(sa_ms))
+    sa_gbs = npix * (4.0 * n1 + 8.0 * n2) / (sa_ms * 1e-3) / 1e9
+    log("stratified sample arrays: %.1f us (%.0f GB/s written)" % (sa_ms * 1e3, sa_gbs))
     extra = {"film_output": {"us": f_ms * 1e3, "gbs": f_gbs, "hbm_frac": f_gbs / HBM_PEAK_GBS,
                              "pixels": npix},
+             "sample_arrays": {"us": sa_ms * 1e3, "gbs": sa_gbs, "hbm_frac": sa_gbs / HBM_PEAK_GBS,
+                               "n1": n1, "n2": n2, "pixels": npix},
              "image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
              "device_ms_per_step": float(np.mean(dev_ms)),
              "trace_ms_per_step": t_ms,
@@ -305,6 +321,21 @@ def main():
              "trace_config": tuned,
              "render_mrays_per_s": rays_per_step / (t_ms * 1e-3) / 1e6}
     r.destroy()
+    if not args.no_stratified:
+        # the same render with shuffled stratified sample arrays (computed where consumed)
+        scene.sampler = lib.SAMPLER_STRATIFIED_N
+        r2 = lib.Renderer(ctx, scene)
+        r2.render()
+        ctx.sync()
+        t0s = time.perf_counter()
+        for _ in range(args.steps):
+            r2.render()
+        ctx.sync()
+        st_ms = (time.perf_counter() - t0s) * 1e3 / args.steps
+        r2.destroy()
+        scene.sampler = 0
+        extra["render_stratified"] = {"ms_per_step": st_ms, "msamples_per_s": samples / (st_ms * 1e-3) / 1e6}
+        log("stratified render: %.2f ms/step" % st_ms)
     # Dominant kernel: k_trace.  Its launches in one render (a primary and a
     # secondary-queue launch per pass) timed with HIP events on the render stream.
     avg_launch_ms = t_ms / n_launch

@@ -101,18 +101,31 @@ WPBR_D void sample2_at(const SamplerDef& S, uint32_t pixel, uint32_t c, float* u
 WPBR_D float next1d(const SamplerDef& S, uint32_t pixel, uint32_t& c1) { return sample1_at(S, pixel, c1++); }
 WPBR_D void next2d(const SamplerDef& S, uint32_t pixel, uint32_t& c2, float* u) { sample2_at(S, pixel, c2++, u); }
 
-// The arrays themselves, one thread per element (wpbr_sample_arrays).
+// The arrays themselves (wpbr_sample_arrays): block b writes elements
+// [b per_block, (b + 1) per_block) of the npix x len array, each thread every
+// 256th; a thread divides once for its first (pixel, position) and then steps
+// by 256 = qs len + rs.
 __global__ __launch_bounds__(256) void k_sample_arrays(SamplerDef S, uint32_t pixel0, uint64_t n_el,
-                                                       uint32_t len, uint32_t dim, float* __restrict__ out) {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n_el; t += stride) {
-        const uint32_t p = (uint32_t)(t / len), c = (uint32_t)(t - (uint64_t)p * len);
+                                                       uint32_t len, uint32_t dim, uint64_t per_block,
+                                                       float* __restrict__ out) {
+    const uint64_t e0 = (uint64_t)blockIdx.x * per_block + threadIdx.x;
+    const uint64_t e1 = min(n_el, (uint64_t)(blockIdx.x + 1) * per_block);
+    if (e0 >= e1) return;
+    uint32_t p = (uint32_t)(e0 / len), c = (uint32_t)(e0 - (uint64_t)p * len);
+    const uint32_t qs = 256u / len, rs = 256u % len;
+    for (uint64_t e = e0; e < e1; e += 256) {
         if (dim == 1u) {
-            out[t] = sample1_at(S, pixel0 + p, c);
+            out[e] = sample1_at(S, pixel0 + p, c);
         } else {
             float u[2];
             sample2_at(S, pixel0 + p, c, u);
-            reinterpret_cast<float2*>(out)[t] = make_float2(u[0], u[1]);
+            reinterpret_cast<float2*>(out)[e] = make_float2(u[0], u[1]);
+        }
+        c += rs;
+        p += qs;
+        if (c >= len) {
+            c -= len;
+            p++;
         }
     }
 }
@@ -121,11 +134,13 @@ wpbr_status sample_arrays_launch(const SamplerDef& S, uint32_t pixel0, uint32_t 
                                  hipStream_t st) {
     for (uint32_t dim = 1; dim <= 2; dim++) {
         float* out = dim == 1 ? u1 : u2;
-        const uint64_t n_el = (uint64_t)npix * (dim == 1 ? S.n1 : S.n2);
+        const uint32_t len = dim == 1 ? S.n1 : S.n2;
+        const uint64_t n_el = (uint64_t)npix * len;
         if (!out || n_el == 0) continue;
-        const uint64_t blocks = std::min<uint64_t>((n_el + 255) / 256, 16384);
-        hipLaunchKernelGGL(k_sample_arrays, dim3((uint32_t)blocks), dim3(256), 0, st, S, pixel0, n_el,
-                           dim == 1 ? S.n1 : S.n2, dim, out);
+        const uint64_t blocks = std::min<uint64_t>((n_el + 255) / 256, 8192);
+        const uint64_t per_block = (n_el + blocks * 256 - 1) / (blocks * 256) * 256;
+        hipLaunchKernelGGL(k_sample_arrays, dim3((uint32_t)blocks), dim3(256), 0, st, S, pixel0, n_el, len, dim,
+                           per_block, out);
     }
     return launch_status();
 }
