@@ -294,6 +294,28 @@ wpbr_status wpbr_scene_set_params(wpbr_ctx* ctx, wpbr_scene* scene, const float*
  * output may be null. */
 wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
                            const float* d_target_rgb, float* d_dL_drgb, float* d_rgb, double* loss);
+/* A square RGB texture and its MIP pyramid on the device: CTexture2DRGB as
+ * SLoadTexture::load builds it (CSTextureProcess.h:267-345) from texels already
+ * linear (the reference applies gammaCorrectInv to its PNG's values first):
+ * log2(res) - 1 levels, each halving the previous one (2x2 means; the last
+ * column and row average the previous level's last column / row; the corner is
+ * copied).  rgb: res x res x 3 floats, host, row-major; res a power of two in
+ * [4, 16384] (the reference's 4x4 texel blocks then stay within every level). */
+typedef struct wpbr_texture wpbr_texture;
+wpbr_status wpbr_texture_create(wpbr_ctx* ctx, uint32_t res, const float* rgb, wpbr_texture** out);
+void wpbr_texture_destroy(wpbr_texture* tex);
+uint32_t wpbr_texture_levels(const wpbr_texture* tex);
+/* MIP level `level` to host memory ((res >> level)^2 x 3 floats). */
+wpbr_status wpbr_texture_level(wpbr_ctx* ctx, const wpbr_texture* tex, uint32_t level, float* rgb);
+/* STexture2DSamplerAnisotropic::eval (CSTextureProcess.h:134-251) with
+ * maxAnisotropy 32 (CMaterialMetal.cpp:28) and the 64-entry Gaussian table of
+ * JobFilterTableGaussing (CSTextureProcess.cpp:16-32, MEngine.cpp:92-94) for n
+ * texture-mapped points: d_query holds 6 floats each (s, t, ds/dx, dt/dx,
+ * ds/dy, dt/dy as CTextureMappedPoint, CTextureMapping.h:17-30), d_rgb receives
+ * 3 floats each. */
+wpbr_status wpbr_texture_lookup(wpbr_ctx* ctx, const wpbr_texture* tex, uint32_t n, const float* d_query,
+                                float* d_rgb);
+
 /* The film as an 8-bit RGB image (3 bytes per pixel, row-major, d_rgb8 on the
  * device): JobOutputFilmTitles (CSFilm.cpp:141-170) per pixel: black if the
  * accumulated radiance is black, else xyzToRGB(sum XYZ w) / (sum w * 800000),

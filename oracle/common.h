@@ -33,6 +33,7 @@ inline float cr_acos(float x) { return (float)std::acos((double)x); }
 inline float cr_log(float x) { return (float)std::log((double)x); }
 inline float cr_exp(float x) { return (float)std::exp((double)x); }
 inline float cr_pow(float x, float y) { return (float)std::pow((double)x, (double)y); }
+inline float cr_log2(float x) { return (float)std::log2((double)x); }
 
 struct V3 {
     float x, y, z;

@@ -289,6 +289,43 @@ def permute_index(c, n, key):
     return int(lib.orc_permute_index(c, n, key))
 
 
+def texture_levels(rgb):
+    """MIP levels of a square RGB texture (list of r x r x 3 arrays)."""
+    lib = load()
+    lib.orc_texture_levels.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.orc_texture_levels.restype = C.c_uint32
+    rgb = _f32(rgb)
+    res = rgb.shape[0]
+    n = int(lib.orc_texture_levels(res, _p(rgb), None))
+    out = np.empty(sum((res >> l) ** 2 * 3 for l in range(n)), np.float32)
+    lib.orc_texture_levels(res, _p(rgb), _p(out))
+    levels, o = [], 0
+    for l in range(n):
+        r = res >> l
+        levels.append(out[o:o + r * r * 3].reshape(r, r, 3))
+        o += r * r * 3
+    return levels
+
+
+def texture_filter():
+    lib = load()
+    lib.orc_texture_filter.argtypes = [C.c_void_p]
+    out = np.empty(64, np.float32)
+    lib.orc_texture_filter(_p(out))
+    return out
+
+
+def texture_lookup(rgb, queries):
+    """Anisotropic (EWA) lookups: queries n x 6 (s, t, ds/dx, dt/dx, ds/dy, dt/dy)."""
+    lib = load()
+    lib.orc_texture_lookup.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    rgb = _f32(rgb)
+    q = _f32(queries).reshape(-1, 6)
+    out = np.empty((len(q), 3), np.float32)
+    lib.orc_texture_lookup(rgb.shape[0], _p(rgb), len(q), _p(q), _p(out))
+    return out
+
+
 def samples(seed, pixel, n1, n2):
     lib = load()
     lib.orc_samples.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]

@@ -30,6 +30,7 @@ WPBR_D float cr_atan2(float y, float x) { return (float)atan2((double)y, (double
 WPBR_D float cr_acos(float x) { return (float)acos((double)x); }
 WPBR_D float cr_log(float x) { return (float)log((double)x); }
 WPBR_D float cr_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+WPBR_D float cr_log2(float x) { return (float)log2((double)x); }
 
 struct f3 { float x, y, z; };
 WPBR_HD f3 mk3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }

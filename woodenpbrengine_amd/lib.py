@@ -28,7 +28,8 @@ EXPORTS = [
     "wpbr_render_backward", "wpbr_bsdf_derivatives", "wpbr_lbvh_build_segmented",
     "wpbr_scenes_create", "wpbr_batch_create", "wpbr_batch_destroy", "wpbr_batch_render",
     "wpbr_batch_stats", "wpbr_autotune_batch", "wpbr_film_rgb8", "wpbr_write_png",
-    "wpbr_sample_arrays",
+    "wpbr_sample_arrays", "wpbr_texture_create", "wpbr_texture_destroy", "wpbr_texture_levels",
+    "wpbr_texture_level", "wpbr_texture_lookup",
 ]
 
 
@@ -125,6 +126,11 @@ def load():
         "wpbr_film_rgb8": (i32, [vp, u32, u32, vp, vp]),
         "wpbr_write_png": (i32, [C.c_char_p, u32, u32, vp]),
         "wpbr_sample_arrays": (i32, [vp, u32, u32, u32, u32, u32, u32, vp, vp]),
+        "wpbr_texture_create": (i32, [vp, u32, vp, C.POINTER(vp)]),
+        "wpbr_texture_destroy": (None, [vp]),
+        "wpbr_texture_levels": (u32, [vp]),
+        "wpbr_texture_level": (i32, [vp, vp, u32, vp]),
+        "wpbr_texture_lookup": (i32, [vp, vp, u32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -334,6 +340,42 @@ def sample_arrays(ctx, seed, sampler, pixel0, npix, n1, n2):
     d1.free()
     d2.free()
     return out
+
+
+class Texture:
+    """A device texture with its MIP pyramid (wpbr_texture)."""
+
+    def __init__(self, ctx, rgb):
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        if rgb.ndim != 3 or rgb.shape[0] != rgb.shape[1] or rgb.shape[2] != 3:
+            raise WpbrError(1, "texture must be res x res x 3")
+        self.ctx, self.res = ctx, rgb.shape[0]
+        h = C.c_void_p()
+        ctx.check(ctx.lib.wpbr_texture_create(ctx.h, self.res, _ptr(rgb), C.byref(h)))
+        self.h = h
+        self.n_levels = int(ctx.lib.wpbr_texture_levels(h))
+
+    def level(self, l):
+        r = self.res >> l
+        out = np.empty((r, r, 3), np.float32)
+        self.ctx.check(self.ctx.lib.wpbr_texture_level(self.ctx.h, self.h, l, _ptr(out)))
+        return out
+
+    def lookup(self, queries):
+        """queries: n x 6 (s, t, ds/dx, dt/dx, ds/dy, dt/dy) -> n x 3 RGB."""
+        q = np.ascontiguousarray(queries, dtype=np.float32).reshape(-1, 6)
+        d_q = self.ctx.array(q)
+        d_o = self.ctx.empty((len(q), 3), np.float32)
+        self.ctx.check(self.ctx.lib.wpbr_texture_lookup(self.ctx.h, self.h, len(q), d_q.ptr, d_o.ptr))
+        out = d_o.numpy()
+        d_q.free()
+        d_o.free()
+        return out
+
+    def destroy(self):
+        if self.h:
+            self.ctx.lib.wpbr_texture_destroy(self.h)
+            self.h = None
 
 
 def film_rgb8(ctx, d_film, width, height):
