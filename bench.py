@@ -308,10 +308,50 @@ def main():
     sa_ms = float(np.median(sa_ms))
     sa_gbs = npix * (4.0 * n1 + 8.0 * n2) / (sa_ms * 1e-3) / 1e9
     log("stratified sample arrays: %.1f us (%.0f GB/s written)" % (sa_ms * 1e3, sa_gbs))
+    # scene ingestion: the render's mesh as an OBJ reader hands it over (one vertex
+    # per corner, with normals) baked to world space and gathered into the triangle
+    # array; then the env map's 8-bit texels decoded to linear
+    nt = len(scene.tri_verts)
+    d_pos = ctx.array(scene.tri_verts.reshape(-1, 3))
+    with_nrm = len(scene.tri_normal) == nt
+    d_nrm = ctx.array(scene.tri_normal.reshape(-1, 3)) if with_nrm else None
+    d_idx = ctx.array(np.arange(3 * nt, dtype=np.uint32).reshape(-1, 3))
+    xf = np.concatenate([np.eye(3, 4), np.eye(3, 4)]).astype(np.float32).reshape(-1)
+    xf[3], xf[15] = 0.25, -0.25  # a translation and its inverse
+    bake_ms = []
+    for _ in range(max(3, args.steps)):
+        outs = lib.mesh_bake(ctx, xf, d_pos, 3 * nt, d_idx, nt, d_normals=d_nrm)
+        bake_ms.append(ctx.last_ms())
+        for a in outs:
+            if a is not None:
+                a.free()
+    for a in (d_pos, d_nrm, d_idx):
+        if a is not None:
+            a.free()
+    bake_ms = float(np.median(bake_ms))
+    # positions (and normals) read + written (36 B each way), indices 12 B, triangle array 36 B
+    bake_bytes = 192 if with_nrm else 120
+    bake_gbs = bake_bytes * nt / (bake_ms * 1e-3) / 1e9
+    log("mesh bake: %.1f us for %d triangles (%.0f GB/s)" % (bake_ms * 1e3, nt, bake_gbs))
+    env8 = np.random.default_rng(1).integers(0, 256, 3 * 1024 * 512, dtype=np.uint8)
+    d_env8 = ctx.array(env8)
+    dec_ms = []
+    for _ in range(max(3, args.steps)):
+        d_lin = lib.srgb8_to_linear(ctx, d_env8, env8.size)
+        dec_ms.append(ctx.last_ms())
+        d_lin.free()
+    d_env8.free()
+    dec_ms = float(np.median(dec_ms))
+    dec_gbs = 5.0 * env8.size / (dec_ms * 1e-3) / 1e9
+    log("texel decode: %.1f us for a 1024x512 RGB image (%.0f GB/s)" % (dec_ms * 1e3, dec_gbs))
     extra = {"film_output": {"us": f_ms * 1e3, "gbs": f_gbs, "hbm_frac": f_gbs / HBM_PEAK_GBS,
                              "pixels": npix},
              "sample_arrays": {"us": sa_ms * 1e3, "gbs": sa_gbs, "hbm_frac": sa_gbs / HBM_PEAK_GBS,
                                "n1": n1, "n2": n2, "pixels": npix},
+             "mesh_bake": {"us": bake_ms * 1e3, "gbs": bake_gbs, "hbm_frac": bake_gbs / HBM_PEAK_GBS,
+                           "triangles": nt, "bytes_per_triangle": bake_bytes},
+             "texel_decode": {"us": dec_ms * 1e3, "gbs": dec_gbs, "hbm_frac": dec_gbs / HBM_PEAK_GBS,
+                              "channels": int(env8.size)},
              "image_mean_rgb": [float(x) for x in np.nanmean(img, axis=(0, 1))],
              "device_ms_per_step": float(np.mean(dev_ms)),
              "trace_ms_per_step": t_ms,

@@ -347,6 +347,48 @@ wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* scene, const float* 
  * reflected direction: wi_z, dwi_z/dalpha.  No counterpart in the reference. */
 wpbr_status wpbr_bsdf_derivatives(wpbr_ctx* ctx, uint32_t n, const float* d_in, float* d_out);
 
+/* ---- scene ingestion (the formats before the path) ---------------------- */
+/* An OBJ mesh as MeshTools::loadMesh reads it through assimp (MeshTools.h:14-99,
+ * aiProcess_Triangulate | aiProcess_ConvertToLeftHanded): the first mesh of the
+ * file (faces up to the first o / g / usemtl line after a face); every face
+ * corner its own vertex; z of positions and normals negated, texture v
+ * replaced by 1 - v, each face's corner order reversed, then fanned from its
+ * first (reversed) corner.  Normals (uvs) are present only if every corner
+ * names one.  Host code only; an unreadable or malformed file is
+ * WPBR_ERR_INVALID_ARG. */
+typedef struct wpbr_obj wpbr_obj;
+wpbr_status wpbr_obj_load(const char* path, wpbr_obj** out);
+/* counts: [0] vertices, [1] indices (3 per triangle), [2] 1 if normals, [3] 1 if uvs. */
+wpbr_status wpbr_obj_counts(const wpbr_obj* obj, uint32_t counts[4]);
+/* Copies the mesh to host arrays (a null array is skipped): positions and
+ * normals 3 floats per vertex, uvs 2, indices u32 (CTriangleMesh::iVertices). */
+wpbr_status wpbr_obj_copy(const wpbr_obj* obj, float* positions, float* normals, float* uvs, uint32_t* indices);
+void wpbr_obj_free(wpbr_obj* obj);
+/* A PNG file as 8-bit RGB, what lodepng_decode24_file returns to readImagePNG
+ * (CSTextureProcess.h:462-496): every colour type and bit depth, Adam7 or not;
+ * alpha dropped, 16-bit samples to their high byte, grey replicated (1, 2, 4-bit
+ * grey scaled to 0..255), palette looked up; chunk CRCs and the zlib Adler-32
+ * checked.  rgb8 null: only *width / *height; otherwise cap >= 3 w h bytes.
+ * Host code only. */
+wpbr_status wpbr_read_png(const char* path, uint32_t* width, uint32_t* height, uint8_t* rgb8, size_t cap);
+/* STriangleMesh::create (CSTriangleMesh.cpp:11-39): the mesh to world space,
+ * positions as points by the world 3x4, normals by the transpose of its
+ * inverse, tangents as vectors, and the triangle vertex array wpbr_geometry
+ * takes: d_tri_verts[9 i + 3 k ..] = world position of d_indices[3 i + k] (what
+ * CSTriangle reads through CTriangleMesh::iVertices, CSTriangleMesh.h:24-29).
+ * xform: host, 24 floats (world 3x4 row-major, then its inverse).  d_normals,
+ * d_tangents and every world-space output may be null.  An index at or beyond
+ * n_vertices is WPBR_ERR_INVALID_ARG (its corner is written as zeros). */
+wpbr_status wpbr_mesh_bake(wpbr_ctx* ctx, const float* xform, uint32_t n_vertices, const float* d_positions,
+                           const float* d_normals, const float* d_tangents, uint32_t n_triangles,
+                           const uint32_t* d_indices, float* d_positions_world, float* d_normals_world,
+                           float* d_tangents_world, float* d_tri_verts);
+/* SLoadTexture::load's texel decode (CSTextureProcess.h:322-329 over
+ * readImagePNG's values, :480-490): d_rgb[i] = gammaCorrectInv(d_rgb8[i] /
+ * 255.0f) for n channel values: v / 12.92 up to 0.04045, else ((v + 0.055) /
+ * 1.055)^2.4f with the power correctly rounded. */
+wpbr_status wpbr_srgb8_to_linear(wpbr_ctx* ctx, size_t n, const uint8_t* d_rgb8, float* d_rgb);
+
 #ifdef __cplusplus
 }
 #endif

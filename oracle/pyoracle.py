@@ -386,3 +386,35 @@ class DiffOracle:
 
     def __del__(self):
         self.close()
+
+
+def mesh_bake(xform, positions, indices, normals=None, tangents=None):
+    """World-space bake of STriangleMesh::create plus the triangle vertex gather:
+    returns (positions_w, normals_w or None, tangents_w or None, tri_verts (nt, 9))."""
+    lib = load()
+    lib.orc_mesh_bake.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    xform = _f32(xform).reshape(24)
+    pos = _f32(positions).reshape(-1, 3)
+    idx = np.ascontiguousarray(indices, np.uint32).reshape(-1, 3)
+    assert idx.size == 0 or int(idx.max()) < len(pos)
+    nrm = None if normals is None else _f32(normals).reshape(-1, 3)
+    tan = None if tangents is None else _f32(tangents).reshape(-1, 3)
+    pw = np.empty_like(pos)
+    nw = None if nrm is None else np.empty_like(nrm)
+    tw = None if tan is None else np.empty_like(tan)
+    tri = np.empty((len(idx), 9), np.float32)
+    opt = lambda a: None if a is None else _p(a)  # noqa: E731
+    lib.orc_mesh_bake(_p(xform), len(pos), _p(pos), opt(nrm), opt(tan), len(idx), _p(idx), _p(pw), opt(nw),
+                      opt(tw), _p(tri))
+    return pw, nw, tw, tri
+
+
+def srgb8_to_linear(rgb8):
+    """gammaCorrectInv(b / 255.0f) of every byte (SLoadTexture::load's texel decode)."""
+    lib = load()
+    lib.orc_srgb8_to_linear.argtypes = [C.c_size_t, C.c_void_p, C.c_void_p]
+    a = np.ascontiguousarray(rgb8, np.uint8)
+    out = np.empty(a.shape, np.float32)
+    lib.orc_srgb8_to_linear(a.size, _p(a), _p(out))
+    return out

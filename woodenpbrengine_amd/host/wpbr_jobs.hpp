@@ -58,6 +58,13 @@ struct CSceneTreeLBVH {  // SLBVH.h:28-40
 };
 constexpr uint32_t INVALID_HANDLE = 0xFFFFFFFFu;
 
+// CSTriangleMesh.h: per-vertex arrays (3 floats; uvs 2) and iVertices.
+struct CTriangleMesh {
+    std::vector<float> positions, normals, tangents, uvs;
+    std::vector<uint32_t> iVertices;
+    uint32_t nTriangles = 0, nVertices = 0;
+};
+
 class Error : public std::runtime_error {
 public:
     Error(wpbr_status s, const std::string& what) : std::runtime_error(what), status(s) {}
@@ -115,6 +122,48 @@ public:
         geom_.d_sphere_xform = upload(sphereXform24, &d_xform_);
         geom_.d_sphere_radius = upload(sphereRadius, &d_radius_);
         geom_.d_tri_verts = upload(triVerts9, &d_tris_);
+    }
+
+    // STriangleMesh::create (CSTriangleMesh.cpp:11-39) on the device: the mesh
+    // to world space (world24 = world 3x4 then its inverse) and its triangles
+    // appended to triVerts9, the array setShapes takes.
+    void bakeTriangleMesh(const float* world24, CTriangleMesh& mesh, std::vector<float>& triVerts9) {
+        void *d_pos = nullptr, *d_nrm = nullptr, *d_tan = nullptr, *d_idx = nullptr, *d_tri = nullptr;
+        void *d_pw = nullptr, *d_nw = nullptr, *d_tw = nullptr;
+        const bool hasN = !mesh.normals.empty(), hasT = !mesh.tangents.empty();
+        const size_t vb = mesh.positions.size() * 4, tb = (size_t)mesh.nTriangles * 36;
+        auto up = [&](const void* src, size_t bytes, void** d) {
+            check(wpbr_malloc(ctx_, bytes, d), "wpbr_malloc");
+            check(wpbr_upload(ctx_, *d, src, bytes), "wpbr_upload");
+        };
+        auto release = [&] {
+            for (void* d : {d_pos, d_nrm, d_tan, d_idx, d_tri, d_pw, d_nw, d_tw})
+                if (d) wpbr_free(ctx_, d);
+        };
+        try {
+            up(mesh.positions.data(), vb, &d_pos);
+            up(mesh.iVertices.data(), mesh.iVertices.size() * 4, &d_idx);
+            if (hasN) up(mesh.normals.data(), vb, &d_nrm);
+            if (hasT) up(mesh.tangents.data(), vb, &d_tan);
+            check(wpbr_malloc(ctx_, vb, &d_pw), "wpbr_malloc");
+            if (hasN) check(wpbr_malloc(ctx_, vb, &d_nw), "wpbr_malloc");
+            if (hasT) check(wpbr_malloc(ctx_, vb, &d_tw), "wpbr_malloc");
+            check(wpbr_malloc(ctx_, tb, &d_tri), "wpbr_malloc");
+            check(wpbr_mesh_bake(ctx_, world24, mesh.nVertices, (const float*)d_pos, (const float*)d_nrm,
+                                 (const float*)d_tan, mesh.nTriangles, (const uint32_t*)d_idx, (float*)d_pw,
+                                 (float*)d_nw, (float*)d_tw, (float*)d_tri),
+                  "wpbr_mesh_bake");
+            check(wpbr_download(ctx_, mesh.positions.data(), d_pw, vb), "wpbr_download");
+            if (hasN) check(wpbr_download(ctx_, mesh.normals.data(), d_nw, vb), "wpbr_download");
+            if (hasT) check(wpbr_download(ctx_, mesh.tangents.data(), d_tw, vb), "wpbr_download");
+            const size_t at = triVerts9.size();
+            triVerts9.resize(at + (size_t)mesh.nTriangles * 9);
+            check(wpbr_download(ctx_, triVerts9.data() + at, d_tri, tb), "wpbr_download");
+        } catch (...) {
+            release();
+            throw;
+        }
+        release();
     }
 
     // MEngine.cpp:341-350
@@ -269,5 +318,28 @@ inline void JobProcessRayCastsResults::run(MEngine* e) { e->fetch_hits(); }
 inline void JobCollisionFinish::run(MEngine* e) { e->finish_interactions(); }
 
 #undef WPBR_JOB_RUN
+
+// MeshTools::loadMesh (MeshTools.h:14-99): the OBJ file's first mesh,
+// triangulated and left-handed as assimp hands it over.
+namespace MeshTools {
+inline CTriangleMesh loadMesh(const std::string& filename) {
+    wpbr_obj* obj = nullptr;
+    if (wpbr_obj_load(filename.c_str(), &obj) != WPBR_OK)
+        throw Error(WPBR_ERR_INVALID_ARG, "The file " + filename + " can't be opened");
+    uint32_t c[4];
+    wpbr_obj_counts(obj, c);
+    CTriangleMesh mesh;
+    mesh.nVertices = c[0];
+    mesh.nTriangles = c[1] / 3;
+    mesh.positions.resize(3 * (size_t)c[0]);
+    if (c[2]) mesh.normals.resize(3 * (size_t)c[0]);
+    if (c[3]) mesh.uvs.resize(2 * (size_t)c[0]);
+    mesh.iVertices.resize(c[1]);
+    wpbr_obj_copy(obj, mesh.positions.data(), c[2] ? mesh.normals.data() : nullptr,
+                  c[3] ? mesh.uvs.data() : nullptr, mesh.iVertices.data());
+    wpbr_obj_free(obj);
+    return mesh;
+}
+}  // namespace MeshTools
 
 }  // namespace wpbr_host

@@ -29,7 +29,8 @@ EXPORTS = [
     "wpbr_scenes_create", "wpbr_batch_create", "wpbr_batch_destroy", "wpbr_batch_render",
     "wpbr_batch_stats", "wpbr_autotune_batch", "wpbr_film_rgb8", "wpbr_write_png",
     "wpbr_sample_arrays", "wpbr_texture_create", "wpbr_texture_destroy", "wpbr_texture_levels",
-    "wpbr_texture_level", "wpbr_texture_lookup",
+    "wpbr_texture_level", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
+    "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
 ]
 
 
@@ -131,6 +132,13 @@ def load():
         "wpbr_texture_levels": (u32, [vp]),
         "wpbr_texture_level": (i32, [vp, vp, u32, vp]),
         "wpbr_texture_lookup": (i32, [vp, vp, u32, vp, vp]),
+        "wpbr_obj_load": (i32, [C.c_char_p, C.POINTER(vp)]),
+        "wpbr_obj_counts": (i32, [vp, C.POINTER(u32)]),
+        "wpbr_obj_copy": (i32, [vp, vp, vp, vp, vp]),
+        "wpbr_obj_free": (None, [vp]),
+        "wpbr_read_png": (i32, [C.c_char_p, C.POINTER(u32), C.POINTER(u32), vp, sz]),
+        "wpbr_mesh_bake": (i32, [vp, vp, u32, vp, vp, vp, u32, vp, vp, vp, vp, vp]),
+        "wpbr_srgb8_to_linear": (i32, [vp, sz, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -376,6 +384,70 @@ class Texture:
         if self.h:
             self.ctx.lib.wpbr_texture_destroy(self.h)
             self.h = None
+
+
+def load_obj(path):
+    """The first mesh of an OBJ file as MeshTools::loadMesh gets it through assimp:
+    dict of positions (nv x 3), normals / uvs (or None) and indices (nt x 3)."""
+    so = load()
+    h = C.c_void_p()
+    st = so.wpbr_obj_load(str(path).encode(), C.byref(h))
+    if st:
+        raise WpbrError(st, "cannot read OBJ mesh from %s" % path)
+    try:
+        c = (C.c_uint32 * 4)()
+        so.wpbr_obj_counts(h, c)
+        nv, ni = int(c[0]), int(c[1])
+        pos = np.empty((nv, 3), np.float32)
+        nrm = np.empty((nv, 3), np.float32) if c[2] else None
+        uvs = np.empty((nv, 2), np.float32) if c[3] else None
+        idx = np.empty((ni // 3, 3), np.uint32)
+        so.wpbr_obj_copy(h, _ptr(pos), _ptr(nrm) if nrm is not None else None,
+                         _ptr(uvs) if uvs is not None else None, _ptr(idx))
+    finally:
+        so.wpbr_obj_free(h)
+    return {"positions": pos, "normals": nrm, "uvs": uvs, "indices": idx}
+
+
+def read_png(path):
+    """A PNG file as an H x W x 3 uint8 image (what lodepng_decode24_file gives)."""
+    so = load()
+    w, h = C.c_uint32(), C.c_uint32()
+    st = so.wpbr_read_png(str(path).encode(), C.byref(w), C.byref(h), None, 0)
+    if st:
+        raise WpbrError(st, "cannot read PNG %s" % path)
+    out = np.empty((h.value, w.value, 3), np.uint8)
+    st = so.wpbr_read_png(str(path).encode(), C.byref(w), C.byref(h), _ptr(out), out.nbytes)
+    if st:
+        raise WpbrError(st, "cannot decode PNG %s" % path)
+    return out
+
+
+def mesh_bake(ctx, xform, d_positions, n_vertices, d_indices, n_triangles, d_normals=None, d_tangents=None,
+              world_vertices=True):
+    """STriangleMesh::create on the device: returns (d_tri_verts, d_positions_w,
+    d_normals_w, d_tangents_w) as DeviceArrays (None where not asked for)."""
+    xform = np.ascontiguousarray(xform, np.float32).reshape(24)
+    d_tri = ctx.empty((n_triangles, 9), np.float32)
+    d_pw = ctx.empty((n_vertices, 3), np.float32) if world_vertices else None
+    d_nw = ctx.empty((n_vertices, 3), np.float32) if (world_vertices and d_normals is not None) else None
+    d_tw = ctx.empty((n_vertices, 3), np.float32) if (world_vertices and d_tangents is not None) else None
+    p = lambda a: None if a is None else a.ptr  # noqa: E731
+    st = ctx.lib.wpbr_mesh_bake(ctx.h, _ptr(xform), n_vertices, d_positions.ptr, p(d_normals), p(d_tangents),
+                                n_triangles, d_indices.ptr, p(d_pw), p(d_nw), p(d_tw), d_tri.ptr)
+    if st:
+        for a in (d_tri, d_pw, d_nw, d_tw):
+            if a is not None:
+                a.free()
+        ctx.check(st)
+    return d_tri, d_pw, d_nw, d_tw
+
+
+def srgb8_to_linear(ctx, d_rgb8, n):
+    """gammaCorrectInv(b / 255) of n device bytes into a new float32 DeviceArray."""
+    d_out = ctx.empty((n,), np.float32)
+    ctx.check(ctx.lib.wpbr_srgb8_to_linear(ctx.h, n, d_rgb8.ptr, d_out.ptr))
+    return d_out
 
 
 def film_rgb8(ctx, d_film, width, height):
