@@ -49,23 +49,35 @@ __global__ __launch_bounds__(256) void k_bake_vertices(BakeArgs a) {
 // Corner k of triangle i is the world position of vertex idx[3 i + k]; the
 // transform is evaluated again at the corner (same arithmetic, same bits as the
 // baked vertex), so this kernel does not wait for k_bake_vertices.  An index
-// out of range is counted and its corner written as zeros.
+// out of range is counted and its corner written as zeros.  A workgroup's 256
+// triangles move through LDS: their indices are read and their 9 floats
+// written as contiguous dwords (strides 3 and 9 in LDS are free of bank
+// conflicts), one triangle per lane in between.
 __global__ __launch_bounds__(256) void k_bake_triangles(BakeArgs a) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_triangles) return;
-    float* o = a.tri + 9 * i;
+    __shared__ uint32_t s_idx[256 * 3];
+    __shared__ float s_out[256 * 9];
+    const size_t base = (size_t)blockIdx.x * 256;
+    const uint32_t nb = (uint32_t)min((size_t)256, (size_t)a.n_triangles - base);
+    for (uint32_t k = threadIdx.x; k < 3 * nb; k += 256) s_idx[k] = a.idx[3 * base + k];
+    __syncthreads();
+    const uint32_t t = threadIdx.x;
+    if (t < nb) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const uint32_t v = a.idx[3 * i + k];
-        f3 p = mk3(0.0f, 0.0f, 0.0f);
-        if (v < a.n_vertices)
-            p = xf_point(a.world, ld3(a.pos, v));
-        else
-            atomicAdd(a.bad, 1u);
-        o[3 * k] = p.x;
-        o[3 * k + 1] = p.y;
-        o[3 * k + 2] = p.z;
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = s_idx[3 * t + k];
+            f3 p = mk3(0.0f, 0.0f, 0.0f);
+            if (v < a.n_vertices)
+                p = xf_point(a.world, ld3(a.pos, v));
+            else
+                atomicAdd(a.bad, 1u);
+            s_out[9 * t + 3 * k] = p.x;
+            s_out[9 * t + 3 * k + 1] = p.y;
+            s_out[9 * t + 3 * k + 2] = p.z;
+        }
     }
+    __syncthreads();
+    float* o = a.tri + 9 * base;
+    for (uint32_t k = threadIdx.x; k < 9 * nb; k += 256) o[k] = s_out[k];
 }
 
 // gammaCorrectInv(i / 255.0f) (CSTextureProcess.h:517-522): value / 12.92 up to
