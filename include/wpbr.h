@@ -81,16 +81,33 @@ wpbr_status wpbr_shape_bounds(wpbr_ctx* ctx, const wpbr_geometry* geom, float* d
                               float* d_centroid);
 
 /* ---- LBVH (a2-a7) -------------------------------------------------------- */
+/* Key order of a build.  REFERENCE (the default) is what the reference's
+ * JobSortsShapesByMortonCode leaves in shapeMortonCodes for the rest of its
+ * jobs: its 5 x 6-bit passes alternate between two buffers and the last pass's
+ * output lands in the temporary one (SLBVH.cpp:245-291), so the order read by
+ * the gather, the treelet split (SLBVH.cpp:297-319) and emitLBVH (:374-387) is
+ * a stable sort on bits 0..23 only: treelets are runs of equal top-12 bits in
+ * that order (up to one per shape) and the upper SAH (:457-563) runs over all
+ * of them.  FULL: the complete 30-bit stable sort (at most 4096 treelets), the
+ * order the job was written to produce; a different tree with the same hits. */
+enum { WPBR_LBVH_ORDER_REFERENCE = 0, WPBR_LBVH_ORDER_FULL = 1 };
 wpbr_status wpbr_lbvh_create(wpbr_ctx* ctx, wpbr_lbvh** out);
 void wpbr_lbvh_destroy(wpbr_lbvh* bvh);
+/* The key order of the next builds of bvh (created with the context's
+ * default). */
+wpbr_status wpbr_lbvh_set_order(wpbr_lbvh* bvh, int order);
+int wpbr_lbvh_get_order(const wpbr_lbvh* bvh);
+/* The key order of LBVH objects created afterwards on ctx, including the ones
+ * wpbr_scene_create and wpbr_scenes_create make (default REFERENCE). */
+wpbr_status wpbr_set_default_lbvh_order(wpbr_ctx* ctx, int order);
 /* Builds the tree over the shapes of geom (bounds, centroid bounds, Morton
  * codes, radix sort, treelets, upper SAH, flatten, traversal layout).  Device
  * buffers are owned by the lbvh object and reused across builds. */
 wpbr_status wpbr_lbvh_build(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* bvh);
 /* Segmented build: the trees of n_scenes independent scenes (geoms[i] into
- * bvhs[i], each from wpbr_lbvh_create, all distinct), one kernel launch per
- * build kernel for all scenes together; each tree equals what wpbr_lbvh_build
- * gives for that scene alone.  Replaces MEngine::buildLBVH (MEngine.cpp:341-350)
+ * bvhs[i], each from wpbr_lbvh_create, all distinct, all of one key order),
+ * one kernel launch per build kernel for all scenes together; each tree equals
+ * what wpbr_lbvh_build gives for that scene alone.  Replaces MEngine::buildLBVH (MEngine.cpp:341-350)
  * run once per scene; the segmented form itself has no counterpart in the
  * reference.  At most 65535 scenes; wpbr_last_time_ms is the whole batch. */
 wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wpbr_geometry* geoms,

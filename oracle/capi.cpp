@@ -56,10 +56,11 @@ void orc_centroid_bounds(uint32_t n, const float* centroid, float* out6) {
     std::memcpy(out6, &b, 24);
 }
 
+// order: LBVH_ORDER_REFERENCE (0) or LBVH_ORDER_FULL (1), see oracle.h.
 void* orc_lbvh_build(uint32_t n, const float* bounds, const float* centroid,
-                     const uint32_t* entities) {
+                     const uint32_t* entities, int order) {
     LbvhResult* r = new LbvhResult();
-    lbvh_build(n, bounds, centroid, entities, *r);
+    lbvh_build(n, bounds, centroid, entities, *r, order);
     return r;
 }
 

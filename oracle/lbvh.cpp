@@ -4,7 +4,9 @@
 //   shape bounds/centroids   CSTriangle.cpp:116-142, CSSphere.h:136-152, CCentroid.h:10-13
 //   centroid bounds          SLBVH.cpp:164-181
 //   Morton codes             SLBVH.cpp:186-227
-//   6-bit LSD radix sort     SLBVH.cpp:231-295 (stable; 5 passes over 30 bits)
+//   6-bit LSD radix sort     SLBVH.cpp:231-295 (stable; 5 passes over 30 bits into
+//                            two buffers that alternate; see lbvh_build for which
+//                            buffer the rest of the job reads)
 //   treelets (top 12 bits)   SLBVH.cpp:297-322
 //   emitLBVH                 SLBVH.cpp:328-406
 //   upper SAH + flatten      SLBVH.cpp:410-563
@@ -216,7 +218,7 @@ Ref build_upper(Builder& B, uint32_t iStart, uint32_t iLast) {
 }  // namespace
 
 void lbvh_build(uint32_t n, const float* bounds, const float* centroid, const uint32_t* entities,
-                LbvhResult& out) {
+                LbvhResult& out, int order) {
     out = LbvhResult();
     if (n == 0) return;
     Bounds cb = centroid_bounds(n, centroid);
@@ -228,21 +230,36 @@ void lbvh_build(uint32_t n, const float* bounds, const float* centroid, const ui
         code[i] = out.morton[i];
         index[i] = i;
     }
-    // 6-bit LSD radix sort, 5 passes (SLBVH.cpp:245-283).
+    // 6-bit LSD radix sort, 5 passes (SLBVH.cpp:245-283).  Pass p reads `in`
+    // and writes `out`, chosen afresh from the pass parity at the top of every
+    // iteration (SLBVH.cpp:253-254): passes 0, 2, 4 go mortonCodes -> tmp and
+    // passes 1, 3 go tmp -> mortonCodes.  The closing std::swap(in, out)
+    // (SLBVH.cpp:283) exchanges the loop's locals only.  So the fully sorted
+    // keys end in tmp, which the job never reads again, and mortonCodes -- the
+    // array the gather (SLBVH.cpp:286-291), the treelet split (:297-319) and
+    // emitLBVH (:374-387) read -- holds pass 3's output: a stable sort on bits
+    // 0..23 only.  LBVH_ORDER_REFERENCE keeps that array; LBVH_ORDER_FULL takes
+    // tmp (the fully sorted order the job's loop was written to produce).
     if (n > 1) {
         std::vector<uint32_t> tc(n), ti(n);
         for (int pass = 0; pass < 5; pass++) {
+            uint32_t* in_c = (pass & 1) ? tc.data() : code.data();
+            uint32_t* in_i = (pass & 1) ? ti.data() : index.data();
+            uint32_t* out_c = (pass & 1) ? code.data() : tc.data();
+            uint32_t* out_i = (pass & 1) ? index.data() : ti.data();
             int lo = pass * 6;
             uint32_t cnt[64] = {0};
-            for (uint32_t i = 0; i < n; i++) cnt[(code[i] >> lo) & 63]++;
+            for (uint32_t i = 0; i < n; i++) cnt[(in_c[i] >> lo) & 63]++;
             uint32_t total = 0;
             for (int b = 0; b < 64; b++) { uint32_t t = cnt[b]; cnt[b] = total; total += t; }
             for (uint32_t i = 0; i < n; i++) {
-                uint32_t d = (code[i] >> lo) & 63;
+                uint32_t d = (in_c[i] >> lo) & 63;
                 uint32_t pos = cnt[d]++;
-                tc[pos] = code[i];
-                ti[pos] = index[i];
+                out_c[pos] = in_c[i];
+                out_i[pos] = in_i[i];
             }
+        }
+        if (order == LBVH_ORDER_FULL) {
             code.swap(tc);
             index.swap(ti);
         }

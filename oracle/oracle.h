@@ -72,8 +72,13 @@ void sphere_bounds(uint32_t n, const float* xform, const float* radius, float* b
                    float* centroid);
 Bounds centroid_bounds(uint32_t n, const float* centroid);
 uint32_t morton_code(const Bounds& cb, const float* c);
+// Key order of the LBVH build.  REFERENCE: what JobSortsShapesByMortonCode
+// leaves in shapeMortonCodes (pass 3 of 5: bits 0..23, stable), so treelets are
+// runs of equal top-12 bits in that order, up to n of them.  FULL: the complete
+// 30-bit stable sort (at most 4096 treelets).
+enum { LBVH_ORDER_REFERENCE = 0, LBVH_ORDER_FULL = 1 };
 void lbvh_build(uint32_t n, const float* bounds, const float* centroid, const uint32_t* entities,
-                LbvhResult& out);
+                LbvhResult& out, int order = LBVH_ORDER_REFERENCE);
 
 bool box_hit(const float* bmn, const float* bmx, V3 o, V3 inv, const int* neg, float tMax,
              float* tNearOut);
@@ -154,7 +159,8 @@ struct RenderOut {
     std::vector<float> film_rgb;     // width*height x 3 (toRGB(sum L w) / sum w)
     std::vector<float> film_xyz;     // width*height x 3 (sum of XYZ(L) w)
     std::vector<float> film_w;
-    uint64_t counters[8] = {0};      // primary hits, misses, shadow rays, bsdf rays, ...
+    uint64_t counters[8] = {0};      // primary hits, misses, shadow rays, bsdf rays, specular hits,
+                                     // clamped light picks, env picks, render ns
 };
 
 // Every discrete decision of one sample (the finite-difference reference of

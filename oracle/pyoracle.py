@@ -31,7 +31,7 @@ def load():
     vp, u32, f32 = C.c_void_p, C.c_uint32, C.c_float
     lib.orc_shape_bounds.argtypes = [u32, vp, vp, u32, vp, vp, vp]
     lib.orc_centroid_bounds.argtypes = [u32, vp, vp]
-    lib.orc_lbvh_build.argtypes = [u32, vp, vp, vp]
+    lib.orc_lbvh_build.argtypes = [u32, vp, vp, vp, C.c_int]
     lib.orc_lbvh_build.restype = vp
     lib.orc_lbvh_free.argtypes = [vp]
     lib.orc_lbvh_info.argtypes = [vp, vp]
@@ -84,15 +84,22 @@ def shape_bounds(sc):
     return b, c
 
 
+# Key orders of the LBVH build (oracle.h): the reference's (what its sort job
+# leaves in shapeMortonCodes: bits 0..23, stable) or the complete 30-bit sort.
+ORDER_REFERENCE = 0
+ORDER_FULL = 1
+
+
 class OLbvh:
-    def __init__(self, bounds, centroid, entities=None):
+    def __init__(self, bounds, centroid, entities=None, order=ORDER_REFERENCE):
         lib = load()
         self.lib = lib
         self.n = len(bounds)
         self.bounds = _f32(bounds, (-1, 6))
         self.centroid = _f32(centroid, (-1, 3))
         ent = None if entities is None else np.ascontiguousarray(entities, np.uint32)
-        self.h = lib.orc_lbvh_build(self.n, _p(self.bounds), _p(self.centroid), _p(ent))
+        self.order = order
+        self.h = lib.orc_lbvh_build(self.n, _p(self.bounds), _p(self.centroid), _p(ent), order)
         info = np.zeros(3, np.uint32)
         lib.orc_lbvh_info(self.h, _p(info))
         self.n_nodes, self.n_sah_nodes, self.n_treelets = (int(x) for x in info)
@@ -119,9 +126,9 @@ class OLbvh:
             pass
 
 
-def build_scene_bvh(sc):
+def build_scene_bvh(sc, order=ORDER_REFERENCE):
     b, c = shape_bounds(sc)
-    return OLbvh(b, c)
+    return OLbvh(b, c, order=order)
 
 
 def collide(sc, bvh, rays, nthreads=None):

@@ -595,8 +595,9 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     const uint32_t nLights = sc.n_sphere_lights + (sc.has_env ? 1u : 0u);
     float u1 = sample_1d(sc.smp, pixel, c1++);
     uint32_t li = (uint32_t)((u1 + 0.5f) * (float)(nLights - 1));
-    if (li > nLights - 1) li = nLights - 1;
+    if (li > nLights - 1) { li = nLights - 1; counters[5]++; }
     bool envPick = sc.has_env && li == sc.n_sphere_lights;
+    if (envPick) counters[6]++;
     // light sample (2D)
     float ul[2];
     sample_2d(sc.smp, pixel, c2++, ul);
@@ -614,6 +615,7 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     if (rec) { rec->kind = 2; rec->mid = mid; rec->li = li; rec->env_pick = envPick; rec->pdfL = pdfL; }
     const Material& m = sc.materials[mid];
     const MatSpec& ms = C.mats[mid];
+    if (m.type == MAT_SPECULAR) counters[4]++;
     // BSDF frame (CBSDF.cpp:12-24)
     V3 ss = normalize3(si.dpdu);
     V3 ns = si.ns;

@@ -3,8 +3,9 @@ repository root: python tests/golden/make_golden.py).  They freeze the oracle's
 outputs on small seeded inputs so that a change of the oracle, or of the GPU
 path, shows up against data that does not depend on building either.
 
-  lbvh_small.npz   300 triangles + 2 spheres: the LBVH arrays and the closest
-                   hits of 512 rays
+  lbvh_small.npz   300 triangles + 2 spheres: the LBVH arrays (the reference's
+                   key order) and the closest hits of 512 rays
+  lbvh_small_full.npz  the same with the full 30-bit key order
   render_tiny.npz  the plain scene at 32x32, 1 spp: per-sample XYZ and the film
 """
 import pathlib
@@ -34,18 +35,22 @@ def render_scene():
     return scenes.c0_scene(width=32, height=32, spp=1)
 
 
+LBVH_FIXTURES = {po.ORDER_REFERENCE: "lbvh_small.npz", po.ORDER_FULL: "lbvh_small_full.npz"}
+
+
 def main():
     sx, sr, tris, rays = lbvh_inputs()
     osc = po.OScene(sx, sr, tris)
-    ob = po.build_scene_bvh(osc)
-    hit, t, _, ovf = po.collide(osc, ob, rays)
-    assert not ovf
-    out = {k: ob.get(k) for k in ("morton", "sorted_index", "node_offset", "node_nshapes", "node_bounds",
-                                  "treelet_start")}
-    ns = out["node_nshapes"]
-    out["node_axis_interior"] = ob.get("node_axis")[ns == 0]
-    np.savez_compressed(HERE / "lbvh_small.npz", sphere_xform=sx, sphere_radius=sr, tri_verts=tris, rays=rays,
-                        hit_shape=hit, hit_t=t, **out)
+    for order, name in LBVH_FIXTURES.items():
+        ob = po.build_scene_bvh(osc, order=order)
+        hit, t, _, ovf = po.collide(osc, ob, rays)
+        assert not ovf
+        out = {k: ob.get(k) for k in ("morton", "sorted_index", "node_offset", "node_nshapes", "node_bounds",
+                                      "treelet_start")}
+        ns = out["node_nshapes"]
+        out["node_axis_interior"] = ob.get("node_axis")[ns == 0]
+        np.savez_compressed(HERE / name, sphere_xform=sx, sphere_radius=sr, tri_verts=tris, rays=rays,
+                            hit_shape=hit, hit_t=t, **out)
     r = po.render(render_scene(), nthreads=1)
     np.savez_compressed(HERE / "render_tiny.npz", sample_xyz=r["sample_xyz"], film_xyz=r["film_xyz"],
                         film_w=r["film_w"], film_rgb=r["film_rgb"])

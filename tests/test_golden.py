@@ -12,14 +12,18 @@ from woodenpbrengine_amd import lib, scenes
 G = pathlib.Path(__file__).resolve().parent / "golden"
 
 
-def _lbvh():
-    return np.load(G / "lbvh_small.npz")
+ORDERS = {po.ORDER_REFERENCE: "lbvh_small.npz", po.ORDER_FULL: "lbvh_small_full.npz"}
 
 
-def test_oracle_reproduces_lbvh_fixture():
-    f = _lbvh()
+def _lbvh(order):
+    return np.load(G / ORDERS[order])
+
+
+@pytest.mark.parametrize("order", list(ORDERS))
+def test_oracle_reproduces_lbvh_fixture(order):
+    f = _lbvh(order)
     osc = po.OScene(f["sphere_xform"], f["sphere_radius"], f["tri_verts"])
-    ob = po.build_scene_bvh(osc)
+    ob = po.build_scene_bvh(osc, order=order)
     for k in ("morton", "sorted_index", "node_offset", "node_nshapes", "node_bounds", "treelet_start"):
         np.testing.assert_array_equal(ob.get(k), f[k], err_msg=k)
     hit, t, _, _ = po.collide(osc, ob, f["rays"])
@@ -35,10 +39,11 @@ def test_oracle_reproduces_render_fixture():
 
 
 @pytest.mark.gpu
-def test_gpu_matches_lbvh_fixture(ctx):
-    f = _lbvh()
+@pytest.mark.parametrize("order", list(ORDERS))
+def test_gpu_matches_lbvh_fixture(ctx, order):
+    f = _lbvh(order)
     sc = lib.Scene(ctx, f["sphere_xform"], f["sphere_radius"], f["tri_verts"])
-    b = lib.Lbvh(ctx)
+    b = lib.Lbvh(ctx, order)
     b.build(sc)
     for k in ("morton", "sorted_index", "node_offset", "node_nshapes", "node_bounds", "treelet_start"):
         np.testing.assert_array_equal(b.download(k), f[k], err_msg=k)

@@ -90,3 +90,28 @@ def test_batch_autotune_keeps_films(ctx):
     finally:
         ctx.set_trace_config(**saved)
     br.destroy()
+
+
+def test_batch_c3_scenes_match_oracle(ctx):
+    """Scenes of the batched workload (displaced icospheres from per-scene seeds,
+    64 spheres, 4 sphere lights and an env map, the full render's material mix,
+    films of the batch's sizes) rendered in one batch: each film equals the
+    oracle's render of that scene (film RMSE below 1e-5 of the image RMS)."""
+    ss = scenes.c3_scenes(n_scenes=4, spp=2, max_triangles=30_000)
+    br = lib.BatchRenderer(ctx, ss)
+    br.render()
+    M = np.array([[3.240479, -1.537150, -0.498535], [-0.969256, 1.875991, 0.041556],
+                  [0.041556, -0.204043, 1.057311]])
+    for i, s in enumerate(ss):
+        f = br.film(i).astype(np.float64)
+        img = (f[..., :3] @ M.T) / f[..., 3:4]
+        ref = po.render(s)["film_rgb"].astype(np.float64)
+        ok = np.isfinite(ref).all(axis=-1) & np.isfinite(img).all(axis=-1)
+        rel = np.sqrt(np.mean((img[ok] - ref[ok]) ** 2)) / np.sqrt(np.mean(ref[ok] ** 2))
+        print("scene %d: %dx%d, %d shapes, film rmse / rms %.3e, non-finite pixels %d / %d"
+              % (i, s.width, s.height, s.n_shapes, rel, (~ok).sum(), (~np.isfinite(ref).all(axis=-1)).sum()),
+              flush=True)
+        assert rel < 1e-5
+        # the same pixels are non-finite on both sides
+        np.testing.assert_array_equal(np.isfinite(img).all(axis=-1), np.isfinite(ref).all(axis=-1))
+    br.destroy()

@@ -1,4 +1,5 @@
-"""GPU parity: LBVH build and closest-hit traversal against the oracle.
+"""GPU parity: LBVH build and closest-hit traversal against the oracle, in both
+key orders (the reference's bits-0..23 order, the default, and the full sort).
 
 Bit-exact: Morton codes, sort permutation, every node array (bounds compared as
 float32 values), hit shape ids; t equal bit for bit (the north-star bound is
@@ -60,14 +61,18 @@ TRACE_CONFIGS = [(128, 32, 1, 2, 16, 256), (64, 16, 1, 4, 16, 256), (256, 8, 2, 
 DEFAULT_TRACE = (64, 16, 1, 4, 16, 256)
 
 
+ORDERS = [lib.ORDER_REFERENCE, lib.ORDER_FULL]
+
+
+@pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("name", CASES)
-def test_lbvh_build_bit_exact(ctx, name):
+def test_lbvh_build_bit_exact(ctx, name, order):
     sx, sr, tris = _case(name)
-    print("case", name, len(sr), len(tris), flush=True)
+    print("case", name, len(sr), len(tris), "order", order, flush=True)
     osc = po.OScene(sx, sr, tris)
-    ob = po.build_scene_bvh(osc)
+    ob = po.build_scene_bvh(osc, order=order)
     sc = lib.Scene(ctx, sx, sr, tris)
-    b = lib.Lbvh(ctx)
+    b = lib.Lbvh(ctx, order)
     b.build(sc)
     c = b.counts()
     assert c["nodes"] == ob.n_nodes
@@ -87,18 +92,19 @@ def test_lbvh_build_bit_exact(ctx, name):
     b.destroy()
 
 
+@pytest.mark.parametrize("order", ORDERS)
 @pytest.mark.parametrize("name", CASES)
-def test_collide_matches_oracle(ctx, name):
+def test_collide_matches_oracle(ctx, name, order):
     sx, sr, tris = _case(name)
-    print("case", name, flush=True)
+    print("case", name, "order", order, flush=True)
     osc = po.OScene(sx, sr, tris)
-    ob = po.build_scene_bvh(osc)
+    ob = po.build_scene_bvh(osc, order=order)
     rays = np.concatenate([scenes.pinhole_rays(128, 64, (0.5, 0.5, -1.0)),
                            scenes.random_rays(8192, 21)], axis=1)
     os_, ot, _, ovf = po.collide(osc, ob, rays)
     assert not ovf
     sc = lib.Scene(ctx, sx, sr, tris)
-    b = lib.Lbvh(ctx)
+    b = lib.Lbvh(ctx, order)
     b.build(sc)
     d_rays = ctx.array(rays)
     try:
@@ -116,14 +122,19 @@ def test_collide_matches_oracle(ctx, name):
 
 
 def test_rebuild_reuses_buffers(ctx):
-    """Two builds in a row (the second smaller) give the oracle's trees."""
+    """Builds in a row (smaller, then larger, with the key order switched
+    between them) give the oracle's trees."""
     b = lib.Lbvh(ctx)
-    for n, seed in ((20000, 31), (700, 32)):
+    assert b.order == lib.ORDER_REFERENCE
+    for n, seed, order in ((20000, 31, lib.ORDER_REFERENCE), (700, 32, lib.ORDER_FULL),
+                           (700, 32, lib.ORDER_REFERENCE), (30000, 33, lib.ORDER_FULL)):
+        lib.load().wpbr_lbvh_set_order(b.h, order)
         tris = scenes.random_triangles(n, seed, median_edge=0.02)
         sx, sr = _spheres(2, seed)
         sc = lib.Scene(ctx, sx, sr, tris)
         b.build(sc)
-        ob = po.build_scene_bvh(po.OScene(sx, sr, tris))
+        ob = po.build_scene_bvh(po.OScene(sx, sr, tris), order=order)
+        assert b.counts()["treelets"] == ob.n_treelets
         np.testing.assert_array_equal(b.download("node_offset"), ob.get("node_offset"))
         np.testing.assert_array_equal(b.download("node_bounds"), ob.get("node_bounds"))
     b.destroy()
@@ -152,17 +163,18 @@ def test_autotune_collide(ctx):
     b.destroy()
 
 
-def test_staged_build_matches_and_checks_order(ctx):
+@pytest.mark.parametrize("order", ORDERS)
+def test_staged_build_matches_and_checks_order(ctx, order):
     """wpbr_lbvh_build_stage: the six jobs in order equal wpbr_lbvh_build; a
     stage out of order is rejected."""
     sx, sr, tris = _case("small")
     sc = lib.Scene(ctx, sx, sr, tris)
-    b = lib.Lbvh(ctx)
+    b = lib.Lbvh(ctx, order)
     with pytest.raises(lib.WpbrError):
         b.build_stage(sc, 3)
     for stage in range(6):
         b.build_stage(sc, stage)
-    ob = po.build_scene_bvh(po.OScene(sx, sr, tris))
+    ob = po.build_scene_bvh(po.OScene(sx, sr, tris), order=order)
     np.testing.assert_array_equal(b.download("node_offset"), ob.get("node_offset"))
     np.testing.assert_array_equal(b.download("node_bounds"), ob.get("node_bounds"))
     b.build_stage(sc, 0)
@@ -181,11 +193,13 @@ def _oracle_compare(b, ob):
     np.testing.assert_array_equal(b.download("node_axis")[ns == 0], ob.get("node_axis")[ns == 0])
 
 
-def test_segmented_build_matches_oracle(ctx):
+@pytest.mark.parametrize("order", ORDERS)
+def test_segmented_build_matches_oracle(ctx, order):
     """wpbr_lbvh_build_segmented: every scene's tree equals the oracle's (and so
     the single-scene build's), for scenes from 1 to 100 000 shapes in one batch;
     a second batch of other sizes reuses the same objects; collision on the
-    segmented trees gives the oracle's hits."""
+    segmented trees gives the oracle's hits; scenes of two key orders in one
+    batch are refused."""
     cases = [_case(n) for n in CASES]
     cases.append((np.zeros((0, 24), np.float32), np.zeros(0, np.float32), scenes.random_triangles(1, 41)))
     sx1, sr1 = _spheres(1, 42)
@@ -193,7 +207,7 @@ def test_segmented_build_matches_oracle(ctx):
     for k, n in enumerate((40, 900, 5000)):
         cases.append((*_spheres(k, 43 + k), scenes.random_triangles(n, 50 + k, median_edge=0.03)))
     scs = [lib.Scene(ctx, *c) for c in cases]
-    bvhs = [lib.Lbvh(ctx) for _ in cases]
+    bvhs = [lib.Lbvh(ctx, order) for _ in cases]
     print("segmented build of %d scenes" % len(scs), flush=True)
     lib.build_segmented(ctx, scs, bvhs)
     rays = np.concatenate([scenes.pinhole_rays(32, 32, (0.5, 0.5, -1.0)), scenes.random_rays(2048, 44)], axis=1)
@@ -201,7 +215,7 @@ def test_segmented_build_matches_oracle(ctx):
     for i, (c, sc, b) in enumerate(zip(cases, scs, bvhs)):
         print("scene", i, sc.n_shapes, flush=True)
         osc = po.OScene(*c)
-        ob = po.build_scene_bvh(osc)
+        ob = po.build_scene_bvh(osc, order=order)
         _oracle_compare(b, ob)
         os_, ot, _, ovf = po.collide(osc, ob, rays)
         d_s, d_t, _ = lib.collide(ctx, sc, b, d_rays, rays.shape[1])
@@ -213,8 +227,12 @@ def test_segmented_build_matches_oracle(ctx):
     scs2 = [lib.Scene(ctx, *c) for c in cases2]
     lib.build_segmented(ctx, scs2, bvhs[:3])
     for c, b in zip(cases2, bvhs[:3]):
-        _oracle_compare(b, po.build_scene_bvh(po.OScene(*c)))
+        _oracle_compare(b, po.build_scene_bvh(po.OScene(*c), order=order))
     with pytest.raises(lib.WpbrError):
         lib.build_segmented(ctx, scs2[:2], [bvhs[0], bvhs[0]])
+    other = lib.Lbvh(ctx, 1 - order)
+    with pytest.raises(lib.WpbrError):
+        lib.build_segmented(ctx, scs2[:2], [bvhs[0], other])
+    other.destroy()
     for b in bvhs:
         b.destroy()

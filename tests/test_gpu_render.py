@@ -15,8 +15,8 @@ pytestmark = pytest.mark.gpu
 REL_TOL = 1e-4
 
 
-def _compare(scene, ctx):
-    ref = po.render(scene)
+def _compare(scene, ctx, ref=None):
+    ref = po.render(scene) if ref is None else ref
     r = lib.Renderer(ctx, scene)
     r.render(keep_samples=True)
     got = r.d_samples.numpy()
@@ -51,5 +51,23 @@ def test_render_c0_full(ctx):
     """The CPU-runnable workload: 128x128, 4 spp (65 536 samples)."""
     s = scenes.c0_scene()
     frac_bad, rel_rmse, exact = _compare(s, ctx)
+    assert frac_bad == 0.0
+    assert rel_rmse < 1e-5
+
+
+def test_render_c2_mix_per_sample(ctx):
+    """The full render's material and light mix, reduced: 64 material groups
+    (40 % conductor, 40 % dielectric, 20 % perfect glass), 64 spheres, 4 sphere
+    lights plus the 1024x512 env map (5 lights: the light pick's clamp,
+    SScattering.cpp:82-88), on an 81 920-triangle displaced icosphere at
+    128x128, 2 spp.  The oracle's stage counters show that specular hits (the
+    w = 1 branch, CBSDFSpecularReflectance.cpp:63-93, SScattering.cpp:232),
+    clamped light picks and env picks all occur."""
+    s = scenes.c2_scene(width=128, height=128, spp=2, freq=64)
+    ref = po.render(s)
+    cnt = ref["counters"]
+    print("specular hits %d, clamped picks %d, env picks %d" % (cnt[4], cnt[5], cnt[6]), flush=True)
+    assert cnt[4] > 100 and cnt[5] > 100 and cnt[6] > 100
+    frac_bad, rel_rmse, exact = _compare(s, ctx, ref)
     assert frac_bad == 0.0
     assert rel_rmse < 1e-5

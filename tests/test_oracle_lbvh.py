@@ -58,15 +58,62 @@ def test_morton_closed_form():
     assert bvh.get("morton").max() < (1 << 30)
 
 
-def test_sort_is_stable_sort():
+def _reference_sort_buffers(m):
+    """The two buffers of JobSortsShapesByMortonCode (SLBVH.cpp:245-283) after
+    its loop: pass p of 5 counting-sorts 6 bits from `in` to `out`, where `in`
+    is tmp for odd passes and mortonCodes for even ones (:253-254, recomputed
+    at every iteration, so the std::swap at :283 has no effect).  Each pass is
+    a stable sort on its digit (np.argsort kind="stable")."""
+    bufs = {"mortonCodes": (m.copy(), np.arange(len(m), dtype=np.uint32)), "tmp": None}
+    for p in range(5):
+        src, dst = ("tmp", "mortonCodes") if p & 1 else ("mortonCodes", "tmp")
+        v, i = bufs[src]
+        o = np.argsort((v >> np.uint32(6 * p)) & np.uint32(63), kind="stable")
+        bufs[dst] = (v[o], i[o])
+    return bufs
+
+
+def _dup_scene():
     # duplicate codes on purpose: repeated triangles share centroids
     tris = np.repeat(scenes.random_triangles(300, 7, median_edge=0.03), 7, axis=0)
-    sc = po.OScene(np.zeros((0, 24)), np.zeros(0), tris)
-    bvh = po.build_scene_bvh(sc)
+    return po.OScene(np.zeros((0, 24)), np.zeros(0), tris)
+
+
+def test_reference_sort_keeps_pass3_buffer():
+    """What the reference's later steps read (mortonCodes) is a stable sort on
+    bits 0..23 only; the full 30-bit sort ends in the discarded tmp buffer."""
+    m = po.build_scene_bvh(_dup_scene()).get("morton")
+    bufs = _reference_sort_buffers(m)
+    low24 = np.argsort(m & np.uint32(0xFFFFFF), kind="stable")
+    np.testing.assert_array_equal(bufs["mortonCodes"][1], low24)
+    np.testing.assert_array_equal(bufs["tmp"][1], np.argsort(m, kind="stable"))
+    # the two differ on this scene (codes use bits 24..29)
+    assert not np.array_equal(low24, np.argsort(m, kind="stable"))
+
+
+@pytest.mark.parametrize("order", [po.ORDER_REFERENCE, po.ORDER_FULL])
+def test_sort_is_stable_sort(order):
+    bvh = po.build_scene_bvh(_dup_scene(), order=order)
     m = bvh.get("morton")
-    order = np.argsort(m, kind="stable")
-    np.testing.assert_array_equal(bvh.get("sorted_index"), order)
-    np.testing.assert_array_equal(bvh.get("sorted_code"), m[order])
+    key = m & np.uint32(0xFFFFFF) if order == po.ORDER_REFERENCE else m
+    o = np.argsort(key, kind="stable")
+    np.testing.assert_array_equal(bvh.get("sorted_index"), o)
+    np.testing.assert_array_equal(bvh.get("sorted_code"), m[o])
+
+
+def test_reference_order_treelets():
+    """Treelets are runs of equal top-12 bits in the bits-0..23 order: on random
+    codes nearly one per shape (a full sort gives at most 4096)."""
+    sc = _scene(20000, 21, 1)
+    ref = po.build_scene_bvh(sc)
+    full = po.build_scene_bvh(sc, order=po.ORDER_FULL)
+    m = ref.get("morton")
+    top = (m[np.argsort(m & np.uint32(0xFFFFFF), kind="stable")] >> 18).astype(np.int64)
+    runs = 1 + int((top[1:] != top[:-1]).sum())
+    assert ref.n_treelets == runs
+    assert ref.n_treelets > 15000
+    assert full.n_treelets <= 4096
+    assert ref.n_sah_nodes == ref.n_treelets - 1
 
 
 def _walk(bvh):
@@ -92,12 +139,13 @@ def _walk(bvh):
     return leaves
 
 
+@pytest.mark.parametrize("order", [po.ORDER_REFERENCE, po.ORDER_FULL])
 @pytest.mark.parametrize("dup", [1, 40])
-def test_tree_invariants(dup):
+def test_tree_invariants(dup, order):
     tris = np.repeat(scenes.random_triangles(3000 // dup, 9, median_edge=0.02), dup, axis=0)
     sc = po.OScene(np.zeros((0, 24)), np.zeros(0), tris)
     b, c = po.shape_bounds(sc)
-    bvh = po.OLbvh(b, c)
+    bvh = po.OLbvh(b, c, order=order)
     codes = bvh.get("sorted_code")
     idx = bvh.get("sorted_index")
     leaves = _walk(bvh)

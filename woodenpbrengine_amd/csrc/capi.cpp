@@ -42,6 +42,12 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     Carver c{base};
     const size_t N = std::max<uint32_t>(n, 1);
     const size_t tiles = (N + LBVH_SORT_TILE - 1) / LBVH_SORT_TILE;
+    // treelets number at most 4096 in the complete order and n in the reference order
+    const uint32_t CT = std::max<uint32_t>(4096, (uint32_t)N);
+    s.n = n;
+    s.cap_t = CT;
+    s.cap_r = ref_cap_r(CT);
+    s.cap_c = ref_cap_c(CT);
     s.slot_bounds = c.take<float>(6 * N);
     s.centroid = c.take<float>(3 * N);
     s.cb_ord = c.take<uint32_t>(8 + 6 * 512);
@@ -58,12 +64,12 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.err = c.take<uint32_t>(4);
     s.bin_start = c.take<uint32_t>(4096);
     s.presence = c.take<uint32_t>(128);
-    s.t_start = c.take<uint32_t>(4096);
-    s.t_count = c.take<uint32_t>(4096);
-    s.t_slot = c.take<uint32_t>(4096);
-    s.t_nodes = c.take<uint32_t>(4096);
-    s.t_final = c.take<uint32_t>(4096);
-    s.t_box = c.take<float>(6 * 4096);
+    s.t_start = c.take<uint32_t>(CT);
+    s.t_count = c.take<uint32_t>(CT);
+    s.t_slot = c.take<uint32_t>(CT);
+    s.t_nodes = c.take<uint32_t>(CT);
+    s.t_final = c.take<uint32_t>(CT);
+    s.t_box = c.take<float>(6 * (size_t)CT);
     s.r_start = c.take<uint32_t>(2 * N);
     s.r_n = c.take<uint32_t>(2 * N);
     s.r_c0 = c.take<uint32_t>(2 * N);
@@ -75,22 +81,22 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.task_i1 = c.take<uint32_t>(SAH_TASKS);
     s.task_dep = c.take<uint32_t>(SAH_TASKS);
     s.task_rt = c.take<uint32_t>(SAH_TASKS);
-    s.g_perm = c.take<uint32_t>(4096);
-    s.g_la = c.take<uint32_t>(4096);
-    s.g_lb = c.take<uint32_t>(4096);
-    s.sub_i0 = c.take<uint32_t>(4096);
-    s.sub_i1 = c.take<uint32_t>(4096);
-    s.sub_dep = c.take<uint32_t>(4096);
-    s.sub_rt = c.take<uint32_t>(4096);
+    s.g_perm = c.take<uint32_t>(CT);
+    s.g_la = c.take<uint32_t>(CT);
+    s.g_lb = c.take<uint32_t>(CT);
+    s.sub_i0 = c.take<uint32_t>(CT);
+    s.sub_i1 = c.take<uint32_t>(CT);
+    s.sub_dep = c.take<uint32_t>(CT);
+    s.sub_rt = c.take<uint32_t>(CT);
     s.sub_n = c.take<uint32_t>(1);
-    s.sub_ub = c.take<uint32_t>(4096);
-    s.u_i0 = c.take<uint32_t>(4096);
-    s.u_i1 = c.take<uint32_t>(4096);
-    s.u_mid = c.take<uint32_t>(4096);
-    s.u_dep = c.take<uint32_t>(4096);
-    s.u_rt = c.take<uint32_t>(4096);
-    s.u_axis = c.take<uint32_t>(4096);
-    s.u_box = c.take<float>(6 * 4096);
+    s.sub_ub = c.take<uint32_t>(CT);
+    s.u_i0 = c.take<uint32_t>(CT);
+    s.u_i1 = c.take<uint32_t>(CT);
+    s.u_mid = c.take<uint32_t>(CT);
+    s.u_dep = c.take<uint32_t>(CT);
+    s.u_rt = c.take<uint32_t>(CT);
+    s.u_axis = c.take<uint32_t>(CT);
+    s.u_box = c.take<float>(6 * (size_t)CT);
     s.upper_count = c.take<uint32_t>(1);
     s.n_nodes = c.take<uint32_t>(1);
     s.n_treelets = c.take<uint32_t>(1);
@@ -103,6 +109,21 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.prims = c.take<float4>(3 * N);
     s.big_list = c.take<uint32_t>(N / 64 + 1);
     s.big_count = c.take<uint32_t>(1);
+    // reference order
+    s.head_blk = c.take<uint32_t>((N + 255) / 256);
+    s.bigt = c.take<uint32_t>(N / 30 + 1);
+    s.bigt_n = c.take<uint32_t>(1);
+    s.gp_box = c.take<float>(6 * (size_t)CT);
+    s.gp_bucket = c.take<uint8_t>(CT);
+    s.pl_a = c.take<uint32_t>(CT);
+    s.pl_b = c.take<uint32_t>(CT);
+    s.lv_n = c.take<uint32_t>(4);
+    s.lv_rec = c.take<uint32_t>(2 * (size_t)s.cap_r * REF_RG_WORDS);
+    s.lv_stat = c.take<uint32_t>(2 * (size_t)s.cap_r * REF_ST_WORDS);
+    s.ch_range = c.take<uint32_t>(2 * (size_t)s.cap_c);
+    s.ch_cnt = c.take<uint32_t>(2 * (size_t)s.cap_c);
+    s.fin_p = c.take<uint32_t>((size_t)CT + 1);
+    s.fin_blk = c.take<uint32_t>((size_t)CT / REF_FIN_BLOCK + 1);
     return c.off + 256;
 }
 
@@ -124,15 +145,24 @@ static wpbr_status lbvh_prepare(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_l
         (geom->n_triangles && !geom->d_tri_verts))
         return set_err(ctx, WPBR_ERR_INVALID_ARG, "null geometry pointer");
     uint32_t n = (uint32_t)n64;
+    // the scratch arrays, then a device copy of their descriptor (the
+    // reference order's kernels read it)
     if (n > b->cap) {
         if (b->mem) { hipFree(b->mem); b->mem = nullptr; }
         LbvhScratch tmp;
-        size_t bytes = lbvh_layout(tmp, nullptr, n);
+        size_t bytes = ((lbvh_layout(tmp, nullptr, n) + 255) & ~size_t(255)) + sizeof(LbvhScratch);
         wpbr_status s = hip_check(ctx, hipMalloc(&b->mem, bytes), "hipMalloc lbvh");
         if (s) { b->cap = 0; return s; }
         b->cap = n;
     }
-    lbvh_layout(b->s, (char*)b->mem, n);
+    const size_t desc_off = (lbvh_layout(b->s, (char*)b->mem, b->cap) + 255) & ~size_t(255);
+    b->s.n = n;
+    b->s.order = (uint32_t)b->order;
+    b->d_self = reinterpret_cast<LbvhScratch*>((char*)b->mem + desc_off);
+    wpbr_status su = hip_check(ctx, hipMemcpyAsync(b->d_self, &b->s, sizeof(LbvhScratch), hipMemcpyHostToDevice,
+                                                   ctx->stream), "lbvh descriptor");
+    if (!su) su = hip_check(ctx, hipStreamSynchronize(ctx->stream), "lbvh descriptor");
+    if (su) return su;
     b->n = n;
     b->geom = to_view(geom);
     b->built = false;
@@ -266,6 +296,7 @@ wpbr_status wpbr_lbvh_create(wpbr_ctx* ctx, wpbr_lbvh** out) {
     if (!ctx || !out) return WPBR_ERR_INVALID_ARG;
     wpbr_lbvh* b = new wpbr_lbvh();
     b->ctx = ctx;
+    b->order = ctx->lbvh_order;
     hipSetDevice(ctx->device);
     for (int i = 0; i < wpbr_lbvh::NEV; i++) hipEventCreate(&b->ev[i]);
     *out = b;
@@ -279,6 +310,21 @@ void wpbr_lbvh_destroy(wpbr_lbvh* b) {
     if (b->mem) hipFree(b->mem);
     for (int i = 0; i < wpbr_lbvh::NEV; i++) hipEventDestroy(b->ev[i]);
     delete b;
+}
+
+wpbr_status wpbr_lbvh_set_order(wpbr_lbvh* b, int order) {
+    if (!b || (order != WPBR_LBVH_ORDER_REFERENCE && order != WPBR_LBVH_ORDER_FULL)) return WPBR_ERR_INVALID_ARG;
+    b->order = order;
+    b->built = false;
+    return WPBR_OK;
+}
+
+int wpbr_lbvh_get_order(const wpbr_lbvh* b) { return b ? b->order : -1; }
+
+wpbr_status wpbr_set_default_lbvh_order(wpbr_ctx* ctx, int order) {
+    if (!ctx || (order != WPBR_LBVH_ORDER_REFERENCE && order != WPBR_LBVH_ORDER_FULL)) return WPBR_ERR_INVALID_ARG;
+    ctx->lbvh_order = order;
+    return WPBR_OK;
 }
 
 wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr_lbvh* b, int stage) {
@@ -297,7 +343,7 @@ wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr
             g.sphere_xform != b->geom.sphere_xform || g.tri_verts != b->geom.tri_verts)
             return set_err(ctx, WPBR_ERR_INVALID_ARG, "geometry changed between lbvh stages");
     }
-    wpbr_status s = lbvh_launch_stage(b->s, b->geom, b->n, ctx->stream, stage, b->ev, wpbr_lbvh::NEV);
+    wpbr_status s = lbvh_launch_stage(b->s, b->d_self, b->geom, b->n, ctx->stream, stage, b->ev, wpbr_lbvh::NEV);
     if (s) {
         b->next_stage = 0;
         return set_err(ctx, s, "lbvh launch failed");
@@ -327,6 +373,8 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
         if (!bvhs[i]) return set_err(ctx, WPBR_ERR_INVALID_ARG, "null lbvh");
         for (uint32_t j = 0; j < i; j++)
             if (bvhs[j] == bvhs[i]) return set_err(ctx, WPBR_ERR_INVALID_ARG, "an lbvh appears twice");
+        if (bvhs[i]->order != bvhs[0]->order)
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "the scenes of a segmented build have one key order");
     }
     for (uint32_t i = 0; i < n_scenes; i++) {
         wpbr_status s = lbvh_prepare(ctx, &geoms[i], bvhs[i]);
@@ -358,7 +406,7 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
     const size_t o_s = reserve(sizeof(LbvhScratch) * n_scenes), o_g = reserve(sizeof(GeomView) * n_scenes),
                  o_n = reserve(4 * (size_t)n_scenes), o_work = reserve(4 * ((size_t)n_scenes + 1)),
                  o_subwork = reserve(4 * ((size_t)n_scenes + 1)),
-                 o_info = reserve(20 * (size_t)n_scenes);
+                 o_info = reserve(20 * (size_t)n_scenes + 4);  // + the level loop's range count
     size_t o_first[SEG_KINDS], o_count[SEG_KINDS], o_scene[SEG_KINDS];
     for (int k = 0; k < SEG_KINDS; k++) {
         o_first[k] = reserve(4 * (size_t)n_scenes);
@@ -398,7 +446,8 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
     if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "segmented build upload");
     if (s) return s;
     hipEventRecord(ctx->ev0, ctx->stream);
-    s = lbvh_launch_segmented(B, grid, ctx->stream);
+    const uint32_t max_n = *std::max_element(nv.begin(), nv.end());
+    s = lbvh_launch_segmented(B, grid, (uint32_t)bvhs[0]->order, max_n, ctx->stream);
     hipEventRecord(ctx->ev1, ctx->stream);
     if (s) return set_err(ctx, s, std::string("segmented lbvh launch failed: ") + hipGetErrorString(last_launch_error));
     std::vector<uint32_t> info(5 * (size_t)n_scenes);

@@ -449,41 +449,26 @@ constexpr int EMIT_MAX_LEVELS = 64;
 
 constexpr uint32_t EMIT_LDS_CODES = 1024;  // a treelet's codes are staged in LDS up to this size
 
-WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t bin) {
+// The emitLBVH tree of treelet t (slots start .. start + cnt - 1) by one
+// workgroup: breadth-first construction, then subtree sizes and bounds
+// bottom-up and DFS-preorder offsets top-down.  Its records sit at the
+// treelet's reserved slot range (2 cnt - 1 entries from 2 start - t: the sum of
+// 2 n_j - 1 over the treelets before it, SLBVH.cpp:307-310).
+WPBR_D void emit_tree(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t t, uint32_t start,
+                      uint32_t cnt) {
     __shared__ uint32_t s_codes[EMIT_LDS_CODES];
     __shared__ uint32_t s_q;
     __shared__ uint32_t s_lvl[EMIT_MAX_LEVELS + 2];
     __shared__ int s_nlvl;
-    __shared__ uint32_t s_tidx, s_start, s_cnt;
     const int tid = threadIdx.x;
-    if (!((s.presence[bin >> 5] >> (bin & 31)) & 1u)) return;
-    // the treelet's rank (occupied bins before it) and the next occupied bin,
-    // from the 128 presence words, two per lane (EMIT_BLOCK is one wave)
-    const uint32_t bw = bin >> 5, below = (1u << (bin & 31)) - 1u, above = ~((2u << (bin & 31)) - 1u);
-    const uint32_t q0 = 2 * (uint32_t)tid, q1 = q0 + 1;
-    const uint32_t p0 = s.presence[q0], p1 = s.presence[q1];
-    uint32_t cnt = (q0 < bw ? __popc(p0) : (q0 == bw ? __popc(p0 & below) : 0u)) +
-                   (q1 < bw ? __popc(p1) : (q1 == bw ? __popc(p1 & below) : 0u));
-    for (int d = 32; d >= 1; d >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, d);
-    const uint32_t m0 = q0 > bw ? p0 : (q0 == bw ? p0 & above : 0u);
-    const uint32_t m1 = q1 > bw ? p1 : (q1 == bw ? p1 & above : 0u);
-    const uint64_t has = ballot64((m0 | m1) != 0u);
-    const uint32_t nb_lane = m0 ? 32 * q0 + (uint32_t)(__ffs(m0) - 1) : 32 * q1 + (uint32_t)(__ffs(m1) - 1);
-    const uint32_t next_bin = has ? (uint32_t)__shfl((int)nb_lane, (int)(__ffsll((unsigned long long)has) - 1)) : 0u;
+    const uint32_t base = 2 * start - t;
     if (tid == 0) {
-        uint32_t t = cnt;
-        uint32_t start = s.bin_start[bin];
-        uint32_t end = has ? s.bin_start[next_bin] : n;
-        s_tidx = t;
-        s_start = start;
-        s_cnt = end - start;
         s.t_start[t] = start;
-        s.t_count[t] = end - start;
-        uint32_t base = 2 * start - t;  // sum of (2 n_j - 1) over earlier treelets
+        s.t_count[t] = cnt;
         s.t_slot[t] = base;
         // root task
         s.r_start[base] = start;
-        s.r_n[base] = end - start;
+        s.r_n[base] = cnt;
         s.r_meta[base] = 17u;  // first bit index = 30 - 12 - 1 (SLBVH.cpp:342)
         s_q = 1;
         s_lvl[0] = 0;
@@ -491,9 +476,9 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
         s_nlvl = 1;
     }
     __syncthreads();
-    const uint32_t base = 2 * s_start - s_tidx;
+    const uint32_t s_tidx = t;
     // the treelet's codes in LDS when they fit (the split searches read them)
-    const uint32_t tstart = s_start, tcnt = s_cnt;
+    const uint32_t tstart = start, tcnt = cnt;
     const bool staged = tcnt <= EMIT_LDS_CODES;
     if (staged)
         for (uint32_t k = tid; k < tcnt; k += EMIT_BLOCK) s_codes[k] = codes[tstart + k];
@@ -590,6 +575,35 @@ WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, 
         store_box(s.t_box + 6 * (size_t)s_tidx, b);
     }
 }
+// Treelet of occupied top-12-bit bin `bin` (the complete sort: one treelet per
+// occupied bin, its index the rank of the bin among the occupied ones).
+WPBR_D void emit_body(const LbvhScratch& s, const uint32_t* __restrict__ codes, uint32_t n, uint32_t bin) {
+    __shared__ uint32_t s_tidx, s_start, s_cnt;
+    const int tid = threadIdx.x;
+    if (!((s.presence[bin >> 5] >> (bin & 31)) & 1u)) return;
+    // the treelet's rank (occupied bins before it) and the next occupied bin,
+    // from the 128 presence words, two per lane (EMIT_BLOCK is one wave)
+    const uint32_t bw = bin >> 5, below = (1u << (bin & 31)) - 1u, above = ~((2u << (bin & 31)) - 1u);
+    const uint32_t q0 = 2 * (uint32_t)tid, q1 = q0 + 1;
+    const uint32_t p0 = s.presence[q0], p1 = s.presence[q1];
+    uint32_t cnt = (q0 < bw ? __popc(p0) : (q0 == bw ? __popc(p0 & below) : 0u)) +
+                   (q1 < bw ? __popc(p1) : (q1 == bw ? __popc(p1 & below) : 0u));
+    for (int d = 32; d >= 1; d >>= 1) cnt += (uint32_t)__shfl_xor((int)cnt, d);
+    const uint32_t m0 = q0 > bw ? p0 : (q0 == bw ? p0 & above : 0u);
+    const uint32_t m1 = q1 > bw ? p1 : (q1 == bw ? p1 & above : 0u);
+    const uint64_t has = ballot64((m0 | m1) != 0u);
+    const uint32_t nb_lane = m0 ? 32 * q0 + (uint32_t)(__ffs(m0) - 1) : 32 * q1 + (uint32_t)(__ffs(m1) - 1);
+    const uint32_t next_bin = has ? (uint32_t)__shfl((int)nb_lane, (int)(__ffsll((unsigned long long)has) - 1)) : 0u;
+    if (tid == 0) {
+        uint32_t start = s.bin_start[bin];
+        uint32_t end = has ? s.bin_start[next_bin] : n;
+        s_tidx = cnt;
+        s_start = start;
+        s_cnt = end - start;
+    }
+    __syncthreads();
+    emit_tree(s, codes, s_tidx, s_start, s_cnt);
+}
 __global__ __launch_bounds__(EMIT_BLOCK) void k_emit(LbvhScratch s, const uint32_t* __restrict__ codes, uint32_t n) {
     emit_body(s, codes, n, blockIdx.x);
 }
@@ -677,19 +691,22 @@ WPBR_D uint32_t sah_bucket(float c, float cmin, float dist) {
 // A task list (ranges i0..i1 of final positions with their depth and right
 // turns) and the counter that numbers upper nodes: global memory for the top
 // kernel, LDS for a workgroup of the sub-range kernel.
+// Positions, depths and right turns of the tasks are relative to pos_base,
+// dep_base and rt_base (a range run in LDS from local position 0).
 struct SahTasks {
     uint32_t* i0; uint32_t* i1; uint32_t* dep; uint32_t* rt;
     uint32_t* uctr;
+    uint32_t pos_base = 0, dep_base = 0, rt_base = 0;
 };
 
 WPBR_D uint32_t sah_node(LbvhScratch& s, const SahTasks& T, uint32_t i0, uint32_t i1, uint32_t mid, int dim,
                          const bbox& bounds, uint32_t dep, uint32_t rt) {
     uint32_t node = atomicAdd(T.uctr, 1u);
-    s.u_i0[node] = i0;
-    s.u_i1[node] = i1;
-    s.u_mid[node] = mid;
-    s.u_dep[node] = dep;
-    s.u_rt[node] = rt;
+    s.u_i0[node] = T.pos_base + i0;
+    s.u_i1[node] = T.pos_base + i1;
+    s.u_mid[node] = T.pos_base + mid;
+    s.u_dep[node] = T.dep_base + dep;
+    s.u_rt[node] = T.rt_base + rt;
     s.u_axis[node] = (uint32_t)dim;
     store_box(s.u_box + 6 * (size_t)node, bounds);
     return node;
@@ -1897,6 +1914,831 @@ WPBR_D void bvh4_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
 }
 __global__ __launch_bounds__(256) void k_bvh4(LbvhScratch s) { bvh4_body(s, blockIdx.x, gridDim.x); }
 
+// ===========================================================================
+// Reference key order (WPBR_LBVH_ORDER_REFERENCE).  The sort keeps bits 0..23
+// (three 8-bit passes), so treelets are runs of equal top-12 bits in that order
+// and number up to one per shape.  The kernels below take the build's scratch
+// through a device array (one entry per scene; block b of a launch with G
+// blocks per scene works for scene b / G), so the single-scene and the
+// segmented build share them.
+//   kr_heads / kr_head_scan / kr_treelets   treelet starts by a scan of the
+//                   run heads (SLBVH.cpp:299-319), the upper SAH's first range
+//   kr_emit_small / kr_emit_big   treelets below 30 shapes are one leaf
+//                   (SLBVH.cpp:356-367), larger ones get emitLBVH by a workgroup
+//   kr_lv_stats / kr_lv_split / kr_lv_lists / kr_lv_swap   one level of the
+//                   upper SAH (SLBVH.cpp:457-563) over every range of at least
+//                   REF_CUT roots: bucket statistics, the split (cost, first
+//                   minimum, midpoint fallback) and the libstdc++ two-pointer
+//                   std::partition in rank form (the r-th misplaced position
+//                   left of the split swaps with the r-th misplaced position
+//                   right of it counted from the right)
+//   kr_sub          ranges below REF_CUT, one workgroup each in LDS
+//   kr_fin_*        preorder layout with a device-wide prefix of the treelet
+//                   node counts; kr_scatter_ref writes the treelet nodes
+// Ranges and chunks of level k live in list k & 1; the level's split pushes
+// the next level's ranges into the other list.
+
+enum {
+    RG_I0 = 0, RG_I1, RG_DEP, RG_RT,
+    RG_CB,             // 6 order-preserving uints: centroid bounds min xyz, max xyz
+    RG_CH = RG_CB + 6, // first chunk
+    RG_NCH,            // chunks
+    RG_BEST,           // split bucket
+    RG_L,              // positions left of the split (= size: midpoint fallback)
+    RG_ML,             // misplaced pairs
+    RG_MID
+};
+static_assert(RG_MID < (int)REF_RG_WORDS, "range record");
+// Statistics: per bucket 13 words (count, box min xyz / max xyz, centroid min
+// xyz / max xyz as order-preserving uints), then the range's box.
+constexpr uint32_t ST_BOX = NB * 13;
+static_assert(ST_BOX + 6 <= REF_ST_WORDS, "statistics record");
+
+#define REF_SCENE(S, G)                                   \
+    const uint32_t sc = blockIdx.x / (G);                 \
+    const uint32_t rj = blockIdx.x - sc * (G);            \
+    const LbvhScratch& s = (S)[sc];
+
+WPBR_D uint32_t* ref_rec(const LbvhScratch& s, uint32_t list, uint32_t r) {
+    return s.lv_rec + ((size_t)list * s.cap_r + r) * REF_RG_WORDS;
+}
+WPBR_D uint32_t* ref_stat(const LbvhScratch& s, uint32_t list, uint32_t r) {
+    return s.lv_stat + ((size_t)list * s.cap_r + r) * REF_ST_WORDS;
+}
+// Initial value of statistics word k: 0 for counts, the empty box otherwise.
+WPBR_D uint32_t stat_init_word(uint32_t k) {
+    if (k < ST_BOX) {
+        const uint32_t j = k % 13;
+        if (j == 0) return 0u;
+        return (((j - 1) / 3) & 1u) ? f2ord(FLT_LOWEST_) : f2ord(FLT_MAX_);
+    }
+    if (k < ST_BOX + 3) return f2ord(FLT_MAX_);
+    if (k < ST_BOX + 6) return f2ord(FLT_LOWEST_);
+    return 0u;
+}
+WPBR_D bool stat_is_max(uint32_t k) {
+    return k < ST_BOX ? ((((k % 13) - 1) / 3) & 1u) != 0 : k >= ST_BOX + 3;
+}
+WPBR_D bbox ord_box(const uint32_t* w) {
+    bbox b;
+#pragma unroll
+    for (int c = 0; c < 3; c++) { b.mn[c] = ord2f(w[c]); b.mx[c] = ord2f(w[3 + c]); }
+    return b;
+}
+// Root-box centroid of a treelet on every axis (pMin + pMax) * 0.5 (SLBVH.cpp:472).
+WPBR_D f3 box_cen(const bbox& b) {
+    return mk3((float)((double)(b.mn[0] + b.mx[0]) * 0.5), (float)((double)(b.mn[1] + b.mx[1]) * 0.5),
+               (float)((double)(b.mn[2] + b.mx[2]) * 0.5));
+}
+
+// Block-wide exclusive scan of one value per thread (REF_BLOCK threads);
+// *total receives the sum.  Every thread must call it.
+constexpr int REF_BLOCK = 256;
+WPBR_D uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t* total) {
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t incl = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t x = (uint32_t)__shfl_up((int)incl, d);
+        if (lane >= d) incl += x;
+    }
+    if (lane == 63) s_w[w] = incl;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
+        const uint32_t x = s_w[k];
+        before += (k < w) ? x : 0u;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return before + incl - v;
+}
+
+WPBR_D bool ref_head(const uint32_t* codes, uint32_t i, uint32_t n) {
+    return i < n && (i == 0 || (codes[i] >> 18) != (codes[i - 1] >> 18));
+}
+
+// Treelet heads per 256 slots.
+__global__ __launch_bounds__(REF_BLOCK) void kr_heads(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t s_w[REF_BLOCK / 64];
+    const uint32_t n = s.n, nblk = (n + REF_BLOCK - 1) / REF_BLOCK;
+    for (uint32_t b = rj; b < nblk; b += G) {
+        const bool h = ref_head(s.keys_a, b * REF_BLOCK + threadIdx.x, n);
+        const uint64_t m = ballot64(h);
+        if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = popc64(m);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (int w = 0; w < REF_BLOCK / 64; w++) c += s_w[w];
+            s.head_blk[b] = c;
+        }
+        __syncthreads();
+    }
+}
+
+// One workgroup per scene: exclusive prefix of the head counts, the treelet
+// count, the upper SAH's counters and its first range (the whole position
+// range, as a level range, a deferred LDS range, or nothing for one treelet).
+__global__ __launch_bounds__(1024) void kr_head_scan(const LbvhScratch* S) {
+    const LbvhScratch& s = S[blockIdx.x];
+    __shared__ uint32_t s_sum[1024];
+    __shared__ uint32_t s_run;
+    const uint32_t nblk = (s.n + REF_BLOCK - 1) / REF_BLOCK;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_run = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nblk; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t v = i < nblk ? s.head_blk[i] : 0u;
+        s_sum[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t x = tid >= off ? s_sum[tid - off] : 0u;
+            __syncthreads();
+            s_sum[tid] += x;
+            __syncthreads();
+        }
+        if (i < nblk) s.head_blk[i] = s_run + s_sum[tid] - v;
+        __syncthreads();
+        if (tid == 0) s_run += s_sum[1023];
+        __syncthreads();
+    }
+    const uint32_t T = s_run;
+    if (tid == 0) {
+        *s.n_treelets = T;
+        *s.bigt_n = 0;
+        *s.sub_n = 0;
+        *s.upper_count = 0;
+        for (int k = 0; k < 4; k++) s.lv_n[k] = 0;
+        if (T == 1) {
+            s.g_la[0] = 0;
+            s.g_lb[0] = 0;
+        } else if (T < REF_CUT) {
+            s.sub_i0[0] = 0;
+            s.sub_i1[0] = T - 1;
+            s.sub_dep[0] = 0;
+            s.sub_rt[0] = 0;
+            s.sub_ub[0] = 0;
+            *s.sub_n = 1;
+            *s.upper_count = T - 1;
+        } else {
+            uint32_t* rec = ref_rec(s, 0, 0);
+            rec[RG_I0] = 0;
+            rec[RG_I1] = T - 1;
+            rec[RG_DEP] = 0;
+            rec[RG_RT] = 0;
+            for (int c = 0; c < 3; c++) {
+                rec[RG_CB + c] = f2ord(FLT_MAX_);
+                rec[RG_CB + 3 + c] = f2ord(FLT_LOWEST_);
+            }
+            rec[RG_CH] = 0;
+            rec[RG_NCH] = (T + REF_CHUNK - 1) / REF_CHUNK;
+            s.lv_n[0] = 1;
+            s.lv_n[2] = rec[RG_NCH];
+        }
+    }
+    if (T >= REF_CUT) {
+        const uint32_t nch = (T + REF_CHUNK - 1) / REF_CHUNK;
+        for (uint32_t c = tid; c < nch; c += 1024) s.ch_range[c] = 0;
+        uint32_t* st = ref_stat(s, 0, 0);
+        for (uint32_t k = tid; k < REF_ST_WORDS; k += 1024) st[k] = stat_init_word(k);
+    }
+}
+
+// Treelet starts: t_start[rank of the head] = slot.
+__global__ __launch_bounds__(REF_BLOCK) void kr_treelets(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t s_w[REF_BLOCK / 64];
+    const uint32_t n = s.n, nblk = (n + REF_BLOCK - 1) / REF_BLOCK;
+    for (uint32_t b = rj; b < nblk; b += G) {
+        const uint32_t i = b * REF_BLOCK + threadIdx.x;
+        const bool h = ref_head(s.keys_a, i, n);
+        uint32_t tot;
+        const uint32_t r = block_excl_scan(h ? 1u : 0u, s_w, &tot);
+        if (h) s.t_start[s.head_blk[b] + r] = i;
+    }
+}
+
+// Contributes a treelet root's centroid to the first level range's centroid
+// bounds (wave-reduced, then one atomic per coordinate).
+WPBR_D void ref_root_cb(const LbvhScratch& s, bool valid, const f3& c) {
+    uint32_t v[6];
+    v[0] = valid ? f2ord(c.x) : 0xFFFFFFFFu; v[1] = valid ? f2ord(c.y) : 0xFFFFFFFFu;
+    v[2] = valid ? f2ord(c.z) : 0xFFFFFFFFu;
+    v[3] = valid ? f2ord(c.x) : 0u; v[4] = valid ? f2ord(c.y) : 0u; v[5] = valid ? f2ord(c.z) : 0u;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int off = 32; off >= 1; off >>= 1) {
+            v[k] = min(v[k], (uint32_t)__shfl_xor((int)v[k], off));
+            v[3 + k] = max(v[3 + k], (uint32_t)__shfl_xor((int)v[3 + k], off));
+        }
+    const bool any = ballot64(valid) != 0;
+    if (lane_id() == 0 && any) {
+        uint32_t* rec = ref_rec(s, 0, 0);
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&rec[RG_CB + k], v[k]);
+            atomicMax(&rec[RG_CB + 3 + k], v[3 + k]);
+        }
+    }
+}
+
+// Treelets below 30 shapes: one leaf each (emitLBVH's first case,
+// SLBVH.cpp:356-367); larger ones are listed for kr_emit_big.  Positions start
+// as treelet order.
+__global__ __launch_bounds__(REF_BLOCK) void kr_emit_small(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    const uint32_t T = *s.n_treelets, n = s.n;
+    const uint32_t T_pad = (T + REF_BLOCK - 1) / REF_BLOCK * REF_BLOCK;
+    for (uint32_t t = rj * REF_BLOCK + threadIdx.x; t < T_pad; t += G * REF_BLOCK) {
+        bool leaf = false;
+        bbox b = box_empty();
+        if (t < T) {
+            const uint32_t start = s.t_start[t];
+            const uint32_t end = t + 1 < T ? s.t_start[t + 1] : n;
+            const uint32_t cnt = end - start;
+            if (cnt < (uint32_t)MAX_PRIMS_IN_LEAF) {
+                leaf = true;
+                bbox unused;
+                slot_range_boxes(s.slot_bounds, start, end, end, b, unused);
+                const uint32_t base = 2 * start - t;
+                s.r_start[base] = start;
+                s.r_n[base] = cnt;
+                s.r_meta[base] = 0x80000000u;
+                s.r_size[base] = 1;
+                s.r_off[base] = 0;
+                store_box(s.r_box + 6 * (size_t)base, b);
+                s.t_count[t] = cnt;
+                s.t_slot[t] = base;
+                s.t_nodes[t] = 1;
+                store_box(s.t_box + 6 * (size_t)t, b);
+                s.g_perm[t] = t;
+                store_box(s.gp_box + 6 * (size_t)t, b);
+            } else {
+                s.bigt[atomicAdd(s.bigt_n, 1u)] = t;
+            }
+        }
+        if (T >= REF_CUT) ref_root_cb(s, leaf, box_cen(b));
+    }
+}
+
+// Treelets of at least 30 shapes: emitLBVH by one workgroup each.
+__global__ __launch_bounds__(EMIT_BLOCK) void kr_emit_big(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    const uint32_t nbig = *s.bigt_n, T = *s.n_treelets, n = s.n;
+    for (uint32_t q = rj; q < nbig; q += G) {
+        const uint32_t t = s.bigt[q];
+        const uint32_t start = s.t_start[t];
+        const uint32_t end = t + 1 < T ? s.t_start[t + 1] : n;
+        emit_tree(s, s.keys_a, t, start, end - start);
+        __syncthreads();
+        bbox b = box_empty();
+        if (threadIdx.x == 0) {
+            b = load_box(s.t_box + 6 * (size_t)t);
+            s.g_perm[t] = t;
+            store_box(s.gp_box + 6 * (size_t)t, b);
+        }
+        if (T >= REF_CUT) ref_root_cb(s, threadIdx.x == 0, box_cen(b));
+        __syncthreads();
+    }
+}
+
+// ---- one level of the upper SAH over the large ranges ----------------------
+struct RefRange {
+    uint32_t r, i0, i1, ch0, nch, p0, p1;
+    const uint32_t* rec;
+};
+WPBR_D RefRange ref_range(const LbvhScratch& s, uint32_t list, uint32_t c) {
+    RefRange R;
+    R.r = s.ch_range[(size_t)list * s.cap_c + c];
+    R.rec = ref_rec(s, list, R.r);
+    R.i0 = R.rec[RG_I0];
+    R.i1 = R.rec[RG_I1];
+    R.ch0 = R.rec[RG_CH];
+    R.nch = R.rec[RG_NCH];
+    R.p0 = R.i0 + (c - R.ch0) * REF_CHUNK;
+    R.p1 = min(R.p0 + REF_CHUNK, R.i1 + 1);
+    return R;
+}
+
+// Bucket of every position and the bucket statistics of every range
+// (SLBVH.cpp:465-500): counts, bucket boxes, bucket centroid bounds (for the
+// children's centroid bounds) and the range's box.
+__global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, uint32_t G, uint32_t level) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t st[REF_ST_WORDS];
+    const uint32_t cur = level & 1u;
+    const uint32_t nch = s.lv_n[2 + cur];
+    if (rj == 0 && threadIdx.x == 0) { s.lv_n[cur ^ 1u] = 0; s.lv_n[2 + (cur ^ 1u)] = 0; }
+    for (uint32_t c = rj; c < nch; c += G) {
+        const RefRange R = ref_range(s, cur, c);
+        for (uint32_t k = threadIdx.x; k < REF_ST_WORDS; k += REF_BLOCK) st[k] = stat_init_word(k);
+        const bbox cb = ord_box(R.rec + RG_CB);
+        const int dim = box_max_extent(cb);
+        const float cmin = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
+        const float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cmin;
+        __syncthreads();
+        uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+        for (uint32_t p = R.p0 + threadIdx.x; p < R.p1; p += REF_BLOCK) {
+            const bbox b = load_box(s.gp_box + 6 * (size_t)p);
+            const f3 cc = box_cen(b);
+            const uint32_t ib = sah_bucket(comp(cc, dim), cmin, dist);
+            s.gp_bucket[p] = (uint8_t)ib;
+            uint32_t* sb = st + 13 * ib;
+            atomicAdd(&sb[0], 1u);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const uint32_t lo = f2ord(b.mn[k]), hi = f2ord(b.mx[k]), ce = f2ord(comp(cc, k));
+                atomicMin(&sb[1 + k], lo);
+                atomicMax(&sb[4 + k], hi);
+                atomicMin(&sb[7 + k], ce);
+                atomicMax(&sb[10 + k], ce);
+                v[k] = min(v[k], lo);
+                v[3 + k] = max(v[3 + k], hi);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            for (int off = 32; off >= 1; off >>= 1) {
+                v[k] = min(v[k], (uint32_t)__shfl_xor((int)v[k], off));
+                v[3 + k] = max(v[3 + k], (uint32_t)__shfl_xor((int)v[3 + k], off));
+            }
+        if (lane_id() == 0)
+            for (int k = 0; k < 3; k++) {
+                atomicMin(&st[ST_BOX + k], v[k]);
+                atomicMax(&st[ST_BOX + 3 + k], v[3 + k]);
+            }
+        __syncthreads();
+        uint32_t* gs = ref_stat(s, cur, R.r);
+        for (uint32_t k = threadIdx.x; k < ST_BOX + 6; k += REF_BLOCK) {
+            const uint32_t x = st[k];
+            if (x == stat_init_word(k)) continue;
+            if (k < ST_BOX && k % 13 == 0) atomicAdd(&gs[k], x);
+            else if (stat_is_max(k)) atomicMax(&gs[k], x);
+            else atomicMin(&gs[k], x);
+        }
+        __syncthreads();
+    }
+}
+
+// The split of every range (SLBVH.cpp:504-553), made identically by every
+// workgroup of the range from its statistics; the range's first chunk records
+// the upper node and its children (next-level ranges, deferred LDS ranges or
+// single positions); every chunk counts its misplaced positions.
+__global__ __launch_bounds__(REF_BLOCK) void kr_lv_split(const LbvhScratch* S, uint32_t G, uint32_t level) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t s_split[4];
+    __shared__ uint32_t s_child[2][4];  // next-level range: index, first chunk, chunks, valid
+    __shared__ uint32_t s_w[REF_BLOCK / 64];
+    const uint32_t cur = level & 1u, nxt = cur ^ 1u;
+    const uint32_t nch = s.lv_n[2 + cur];
+    const int lane = threadIdx.x & 63;
+    for (uint32_t c = rj; c < nch; c += G) {
+        const RefRange R = ref_range(s, cur, c);
+        const uint32_t m = R.i1 - R.i0 + 1;
+        const uint32_t* gs = ref_stat(s, cur, R.r);
+        if (threadIdx.x < 64) {
+            const bbox bounds = ord_box(gs + ST_BOX);
+            float cost = INF;
+            uint32_t cnt_le = 0;
+            if (lane < NB - 1) {
+                bbox b0 = box_empty(), b1 = box_empty();
+                uint32_t c0 = 0, c1 = 0;
+                for (int j = 0; j < NB; j++) {
+                    const bbox bj = ord_box(gs + 13 * j + 1);
+                    const uint32_t cj = gs[13 * j];
+                    if (j <= lane) { b0 = box_union(b0, bj); c0 += cj; }
+                    else { b1 = box_union(b1, bj); c1 += cj; }
+                }
+                const float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+                const float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+                cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
+                cnt_le = c0;
+            }
+            const int best = sah_best(cost);
+            const uint32_t L = (uint32_t)__shfl((int)cnt_le, best);
+            if (lane == 0) {
+                s_split[0] = (uint32_t)best;
+                s_split[1] = L;
+                s_split[2] = (L == m) ? m / 2 + R.i0 : R.i0 + L;
+            }
+        }
+        __syncthreads();
+        const uint32_t best = s_split[0], L = s_split[1], mid = s_split[2];
+        const bool fallback = L == m;
+        if (c == R.ch0 && threadIdx.x < 2) {
+            const uint32_t dep = R.rec[RG_DEP], rt = R.rec[RG_RT];
+            const int k = (int)threadIdx.x;
+            uint32_t* rec = const_cast<uint32_t*>(R.rec);
+            if (k == 0) {
+                const bbox cb = ord_box(R.rec + RG_CB);
+                const uint32_t node = atomicAdd(s.upper_count, 1u);
+                s.u_i0[node] = R.i0;
+                s.u_i1[node] = R.i1;
+                s.u_mid[node] = mid;
+                s.u_dep[node] = dep;
+                s.u_rt[node] = rt;
+                s.u_axis[node] = (uint32_t)box_max_extent(cb);
+                store_box(s.u_box + 6 * (size_t)node, ord_box(gs + ST_BOX));
+                rec[RG_BEST] = best;
+                rec[RG_L] = L;
+                rec[RG_MID] = mid;
+                rec[RG_ML] = 0;
+            }
+            const uint32_t a = k == 0 ? R.i0 : mid, b = k == 0 ? mid - 1 : R.i1;
+            const uint32_t cdep = dep + 1, crt = rt + (uint32_t)k;
+            const uint32_t size = b - a + 1;
+            s_child[k][3] = 0;
+            if (size == 1) {
+                s.g_la[a] = cdep;
+                s.g_lb[a] = crt;
+            } else if (size < REF_CUT) {
+                const uint32_t e = atomicAdd(s.sub_n, 1u);
+                s.sub_i0[e] = a;
+                s.sub_i1[e] = b;
+                s.sub_dep[e] = cdep;
+                s.sub_rt[e] = crt;
+                s.sub_ub[e] = atomicAdd(s.upper_count, size - 1);
+            } else {
+                const uint32_t rr = atomicAdd(&s.lv_n[nxt], 1u);
+                const uint32_t cn = (size + REF_CHUNK - 1) / REF_CHUNK;
+                const uint32_t c0 = atomicAdd(&s.lv_n[2 + nxt], cn);
+                uint32_t* cr = ref_rec(s, nxt, rr);
+                cr[RG_I0] = a;
+                cr[RG_I1] = b;
+                cr[RG_DEP] = cdep;
+                cr[RG_RT] = crt;
+                cr[RG_CH] = c0;
+                cr[RG_NCH] = cn;
+                // children's centroid bounds: the union of their buckets' (all
+                // centroids coincide when the midpoint fallback splits)
+                for (int q = 0; q < 6; q++) {
+                    uint32_t x = R.rec[RG_CB + q];
+                    if (!fallback) {
+                        x = q < 3 ? 0xFFFFFFFFu : 0u;
+                        const int j0 = k == 0 ? 0 : (int)best + 1, j1 = k == 0 ? (int)best : NB - 1;
+                        for (int j = j0; j <= j1; j++) {
+                            if (gs[13 * j] == 0) continue;
+                            const uint32_t y = gs[13 * j + 7 + q];
+                            x = q < 3 ? min(x, y) : max(x, y);
+                        }
+                    }
+                    cr[RG_CB + q] = x;
+                }
+                s_child[k][0] = rr;
+                s_child[k][1] = c0;
+                s_child[k][2] = cn;
+                s_child[k][3] = 1;
+            }
+        }
+        __syncthreads();
+        if (c == R.ch0) {
+            for (int k = 0; k < 2; k++) {
+                if (!s_child[k][3]) continue;
+                const uint32_t rr = s_child[k][0], c0 = s_child[k][1], cn = s_child[k][2];
+                for (uint32_t q = threadIdx.x; q < cn; q += REF_BLOCK) s.ch_range[(size_t)nxt * s.cap_c + c0 + q] = rr;
+                uint32_t* st = ref_stat(s, nxt, rr);
+                for (uint32_t q = threadIdx.x; q < REF_ST_WORDS; q += REF_BLOCK) st[q] = stat_init_word(q);
+            }
+        }
+        // misplaced positions of this chunk
+        uint32_t cl = 0, cr = 0;
+        if (!fallback)
+            for (uint32_t p = R.p0 + threadIdx.x; p < R.p1; p += REF_BLOCK) {
+                const uint32_t ib = s.gp_bucket[p];
+                if (p < R.i0 + L) cl += ib > best;
+                else cr += ib <= best;
+            }
+        for (int off = 32; off >= 1; off >>= 1) {
+            cl += (uint32_t)__shfl_xor((int)cl, off);
+            cr += (uint32_t)__shfl_xor((int)cr, off);
+        }
+        if (lane == 0) s_w[threadIdx.x >> 6] = cl | (cr << 16);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t tl = 0, tr = 0;
+            for (int w = 0; w < REF_BLOCK / 64; w++) { tl += s_w[w] & 0xFFFFu; tr += s_w[w] >> 16; }
+            s.ch_cnt[2 * (size_t)c] = tl;
+            s.ch_cnt[2 * (size_t)c + 1] = tr;
+        }
+        __syncthreads();
+    }
+}
+static_assert(REF_CHUNK < 65536, "chunk counts packed in 16 bits");
+
+// Ranks of the misplaced positions: pl_a[i0 + r] = the r-th misplaced position
+// left of the split (from the left), pl_b[i0 + r] = the r-th misplaced
+// position right of it (from the right).
+constexpr uint32_t REF_PER = REF_CHUNK / REF_BLOCK;
+__global__ __launch_bounds__(REF_BLOCK) void kr_lv_lists(const LbvhScratch* S, uint32_t G, uint32_t level) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t s_w[REF_BLOCK / 64];
+    __shared__ uint32_t s_base[2];
+    const uint32_t cur = level & 1u;
+    const uint32_t nch = s.lv_n[2 + cur];
+    for (uint32_t c = rj; c < nch; c += G) {
+        const RefRange R = ref_range(s, cur, c);
+        const uint32_t m = R.i1 - R.i0 + 1, L = R.rec[RG_L], best = R.rec[RG_BEST];
+        if (L == m) continue;  // uniform over the workgroup
+        // misplaced counts of the range's chunks before (left) and after (right) this one
+        uint32_t bl = 0, br = 0;
+        for (uint32_t q = R.ch0 + threadIdx.x; q < R.ch0 + R.nch; q += REF_BLOCK) {
+            if (q < c) bl += s.ch_cnt[2 * (size_t)q];
+            if (q > c) br += s.ch_cnt[2 * (size_t)q + 1];
+        }
+        uint32_t tbl, tbr;
+        (void)block_excl_scan(bl, s_w, &tbl);
+        (void)block_excl_scan(br, s_w, &tbr);
+        const uint32_t split = R.i0 + L;
+        const uint32_t q0 = R.p0 + threadIdx.x * REF_PER;
+        uint32_t fl = 0, fr = 0;  // flags of this thread's positions
+#pragma unroll
+        for (uint32_t j = 0; j < REF_PER; j++) {
+            const uint32_t p = q0 + j;
+            if (p < R.p1) {
+                const uint32_t ib = s.gp_bucket[p];
+                if (p < split) fl |= (uint32_t)(ib > best) << j;
+                else fr |= (uint32_t)(ib <= best) << j;
+            }
+        }
+        uint32_t tl, tr;
+        const uint32_t el = block_excl_scan(__popc(fl), s_w, &tl);
+        const uint32_t er = block_excl_scan(__popc(fr), s_w, &tr);
+        uint32_t kl = 0, kr = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < REF_PER; j++) {
+            const uint32_t p = q0 + j;
+            if ((fl >> j) & 1u) s.pl_a[R.i0 + tbl + el + kl++] = p;
+            if ((fr >> j) & 1u) {
+                kr++;
+                // right-misplaced positions after p: in later chunks, then in this one
+                s.pl_b[R.i0 + tbr + (tr - (er + kr))] = p;
+            }
+        }
+        if (c == R.ch0 + R.nch - 1 && threadIdx.x == 0) const_cast<uint32_t*>(R.rec)[RG_ML] = tbl + tl;
+    }
+}
+
+// The swaps of the two-pointer partition: pair r exchanges pl_a[i0 + r] and
+// pl_b[i0 + r] (treelet and root box).
+__global__ __launch_bounds__(REF_BLOCK) void kr_lv_swap(const LbvhScratch* S, uint32_t G, uint32_t level) {
+    REF_SCENE(S, G);
+    const uint32_t cur = level & 1u;
+    const uint32_t nch = s.lv_n[2 + cur];
+    for (uint32_t c = rj; c < nch; c += G) {
+        const RefRange R = ref_range(s, cur, c);
+        const uint32_t m = R.i1 - R.i0 + 1, L = R.rec[RG_L];
+        if (L == m) continue;
+        const uint32_t ml = R.rec[RG_ML];
+        for (uint32_t q = R.p0 + threadIdx.x; q < R.p1; q += REF_BLOCK) {
+            if (q - R.i0 >= ml) break;
+            const uint32_t a = s.pl_a[q], b = s.pl_b[q];
+            const uint32_t ta = s.g_perm[a], tb = s.g_perm[b];
+            s.g_perm[a] = tb;
+            s.g_perm[b] = ta;
+            const bbox ba = load_box(s.gp_box + 6 * (size_t)a), bb = load_box(s.gp_box + 6 * (size_t)b);
+            store_box(s.gp_box + 6 * (size_t)a, bb);
+            store_box(s.gp_box + 6 * (size_t)b, ba);
+        }
+    }
+}
+
+// Sum of the ranges of the next level over all scenes (the host's loop ends
+// when it is 0).
+__global__ void kr_active(const LbvhScratch* S, uint32_t n_scenes, uint32_t list, uint32_t* out) {
+    __shared__ uint32_t s_t;
+    if (threadIdx.x == 0) s_t = 0;
+    __syncthreads();
+    uint32_t v = 0;
+    for (uint32_t i = threadIdx.x; i < n_scenes; i += blockDim.x) v += S[i].lv_n[list];
+    atomicAdd(&s_t, v);
+    __syncthreads();
+    if (threadIdx.x == 0) *out = s_t;
+}
+
+// ---- ranges below REF_CUT: one workgroup each, in LDS ----------------------
+// The range's positions are local 0 .. m - 1 (their boxes staged from gp_box);
+// the recorded nodes and leaf depths are offset back to the range.
+WPBR_D void sah_sub_ref_one(LbvhScratch& s, SahSubLds& S, uint32_t st) {
+    SahLds& L = S.L;
+    const int tid = threadIdx.x;
+    SahTasks T{S.ti0, S.ti1, S.tdep, S.trt, &S.uctr};
+    if (tid == 0) {
+        S.r[0] = s.sub_i0[st];
+        S.r[1] = s.sub_i1[st];
+        S.r[2] = s.sub_dep[st];
+        S.r[3] = s.sub_rt[st];
+        S.ti0[0] = 0;
+        S.ti1[0] = S.r[1] - S.r[0];
+        S.tdep[0] = 0;
+        S.trt[0] = 0;
+        S.uctr = s.sub_ub[st];
+        L.task_next = 1;
+        L.nbig = 0;
+        L.nbig_next = 0;
+    }
+    __syncthreads();
+    const uint32_t i0 = S.r[0], m = S.r[1] - S.r[0] + 1;
+    T.pos_base = i0;
+    T.dep_base = S.r[2];
+    T.rt_base = S.r[3];
+    for (uint32_t q = tid; q < m; q += SAH_BLOCK) {
+        L.perm[q] = (uint16_t)q;
+        for (int c = 0; c < 6; c++) L.tbox[6 * q + c] = s.gp_box[6 * (size_t)(i0 + q) + c];
+    }
+    __threadfence_block();
+    __syncthreads();
+    sah_run(s, L, T, 0, 1, LANE_TASK_MAX_SUB, 0);
+    for (uint32_t q = tid; q < m; q += SAH_BLOCK) L.aux[q] = s.g_perm[i0 + q];
+    __syncthreads();
+    for (uint32_t q = tid; q < m; q += SAH_BLOCK) {
+        s.g_perm[i0 + q] = L.aux[L.perm[q]];
+        s.g_la[i0 + q] = T.dep_base + L.la[q];
+        s.g_lb[i0 + q] = T.rt_base + L.lb[q];
+    }
+    __syncthreads();
+}
+static_assert(REF_CUT <= SAH_CUT, "deferred ranges fit the sub-range kernel's task list");
+__global__ __launch_bounds__(SAH_BLOCK) void kr_sub(const LbvhScratch* S, uint32_t G) {
+    __shared__ SahSubLds L;
+    const uint32_t sc = blockIdx.x / G, rj = blockIdx.x - sc * G;
+    LbvhScratch s = S[sc];
+    const uint32_t nsub = *s.sub_n;
+    for (uint32_t st = rj; st < nsub; st += G) sah_sub_ref_one(s, L, st);
+}
+
+// ---- preorder layout ---------------------------------------------------------
+// P = exclusive prefix of t_nodes over final positions; a treelet at final
+// position i with leaf depth d and r right turns starts at d + (i - r) + P[i],
+// an upper node [i0, i1] split at mid at dep + (i0 - rt) + P[i0] with its
+// second child after (mid - 1 - i0) + P[mid] - P[i0] more nodes.
+constexpr uint32_t REF_FIN_PER = REF_FIN_BLOCK / REF_BLOCK;
+__global__ __launch_bounds__(REF_BLOCK) void kr_fin_sum(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t s_w[REF_BLOCK / 64];
+    const uint32_t T = *s.n_treelets, nb = (T + REF_FIN_BLOCK - 1) / REF_FIN_BLOCK;
+    for (uint32_t b = rj; b < nb; b += G) {
+        uint32_t v = 0;
+        for (uint32_t j = 0; j < REF_FIN_PER; j++) {
+            const uint32_t i = b * REF_FIN_BLOCK + threadIdx.x * REF_FIN_PER + j;
+            if (i < T) v += s.t_nodes[s.g_perm[i]];
+        }
+        uint32_t tot;
+        (void)block_excl_scan(v, s_w, &tot);
+        if (threadIdx.x == 0) s.fin_blk[b] = tot;
+    }
+}
+__global__ __launch_bounds__(1024) void kr_fin_scan(const LbvhScratch* S) {
+    const LbvhScratch& s = S[blockIdx.x];
+    __shared__ uint32_t s_sum[1024];
+    __shared__ uint32_t s_run;
+    const uint32_t T = *s.n_treelets, nb = (T + REF_FIN_BLOCK - 1) / REF_FIN_BLOCK;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_run = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i = base + tid;
+        const uint32_t v = i < nb ? s.fin_blk[i] : 0u;
+        s_sum[tid] = v;
+        __syncthreads();
+        for (int off = 1; off < 1024; off <<= 1) {
+            uint32_t x = tid >= off ? s_sum[tid - off] : 0u;
+            __syncthreads();
+            s_sum[tid] += x;
+            __syncthreads();
+        }
+        if (i < nb) s.fin_blk[i] = s_run + s_sum[tid] - v;
+        __syncthreads();
+        if (tid == 0) s_run += s_sum[1023];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        s.fin_p[T] = s_run;
+        *s.n_nodes = *s.upper_count + s_run;
+    }
+}
+__global__ __launch_bounds__(REF_BLOCK) void kr_fin_place(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    __shared__ uint32_t s_w[REF_BLOCK / 64];
+    const uint32_t T = *s.n_treelets, nb = (T + REF_FIN_BLOCK - 1) / REF_FIN_BLOCK;
+    for (uint32_t b = rj; b < nb; b += G) {
+        uint32_t v[REF_FIN_PER], t[REF_FIN_PER], sum = 0;
+        const uint32_t i0 = b * REF_FIN_BLOCK + threadIdx.x * REF_FIN_PER;
+        for (uint32_t j = 0; j < REF_FIN_PER; j++) {
+            const uint32_t i = i0 + j;
+            t[j] = i < T ? s.g_perm[i] : 0u;
+            v[j] = i < T ? s.t_nodes[t[j]] : 0u;
+            sum += v[j];
+        }
+        uint32_t tot;
+        uint32_t run = s.fin_blk[b] + block_excl_scan(sum, s_w, &tot);
+        for (uint32_t j = 0; j < REF_FIN_PER; j++) {
+            const uint32_t i = i0 + j;
+            if (i < T) {
+                s.fin_p[i] = run;
+                s.t_final[t[j]] = s.g_la[i] + (i - s.g_lb[i]) + run;
+            }
+            run += v[j];
+        }
+    }
+}
+__global__ __launch_bounds__(REF_BLOCK) void kr_fin_upper(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    const uint32_t nu = *s.upper_count;
+    for (uint32_t u = rj * REF_BLOCK + threadIdx.x; u < nu; u += G * REF_BLOCK) {
+        const uint32_t i0 = s.u_i0[u], mid = s.u_mid[u];
+        const uint32_t p0 = s.fin_p[i0];
+        const uint32_t off = s.u_dep[u] + (i0 - s.u_rt[u]) + p0;
+        const uint32_t size0 = (mid - 1 - i0) + (s.fin_p[mid] - p0);
+        s.node_offset[off] = off + 1 + size0;
+        s.node_nshapes[off] = 0;
+        s.node_axis[off] = (uint8_t)s.u_axis[u];
+        store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
+    }
+}
+// Flatten (SLBVH.cpp:432-455) of the treelet nodes, one thread per treelet.
+__global__ __launch_bounds__(REF_BLOCK) void kr_scatter_ref(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    const uint32_t T = *s.n_treelets;
+    for (uint32_t t = rj * REF_BLOCK + threadIdx.x; t < T; t += G * REF_BLOCK) {
+        const uint32_t base = s.t_slot[t], cnt = s.t_nodes[t], fbase = s.t_final[t];
+        for (uint32_t q = 0; q < cnt; q++) {
+            const uint32_t meta = s.r_meta[base + q];
+            const uint32_t fi = fbase + s.r_off[base + q];
+            store_box(s.node_bounds + 6 * (size_t)fi, load_box(s.r_box + 6 * (size_t)(base + q)));
+            if (meta & 0x80000000u) {
+                s.node_offset[fi] = s.r_start[base + q];
+                s.node_nshapes[fi] = s.r_n[base + q];
+                s.node_axis[fi] = 0;
+            } else {
+                const uint32_t c0 = s.r_c0[base + q];
+                s.node_offset[fi] = fbase + s.r_off[base + c0 + 1];
+                s.node_nshapes[fi] = 0;
+                s.node_axis[fi] = (uint8_t)((meta >> 8) & 3u);
+            }
+        }
+    }
+}
+
+// Host side of the reference order: blocks per scene of each kernel, the
+// launch sequence of the sort-and-treelet, emit and upper-SAH stages.
+struct RefGrid {
+    uint32_t slots, treelets, emit_big, level, sub, fin, upper;
+};
+static RefGrid ref_grid(uint32_t max_n, uint32_t n_scenes) {
+    auto clampu = [](uint64_t v, uint32_t lo, uint32_t hi) { return (uint32_t)std::min<uint64_t>(hi, std::max<uint64_t>(lo, v)); };
+    const uint32_t cap = n_scenes > 1 ? 256u : 2048u;
+    RefGrid g;
+    g.slots = clampu((max_n + REF_BLOCK - 1) / REF_BLOCK, 1, cap);
+    g.treelets = g.slots;
+    g.emit_big = clampu(max_n / 512 + 1, 1, n_scenes > 1 ? 64u : 1024u);
+    g.level = clampu(max_n / REF_CHUNK + 1, 1, n_scenes > 1 ? 64u : 1024u);
+    g.sub = clampu(max_n / 512 + 1, 1, n_scenes > 1 ? 16u : 256u);
+    g.fin = clampu(max_n / REF_FIN_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
+    g.upper = clampu(max_n / REF_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
+    return g;
+}
+
+static void ref_launch_treelets(const LbvhScratch* S, uint32_t ns, const RefGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(kr_heads, dim3(ns * g.slots), dim3(REF_BLOCK), 0, st, S, g.slots);
+    hipLaunchKernelGGL(kr_head_scan, dim3(ns), dim3(1024), 0, st, S);
+    hipLaunchKernelGGL(kr_treelets, dim3(ns * g.treelets), dim3(REF_BLOCK), 0, st, S, g.treelets);
+}
+static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g, hipStream_t st) {
+    hipLaunchKernelGGL(kr_emit_small, dim3(ns * g.treelets), dim3(REF_BLOCK), 0, st, S, g.treelets);
+    hipLaunchKernelGGL(kr_emit_big, dim3(ns * g.emit_big), dim3(EMIT_BLOCK), 0, st, S, g.emit_big);
+}
+// The level loop: four levels between host checks of the next level's range
+// count (kernels of an empty level return at once).
+constexpr uint32_t REF_LEVELS_PER_CHECK = 4;
+constexpr uint32_t REF_MAX_LEVELS = 4096;
+static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const RefGrid& g, uint32_t* d_active,
+                                    hipStream_t st) {
+    for (uint32_t level = 0;; ) {
+        for (uint32_t k = 0; k < REF_LEVELS_PER_CHECK; k++, level++) {
+            hipLaunchKernelGGL(kr_lv_stats, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
+            hipLaunchKernelGGL(kr_lv_split, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
+            hipLaunchKernelGGL(kr_lv_lists, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
+            hipLaunchKernelGGL(kr_lv_swap, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
+        }
+        hipLaunchKernelGGL(kr_active, dim3(1), dim3(256), 0, st, S, ns, level & 1u, d_active);
+        uint32_t active = 0;
+        if (hipMemcpyAsync(&active, d_active, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return WPBR_ERR_HIP;
+        if (!active) break;
+        // both children of a split are non-empty, so a level range shrinks every
+        // level; deeper chains of ranges of at least REF_CUT roots are refused
+        if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
+    }
+    hipLaunchKernelGGL(kr_sub, dim3(ns * g.sub), dim3(SAH_BLOCK), 0, st, S, g.sub);
+    hipLaunchKernelGGL(kr_fin_sum, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
+    hipLaunchKernelGGL(kr_fin_scan, dim3(ns), dim3(1024), 0, st, S);
+    hipLaunchKernelGGL(kr_fin_place, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
+    hipLaunchKernelGGL(kr_fin_upper, dim3(ns * g.upper), dim3(REF_BLOCK), 0, st, S, g.upper);
+    return launch_status();
+}
+
 // ---------------------------------------------------------------------------
 __global__ void k_cb_final(uint32_t* cb_ord) {
     __shared__ uint32_t s_cb[6];
@@ -2137,21 +2979,32 @@ void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts) {
     counts[SEG_BVH4] = std::min<uint32_t>(BVH4_BLOCKS, std::max<uint32_t>(1, (n + 511) / 512));
 }
 
-wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st) {
+wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint32_t order, uint32_t max_n,
+                                  hipStream_t st) {
     const uint32_t S = B.n_scenes;
+    const bool ref = order == WPBR_LBVH_ORDER_REFERENCE;
     hipLaunchKernelGGL(ks_bounds, dim3(grid[SEG_BOUNDS]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_cb_final, dim3(S), dim3(64), 0, st, B);
     hipLaunchKernelGGL(ks_morton, dim3(grid[SEG_MORTON]), dim3(MORTON_BLOCK), 0, st, B);
-    for (int p = 0; p < 4; p++)
+    for (int p = 0; p < (ref ? 3 : 4); p++)
         hipLaunchKernelGGL(ks_onesweep, dim3(grid[SEG_SORT]), dim3(SORT_BLOCK), 0, st, B, p);
     hipLaunchKernelGGL(ks_slot_prims, dim3(grid[SEG_MARK]), dim3(256), 0, st, B);
-    hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
-    hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
-    hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
-    hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
-    hipLaunchKernelGGL(ks_sah_sub, dim3(SAH_SEG_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
-    hipLaunchKernelGGL(ks_sah_final, dim3(S * SEG_SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
-    hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
+    if (ref) {
+        const RefGrid g = ref_grid(max_n, S);
+        ref_launch_treelets(B.s, S, g, st);
+        ref_launch_emit(B.s, S, g, st);
+        wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st);
+        if (e) return e;
+        hipLaunchKernelGGL(kr_scatter_ref, dim3(S * g.upper), dim3(REF_BLOCK), 0, st, B.s, g.upper);
+    } else {
+        hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
+        hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
+        hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
+        hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
+        hipLaunchKernelGGL(ks_sah_sub, dim3(SAH_SEG_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
+        hipLaunchKernelGGL(ks_sah_final, dim3(S * SEG_SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
+        hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
+    }
     hipLaunchKernelGGL(ks_subtree, dim3(grid[SEG_SUBTREE]), dim3(256), 0, st, B);
     hipLaunchKernelGGL(ks_subtree_leaf, dim3(grid[SEG_SUBTREE_LEAF]), dim3(64 * LEAF_WAVES), 0, st, B);
     hipLaunchKernelGGL(ks_subtree_big, dim3(grid[SEG_SUBTREE_BIG]), dim3(256), 0, st, B);
@@ -2170,9 +3023,11 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipS
 //   4 treelet trees           JobEmitLBVH
 //   5 upper SAH + flatten     JobBuildUpperSAH (+ the derived traversal layout)
 // ev[0..11] bracket the kernels (the per-kernel times of wpbr_lbvh_kernel_times).
-wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
-                              hipEvent_t* ev, int nev) {
+wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const GeomView& g, uint32_t n,
+                              hipStream_t st, int stage, hipEvent_t* ev, int nev) {
     auto rec = [&](int i) { if (ev && i < nev) (void)hipEventRecord(ev[i], st); };
+    const bool ref = s.order == WPBR_LBVH_ORDER_REFERENCE;
+    const RefGrid rg = ref_grid(n, 1);
     switch (stage) {
         case 0:
             rec(0);
@@ -2190,26 +3045,39 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hip
             break;
         case 3: {
             const uint32_t num_tiles = (n + SORT_TILE - 1) / SORT_TILE;
-            for (int p = 0; p < 4; p++) {
+            const int np = sort_passes(s);
+            for (int p = 0; p < np; p++) {
                 const SortPass q = sort_pass(s, p);
                 hipLaunchKernelGGL(k_onesweep, dim3(num_tiles), dim3(SORT_BLOCK), 0, st, q.kin, q.vin, q.kout,
                                    q.vout, n, p, s.hist, q.status, q.status_next, s.tile_counter, s.err);
                 rec(3 + p);
             }
             hipLaunchKernelGGL(k_slot_prims, dim3((n + 255) / 256), dim3(256), 0, st, s, g, n);
+            if (ref) {
+                ref_launch_treelets(d_self, 1, rg, st);
+                rec(6);  // events 3..6: three sort passes, then the treelet scan
+            }
             rec(7);
             break;
         }
         case 4:
-            hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, n);
+            if (ref) ref_launch_emit(d_self, 1, rg, st);
+            else hipLaunchKernelGGL(k_emit, dim3(4096), dim3(EMIT_BLOCK), 0, st, s, s.keys_a, n);
             rec(8);
             break;
         case 5: {
-            hipLaunchKernelGGL(k_sah_top, dim3(1), dim3(SAH_BLOCK), 0, st, s);
-            hipLaunchKernelGGL(k_sah_sub, dim3(SAH_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
-            hipLaunchKernelGGL(k_sah_final, dim3(SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
-            rec(9);
-            hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
+            if (ref) {
+                wpbr_status e = ref_launch_upper(d_self, 1, rg, s.err + 3, st);
+                if (e) return e;
+                rec(9);
+                hipLaunchKernelGGL(kr_scatter_ref, dim3(rg.upper), dim3(REF_BLOCK), 0, st, d_self, rg.upper);
+            } else {
+                hipLaunchKernelGGL(k_sah_top, dim3(1), dim3(SAH_BLOCK), 0, st, s);
+                hipLaunchKernelGGL(k_sah_sub, dim3(SAH_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
+                hipLaunchKernelGGL(k_sah_final, dim3(SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
+                rec(9);
+                hipLaunchKernelGGL(k_scatter, dim3(4096), dim3(256), 0, st, s);
+            }
             rec(10);
             const uint32_t max_nodes = 2 * n;
             hipLaunchKernelGGL(k_subtree, dim3(std::min<uint32_t>(SUBTREE_BLOCKS, (max_nodes + 255) / 256)), dim3(256),

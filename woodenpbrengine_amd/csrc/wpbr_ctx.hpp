@@ -28,6 +28,7 @@ struct wpbr_ctx {
     uint32_t spill_depth = 0;
     uint32_t* err_flag = nullptr;
     wpbr_render_state* render = nullptr;
+    int lbvh_order = WPBR_LBVH_ORDER_REFERENCE;  // key order of new LBVH objects
     // segmented LBVH builds: per-scene descriptors and block maps
     void* batch_mem = nullptr;
     size_t batch_cap = 0;
@@ -43,7 +44,9 @@ struct wpbr_lbvh {
     uint32_t root_word = 0;
     bool built = false;
     int next_stage = 0;  // staged build: the stage wpbr_lbvh_build_stage accepts next
+    int order = WPBR_LBVH_ORDER_REFERENCE;
     wpbr::LbvhScratch s{};
+    wpbr::LbvhScratch* d_self = nullptr;  // device copy of s
     wpbr::GeomView geom{};
     hipEvent_t ev[NEV] = {};
 };

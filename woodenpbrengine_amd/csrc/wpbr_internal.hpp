@@ -75,24 +75,68 @@ struct LbvhScratch {
     void* prims;          // 48 B per shape, at its slot
     uint32_t* big_list;   // reference leaves of more than 64 shapes
     uint32_t* big_count;
+    // ---- key order and sizes
+    uint32_t order;       // WPBR_LBVH_ORDER_REFERENCE or WPBR_LBVH_ORDER_FULL
+    uint32_t n;           // shapes of this build
+    uint32_t cap_t;       // capacity of the treelet, position, deferred-range and upper-node arrays
+    uint32_t cap_r;       // level ranges per list
+    uint32_t cap_c;       // level chunks per list
+    // ---- reference order: treelets as a list (up to one per shape)
+    uint32_t* head_blk;   // per 256 slots: treelet heads, then their exclusive prefix
+    uint32_t* bigt;       // treelets of at least 30 shapes (one workgroup each)
+    uint32_t* bigt_n;
+    // level-synchronous upper SAH over the ranges of at least REF_CUT positions
+    float* gp_box;        // cap_t x 6: root box of the treelet at each position
+    uint8_t* gp_bucket;   // cap_t: SAH bucket of each position
+    uint32_t* pl_a;       // cap_t: misplaced positions left of the split, by rank
+    uint32_t* pl_b;       // cap_t: misplaced positions right of the split, by rank from the right
+    uint32_t* lv_n;       // [k]: ranges of list k, [2 + k]: chunks of list k (k = level parity)
+    uint32_t* lv_rec;     // 2 x cap_r x REF_RG_WORDS range records
+    uint32_t* lv_stat;    // 2 x cap_r x REF_ST_WORDS bucket statistics
+    uint32_t* ch_range;   // 2 x cap_c: range of each chunk
+    uint32_t* ch_cnt;     // cap_c x 2: misplaced positions left / right per chunk
+    uint32_t* fin_p;      // cap_t + 1: prefix of treelet node counts over final positions
+    uint32_t* fin_blk;    // per REF_FIN_BLOCK positions: node sums, then their exclusive prefix
 };
 
-// Buffers of radix-sort pass p (0..3): each pass reads what the previous one
-// wrote, keys {morton, B, A, B} -> {B, A, B, A} (values alike, none for pass 0
-// whose value is the key's index); look-back status alternates 0, 1, 0, 1 and
-// each pass clears the next pass's status.
+// Reference key order: ranges of at least REF_CUT treelet roots are split by
+// level-synchronous kernels over REF_CHUNK-position chunks; smaller ones by one
+// workgroup each in LDS.
+constexpr uint32_t REF_CUT = 256;
+constexpr uint32_t REF_CHUNK = 2048;
+constexpr uint32_t REF_FIN_BLOCK = 1024;
+constexpr uint32_t REF_RG_WORDS = 16;
+constexpr uint32_t REF_ST_WORDS = 164;
+inline uint32_t ref_cap_r(uint32_t cap_t) { return cap_t / REF_CUT + 2; }
+inline uint32_t ref_cap_c(uint32_t cap_t) { return cap_t / REF_CHUNK + ref_cap_r(cap_t) + 1; }
+
+// Radix-sort passes of a build: 4 (8-bit digits over bits 0..31, the complete
+// order) or 3 (bits 0..23: the order the reference's sort job leaves in
+// shapeMortonCodes, SLBVH.cpp:245-291).
+__host__ __device__ inline int sort_passes(const LbvhScratch& s) {
+    return s.order == WPBR_LBVH_ORDER_REFERENCE ? 3 : 4;
+}
+
+// Buffers of radix-sort pass p: each pass reads what the previous one wrote,
+// the first reads the Morton codes (no values: the value is the key's index)
+// and the last writes keys_a / vals_a, so the outputs alternate ... B, A, B, A
+// (4 passes: B A B A; 3 passes: A B A); look-back status alternates 0, 1, 0, ...
+// and each pass clears the next pass's status.
 struct SortPass {
     const uint32_t* kin; const uint32_t* vin; uint32_t* kout; uint32_t* vout;
     uint32_t* status; uint32_t* status_next;
 };
 __host__ __device__ inline SortPass sort_pass(const LbvhScratch& s, int p) {
+    const int np = sort_passes(s);
+    const bool in_a = p > 0 && ((np - p) & 1) == 0;   // pass p - 1 wrote A
+    const bool out_a = ((np - 1 - p) & 1) == 0;
     SortPass r;
-    r.kin = p == 0 ? s.morton : (p == 2 ? s.keys_a : s.keys_b);
-    r.vin = p == 0 ? nullptr : (p == 2 ? s.vals_a : s.vals_b);
-    r.kout = (p & 1) ? s.keys_a : s.keys_b;
-    r.vout = (p & 1) ? s.vals_a : s.vals_b;
+    r.kin = p == 0 ? s.morton : (in_a ? s.keys_a : s.keys_b);
+    r.vin = p == 0 ? nullptr : (in_a ? s.vals_a : s.vals_b);
+    r.kout = out_a ? s.keys_a : s.keys_b;
+    r.vout = out_a ? s.vals_a : s.vals_b;
     r.status = (p & 1) ? s.status1 : s.status0;
-    r.status_next = p == 3 ? nullptr : ((p & 1) ? s.status0 : s.status1);
+    r.status_next = p == np - 1 ? nullptr : ((p & 1) ? s.status0 : s.status1);
     return r;
 }
 
@@ -116,7 +160,9 @@ struct LbvhBatch {
     SegMap map[SEG_KINDS];
 };
 void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts);
-wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, hipStream_t st);
+// order: the key order of every scene of the batch; max_n: the largest scene.
+wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint32_t order, uint32_t max_n,
+                                  hipStream_t st);
 
 constexpr int LBVH_STAGES = 6;
 // SAH top: task list of the one-workgroup top kernel; workgroups of the
@@ -124,8 +170,10 @@ constexpr int LBVH_STAGES = 6;
 constexpr uint32_t SAH_TASKS = 8192;
 constexpr uint32_t SAH_SUB_BLOCKS = 32;       // single scene
 constexpr uint32_t SAH_SEG_SUB_BLOCKS = 256;  // segmented build, over all scenes' ranges
-wpbr_status lbvh_launch_stage(LbvhScratch& s, const GeomView& g, uint32_t n, hipStream_t st, int stage,
-                              hipEvent_t* ev, int nev);
+// d_self: a device copy of s (the reference order's kernels read the scratch
+// through it).
+wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const GeomView& g, uint32_t n,
+                              hipStream_t st, int stage, hipEvent_t* ev, int nev);
 
 struct TraceArgs {
     GeomView g;

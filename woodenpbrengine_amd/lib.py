@@ -31,7 +31,13 @@ EXPORTS = [
     "wpbr_sample_arrays", "wpbr_texture_create", "wpbr_texture_destroy", "wpbr_texture_levels",
     "wpbr_texture_level", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
+    "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
 ]
+
+# LBVH key orders (wpbr.h WPBR_LBVH_ORDER_*): the reference's (bits 0..23 of the
+# Morton codes, what its sort job leaves for the later jobs) or the full sort.
+ORDER_REFERENCE = 0
+ORDER_FULL = 1
 
 
 class WpbrError(RuntimeError):
@@ -100,6 +106,9 @@ def load():
         "wpbr_lbvh_download": (i32, [vp, vp, C.c_int, vp, sz]),
         "wpbr_lbvh_kernel_times": (C.c_int, [vp, C.POINTER(C.c_float), C.c_int]),
         "wpbr_lbvh_build_stage": (i32, [vp, C.POINTER(Geometry), vp, C.c_int]),
+        "wpbr_lbvh_set_order": (i32, [vp, C.c_int]),
+        "wpbr_lbvh_get_order": (C.c_int, [vp]),
+        "wpbr_set_default_lbvh_order": (i32, [vp, C.c_int]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
@@ -211,6 +220,10 @@ class Context:
     def empty(self, shape, dtype):
         return DeviceArray(self, shape, dtype)
 
+    def set_default_lbvh_order(self, order):
+        """Key order of LBVH objects (and scenes) created afterwards."""
+        self.check(self.lib.wpbr_set_default_lbvh_order(self.h, order))
+
     def sync(self):
         self.check(self.lib.wpbr_sync(self.h))
 
@@ -258,11 +271,17 @@ class Lbvh:
               "node_bounds": (6, np.float32), "centroid_bounds": (7, np.float32),
               "treelet_start": (8, np.uint32)}
 
-    def __init__(self, ctx):
+    def __init__(self, ctx, order=None):
         self.ctx = ctx
         h = C.c_void_p()
         ctx.check(ctx.lib.wpbr_lbvh_create(ctx.h, C.byref(h)))
         self.h = h
+        if order is not None:
+            ctx.check(ctx.lib.wpbr_lbvh_set_order(h, order))
+
+    @property
+    def order(self):
+        return int(self.ctx.lib.wpbr_lbvh_get_order(self.h))
 
     def build(self, scene):
         self.ctx.check(self.ctx.lib.wpbr_lbvh_build(self.ctx.h, C.byref(scene.geom), self.h))
