@@ -49,55 +49,98 @@ def measured_traffic():
     return None
 
 
-def cpu_baseline(scene, max_threads=16):
-    """The oracle (CPU restatement of the reference path) on every 4th film tile."""
+CPU_SINGLE_TILE_STRIDE = 64
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_share():
+    """Threads for the CPU baseline: the host-CPU share of one GPU on the box
+    (OMP_NUM_THREADS, which the box sets to it), else every visible core."""
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(share, visible) if share > 0 else visible), visible
+
+
+def cpu_baseline(scene):
+    """The oracle (CPU restatement of the reference path) on every 4th film tile
+    with the GPU's CPU share of threads, and on every 64th tile single-threaded."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
     ntx = scene.width // 32
     ntiles = ntx * (scene.height // 32)
-    tiles = np.arange(0, ntiles, CPU_TILE_STRIDE, dtype=np.uint32)
-    threads = max(1, min(max_threads, os.cpu_count() or 1))
-    out = po.render(scene, nthreads=threads, tiles=tiles)
-    secs = float(out["counters"][7]) * 1e-9
-    samples = len(tiles) * 32 * 32 * scene.spp
-    return {"value": samples / secs / 1e6, "unit": "Msamples/s", "cores": threads,
+    threads, visible = cpu_share()
+
+    def run(stride, nthreads):
+        tiles = np.arange(0, ntiles, stride, dtype=np.uint32)
+        out = po.render(scene, nthreads=nthreads, tiles=tiles)
+        secs = float(out["counters"][7]) * 1e-9
+        samples = len(tiles) * 32 * 32 * scene.spp
+        return samples / secs / 1e6, len(tiles), samples, secs
+
+    v, nt, ns, secs = run(CPU_TILE_STRIDE, threads)
+    v1, nt1, ns1, secs1 = run(CPU_SINGLE_TILE_STRIDE, 1)
+    return {"value": v, "unit": "Msamples/s", "cores": threads,
             "kind": "port",
             "sample": "every %dth 32x32 tile of the render (%d of %d tiles, %d samples), "
-                      "%.2f s wall on %d threads" % (CPU_TILE_STRIDE, len(tiles), ntiles, samples, secs,
-                                                     threads)}
+                      "%.2f s wall on %d threads" % (CPU_TILE_STRIDE, nt, ntiles, ns, secs, threads),
+            "single_thread": {"value": v1, "unit": "Msamples/s",
+                              "sample": "every %dth tile (%d tiles, %d samples), %.2f s wall on 1 thread"
+                                        % (CPU_SINGLE_TILE_STRIDE, nt1, ns1, secs1)},
+            "cpu_model": cpu_model(), "cpus_visible": visible,
+            "threads_rule": "the host-CPU share of one GPU (OMP_NUM_THREADS) when set, else every visible core"}
+
+
+# wpbr_lbvh_kernel_times intervals (events 0..11 of a build)
+KERNEL_NAMES = ["bounds", "centroid_bounds+morton", "sort_pass0", "sort_pass1", "sort_pass2",
+                "sort_pass3 | treelets", "slot_gather+treelets", "emit", "upper_sah", "flatten",
+                "traversal_layout"]
 
 
 def bench_microbench(lib, scenes, ctx, steps):
+    """LBVH build (1 048 576 triangles) in both key orders, and closest-hit
+    traversal of 4 194 304 rays over each tree."""
     tris, rays = scenes.microbench()
     sc = lib.Scene(ctx, np.zeros((0, 24), np.float32), np.zeros(0, np.float32), tris)
-    bvh = lib.Lbvh(ctx)
-    bvh.build(sc)
-    build_ms = []
-    for _ in range(max(steps, 3)):
-        build_ms.append(bvh.build(sc))
-    ktimes = bvh.kernel_times()
-    nodes = bvh.counts()["nodes"]
     d_rays = ctx.array(rays)
     n = rays.shape[1]
     saved = ctx.trace_config()
-    tuned = lib.autotune_collide(ctx, sc, bvh, d_rays, n)
-    d_s, d_t, _ = lib.collide(ctx, sc, bvh, d_rays, n)
-    trace_ms = []
-    for _ in range(max(steps, 3)):
-        lib.collide(ctx, sc, bvh, d_rays, n)
-        trace_ms.append(ctx.last_ms())
-    b = float(np.median(build_ms))
-    t = float(np.median(trace_ms))
-    lbvh_bytes = LBVH_BYTES_PER_PRIM * len(tris) + LBVH_BYTES_PER_NODE * nodes
-    res = {
-        "lbvh_build": {"prims": len(tris), "nodes": nodes, "ms": b,
-                       "mprims_per_s": len(tris) / b / 1e3,
-                       "hbm_frac": lbvh_bytes / (b * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                       "kernel_ms": [round(x, 4) for x in ktimes]},
-        "traversal": {"rays": n, "ms": t, "mrays_per_s": n / t / 1e3, "config": tuned},
-    }
+    res = {}
+    for order, key in ((lib.ORDER_REFERENCE, ""), (lib.ORDER_FULL, "_full")):
+        bvh = lib.Lbvh(ctx, order)
+        bvh.build(sc)
+        build_ms = [bvh.build(sc) for _ in range(max(steps, 3))]
+        ktimes = bvh.kernel_times()
+        cnt = bvh.counts()
+        nodes = cnt["nodes"]
+        tuned = lib.autotune_collide(ctx, sc, bvh, d_rays, n)
+        trace_ms = []
+        for _ in range(max(steps, 3)):
+            lib.collide(ctx, sc, bvh, d_rays, n)
+            trace_ms.append(ctx.last_ms())
+        b = float(np.median(build_ms))
+        t = float(np.median(trace_ms))
+        layout = ktimes[10] if len(ktimes) > 10 else 0.0
+        lbvh_bytes = LBVH_BYTES_PER_PRIM * len(tris) + LBVH_BYTES_PER_NODE * nodes
+        res["lbvh_build" + key] = {
+            "order": "reference (bits 0..23, what the reference's sort job leaves)" if order == lib.ORDER_REFERENCE
+                     else "full 30-bit sort",
+            "prims": len(tris), "nodes": nodes, "treelets": cnt["treelets"], "ms": b,
+            "mprims_per_s": len(tris) / b / 1e3,
+            "hbm_frac": lbvh_bytes / (b * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "ms_without_traversal_layout": b - layout,
+            "kernel_ms": {k: round(x, 4) for k, x in zip(KERNEL_NAMES, ktimes)}}
+        res["traversal" + key] = {"rays": n, "ms": t, "mrays_per_s": n / t / 1e3, "config": tuned}
+        bvh.destroy()
     ctx.set_trace_config(**saved)
-    bvh.destroy()
     return res
 
 
@@ -227,28 +270,36 @@ def main():
         ctx = lib.Context(0)
         tris, _ = scenes.microbench()
         sc = lib.Scene(ctx, np.zeros((0, 24), np.float32), np.zeros(0, np.float32), tris)
-        bvh = lib.Lbvh(ctx)
-        bvh.build(sc)
-        one = [bvh.build(sc) for _ in range(max(args.steps, 3))]
         ss = scenes.c3_scenes()
         geo = [lib.Scene(ctx, s.sphere_xform, s.sphere_radius, s.tri_verts) for s in ss]
-        bv = [lib.Lbvh(ctx) for _ in ss]
-        lib.build_segmented(ctx, geo, bv)
-        seg = [lib.build_segmented(ctx, geo, bv) for _ in range(max(args.steps, 3))]
-        print(json.dumps({"lbvh_build_ms": float(np.median(one)), "prims": len(tris),
-                          "kernel_ms": [round(x, 4) for x in bvh.kernel_times()],
-                          "segmented_build_ms": float(np.median(seg)),
-                          "segmented_shapes": int(sum(s.n_shapes for s in ss))}), flush=True)
+        out = {"prims": len(tris), "segmented_shapes": int(sum(s.n_shapes for s in ss))}
+        for order, key in ((lib.ORDER_REFERENCE, "reference"), (lib.ORDER_FULL, "full")):
+            bvh = lib.Lbvh(ctx, order)
+            bvh.build(sc)
+            one = [bvh.build(sc) for _ in range(max(args.steps, 3))]
+            bv = [lib.Lbvh(ctx, order) for _ in ss]
+            lib.build_segmented(ctx, geo, bv)
+            seg = [lib.build_segmented(ctx, geo, bv) for _ in range(max(args.steps, 3))]
+            out[key] = {"lbvh_build_ms": float(np.median(one)),
+                        "kernel_ms": {k: round(x, 4) for k, x in zip(KERNEL_NAMES, bvh.kernel_times())},
+                        "segmented_build_ms": float(np.median(seg))}
+            log(key, json.dumps(out[key]))
+            for b in bv + [bvh]:
+                b.destroy()
+        print(json.dumps(out), flush=True)
         ctx.close()
         return
 
     t0 = time.time()
     if args.workload == "render":
         scene = scenes.c2_scene()
-        wl = "full wavefront render: 1024x1024 film, 16 spp, displaced icosphere (2009780 tris)"
+        wl = ("full wavefront render: 1024x1024 film, 16 spp stratified, displaced icosphere (2009780 tris), "
+              "reference-order LBVH")
     else:
         scene = scenes.c0_scene()
-        wl = "plain render: 128x128 film, 4 spp, icosphere (20480 tris)"
+        wl = "plain render: 128x128 film, 4 spp stratified, icosphere (20480 tris), reference-order LBVH"
+    # the reference's stratified sampler (JobSamplerStratifiedGenerateSampels1D/2D)
+    scene.sampler = lib.SAMPLER_STRATIFIED
     log("scene generated in %.1f s (%d shapes)" % (time.time() - t0, scene.n_shapes))
     ctx = lib.Context(0)
     r = lib.Renderer(ctx, scene)
@@ -360,10 +411,34 @@ def main():
              "rays_per_step": rays_per_step,
              "trace_config": tuned,
              "render_mrays_per_s": rays_per_step / (t_ms * 1e-3) / 1e6}
+    # Dominant kernel: k_trace.  The same render with its passes one after
+    # another (wpbr_set_render_overlap(0): no kernel shares the GPU), its
+    # launches (a primary and a secondary-queue launch per pass) timed with HIP
+    # events on the stream each is launched on.
+    ctx.set_render_overlap(0)
+    r.render()
+    ser_ms, ser_trace, ser_launch, ser_rays = [], [], [], []
+    for _ in range(args.steps):
+        ser_ms.append(r.render())
+        st = r.stats()
+        ser_trace.append(st["trace_ms"])
+        ser_launch.append(st["trace_launches"])
+        ser_rays.append(st["rays"])
+    ctx.set_render_overlap(1)
+    s_trace = float(np.mean(ser_trace))
+    s_launch = float(np.mean(ser_launch))
+    s_rays = float(np.mean(ser_rays))
+    extra["render_serial_passes"] = {"device_ms_per_step": float(np.mean(ser_ms)), "trace_ms_per_step": s_trace,
+                                     "trace_launches": s_launch, "rays_per_step": s_rays}
+    log("render with serial passes: %.2f ms device, trace %.2f ms in %d launches" % (np.mean(ser_ms), s_trace,
+                                                                                     s_launch))
     r.destroy()
-    if not args.no_stratified:
-        # the same render with shuffled stratified sample arrays (computed where consumed)
-        scene.sampler = lib.SAMPLER_STRATIFIED_N
+    for name, mode in (("render_independent", lib.SAMPLER_INDEPENDENT),
+                       ("render_stratified_n", lib.SAMPLER_STRATIFIED_N)):
+        if args.no_stratified:
+            break
+        # the same render with the hash streams / the 1/n-scaled stratified arrays
+        scene.sampler = mode
         r2 = lib.Renderer(ctx, scene)
         r2.render()
         ctx.sync()
@@ -373,19 +448,19 @@ def main():
         ctx.sync()
         st_ms = (time.perf_counter() - t0s) * 1e3 / args.steps
         r2.destroy()
-        scene.sampler = 0
-        extra["render_stratified"] = {"ms_per_step": st_ms, "msamples_per_s": samples / (st_ms * 1e-3) / 1e6}
-        log("stratified render: %.2f ms/step" % st_ms)
-    # Dominant kernel: k_trace.  Its launches in one render (a primary and a
-    # secondary-queue launch per pass) timed with HIP events on the render stream.
-    avg_launch_ms = t_ms / n_launch
-    bytes_per_launch = TRACE_BYTES_PER_RAY * rays_per_step / n_launch
+        extra[name] = {"ms_per_step": st_ms, "msamples_per_s": samples / (st_ms * 1e-3) / 1e6}
+        log("%s: %.2f ms/step" % (name, st_ms))
+    scene.sampler = lib.SAMPLER_STRATIFIED
+    avg_launch_ms = s_trace / s_launch
+    bytes_per_launch = TRACE_BYTES_PER_RAY * s_rays / s_launch
     achieved = bytes_per_launch / (avg_launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                "kernel": "k_trace in the render (36 B of ray I/O per ray; %.0f rays per launch)"
-                          % (rays_per_step / n_launch),
-                "avg_launch_us": avg_launch_ms * 1e3}
+                "kernel": "k_trace in the render, passes serialised (36 B of ray I/O per ray; %.0f rays per launch)"
+                          % (s_rays / s_launch),
+                "avg_launch_us": avg_launch_ms * 1e3,
+                "limiter": "L2/MALL latency of the node and primitive gathers, not HBM bandwidth "
+                           "(profiles/k_trace_render_c2_pmc.txt: L2 hit rate, VALU busy)"}
     mb = {}
     if not args.no_microbench:
         mb = bench_microbench(lib, scenes, ctx, args.steps)

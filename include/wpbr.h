@@ -256,6 +256,13 @@ wpbr_status wpbr_scene_rebuild(wpbr_ctx* ctx, wpbr_scene* scene);
  * tonemap).  d_sample_xyz (optional) receives width*height*spp*3 floats, the
  * XYZ of each sample's radiance at index (pixel * spp + j). */
 wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* d_sample_xyz);
+/* overlap 1 (the default): consecutive sample passes of wpbr_render overlap on
+ * two streams (pass j's camera, primary trace and shading run beside pass
+ * j - 1's secondary trace and splat).  0: each pass starts after the previous
+ * one has finished, so every kernel runs alone on the GPU (for per-kernel
+ * timing).  Films and samples are the same bit for bit either way.  No
+ * counterpart in the reference, which renders one tile at a time. */
+wpbr_status wpbr_set_render_overlap(wpbr_ctx* ctx, int overlap);
 /* Statistics of the last wpbr_render: [0] total ms, [1] trace ms (all trace
  * launches), [2] number of trace launches, [3] rays traced, [4] shade ms,
  * [5] camera ms, [6] secondary rays, [7] trace ms of the primary rays.

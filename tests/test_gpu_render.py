@@ -71,3 +71,20 @@ def test_render_c2_mix_per_sample(ctx):
     frac_bad, rel_rmse, exact = _compare(s, ctx, ref)
     assert frac_bad == 0.0
     assert rel_rmse < 1e-5
+
+
+def test_render_serial_passes_identical(ctx):
+    """wpbr_set_render_overlap(0) runs the sample passes one after another:
+    film and per-sample values equal the overlapped render bit for bit."""
+    s = scenes.c0_scene(width=64, height=64, spp=3)
+    r = lib.Renderer(ctx, s)
+    r.render(keep_samples=True)
+    film, samp = r.d_film.numpy(), r.d_samples.numpy()
+    ctx.set_render_overlap(0)
+    try:
+        r.render(keep_samples=True)
+        np.testing.assert_array_equal(r.d_film.numpy().view(np.uint32), film.view(np.uint32))
+        np.testing.assert_array_equal(r.d_samples.numpy().view(np.uint32), samp.view(np.uint32))
+    finally:
+        ctx.set_render_overlap(1)
+    r.destroy()

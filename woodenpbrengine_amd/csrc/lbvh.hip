@@ -380,7 +380,9 @@ WPBR_D void treelet_mark_body(const uint32_t* __restrict__ sorted, uint32_t n, u
 WPBR_D void slot_prims_body(const LbvhScratch& s, const GeomView& g, uint32_t n, uint32_t bid) {
     const uint32_t p = bid * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    treelet_mark_body(s.keys_a, n, s.bin_start, s.presence, bid);
+    // occupied top-12-bit bins (the complete order's treelets); the reference
+    // order finds its treelets by a scan of the run heads instead
+    if (s.order != WPBR_LBVH_ORDER_REFERENCE) treelet_mark_body(s.keys_a, n, s.bin_start, s.presence, bid);
     const uint32_t id = s.vals_a[p];
     float v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     bbox b;
@@ -2120,27 +2122,39 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_treelets(const LbvhScratch* S, u
     }
 }
 
-// Contributes a treelet root's centroid to the first level range's centroid
-// bounds (wave-reduced, then one atomic per coordinate).
-WPBR_D void ref_root_cb(const LbvhScratch& s, bool valid, const f3& c) {
+// Centroid bounds of the treelet roots (the first level range's): every block
+// of kr_emit_small and kr_emit_big writes its partial min/max (order-preserving
+// uints) to cb_ord[8 + 6 k] (k = REF_CB_PARTS slots per kernel), kr_root_cb
+// reduces them.  The k_bounds partials there are no longer needed by then.
+constexpr uint32_t REF_CB_PARTS = 256;
+struct CbAcc {
     uint32_t v[6];
-    v[0] = valid ? f2ord(c.x) : 0xFFFFFFFFu; v[1] = valid ? f2ord(c.y) : 0xFFFFFFFFu;
-    v[2] = valid ? f2ord(c.z) : 0xFFFFFFFFu;
-    v[3] = valid ? f2ord(c.x) : 0u; v[4] = valid ? f2ord(c.y) : 0u; v[5] = valid ? f2ord(c.z) : 0u;
+    WPBR_D CbAcc() { v[0] = v[1] = v[2] = 0xFFFFFFFFu; v[3] = v[4] = v[5] = 0u; }
+    WPBR_D void add(const f3& c) {
+        const uint32_t o[3] = {f2ord(c.x), f2ord(c.y), f2ord(c.z)};
+#pragma unroll
+        for (int k = 0; k < 3; k++) { v[k] = min(v[k], o[k]); v[3 + k] = max(v[3 + k], o[k]); }
+    }
+};
+// Block reduction of the accumulators; thread 0 writes partial slot `part`.
+WPBR_D void cb_block_write(const LbvhScratch& s, CbAcc a, uint32_t part, uint32_t* s_red) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
 #pragma unroll
     for (int k = 0; k < 3; k++)
         for (int off = 32; off >= 1; off >>= 1) {
-            v[k] = min(v[k], (uint32_t)__shfl_xor((int)v[k], off));
-            v[3 + k] = max(v[3 + k], (uint32_t)__shfl_xor((int)v[3 + k], off));
+            a.v[k] = min(a.v[k], (uint32_t)__shfl_xor((int)a.v[k], off));
+            a.v[3 + k] = max(a.v[3 + k], (uint32_t)__shfl_xor((int)a.v[3 + k], off));
         }
-    const bool any = ballot64(valid) != 0;
-    if (lane_id() == 0 && any) {
-        uint32_t* rec = ref_rec(s, 0, 0);
-        for (int k = 0; k < 3; k++) {
-            atomicMin(&rec[RG_CB + k], v[k]);
-            atomicMax(&rec[RG_CB + 3 + k], v[3 + k]);
-        }
+    if (lane == 0)
+        for (int k = 0; k < 6; k++) s_red[6 * w + k] = a.v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t r = s_red[k];
+        for (int q = 1; q < nw; q++) r = k < 3 ? min(r, s_red[6 * q + k]) : max(r, s_red[6 * q + k]);
+        s.cb_ord[8 + 6 * (size_t)part + k] = r;
     }
+    __syncthreads();
 }
 
 // Treelets below 30 shapes: one leaf each (emitLBVH's first case,
@@ -2148,18 +2162,16 @@ WPBR_D void ref_root_cb(const LbvhScratch& s, bool valid, const f3& c) {
 // as treelet order.
 __global__ __launch_bounds__(REF_BLOCK) void kr_emit_small(const LbvhScratch* S, uint32_t G) {
     REF_SCENE(S, G);
+    __shared__ uint32_t s_red[6 * REF_BLOCK / 64];
     const uint32_t T = *s.n_treelets, n = s.n;
-    const uint32_t T_pad = (T + REF_BLOCK - 1) / REF_BLOCK * REF_BLOCK;
-    for (uint32_t t = rj * REF_BLOCK + threadIdx.x; t < T_pad; t += G * REF_BLOCK) {
-        bool leaf = false;
-        bbox b = box_empty();
-        if (t < T) {
+    CbAcc acc;
+    for (uint32_t t = rj * REF_BLOCK + threadIdx.x; t < T; t += G * REF_BLOCK) {
+        {
             const uint32_t start = s.t_start[t];
             const uint32_t end = t + 1 < T ? s.t_start[t + 1] : n;
             const uint32_t cnt = end - start;
             if (cnt < (uint32_t)MAX_PRIMS_IN_LEAF) {
-                leaf = true;
-                bbox unused;
+                bbox b, unused;
                 slot_range_boxes(s.slot_bounds, start, end, end, b, unused);
                 const uint32_t base = 2 * start - t;
                 s.r_start[base] = start;
@@ -2174,32 +2186,57 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_emit_small(const LbvhScratch* S,
                 store_box(s.t_box + 6 * (size_t)t, b);
                 s.g_perm[t] = t;
                 store_box(s.gp_box + 6 * (size_t)t, b);
+                acc.add(box_cen(b));
             } else {
                 s.bigt[atomicAdd(s.bigt_n, 1u)] = t;
             }
         }
-        if (T >= REF_CUT) ref_root_cb(s, leaf, box_cen(b));
     }
+    cb_block_write(s, acc, rj, s_red);
 }
 
 // Treelets of at least 30 shapes: emitLBVH by one workgroup each.
 __global__ __launch_bounds__(EMIT_BLOCK) void kr_emit_big(const LbvhScratch* S, uint32_t G) {
     REF_SCENE(S, G);
+    __shared__ uint32_t s_red[6];
     const uint32_t nbig = *s.bigt_n, T = *s.n_treelets, n = s.n;
+    CbAcc acc;
     for (uint32_t q = rj; q < nbig; q += G) {
         const uint32_t t = s.bigt[q];
         const uint32_t start = s.t_start[t];
         const uint32_t end = t + 1 < T ? s.t_start[t + 1] : n;
         emit_tree(s, s.keys_a, t, start, end - start);
         __syncthreads();
-        bbox b = box_empty();
         if (threadIdx.x == 0) {
-            b = load_box(s.t_box + 6 * (size_t)t);
+            const bbox b = load_box(s.t_box + 6 * (size_t)t);
             s.g_perm[t] = t;
             store_box(s.gp_box + 6 * (size_t)t, b);
+            acc.add(box_cen(b));
         }
-        if (T >= REF_CUT) ref_root_cb(s, threadIdx.x == 0, box_cen(b));
         __syncthreads();
+    }
+    cb_block_write(s, acc, REF_CB_PARTS + rj, s_red);
+}
+
+// The first level range's centroid bounds from the 2 REF_CB_PARTS partials.
+__global__ __launch_bounds__(64) void kr_root_cb(const LbvhScratch* S) {
+    const LbvhScratch& s = S[blockIdx.x];
+    if (*s.n_treelets < REF_CUT) return;
+    uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    for (uint32_t p = threadIdx.x; p < 2 * REF_CB_PARTS; p += 64)
+        for (int k = 0; k < 6; k++) {
+            const uint32_t x = s.cb_ord[8 + 6 * (size_t)p + k];
+            v[k] = k < 3 ? min(v[k], x) : max(v[k], x);
+        }
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        for (int off = 32; off >= 1; off >>= 1) {
+            v[k] = min(v[k], (uint32_t)__shfl_xor((int)v[k], off));
+            v[3 + k] = max(v[3 + k], (uint32_t)__shfl_xor((int)v[3 + k], off));
+        }
+    if (threadIdx.x == 0) {
+        uint32_t* rec = ref_rec(s, 0, 0);
+        for (int k = 0; k < 6; k++) rec[RG_CB + k] = v[k];
     }
 }
 
@@ -2705,8 +2742,11 @@ static void ref_launch_treelets(const LbvhScratch* S, uint32_t ns, const RefGrid
     hipLaunchKernelGGL(kr_treelets, dim3(ns * g.treelets), dim3(REF_BLOCK), 0, st, S, g.treelets);
 }
 static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g, hipStream_t st) {
-    hipLaunchKernelGGL(kr_emit_small, dim3(ns * g.treelets), dim3(REF_BLOCK), 0, st, S, g.treelets);
-    hipLaunchKernelGGL(kr_emit_big, dim3(ns * g.emit_big), dim3(EMIT_BLOCK), 0, st, S, g.emit_big);
+    (void)g;
+    // exactly REF_CB_PARTS blocks per scene each: every partial slot is written
+    hipLaunchKernelGGL(kr_emit_small, dim3(ns * REF_CB_PARTS), dim3(REF_BLOCK), 0, st, S, REF_CB_PARTS);
+    hipLaunchKernelGGL(kr_emit_big, dim3(ns * REF_CB_PARTS), dim3(EMIT_BLOCK), 0, st, S, REF_CB_PARTS);
+    hipLaunchKernelGGL(kr_root_cb, dim3(ns), dim3(64), 0, st, S);
 }
 // The level loop: four levels between host checks of the next level's range
 // count (kernels of an empty level return at once).

@@ -461,6 +461,7 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
         ta.spill = spills[k];
         ta.claim = claims[k];
         if (j) hipStreamWaitEvent(sj, r->evs[6 * (size_t)j - 2], 0);  // pass j - 1's shade1
+        if (j && !ctx->render_overlap) hipStreamWaitEvent(sj, r->evs[6 * (size_t)j], 0);  // all of pass j - 1
         hipEventRecord(e[0], sj);
         s = render_pass_launch(L, 0, sj);
         hipEventRecord(e[1], sj);
@@ -529,6 +530,12 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     r->stats[5] = cam_ms;
     r->stats[6] = (double)qtotal;
     r->stats[7] = prim_ms;
+    return WPBR_OK;
+}
+
+wpbr_status wpbr_set_render_overlap(wpbr_ctx* ctx, int overlap) {
+    if (!ctx || (overlap != 0 && overlap != 1)) return WPBR_ERR_INVALID_ARG;
+    ctx->render_overlap = overlap;
     return WPBR_OK;
 }
 

@@ -32,6 +32,7 @@ EXPORTS = [
     "wpbr_texture_level", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
     "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
+    "wpbr_set_render_overlap",
 ]
 
 # LBVH key orders (wpbr.h WPBR_LBVH_ORDER_*): the reference's (bits 0..23 of the
@@ -109,6 +110,7 @@ def load():
         "wpbr_lbvh_set_order": (i32, [vp, C.c_int]),
         "wpbr_lbvh_get_order": (C.c_int, [vp]),
         "wpbr_set_default_lbvh_order": (i32, [vp, C.c_int]),
+        "wpbr_set_render_overlap": (i32, [vp, C.c_int]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
@@ -219,6 +221,10 @@ class Context:
 
     def empty(self, shape, dtype):
         return DeviceArray(self, shape, dtype)
+
+    def set_render_overlap(self, overlap):
+        """1: consecutive render passes overlap on two streams; 0: one after another."""
+        self.check(self.lib.wpbr_set_render_overlap(self.h, int(overlap)))
 
     def set_default_lbvh_order(self, order):
         """Key order of LBVH objects (and scenes) created afterwards."""
