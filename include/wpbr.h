@@ -340,6 +340,16 @@ wpbr_status wpbr_texture_level(wpbr_ctx* ctx, const wpbr_texture* tex, uint32_t 
 wpbr_status wpbr_texture_lookup(wpbr_ctx* ctx, const wpbr_texture* tex, uint32_t n, const float* d_query,
                                 float* d_rgb);
 
+/* wpbr_film_loss without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc); no counterpart in the reference.  Same arguments and
+ * results as wpbr_film_loss (the loss bit for bit).  Its per-block partials and
+ * their sum live in a file-scope device array of the library, one copy per GPU,
+ * held until the process ends and sized for films of at most 16 777 216 pixels
+ * (larger films are WPBR_ERR_INVALID_ARG).  Calls that overlap on one GPU
+ * share that array and race; each call overwrites what it reads. */
+wpbr_status wpbr_film_loss_noalloc(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
+                                   const float* d_target_rgb, float* d_dL_drgb, float* d_rgb, double* loss);
 /* The film as an 8-bit RGB image (3 bytes per pixel, row-major, d_rgb8 on the
  * device): JobOutputFilmTitles (CSFilm.cpp:141-170) per pixel: black if the
  * accumulated radiance is black, else xyzToRGB(sum XYZ w) / (sum w * 800000),

@@ -632,4 +632,18 @@ wpbr_status film_loss_launch(const float4* film, const float* target, uint32_t n
     return launch_status();
 }
 
+// The loss from the per-block partials, summed as wpbr_film_loss sums them on
+// the host: in block order, each partial widened to double (no contraction).
+__global__ void k_loss_sum(const float* partial, uint32_t nb, double* out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double acc = 0.0;
+    for (uint32_t b = 0; b < nb; b++) acc += (double)partial[b];
+    *out = acc;
+}
+
+wpbr_status film_loss_sum_launch(const float* partial, uint32_t nb, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(64), 0, st, partial, nb, out);
+    return launch_status();
+}
+
 }  // namespace wpbr

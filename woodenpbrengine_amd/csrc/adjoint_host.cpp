@@ -17,6 +17,7 @@ wpbr_status adjoint_finish(const float* part, uint32_t rows, uint32_t n_mat, flo
 wpbr_status film_loss_launch(const float4* film, const float* target, uint32_t npix, float* rgb, float* dL,
                              float* partial, hipStream_t st);
 wpbr_status bsdf_grad_launch(const float* in, float* out, uint32_t n, hipStream_t st);
+wpbr_status film_loss_sum_launch(const float* partial, uint32_t nb, double* out, hipStream_t st);
 
 namespace {
 
@@ -202,6 +203,30 @@ wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_film, c
         if (loss) *loss = 0.0;
     }
     if (partial) hipFreeAsync(partial, ctx->stream);
+    return s;
+}
+
+wpbr_status wpbr_film_loss_noalloc(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_film, const float* d_target_rgb,
+                                   float* d_dL_drgb, float* d_rgb, double* loss) {
+    if (!ctx || !sc || !d_film) return WPBR_ERR_INVALID_ARG;
+    const uint32_t npix = sc->sv.width * sc->sv.height;
+    const uint32_t nb = (npix + 255) / 256;
+    if (nb > LOSS_NOALLOC_PARTIALS)
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "wpbr_film_loss_noalloc: films of at most 16777216 pixels");
+    if (!ctx->loss_scratch)
+        return set_err(ctx, WPBR_ERR_HIP, "wpbr_film_loss_noalloc: scratch symbol lookup failed at wpbr_open");
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    // every partial the sum reads is written by this call's k_film_loss
+    float* partial = (loss && d_target_rgb) ? ctx->loss_scratch : nullptr;
+    double* total = reinterpret_cast<double*>(ctx->loss_scratch + LOSS_NOALLOC_PARTIALS);
+    wpbr_status s = film_loss_launch((const float4*)d_film, d_target_rgb, npix, d_rgb, d_dL_drgb, partial, ctx->stream);
+    if (!s && partial) s = film_loss_sum_launch(partial, nb, total, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("film loss launch failed: ") + hipGetErrorString(last_launch_error));
+    double acc = 0.0;
+    if (partial) s = hip_check(ctx, hipMemcpyAsync(&acc, total, 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "film loss");
+    if (!s && loss) *loss = acc;
     return s;
 }
 

@@ -25,6 +25,30 @@ wpbr_status hip_check(wpbr_ctx* ctx, hipError_t e, const char* what) {
                    std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Scratch of wpbr_film_loss_noalloc: a file-scope device array (one copy per
+// GPU the library runs on, held until the process ends) with the per-block
+// loss partials of k_film_loss and, after them, the loss they sum to (a double,
+// 8-byte aligned).  In host code `scratch` names no device memory: kernels and
+// copies use the address hipGetSymbolAddress gives for the current device,
+// looked up once per device (at wpbr_open) and kept here.
+static __device__ __align__(256) float scratch[LOSS_NOALLOC_PARTIALS + 2];
+static_assert(LOSS_NOALLOC_PARTIALS % 2 == 0, "the double after the partials is 8-byte aligned");
+static std::mutex g_scratch_mu;
+static float* g_scratch[LOSS_NOALLOC_DEVICES] = {};
+
+// The device copy of `scratch` on `device` (which must be current); null if
+// the lookup failed.
+float* noalloc_scratch(int device) {
+    if (device < 0 || device >= LOSS_NOALLOC_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+        g_scratch[device] = static_cast<float*>(p);
+    }
+    return g_scratch[device];
+}
+
 // Bump allocator over one device allocation.
 struct Carver {
     char* base;
@@ -205,6 +229,7 @@ wpbr_status wpbr_open(int device, wpbr_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    c->loss_scratch = noalloc_scratch(device);  // null: wpbr_film_loss_noalloc reports it
     *out = c;
     return WPBR_OK;
 }

@@ -30,6 +30,7 @@ struct wpbr_ctx {
     wpbr_render_state* render = nullptr;
     int lbvh_order = WPBR_LBVH_ORDER_REFERENCE;  // key order of new LBVH objects
     int render_overlap = 1;   // wpbr_render: consecutive passes overlap on two streams
+    float* loss_scratch = nullptr;  // this device's copy of wpbr_film_loss_noalloc's scratch
     // segmented LBVH builds: per-scene descriptors and block maps
     void* batch_mem = nullptr;
     size_t batch_cap = 0;

@@ -164,6 +164,13 @@ void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts);
 wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint32_t order, uint32_t max_n,
                                   hipStream_t st);
 
+// wpbr_film_loss_noalloc: the file-scope scratch holds this many per-block loss
+// partials (256 pixels each: films of up to 16 777 216 pixels), then their sum
+// as a double; one copy per GPU, for up to this many devices.
+constexpr uint32_t LOSS_NOALLOC_PARTIALS = 65536;
+constexpr int LOSS_NOALLOC_DEVICES = 64;
+float* noalloc_scratch(int device);
+
 constexpr int LBVH_STAGES = 6;
 // SAH top: task list of the one-workgroup top kernel; workgroups of the
 // sub-range kernel

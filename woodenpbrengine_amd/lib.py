@@ -32,7 +32,7 @@ EXPORTS = [
     "wpbr_texture_level", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
     "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
-    "wpbr_set_render_overlap",
+    "wpbr_set_render_overlap", "wpbr_film_loss_noalloc",
 ]
 
 # LBVH key orders (wpbr.h WPBR_LBVH_ORDER_*): the reference's (bits 0..23 of the
@@ -126,6 +126,7 @@ def load():
         "wpbr_scene_param_counts": (i32, [vp, C.POINTER(C.c_uint32)]),
         "wpbr_scene_set_params": (i32, [vp, vp, vp, vp, vp]),
         "wpbr_film_loss": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]),
+        "wpbr_film_loss_noalloc": (i32, [vp, vp, vp, vp, vp, vp, C.POINTER(C.c_double)]),
         "wpbr_render_backward": (i32, [vp, vp, vp, vp, vp]),
         "wpbr_bsdf_derivatives": (i32, [vp, C.c_uint32, vp, vp]),
         "wpbr_lbvh_build_segmented": (i32, [vp, u32, C.POINTER(Geometry), C.POINTER(vp)]),
