@@ -263,6 +263,51 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* 
  * timing).  Films and samples are the same bit for bit either way.  No
  * counterpart in the reference, which renders one tile at a time. */
 wpbr_status wpbr_set_render_overlap(wpbr_ctx* ctx, int overlap);
+/* runCollisionSystem on a scene's geometry (MEngine.cpp:352-361) with the
+ * interactions it attaches to each CRayCast (CSufraceInteraction.h:36-199):
+ *   d_rays      6 x n SoA floats (CRayCast::ray: origin xyz, dir xyz), tmax = inf
+ *   d_surf      n bytes, CRayCast::bSurfInteraction (null: every ray 1)
+ *   d_ray_diff  12 x n SoA floats, the rays' CRayDifferential (CRayDifferential.h:7-22:
+ *               difXRay origin, dir, difYRay origin, dir), or null
+ *   d_hit_shape, d_hit_t   closest hit as wpbr_collide (0xFFFFFFFF / +inf: a miss)
+ *   d_si        WPBR_SI_FLOATS x n SoA floats (field f of ray i at d_si[f * n + i]),
+ *               or null: for a hit with bSurfInteraction the CSurfaceInteraction of
+ *               JobProcessTriangleFullInteractionRequests (CSTriangle.cpp:149-319) or
+ *               JobProcessSphereFullInteractionRequests (CSSphere.cpp:53-186, in
+ *               world space by CTransform.h:41-66), with the differentials of
+ *               JobComputeDifferentialsForSurfInter (CSufraceInteraction.h:98-148,
+ *               200-225) when d_ray_diff is given; for a hit without it the
+ *               CInteraction of JobProcessRayCastsResults (SLBVH.cpp:55-59: p, wo,
+ *               time); zeros on a miss (and in every field a job leaves default).
+ * Replaces the collision jobs of wpbr_collide plus the full-interaction and
+ * differential jobs. */
+enum {
+    WPBR_SI_P = 0, WPBR_SI_WO = 3, WPBR_SI_TIME = 6, WPBR_SI_N = 7, WPBR_SI_UV = 10,
+    WPBR_SI_DPDU = 12, WPBR_SI_DPDV = 15, WPBR_SI_DNDU = 18, WPBR_SI_DNDV = 21,
+    WPBR_SI_SHADING_N = 24, WPBR_SI_SHADING_DPDU = 27, WPBR_SI_SHADING_DPDV = 30,
+    WPBR_SI_SHADING_DNDU = 33, WPBR_SI_SHADING_DNDV = 36, WPBR_SI_DPDX = 39, WPBR_SI_DPDY = 42,
+    WPBR_SI_DUDX = 45, WPBR_SI_DVDX = 46, WPBR_SI_DUDY = 47, WPBR_SI_DVDY = 48,
+    WPBR_SI_FLOATS = 49
+};
+wpbr_status wpbr_scene_collide(wpbr_ctx* ctx, wpbr_scene* scene, uint32_t n, const float* d_rays,
+                               const uint8_t* d_surf, const float* d_ray_diff, uint32_t* d_hit_shape,
+                               float* d_hit_t, float* d_si);
+/* The same over bare geometry and its LBVH (as wpbr_collide), with the
+ * triangles' mesh attributes as CTriangleMesh holds them after
+ * STriangleMesh::create (device arrays, per triangle, triangle order):
+ * d_tri_flags bit0 = the uvs are given, bit1 = the normals and tangents are
+ * given (a null d_tri_flags or a null attrs: none, default uvs (0,0), (1,0),
+ * (1,1) as CSTriangle.cpp:180-185). */
+typedef struct {
+    const uint8_t* d_tri_flags;   /* n_triangles */
+    const float* d_tri_uv;        /* n_triangles x 6: uv of vertices 0, 1, 2 */
+    const float* d_tri_normal;    /* n_triangles x 9 */
+    const float* d_tri_tangent;   /* n_triangles x 9 */
+} wpbr_tri_attributes;
+wpbr_status wpbr_collide_interactions(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_tri_attributes* attrs,
+                                      const wpbr_lbvh* bvh, uint32_t n, const float* d_rays,
+                                      const uint8_t* d_surf, const float* d_ray_diff, uint32_t* d_hit_shape,
+                                      float* d_hit_t, float* d_si);
 /* Statistics of the last wpbr_render: [0] total ms, [1] trace ms (all trace
  * launches), [2] number of trace launches, [3] rays traced, [4] shade ms,
  * [5] camera ms, [6] secondary rays, [7] trace ms of the primary rays.

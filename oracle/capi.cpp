@@ -265,6 +265,14 @@ static SceneHold* make_scene(const orc_scene_desc* d) {
 
 extern "C" {
 
+// The scene's closest hits and interactions for n ray casts (interaction.cpp).
+void orc_scene_collide(const orc_scene_desc* d, int nthreads, uint32_t n, const float* rays, const uint8_t* surf,
+                       const float* diff, uint32_t* hit_shape, float* hit_t, float* si) {
+    SceneHold* H = make_scene(d);
+    scene_collide(H->sc, H->bvh, n, rays, surf, diff, hit_shape, hit_t, si, nthreads);
+    delete H;
+}
+
 void* orc_render(const orc_scene_desc* d, int nthreads, const uint32_t* tiles, uint32_t n_tiles) {
     SceneHold* H = make_scene(d);
     RenderOut* out = new RenderOut();

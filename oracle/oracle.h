@@ -163,6 +163,25 @@ struct RenderOut {
                                      // clamped light picks, env picks, render ns
 };
 
+// ---- interactions at the collision boundary (interaction.cpp) ----------------
+// CSurfaceInteraction (CSufraceInteraction.h:44-199) as SoA floats, field f of
+// ray i at si[f * n + i] (the layout of wpbr.h's WPBR_SI_*).
+enum {
+    SI_P = 0, SI_WO = 3, SI_TIME = 6, SI_N = 7, SI_UV = 10, SI_DPDU = 12, SI_DPDV = 15, SI_DNDU = 18,
+    SI_DNDV = 21, SI_SHADING_N = 24, SI_SHADING_DPDU = 27, SI_SHADING_DPDV = 30, SI_SHADING_DNDU = 33,
+    SI_SHADING_DNDV = 36, SI_DPDX = 39, SI_DPDY = 42, SI_DUDX = 45, SI_DVDX = 46, SI_DUDY = 47, SI_DVDY = 48,
+    SI_FLOATS = 49
+};
+// runCollisionSystem's results per ray cast: the closest hit (shape, t) and,
+// when si is given, the interaction the reference attaches (a surface
+// interaction for bSurfInteraction rays, surf == null meaning all; the plain
+// CInteraction fields p, wo, time otherwise; zeros on a miss) with the
+// differentials of the rays that carry a CRayDifferential (diff: 12 x n SoA,
+// difXRay origin, dir, difYRay origin, dir; null: none).
+void scene_collide(const RenderScene& sc, const LbvhResult& bvh, uint32_t n, const float* rays,
+                   const uint8_t* surf, const float* diff, uint32_t* hit_shape, float* hit_t, float* si,
+                   int nthreads);
+
 // Every discrete decision of one sample (the finite-difference reference of
 // diff.cpp re-evaluates the estimator with them frozen).
 struct EnvCell {

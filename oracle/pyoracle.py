@@ -217,6 +217,30 @@ def scene_desc(s):
     return d, keep
 
 
+SI_FLOATS = 49
+
+
+def scene_collide(s, rays, surf=None, diff=None, nthreads=None):
+    """runCollisionSystem on a render scene's geometry: closest hit shape and t
+    per ray, and the interaction each ray cast gets (SI_FLOATS x n SoA floats,
+    the layout of wpbr.h's WPBR_SI_*)."""
+    lib = load()
+    lib.orc_scene_collide.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_void_p]
+    d, keep = scene_desc(s)
+    rays = _f32(rays, (6, -1))
+    n = rays.shape[1]
+    sf = None if surf is None else np.ascontiguousarray(surf, np.uint8)
+    df = None if diff is None else _f32(diff, (12, n))
+    shape = np.empty(n, np.uint32)
+    t = np.empty(n, np.float32)
+    si = np.empty((SI_FLOATS, n), np.float32)
+    lib.orc_scene_collide(C.byref(d), nthreads or default_threads(), n, _p(rays), _p(sf), _p(df), _p(shape),
+                          _p(t), _p(si))
+    del keep
+    return shape, t, si
+
+
 def render(s, nthreads=None, tiles=None):
     """Render a scene on the CPU; returns a dict of numpy arrays."""
     lib = load()

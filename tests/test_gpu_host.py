@@ -2,7 +2,9 @@
 the reference's JOB_RUN sequences MEngine::buildLBVH and runCollisionSystem
 run through the staged C ABI by a plain C++ program, checked against the
 oracle: node arrays and shapesEntities bit-exact, interactions bit-exact
-(hit shape and t; p = o + d t and wo = -d in float32)."""
+(hit shape and t; p = o + d t and wo = -d in float32), and with
+bSurfInteraction rays, ray differentials and mesh attributes the
+CSurfaceInteraction of every ray bit-exact."""
 import subprocess
 
 import numpy as np
@@ -27,7 +29,7 @@ def test_job_sequences_match_oracle(tmp_path):
     n = rays.shape[1]
     inp = tmp_path / "in.bin"
     with open(inp, "wb") as f:
-        f.write(np.array([len(sr), len(tris), n], np.uint32).tobytes())
+        f.write(np.array([len(sr), len(tris), n, 0], np.uint32).tobytes())
         f.write(sx.astype(np.float32).tobytes())
         f.write(sr.tobytes())
         f.write(tris.astype(np.float32).tobytes())
@@ -61,3 +63,66 @@ def test_job_sequences_match_oracle(tmp_path):
     p = rays[:3].T[m] + rays[3:].T[m] * ot[m, None]
     np.testing.assert_array_equal(inter[m, 2:5], p.astype(np.float32))
     np.testing.assert_array_equal(inter[m, 5:8], -rays[3:].T[m])
+
+
+def test_job_sequences_surface_interactions(tmp_path):
+    """runCollisionSystem with bSurfInteraction rays (80 %), ray differentials
+    and per-triangle uvs, normals and tangents on half of the triangles: the
+    mirror's interactions (the full-interaction jobs and
+    JobComputeDifferentialsForSurfInter) equal the oracle's field by field."""
+    assert DEMO.exists(), "build the host demo first (__graft_entry__.build())"
+    g = np.random.default_rng(52)
+    tris = scenes.random_triangles(8000, 53, median_edge=0.03)
+    k = len(tris) // 2
+    uv = g.random((len(tris), 6)).astype(np.float32)
+    nrm = g.normal(size=(len(tris), 9)).astype(np.float32)
+    tan = g.normal(size=(len(tris), 9)).astype(np.float32)
+    flags = np.zeros(len(tris), np.uint8)
+    flags[:k] = 3
+    uv[k:], nrm[k:], tan[k:] = 0, 0, 0
+    cs = g.random((3, 3))
+    sr = np.array([0.05, 0.08, 0.03], np.float32)
+    sx = np.stack([scenes.translate_xform(c) for c in cs]).astype(np.float32)
+    rays = np.concatenate([scenes.pinhole_rays(64, 32, (0.5, 0.5, -1.0)),
+                           scenes.random_rays(4096, 54)], axis=1).astype(np.float32)
+    n = rays.shape[1]
+    surf = (g.random(n) < 0.8).astype(np.uint8)
+    o, d = rays[:3], rays[3:]
+    diff = np.concatenate([o + np.float32(1e-3), d + g.normal(scale=1e-3, size=d.shape).astype(np.float32),
+                           o - np.float32(1e-3), d + g.normal(scale=1e-3, size=d.shape).astype(np.float32)])
+    diff = diff.astype(np.float32)
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as f:
+        f.write(np.array([len(sr), len(tris), n, 7], np.uint32).tobytes())
+        f.write(sx.tobytes())
+        f.write(sr.tobytes())
+        f.write(tris.astype(np.float32).tobytes())
+        f.write(np.ascontiguousarray(rays.T).tobytes())
+        f.write(surf.tobytes())
+        f.write(np.ascontiguousarray(diff.T).tobytes())
+        f.write(flags.tobytes())
+        for a in (uv, nrm, tan):
+            f.write(np.ascontiguousarray(a).tobytes())
+    out = tmp_path / "out.bin"
+    subprocess.run([str(DEMO), str(inp), str(out)], check=True, timeout=120)
+    buf = out.read_bytes()
+    nn, ns = np.frombuffer(buf, np.uint32, 2)
+    o_ = 8 + 12 * nn + 24 * nn + 4 * ns + 32 * n
+    si = np.frombuffer(buf, np.float32, 49 * n, o_).reshape(n, 49).T
+    # the same scene as a render scene for the oracle's interactions
+    rs = scenes.RenderScene()
+    m = rs.material(scenes.MAT_DIELECTRIC)
+    for c, r in zip(cs, sr):
+        rs.add_sphere(c, float(r), m)
+    rs.add_mesh(tris[:k], m, uv[:k], nrm[:k], tan[:k])
+    rs.add_mesh(tris[k:], m)
+    rs.cam = scenes.camera(32, 32, (0.5, 0.5, -1.0), (0.5, 0.5, 0.5))
+    rs.width, rs.height, rs.spp, rs.seed = 32, 32, 1, 1
+    rs = rs.finalize()
+    np.testing.assert_array_equal(rs.sphere_xform, sx)
+    es, et, esi = po.scene_collide(rs, rays, surf=surf, diff=diff)
+    same = (si.view(np.uint32) == esi.view(np.uint32)) | (np.isnan(si) & np.isnan(esi))
+    bad = np.nonzero(~same.all(axis=1))[0]
+    print("surface interactions: %d; fields differing: %s" % (((es != 0xFFFFFFFF) & (surf == 1)).sum(), bad),
+          flush=True)
+    assert same.all()

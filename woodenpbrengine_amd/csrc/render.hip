@@ -923,6 +923,247 @@ __global__ __launch_bounds__(256) void k_film_rgb8(const float4* __restrict__ fi
     }
 }
 
+// ---- interactions at the collision boundary (wpbr_scene_collide) ------------------
+// The interaction runCollisionSystem attaches to each ray cast, in the SoA layout
+// of WPBR_SI_* (wpbr.h): CSurfaceInteraction for bSurfInteraction rays
+// (CSTriangle.cpp:149-319; CSSphere.cpp:122-186 then CTransform.h:41-66), the plain
+// CInteraction fields for the others (SLBVH.cpp:55-59), zeros on a miss; the
+// differentials of the rays that carry a CRayDifferential
+// (CSufraceInteraction.h:98-148).  Default-constructed members are zero: the
+// sphere's world interaction keeps uv = 0 and dpdx = dpdy = 0 (the transform
+// leaves uv uncopied and sets dpdy from dpdx), and a triangle's geometric
+// dndu / dndv stay 0.
+struct FullSI {
+    f3 p, wo, n, dpdu, dpdv, dndu, dndv, sn, sdpdu, sdpdv, sdndu, sdndv, dpdx, dpdy;
+    float time, u, v, dudx, dvdx, dudy, dvdy;
+};
+
+WPBR_D FullSI fsi_zero() {
+    FullSI s;
+    const f3 z = mk3(0.0f, 0.0f, 0.0f);
+    s.p = s.wo = s.n = s.dpdu = s.dpdv = s.dndu = s.dndv = z;
+    s.sn = s.sdpdu = s.sdpdv = s.sdndu = s.sdndv = s.dpdx = s.dpdy = z;
+    s.time = s.u = s.v = s.dudx = s.dvdx = s.dudy = s.dvdy = 0.0f;
+    return s;
+}
+
+// CSurfaceInteraction's constructor (CSufraceInteraction.h:58-83).
+WPBR_D void fsi_construct(FullSI& s, f3 p, float u, float v, f3 wo, f3 dpdu, f3 dpdv, f3 dndu, f3 dndv,
+                          float time) {
+    s.p = p; s.u = u; s.v = v; s.wo = wo; s.time = time;
+    s.dpdu = dpdu; s.dpdv = dpdv; s.dndu = dndu; s.dndv = dndv;
+    s.n = normalize3(cross3(dpdu, dpdv));
+    s.sn = s.n; s.sdpdu = dpdu; s.sdpdv = dpdv; s.sdndu = dndu; s.sdndv = dndv;
+}
+
+WPBR_D FullSI si_full_triangle(const SceneView& sv, uint32_t tri, f3 rd, const float* b, float t) {
+    const float* v = sv.g.tri_verts + 9 * (size_t)tri;
+    const f3 P0 = mk3(v[0], v[1], v[2]), P1 = mk3(v[3], v[4], v[5]), P2 = mk3(v[6], v[7], v[8]);
+    const uint8_t flags = sv.tri_flags ? sv.tri_flags[tri] : (uint8_t)0;
+    float uv[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+    if (flags & 1) {
+        const float* u = sv.tri_uv + 6 * (size_t)tri;
+        for (int i = 0; i < 3; i++) { uv[i][0] = u[2 * i]; uv[i][1] = u[2 * i + 1]; }
+    }
+    const f3 dp02 = sub3(P0, P2), dp12 = sub3(P1, P2);
+    float duv02[2] = {uv[0][0] - uv[2][0], uv[0][1] - uv[2][1]};
+    float duv12[2] = {uv[1][0] - uv[2][0], uv[1][1] - uv[2][1]};
+    float det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+    if (det == 0) {
+        const float k = (float)(1.0 * 0.001);
+        uv[0][0] = 0; uv[0][1] = 0; uv[1][0] = k; uv[1][1] = 0; uv[2][0] = k; uv[2][1] = k;
+        duv02[0] = uv[0][0] - uv[2][0]; duv02[1] = uv[0][1] - uv[2][1];
+        duv12[0] = uv[1][0] - uv[2][0]; duv12[1] = uv[1][1] - uv[2][1];
+        det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+    }
+    const float detInv = 1 / det;
+    const f3 dpdu = mul3(sub3(mul3(dp02, duv12[1]), mul3(dp12, duv02[1])), detInv);
+    const f3 dpdv = mul3(add3(mul3(neg3(dp02), duv12[0]), mul3(dp12, duv02[0])), detInv);
+    const f3 pHit = add3(add3(mul3(P0, b[0]), mul3(P1, b[1])), mul3(P2, b[2]));
+    const float uh = (uv[0][0] * b[0] + uv[1][0] * b[1]) + uv[2][0] * b[2];
+    const float vh = (uv[0][1] * b[0] + uv[1][1] * b[1]) + uv[2][1] * b[2];
+    FullSI s = fsi_zero();
+    const f3 z = mk3(0.0f, 0.0f, 0.0f);
+    fsi_construct(s, pHit, uh, vh, neg3(rd), dpdu, dpdv, z, z, t);
+    s.n = s.sn = normalize3(cross3(dp02, dp12));
+    if (flags & 2) {
+        const float* nn = sv.tri_normal + 9 * (size_t)tri;
+        const float* tt = sv.tri_tangent + 9 * (size_t)tri;
+        const f3 N0 = mk3(nn[0], nn[1], nn[2]), N1 = mk3(nn[3], nn[4], nn[5]), N2 = mk3(nn[6], nn[7], nn[8]);
+        const f3 T0 = mk3(tt[0], tt[1], tt[2]), T1 = mk3(tt[3], tt[4], tt[5]), T2 = mk3(tt[6], tt[7], tt[8]);
+        f3 ns = add3(add3(mul3(N0, b[0]), mul3(N1, b[1])), mul3(N2, b[2]));
+        ns = is_zero(ns) ? s.n : normalize3(ns);
+        f3 ss = add3(add3(mul3(T0, b[0]), mul3(T1, b[1])), mul3(T2, b[2]));
+        ss = is_zero(ss) ? normalize3(s.dpdu) : normalize3(ss);
+        f3 ts = cross3(ss, ns);
+        if (len2(ts) > 0) {
+            ts = normalize3(ts);
+            ss = cross3(ts, ns);
+        }
+        const f3 dn02 = sub3(N0, N2), dn12 = sub3(N1, N2);
+        f3 dndu = z, dndv = z;
+        const float det2 = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+        if (det2 != 0) {
+            const float inv2 = 1 / det2;
+            dndu = mul3(sub3(mul3(dn02, duv12[1]), mul3(dn12, duv02[1])), inv2);
+            dndv = mul3(add3(mul3(neg3(dn02), duv12[0]), mul3(dn12, duv02[0])), inv2);
+        }
+        s.sn = normalize3(cross3(ss, ts));
+        s.sdpdu = ss;
+        s.sdpdv = ts;
+        s.sdndu = dndu;
+        s.sdndv = dndv;
+        s.n = dot3(s.n, s.sn) < 0.0f ? neg3(s.n) : s.n;
+    }
+    return s;
+}
+
+WPBR_D FullSI si_full_sphere(const SceneView& sv, uint32_t sid, f3 o, f3 d, float t) {
+    const float* xf = sv.g.sphere_xform + 24 * (size_t)sid;
+    const affine W = load_affine(xf), Inv = load_affine(xf + 12);
+    const float radius = sv.g.sphere_radius[sid];
+    const f3 oL = xf_point(Inv, o), dL = xf_vector(Inv, d);
+    const f3 pH = add3(oL, mul3(dL, t));
+    float phi = cr_atan2(pH.y, pH.x);
+    if (phi < 0.0f) phi = (float)((double)phi + 2 * PI_REF);
+    const float u = (float)((double)phi / (2 * PI_REF));
+    const float theta = cr_acos(pH.z / radius);
+    const float v = (float)((double)theta / PI_REF);
+    const float invZR = (float)(1.0 / (double)sqrtf(pH.x * pH.x + pH.y * pH.y));
+    const float cphi = pH.x * invZR, sphi = pH.y * invZR;
+    const f3 dpdu = mul3(mk3(-pH.y, pH.x, 0.0f), (float)(2 * PI_REF));
+    const f3 dpdv = mul3(mk3(pH.z * cphi, pH.z * sphi, -radius * cr_sin(theta)), (float)PI_REF);
+    const f3 d2Pduu = mul3(mk3(pH.x, pH.y, 0.0f), (float)(-2 * PI_REF * 2 * PI_REF));
+    const f3 d2Pduv = mul3(mul3(mul3(mul3(mk3(-sphi, cphi, 0.0f), (float)PI_REF), pH.z), 2.0f), (float)PI_REF);
+    const f3 d2Pdvv = mul3(mul3(mk3(pH.x, pH.y, pH.z), (float)(-PI_REF)), (float)PI_REF);
+    const float E = dot3(dpdu, dpdu), F = dot3(dpdu, dpdv), G = dot3(dpdv, dpdv);
+    const f3 N = normalize3(cross3(dpdu, dpdv));
+    const float e = dot3(N, d2Pduu), f = dot3(N, d2Pduv), g = dot3(N, d2Pdvv);
+    const float invEGF2 = 1 / (E * G - F * F);
+    const f3 dndu = add3(mul3(dpdu, (f * F - e * G) * invEGF2), mul3(dpdv, (e * F - f * E) * invEGF2));
+    const f3 dndv = add3(mul3(dpdu, (g * F - f * G) * invEGF2), mul3(dpdv, (f * F - g * E) * invEGF2));
+    FullSI l = fsi_zero();
+    fsi_construct(l, pH, u, v, neg3(dL), dpdu, dpdv, dndu, dndv, t);
+    // CTransform::operator()(const CSurfaceInteraction&) into a fresh interaction
+    FullSI s = fsi_zero();
+    s.time = l.time;
+    s.dndu = xf_normal(Inv, l.dndu);
+    s.dndv = xf_normal(Inv, l.dndv);
+    s.dpdu = xf_vector(W, l.dpdu);
+    s.dpdv = xf_vector(W, l.dpdv);
+    s.p = xf_point(W, l.p);
+    s.dpdx = xf_vector(W, l.dpdx);
+    s.dpdy = xf_vector(W, l.dpdx);
+    s.n = normalize3(xf_normal(Inv, l.n));
+    s.wo = normalize3(xf_vector(W, l.wo));
+    s.sdndu = xf_vector(W, l.sdndu);
+    s.sdndv = xf_vector(W, l.sdndv);
+    s.sdpdu = xf_vector(W, l.sdpdu);
+    s.sdpdv = xf_vector(W, l.sdpdv);
+    s.sn = normalize3(xf_normal(Inv, l.sn));
+    return s;
+}
+
+// pbrt-v3 SolveLinearSystem2x2 (WoodenMath's solveLinearSystem2x2).
+WPBR_D bool solve2x2(float a00, float a01, float a10, float a11, float b0, float b1, float* x0, float* x1) {
+    const float det = a00 * a11 - a01 * a10;
+    if (fabsf(det) < 1e-10f) return false;
+    *x0 = (a11 * b0 - a01 * b1) / det;
+    *x1 = (a00 * b1 - a10 * b0) / det;
+    return !(__builtin_isnan(*x0) || __builtin_isnan(*x1));
+}
+
+// CSurfaceInteraction::computeDifferentials (CSufraceInteraction.h:98-148).
+WPBR_D void si_differentials(FullSI& s, f3 xo, f3 xd, f3 yo, f3 yd) {
+    const float dd = dot3(s.n, s.p);
+    const float tx = -(dot3(s.n, xo) - dd) / dot3(s.n, xd);
+    const f3 px = add3(xo, mul3(xd, tx));
+    const float ty = -(dot3(s.n, yo) - dd) / dot3(s.n, yd);
+    const f3 py = add3(yo, mul3(yd, ty));
+    s.dpdx = sub3(px, s.p);
+    s.dpdy = sub3(py, s.p);
+    int d0, d1;
+    if (fabsf(s.n.x) > fabsf(s.n.y) && fabsf(s.n.x) > fabsf(s.n.z)) { d0 = 1; d1 = 2; }
+    else if (fabsf(s.n.y) > fabsf(s.n.z)) { d0 = 0; d1 = 2; }
+    else { d0 = 0; d1 = 1; }
+    const float a00 = comp(s.dpdu, d0), a01 = comp(s.dpdv, d0), a10 = comp(s.dpdu, d1), a11 = comp(s.dpdv, d1);
+    if (!solve2x2(a00, a01, a10, a11, comp(px, d0) - comp(s.p, d0), comp(px, d1) - comp(s.p, d1), &s.dudx, &s.dvdx))
+        s.dudx = s.dvdx = 0;
+    if (!solve2x2(a00, a01, a10, a11, comp(py, d0) - comp(s.p, d0), comp(py, d1) - comp(s.p, d1), &s.dudy, &s.dvdy))
+        s.dudy = s.dvdy = 0;
+}
+
+WPBR_D void put3(float* si, size_t n, size_t i, int f, f3 v) {
+    si[(size_t)f * n + i] = v.x;
+    si[(size_t)(f + 1) * n + i] = v.y;
+    si[(size_t)(f + 2) * n + i] = v.z;
+}
+
+// One thread per ray cast; the traversal left the barycentrics of its closest
+// triangle hit in rows WPBR_SI_DUDX .. +2 of si (each thread reads its own
+// ray's before writing them).
+__global__ __launch_bounds__(256) void k_scene_si(SceneView sv, uint32_t n, const float* __restrict__ rays,
+                                                  const uint8_t* __restrict__ surf, const float* __restrict__ diff,
+                                                  const uint32_t* __restrict__ hit_shape,
+                                                  const float* __restrict__ hit_t, float* __restrict__ si) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const f3 o = mk3(rays[i], rays[(size_t)n + i], rays[2 * (size_t)n + i]);
+    const f3 d = mk3(rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]);
+    const uint32_t shape = hit_shape[i];
+    const float t = hit_t[i];
+    const bool want_si = !surf || surf[i];
+    float b[3];
+    for (int k = 0; k < 3; k++) b[k] = si[(size_t)(WPBR_SI_DUDX + k) * n + i];
+    FullSI s = fsi_zero();
+    if (shape != 0xFFFFFFFFu) {
+        if (!want_si) {
+            s.p = add3(o, mul3(d, t));
+            s.time = t;
+            s.wo = neg3(d);
+        } else if (shape < sv.g.n_spheres) {
+            s = si_full_sphere(sv, shape, o, d, t);
+        } else {
+            s = si_full_triangle(sv, shape - sv.g.n_spheres, d, b, t);
+        }
+        if (diff && want_si) {
+            auto g = [&](int k) { return diff[(size_t)k * n + i]; };
+            si_differentials(s, mk3(g(0), g(1), g(2)), mk3(g(3), g(4), g(5)), mk3(g(6), g(7), g(8)),
+                             mk3(g(9), g(10), g(11)));
+        }
+    }
+    put3(si, n, i, WPBR_SI_P, s.p);
+    put3(si, n, i, WPBR_SI_WO, s.wo);
+    si[(size_t)WPBR_SI_TIME * n + i] = s.time;
+    put3(si, n, i, WPBR_SI_N, s.n);
+    si[(size_t)WPBR_SI_UV * n + i] = s.u;
+    si[(size_t)(WPBR_SI_UV + 1) * n + i] = s.v;
+    put3(si, n, i, WPBR_SI_DPDU, s.dpdu);
+    put3(si, n, i, WPBR_SI_DPDV, s.dpdv);
+    put3(si, n, i, WPBR_SI_DNDU, s.dndu);
+    put3(si, n, i, WPBR_SI_DNDV, s.dndv);
+    put3(si, n, i, WPBR_SI_SHADING_N, s.sn);
+    put3(si, n, i, WPBR_SI_SHADING_DPDU, s.sdpdu);
+    put3(si, n, i, WPBR_SI_SHADING_DPDV, s.sdpdv);
+    put3(si, n, i, WPBR_SI_SHADING_DNDU, s.sdndu);
+    put3(si, n, i, WPBR_SI_SHADING_DNDV, s.sdndv);
+    put3(si, n, i, WPBR_SI_DPDX, s.dpdx);
+    put3(si, n, i, WPBR_SI_DPDY, s.dpdy);
+    si[(size_t)WPBR_SI_DUDX * n + i] = s.dudx;
+    si[(size_t)WPBR_SI_DVDX * n + i] = s.dvdx;
+    si[(size_t)WPBR_SI_DUDY * n + i] = s.dudy;
+    si[(size_t)WPBR_SI_DVDY * n + i] = s.dvdy;
+}
+
+wpbr_status scene_si_launch(const SceneView& sv, uint32_t n, const float* rays, const uint8_t* surf,
+                            const float* diff, const uint32_t* hit_shape, const float* hit_t, float* si,
+                            hipStream_t st) {
+    if (!n) return WPBR_OK;
+    hipLaunchKernelGGL(k_scene_si, dim3((n + 255) / 256), dim3(256), 0, st, sv, n, rays, surf, diff, hit_shape,
+                       hit_t, si);
+    return launch_status();
+}
+
 wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st) {
     if (npix) hipLaunchKernelGGL(k_film_rgb8, dim3((npix + 255) / 256), dim3(256), 0, st,
                                  reinterpret_cast<const float4*>(film), npix, out);

@@ -533,6 +533,82 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     return WPBR_OK;
 }
 
+}  // extern "C"
+
+// Closest hits of n ray casts over the tree b of geometry sv.g, then (d_si) the
+// interaction of each from sv's per-triangle attributes (the only other fields
+// of sv it reads).
+static wpbr_status collide_interactions(wpbr_ctx* ctx, const SceneView& sv, const wpbr_lbvh* b, uint32_t n,
+                                        const float* d_rays, const uint8_t* d_surf, const float* d_ray_diff,
+                                        uint32_t* d_hit_shape, float* d_hit_t, float* d_si) {
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();  // launch checks below must not see stale errors
+    if (!b->built) return set_err(ctx, WPBR_ERR_NOT_BUILT, "lbvh not built");
+    wpbr_status s = ctx_ensure_trace(ctx, n);
+    if (s || !n) return s;
+    TraceArgs ta;
+    ta.g = sv.g;
+    ta.trav = (const float4*)b->s.trav;
+    ta.trav4 = (const float4*)b->s.trav4;
+    ta.prims = (const float4*)b->s.prims;
+    ta.root_box = b->s.node_bounds;
+    ta.root_word = b->root_word;
+    ta.n = n;
+    ta.n_dev = nullptr;
+    ta.stride = 0;
+    ta.rays = d_rays;
+    ta.hit_shape = d_hit_shape;
+    ta.hit_t = d_hit_t;
+    // the closest triangle's barycentrics go to the rows the interaction kernel
+    // fills last (it reads each ray's before writing them)
+    ta.bary = d_si ? d_si + (size_t)WPBR_SI_DUDX * n : nullptr;
+    ta.target = nullptr;
+    ta.spill = ctx->spill;
+    ta.spill_depth = ctx->spill_depth;
+    ta.err = ctx->err_flag;
+    const uint32_t dyn_blocks = ctx_trace_shape(ctx, ta);
+    hipEventRecord(ctx->ev0, ctx->stream);
+    s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, ctx->stream, dyn_blocks);
+    if (!s && d_si)
+        s = scene_si_launch(sv, n, d_rays, d_surf, d_ray_diff, d_hit_shape, d_hit_t, d_si, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("collide launch failed: ") + hipGetErrorString(last_launch_error));
+    s = ctx_finish_trace(ctx);
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
+}
+
+extern "C" {
+
+wpbr_status wpbr_scene_collide(wpbr_ctx* ctx, wpbr_scene* sc, uint32_t n, const float* d_rays,
+                               const uint8_t* d_surf, const float* d_ray_diff, uint32_t* d_hit_shape,
+                               float* d_hit_t, float* d_si) {
+    if (!ctx || !sc || (n && (!d_rays || !d_hit_shape || !d_hit_t))) return WPBR_ERR_INVALID_ARG;
+    return collide_interactions(ctx, sc->sv, sc->bvh, n, d_rays, d_surf, d_ray_diff, d_hit_shape, d_hit_t, d_si);
+}
+
+wpbr_status wpbr_collide_interactions(wpbr_ctx* ctx, const wpbr_geometry* geom, const wpbr_tri_attributes* attrs,
+                                      const wpbr_lbvh* bvh, uint32_t n, const float* d_rays, const uint8_t* d_surf,
+                                      const float* d_ray_diff, uint32_t* d_hit_shape, float* d_hit_t,
+                                      float* d_si) {
+    if (!ctx || !geom || !bvh || (n && (!d_rays || !d_hit_shape || !d_hit_t))) return WPBR_ERR_INVALID_ARG;
+    if ((uint64_t)geom->n_spheres + geom->n_triangles != bvh->n)
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "geometry does not match the lbvh");
+    if (attrs && attrs->d_tri_flags && geom->n_triangles &&
+        (!attrs->d_tri_uv || !attrs->d_tri_normal || !attrs->d_tri_tangent))
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "triangle flags given without uv, normal and tangent arrays");
+    SceneView sv;
+    std::memset(&sv, 0, sizeof(sv));
+    sv.g = to_view(geom);
+    if (attrs) {
+        sv.tri_flags = attrs->d_tri_flags;
+        sv.tri_uv = attrs->d_tri_uv;
+        sv.tri_normal = attrs->d_tri_normal;
+        sv.tri_tangent = attrs->d_tri_tangent;
+    }
+    return collide_interactions(ctx, sv, bvh, n, d_rays, d_surf, d_ray_diff, d_hit_shape, d_hit_t, d_si);
+}
+
 wpbr_status wpbr_set_render_overlap(wpbr_ctx* ctx, int overlap) {
     if (!ctx || (overlap != 0 && overlap != 1)) return WPBR_ERR_INVALID_ARG;
     ctx->render_overlap = overlap;

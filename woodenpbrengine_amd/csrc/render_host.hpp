@@ -10,6 +10,9 @@ namespace wpbr {
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
 wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st);
+wpbr_status scene_si_launch(const SceneView& sv, uint32_t n, const float* rays, const uint8_t* surf,
+                            const float* diff, const uint32_t* hit_shape, const float* hit_t, float* si,
+                            hipStream_t st);
 wpbr_status sample_arrays_launch(const SamplerDef& S, uint32_t pixel0, uint32_t npix, float* u1, float* u2,
                                  hipStream_t st);
 SamplerDef make_sampler(uint32_t seed, uint32_t mode, uint32_t n1, uint32_t n2);

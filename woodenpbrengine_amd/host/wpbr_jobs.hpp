@@ -40,11 +40,34 @@ struct CRayCast {  // CSufraceInteraction.h:36-42
     DRayf ray;
     bool bSurfInteraction;
 };
-struct CInteraction {  // CSufraceInteraction.h:16-34
+struct CInteraction {  // CSufraceInteraction.h:44-51
     DPoint3f p;
     DVector3f wo;
     uint32_t hCollision;  // shape entity, INVALID_HANDLE on a miss
     float time;
+};
+using DNormal3f = DVector3f;
+struct DVector2f {
+    float x, y;
+};
+// CSufraceInteraction.h:54-199 (its members in declaration order).
+struct CSurfaceInteraction : CInteraction {
+    DNormal3f n;
+    DVector2f uv;
+    DVector3f dpdu, dpdv;
+    DNormal3f dndu, dndv;
+    struct {
+        DNormal3f n;
+        DVector3f dpdu, dpdv;
+        DVector3f dndu, dndv;
+    } shading;
+    DVector3f dpdx, dpdy;
+    float dudx = 0, dvdx = 0, dudy = 0, dvdy = 0;
+};
+// CRayDifferential.h:7-22 over DRayDifferential.h:8-31: the ray and its x / y
+// neighbours.
+struct CRayDifferential : DRayf {
+    DRayf difXRay, difYRay;
 };
 struct LinearBVHNode {  // SLBVH.h:17-27
     uint32_t offset;    // shapes offset (leaf) or second child offset (interior)
@@ -87,9 +110,12 @@ struct JobProcessRayCasts { void run(MEngine* e); };                       // SL
 struct JobProcessSphereSurfInteractionRequests { void run(MEngine*) {} };   // in the traversal
 struct JobProcessTriangleInteractionRequests { void run(MEngine*) {} };     // in the traversal
 struct JobProcessRayCastsResults { void run(MEngine* e); };                // SLBVH.cpp:9-66
-struct JobProcessSphereFullInteractionRequests { void run(MEngine*) {} };   // render path only
-struct JobProcessTriangleFullInteractionRequests { void run(MEngine*) {} }; // render path only
+struct JobProcessSphereFullInteractionRequests { void run(MEngine* e); };   // CSSphere.cpp:53-186
+struct JobProcessTriangleFullInteractionRequests { void run(MEngine* e); }; // CSTriangle.cpp:149-319
 struct JobCollisionFinish { void run(MEngine* e); };                       // SLBVH.cpp:72-82
+// CSufraceInteraction.h:200-225 (the differentials come with the interactions
+// from the device; this step moves them into surfaceInteractions)
+struct JobComputeDifferentialsForSurfInter { void run(MEngine* e); };
 
 #define WPBR_JOB_RUN(JobT) \
     {                      \
@@ -106,6 +132,7 @@ public:
     ~MEngine() {
         release_shapes();
         release_rays();
+        release_attributes();
         if (bvh_) wpbr_lbvh_destroy(bvh_);
         if (ctx_) wpbr_close(ctx_);
     }
@@ -122,6 +149,20 @@ public:
         geom_.d_sphere_xform = upload(sphereXform24, &d_xform_);
         geom_.d_sphere_radius = upload(sphereRadius, &d_radius_);
         geom_.d_tri_verts = upload(triVerts9, &d_tris_);
+    }
+
+    // Per-triangle mesh attributes (triangle order, as setShapes' triangles):
+    // flags bit0 uvs given, bit1 normals and tangents given; uv 6 floats,
+    // normal and tangent 9 floats per triangle.  Empty vectors: none (default
+    // uvs, geometric shading frame).
+    void setTriangleAttributes(const std::vector<uint8_t>& flags, const std::vector<float>& uv6,
+                               const std::vector<float>& normal9, const std::vector<float>& tangent9) {
+        release_attributes();
+        if (flags.empty()) return;
+        attrs_.d_tri_flags = (const uint8_t*)upload_bytes(flags.data(), flags.size(), &d_flags_);
+        attrs_.d_tri_uv = (const float*)upload_bytes(uv6.data(), uv6.size() * 4, &d_uv_);
+        attrs_.d_tri_normal = (const float*)upload_bytes(normal9.data(), normal9.size() * 4, &d_nrm_);
+        attrs_.d_tri_tangent = (const float*)upload_bytes(tangent9.data(), tangent9.size() * 4, &d_tan_);
     }
 
     // STriangleMesh::create (CSTriangleMesh.cpp:11-39) on the device: the mesh
@@ -177,7 +218,9 @@ public:
         WPBR_JOB_RUN(JobBuildUpperSAH)
     }
 
-    // MEngine.cpp:352-361: rayCasts in, interactions out (one per ray cast).
+    // MEngine.cpp:352-361: rayCasts (and optionally rayDifferentials, one per
+    // ray cast) in; interactions out (one per ray cast), and for every ray cast
+    // with bSurfInteraction that hits, its surfaceInteractions entry.
     void runCollisionSystem() {
         WPBR_JOB_RUN(JobProcessRayCasts)
         WPBR_JOB_RUN(JobProcessSphereSurfInteractionRequests)
@@ -208,8 +251,10 @@ public:
         return t;
     }
 
-    std::vector<CRayCast> rayCasts;          // input of runCollisionSystem
-    std::vector<CInteraction> interactions;  // its output, one per ray cast
+    std::vector<CRayCast> rayCasts;                   // input of runCollisionSystem
+    std::vector<CRayDifferential> rayDifferentials;   // optional input: empty, or one per ray cast
+    std::vector<CInteraction> interactions;           // output, one per ray cast
+    std::vector<CSurfaceInteraction> surfaceInteractions;  // output, one per ray cast (bSurfInteraction hits)
 
     wpbr_ctx* ctx() const { return ctx_; }
     wpbr_lbvh* lbvh() const { return bvh_; }
@@ -219,34 +264,93 @@ public:
     void build_stage(int stage) { check(wpbr_lbvh_build_stage(ctx_, &geom_, bvh_, stage), "wpbr_lbvh_build_stage"); }
     void trace_rays() {
         const uint32_t n = (uint32_t)rayCasts.size();
-        std::vector<float> soa(6 * (size_t)n);
+        const bool diff = !rayDifferentials.empty();
+        if (diff && rayDifferentials.size() != n)
+            throw Error(WPBR_ERR_INVALID_ARG, "rayDifferentials: none or one per ray cast");
+        std::vector<float> soa(6 * (size_t)n), dsoa(diff ? 12 * (size_t)n : 0);
+        std::vector<uint8_t> surf(n);
         for (uint32_t i = 0; i < n; i++) {
             const DRayf& r = rayCasts[i].ray;
-            soa[i] = r.origin.x;
-            soa[n + i] = r.origin.y;
-            soa[2 * (size_t)n + i] = r.origin.z;
-            soa[3 * (size_t)n + i] = r.dir.x;
-            soa[4 * (size_t)n + i] = r.dir.y;
-            soa[5 * (size_t)n + i] = r.dir.z;
+            const float v[6] = {r.origin.x, r.origin.y, r.origin.z, r.dir.x, r.dir.y, r.dir.z};
+            for (int k = 0; k < 6; k++) soa[(size_t)k * n + i] = v[k];
+            surf[i] = rayCasts[i].bSurfInteraction ? 1 : 0;
+            if (diff) {
+                const DRayf& x = rayDifferentials[i].difXRay;
+                const DRayf& y = rayDifferentials[i].difYRay;
+                const float w[12] = {x.origin.x, x.origin.y, x.origin.z, x.dir.x, x.dir.y, x.dir.z,
+                                     y.origin.x, y.origin.y, y.origin.z, y.dir.x, y.dir.y, y.dir.z};
+                for (int k = 0; k < 12; k++) dsoa[(size_t)k * n + i] = w[k];
+            }
         }
         ensure_rays(n);
-        if (n) check(wpbr_upload(ctx_, d_rays_, soa.data(), soa.size() * 4), "upload rays");
-        check(wpbr_collide_async(ctx_, &geom_, bvh_, n, (const float*)d_rays_, (uint32_t*)d_hit_,
-                                 (float*)d_t_, nullptr),
-              "wpbr_collide_async");
+        if (n) {
+            check(wpbr_upload(ctx_, d_rays_, soa.data(), soa.size() * 4), "upload rays");
+            check(wpbr_upload(ctx_, d_surf_, surf.data(), n), "upload bSurfInteraction");
+            if (diff) check(wpbr_upload(ctx_, d_diff_, dsoa.data(), dsoa.size() * 4), "upload ray differentials");
+        }
+        check(wpbr_collide_interactions(ctx_, &geom_, attrs_.d_tri_flags ? &attrs_ : nullptr, bvh_, n,
+                                        (const float*)d_rays_, (const uint8_t*)d_surf_,
+                                        diff ? (const float*)d_diff_ : nullptr, (uint32_t*)d_hit_, (float*)d_t_,
+                                        (float*)d_si_),
+              "wpbr_collide_interactions");
         n_traced_ = n;
     }
     void fetch_hits() {
         hit_.resize(n_traced_);
         t_.resize(n_traced_);
-        check(wpbr_sync(ctx_), "wpbr_sync");
+        si_.resize((size_t)WPBR_SI_FLOATS * n_traced_);
         if (n_traced_) {
             check(wpbr_download(ctx_, hit_.data(), d_hit_, 4 * (size_t)n_traced_), "download hits");
             check(wpbr_download(ctx_, t_.data(), d_t_, 4 * (size_t)n_traced_), "download t");
+            check(wpbr_download(ctx_, si_.data(), d_si_, si_.size() * 4), "download interactions");
+        }
+        surfaceInteractions.assign(n_traced_, CSurfaceInteraction{});
+    }
+    // The surface interactions of the ray casts with bSurfInteraction that hit
+    // a sphere (spheres = true) or a triangle.
+    void fill_surface_interactions(bool spheres) {
+        const uint32_t n = n_traced_;
+        auto f = [&](int field, uint32_t i) { return si_[(size_t)field * n + i]; };
+        auto v3 = [&](int field, uint32_t i) { return DVector3f{f(field, i), f(field + 1, i), f(field + 2, i)}; };
+        for (uint32_t i = 0; i < n; i++) {
+            if (hit_[i] == INVALID_HANDLE || !rayCasts[i].bSurfInteraction) continue;
+            if ((hit_[i] < geom_.n_spheres) != spheres) continue;
+            CSurfaceInteraction& s = surfaceInteractions[i];
+            s.p = v3(WPBR_SI_P, i);
+            s.wo = v3(WPBR_SI_WO, i);
+            s.hCollision = hit_[i];
+            s.time = f(WPBR_SI_TIME, i);
+            s.n = v3(WPBR_SI_N, i);
+            s.uv = {f(WPBR_SI_UV, i), f(WPBR_SI_UV + 1, i)};
+            s.dpdu = v3(WPBR_SI_DPDU, i);
+            s.dpdv = v3(WPBR_SI_DPDV, i);
+            s.dndu = v3(WPBR_SI_DNDU, i);
+            s.dndv = v3(WPBR_SI_DNDV, i);
+            s.shading.n = v3(WPBR_SI_SHADING_N, i);
+            s.shading.dpdu = v3(WPBR_SI_SHADING_DPDU, i);
+            s.shading.dpdv = v3(WPBR_SI_SHADING_DPDV, i);
+            s.shading.dndu = v3(WPBR_SI_SHADING_DNDU, i);
+            s.shading.dndv = v3(WPBR_SI_SHADING_DNDV, i);
         }
     }
-    // SLBVH.cpp:50-62: a miss keeps INVALID_HANDLE; a hit records p = o + d t,
-    // wo = -d, the shape and t.
+    void fill_differentials() {
+        const uint32_t n = n_traced_;
+        if (rayDifferentials.empty()) return;
+        auto f = [&](int field, uint32_t i) { return si_[(size_t)field * n + i]; };
+        for (uint32_t i = 0; i < n; i++) {
+            if (hit_[i] == INVALID_HANDLE || !rayCasts[i].bSurfInteraction) continue;
+            CSurfaceInteraction& s = surfaceInteractions[i];
+            s.dpdx = {f(WPBR_SI_DPDX, i), f(WPBR_SI_DPDX + 1, i), f(WPBR_SI_DPDX + 2, i)};
+            s.dpdy = {f(WPBR_SI_DPDY, i), f(WPBR_SI_DPDY + 1, i), f(WPBR_SI_DPDY + 2, i)};
+            s.dudx = f(WPBR_SI_DUDX, i);
+            s.dvdx = f(WPBR_SI_DVDX, i);
+            s.dudy = f(WPBR_SI_DUDY, i);
+            s.dvdy = f(WPBR_SI_DVDY, i);
+        }
+    }
+    // SLBVH.cpp:40-62: a miss keeps INVALID_HANDLE; a hit records p = o + d t,
+    // wo = -d, the shape and t (the device wrote them for rays without
+    // bSurfInteraction; the surface ones take the same fields).
     void finish_interactions() {
         interactions.resize(n_traced_);
         for (uint32_t i = 0; i < n_traced_; i++) {
@@ -270,6 +374,17 @@ private:
             throw Error(s, std::string(what) + ": " + wpbr_status_string(s) + " (" +
                                (ctx_ ? wpbr_last_error(ctx_) : "") + ")");
     }
+    const void* upload_bytes(const void* src, size_t bytes, void** d) {
+        if (!bytes) return nullptr;
+        check(wpbr_malloc(ctx_, bytes, d), "wpbr_malloc");
+        check(wpbr_upload(ctx_, *d, src, bytes), "wpbr_upload");
+        return *d;
+    }
+    void release_attributes() {
+        for (void** p : {&d_flags_, &d_uv_, &d_nrm_, &d_tan_})
+            if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
+        attrs_ = wpbr_tri_attributes{};
+    }
     const float* upload(const std::vector<float>& v, void** d) {
         if (v.empty()) return nullptr;
         check(wpbr_malloc(ctx_, v.size() * 4, d), "wpbr_malloc");
@@ -284,12 +399,15 @@ private:
         if (n <= ray_cap_) return;
         release_rays();
         check(wpbr_malloc(ctx_, 24 * (size_t)n, &d_rays_), "wpbr_malloc");
+        check(wpbr_malloc(ctx_, (size_t)n, &d_surf_), "wpbr_malloc");
+        check(wpbr_malloc(ctx_, 48 * (size_t)n, &d_diff_), "wpbr_malloc");
         check(wpbr_malloc(ctx_, 4 * (size_t)n, &d_hit_), "wpbr_malloc");
         check(wpbr_malloc(ctx_, 4 * (size_t)n, &d_t_), "wpbr_malloc");
+        check(wpbr_malloc(ctx_, 4 * (size_t)WPBR_SI_FLOATS * n, &d_si_), "wpbr_malloc");
         ray_cap_ = n;
     }
     void release_rays() {
-        for (void** p : {&d_rays_, &d_hit_, &d_t_})
+        for (void** p : {&d_rays_, &d_surf_, &d_diff_, &d_hit_, &d_t_, &d_si_})
             if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
         ray_cap_ = 0;
     }
@@ -298,10 +416,13 @@ private:
     wpbr_lbvh* bvh_ = nullptr;
     wpbr_geometry geom_{};
     void *d_xform_ = nullptr, *d_radius_ = nullptr, *d_tris_ = nullptr;
-    void *d_rays_ = nullptr, *d_hit_ = nullptr, *d_t_ = nullptr;
+    void *d_rays_ = nullptr, *d_surf_ = nullptr, *d_diff_ = nullptr, *d_hit_ = nullptr, *d_t_ = nullptr,
+         *d_si_ = nullptr;
+    void *d_flags_ = nullptr, *d_uv_ = nullptr, *d_nrm_ = nullptr, *d_tan_ = nullptr;
+    wpbr_tri_attributes attrs_{};
     uint32_t ray_cap_ = 0, n_traced_ = 0;
     std::vector<uint32_t> hit_;
-    std::vector<float> t_;
+    std::vector<float> t_, si_;
 };
 
 // Spheres and triangles are bounded by one kernel (stage 0); the sphere job
@@ -315,7 +436,10 @@ inline void JobEmitLBVH::run(MEngine* e) { e->build_stage(4); }
 inline void JobBuildUpperSAH::run(MEngine* e) { e->build_stage(5); }
 inline void JobProcessRayCasts::run(MEngine* e) { e->trace_rays(); }
 inline void JobProcessRayCastsResults::run(MEngine* e) { e->fetch_hits(); }
+inline void JobProcessSphereFullInteractionRequests::run(MEngine* e) { e->fill_surface_interactions(true); }
+inline void JobProcessTriangleFullInteractionRequests::run(MEngine* e) { e->fill_surface_interactions(false); }
 inline void JobCollisionFinish::run(MEngine* e) { e->finish_interactions(); }
+inline void JobComputeDifferentialsForSurfInter::run(MEngine* e) { e->fill_differentials(); }
 
 #undef WPBR_JOB_RUN
 

@@ -32,8 +32,14 @@ EXPORTS = [
     "wpbr_texture_level", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
     "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
-    "wpbr_set_render_overlap", "wpbr_film_loss_noalloc",
+    "wpbr_set_render_overlap", "wpbr_film_loss_noalloc", "wpbr_scene_collide", "wpbr_collide_interactions",
 ]
+
+# CSurfaceInteraction as SoA floats (wpbr.h WPBR_SI_*): field offsets, then the count.
+SI_FIELDS = {"p": 0, "wo": 3, "time": 6, "n": 7, "uv": 10, "dpdu": 12, "dpdv": 15, "dndu": 18, "dndv": 21,
+             "shading_n": 24, "shading_dpdu": 27, "shading_dpdv": 30, "shading_dndu": 33, "shading_dndv": 36,
+             "dpdx": 39, "dpdy": 42, "dudx": 45, "dvdx": 46, "dudy": 47, "dvdy": 48}
+SI_FLOATS = 49
 
 # LBVH key orders (wpbr.h WPBR_LBVH_ORDER_*): the reference's (bits 0..23 of the
 # Morton codes, what its sort job leaves for the later jobs) or the full sort.
@@ -111,6 +117,8 @@ def load():
         "wpbr_lbvh_get_order": (C.c_int, [vp]),
         "wpbr_set_default_lbvh_order": (i32, [vp, C.c_int]),
         "wpbr_set_render_overlap": (i32, [vp, C.c_int]),
+        "wpbr_scene_collide": (i32, [vp, vp, u32, vp, vp, vp, vp, vp, vp]),
+        "wpbr_collide_interactions": (i32, [vp, C.POINTER(Geometry), vp, vp, u32, vp, vp, vp, vp, vp, vp]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
@@ -544,6 +552,17 @@ class Renderer:
             self.ctx.h, self.h, self.d_film.ptr,
             self.d_samples.ptr if keep_samples else None))
         return self.ctx.last_ms()
+
+    def collide(self, d_rays, n, d_surf=None, d_diff=None, with_si=True):
+        """wpbr_scene_collide: closest hits (shape, t) and, with with_si, the
+        interactions (SI_FLOATS x n device floats) of n ray casts."""
+        d_s = self.ctx.empty(n, np.uint32)
+        d_t = self.ctx.empty(n, np.float32)
+        d_si = self.ctx.empty((SI_FLOATS, n), np.float32) if with_si else None
+        self.ctx.check(self.ctx.lib.wpbr_scene_collide(
+            self.ctx.h, self.h, n, d_rays.ptr, d_surf.ptr if d_surf is not None else None,
+            d_diff.ptr if d_diff is not None else None, d_s.ptr, d_t.ptr, d_si.ptr if d_si is not None else None))
+        return d_s, d_t, d_si
 
     def stats(self):
         buf = (C.c_double * 8)()
