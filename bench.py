@@ -227,6 +227,78 @@ def bench_batch(lib, scenes, ctx, steps, tune=True):
                       "trace_config": tuned}}
 
 
+# BSDF jobs: per request read frame 36 B, wo 12 B, material 4 B, the queue
+# index 4 B and wi 12 B (Compute) or u 8 B (Sample); written: 64 spectral values
+# 256 B, pdf 4 B and for Sample wi 12 B
+BSDF_BYTES = {"compute": 68 + 260, "sample": 64 + 272}
+
+
+def bench_bsdf(lib, scenes, ctx, scene, r, steps):
+    """The BSDF Compute / Sample jobs over the surface interactions of the
+    render's primary rays (one per pixel through the camera's eye): per-kind
+    queues and one launch per kind against one fused launch over all requests."""
+    W, H = scene.width, scene.height
+    rays = np.ascontiguousarray(scenes.pinhole_rays(W, H, (0.0, 1.2, -5.2)), np.float32)
+    n = rays.shape[1]
+    d_rays, d_surf = ctx.array(rays), ctx.array(np.ones(n, np.uint8))
+    d_s, d_t, d_si = r.collide(d_rays, n, d_surf)
+    hit, si = d_s.numpy(), d_si.numpy()
+    for a in (d_rays, d_surf, d_s, d_t, d_si):
+        a.free()
+    keep = hit != 0xFFFFFFFF
+    hit, si = hit[keep], si[:, keep].astype(np.float64)
+    f = lib.SI_FIELDS
+    ns = si[f["shading_n"]:f["shading_n"] + 3]
+    ss = si[f["shading_dpdu"]:f["shading_dpdu"] + 3]
+    ss = ss / np.maximum(np.linalg.norm(ss, axis=0), 1e-30)
+    frame = np.concatenate([ss, np.cross(ns.T, ss.T).T, ns]).astype(np.float32)
+    wo = si[f["wo"]:f["wo"] + 3].astype(np.float32)
+    mat = scene.shape_material[hit].astype(np.uint32)
+    ok = mat < len(scene.material_table)
+    frame, wo, mat = frame[:, ok], wo[:, ok], mat[ok]
+    m = len(mat)
+    g = np.random.default_rng(7)
+    wi = g.normal(size=(3, m))
+    wi = (wi / np.linalg.norm(wi, axis=0, keepdims=True)).astype(np.float32)
+    req, arrs = r.bsdf_requests(frame, wo, mat, wi=wi, u=g.random((2, m)).astype(np.float32))
+    d_q, counts = r.bsdf_queues(req, m)
+    q_us = []
+    for _ in range(max(steps, 5)):
+        r.bsdf_queues(req, m)
+        q_us.append(ctx.last_ms() * 1e3)
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    subs = []
+    for k in range(3):
+        sub = ctx.empty(max(counts[k], 1), np.uint32)
+        if counts[k]:
+            sub.upload(d_q.numpy()[offs[k]:offs[k + 1]])
+        subs.append(sub)
+    res = {"requests": m, "kinds": {"conductor": counts[0], "dielectric": counts[1], "specular": counts[2]},
+           "queues_us": float(np.median(q_us))}
+    for sample, tag in ((False, "compute"), (True, "sample")):
+        out = r.bsdf(sample, lib.BSDF_ANY, req, m)
+        split, fused = [], []
+        for _ in range(max(steps, 5)):
+            t = 0.0
+            for k in range(3):
+                r.bsdf(sample, k, req, m, subs[k], counts[k], out=out)
+                t += ctx.last_ms()
+            split.append(t * 1e3)
+            r.bsdf(sample, lib.BSDF_ANY, req, m, out=out)
+            fused.append(ctx.last_ms() * 1e3)
+        sp, fu = float(np.median(split)), float(np.median(fused))
+        gbs = BSDF_BYTES[tag] * m / (sp * 1e-6) / 1e9
+        res[tag] = {"queue_split_us": sp, "fused_us": fu, "queue_split_gbs": gbs,
+                    "queue_split_hbm_frac": gbs / HBM_PEAK_GBS,
+                    "fused_gbs": BSDF_BYTES[tag] * m / (fu * 1e-6) / 1e9, "bytes_per_request": BSDF_BYTES[tag]}
+        for a in out:
+            if a is not None:
+                a.free()
+    for a in subs + [d_q] + list(arrs.values()):
+        a.free()
+    return {"bsdf_jobs": res}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +310,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-stratified", action="store_true", help="skip the stratified-sampler render")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-bsdf", action="store_true", help="skip the BSDF job timings")
+    ap.add_argument("--bsdf-only", action="store_true",
+                    help="only the BSDF jobs over the full render's primary interactions, for kernel profiles")
     ap.add_argument("--trace-config", default=None,
                     help="fixed traversal configuration block,stack_depth,mapping,width,refill,chunk "
                          "instead of the autotuner (for profiling one configuration)")
@@ -287,6 +362,16 @@ def main():
             for b in bv + [bvh]:
                 b.destroy()
         print(json.dumps(out), flush=True)
+        ctx.close()
+        return
+
+    if args.bsdf_only:
+        scene = scenes.c2_scene()
+        ctx = lib.Context(0)
+        r = lib.Renderer(ctx, scene)
+        res = bench_bsdf(lib, scenes, ctx, scene, r, args.steps)
+        r.destroy()
+        print(json.dumps(res), flush=True)
         ctx.close()
         return
 
@@ -432,6 +517,9 @@ def main():
                                      "trace_launches": s_launch, "rays_per_step": s_rays}
     log("render with serial passes: %.2f ms device, trace %.2f ms in %d launches" % (np.mean(ser_ms), s_trace,
                                                                                      s_launch))
+    if not args.no_bsdf:
+        extra.update(bench_bsdf(lib, scenes, ctx, scene, r, args.steps))
+        log("bsdf jobs:", json.dumps(extra["bsdf_jobs"]))
     r.destroy()
     for name, mode in (("render_independent", lib.SAMPLER_INDEPENDENT),
                        ("render_stratified_n", lib.SAMPLER_STRATIFIED_N)):

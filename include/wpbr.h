@@ -80,7 +80,7 @@ typedef struct {
 wpbr_status wpbr_shape_bounds(wpbr_ctx* ctx, const wpbr_geometry* geom, float* d_bounds,
                               float* d_centroid);
 
-/* ---- LBVH (a2-a7) -------------------------------------------------------- */
+/* ---- LBVH --------------------------------------------------------------- */
 /* Key order of a build.  REFERENCE (the default) is what the reference's
  * JobSortsShapesByMortonCode leaves in shapeMortonCodes for the rest of its
  * jobs: its 5 x 6-bit passes alternate between two buffers and the last pass's
@@ -136,7 +136,7 @@ wpbr_status wpbr_lbvh_download(wpbr_ctx* ctx, wpbr_lbvh* bvh, int field, void* d
  * [10] traversal layout; returns the number written. */
 int wpbr_lbvh_kernel_times(wpbr_lbvh* bvh, float* ms, int cap);
 
-/* ---- collision (a8-a13) -------------------------------------------------- *
+/* ---- collision ---------------------------------------------------------- *
  * Rays are SoA: d_rays = 6 x n floats (ox[n], oy[n], oz[n], dx[n], dy[n],
  * dz[n]); tmax = +inf as in the reference.  Output per ray: the hit shape id
  * (0xFFFFFFFF = INVALID_HANDLE, a miss) and t (+inf on a miss); optional
@@ -178,7 +178,7 @@ wpbr_status wpbr_autotune_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, cons
                                   uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
                                   float* d_hit_t, wpbr_trace_config* best);
 
-/* ---- rendering (a14-a31) ------------------------------------------------- *
+/* ---- rendering ---------------------------------------------------------- *
  * Scene description: host pointers, copied to the device by wpbr_scene_create.
  * Shape order as in wpbr_geometry.  Materials are 9 floats each: type (u32
  * bits: 0 conductor microfacet, 1 dielectric microfacet, 2 specular reflection
@@ -308,6 +308,44 @@ wpbr_status wpbr_collide_interactions(wpbr_ctx* ctx, const wpbr_geometry* geom, 
                                       const wpbr_lbvh* bvh, uint32_t n, const float* d_rays,
                                       const uint8_t* d_surf, const float* d_ray_diff, uint32_t* d_hit_shape,
                                       float* d_hit_t, float* d_si);
+/* ---- BSDF Compute / Sample jobs --------------------------------------------- *
+ * The jobs the render runs on every hit's BSDF entity, over request queues:
+ * JobGenerateBSDFRequests splits the hits into per-material queues (CMaterial.h:30-38)
+ * and each BSDF family's Compute / Sample job runs over its own:
+ *   WPBR_BSDF_CONDUCTOR_MF   JobBSDFConductorMicrofaceCompute / Sample (CBSDFConductorMicroface.cpp:84-115)
+ *   WPBR_BSDF_DIELECTRIC_MF  JobBSDFDielectricMicrofaceCompute / Sample (CBSDFDielectricMicroface.cpp:84-115)
+ *   WPBR_BSDF_SPECULAR_REFL  JobBSDFSpecularReflectionCompute / Sample (CBSDFSpecularReflectance.cpp:63-93)
+ * A request (SoA device floats, n of them) is a BSDF entity's shading frame
+ * (CBSDFTransform rows ss, ts, ns: d_frame 9 x n), wo (3 x n, world), wi
+ * (Compute, 3 x n, world), the 2D sample (Sample, 2 x n) and the material index
+ * into the scene's table (its type is the BSDF kind; 0xFFFFFFFF: none).
+ * Results at each request's index: d_value n x 64 (16-byte aligned; a request's
+ * spectrum contiguous, as the dense CSampledBSDFValue components, CBSDF.h:24-33),
+ * f(wo, wi) |cos wi| per wavelength; d_pdf n, CSampledBSDFPDF; d_wi 3 x n
+ * (Sample, world), CSampledWI.  A microfacet sample below the surface (the reference leaves its pdf
+ * and wi unset) and a request without material give value 0, pdf 0, wi 0. */
+enum { WPBR_BSDF_CONDUCTOR_MF = 0, WPBR_BSDF_DIELECTRIC_MF = 1, WPBR_BSDF_SPECULAR_REFL = 2, WPBR_BSDF_ANY = -1 };
+typedef struct {
+    const float* d_frame;
+    const float* d_wo;
+    const float* d_wi;
+    const float* d_u;
+    const uint32_t* d_material;
+} wpbr_bsdf_requests;
+/* The per-kind queues: d_queue (n entries) receives the indices of the
+ * conductor requests, then the dielectric ones, then the specular ones, each in
+ * request order; counts[k] their numbers (requests without material are left
+ * out).  Compacted on the device with wave-ballot prefix sums. */
+wpbr_status wpbr_bsdf_queues(wpbr_ctx* ctx, wpbr_scene* scene, const wpbr_bsdf_requests* req, uint32_t n,
+                             uint32_t* d_queue, uint32_t counts[3]);
+/* One family's Compute / Sample over the q requests listed in d_queue (all of
+ * that kind), or with kind WPBR_BSDF_ANY every one of the n requests in one
+ * launch, each by its own kind (d_queue and q ignored). */
+wpbr_status wpbr_bsdf_compute(wpbr_ctx* ctx, wpbr_scene* scene, int kind, const wpbr_bsdf_requests* req,
+                              uint32_t n, const uint32_t* d_queue, uint32_t q, float* d_value, float* d_pdf);
+wpbr_status wpbr_bsdf_sample(wpbr_ctx* ctx, wpbr_scene* scene, int kind, const wpbr_bsdf_requests* req,
+                             uint32_t n, const uint32_t* d_queue, uint32_t q, float* d_wi, float* d_value,
+                             float* d_pdf);
 /* Statistics of the last wpbr_render: [0] total ms, [1] trace ms (all trace
  * launches), [2] number of trace launches, [3] rays traced, [4] shade ms,
  * [5] camera ms, [6] secondary rays, [7] trace ms of the primary rays.

@@ -265,6 +265,15 @@ static SceneHold* make_scene(const orc_scene_desc* d) {
 
 extern "C" {
 
+// The BSDF Compute / Sample jobs over n requests of the scene's materials (render.cpp).
+void orc_bsdf_eval(const orc_scene_desc* d, int sample, uint32_t n, const float* frame, const float* wo,
+                   const float* wi, const float* u, const uint32_t* material, float* value, float* pdf,
+                   float* wi_out) {
+    SceneHold* H = make_scene(d);
+    bsdf_eval(H->sc, sample, n, frame, wo, wi, u, material, value, pdf, wi_out);
+    delete H;
+}
+
 // The scene's closest hits and interactions for n ray casts (interaction.cpp).
 void orc_scene_collide(const orc_scene_desc* d, int nthreads, uint32_t n, const float* rays, const uint8_t* surf,
                        const float* diff, uint32_t* hit_shape, float* hit_t, float* si) {

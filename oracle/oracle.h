@@ -182,6 +182,11 @@ void scene_collide(const RenderScene& sc, const LbvhResult& bvh, uint32_t n, con
                    const uint8_t* surf, const float* diff, uint32_t* hit_shape, float* hit_t, float* si,
                    int nthreads);
 
+// The BSDF Compute (sample = 0) / Sample (sample = 1) jobs for n requests (render.cpp).
+void bsdf_eval(const RenderScene& sc, int sample, uint32_t n, const float* frame, const float* wo,
+               const float* wi, const float* u, const uint32_t* material, float* value, float* pdf,
+               float* wi_out);
+
 // Every discrete decision of one sample (the finite-difference reference of
 // diff.cpp re-evaluates the estimator with them frozen).
 struct EnvCell {

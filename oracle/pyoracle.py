@@ -241,6 +241,30 @@ def scene_collide(s, rays, surf=None, diff=None, nthreads=None):
     return shape, t, si
 
 
+def bsdf_eval(s, sample, frame, wo, material, wi=None, u=None):
+    """The BSDF Compute (sample False) or Sample (True) jobs for requests of the
+    scene's materials (SoA: frame 9 x n, wo 3 x n, wi 3 x n, u 2 x n): value
+    n x 64 (f |cos wi|, a request's spectrum contiguous), pdf n, and for Sample
+    wi 3 x n (world)."""
+    lib = load()
+    vp = C.c_void_p
+    lib.orc_bsdf_eval.argtypes = [vp, C.c_int, C.c_uint32, vp, vp, vp, vp, vp, vp, vp, vp]
+    d, keep = scene_desc(s)
+    frame = _f32(frame, (9, -1))
+    n = frame.shape[1]
+    wo = _f32(wo, (3, n))
+    wi = None if wi is None else _f32(wi, (3, n))
+    u = None if u is None else _f32(u, (2, n))
+    mat = np.ascontiguousarray(material, np.uint32)
+    value = np.empty((n, 64), np.float32)
+    pdf = np.empty(n, np.float32)
+    wi_out = np.empty((3, n), np.float32)
+    lib.orc_bsdf_eval(C.byref(d), 1 if sample else 0, n, _p(frame), _p(wo), _p(wi), _p(u), _p(mat), _p(value),
+                      _p(pdf), _p(wi_out))
+    del keep
+    return (value, pdf, wi_out) if sample else (value, pdf)
+
+
 def render(s, nthreads=None, tiles=None):
     """Render a scene on the CPU; returns a dict of numpy arrays."""
     lib = load()

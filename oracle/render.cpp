@@ -735,6 +735,89 @@ static float mitchell1(float x) {
 
 float mitchell_weight(float x) { return mitchell1(x); }
 
+// The BSDF Compute / Sample jobs for n requests (SoA, world space): the
+// microfacet conductor and dielectric families (CBSDFConductorMicroface.cpp:11-115,
+// CBSDFDielectricMicroface.cpp:11-115) and the specular reflection
+// (CBSDFSpecularReflectance.cpp:26-93), chosen by each request's material, in the
+// shading frame of the request (CBSDF.cpp:12-24: rows ss, ts, ns).  value is
+// f * |cos wi| per wavelength (n x 64: a request's spectrum contiguous, as the
+// dense CSampledBSDFValue components of CBSDF.h:24-33), pdf per request, wi
+// (Sample: 3 x n, world).
+// A rejected microfacet sample (wi below the surface; the reference leaves pdf and
+// wi unset) gives value 0, pdf 0, wi 0; a request without material gives zeros.
+void bsdf_eval(const RenderScene& sc, int sample, uint32_t n, const float* frame, const float* wo,
+               const float* wi, const float* u, const uint32_t* material, float* value, float* pdf,
+               float* wi_out) {
+    std::vector<MatSpec> mats(sc.n_materials);
+    for (uint32_t m = 0; m < sc.n_materials; m++) {
+        mats[m].eta = from_rgb(sc.materials[m].eta_rgb, false);
+        mats[m].k = from_rgb(sc.materials[m].k_rgb, false);
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        auto g = [&](const float* a, int k) { return a[(size_t)k * n + i]; };
+        for (int k = 0; k < NSPEC; k++) value[(size_t)i * NSPEC + k] = 0.0f;
+        pdf[i] = 0.0f;
+        if (sample)
+            for (int k = 0; k < 3; k++) wi_out[(size_t)k * n + i] = 0.0f;
+        const uint32_t mid = material[i];
+        if (mid >= sc.n_materials) continue;
+        const Material& m = sc.materials[mid];
+        const V3 ss = v3(g(frame, 0), g(frame, 1), g(frame, 2));
+        const V3 ts = v3(g(frame, 3), g(frame, 4), g(frame, 5));
+        const V3 ns = v3(g(frame, 6), g(frame, 7), g(frame, 8));
+        auto toLocal = [&](V3 w) { return v3(dot3(ss, w), dot3(ts, w), dot3(ns, w)); };
+        const V3 woL = toLocal(v3(g(wo, 0), g(wo, 1), g(wo, 2)));
+        Spec val;
+        float p = 0.0f;
+        V3 wiL = v3(0, 0, 0);
+        if (!sample) {
+            if (m.type == MAT_SPECULAR) continue;  // Compute: f = 0, pdf = 0 (:42-49)
+            wiL = toLocal(v3(g(wi, 0), g(wi, 1), g(wi, 2)));
+            bsdf_microfacet_f(m, mats[mid], woL, wiL, val, &p);
+        } else if (m.type == MAT_SPECULAR) {
+            wiL = v3(-woL.x, -woL.y, woL.z);
+            p = 1.0f;
+            const float c = abscos(wiL);
+            if (c == 0) val.fill(0.0f * c);
+            else {
+                const float fr = fresnel_dielectric(wiL.z, 1.0f, m.eta);
+                for (int k = 0; k < NSPEC; k++) val[k] = (1.0f * (fr / c)) * c;
+            }
+        } else {
+            const float uu[2] = {g(u, 0), g(u, 1)};
+            float whD;
+            const V3 wh = ggx_sample(woL, uu, m.alpha, m.alpha, &whD);
+            wiL = (-woL) + wh * (2 * dot3(woL, wh));
+            if (!same_hemi(wiL, woL)) continue;
+            // sample_f evaluates f with the sampled half vector and its D
+            // (CBSDFConductorMicroface.cpp:60-82, f_impl :20-41)
+            p = (whD * abscos(wh)) / (4 * dot3(woL, wh));
+            const float cO = abscos(woL), cI = abscos(wiL), ac = abscos(wiL);
+            if (cI == 0 || cO == 0 || (wh.x == 0 && wh.y == 0 && wh.z == 0)) {
+                val.fill(0.0f * ac);
+            } else if (m.type == MAT_CONDUCTOR) {
+                const float sc_ = whD / (4 * cI * cO);
+                const float c = dot3(wiL, wh);
+                for (int k = 0; k < NSPEC; k++)
+                    val[k] = ((fresnel_conductor1(c, mats[mid].eta[k], mats[mid].k[k]) * 1.0f) * sc_) * ac;
+            } else {
+                const float fr = fresnel_dielectric(dot3(wiL, wh), 1.0f, m.eta);
+                const float sc_ = fr * whD / (4 * cI * cO);
+                for (int k = 0; k < NSPEC; k++) val[k] = (1.0f * sc_) * ac;
+            }
+        }
+        for (int k = 0; k < NSPEC; k++) value[(size_t)i * NSPEC + k] = val[k];
+        pdf[i] = p;
+        if (sample) {
+            const V3 w = v3((ss.x * wiL.x + ts.x * wiL.y) + ns.x * wiL.z, (ss.y * wiL.x + ts.y * wiL.y) + ns.y * wiL.z,
+                            (ss.z * wiL.x + ts.z * wiL.y) + ns.z * wiL.z);
+            wi_out[i] = w.x;
+            wi_out[n + i] = w.y;
+            wi_out[2 * (size_t)n + i] = w.z;
+        }
+    }
+}
+
 void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
             const uint32_t* tile_subset, uint32_t n_tiles_subset, SampleRec* recs) {
     Ctx C{sc, bvh, {}, {}, {}, {}};

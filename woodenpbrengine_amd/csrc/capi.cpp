@@ -241,6 +241,7 @@ void wpbr_close(wpbr_ctx* ctx) {
     if (ctx->spill) hipFree(ctx->spill);
     if (ctx->err_flag) hipFree(ctx->err_flag);
     if (ctx->batch_mem) hipFree(ctx->batch_mem);
+    if (ctx->bsdf_blk) hipFree(ctx->bsdf_blk);
     ctx_release_render(ctx);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);

@@ -1164,6 +1164,242 @@ wpbr_status scene_si_launch(const SceneView& sv, uint32_t n, const float* rays, 
     return launch_status();
 }
 
+// ---- BSDF Compute / Sample jobs over request queues (wpbr_bsdf_*) ----------------
+// One request: the shading frame (rows ss, ts, ns of CBSDFTransform, CBSDF.cpp:12-24),
+// wo and wi (world), the 2D sample and the material.  The kind (conductor
+// microfacet, dielectric microfacet, specular reflection) is the material's.
+// value = f |cos wi| per wavelength, as the BSDF jobs store CSampledBSDFValue.
+constexpr int BSDF_KIND_NONE = 3;  // no material (or an index beyond the table)
+
+WPBR_D int bsdf_kind_of(const SceneView& sv, const BsdfArgs& a, uint32_t i) {
+    const uint32_t mid = a.material[i];
+    return mid >= a.n_materials ? BSDF_KIND_NONE : (int)sv.materials[mid].type;
+}
+
+// KIND: the request's kind when known at compile time (a queue launch), -1 to
+// branch on it per lane (the fused launch).  BSDF_LANES lanes share a request
+// (the per-request terms are recomputed by each): lane sub stores the float4s
+// sub, sub + 4, sub + 8, sub + 12 of the request's 64-float spectrum (n x 64,
+// request-major), so each store instruction covers 64 contiguous bytes of 16
+// requests; lane 0 of the group writes pdf and wi.  Every output is stored once.
+constexpr int BSDF_LANES = 4;
+template <int KIND, bool SAMPLE>
+WPBR_D void bsdf_request(const SceneView& sv, const BsdfArgs& a, uint32_t i, int sub) {
+    const size_t n = a.n;
+    float4* out = reinterpret_cast<float4*>(a.value + (size_t)i * NSPEC) + sub;
+    // a queued request keeps its queue's kind; one without material gives zeros
+    const int kind = KIND < 0 ? bsdf_kind_of(sv, a, i) : (a.material[i] < a.n_materials ? KIND : BSDF_KIND_NONE);
+    float v = 0.0f;      // the value of every wavelength (all but a conductor)
+    float pdf = 0.0f;
+    f3 wiL = mk3(0.f, 0.f, 0.f), ss = wiL, ts = wiL, ns = wiL;
+    bool conductor = false, has_wi = false;
+    BsdfTerms bt{};
+    DevMaterial m{};
+    if (kind != BSDF_KIND_NONE && (SAMPLE || kind != MAT_SPECULAR)) {
+        // (Compute of a specular reflection: f = 0, pdf = 0, CBSDFSpecularReflectance.cpp:42-49)
+        m = sv.materials[a.material[i]];
+        auto g = [&](const float* p, int k) { return p[(size_t)k * n + i]; };
+        ss = mk3(g(a.frame, 0), g(a.frame, 1), g(a.frame, 2));
+        ts = mk3(g(a.frame, 3), g(a.frame, 4), g(a.frame, 5));
+        ns = mk3(g(a.frame, 6), g(a.frame, 7), g(a.frame, 8));
+        const f3 wo = mk3(g(a.wo, 0), g(a.wo, 1), g(a.wo, 2));
+        const f3 woL = mk3(dot3(ss, wo), dot3(ts, wo), dot3(ns, wo));
+        if (kind == MAT_SPECULAR) {
+            // Sample: mirror direction, pdf 1, R Fr(cos) / |cos| times |cos| (:26-61, 77-93)
+            wiL = mk3(-woL.x, -woL.y, woL.z);
+            const float c = abscos(wiL);
+            v = (c == 0) ? 0.0f * c : (1.0f * (fresnel_dielectric(wiL.z, 1.0f, m.eta) / c)) * c;
+            pdf = 1.0f;
+            has_wi = true;
+        } else {
+            f3 wh;
+            float whD;
+            bool ok = true;
+            if (!SAMPLE) {
+                // f (CBSDFConductorMicroface.cpp:44-58): the half vector of wo and wi
+                const f3 wi = mk3(g(a.wi, 0), g(a.wi, 1), g(a.wi, 2));
+                wiL = mk3(dot3(ss, wi), dot3(ts, wi), dot3(ns, wi));
+                wh = normalize3(add3(woL, wiL));
+                whD = ggx_d(wh, m.alpha, m.alpha);
+            } else {
+                // sample_f (:60-82): the sampled half vector (MicrofaceDistr.h:49-85), its
+                // reflection; a sample below the surface leaves everything 0
+                const float u0 = g(a.u, 0), u1 = g(a.u, 1);
+                const float phi = (float)((2 * PI_REF) * (double)u1);
+                const float tan2 = m.alpha * m.alpha * u0 / (1.0f - u0);
+                const float cosT = 1 / sqrtf(1 + tan2);
+                const float sinT = sqrtf(smax(0.f, 1.f - cosT * cosT));
+                wh = mk3(sinT * cr_cos(phi), sinT * cr_sin(phi), cosT);
+                if (!same_hemi(woL, wh)) wh = neg3(wh);
+                whD = ggx_d(wh, m.alpha, m.alpha);
+                wiL = add3(neg3(woL), mul3(wh, 2 * dot3(woL, wh)));
+                ok = has_wi = same_hemi(wiL, woL);
+            }
+            if (ok) {
+                bt = microfacet_terms(m, woL, wiL, wh, whD);
+                pdf = bt.pdf;
+                conductor = (KIND == MAT_CONDUCTOR || (KIND < 0 && kind == MAT_CONDUCTOR));
+                if (!conductor) v = bt.zero ? 0.0f * bt.ac : (1.0f * bt.sc) * bt.ac;
+            }
+        }
+    }
+    if (conductor) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int k = 16 * j + 4 * sub;
+            out[4 * j] = make_float4(bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k),
+                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k + 1),
+                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k + 2),
+                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k + 3));
+        }
+    } else {
+        const float4 v4 = make_float4(v, v, v, v);
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[4 * j] = v4;
+    }
+    if (sub) return;
+    a.pdf[i] = pdf;
+    if (SAMPLE) {
+        a.wi_out[i] = has_wi ? (ss.x * wiL.x + ts.x * wiL.y) + ns.x * wiL.z : 0.0f;
+        a.wi_out[n + i] = has_wi ? (ss.y * wiL.x + ts.y * wiL.y) + ns.y * wiL.z : 0.0f;
+        a.wi_out[2 * n + i] = has_wi ? (ss.z * wiL.x + ts.z * wiL.y) + ns.z * wiL.z : 0.0f;
+    }
+}
+
+// A queue launch (q requests listed in queue, all of kind KIND) or the fused
+// launch (KIND -1, queue null: requests 0 .. q - 1).  A wave takes 16 requests:
+// lane l serves request 16 w + (l & 15) with wavelength offset l >> 4, so each
+// spectral store covers 16 consecutive requests in each of 4 rows.
+constexpr uint32_t BSDF_PER_WAVE = 64 / BSDF_LANES;
+template <int KIND, bool SAMPLE>
+__global__ __launch_bounds__(256) void k_bsdf(SceneView sv, BsdfArgs a, const uint32_t* __restrict__ queue,
+                                              uint32_t q) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t t = wave * BSDF_PER_WAVE + (lane % BSDF_PER_WAVE);
+    if (t >= q) return;
+    const uint32_t i = queue ? queue[t] : t;
+    if (i < a.n) bsdf_request<KIND, SAMPLE>(sv, a, i, (int)(lane / BSDF_PER_WAVE));
+}
+
+// Per-kind queues in request order (JobGenerateBSDFRequests, CMaterial.h:30-38):
+// per 256 requests the counts of each kind (wave ballots), a one-workgroup scan
+// into block offsets (kinds one after another), then each request written at its
+// kind's offset plus its rank in the block.
+constexpr uint32_t BQ_BLOCK = 256;
+WPBR_D void bq_ranks(const SceneView& sv, const BsdfArgs& a, uint32_t i, uint32_t* s_cnt /* [4 waves][3] */,
+                     int* kind_out, uint32_t* rank_out) {
+    const int kind = i < a.n ? bsdf_kind_of(sv, a, i) : BSDF_KIND_NONE;
+    const int w = threadIdx.x >> 6;
+    uint32_t rank = 0;
+    for (int k = 0; k < 3; k++) {
+        const uint64_t m = ballot64(kind == k);
+        if (kind == k) rank = popc64(m & lanemask_lt());
+        if (lane_id() == 0) s_cnt[3 * w + k] = popc64(m);
+    }
+    *kind_out = kind;
+    *rank_out = rank;
+}
+__global__ __launch_bounds__(BQ_BLOCK) void k_bq_count(SceneView sv, BsdfArgs a, uint32_t* blk) {
+    __shared__ uint32_t s_cnt[3 * (BQ_BLOCK / 64)];
+    int kind;
+    uint32_t rank;
+    bq_ranks(sv, a, blockIdx.x * BQ_BLOCK + threadIdx.x, s_cnt, &kind, &rank);
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        uint32_t c = 0;
+        for (int w = 0; w < (int)(BQ_BLOCK / 64); w++) c += s_cnt[3 * w + threadIdx.x];
+        blk[3 * (size_t)blockIdx.x + threadIdx.x] = c;
+    }
+}
+// One workgroup: thread t takes a run of consecutive blocks, wave shuffles scan
+// the runs' sums, a thread per kind scans the 16 wave totals; each block's entry
+// becomes its offset in the kind-major queue and counts[] the kind totals.
+__global__ __launch_bounds__(1024) void k_bq_scan(uint32_t* blk, uint32_t nb, uint32_t* counts) {
+    __shared__ uint32_t s_w[16][3];
+    __shared__ uint32_t s_tot[3];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t per = (nb + 1023) / 1024;
+    const uint32_t b0 = min(nb, tid * per), b1 = min(nb, b0 + per);
+    uint32_t loc[3] = {0u, 0u, 0u}, inc[3];
+    for (uint32_t b = b0; b < b1; b++)
+        for (int k = 0; k < 3; k++) loc[k] += blk[3 * (size_t)b + k];
+    for (int k = 0; k < 3; k++) {
+        uint32_t v = loc[k];
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t x = __shfl_up(v, off, 64);
+            if (lane >= (uint32_t)off) v += x;
+        }
+        inc[k] = v;
+    }
+    if (lane == 63)
+        for (int k = 0; k < 3; k++) s_w[w][k] = inc[k];
+    __syncthreads();
+    if (tid < 3) {
+        uint32_t run = 0;
+        for (int ww = 0; ww < 16; ww++) {
+            const uint32_t c = s_w[ww][tid];
+            s_w[ww][tid] = run;
+            run += c;
+        }
+        s_tot[tid] = run;
+    }
+    __syncthreads();
+    const uint32_t kind_off[3] = {0u, s_tot[0], s_tot[0] + s_tot[1]};
+    uint32_t base[3];
+    for (int k = 0; k < 3; k++) base[k] = kind_off[k] + s_w[w][k] + inc[k] - loc[k];
+    for (uint32_t b = b0; b < b1; b++)
+        for (int k = 0; k < 3; k++) {
+            const uint32_t c = blk[3 * (size_t)b + k];
+            blk[3 * (size_t)b + k] = base[k];
+            base[k] += c;
+        }
+    if (tid < 3) counts[tid] = s_tot[tid];
+}
+__global__ __launch_bounds__(BQ_BLOCK) void k_bq_scatter(SceneView sv, BsdfArgs a, const uint32_t* blk,
+                                                         uint32_t* queue) {
+    __shared__ uint32_t s_cnt[3 * (BQ_BLOCK / 64)];
+    const uint32_t i = blockIdx.x * BQ_BLOCK + threadIdx.x;
+    int kind;
+    uint32_t rank;
+    bq_ranks(sv, a, i, s_cnt, &kind, &rank);
+    __syncthreads();
+    if (kind < 3) {
+        uint32_t before = 0;
+        for (int w = 0; w < (int)(threadIdx.x >> 6); w++) before += s_cnt[3 * w + kind];
+        queue[blk[3 * (size_t)blockIdx.x + kind] + before + rank] = i;
+    }
+}
+
+wpbr_status bsdf_queues_launch(const SceneView& sv, const BsdfArgs& a, uint32_t* blk, uint32_t* counts,
+                               uint32_t* queue, hipStream_t st) {
+    const uint32_t nb = (a.n + BQ_BLOCK - 1) / BQ_BLOCK;
+    if (!nb) return WPBR_OK;
+    hipLaunchKernelGGL(k_bq_count, dim3(nb), dim3(BQ_BLOCK), 0, st, sv, a, blk);
+    hipLaunchKernelGGL(k_bq_scan, dim3(1), dim3(1024), 0, st, blk, nb, counts);
+    hipLaunchKernelGGL(k_bq_scatter, dim3(nb), dim3(BQ_BLOCK), 0, st, sv, a, blk, queue);
+    return launch_status();
+}
+
+// kind: MAT_CONDUCTOR / MAT_DIELECTRIC / MAT_SPECULAR over q queued requests, or
+// -1: the fused launch over all n.
+wpbr_status bsdf_launch(const SceneView& sv, const BsdfArgs& a, int kind, bool sample, const uint32_t* queue,
+                        uint32_t q, hipStream_t st) {
+    if (!q) return WPBR_OK;
+    const dim3 grid((uint32_t)(((uint64_t)q * BSDF_LANES + 255) / 256)), block(256);
+#define WPBR_BSDF_LAUNCH(K)                                                                        \
+    if (sample) hipLaunchKernelGGL((k_bsdf<K, true>), grid, block, 0, st, sv, a, queue, q);        \
+    else hipLaunchKernelGGL((k_bsdf<K, false>), grid, block, 0, st, sv, a, queue, q);
+    switch (kind) {
+        case MAT_CONDUCTOR: WPBR_BSDF_LAUNCH(MAT_CONDUCTOR) break;
+        case MAT_DIELECTRIC: WPBR_BSDF_LAUNCH(MAT_DIELECTRIC) break;
+        case MAT_SPECULAR: WPBR_BSDF_LAUNCH(MAT_SPECULAR) break;
+        default: WPBR_BSDF_LAUNCH(-1) break;
+    }
+#undef WPBR_BSDF_LAUNCH
+    return launch_status();
+}
+
 wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st) {
     if (npix) hipLaunchKernelGGL(k_film_rgb8, dim3((npix + 255) / 256), dim3(256), 0, st,
                                  reinterpret_cast<const float4*>(film), npix, out);

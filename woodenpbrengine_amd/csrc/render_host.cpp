@@ -609,6 +609,94 @@ wpbr_status wpbr_collide_interactions(wpbr_ctx* ctx, const wpbr_geometry* geom, 
     return collide_interactions(ctx, sv, bvh, n, d_rays, d_surf, d_ray_diff, d_hit_shape, d_hit_t, d_si);
 }
 
+}  // extern "C"
+
+static wpbr_status bsdf_args(wpbr_ctx* ctx, wpbr_scene* sc, const wpbr_bsdf_requests* req, uint32_t n, bool sample,
+                             float* d_value, float* d_pdf, float* d_wi, BsdfArgs& a) {
+    if (!ctx || !sc || !req) return WPBR_ERR_INVALID_ARG;
+    if (n > (1u << 30)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "bsdf requests: more than 2^30");
+    if (n && (!req->d_frame || !req->d_wo || !req->d_material || !d_value || !d_pdf ||
+              (sample ? (!req->d_u || !d_wi) : !req->d_wi)))
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "bsdf requests: a required array is null");
+    if (reinterpret_cast<uintptr_t>(d_value) % 16)
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "bsdf value array: not 16-byte aligned");
+    a.frame = req->d_frame;
+    a.wo = req->d_wo;
+    a.wi = req->d_wi;
+    a.u = req->d_u;
+    a.material = req->d_material;
+    a.n = n;
+    a.n_materials = (uint32_t)(sc->h_materials.size() / 9);
+    a.value = d_value;
+    a.pdf = d_pdf;
+    a.wi_out = d_wi;
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    return WPBR_OK;
+}
+
+static wpbr_status bsdf_run(wpbr_ctx* ctx, wpbr_scene* sc, int kind, const BsdfArgs& a, bool sample,
+                            const uint32_t* d_queue, uint32_t q) {
+    if (kind != WPBR_BSDF_ANY && (kind < 0 || kind > 2)) return set_err(ctx, WPBR_ERR_INVALID_ARG, "bsdf kind");
+    if (kind != WPBR_BSDF_ANY && q && !d_queue) return set_err(ctx, WPBR_ERR_INVALID_ARG, "null queue");
+    hipEventRecord(ctx->ev0, ctx->stream);
+    wpbr_status s = bsdf_launch(sc->sv, a, kind == WPBR_BSDF_ANY ? -1 : kind, sample,
+                                kind == WPBR_BSDF_ANY ? nullptr : d_queue, kind == WPBR_BSDF_ANY ? a.n : q,
+                                ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("bsdf launch failed: ") + hipGetErrorString(last_launch_error));
+    s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "bsdf");
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
+}
+
+extern "C" {
+
+wpbr_status wpbr_bsdf_queues(wpbr_ctx* ctx, wpbr_scene* sc, const wpbr_bsdf_requests* req, uint32_t n,
+                             uint32_t* d_queue, uint32_t counts[3]) {
+    if (!ctx || !sc || !req || !counts || (n && (!req->d_material || !d_queue))) return WPBR_ERR_INVALID_ARG;
+    BsdfArgs a{};
+    a.material = req->d_material;
+    a.n = n;
+    a.n_materials = (uint32_t)(sc->h_materials.size() / 9);
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();
+    counts[0] = counts[1] = counts[2] = 0;
+    if (!n) return WPBR_OK;
+    const size_t words = 3 * (((size_t)n + 255) / 256) + 4;
+    if (words > ctx->bsdf_blk_words) {
+        if (ctx->bsdf_blk) hipFree(ctx->bsdf_blk);
+        ctx->bsdf_blk = nullptr;
+        ctx->bsdf_blk_words = 0;
+        wpbr_status s = hip_check(ctx, hipMalloc(&ctx->bsdf_blk, 4 * words), "hipMalloc bsdf queues");
+        if (s) return s;
+        ctx->bsdf_blk_words = words;
+    }
+    uint32_t* d_counts = ctx->bsdf_blk + words - 4;
+    hipEventRecord(ctx->ev0, ctx->stream);
+    wpbr_status s = bsdf_queues_launch(sc->sv, a, ctx->bsdf_blk, d_counts, d_queue, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (s) return set_err(ctx, s, std::string("bsdf queues launch failed: ") + hipGetErrorString(last_launch_error));
+    s = hip_check(ctx, hipMemcpyAsync(counts, d_counts, 12, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "bsdf queues");
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
+}
+
+wpbr_status wpbr_bsdf_compute(wpbr_ctx* ctx, wpbr_scene* sc, int kind, const wpbr_bsdf_requests* req, uint32_t n,
+                              const uint32_t* d_queue, uint32_t q, float* d_value, float* d_pdf) {
+    BsdfArgs a{};
+    wpbr_status s = bsdf_args(ctx, sc, req, n, false, d_value, d_pdf, nullptr, a);
+    return s ? s : bsdf_run(ctx, sc, kind, a, false, d_queue, q);
+}
+
+wpbr_status wpbr_bsdf_sample(wpbr_ctx* ctx, wpbr_scene* sc, int kind, const wpbr_bsdf_requests* req, uint32_t n,
+                             const uint32_t* d_queue, uint32_t q, float* d_wi, float* d_value, float* d_pdf) {
+    BsdfArgs a{};
+    wpbr_status s = bsdf_args(ctx, sc, req, n, true, d_value, d_pdf, d_wi, a);
+    return s ? s : bsdf_run(ctx, sc, kind, a, true, d_queue, q);
+}
+
 wpbr_status wpbr_set_render_overlap(wpbr_ctx* ctx, int overlap) {
     if (!ctx || (overlap != 0 && overlap != 1)) return WPBR_ERR_INVALID_ARG;
     ctx->render_overlap = overlap;

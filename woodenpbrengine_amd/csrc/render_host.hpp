@@ -10,6 +10,11 @@ namespace wpbr {
 
 wpbr_status render_pass_launch(RenderLaunch& L, int stage, hipStream_t st);
 wpbr_status film_rgb8_launch(const float* film, uint32_t npix, uint8_t* out, hipStream_t st);
+struct BsdfArgs;
+wpbr_status bsdf_queues_launch(const SceneView& sv, const BsdfArgs& a, uint32_t* blk, uint32_t* counts,
+                               uint32_t* queue, hipStream_t st);
+wpbr_status bsdf_launch(const SceneView& sv, const BsdfArgs& a, int kind, bool sample, const uint32_t* queue,
+                        uint32_t q, hipStream_t st);
 wpbr_status scene_si_launch(const SceneView& sv, uint32_t n, const float* rays, const uint8_t* surf,
                             const float* diff, const uint32_t* hit_shape, const float* hit_t, float* si,
                             hipStream_t st);

@@ -44,6 +44,19 @@ struct SceneView {
     uint32_t width, height;
 };
 
+// Requests of the BSDF Compute / Sample jobs (wpbr_bsdf_*), SoA device arrays.
+struct BsdfArgs {
+    const float* frame;       // 9 x n: ss, ts, ns
+    const float* wo;          // 3 x n
+    const float* wi;          // 3 x n (Compute)
+    const float* u;           // 2 x n (Sample)
+    const uint32_t* material; // n
+    uint32_t n, n_materials;
+    float* value;             // 64 x n
+    float* pdf;               // n
+    float* wi_out;            // 3 x n (Sample)
+};
+
 // Per-sample state between the wavefront stages (SoA, one entry per pixel).
 struct WaveState {
     uint32_t* c1;         // 1D cursor

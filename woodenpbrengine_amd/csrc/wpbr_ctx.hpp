@@ -31,6 +31,8 @@ struct wpbr_ctx {
     int lbvh_order = WPBR_LBVH_ORDER_REFERENCE;  // key order of new LBVH objects
     int render_overlap = 1;   // wpbr_render: consecutive passes overlap on two streams
     float* loss_scratch = nullptr;  // this device's copy of wpbr_film_loss_noalloc's scratch
+    uint32_t* bsdf_blk = nullptr;   // wpbr_bsdf_queues: 3 counts per 256 requests + 3 totals
+    size_t bsdf_blk_words = 0;
     // segmented LBVH builds: per-scene descriptors and block maps
     void* batch_mem = nullptr;
     size_t batch_cap = 0;

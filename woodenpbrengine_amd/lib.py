@@ -33,7 +33,18 @@ EXPORTS = [
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
     "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
     "wpbr_set_render_overlap", "wpbr_film_loss_noalloc", "wpbr_scene_collide", "wpbr_collide_interactions",
+    "wpbr_bsdf_queues", "wpbr_bsdf_compute", "wpbr_bsdf_sample",
 ]
+
+# BSDF families of the Compute / Sample jobs (wpbr.h WPBR_BSDF_*).
+BSDF_CONDUCTOR_MF, BSDF_DIELECTRIC_MF, BSDF_SPECULAR_REFL, BSDF_ANY = 0, 1, 2, -1
+
+
+class BsdfRequests(C.Structure):
+    """wpbr_bsdf_requests: device pointers (frame 9 x n, wo 3 x n, wi 3 x n,
+    u 2 x n, material n)."""
+    _fields_ = [("d_frame", C.c_void_p), ("d_wo", C.c_void_p), ("d_wi", C.c_void_p), ("d_u", C.c_void_p),
+                ("d_material", C.c_void_p)]
 
 # CSurfaceInteraction as SoA floats (wpbr.h WPBR_SI_*): field offsets, then the count.
 SI_FIELDS = {"p": 0, "wo": 3, "time": 6, "n": 7, "uv": 10, "dpdu": 12, "dpdv": 15, "dndu": 18, "dndv": 21,
@@ -119,6 +130,9 @@ def load():
         "wpbr_set_render_overlap": (i32, [vp, C.c_int]),
         "wpbr_scene_collide": (i32, [vp, vp, u32, vp, vp, vp, vp, vp, vp]),
         "wpbr_collide_interactions": (i32, [vp, C.POINTER(Geometry), vp, vp, u32, vp, vp, vp, vp, vp, vp]),
+        "wpbr_bsdf_queues": (i32, [vp, vp, C.POINTER(BsdfRequests), u32, vp, C.POINTER(u32)]),
+        "wpbr_bsdf_compute": (i32, [vp, vp, C.c_int, C.POINTER(BsdfRequests), u32, vp, u32, vp, vp]),
+        "wpbr_bsdf_sample": (i32, [vp, vp, C.c_int, C.POINTER(BsdfRequests), u32, vp, u32, vp, vp, vp]),
         "wpbr_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_collide_async": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp, vp]),
         "wpbr_set_trace_config": (i32, [vp, C.POINTER(TraceConfig)]),
@@ -563,6 +577,40 @@ class Renderer:
             self.ctx.h, self.h, n, d_rays.ptr, d_surf.ptr if d_surf is not None else None,
             d_diff.ptr if d_diff is not None else None, d_s.ptr, d_t.ptr, d_si.ptr if d_si is not None else None))
         return d_s, d_t, d_si
+
+    def bsdf_requests(self, frame, wo, material, wi=None, u=None):
+        """Device copies of BSDF requests (SoA host arrays) and their wpbr_bsdf_requests."""
+        arrs = {"frame": self.ctx.array(np.ascontiguousarray(frame, np.float32)),
+                "wo": self.ctx.array(np.ascontiguousarray(wo, np.float32)),
+                "material": self.ctx.array(np.ascontiguousarray(material, np.uint32))}
+        if wi is not None:
+            arrs["wi"] = self.ctx.array(np.ascontiguousarray(wi, np.float32))
+        if u is not None:
+            arrs["u"] = self.ctx.array(np.ascontiguousarray(u, np.float32))
+        req = BsdfRequests(arrs["frame"].ptr, arrs["wo"].ptr, arrs["wi"].ptr if "wi" in arrs else None,
+                           arrs["u"].ptr if "u" in arrs else None, arrs["material"].ptr)
+        return req, arrs
+
+    def bsdf_queues(self, req, n):
+        d_q = self.ctx.empty(n, np.uint32)
+        c = (C.c_uint32 * 3)()
+        self.ctx.check(self.ctx.lib.wpbr_bsdf_queues(self.ctx.h, self.h, C.byref(req), n, d_q.ptr, c))
+        return d_q, [int(c[0]), int(c[1]), int(c[2])]
+
+    def bsdf(self, sample, kind, req, n, d_queue=None, q=0, out=None):
+        """One BSDF Compute / Sample launch; out = (d_value, d_pdf, d_wi) to reuse."""
+        if out is None:
+            out = (self.ctx.empty((n, 64), np.float32), self.ctx.empty(n, np.float32),
+                   self.ctx.empty((3, n), np.float32) if sample else None)
+        d_v, d_p, d_w = out
+        qp = d_queue.ptr if d_queue is not None else None
+        if sample:
+            self.ctx.check(self.ctx.lib.wpbr_bsdf_sample(self.ctx.h, self.h, kind, C.byref(req), n, qp, q,
+                                                         d_w.ptr, d_v.ptr, d_p.ptr))
+        else:
+            self.ctx.check(self.ctx.lib.wpbr_bsdf_compute(self.ctx.h, self.h, kind, C.byref(req), n, qp, q,
+                                                          d_v.ptr, d_p.ptr))
+        return out
 
     def stats(self):
         buf = (C.c_double * 8)()
