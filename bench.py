@@ -33,7 +33,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-TRAFFIC_FILE = ROOT / "profiles" / "k_trace_render_c2_pmc.txt"
+TRAFFIC_FILE = ROOT / "profiles" / "render_c2_pmc.txt"
 CPU_TILE_STRIDE = 4
 
 
@@ -311,6 +311,9 @@ def main():
     ap.add_argument("--no-stratified", action="store_true", help="skip the stratified-sampler render")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-bsdf", action="store_true", help="skip the BSDF job timings")
+    ap.add_argument("--serial-passes", action="store_true",
+                    help="run the timed renders with their sample passes one after another (no two "
+                         "kernels share the GPU), for per-kernel counter profiles")
     ap.add_argument("--bsdf-only", action="store_true",
                     help="only the BSDF jobs over the full render's primary interactions, for kernel profiles")
     ap.add_argument("--trace-config", default=None,
@@ -396,6 +399,8 @@ def main():
     else:
         tuned = r.autotune()
     log("traversal configuration:", tuned)
+    if args.serial_passes:
+        ctx.set_render_overlap(0)
     for _ in range(args.warmup):
         r.render()
     ctx.sync()
@@ -547,8 +552,9 @@ def main():
                 "kernel": "k_trace in the render, passes serialised (36 B of ray I/O per ray; %.0f rays per launch)"
                           % (s_rays / s_launch),
                 "avg_launch_us": avg_launch_ms * 1e3,
-                "limiter": "L2/MALL latency of the node and primitive gathers, not HBM bandwidth "
-                           "(profiles/k_trace_render_c2_pmc.txt: L2 hit rate, VALU busy)"}
+                "limiter": "latency/L2: divergence and the latency of node and primitive gathers, not HBM "
+                           "bandwidth (profiles/render_c2_pmc.txt: 28 % VALU busy, 28 % of lanes active, "
+                           "L1 94 %, L2 56 %, 254 GB/s of HBM-side traffic)"}
     mb = {}
     if not args.no_microbench:
         mb = bench_microbench(lib, scenes, ctx, args.steps)

@@ -381,6 +381,17 @@ def texture_lookup(rgb, queries):
     return out
 
 
+def texture_ewa(rgb, level, queries):
+    """The EWA filter of one MIP level on its own (queries n x 6 as texture_lookup)."""
+    lib = load()
+    lib.orc_texture_ewa.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    rgb = _f32(rgb)
+    q = _f32(queries).reshape(-1, 6)
+    out = np.empty((len(q), 3), np.float32)
+    lib.orc_texture_ewa(rgb.shape[0], _p(rgb), level, len(q), _p(q), _p(out))
+    return out
+
+
 def samples(seed, pixel, n1, n2):
     lib = load()
     lib.orc_samples.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]

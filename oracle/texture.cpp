@@ -136,10 +136,11 @@ RGB anisotropic(const float* q, const Texture& tex, const std::vector<float>& fi
     // the reference's swap test compares dstdx with itself: never taken
     float majorLength = std::sqrt(dst0[0] * dst0[0] + dst0[1] * dst0[1]);
     float minorLength = std::sqrt(dst1[0] * dst1[0] + dst1[1] * dst1[1]);
+    // The clamp lengthens the minor axis for the level choice only: ewa() reads
+    // the caller's dstdx / dstdy again (CSTextureProcess.h:183-185), so the
+    // filter ellipse keeps the unscaled derivatives.
     if (minorLength * maxAnisotropy < majorLength && minorLength > 0) {
         float scale = majorLength / (minorLength * maxAnisotropy);
-        dst1[0] *= scale;
-        dst1[1] *= scale;
         minorLength *= scale;
     }
     if (minorLength == 0) return triangle(q, tex, 0);
@@ -171,6 +172,16 @@ uint32_t orc_texture_levels(uint32_t res, const float* rgb, float* out) {
 void orc_texture_filter(float* w64) {
     std::vector<float> w = orc::filter_table();
     for (size_t i = 0; i < w.size(); i++) w64[i] = w[i];
+}
+
+void orc_texture_ewa(uint32_t res, const float* rgb, uint32_t level, uint32_t n, const float* q, float* out) {
+    orc::Texture t = orc::generate_mips(res, rgb);
+    std::vector<float> f = orc::filter_table();
+    for (uint32_t i = 0; i < n; i++) {
+        const float* qi = q + 6 * (size_t)i;
+        orc::RGB v = orc::ewa(qi, qi + 2, qi + 4, t, f, level);
+        for (int c = 0; c < 3; c++) out[3 * (size_t)i + c] = v.v[c];
+    }
 }
 
 void orc_texture_lookup(uint32_t res, const float* rgb, uint32_t n, const float* q, float* out) {

@@ -164,13 +164,20 @@ def test_png_round_trip_with_writer(tmp_path):
     assert np.array_equal(lib.read_png(tmp_path / "w.png"), img)
 
 
+def test_png_palette_index_beyond_palette_is_black(tmp_path):
+    # lodepng fills the 256 palette entries with black before reading PLTE and
+    # does not check indices (lodepng.cpp:2607-2620, 3378-3384)
+    idx = np.array([[0, 3], [1, 2]]).reshape(2, 2, 1)
+    out = _decode(tmp_path, _png(idx, 2, 3, palette=[[1, 2, 3], [40, 50, 60]]))
+    exp = np.array([[[1, 2, 3], [0, 0, 0]], [[40, 50, 60], [0, 0, 0]]], np.uint8)
+    assert np.array_equal(out, exp)
+
+
 def test_png_errors(tmp_path):
     good = _png(np.zeros((4, 4, 3), int), 8, 2)
     bad_crc = bytearray(good)
     bad_crc[8 + 8 + 13] ^= 1  # IHDR CRC
     cases = {"crc": bytes(bad_crc), "sig": b"\x00" + good[1:], "trunc": good[:60]}
-    # palette index beyond the palette
-    cases["pal"] = _png(np.full((2, 2, 1), 3), 2, 3, palette=[[1, 2, 3]])
     for name, data in cases.items():
         with pytest.raises(lib.WpbrError):
             _decode(tmp_path, data, name + ".png")

@@ -88,3 +88,44 @@ def test_constant_texture_is_constant_everywhere():
     q[:, 2:] = rng.normal(0, 1, (300, 4)) * np.exp(rng.uniform(-8, 0, (300, 1)))
     out = po.texture_lookup(tex, q)
     np.testing.assert_allclose(out, 0.3, rtol=1e-5)
+
+
+def test_anisotropy_clamp_picks_the_level_only():
+    """Beyond 32:1 the clamp lengthens the minor axis for the level choice only
+    (CSTextureProcess.h:151-157); ewa() reads the caller's dstdx / dstdy again
+    (:183-185).  A clamped lookup equals the lerp of the two levels' EWA with the
+    unscaled derivatives, at the levels the clamped length picks, and differs
+    from EWA with the scaled minor axis."""
+    res = 64
+    rng = np.random.default_rng(21)
+    tex = rng.random((res, res, 3), dtype=np.float32)
+    n_levels = len(po.texture_levels(tex))
+    q = []
+    for k in range(40):
+        ang = rng.uniform(0, np.pi)
+        # lod = n_levels - 1 + log2(major / 32) at k / 10 + 0.05 (levels 0 .. n_levels - 2)
+        lod_target = (k % (10 * (n_levels - 1))) / 10 + 0.05
+        major = 32 * 2.0 ** (lod_target - (n_levels - 1))
+        minor = major / rng.uniform(60.0, 400.0)
+        u = np.array([np.cos(ang), np.sin(ang)])
+        v = np.array([-np.sin(ang), np.cos(ang)])
+        q.append([rng.uniform(0.2, 0.8), rng.uniform(0.2, 0.8), *(major * u), *(minor * v)])
+    q = np.array(q, np.float32)
+    got = po.texture_lookup(tex, q)
+    worst_scaled = 0.0
+    for i in range(len(q)):
+        major = np.hypot(np.float64(q[i, 2]), np.float64(q[i, 3]))
+        lod = max(0.0, n_levels - 1 + np.log2(major / 32))
+        il = int(np.floor(lod))
+        assert il < n_levels - 1 and 1e-2 < lod - il < 1 - 1e-2
+        d = lod - il
+        a = po.texture_ewa(tex, il, q[i:i + 1])[0].astype(np.float64)
+        b = po.texture_ewa(tex, il + 1, q[i:i + 1])[0].astype(np.float64)
+        np.testing.assert_allclose(got[i], (1 - d) * a + d * b, rtol=2e-6, atol=1e-7)
+        # the ellipse with the minor derivative scaled up to major / 32 is another filter
+        qs = q[i:i + 1].copy()
+        qs[0, 4:] *= np.float32(major / 32 / np.hypot(np.float64(q[i, 4]), np.float64(q[i, 5])))
+        sa = po.texture_ewa(tex, il, qs)[0].astype(np.float64)
+        sb = po.texture_ewa(tex, il + 1, qs)[0].astype(np.float64)
+        worst_scaled = max(worst_scaled, float(np.abs((1 - d) * sa + d * sb - got[i]).max()))
+    assert worst_scaled > 1e-3

@@ -150,17 +150,19 @@ bool inflate(const uint8_t* src, size_t n, std::vector<uint8_t>& out) {
             out.insert(out.end(), src + br.pos, src + br.pos + len);
             br.pos += len;
         } else if (type == 1) {
-            static Huffman lit, dist;
-            static bool ready = false;
-            if (!ready) {
+            // the fixed codes (RFC 1951 3.2.6), built once; a function-local static's
+            // initialisation is thread-safe
+            struct Fixed { Huffman lit, dist; };
+            static const Fixed fixed = [] {
+                Fixed f;
                 uint8_t l[288];
                 for (int s = 0; s < 288; s++) l[s] = s < 144 ? 8 : s < 256 ? 9 : s < 280 ? 7 : 8;
-                huff_build(lit, l, 288);
+                huff_build(f.lit, l, 288);
                 for (int s = 0; s < 30; s++) l[s] = 5;
-                huff_build(dist, l, 30);
-                ready = true;
-            }
-            if (!inflate_codes(br, lit, dist, out)) return false;
+                huff_build(f.dist, l, 30);
+                return f;
+            }();
+            if (!inflate_codes(br, fixed.lit, fixed.dist, out)) return false;
         } else if (type == 2) {
             const int nlit = (int)br.bits(5) + 257, ndist = (int)br.bits(5) + 1, ncode = (int)br.bits(4) + 4;
             static const uint8_t order[19] = {16, 17, 18, 0, 8, 7, 9, 6, 10, 5, 11, 4, 12, 3, 13, 2, 14, 1, 15};
@@ -222,18 +224,19 @@ bool zlib_decompress(const std::vector<uint8_t>& z, std::vector<uint8_t>& out) {
 uint32_t be32(const uint8_t* p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
 
 uint32_t crc32_bytes(const uint8_t* p, size_t n) {
-    static uint32_t table[256];
-    static bool ready = false;
-    if (!ready) {
+    // built once (a function-local static's initialisation is thread-safe)
+    struct Table { uint32_t t[256]; };
+    static const Table table = [] {
+        Table tb;
         for (uint32_t i = 0; i < 256; i++) {
             uint32_t c = i;
             for (int k = 0; k < 8; k++) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            table[i] = c;
+            tb.t[i] = c;
         }
-        ready = true;
-    }
+        return tb;
+    }();
     uint32_t crc = 0xFFFFFFFFu;
-    for (size_t i = 0; i < n; i++) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    for (size_t i = 0; i < n; i++) crc = table.t[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
     return ~crc;
 }
 
@@ -357,7 +360,9 @@ uint32_t sample_at(const uint8_t* row, uint32_t x, int c, const PngInfo& pi) {
 }
 
 // lodepng's conversion to 8-bit RGB: grey replicated (sub-byte grey scaled by
-// 255 / (2^depth - 1)), palette looked up, alpha dropped.
+// 255 / (2^depth - 1)), palette looked up (an index beyond the palette gives
+// black: lodepng fills the unused entries with black, lodepng.cpp:2607-2620, and
+// does not check the index, :3378-3384), alpha dropped.
 bool to_rgb(const uint8_t* row, uint32_t x, const PngInfo& pi, uint8_t* o) {
     switch (pi.color) {
         case 0: case 4: {
@@ -368,8 +373,8 @@ bool to_rgb(const uint8_t* row, uint32_t x, const PngInfo& pi, uint8_t* o) {
         }
         case 3: {
             const uint32_t i = sample_at(row, x, 0, pi);
-            if (3 * (size_t)i + 2 >= pi.palette.size()) return false;
-            std::memcpy(o, &pi.palette[3 * i], 3);
+            if (3 * (size_t)i + 2 >= pi.palette.size()) o[0] = o[1] = o[2] = 0;
+            else std::memcpy(o, &pi.palette[3 * i], 3);
             return true;
         }
         default:

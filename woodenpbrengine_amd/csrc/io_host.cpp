@@ -17,18 +17,19 @@ using namespace wpbr;
 namespace {
 
 uint32_t crc32_of(const uint8_t* p, size_t n, uint32_t crc) {
-    static uint32_t table[256];
-    static bool ready = false;
-    if (!ready) {
+    // built once (a function-local static's initialisation is thread-safe)
+    struct Table { uint32_t t[256]; };
+    static const Table table = [] {
+        Table tb;
         for (uint32_t i = 0; i < 256; i++) {
             uint32_t c = i;
             for (int k = 0; k < 8; k++) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            table[i] = c;
+            tb.t[i] = c;
         }
-        ready = true;
-    }
+        return tb;
+    }();
     crc = ~crc;
-    for (size_t i = 0; i < n; i++) crc = table[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
+    for (size_t i = 0; i < n; i++) crc = table.t[(crc ^ p[i]) & 0xFFu] ^ (crc >> 8);
     return ~crc;
 }
 

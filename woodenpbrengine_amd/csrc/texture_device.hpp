@@ -88,14 +88,14 @@ WPBR_D float3 tex_ewa(const TexView& T, float s_, float t_, float dsx, float dtx
 
 // STexture2DSamplerAnisotropic::eval (CSTextureProcess.h:136-180).  The
 // reference's swap test compares dstdx with itself, so the major axis is always
-// dstdx; the lerp between levels is (1 - d) a + d b.
+// dstdx; the lerp between levels is (1 - d) a + d b.  The anisotropy clamp
+// lengthens the minor axis for the level choice only: ewa() reads the caller's
+// derivatives again (:183-185), so the filter ellipse keeps them unscaled.
 WPBR_D float3 tex_lookup(const TexView& T, float s, float t, float dsx, float dtx, float dsy, float dty) {
     const float major = sqrtf(dsx * dsx + dtx * dtx);
     float minor = sqrtf(dsy * dsy + dty * dty);
     if (minor * TEX_MAX_ANISOTROPY < major && minor > 0) {
         const float scale = major / (minor * TEX_MAX_ANISOTROPY);
-        dsy *= scale;
-        dty *= scale;
         minor *= scale;
     }
     if (minor == 0) return tex_bilinear(T, s, t);
