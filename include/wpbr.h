@@ -256,6 +256,32 @@ wpbr_status wpbr_scene_rebuild(wpbr_ctx* ctx, wpbr_scene* scene);
  * tonemap).  d_sample_xyz (optional) receives width*height*spp*3 floats, the
  * XYZ of each sample's radiance at index (pixel * spp + j). */
 wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* d_sample_xyz);
+/* The render's JOB_RUN sequence (MEngine::render, MEngine.cpp:380-448) one
+ * stage at a time, each stage one launch over every film tile on the context's
+ * stream, finished when the call returns:
+ *   WPBR_STAGE_BEGIN          film and queue counters cleared (once, before pass 0)
+ *   WPBR_STAGE_CAMERA         JobSamplerUpdateCameraSamples, JobCameraPerspGenerateRaysDifferential (:391-392)
+ *   WPBR_STAGE_PRIMARY_TRACE  runCollisionSystem of the camera rays (:393)
+ *   WPBR_STAGE_SCATTER        JobScatteringRequestEmittedLight .. JobScatteringSampleLight,
+ *                             JobScatteringSampleLightLI, the light sampling requests and
+ *                             JobScatteringCastShadowRays; JobScatteringSampleBSDF, the BSDF
+ *                             Sample jobs and JobScatteringCastShadowRaysWithInteraction (:394-433)
+ *   WPBR_STAGE_SHADOW_TRACE   both shadow-ray runCollisionSystem calls (:412, :434) as one launch
+ *                             over the compacted shadow-ray queue
+ *   WPBR_STAGE_INTEGRATE      JobScatteringProcessShadowRay*, the BSDF Compute jobs,
+ *                             JobSphereLightProcessComputeRequests, JobLightInfiniteAreaLiCompute,
+ *                             JobScatteringIntegrateImportance*, JobScatteringFinish,
+ *                             JobAccumalateLIFromSamples (:413-444)
+ * BEGIN, then CAMERA .. INTEGRATE of pass 0, 1, .. spp - 1 leave d_film (and
+ * d_sample_xyz) equal to wpbr_render's bit for bit; JobOutputFilmTitles /
+ * JobOutputFilm are wpbr_film_rgb8 / wpbr_write_png on the film.  A stage out of
+ * this order reads whatever the previous stages left. */
+enum {
+    WPBR_STAGE_BEGIN = 0, WPBR_STAGE_CAMERA = 1, WPBR_STAGE_PRIMARY_TRACE = 2, WPBR_STAGE_SCATTER = 3,
+    WPBR_STAGE_SHADOW_TRACE = 4, WPBR_STAGE_INTEGRATE = 5
+};
+wpbr_status wpbr_render_stage(wpbr_ctx* ctx, wpbr_scene* scene, float* d_film, float* d_sample_xyz, uint32_t pass,
+                              int stage);
 /* overlap 1 (the default): consecutive sample passes of wpbr_render overlap on
  * two streams (pass j's camera, primary trace and shading run beside pass
  * j - 1's secondary trace and splat).  0: each pass starts after the previous

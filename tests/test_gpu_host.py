@@ -126,3 +126,84 @@ def test_job_sequences_surface_interactions(tmp_path):
     print("surface interactions: %d; fields differing: %s" % (((es != 0xFFFFFFFF) & (surf == 1)).sum(), bad),
           flush=True)
     assert same.all()
+
+
+RENDER_DEMO = PACKAGE_DIR / "host" / "wpbr_render_demo"
+
+
+def _write_scene(path, s):
+    """The scene's wpbr_scene_desc as wpbr_render_demo reads it: each uint32
+    field as 4 bytes, each array as u64 byte count + bytes, in declaration order."""
+    from woodenpbrengine_amd import lib
+    d, keep = lib.scene_desc(s)
+    arrays = iter(keep)
+    with open(path, "wb") as f:
+        for name, ty in lib.SceneDesc._fields_:
+            if ty is lib.C.c_uint32:
+                f.write(np.uint32(getattr(d, name)).tobytes())
+            else:
+                a = np.ascontiguousarray(next(arrays))
+                f.write(np.uint64(a.nbytes).tobytes())
+                f.write(a.tobytes())
+
+
+def test_render_job_sequence_mirror(ctx, tmp_path):
+    """MEngine::render's JOB_RUN sequence (MEngine.cpp:380-448) through the
+    mirror: the film equals wpbr_render's bit for bit, JobOutputFilmTitles the
+    device tonemap and JobOutputFilm a PNG of the same bytes.  Then the BSDF
+    jobs over host CBSDFRequest components (queues, each family's Compute and
+    Sample job) against the oracle: pdf and wi bit-exact, values within 1e-4."""
+    from woodenpbrengine_amd import lib
+    assert RENDER_DEMO.exists(), "build the host demos first (__graft_entry__.build())"
+    s = scenes.c2_scene(width=64, height=64, spp=2, freq=24, env_size=(64, 32))
+    s.sampler = lib.SAMPLER_STRATIFIED
+    scene_bin = tmp_path / "scene.bin"
+    _write_scene(scene_bin, s)
+    # BSDF requests: random frames and directions over the scene's materials
+    g = np.random.default_rng(77)
+    n = 5000
+    q, _ = np.linalg.qr(g.normal(size=(3, 3)))
+    frame = np.repeat(q.astype(np.float32).reshape(9, 1), n, axis=1)
+    wl = g.normal(size=(n, 3))
+    wl[:, 2] = np.abs(wl[:, 2]) + 0.05
+    wl /= np.linalg.norm(wl, axis=1, keepdims=True)
+    wo = (wl @ q).T.astype(np.float32)
+    wi = g.normal(size=(3, n))
+    wi = (wi / np.linalg.norm(wi, axis=0)).astype(np.float32)
+    u = g.random((2, n)).astype(np.float32)
+    mat = g.integers(0, len(s.material_table) + 2, n).astype(np.uint32)
+    mat[mat >= len(s.material_table)] = 0xFFFFFFFF
+    bsdf_in = tmp_path / "bsdf.bin"
+    with open(bsdf_in, "wb") as f:
+        for a in (np.array([n], np.uint32), frame, wo, wi, u, mat):
+            f.write(np.ascontiguousarray(a).tobytes())
+    out, bsdf_out = tmp_path / "film.bin", tmp_path / "bsdf_out.bin"
+    print("running", RENDER_DEMO, flush=True)
+    subprocess.run([str(RENDER_DEMO), str(scene_bin), str(out), str(bsdf_in), str(bsdf_out)], check=True,
+                   timeout=120)
+    npix = s.width * s.height
+    buf = out.read_bytes()
+    film = np.frombuffer(buf, np.float32, 4 * npix).reshape(s.height, s.width, 4)
+    rgb8 = np.frombuffer(buf, np.uint8, 3 * npix, 16 * npix).reshape(s.height, s.width, 3)
+    r = lib.Renderer(ctx, s)
+    r.render()
+    np.testing.assert_array_equal(film.view(np.uint32), r.d_film.numpy().view(np.uint32))
+    np.testing.assert_array_equal(rgb8, r.film_rgb8())
+    np.testing.assert_array_equal(lib.read_png(str(out) + ".png"), rgb8)
+    r.destroy()
+    b = bsdf_out.read_bytes()
+    o = 0
+    cv = np.frombuffer(b, np.float32, 64 * n, o).reshape(n, 64); o += 256 * n
+    cp = np.frombuffer(b, np.float32, n, o); o += 4 * n
+    sv = np.frombuffer(b, np.float32, 64 * n, o).reshape(n, 64); o += 256 * n
+    sp = np.frombuffer(b, np.float32, n, o); o += 4 * n
+    sw = np.frombuffer(b, np.float32, 3 * n, o).reshape(n, 3)
+    ev, ep = po.bsdf_eval(s, False, frame, wo, mat, wi=wi)
+    np.testing.assert_allclose(cv, ev, rtol=1e-4, atol=1e-6)
+    np.testing.assert_array_equal(cp.view(np.uint32), ep.view(np.uint32))
+    ev, ep, ew = po.bsdf_eval(s, True, frame, wo, mat, u=u)
+    np.testing.assert_allclose(sv, ev, rtol=1e-4, atol=1e-6)
+    np.testing.assert_array_equal(sp.view(np.uint32), ep.view(np.uint32))
+    np.testing.assert_array_equal(sw.T.view(np.uint32), ew.view(np.uint32))
+    none = mat == 0xFFFFFFFF
+    assert none.any() and not cv[none].any() and not sv[none].any()

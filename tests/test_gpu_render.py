@@ -88,3 +88,25 @@ def test_render_serial_passes_identical(ctx):
     finally:
         ctx.set_render_overlap(1)
     r.destroy()
+
+
+@pytest.mark.parametrize("sampler", [0, 1])
+def test_render_job_sequence_equals_render(ctx, sampler):
+    """The render's JOB_RUN sequence through wpbr_render_stage (BEGIN, then
+    camera, primary trace, scatter, shadow trace, integrate for every pass; one
+    wavefront state, one stream) leaves the film and every per-sample value of
+    wpbr_render bit for bit, on the full render's material and light mix."""
+    s = scenes.c2_scene(width=64, height=64, spp=3, freq=24, env_size=(64, 32))
+    s.sampler = sampler
+    r = lib.Renderer(ctx, s)
+    r.render(keep_samples=True)
+    film, samp = r.d_film.numpy(), r.d_samples.numpy()
+    r.render_staged(keep_samples=True)
+    np.testing.assert_array_equal(r.d_film.numpy().view(np.uint32), film.view(np.uint32))
+    np.testing.assert_array_equal(r.d_samples.numpy().view(np.uint32), samp.view(np.uint32))
+    # a pass beyond spp and an unknown stage are argument errors
+    with pytest.raises(lib.WpbrError):
+        r.stage(lib.STAGE_CAMERA, s.spp)
+    with pytest.raises(lib.WpbrError):
+        r.stage(9, 0)
+    r.destroy()

@@ -533,6 +533,79 @@ wpbr_status wpbr_render(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_s
     return WPBR_OK;
 }
 
+wpbr_status wpbr_render_stage(wpbr_ctx* ctx, wpbr_scene* sc, float* d_film, float* d_sample_xyz, uint32_t pass,
+                              int stage) {
+    if (!ctx || !sc || !d_film || stage < WPBR_STAGE_BEGIN || stage > WPBR_STAGE_INTEGRATE)
+        return WPBR_ERR_INVALID_ARG;
+    if (stage != WPBR_STAGE_BEGIN && pass >= sc->spp)
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "render stage: pass beyond the scene's spp");
+    hipSetDevice(ctx->device);
+    (void)hipGetLastError();  // launch checks below must not see stale errors
+    const uint32_t n = sc->sv.width * sc->sv.height;
+    wpbr_status s = ensure_wave(ctx, n);
+    if (!s) s = ctx_ensure_trace(ctx, 2 * n);
+    if (s) return s;
+    wpbr_render_state* r = ctx->render;
+    hipStream_t st = ctx->stream;
+    const WaveState& w = r->ws;
+    hipEventRecord(ctx->ev0, st);
+    if (stage == WPBR_STAGE_BEGIN) {
+        s = hip_check(ctx, hipMemsetAsync(d_film, 0, (size_t)n * 16, st), "film clear");
+        if (!s) s = hip_check(ctx, hipMemsetAsync(w.qcount, 0, 64, st), "queue counters");
+    } else if (stage == WPBR_STAGE_PRIMARY_TRACE || stage == WPBR_STAGE_SHADOW_TRACE) {
+        const wpbr_lbvh* b = sc->bvh;
+        TraceArgs ta;
+        ta.g = sc->sv.g;
+        ta.trav = (const float4*)b->s.trav;
+        ta.trav4 = (const float4*)b->s.trav4;
+        ta.prims = (const float4*)b->s.prims;
+        ta.root_box = b->s.node_bounds;
+        ta.root_word = b->root_word;
+        ta.spill_depth = ctx->spill_depth;
+        ta.err = ctx->err_flag;
+        const uint32_t dyn_blocks = ctx_trace_shape(ctx, ta);
+        ta.spill = ctx->spill;
+        if (stage == WPBR_STAGE_PRIMARY_TRACE) {
+            ta.n = n;
+            ta.n_dev = nullptr;
+            ta.stride = 0;
+            ta.rays = w.rays;
+            ta.hit_shape = w.hit;
+            ta.hit_t = w.hit_t;
+            ta.bary = w.bary;
+            ta.target = nullptr;
+        } else {
+            // the queue holds *qcount rays with SoA stride 2n
+            ta.n = 2 * n;
+            ta.n_dev = w.qcount;
+            ta.stride = 2 * n;
+            ta.rays = w.qrays;
+            ta.hit_shape = w.qhit;
+            ta.hit_t = w.qt;
+            ta.bary = nullptr;
+            ta.target = w.qtarget;
+        }
+        s = trace_launch(ta, ctx->trace_block, ctx->trace_stack, st, dyn_blocks);
+    } else {
+        RenderLaunch L;
+        L.sv = sc->sv;
+        L.n = n;
+        L.spp = sc->spp;
+        L.smp = sc->smp;
+        L.film = d_film;
+        L.sample_xyz = d_sample_xyz;
+        L.record = 0;
+        L.ws = w;
+        L.spp_index = pass;
+        s = render_pass_launch(L, stage == WPBR_STAGE_CAMERA ? 0 : stage == WPBR_STAGE_SCATTER ? 1 : 2, st);
+    }
+    hipEventRecord(ctx->ev1, st);
+    if (s) return set_err(ctx, s, std::string("render stage launch failed: ") + hipGetErrorString(last_launch_error));
+    s = ctx_finish_trace(ctx);
+    if (!s && hipEventElapsedTime(&ctx->last_ms, ctx->ev0, ctx->ev1) != hipSuccess) ctx->last_ms = 0.0f;
+    return s;
+}
+
 }  // extern "C"
 
 // Closest hits of n ray casts over the tree b of geometry sv.g, then (d_si) the

@@ -2,13 +2,14 @@
 //
 // The reference runs the path as fixed sequences of ECS jobs, each started by
 // JOB_RUN (construct, set the temp allocator, run, reset; MEngine.cpp:44-50):
-// MEngine::buildLBVH (MEngine.cpp:341-350) and MEngine::runCollisionSystem
-// (MEngine.cpp:352-361).  This header keeps those job names, their order and the
-// component layouts for a C++ host: every Job's run() issues the device work of
-// the reference job through the C ABI in wpbr.h, on the engine's context.
-// Jobs whose work the GPU fuses into a neighbour (the traversal intersects
-// candidates while it walks the tree) are kept as empty steps, so a caller's
-// JOB_RUN sequence stays exactly the reference's.
+// MEngine::buildLBVH (MEngine.cpp:341-350), MEngine::runCollisionSystem
+// (MEngine.cpp:352-361) and MEngine::render (MEngine.cpp:380-448).  This header
+// keeps those job names, their order and the component layouts for a C++ host:
+// every Job's run() issues the device work of the reference job through the C
+// ABI in wpbr.h, on the engine's context.  Jobs whose work the GPU fuses into a
+// neighbour (the traversal intersects candidates while it walks the tree; one
+// render stage runs a run of scattering jobs) are kept as empty steps, so a
+// caller's JOB_RUN sequence stays exactly the reference's.
 //
 // Header-only C++17; link with -lwpbr_hip.  Errors throw wpbr_host::Error
 // (the C ABI itself never throws).
@@ -88,6 +89,32 @@ struct CTriangleMesh {
     uint32_t nTriangles = 0, nVertices = 0;
 };
 
+// ---- render components (CBSDF.h, SScattering.h) ---------------------------------
+constexpr int kSpectralSamples = 64;  // the device's SampledSpectrum bins
+struct Spectrum {
+    float c[kSpectralSamples];
+};
+struct CSampledBSDFValue : Spectrum {};  // CBSDF.h:24-33
+struct CSampledBSDFPDF {                 // CBSDF.h:18-22
+    float p;
+};
+struct CSampledWI : DVector3f {};  // SScattering.h:20-28
+// CBSDF.h:36-43: the BSDF entity's world-to-shading transform; its rotation
+// rows ss (normalised shading dpdu), ts = cross(ns, ss), ns (CBSDF.cpp:12-24).
+struct CBSDFTransform {
+    DVector3f ss, ts, ns;
+};
+// What one BSDF entity's Compute / Sample job reads (CBSDFConductorMicroface.cpp:84-115
+// and its dielectric and specular siblings): its transform, wo, wi (Compute) or
+// the 2D sample (Sample), and its material (the material's type is the family;
+// INVALID_HANDLE: none).
+struct CBSDFRequest {
+    CBSDFTransform transform;
+    DVector3f wo, wi;
+    DVector2f u;
+    uint32_t material;
+};
+
 class Error : public std::runtime_error {
 public:
     Error(wpbr_status s, const std::string& what) : std::runtime_error(what), status(s) {}
@@ -117,6 +144,56 @@ struct JobCollisionFinish { void run(MEngine* e); };                       // SL
 // from the device; this step moves them into surfaceInteractions)
 struct JobComputeDifferentialsForSurfInter { void run(MEngine* e); };
 
+// ---- MEngine::render jobs (MEngine.cpp:380-448) ----------------------------------
+// The device renders every film tile at once, in five launches per sample pass
+// (wpbr_render_stage); the job that starts a stage runs it, the jobs it covers
+// are empty steps.
+#define WPBR_FUSED_JOB(JobT) \
+    struct JobT {            \
+        void run(MEngine*) {} \
+    };
+struct JobGenerateFilmTiles { void run(MEngine* e); };                     // CSFilm.cpp: all tiles, film cleared
+WPBR_FUSED_JOB(JobCreateCameraSamples)                                     // counter-based sample streams
+WPBR_FUSED_JOB(JobSamplerStratifiedGenerateSampels1D)                      // values computed where consumed
+WPBR_FUSED_JOB(JobSamplerStratifiedGenerateSampels2D)
+WPBR_FUSED_JOB(JobSamplerUpdateCameraSamples)                              // in the camera stage
+struct JobCameraPerspGenerateRaysDifferential { void run(MEngine* e); };   // CSCameraPerspective.cpp:9-52
+struct JobScatteringRequestEmittedLight { void run(MEngine* e); };         // the scatter stage, SScattering.cpp:20-276
+WPBR_FUSED_JOB(JobLightInfiniteAreaLeCompute)
+WPBR_FUSED_JOB(JobScatteringAccumulateEmittedLight)
+WPBR_FUSED_JOB(JobMapUVRequestsGenerate)
+WPBR_FUSED_JOB(JobSphereProcessMapUVRequests)
+WPBR_FUSED_JOB(JobTriangleProcessMapUVRequests)
+struct JobGenerateBSDFRequests { void run(MEngine* e); };                  // CMaterial.h:30-38
+WPBR_FUSED_JOB(JobGenerateBSDFMaterialMetal)
+WPBR_FUSED_JOB(JobGenerateBSDFMaterialDielectric)
+WPBR_FUSED_JOB(JobGenerateBSDFMaterialPerfectGlass)
+WPBR_FUSED_JOB(JobBSDFComputeTransform)
+WPBR_FUSED_JOB(JobScatteringSampleLight)
+WPBR_FUSED_JOB(JobScatteringSampleLightLI)
+WPBR_FUSED_JOB(JobSphereLightProcessSamplingRequests)
+WPBR_FUSED_JOB(JobLightInfiniteAreaLiSample)
+WPBR_FUSED_JOB(JobScatteringCastShadowRays)
+struct JobScatteringProcessShadowRay { void run(MEngine* e); };            // the integrate stage
+struct JobBSDFConductorMicrofaceCompute { void run(MEngine* e); };         // CBSDFConductorMicroface.cpp:84-99
+struct JobBSDFSpecularReflectionCompute { void run(MEngine* e); };         // CBSDFSpecularReflectance.cpp:63-76
+struct JobBSDFDielectricMicrofaceCompute { void run(MEngine* e); };        // CBSDFDielectricMicroface.cpp:84-99
+WPBR_FUSED_JOB(JobScatteringIntegrateImportanceLight)
+WPBR_FUSED_JOB(JobScatteringSampleBSDF)
+struct JobBSDFConductorMicrofaceSample { void run(MEngine* e); };          // CBSDFConductorMicroface.cpp:100-115
+struct JobBSDFSpecularReflectionSample { void run(MEngine* e); };          // CBSDFSpecularReflectance.cpp:77-93
+struct JobBSDFDielectricMicrofaceSample { void run(MEngine* e); };         // CBSDFDielectricMicroface.cpp:100-115
+WPBR_FUSED_JOB(JobScatteringCastShadowRaysWithInteraction)
+WPBR_FUSED_JOB(JobScatteringProcessShadowRayWithInteraction)
+WPBR_FUSED_JOB(JobSphereLightProcessComputeRequests)
+WPBR_FUSED_JOB(JobLightInfiniteAreaLiCompute)
+WPBR_FUSED_JOB(JobScatteringIntegrateImportanceBSDF)
+WPBR_FUSED_JOB(JobScatteringFinish)
+WPBR_FUSED_JOB(JobAccumalateLIFromSamples)
+struct JobOutputFilmTitles { void run(MEngine* e); };                      // CSFilm.cpp:141-170
+struct JobOutputFilm { void run(MEngine* e); };                            // CSFilm.cpp:183-226
+#undef WPBR_FUSED_JOB
+
 #define WPBR_JOB_RUN(JobT) \
     {                      \
         JobT j;            \
@@ -133,6 +210,8 @@ public:
         release_shapes();
         release_rays();
         release_attributes();
+        release_scene();
+        release_bsdf();
         if (bvh_) wpbr_lbvh_destroy(bvh_);
         if (ctx_) wpbr_close(ctx_);
     }
@@ -231,6 +310,109 @@ public:
         WPBR_JOB_RUN(JobCollisionFinish)
     }
 
+    // A render scene (wpbr_scene_create: shapes, per-triangle attributes,
+    // materials, sphere lights, env map, camera, film size, spp, sampler), with
+    // its own LBVH built on upload.
+    void setScene(const wpbr_scene_desc& desc) {
+        release_scene();
+        check(wpbr_scene_create(ctx_, &desc, &scene_), "wpbr_scene_create");
+        width_ = desc.width;
+        height_ = desc.height;
+        spp_ = desc.spp;
+        check(wpbr_malloc(ctx_, 16 * (size_t)width_ * height_, &d_film_), "wpbr_malloc");
+        check(wpbr_malloc(ctx_, 3 * (size_t)width_ * height_, &d_rgb8_), "wpbr_malloc");
+    }
+
+    // MEngine.cpp:380-448.  The reference's outer loop renders one 32x32 tile
+    // per iteration; here one iteration covers every tile.  Outputs: film
+    // (width*height*4 floats: the Mitchell-weighted X, Y, Z sums and the weight
+    // sum), filmRGB8 (JobOutputFilmTitles) and, when outputFilename is set, the
+    // PNG file (JobOutputFilm).
+    void render() {
+        if (!scene_) throw Error(WPBR_ERR_INVALID_ARG, "render: no scene (setScene)");
+        rendering_ = true;
+        try {
+            WPBR_JOB_RUN(JobGenerateFilmTiles)
+            WPBR_JOB_RUN(JobCreateCameraSamples)
+            WPBR_JOB_RUN(JobSamplerStratifiedGenerateSampels1D)
+            WPBR_JOB_RUN(JobSamplerStratifiedGenerateSampels2D)
+            for (pass_ = 0; pass_ < spp_; pass_++) {
+                WPBR_JOB_RUN(JobSamplerUpdateCameraSamples)
+                WPBR_JOB_RUN(JobCameraPerspGenerateRaysDifferential)
+                renderCollisionSystem(0);
+                WPBR_JOB_RUN(JobScatteringRequestEmittedLight)
+                WPBR_JOB_RUN(JobLightInfiniteAreaLeCompute)
+                WPBR_JOB_RUN(JobScatteringAccumulateEmittedLight)
+                WPBR_JOB_RUN(JobComputeDifferentialsForSurfInter)
+                WPBR_JOB_RUN(JobMapUVRequestsGenerate)
+                WPBR_JOB_RUN(JobSphereProcessMapUVRequests)
+                WPBR_JOB_RUN(JobTriangleProcessMapUVRequests)
+                WPBR_JOB_RUN(JobGenerateBSDFRequests)
+                WPBR_JOB_RUN(JobGenerateBSDFMaterialMetal)
+                WPBR_JOB_RUN(JobGenerateBSDFMaterialDielectric)
+                WPBR_JOB_RUN(JobGenerateBSDFMaterialPerfectGlass)
+                WPBR_JOB_RUN(JobBSDFComputeTransform)
+                WPBR_JOB_RUN(JobScatteringSampleLight)
+                WPBR_JOB_RUN(JobScatteringSampleLightLI)
+                WPBR_JOB_RUN(JobSphereLightProcessSamplingRequests)
+                WPBR_JOB_RUN(JobLightInfiniteAreaLiSample)
+                WPBR_JOB_RUN(JobScatteringCastShadowRays)
+                renderCollisionSystem(1);
+                WPBR_JOB_RUN(JobScatteringProcessShadowRay)
+                WPBR_JOB_RUN(JobBSDFConductorMicrofaceCompute)
+                WPBR_JOB_RUN(JobBSDFSpecularReflectionCompute)
+                WPBR_JOB_RUN(JobBSDFDielectricMicrofaceCompute)
+                WPBR_JOB_RUN(JobScatteringIntegrateImportanceLight)
+                WPBR_JOB_RUN(JobScatteringSampleBSDF)
+                WPBR_JOB_RUN(JobBSDFConductorMicrofaceSample)
+                WPBR_JOB_RUN(JobBSDFSpecularReflectionSample)
+                WPBR_JOB_RUN(JobBSDFDielectricMicrofaceSample)
+                WPBR_JOB_RUN(JobScatteringCastShadowRaysWithInteraction)
+                renderCollisionSystem(2);
+                WPBR_JOB_RUN(JobScatteringProcessShadowRayWithInteraction)
+                WPBR_JOB_RUN(JobSphereLightProcessComputeRequests)
+                WPBR_JOB_RUN(JobLightInfiniteAreaLiCompute)
+                WPBR_JOB_RUN(JobScatteringIntegrateImportanceBSDF)
+                WPBR_JOB_RUN(JobScatteringFinish)
+                WPBR_JOB_RUN(JobAccumalateLIFromSamples)
+            }
+            WPBR_JOB_RUN(JobOutputFilmTitles)
+            WPBR_JOB_RUN(JobOutputFilm)
+        } catch (...) {
+            rendering_ = false;
+            throw;
+        }
+        rendering_ = false;
+    }
+    std::vector<float> film;         // after render(): width*height*4
+    std::vector<uint8_t> filmRGB8;   // after render(): width*height*3
+    std::string outputFilename;      // JobOutputFilm's file (empty: none)
+
+    // The BSDF jobs over host components, outside the render: the requests in
+    // bsdfRequests, their queues (JobGenerateBSDFRequests), then each family's
+    // Compute (MEngine.cpp:415-417 order) or Sample (:421-424) job.  Outputs per
+    // request: sampledBSDFValues, sampledBSDFPDFs and (Sample) sampledWIs; a
+    // request without material, or a rejected microfacet sample, gets zeros.
+    void computeBSDFs() {
+        WPBR_JOB_RUN(JobGenerateBSDFRequests)
+        WPBR_JOB_RUN(JobBSDFConductorMicrofaceCompute)
+        WPBR_JOB_RUN(JobBSDFSpecularReflectionCompute)
+        WPBR_JOB_RUN(JobBSDFDielectricMicrofaceCompute)
+        fetch_bsdf(false);
+    }
+    void sampleBSDFs() {
+        WPBR_JOB_RUN(JobGenerateBSDFRequests)
+        WPBR_JOB_RUN(JobBSDFConductorMicrofaceSample)
+        WPBR_JOB_RUN(JobBSDFSpecularReflectionSample)
+        WPBR_JOB_RUN(JobBSDFDielectricMicrofaceSample)
+        fetch_bsdf(true);
+    }
+    std::vector<CBSDFRequest> bsdfRequests;
+    std::vector<CSampledBSDFValue> sampledBSDFValues;
+    std::vector<CSampledBSDFPDF> sampledBSDFPDFs;
+    std::vector<CSampledWI> sampledWIs;
+    uint32_t bsdfQueueCounts[3] = {0, 0, 0};  // conductor, dielectric, specular
+
     // The built tree as the reference's CSceneTreeLBVH.
     CSceneTreeLBVH tree() {
         uint32_t c[4];
@@ -262,6 +444,97 @@ public:
 
     // used by the jobs
     void build_stage(int stage) { check(wpbr_lbvh_build_stage(ctx_, &geom_, bvh_, stage), "wpbr_lbvh_build_stage"); }
+    bool rendering() const { return rendering_; }
+    void render_stage(int stage) {
+        check(wpbr_render_stage(ctx_, scene_, (float*)d_film_, nullptr, pass_, stage), "wpbr_render_stage");
+    }
+    // The render's three runCollisionSystem calls (MEngine.cpp:393, 412, 434): the
+    // camera rays, then both shadow-ray sets in one launch over their queue.
+    void renderCollisionSystem(int which) {
+        if (which == 0) render_stage(WPBR_STAGE_PRIMARY_TRACE);
+        else if (which == 1) render_stage(WPBR_STAGE_SHADOW_TRACE);
+    }
+    void output_film_tiles() {
+        const size_t npix = (size_t)width_ * height_;
+        check(wpbr_film_rgb8(ctx_, width_, height_, (const float*)d_film_, (uint8_t*)d_rgb8_), "wpbr_film_rgb8");
+        film.resize(4 * npix);
+        filmRGB8.resize(3 * npix);
+        check(wpbr_download(ctx_, film.data(), d_film_, 16 * npix), "download film");
+        check(wpbr_download(ctx_, filmRGB8.data(), d_rgb8_, 3 * npix), "download 8-bit film");
+    }
+    void output_film() {
+        if (!outputFilename.empty())
+            check(wpbr_write_png(outputFilename.c_str(), width_, height_, filmRGB8.data()), "wpbr_write_png");
+    }
+    void bsdf_queues() {
+        if (!scene_) throw Error(WPBR_ERR_INVALID_ARG, "BSDF jobs: no scene (setScene)");
+        const uint32_t n = (uint32_t)bsdfRequests.size();
+        release_bsdf();
+        n_bsdf_ = n;
+        if (!n) {
+            bsdfQueueCounts[0] = bsdfQueueCounts[1] = bsdfQueueCounts[2] = 0;
+            return;
+        }
+        std::vector<float> frame(9 * (size_t)n), wo(3 * (size_t)n), wi(3 * (size_t)n), u(2 * (size_t)n);
+        std::vector<uint32_t> mat(n);
+        for (uint32_t i = 0; i < n; i++) {
+            const CBSDFRequest& r = bsdfRequests[i];
+            const DVector3f rows[3] = {r.transform.ss, r.transform.ts, r.transform.ns};
+            for (int k = 0; k < 3; k++) {
+                frame[(size_t)(3 * k) * n + i] = rows[k].x;
+                frame[(size_t)(3 * k + 1) * n + i] = rows[k].y;
+                frame[(size_t)(3 * k + 2) * n + i] = rows[k].z;
+            }
+            wo[i] = r.wo.x; wo[n + i] = r.wo.y; wo[2 * (size_t)n + i] = r.wo.z;
+            wi[i] = r.wi.x; wi[n + i] = r.wi.y; wi[2 * (size_t)n + i] = r.wi.z;
+            u[i] = r.u.x; u[n + i] = r.u.y;
+            mat[i] = r.material;
+        }
+        req_.d_frame = (const float*)upload_bytes(frame.data(), frame.size() * 4, &d_bframe_);
+        req_.d_wo = (const float*)upload_bytes(wo.data(), wo.size() * 4, &d_bwo_);
+        req_.d_wi = (const float*)upload_bytes(wi.data(), wi.size() * 4, &d_bwi_);
+        req_.d_u = (const float*)upload_bytes(u.data(), u.size() * 4, &d_bu_);
+        req_.d_material = (const uint32_t*)upload_bytes(mat.data(), mat.size() * 4, &d_bmat_);
+        const std::vector<float> zeros((size_t)kSpectralSamples * n, 0.0f);
+        upload_bytes(zeros.data(), zeros.size() * 4, &d_bvalue_);
+        upload_bytes(zeros.data(), 4 * (size_t)n, &d_bpdf_);
+        upload_bytes(zeros.data(), 12 * (size_t)n, &d_bwi_out_);
+        check(wpbr_malloc(ctx_, 4 * (size_t)n, &d_bqueue_), "wpbr_malloc");
+        check(wpbr_bsdf_queues(ctx_, scene_, &req_, n, (uint32_t*)d_bqueue_, bsdfQueueCounts), "wpbr_bsdf_queues");
+    }
+    void bsdf_job(int kind, bool sample) {
+        const uint32_t q = bsdfQueueCounts[kind];
+        if (!q) return;
+        uint32_t first = 0;
+        for (int k = 0; k < kind; k++) first += bsdfQueueCounts[k];
+        const uint32_t* queue = (const uint32_t*)d_bqueue_ + first;
+        if (sample)
+            check(wpbr_bsdf_sample(ctx_, scene_, kind, &req_, n_bsdf_, queue, q, (float*)d_bwi_out_,
+                                   (float*)d_bvalue_, (float*)d_bpdf_),
+                  "wpbr_bsdf_sample");
+        else
+            check(wpbr_bsdf_compute(ctx_, scene_, kind, &req_, n_bsdf_, queue, q, (float*)d_bvalue_,
+                                    (float*)d_bpdf_),
+                  "wpbr_bsdf_compute");
+    }
+    void fetch_bsdf(bool sample) {
+        const uint32_t n = n_bsdf_;
+        sampledBSDFValues.resize(n);
+        sampledBSDFPDFs.resize(n);
+        sampledWIs.assign(sample ? n : 0, CSampledWI{});
+        if (!n) return;
+        check(wpbr_download(ctx_, sampledBSDFValues.data(), d_bvalue_, (size_t)kSpectralSamples * 4 * n), "download");
+        check(wpbr_download(ctx_, sampledBSDFPDFs.data(), d_bpdf_, 4 * (size_t)n), "download");
+        if (sample) {
+            std::vector<float> w(3 * (size_t)n);
+            check(wpbr_download(ctx_, w.data(), d_bwi_out_, w.size() * 4), "download");
+            for (uint32_t i = 0; i < n; i++) {
+                sampledWIs[i].x = w[i];
+                sampledWIs[i].y = w[n + i];
+                sampledWIs[i].z = w[2 * (size_t)n + i];
+            }
+        }
+    }
     void trace_rays() {
         const uint32_t n = (uint32_t)rayCasts.size();
         const bool diff = !rayDifferentials.empty();
@@ -395,6 +668,17 @@ private:
         for (void** p : {&d_xform_, &d_radius_, &d_tris_})
             if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
     }
+    void release_scene() {
+        for (void** p : {&d_film_, &d_rgb8_})
+            if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
+        if (scene_) wpbr_scene_destroy(scene_);
+        scene_ = nullptr;
+    }
+    void release_bsdf() {
+        for (void** p : {&d_bframe_, &d_bwo_, &d_bwi_, &d_bu_, &d_bmat_, &d_bvalue_, &d_bpdf_, &d_bwi_out_, &d_bqueue_})
+            if (*p) { wpbr_free(ctx_, *p); *p = nullptr; }
+        req_ = wpbr_bsdf_requests{};
+    }
     void ensure_rays(uint32_t n) {
         if (n <= ray_cap_) return;
         release_rays();
@@ -421,6 +705,14 @@ private:
     void *d_flags_ = nullptr, *d_uv_ = nullptr, *d_nrm_ = nullptr, *d_tan_ = nullptr;
     wpbr_tri_attributes attrs_{};
     uint32_t ray_cap_ = 0, n_traced_ = 0;
+    wpbr_scene* scene_ = nullptr;
+    uint32_t width_ = 0, height_ = 0, spp_ = 0, pass_ = 0;
+    bool rendering_ = false;
+    void *d_film_ = nullptr, *d_rgb8_ = nullptr;
+    wpbr_bsdf_requests req_{};
+    uint32_t n_bsdf_ = 0;
+    void *d_bframe_ = nullptr, *d_bwo_ = nullptr, *d_bwi_ = nullptr, *d_bu_ = nullptr, *d_bmat_ = nullptr;
+    void *d_bvalue_ = nullptr, *d_bpdf_ = nullptr, *d_bwi_out_ = nullptr, *d_bqueue_ = nullptr;
     std::vector<uint32_t> hit_;
     std::vector<float> t_, si_;
 };
@@ -439,7 +731,38 @@ inline void JobProcessRayCastsResults::run(MEngine* e) { e->fetch_hits(); }
 inline void JobProcessSphereFullInteractionRequests::run(MEngine* e) { e->fill_surface_interactions(true); }
 inline void JobProcessTriangleFullInteractionRequests::run(MEngine* e) { e->fill_surface_interactions(false); }
 inline void JobCollisionFinish::run(MEngine* e) { e->finish_interactions(); }
-inline void JobComputeDifferentialsForSurfInter::run(MEngine* e) { e->fill_differentials(); }
+inline void JobComputeDifferentialsForSurfInter::run(MEngine* e) {
+    if (!e->rendering()) e->fill_differentials();  // in a render: inside the scatter stage
+}
+inline void JobGenerateFilmTiles::run(MEngine* e) { e->render_stage(WPBR_STAGE_BEGIN); }
+inline void JobCameraPerspGenerateRaysDifferential::run(MEngine* e) { e->render_stage(WPBR_STAGE_CAMERA); }
+inline void JobScatteringRequestEmittedLight::run(MEngine* e) { e->render_stage(WPBR_STAGE_SCATTER); }
+inline void JobScatteringProcessShadowRay::run(MEngine* e) { e->render_stage(WPBR_STAGE_INTEGRATE); }
+inline void JobOutputFilmTitles::run(MEngine* e) { e->output_film_tiles(); }
+inline void JobOutputFilm::run(MEngine* e) { e->output_film(); }
+// In a render the BSDF jobs run inside the scatter (Sample) and integrate
+// (Compute) stages; outside it they run over bsdfRequests.
+inline void JobGenerateBSDFRequests::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_queues();
+}
+inline void JobBSDFConductorMicrofaceCompute::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_job(WPBR_BSDF_CONDUCTOR_MF, false);
+}
+inline void JobBSDFSpecularReflectionCompute::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_job(WPBR_BSDF_SPECULAR_REFL, false);
+}
+inline void JobBSDFDielectricMicrofaceCompute::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_job(WPBR_BSDF_DIELECTRIC_MF, false);
+}
+inline void JobBSDFConductorMicrofaceSample::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_job(WPBR_BSDF_CONDUCTOR_MF, true);
+}
+inline void JobBSDFSpecularReflectionSample::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_job(WPBR_BSDF_SPECULAR_REFL, true);
+}
+inline void JobBSDFDielectricMicrofaceSample::run(MEngine* e) {
+    if (!e->rendering()) e->bsdf_job(WPBR_BSDF_DIELECTRIC_MF, true);
+}
 
 #undef WPBR_JOB_RUN
 

@@ -33,8 +33,12 @@ EXPORTS = [
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
     "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
     "wpbr_set_render_overlap", "wpbr_film_loss_noalloc", "wpbr_scene_collide", "wpbr_collide_interactions",
-    "wpbr_bsdf_queues", "wpbr_bsdf_compute", "wpbr_bsdf_sample",
+    "wpbr_bsdf_queues", "wpbr_bsdf_compute", "wpbr_bsdf_sample", "wpbr_render_stage",
 ]
+
+# Stages of the render's job sequence (wpbr.h WPBR_STAGE_*).
+STAGE_BEGIN, STAGE_CAMERA, STAGE_PRIMARY_TRACE, STAGE_SCATTER, STAGE_SHADOW_TRACE, STAGE_INTEGRATE = range(6)
+PASS_STAGES = (STAGE_CAMERA, STAGE_PRIMARY_TRACE, STAGE_SCATTER, STAGE_SHADOW_TRACE, STAGE_INTEGRATE)
 
 # BSDF families of the Compute / Sample jobs (wpbr.h WPBR_BSDF_*).
 BSDF_CONDUCTOR_MF, BSDF_DIELECTRIC_MF, BSDF_SPECULAR_REFL, BSDF_ANY = 0, 1, 2, -1
@@ -141,6 +145,7 @@ def load():
         "wpbr_scene_destroy": (None, [vp]),
         "wpbr_scene_rebuild": (i32, [vp, vp]),
         "wpbr_render": (i32, [vp, vp, vp, vp]),
+        "wpbr_render_stage": (i32, [vp, vp, vp, vp, u32, C.c_int]),
         "wpbr_render_stats": (C.c_int, [vp, C.POINTER(C.c_double), C.c_int]),
         "wpbr_autotune_collide": (i32, [vp, C.POINTER(Geometry), vp, u32, vp, vp, vp,
                                         C.POINTER(TraceConfig)]),
@@ -566,6 +571,21 @@ class Renderer:
             self.ctx.h, self.h, self.d_film.ptr,
             self.d_samples.ptr if keep_samples else None))
         return self.ctx.last_ms()
+
+    def stage(self, stage, pass_index=0, keep_samples=False):
+        """One stage of the render's job sequence (wpbr_render_stage)."""
+        if keep_samples and self.d_samples is None:
+            self.d_samples = self.ctx.empty((self.H * self.W * self.spp, 3), np.float32)
+        self.ctx.check(self.ctx.lib.wpbr_render_stage(
+            self.ctx.h, self.h, self.d_film.ptr, self.d_samples.ptr if keep_samples else None, pass_index, stage))
+        return self.ctx.last_ms()
+
+    def render_staged(self, keep_samples=False):
+        """The render as its job sequence: BEGIN, then every pass's five stages."""
+        self.stage(STAGE_BEGIN, 0, keep_samples)
+        for j in range(self.spp):
+            for st in PASS_STAGES:
+                self.stage(st, j, keep_samples)
 
     def collide(self, d_rays, n, d_surf=None, d_diff=None, with_si=True):
         """wpbr_scene_collide: closest hits (shape, t) and, with with_si, the
