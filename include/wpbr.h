@@ -427,18 +427,22 @@ wpbr_status wpbr_scene_set_params(wpbr_ctx* ctx, wpbr_scene* scene, const float*
  * output may be null. */
 wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film,
                            const float* d_target_rgb, float* d_dL_drgb, float* d_rgb, double* loss);
-/* A square RGB texture and its MIP pyramid on the device: CTexture2DRGB as
- * SLoadTexture::load builds it (CSTextureProcess.h:267-345) from texels already
+/* An RGB texture and its MIP pyramid on the device: CTexture2DRGB as
+ * SLoadTexture::load builds it (CSTextureProcess.h:305-371) from texels already
  * linear (the reference applies gammaCorrectInv to its PNG's values first):
- * log2(res) - 1 levels, each halving the previous one (2x2 means; the last
- * column and row average the previous level's last column / row; the corner is
- * copied).  rgb: res x res x 3 floats, host, row-major; res a power of two in
- * [4, 16384] (the reference's 4x4 texel blocks then stay within every level). */
+ * floor(log2(max(width, height))) - 1 levels, each max(1, half) of the previous
+ * one per axis (2x2 means; the last column and row average the previous
+ * level's last column / row; the last texel is copied), built on the device.
+ * rgb: height x width x 3 floats, host, row-major; sides in [1, 16384], the
+ * longer at least 4. */
 typedef struct wpbr_texture wpbr_texture;
-wpbr_status wpbr_texture_create(wpbr_ctx* ctx, uint32_t res, const float* rgb, wpbr_texture** out);
+wpbr_status wpbr_texture_create(wpbr_ctx* ctx, uint32_t width, uint32_t height, const float* rgb,
+                                wpbr_texture** out);
 void wpbr_texture_destroy(wpbr_texture* tex);
 uint32_t wpbr_texture_levels(const wpbr_texture* tex);
-/* MIP level `level` to host memory ((res >> level)^2 x 3 floats). */
+/* The size of MIP level `level`. */
+wpbr_status wpbr_texture_size(const wpbr_texture* tex, uint32_t level, uint32_t* width, uint32_t* height);
+/* MIP level `level` to host memory (its height x width x 3 floats). */
 wpbr_status wpbr_texture_level(wpbr_ctx* ctx, const wpbr_texture* tex, uint32_t level, float* rgb);
 /* STexture2DSamplerAnisotropic::eval (CSTextureProcess.h:134-251) with
  * maxAnisotropy 32 (CMaterialMetal.cpp:28) and the 64-entry Gaussian table of

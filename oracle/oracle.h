@@ -126,6 +126,25 @@ struct Sampler {
 };
 Sampler make_sampler(uint32_t seed, uint32_t mode, uint32_t n1, uint32_t n2);
 
+// ---- textures (texture.cpp) ---------------------------------------------------
+struct TexRGB {
+    float v[3];
+};
+// CTexture2DRGB: the MIP levels (w[l] x h[l] texels, row-major).
+struct Texture {
+    std::vector<uint32_t> w, h;
+    std::vector<std::vector<TexRGB>> mips;
+    const TexRGB& texel(int x, int y, uint32_t l) const { return mips[l][(size_t)y * w[l] + x]; }
+    TexRGB& texel(int x, int y, uint32_t l) { return mips[l][(size_t)y * w[l] + x]; }
+};
+Texture generate_mips(uint32_t w, uint32_t h, const float* rgb);
+std::vector<float> texture_filter_table();
+// EWA on one level; q = (s, t), dx = (ds/dx, dt/dx), dy = (ds/dy, dt/dy).
+TexRGB texture_ewa(const float* q, const float* dx, const float* dy, const Texture& tex,
+                   const std::vector<float>& filter, uint32_t mip);
+// STexture2DSamplerAnisotropic::eval, maxAnisotropy 32; q = 6 floats as CTextureMappedPoint.
+TexRGB texture_eval(const float* q, const Texture& tex, const std::vector<float>& filter);
+
 struct RenderScene {
     Scene geom;
     // per-triangle shading data (optional, may be null)

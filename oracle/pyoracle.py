@@ -345,20 +345,21 @@ def permute_index(c, n, key):
 
 
 def texture_levels(rgb):
-    """MIP levels of a square RGB texture (list of r x r x 3 arrays)."""
+    """MIP levels of an RGB texture (height x width x 3): list of h_l x w_l x 3 arrays."""
     lib = load()
-    lib.orc_texture_levels.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.orc_texture_levels.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
     lib.orc_texture_levels.restype = C.c_uint32
     rgb = _f32(rgb)
-    res = rgb.shape[0]
-    n = int(lib.orc_texture_levels(res, _p(rgb), None))
-    out = np.empty(sum((res >> l) ** 2 * 3 for l in range(n)), np.float32)
-    lib.orc_texture_levels(res, _p(rgb), _p(out))
+    h, w = rgb.shape[:2]
+    sizes = np.zeros(32, np.uint32)
+    n = int(lib.orc_texture_levels(w, h, _p(rgb), _p(sizes), None))
+    out = np.empty(sum(int(sizes[2 * l]) * int(sizes[2 * l + 1]) * 3 for l in range(n)), np.float32)
+    lib.orc_texture_levels(w, h, _p(rgb), None, _p(out))
     levels, o = [], 0
     for l in range(n):
-        r = res >> l
-        levels.append(out[o:o + r * r * 3].reshape(r, r, 3))
-        o += r * r * 3
+        lw, lh = int(sizes[2 * l]), int(sizes[2 * l + 1])
+        levels.append(out[o:o + lw * lh * 3].reshape(lh, lw, 3))
+        o += lw * lh * 3
     return levels
 
 
@@ -373,22 +374,23 @@ def texture_filter():
 def texture_lookup(rgb, queries):
     """Anisotropic (EWA) lookups: queries n x 6 (s, t, ds/dx, dt/dx, ds/dy, dt/dy)."""
     lib = load()
-    lib.orc_texture_lookup.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.orc_texture_lookup.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
     rgb = _f32(rgb)
     q = _f32(queries).reshape(-1, 6)
     out = np.empty((len(q), 3), np.float32)
-    lib.orc_texture_lookup(rgb.shape[0], _p(rgb), len(q), _p(q), _p(out))
+    lib.orc_texture_lookup(rgb.shape[1], rgb.shape[0], _p(rgb), len(q), _p(q), _p(out))
     return out
 
 
 def texture_ewa(rgb, level, queries):
     """The EWA filter of one MIP level on its own (queries n x 6 as texture_lookup)."""
     lib = load()
-    lib.orc_texture_ewa.argtypes = [C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]
+    lib.orc_texture_ewa.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                    C.c_void_p]
     rgb = _f32(rgb)
     q = _f32(queries).reshape(-1, 6)
     out = np.empty((len(q), 3), np.float32)
-    lib.orc_texture_ewa(rgb.shape[0], _p(rgb), level, len(q), _p(q), _p(out))
+    lib.orc_texture_ewa(rgb.shape[1], rgb.shape[0], _p(rgb), level, len(q), _p(q), _p(out))
     return out
 
 

@@ -49,6 +49,52 @@ def test_mip_levels(res):
         assert cur[r - 1, r - 1] == prev[-1, -1]
 
 
+@pytest.mark.parametrize("wh", [(64, 16), (17, 100), (256, 4), (5, 3), (100, 1), (3, 40)])
+def test_mip_levels_rectangular(wh):
+    """generateMIPS on W x H (CSTextureProcess.h:349-371): (uint16_t)(log2(max
+    side) - 1) levels, each max(1, half) per axis; inside, 2x2 means of a
+    linear ramp; the last column / row from the previous level's last column /
+    row; the last texel copied -- also where a level is one texel wide."""
+    w, h = wh
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    rgb = np.stack([x + 64 * y, 2 * (x + 64 * y), 3 * (x + 64 * y)], -1).astype(np.float32)
+    lv = po.texture_levels(rgb)
+    assert len(lv) == int(math.log2(max(w, h))) - 1
+    pw, ph = w, h
+    np.testing.assert_array_equal(lv[0], rgb)
+    for l in range(1, len(lv)):
+        rw, rh = max(1, pw // 2), max(1, ph // 2)
+        assert lv[l].shape == (rh, rw, 3)
+        prev, cur = lv[l - 1][..., 0].astype(np.float64), lv[l][..., 0]
+        yy, xx = np.mgrid[0:rh - 1, 0:rw - 1]
+        np.testing.assert_array_equal(cur[:rh - 1, :rw - 1],
+                                      (prev[2 * yy, 2 * xx] + prev[2 * yy, 2 * xx + 1] + prev[2 * yy + 1, 2 * xx]
+                                       + prev[2 * yy + 1, 2 * xx + 1]) / 4)
+        ys, xs = np.arange(rh - 1), np.arange(rw - 1)
+        np.testing.assert_array_equal(cur[ys, rw - 1], (prev[2 * ys, pw - 1] + prev[2 * ys + 1, pw - 1]) / 2)
+        np.testing.assert_array_equal(cur[rh - 1, xs], (prev[ph - 1, 2 * xs] + prev[ph - 1, 2 * xs + 1]) / 2)
+        assert cur[rh - 1, rw - 1] == prev[ph - 1, pw - 1]
+        pw, ph = rw, rh
+
+
+def test_rectangular_lookup_scales_per_axis():
+    """On a W x H texture the EWA scales s by W and t by H: a texel centre
+    ((x + 0.5) / W, (y + 0.5) / H) with a zero footprint reads texel (x, y)
+    exactly, and a small isotropic footprint on a ramp along s alone averages
+    to the ramp's value there."""
+    w, h = 64, 16
+    rng = np.random.default_rng(4)
+    tex = rng.random((h, w, 3), dtype=np.float32)
+    xs, ys = rng.integers(0, w, 40), rng.integers(0, h, 40)
+    q = np.zeros((40, 6), np.float32)
+    q[:, 0] = (xs + 0.5) / w
+    q[:, 1] = (ys + 0.5) / h
+    np.testing.assert_array_equal(po.texture_lookup(tex, q), tex[ys, xs])
+    ramp_s = np.repeat(np.arange(w, dtype=np.float32)[None, :, None], h, 0).repeat(3, 2)
+    q = np.array([[0.5, 0.5, 1e-3, 0, 0, 1e-3]], np.float32)
+    np.testing.assert_allclose(po.texture_lookup(ramp_s, q)[0], 0.5 * w - 0.5, rtol=1e-6)
+
+
 def test_zero_footprint_reads_texels():
     res = 16
     rng = np.random.default_rng(3)

@@ -9,18 +9,19 @@ constexpr int TEX_MAX_LEVELS = 16;
 constexpr int TEX_FILTER_SIZE = 64;         // CFilterTableGaussing::size (MEngine.cpp:92-94)
 constexpr float TEX_MAX_ANISOTROPY = 32.0f; // CMaterialMetal.cpp:28
 
-// A square RGB texture and its MIP levels: level l has res >> l texels a side,
-// row-major float4 texels (w unused) from texels + off[l].
+// An RGB texture and its MIP levels: level l is lw[l] x lh[l] texels (level 0
+// the image, each next level max(1, half) per axis), row-major float4 texels (w
+// unused) from texels + off[l].
 struct TexView {
     const float4* texels;
     const float* filter;  // TEX_FILTER_SIZE Gaussian weights
-    uint32_t res;
     uint32_t n_levels;
     uint32_t off[TEX_MAX_LEVELS];
+    uint32_t lw[TEX_MAX_LEVELS], lh[TEX_MAX_LEVELS];
 };
 
 WPBR_D float3 tex_texel(const TexView& T, int x, int y, uint32_t l) {
-    const float4 v = T.texels[T.off[l] + (size_t)y * (T.res >> l) + x];
+    const float4 v = T.texels[T.off[l] + (size_t)y * T.lw[l] + x];
     return make_float3(v.x, v.y, v.z);
 }
 
@@ -31,16 +32,16 @@ WPBR_D float3 f3add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.
 // indices clamped below as the reference does and also above (st beyond 1 would
 // read past the level).
 WPBR_D float3 tex_bilinear(const TexView& T, float s_, float t_) {
-    const int r = (int)T.res;
-    const float s = s_ * (float)r - 0.5f, t = t_ * (float)r - 0.5f;
+    const int rw = (int)T.lw[0], rh = (int)T.lh[0];
+    const float s = s_ * (float)rw - 0.5f, t = t_ * (float)rh - 0.5f;
     int s0 = (int)floorf(s), t0 = (int)floorf(t);
     s0 = s0 < 0 ? 0 : s0;
     t0 = t0 < 0 ? 0 : t0;
     const float ds = s - (float)s0, dt = t - (float)t0;
-    int s0n = s0 + 1 < r - 1 ? s0 + 1 : r - 1;
-    int t0n = t0 + 1 < r - 1 ? t0 + 1 : r - 1;
-    s0 = s0 < r - 1 ? s0 : r - 1;
-    t0 = t0 < r - 1 ? t0 : r - 1;
+    int s0n = s0 + 1 < rw - 1 ? s0 + 1 : rw - 1;
+    int t0n = t0 + 1 < rh - 1 ? t0 + 1 : rh - 1;
+    s0 = s0 < rw - 1 ? s0 : rw - 1;
+    t0 = t0 < rh - 1 ? t0 : rh - 1;
     float3 v = f3s(f3s(tex_texel(T, s0, t0, 0), 1 - ds), 1 - dt);
     v = f3add(v, f3s(f3s(tex_texel(T, s0, t0n, 0), 1 - ds), dt));
     v = f3add(v, f3s(f3s(tex_texel(T, s0n, t0, 0), ds), 1 - dt));
@@ -51,9 +52,10 @@ WPBR_D float3 tex_bilinear(const TexView& T, float s_, float t_) {
 // STexture2DSamplerAnisotropic::ewa (CSTextureProcess.h:182-251) on level l.
 WPBR_D float3 tex_ewa(const TexView& T, float s_, float t_, float dsx, float dtx, float dsy, float dty,
                       uint32_t l) {
-    const float r = (float)(T.res >> l);
-    const float s = s_ * r - 0.5f, t = t_ * r - 0.5f;
-    dsx *= r; dtx *= r; dsy *= r; dty *= r;
+    // st and both derivatives scaled by the level's size per axis
+    const float rw = (float)T.lw[l], rh = (float)T.lh[l];
+    const float s = s_ * rw - 0.5f, t = t_ * rh - 0.5f;
+    dsx *= rw; dtx *= rh; dsy *= rw; dty *= rh;
     float A = (dtx * dtx + dty * dty) + 1;
     float B = -2.0f * (dsx * dtx + dsy * dty);
     float C = (dsx * dsx + dsy * dsy) + 1;
@@ -62,9 +64,9 @@ WPBR_D float3 tex_ewa(const TexView& T, float s_, float t_, float dsx, float dtx
     const float det = (-B) * B + (4.0f * A) * C;
     const float invDet = 1 / det;
     const float us = sqrtf(C * det) * (2.0f * invDet), vs = sqrtf(A * det) * (2.0f * invDet);
-    const float hi = r - 1.0f;
-    const float st0s = fminf(fmaxf(s - us, 0.0f), hi), st0t = fminf(fmaxf(t - vs, 0.0f), hi);
-    const float st1s = fminf(fmaxf(s + us, 0.0f), hi), st1t = fminf(fmaxf(t + vs, 0.0f), hi);
+    const float hiw = rw - 1.0f, hih = rh - 1.0f;
+    const float st0s = fminf(fmaxf(s - us, 0.0f), hiw), st0t = fminf(fmaxf(t - vs, 0.0f), hih);
+    const float st1s = fminf(fmaxf(s + us, 0.0f), hiw), st1t = fminf(fmaxf(t + vs, 0.0f), hih);
     const int s0 = (int)ceilf(st0s), t0 = (int)ceilf(st0t);
     const int s1 = (int)floorf(st1s), t1 = (int)floorf(st1t);
     float3 sum = make_float3(0.f, 0.f, 0.f);

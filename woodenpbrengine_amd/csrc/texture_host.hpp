@@ -13,10 +13,10 @@ struct DevTexture {
     float* d_filter = nullptr;
 };
 
-bool tex_res_ok(uint32_t res);
-void tex_build_mips(uint32_t res, const float* rgb, std::vector<float4>& texels, uint32_t* off, uint32_t& n_levels);
+bool tex_size_ok(uint32_t w, uint32_t h);
+size_t tex_layout(uint32_t w, uint32_t h, TexView& v);
 void tex_filter_table(float* w);
-wpbr_status tex_upload(wpbr_ctx* ctx, uint32_t res, const float* rgb, DevTexture& t);
+wpbr_status tex_upload(wpbr_ctx* ctx, uint32_t w, uint32_t h, const float* rgb, DevTexture& t);
 void tex_free(DevTexture& t);
 
 }  // namespace wpbr

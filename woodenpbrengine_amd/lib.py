@@ -29,7 +29,7 @@ EXPORTS = [
     "wpbr_scenes_create", "wpbr_batch_create", "wpbr_batch_destroy", "wpbr_batch_render",
     "wpbr_batch_stats", "wpbr_autotune_batch", "wpbr_film_rgb8", "wpbr_write_png",
     "wpbr_sample_arrays", "wpbr_texture_create", "wpbr_texture_destroy", "wpbr_texture_levels",
-    "wpbr_texture_level", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
+    "wpbr_texture_level", "wpbr_texture_size", "wpbr_texture_lookup", "wpbr_obj_load", "wpbr_obj_counts", "wpbr_obj_copy",
     "wpbr_obj_free", "wpbr_read_png", "wpbr_mesh_bake", "wpbr_srgb8_to_linear",
     "wpbr_lbvh_set_order", "wpbr_lbvh_get_order", "wpbr_set_default_lbvh_order",
     "wpbr_set_render_overlap", "wpbr_film_loss_noalloc", "wpbr_scene_collide", "wpbr_collide_interactions",
@@ -166,7 +166,8 @@ def load():
         "wpbr_film_rgb8": (i32, [vp, u32, u32, vp, vp]),
         "wpbr_write_png": (i32, [C.c_char_p, u32, u32, vp]),
         "wpbr_sample_arrays": (i32, [vp, u32, u32, u32, u32, u32, u32, vp, vp]),
-        "wpbr_texture_create": (i32, [vp, u32, vp, C.POINTER(vp)]),
+        "wpbr_texture_create": (i32, [vp, u32, u32, vp, C.POINTER(vp)]),
+        "wpbr_texture_size": (i32, [vp, u32, C.POINTER(u32), C.POINTER(u32)]),
         "wpbr_texture_destroy": (None, [vp]),
         "wpbr_texture_levels": (u32, [vp]),
         "wpbr_texture_level": (i32, [vp, vp, u32, vp]),
@@ -408,17 +409,23 @@ class Texture:
 
     def __init__(self, ctx, rgb):
         rgb = np.ascontiguousarray(rgb, dtype=np.float32)
-        if rgb.ndim != 3 or rgb.shape[0] != rgb.shape[1] or rgb.shape[2] != 3:
-            raise WpbrError(1, "texture must be res x res x 3")
-        self.ctx, self.res = ctx, rgb.shape[0]
+        if rgb.ndim != 3 or rgb.shape[2] != 3:
+            raise WpbrError(1, "texture must be height x width x 3")
+        self.ctx = ctx
+        self.height, self.width = rgb.shape[:2]
         h = C.c_void_p()
-        ctx.check(ctx.lib.wpbr_texture_create(ctx.h, self.res, _ptr(rgb), C.byref(h)))
+        ctx.check(ctx.lib.wpbr_texture_create(ctx.h, self.width, self.height, _ptr(rgb), C.byref(h)))
         self.h = h
         self.n_levels = int(ctx.lib.wpbr_texture_levels(h))
 
+    def size(self, l):
+        w, h = C.c_uint32(), C.c_uint32()
+        self.ctx.check(self.ctx.lib.wpbr_texture_size(self.h, l, C.byref(w), C.byref(h)))
+        return w.value, h.value
+
     def level(self, l):
-        r = self.res >> l
-        out = np.empty((r, r, 3), np.float32)
+        w, h = self.size(l)
+        out = np.empty((h, w, 3), np.float32)
         self.ctx.check(self.ctx.lib.wpbr_texture_level(self.ctx.h, self.h, l, _ptr(out)))
         return out
 
