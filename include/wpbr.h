@@ -178,6 +178,8 @@ wpbr_status wpbr_autotune_collide(wpbr_ctx* ctx, const wpbr_geometry* geom, cons
                                   uint32_t n, const float* d_rays, uint32_t* d_hit_shape,
                                   float* d_hit_t, wpbr_trace_config* best);
 
+typedef struct wpbr_texture wpbr_texture;  /* an RGB texture and its MIP pyramid (below) */
+
 /* ---- rendering ---------------------------------------------------------- *
  * Scene description: host pointers, copied to the device by wpbr_scene_create.
  * Shape order as in wpbr_geometry.  Materials are 9 floats each: type (u32
@@ -215,6 +217,21 @@ typedef struct {
      * arrays of JobCreateCameraSamples (CSFilm.cpp:22-23, 57-64): 16 + spp 1D
      * and 16 + 4 spp 2D values, consumed in order over the spp passes. */
     uint32_t sampler;
+    /* Textured metals (CMaterialMetal.cpp:24-38).  material_textures holds two
+     * indices into textures per material (n_materials x 2, -1: none): [0] the
+     * metallic texture, whose value gives the conductor's k (fromRGB,
+     * Reflectance) while its eta is the constant spectrum of the material's
+     * eta (the reference's Spectrum(1.5f) for 1.5); [1] the roughness texture,
+     * whose red value r gives alpha = roughnessToAlpha(clamp(90 r, 0, 90))
+     * (MicrofaceDistr.h:18-24).  Both are looked up (EWA, 32:1) at the camera
+     * ray's hit with the camera's ray differentials (CSCameraPerspective.cpp:
+     * 37-41), UV mapping on triangles and the sphere mapping on spheres
+     * (CTextureMapping.h:39-90).  Only conductors bind textures.  The scene keeps
+     * the texture pointers: they must outlive it.  n_textures 0 (null arrays):
+     * no textures. */
+    uint32_t n_textures;
+    wpbr_texture* const* textures;
+    const int32_t* material_textures;
 } wpbr_scene_desc;
 
 /* Sampler modes.  INDEPENDENT: a counter-based hash stream per (seed, pixel,
@@ -435,7 +452,6 @@ wpbr_status wpbr_film_loss(wpbr_ctx* ctx, wpbr_scene* scene, const float* d_film
  * level's last column / row; the last texel is copied), built on the device.
  * rgb: height x width x 3 floats, host, row-major; sides in [1, 16384], the
  * longer at least 4. */
-typedef struct wpbr_texture wpbr_texture;
 wpbr_status wpbr_texture_create(wpbr_ctx* ctx, uint32_t width, uint32_t height, const float* rgb,
                                 wpbr_texture** out);
 void wpbr_texture_destroy(wpbr_texture* tex);

@@ -201,6 +201,12 @@ struct orc_scene_desc {
     const float* camera_world;
     uint32_t width, height, spp, seed;
     uint32_t sampler;
+    // textured metals: textures (w, h, texels h x w x 3 each) and per material
+    // [metallic, roughness] texture indices (-1 none)
+    uint32_t n_textures;
+    const uint32_t* tex_size;        // n_textures x 2: w, h
+    const float* const* tex_rgb;     // n_textures pointers
+    const int32_t* material_textures;  // n_materials x 2
 };
 
 extern "C" {
@@ -253,6 +259,9 @@ static SceneHold* make_scene(const orc_scene_desc* d) {
     sc.spp = d->spp;
     sc.seed = d->seed;
     sc.smp = make_sampler(d->seed, d->sampler, 16u + d->spp, 16u + 4u * d->spp);
+    for (uint32_t t = 0; t < d->n_textures; t++)
+        sc.textures.push_back(generate_mips(d->tex_size[2 * t], d->tex_size[2 * t + 1], d->tex_rgb[t]));
+    sc.material_textures = d->n_textures ? d->material_textures : nullptr;
     uint32_t n = d->n_spheres + d->n_triangles;
     std::vector<float> b(6 * (size_t)n), c(3 * (size_t)n);
     sphere_bounds(d->n_spheres, d->sphere_xform, d->sphere_radius, b.data(), c.data());
@@ -352,6 +361,12 @@ void orc_film_rgb8(uint32_t npix, const float* xyz, const float* w, uint8_t* out
 }
 
 void orc_shade_probe(int fn, const float* in, float* out) { shade_probe(fn, in, out); }
+
+void orc_texture_points(const orc_scene_desc* d, uint32_t n, const float* pfilm, uint32_t* shape, float* q) {
+    SceneHold* H = make_scene(d);
+    texture_points(H->sc, H->bvh, n, pfilm, shape, q);
+    delete H;
+}
 
 void orc_samples(uint32_t seed, uint32_t pixel, uint32_t n1, uint32_t n2, float* u1, float* u2) {
     const Sampler S = make_sampler(seed, 0, 1, 1);

@@ -223,7 +223,75 @@ void put3(float* out, size_t n, size_t i, int f, V3 v) {
     out[(size_t)(f + 2) * n + i] = v.z;
 }
 
+// The raster->camera transform of a point (the render's camera, render.cpp).
+V3 xf_point4_cam(const float* m, V3 p) {
+    float x = ((m[0] * p.x + m[1] * p.y) + m[2] * p.z) + m[3];
+    float y = ((m[4] * p.x + m[5] * p.y) + m[6] * p.z) + m[7];
+    float z = ((m[8] * p.x + m[9] * p.y) + m[10] * p.z) + m[11];
+    float w = ((m[12] * p.x + m[13] * p.y) + m[14] * p.z) + m[15];
+    if (w == 1.0f) return v3(x, y, z);
+    return v3(x / w, y / w, z / w);
+}
+
+// pbrt-v3 SphericalTheta / SphericalPhi (phi wrapped by WoodenMath's own pi).
+float sph_theta(V3 v) { return cr_acos(v.z < -1.0f ? -1.0f : (v.z > 1.0f ? 1.0f : v.z)); }
+float sph_phi(V3 v) {
+    const float p = cr_atan2(v.y, v.x);
+    return (p < 0) ? (float)((double)p + 2.0 * PI_TRUE) : p;
+}
+
+// TextureMappingSphere::sphere (CTextureMapping.h:60-66).  SSphere::mapUV
+// divides the inverse transform's scale by the radius first (CSSphere.cpp:
+// 188-196); the point is normalised after it, so that scale drops out.
+void sphere_st(const Affine& inv, V3 p, float* st) {
+    const V3 v = normalize3(xf_point(inv, p));
+    st[0] = (float)((double)sph_theta(v) * 1.0 / PI_REF);
+    st[1] = (float)((double)sph_phi(v) / (2 * PI_REF));
+}
+
 }  // namespace
+
+// The CTextureMappedPoint of a camera ray's hit (what JobMapUVRequestsGenerate
+// and the shapes' ProcessMapUVRequests jobs give a textured material,
+// MEngine.cpp:397-402): the full interaction, its differentials from the
+// camera's ray differential (CSCameraPerspective.cpp:37-41, dxCamera /
+// dyCamera = the raster->camera transform of the vectors (1, 0, 0) / (0, 1, 0)
+// minus that of the zero vector, CSCameraPerspective.h:33-34: the matrix's first
+// two columns), then UV mapping (triangles, CTextureMapping.h:39-55) or the
+// sphere mapping with its finite differences (spheres, :68-90).
+void camera_hit_texture_point(const RenderScene& sc, uint32_t shape, V3 o, V3 d, float t, const float* b,
+                              const float* pFilm, float* q) {
+    const bool sphere = shape < sc.geom.n_spheres;
+    FullSI s = sphere ? si_sphere_full(sc, shape, o, d, t) : si_triangle_full(sc, shape - sc.geom.n_spheres, d, b, t);
+    const float* rc = sc.raster_camera;
+    const V3 pCam = xf_point4_cam(rc, v3(pFilm[0], pFilm[1], 0.0f));
+    const V3 dxC = v3(rc[0], rc[4], rc[8]), dyC = v3(rc[1], rc[5], rc[9]);
+    const Affine CW = load_aff(sc.camera_world);
+    const V3 dx = normalize3(xf_vector(CW, normalize3(pCam + dxC)));
+    const V3 dy = normalize3(xf_vector(CW, normalize3(pCam + dyC)));
+    differentials(s, o, dx, o, dy);
+    if (!sphere) {
+        q[0] = s.uv[0]; q[1] = s.uv[1];
+        q[2] = s.dudx; q[3] = s.dvdx;
+        q[4] = s.dudy; q[5] = s.dvdy;
+        return;
+    }
+    const Affine inv = load_aff(sc.geom.sphere_xform + 24 * (size_t)shape + 12);
+    float st[2], sx[2], sy[2];
+    sphere_st(inv, s.p, st);
+    const float dt = 0.1f;
+    sphere_st(inv, s.p + s.dpdx * dt, sx);
+    sphere_st(inv, s.p + s.dpdy * dt, sy);
+    float dstdx[2] = {(sx[0] - st[0]) / dt, (sx[1] - st[1]) / dt};
+    float dstdy[2] = {(sy[0] - st[0]) / dt, (sy[1] - st[1]) / dt};
+    if (dstdx[1] > .5) dstdx[1] = 1 - dstdx[1];
+    else if (dstdx[1] < -.5f) dstdx[1] = -dstdx[1] + 1;
+    if (dstdy[1] > .5) dstdy[1] = 1 - dstdy[1];
+    else if (dstdy[1] < -.5f) dstdy[1] = -dstdy[1] + 1;
+    q[0] = st[0]; q[1] = st[1];
+    q[2] = dstdx[0]; q[3] = dstdx[1];
+    q[4] = dstdy[0]; q[5] = dstdy[1];
+}
 
 void scene_collide(const RenderScene& sc, const LbvhResult& bvh, uint32_t n, const float* rays,
                    const uint8_t* surf, const float* diff, uint32_t* hit_shape, float* hit_t, float* si,

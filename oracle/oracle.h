@@ -170,7 +170,18 @@ struct RenderScene {
     uint32_t width = 0, height = 0, spp = 0;
     uint32_t seed = 0;
     Sampler smp;
+    // textured metals (CMaterialMetal.cpp:24-38): per material [metallic,
+    // roughness] indices into textures (-1 none); null: no textures
+    std::vector<Texture> textures;
+    const int32_t* material_textures = nullptr;
+    int32_t tex_index(uint32_t mid, int k) const { return material_textures ? material_textures[2 * mid + k] : -1; }
 };
+
+// The CTextureMappedPoint (s, t, ds/dx, dt/dx, ds/dy, dt/dy) of a camera ray's
+// hit on `shape` at t (b: its barycentrics on a triangle), from the camera's ray
+// differential at film position pFilm (interaction.cpp).
+void camera_hit_texture_point(const RenderScene& sc, uint32_t shape, V3 o, V3 d, float t, const float* b,
+                              const float* pFilm, float* q);
 
 struct RenderOut {
     std::vector<float> sample_xyz;   // width*height*spp x 3, index (pixel*spp + s)
@@ -200,6 +211,10 @@ enum {
 void scene_collide(const RenderScene& sc, const LbvhResult& bvh, uint32_t n, const float* rays,
                    const uint8_t* surf, const float* diff, uint32_t* hit_shape, float* hit_t, float* si,
                    int nthreads);
+
+// Texture-mapped points of camera rays through film positions (render.cpp).
+void texture_points(const RenderScene& sc, const LbvhResult& bvh, uint32_t n, const float* pfilm, uint32_t* shape,
+                    float* q);
 
 // The BSDF Compute (sample = 0) / Sample (sample = 1) jobs for n requests (render.cpp).
 void bsdf_eval(const RenderScene& sc, int sample, uint32_t n, const float* frame, const float* wo,

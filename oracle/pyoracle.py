@@ -198,7 +198,9 @@ class SceneDesc(C.Structure):
                 ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env_xform", C.c_void_p),
                 ("raster_camera", C.c_void_p), ("camera_world", C.c_void_p),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("seed", C.c_uint32), ("sampler", C.c_uint32)]
+                ("seed", C.c_uint32), ("sampler", C.c_uint32),
+                ("n_textures", C.c_uint32), ("tex_size", C.c_void_p), ("tex_rgb", C.c_void_p),
+                ("material_textures", C.c_void_p)]
 
 
 def scene_desc(s):
@@ -209,11 +211,16 @@ def scene_desc(s):
         s.tri_tangent, s.shape_material, s.material_table, s.light_shape, s.light_emit,
         s.env_emit, s.env_rgb, s.env_xform, s.raster_camera, s.camera_world)]
     (sx, sr, tv, tf, tuv, tn, tt, sm, mt, ls, le, ee, er, ex, rc, cw) = keep
+    texs = [np.ascontiguousarray(t, np.float32) for t in getattr(s, "textures", [])]
+    sizes = np.array([[t.shape[1], t.shape[0]] for t in texs], np.uint32).reshape(-1, 2)
+    ptrs = (C.c_void_p * max(1, len(texs)))(*[_p(t) for t in texs])
+    mtex = np.ascontiguousarray(getattr(s, "material_textures", np.full((len(mt), 2), -1)), np.int32)
+    keep += texs + [sizes, ptrs, mtex]
     d = SceneDesc(len(sr), _p(sx), _p(sr), len(tv), _p(tv), _p(tf), _p(tuv), _p(tn), _p(tt),
                   _p(sm), len(mt), _p(mt), len(ls), _p(ls), _p(le),
                   1 if s.env is not None else 0, _p(ee), _p(er), er.shape[1], er.shape[0],
                   _p(ex), _p(rc), _p(cw), s.width, s.height, s.spp, s.seed,
-                  getattr(s, "sampler", 0))
+                  getattr(s, "sampler", 0), len(texs), _p(sizes), C.cast(ptrs, C.c_void_p), _p(mtex))
     return d, keep
 
 
@@ -405,7 +412,23 @@ def samples(seed, pixel, n1, n2):
 
 PROBE = {"fresnel_dielectric": (0, 1), "fresnel_conductor": (1, 1), "ggx_d": (2, 1),
          "ggx_sample": (3, 4), "mitchell": (4, 1), "cone_pdf": (5, 1), "coordinate_system": (6, 6),
-         "spherical": (7, 2), "dist1d": (8, 4)}
+         "spherical": (7, 2), "dist1d": (8, 4), "roughness_to_alpha": (9, 1)}
+
+
+def texture_points(s, pfilm):
+    """Camera rays through film positions pfilm (2 x n): hit shape and the
+    texture-mapped point (6 x n: s, t, ds/dx, dt/dx, ds/dy, dt/dy) a textured
+    material reads there."""
+    lib = load()
+    lib.orc_texture_points.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]
+    d, keep = scene_desc(s)
+    pf = _f32(pfilm, (2, -1))
+    n = pf.shape[1]
+    shape = np.empty(n, np.uint32)
+    q = np.empty((6, n), np.float32)
+    lib.orc_texture_points(C.byref(d), n, _p(pf), _p(shape), _p(q))
+    del keep
+    return shape, q
 
 
 def probe(name, *args):

@@ -345,6 +345,25 @@ struct MatSpec {
     Spec eta, k;  // conductor
 };
 
+// A conductor's spectra: fromRGB(Reflectance) of its eta and k RGB; a textured
+// one (CMaterialMetal.cpp:24-52) takes the constant spectrum of its eta (the
+// reference's Spectrum(1.5f)).
+static MatSpec material_spectra(const RenderScene& sc, uint32_t m) {
+    MatSpec s;
+    const Material& mt = sc.materials[m];
+    if (sc.tex_index(m, 0) >= 0 || sc.tex_index(m, 1) >= 0) s.eta.fill(mt.eta);
+    else s.eta = from_rgb(mt.eta_rgb, false);
+    s.k = from_rgb(mt.k_rgb, false);
+    return s;
+}
+
+// MicrofaceDistr.h:18-24.
+static float roughness_to_alpha(float roughness) {
+    roughness = std::max(roughness, 1e-3f);
+    const float x = cr_log(roughness);
+    return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+
 // BSDF value (already multiplied by |cos theta_i|) and pdf for a direction pair
 // in the local frame: the Compute jobs (CBSDFConductorMicroface.cpp:84-97,
 // CBSDFDielectricMicroface.cpp:84-97).  Returns false when f is exactly 0.
@@ -503,6 +522,7 @@ struct Ctx {
     std::vector<MatSpec> mats;
     Spec env_emit;
     std::vector<Spec> light_emit;
+    std::vector<float> tex_filter;
 };
 
 static bool trace(const Ctx& C, V3 o, V3 d, uint32_t* shape, float* t) {
@@ -583,11 +603,11 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     if (rec) rec->kind = 1;
     counters[0]++;
     SI si;
+    float b[3] = {0.0f, 0.0f, 0.0f};
     if (shape < sc.geom.n_spheres) {
         si = si_sphere(sc, shape, o, d, t);
     } else {
         uint32_t tri = shape - sc.geom.n_spheres;
-        float b[3];
         tri_intersect(sc.geom.tri_verts + 9 * (size_t)tri, o, d, b);
         si = si_triangle(sc, tri, d, b);
     }
@@ -613,9 +633,24 @@ static Spec run_sample(const Ctx& C, uint32_t px, uint32_t py, uint32_t& c1, uin
     uint32_t mid = sc.shape_material ? sc.shape_material[shape] : 0xFFFFFFFFu;
     if (mid == 0xFFFFFFFFu) return L;
     if (rec) { rec->kind = 2; rec->mid = mid; rec->li = li; rec->env_pick = envPick; rec->pdfL = pdfL; }
-    const Material& m = sc.materials[mid];
-    const MatSpec& ms = C.mats[mid];
+    Material m = sc.materials[mid];
+    MatSpec ms = C.mats[mid];
     if (m.type == MAT_SPECULAR) counters[4]++;
+    const int32_t texK = sc.tex_index(mid, 0), texR = sc.tex_index(mid, 1);
+    if (texK >= 0 || texR >= 0) {
+        // a textured metal (JobGenerateBSDFMaterialMetal, CMaterialMetal.cpp:24-52): both
+        // textures at the hit's mapped point, EWA with maxAnisotropy 32
+        float q[6];
+        camera_hit_texture_point(sc, shape, o, d, t, b, pFilm, q);
+        if (texR >= 0) {
+            const TexRGB r = texture_eval(q, sc.textures[texR], C.tex_filter);
+            m.alpha = roughness_to_alpha(clampf(r.v[0] * 90, 0.0f, 90.0f));
+        }
+        if (texK >= 0) {
+            const TexRGB k = texture_eval(q, sc.textures[texK], C.tex_filter);
+            ms.k = from_rgb(k.v, false);
+        }
+    }
     // BSDF frame (CBSDF.cpp:12-24)
     V3 ss = normalize3(si.dpdu);
     V3 ns = si.ns;
@@ -749,10 +784,7 @@ void bsdf_eval(const RenderScene& sc, int sample, uint32_t n, const float* frame
                const float* wi, const float* u, const uint32_t* material, float* value, float* pdf,
                float* wi_out) {
     std::vector<MatSpec> mats(sc.n_materials);
-    for (uint32_t m = 0; m < sc.n_materials; m++) {
-        mats[m].eta = from_rgb(sc.materials[m].eta_rgb, false);
-        mats[m].k = from_rgb(sc.materials[m].k_rgb, false);
-    }
+    for (uint32_t m = 0; m < sc.n_materials; m++) mats[m] = material_spectra(sc, m);
     for (uint32_t i = 0; i < n; i++) {
         auto g = [&](const float* a, int k) { return a[(size_t)k * n + i]; };
         for (int k = 0; k < NSPEC; k++) value[(size_t)i * NSPEC + k] = 0.0f;
@@ -820,16 +852,14 @@ void bsdf_eval(const RenderScene& sc, int sample, uint32_t n, const float* frame
 
 void render(const RenderScene& sc, const LbvhResult& bvh, RenderOut& out, int nthreads,
             const uint32_t* tile_subset, uint32_t n_tiles_subset, SampleRec* recs) {
-    Ctx C{sc, bvh, {}, {}, {}, {}};
+    Ctx C{sc, bvh, {}, {}, {}, {}, {}};
     if (sc.has_env) {
         build_env_dist(sc, C.env);
         for (int i = 0; i < NSPEC; i++) C.env_emit[i] = sc.env_emit[i];
     }
     C.mats.resize(sc.n_materials);
-    for (uint32_t m = 0; m < sc.n_materials; m++) {
-        C.mats[m].eta = from_rgb(sc.materials[m].eta_rgb, false);
-        C.mats[m].k = from_rgb(sc.materials[m].k_rgb, false);
-    }
+    for (uint32_t m = 0; m < sc.n_materials; m++) C.mats[m] = material_spectra(sc, m);
+    C.tex_filter = texture_filter_table();
     C.light_emit.resize(sc.n_sphere_lights);
     for (uint32_t l = 0; l < sc.n_sphere_lights; l++)
         for (int i = 0; i < NSPEC; i++) C.light_emit[l][i] = sc.light_emit[(size_t)l * NSPEC + i];
@@ -963,7 +993,34 @@ void shade_probe(int fn, const float* in, float* out) {
             out[3] = d.funcInt;
             break;
         }
+        case 9: out[0] = roughness_to_alpha(in[0]); break;
         default: out[0] = 0.0f;
+    }
+}
+
+// The texture-mapped points of camera rays through film positions pfilm (2 x n):
+// the hit shape (0xFFFFFFFF: miss) and q (6 x n, zeros on a miss), as the
+// render's textured materials see them.
+void texture_points(const RenderScene& sc, const LbvhResult& bvh, uint32_t n, const float* pfilm, uint32_t* shape,
+                    float* q) {
+    Ctx C{sc, bvh, {}, {}, {}, {}, {}};
+    for (uint32_t i = 0; i < n; i++) {
+        const float pf[2] = {pfilm[i], pfilm[n + i]};
+        V3 pCam = xf_point4(sc.raster_camera, v3(pf[0], pf[1], 0.0f));
+        V3 dC = normalize3(pCam);
+        Affine CW = load_affine(sc.camera_world);
+        V3 o = xf_point(CW, v3(0, 0, 0));
+        V3 d = normalize3(xf_vector(CW, dC));
+        float t, qi[6] = {0, 0, 0, 0, 0, 0};
+        uint32_t sh;
+        if (trace(C, o, d, &sh, &t)) {
+            float b[3] = {0.0f, 0.0f, 0.0f};
+            if (sh >= sc.geom.n_spheres)
+                tri_intersect(sc.geom.tri_verts + 9 * (size_t)(sh - sc.geom.n_spheres), o, d, b);
+            camera_hit_texture_point(sc, sh, o, d, t, b, pf, qi);
+        }
+        shape[i] = sh;
+        for (int k = 0; k < 6; k++) q[(size_t)k * n + i] = qi[k];
     }
 }
 

@@ -136,11 +136,13 @@ def _write_scene(path, s):
     field as 4 bytes, each array as u64 byte count + bytes, in declaration order."""
     from woodenpbrengine_amd import lib
     d, keep = lib.scene_desc(s)
-    arrays = iter(keep)
+    arrays = iter(keep[:16])  # the host arrays; a serialised scene has no textures
     with open(path, "wb") as f:
         for name, ty in lib.SceneDesc._fields_:
             if ty is lib.C.c_uint32:
                 f.write(np.uint32(getattr(d, name)).tobytes())
+            elif name in ("textures", "material_textures"):
+                f.write(np.uint64(0).tobytes())
             else:
                 a = np.ascontiguousarray(next(arrays))
                 f.write(np.uint64(a.nbytes).tobytes())

@@ -36,23 +36,9 @@ wpbr_status upload(wpbr_ctx* ctx, void* d, const void* h, size_t bytes) {
 // Device copies of the materials and conductor spectra from the host parameters.
 wpbr_status push_materials(wpbr_ctx* ctx, wpbr_scene* sc) {
     const size_t nm = sc->h_materials.size() / 9;
-    std::vector<DevMaterial> mats(nm);
+    std::vector<DevMaterial> mats;
     std::vector<float> specs;
-    for (size_t m = 0; m < nm; m++) {
-        const float* src = &sc->h_materials[9 * m];
-        std::memcpy(&mats[m].type, &src[0], 4);
-        mats[m].alpha = src[1];
-        mats[m].eta = src[2];
-        mats[m].spec = 0;
-        if (mats[m].type == MAT_CONDUCTOR) {
-            mats[m].spec = (uint32_t)(specs.size() / (2 * NSPEC));
-            float e[NSPEC], k[NSPEC];
-            from_rgb_refl(src + 3, e);
-            from_rgb_refl(src + 6, k);
-            specs.insert(specs.end(), e, e + NSPEC);
-            specs.insert(specs.end(), k, k + NSPEC);
-        }
-    }
+    build_materials(sc->h_materials.data(), (uint32_t)nm, sc->h_mat_tex.data(), mats, specs);
     wpbr_status s = upload(ctx, sc->d_materials, mats.data(), sizeof(DevMaterial) * nm);
     if (!s) s = upload(ctx, sc->d_mat_spec, specs.data(), 4 * specs.size());
     if (!s) s = hip_check(ctx, hipStreamSynchronize(ctx->stream), "param upload");
@@ -242,6 +228,8 @@ wpbr_status wpbr_bsdf_derivatives(wpbr_ctx* ctx, uint32_t n, const float* d_in, 
 wpbr_status wpbr_render_backward(wpbr_ctx* ctx, wpbr_scene* sc, const float* d_film, const float* d_dL_drgb,
                                  float* d_grad) {
     if (!ctx || !sc || !d_film || !d_dL_drgb || !d_grad) return WPBR_ERR_INVALID_ARG;
+    if (sc->textured)
+        return set_err(ctx, WPBR_ERR_INVALID_ARG, "render backward: textured materials are not differentiable");
     hipSetDevice(ctx->device);
     (void)hipGetLastError();
     const uint32_t n = sc->sv.width * sc->sv.height;

@@ -318,15 +318,30 @@ WPBR_D BsdfTerms microfacet_terms(const DevMaterial& m, f3 wo, f3 wi, f3 wh, flo
     return r;
 }
 
-// val(lambda) of the BSDF*|cos| for one wavelength.
+// fromRGB(Reflectance) of a textured conductor's metallic value, kept as its
+// Smits case: bases b and weights w (CSpectrumSampled.h:110-158).
+struct TexK {
+    int b[3];
+    float w[3];
+};
+WPBR_D float refl_at(const SceneView& sv, const TexK& t, int i) {
+    float s = 0.0f;
+    s += sv.refl[t.b[0] * NSPEC + i] * t.w[0];
+    s += sv.refl[t.b[1] * NSPEC + i] * t.w[1];
+    s += sv.refl[t.b[2] * NSPEC + i] * t.w[2];
+    return s;
+}
+
+// val(lambda) of the BSDF*|cos| for one wavelength; tk: the k spectrum of a
+// textured conductor hit (null: the material's).
 WPBR_D float bsdf_val(const SceneView& sv, const DevMaterial& m, float cos_fr, float sc, float ac,
-                      uint32_t zero, uint32_t type_spec_fr, int i) {
+                      uint32_t zero, const TexK* tk, int i) {
     if (zero) return 0.0f * ac;
     if (m.type == MAT_CONDUCTOR) {
         const float* es = sv.mat_spec + (size_t)m.spec * 2 * NSPEC;
-        return ((fresnel_conductor1(cos_fr, es[i], es[NSPEC + i]) * 1.0f) * sc) * ac;
+        const float k = tk ? refl_at(sv, *tk, i) : es[NSPEC + i];
+        return ((fresnel_conductor1(cos_fr, es[i], k) * 1.0f) * sc) * ac;
     }
-    (void)type_spec_fr;
     return (1.0f * sc) * ac;
 }
 
@@ -466,6 +481,176 @@ WPBR_D void put_ray(float* q, uint32_t cap, uint32_t slot, f3 o, f3 d) {
     q[3 * (size_t)cap + slot] = d.x; q[4 * (size_t)cap + slot] = d.y; q[5 * (size_t)cap + slot] = d.z;
 }
 
+// ---- full surface interactions (the collision boundary and textured hits) -----------
+// The interaction runCollisionSystem attaches to each ray cast, in the SoA layout
+// of WPBR_SI_* (wpbr.h): CSurfaceInteraction for bSurfInteraction rays
+// (CSTriangle.cpp:149-319; CSSphere.cpp:122-186 then CTransform.h:41-66), the plain
+// CInteraction fields for the others (SLBVH.cpp:55-59), zeros on a miss; the
+// differentials of the rays that carry a CRayDifferential
+// (CSufraceInteraction.h:98-148).  Default-constructed members are zero: the
+// sphere's world interaction keeps uv = 0 and dpdx = dpdy = 0 (the transform
+// leaves uv uncopied and sets dpdy from dpdx), and a triangle's geometric
+// dndu / dndv stay 0.
+struct FullSI {
+    f3 p, wo, n, dpdu, dpdv, dndu, dndv, sn, sdpdu, sdpdv, sdndu, sdndv, dpdx, dpdy;
+    float time, u, v, dudx, dvdx, dudy, dvdy;
+};
+
+WPBR_D FullSI fsi_zero() {
+    FullSI s;
+    const f3 z = mk3(0.0f, 0.0f, 0.0f);
+    s.p = s.wo = s.n = s.dpdu = s.dpdv = s.dndu = s.dndv = z;
+    s.sn = s.sdpdu = s.sdpdv = s.sdndu = s.sdndv = s.dpdx = s.dpdy = z;
+    s.time = s.u = s.v = s.dudx = s.dvdx = s.dudy = s.dvdy = 0.0f;
+    return s;
+}
+
+// CSurfaceInteraction's constructor (CSufraceInteraction.h:58-83).
+WPBR_D void fsi_construct(FullSI& s, f3 p, float u, float v, f3 wo, f3 dpdu, f3 dpdv, f3 dndu, f3 dndv,
+                          float time) {
+    s.p = p; s.u = u; s.v = v; s.wo = wo; s.time = time;
+    s.dpdu = dpdu; s.dpdv = dpdv; s.dndu = dndu; s.dndv = dndv;
+    s.n = normalize3(cross3(dpdu, dpdv));
+    s.sn = s.n; s.sdpdu = dpdu; s.sdpdv = dpdv; s.sdndu = dndu; s.sdndv = dndv;
+}
+
+WPBR_D FullSI si_full_triangle(const SceneView& sv, uint32_t tri, f3 rd, const float* b, float t) {
+    const float* v = sv.g.tri_verts + 9 * (size_t)tri;
+    const f3 P0 = mk3(v[0], v[1], v[2]), P1 = mk3(v[3], v[4], v[5]), P2 = mk3(v[6], v[7], v[8]);
+    const uint8_t flags = sv.tri_flags ? sv.tri_flags[tri] : (uint8_t)0;
+    float uv[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+    if (flags & 1) {
+        const float* u = sv.tri_uv + 6 * (size_t)tri;
+        for (int i = 0; i < 3; i++) { uv[i][0] = u[2 * i]; uv[i][1] = u[2 * i + 1]; }
+    }
+    const f3 dp02 = sub3(P0, P2), dp12 = sub3(P1, P2);
+    float duv02[2] = {uv[0][0] - uv[2][0], uv[0][1] - uv[2][1]};
+    float duv12[2] = {uv[1][0] - uv[2][0], uv[1][1] - uv[2][1]};
+    float det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+    if (det == 0) {
+        const float k = (float)(1.0 * 0.001);
+        uv[0][0] = 0; uv[0][1] = 0; uv[1][0] = k; uv[1][1] = 0; uv[2][0] = k; uv[2][1] = k;
+        duv02[0] = uv[0][0] - uv[2][0]; duv02[1] = uv[0][1] - uv[2][1];
+        duv12[0] = uv[1][0] - uv[2][0]; duv12[1] = uv[1][1] - uv[2][1];
+        det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+    }
+    const float detInv = 1 / det;
+    const f3 dpdu = mul3(sub3(mul3(dp02, duv12[1]), mul3(dp12, duv02[1])), detInv);
+    const f3 dpdv = mul3(add3(mul3(neg3(dp02), duv12[0]), mul3(dp12, duv02[0])), detInv);
+    const f3 pHit = add3(add3(mul3(P0, b[0]), mul3(P1, b[1])), mul3(P2, b[2]));
+    const float uh = (uv[0][0] * b[0] + uv[1][0] * b[1]) + uv[2][0] * b[2];
+    const float vh = (uv[0][1] * b[0] + uv[1][1] * b[1]) + uv[2][1] * b[2];
+    FullSI s = fsi_zero();
+    const f3 z = mk3(0.0f, 0.0f, 0.0f);
+    fsi_construct(s, pHit, uh, vh, neg3(rd), dpdu, dpdv, z, z, t);
+    s.n = s.sn = normalize3(cross3(dp02, dp12));
+    if (flags & 2) {
+        const float* nn = sv.tri_normal + 9 * (size_t)tri;
+        const float* tt = sv.tri_tangent + 9 * (size_t)tri;
+        const f3 N0 = mk3(nn[0], nn[1], nn[2]), N1 = mk3(nn[3], nn[4], nn[5]), N2 = mk3(nn[6], nn[7], nn[8]);
+        const f3 T0 = mk3(tt[0], tt[1], tt[2]), T1 = mk3(tt[3], tt[4], tt[5]), T2 = mk3(tt[6], tt[7], tt[8]);
+        f3 ns = add3(add3(mul3(N0, b[0]), mul3(N1, b[1])), mul3(N2, b[2]));
+        ns = is_zero(ns) ? s.n : normalize3(ns);
+        f3 ss = add3(add3(mul3(T0, b[0]), mul3(T1, b[1])), mul3(T2, b[2]));
+        ss = is_zero(ss) ? normalize3(s.dpdu) : normalize3(ss);
+        f3 ts = cross3(ss, ns);
+        if (len2(ts) > 0) {
+            ts = normalize3(ts);
+            ss = cross3(ts, ns);
+        }
+        const f3 dn02 = sub3(N0, N2), dn12 = sub3(N1, N2);
+        f3 dndu = z, dndv = z;
+        const float det2 = duv02[0] * duv12[1] - duv02[1] * duv12[0];
+        if (det2 != 0) {
+            const float inv2 = 1 / det2;
+            dndu = mul3(sub3(mul3(dn02, duv12[1]), mul3(dn12, duv02[1])), inv2);
+            dndv = mul3(add3(mul3(neg3(dn02), duv12[0]), mul3(dn12, duv02[0])), inv2);
+        }
+        s.sn = normalize3(cross3(ss, ts));
+        s.sdpdu = ss;
+        s.sdpdv = ts;
+        s.sdndu = dndu;
+        s.sdndv = dndv;
+        s.n = dot3(s.n, s.sn) < 0.0f ? neg3(s.n) : s.n;
+    }
+    return s;
+}
+
+WPBR_D FullSI si_full_sphere(const SceneView& sv, uint32_t sid, f3 o, f3 d, float t) {
+    const float* xf = sv.g.sphere_xform + 24 * (size_t)sid;
+    const affine W = load_affine(xf), Inv = load_affine(xf + 12);
+    const float radius = sv.g.sphere_radius[sid];
+    const f3 oL = xf_point(Inv, o), dL = xf_vector(Inv, d);
+    const f3 pH = add3(oL, mul3(dL, t));
+    float phi = cr_atan2(pH.y, pH.x);
+    if (phi < 0.0f) phi = (float)((double)phi + 2 * PI_REF);
+    const float u = (float)((double)phi / (2 * PI_REF));
+    const float theta = cr_acos(pH.z / radius);
+    const float v = (float)((double)theta / PI_REF);
+    const float invZR = (float)(1.0 / (double)sqrtf(pH.x * pH.x + pH.y * pH.y));
+    const float cphi = pH.x * invZR, sphi = pH.y * invZR;
+    const f3 dpdu = mul3(mk3(-pH.y, pH.x, 0.0f), (float)(2 * PI_REF));
+    const f3 dpdv = mul3(mk3(pH.z * cphi, pH.z * sphi, -radius * cr_sin(theta)), (float)PI_REF);
+    const f3 d2Pduu = mul3(mk3(pH.x, pH.y, 0.0f), (float)(-2 * PI_REF * 2 * PI_REF));
+    const f3 d2Pduv = mul3(mul3(mul3(mul3(mk3(-sphi, cphi, 0.0f), (float)PI_REF), pH.z), 2.0f), (float)PI_REF);
+    const f3 d2Pdvv = mul3(mul3(mk3(pH.x, pH.y, pH.z), (float)(-PI_REF)), (float)PI_REF);
+    const float E = dot3(dpdu, dpdu), F = dot3(dpdu, dpdv), G = dot3(dpdv, dpdv);
+    const f3 N = normalize3(cross3(dpdu, dpdv));
+    const float e = dot3(N, d2Pduu), f = dot3(N, d2Pduv), g = dot3(N, d2Pdvv);
+    const float invEGF2 = 1 / (E * G - F * F);
+    const f3 dndu = add3(mul3(dpdu, (f * F - e * G) * invEGF2), mul3(dpdv, (e * F - f * E) * invEGF2));
+    const f3 dndv = add3(mul3(dpdu, (g * F - f * G) * invEGF2), mul3(dpdv, (f * F - g * E) * invEGF2));
+    FullSI l = fsi_zero();
+    fsi_construct(l, pH, u, v, neg3(dL), dpdu, dpdv, dndu, dndv, t);
+    // CTransform::operator()(const CSurfaceInteraction&) into a fresh interaction
+    FullSI s = fsi_zero();
+    s.time = l.time;
+    s.dndu = xf_normal(Inv, l.dndu);
+    s.dndv = xf_normal(Inv, l.dndv);
+    s.dpdu = xf_vector(W, l.dpdu);
+    s.dpdv = xf_vector(W, l.dpdv);
+    s.p = xf_point(W, l.p);
+    s.dpdx = xf_vector(W, l.dpdx);
+    s.dpdy = xf_vector(W, l.dpdx);
+    s.n = normalize3(xf_normal(Inv, l.n));
+    s.wo = normalize3(xf_vector(W, l.wo));
+    s.sdndu = xf_vector(W, l.sdndu);
+    s.sdndv = xf_vector(W, l.sdndv);
+    s.sdpdu = xf_vector(W, l.sdpdu);
+    s.sdpdv = xf_vector(W, l.sdpdv);
+    s.sn = normalize3(xf_normal(Inv, l.sn));
+    return s;
+}
+
+// pbrt-v3 SolveLinearSystem2x2 (WoodenMath's solveLinearSystem2x2).
+WPBR_D bool solve2x2(float a00, float a01, float a10, float a11, float b0, float b1, float* x0, float* x1) {
+    const float det = a00 * a11 - a01 * a10;
+    if (fabsf(det) < 1e-10f) return false;
+    *x0 = (a11 * b0 - a01 * b1) / det;
+    *x1 = (a00 * b1 - a10 * b0) / det;
+    return !(__builtin_isnan(*x0) || __builtin_isnan(*x1));
+}
+
+// CSurfaceInteraction::computeDifferentials (CSufraceInteraction.h:98-148).
+WPBR_D void si_differentials(FullSI& s, f3 xo, f3 xd, f3 yo, f3 yd) {
+    const float dd = dot3(s.n, s.p);
+    const float tx = -(dot3(s.n, xo) - dd) / dot3(s.n, xd);
+    const f3 px = add3(xo, mul3(xd, tx));
+    const float ty = -(dot3(s.n, yo) - dd) / dot3(s.n, yd);
+    const f3 py = add3(yo, mul3(yd, ty));
+    s.dpdx = sub3(px, s.p);
+    s.dpdy = sub3(py, s.p);
+    int d0, d1;
+    if (fabsf(s.n.x) > fabsf(s.n.y) && fabsf(s.n.x) > fabsf(s.n.z)) { d0 = 1; d1 = 2; }
+    else if (fabsf(s.n.y) > fabsf(s.n.z)) { d0 = 0; d1 = 2; }
+    else { d0 = 0; d1 = 1; }
+    const float a00 = comp(s.dpdu, d0), a01 = comp(s.dpdv, d0), a10 = comp(s.dpdu, d1), a11 = comp(s.dpdv, d1);
+    if (!solve2x2(a00, a01, a10, a11, comp(px, d0) - comp(s.p, d0), comp(px, d1) - comp(s.p, d1), &s.dudx, &s.dvdx))
+        s.dudx = s.dvdx = 0;
+    if (!solve2x2(a00, a01, a10, a11, comp(py, d0) - comp(s.p, d0), comp(py, d1) - comp(s.p, d1), &s.dudy, &s.dvdy))
+        s.dudy = s.dvdy = 0;
+}
+
 // ---- kernels ----------------------------------------------------------------------------
 WPBR_D void camera_body(const RenderLaunch& L, uint32_t i) {
     if (i >= L.n) return;
@@ -493,6 +678,65 @@ WPBR_D void camera_body(const RenderLaunch& L, uint32_t i) {
 __global__ __launch_bounds__(256) void k_camera(RenderLaunch L) { camera_body(L, blockIdx.x * blockDim.x + threadIdx.x); }
 
 constexpr uint32_t F_LIGHT = 1u, F_BSDF = 2u, F_ENV = 4u, F_DONE = 8u, F_MISS = 16u;
+constexpr uint32_t F_TEXK = 32u;  // a textured conductor hit: its k RGB in WaveState::tk
+
+// MicrofaceDistr.h:18-24.
+WPBR_D float roughness_to_alpha(float roughness) {
+    roughness = fmaxf(roughness, 1e-3f);
+    const float x = cr_log(roughness);
+    return 1.62142f + 0.819955f * x + 0.1734f * x * x + 0.0171201f * x * x * x + 0.000640711f * x * x * x * x;
+}
+
+// TextureMappingSphere::sphere (CTextureMapping.h:60-66): the point in the
+// sphere's object space, normalised, as (theta / PI, phi / (2 PI)).
+WPBR_D void sphere_st(const affine& inv, f3 p, float* st) {
+    const f3 v = normalize3(xf_point(inv, p));
+    const float th = spherical_theta(v), ph = spherical_phi(v);
+    st[0] = (float)((double)th * 1.0 / PI_REF);
+    st[1] = (float)((double)ph / (2 * PI_REF));
+}
+
+// The CTextureMappedPoint (s, t, ds/dx, dt/dx, ds/dy, dt/dy) of a camera ray's
+// hit: the hit's full interaction, its differentials from the camera's ray
+// differential (CSCameraPerspective.cpp:37-41: the same origin, directions
+// through pCamera + dxCamera / dyCamera, dxCamera the raster->camera transform
+// of the vector (1, 0, 0), i.e. its first column), then the triangle's UV
+// mapping (STriangle::mapUV -> TextureMappingUV, CTextureMapping.h:39-55) or
+// the sphere's (SSphere::mapUV -> TextureMappingSphere, :68-90).
+WPBR_D void camera_hit_texture_point(const RenderLaunch& L, uint32_t i, uint32_t shape, f3 o, f3 d, float t,
+                                     const float* b, float* q) {
+    const SceneView& sv = L.sv;
+    const bool sphere = shape < sv.g.n_spheres;
+    FullSI s = sphere ? si_full_sphere(sv, shape, o, d, t) : si_full_triangle(sv, shape - sv.g.n_spheres, d, b, t);
+    const f3 pCam = xf_point4(sv.raster_camera, mk3(L.ws.pfilm[i], L.ws.pfilm[L.n + i], 0.0f));
+    const float* rc = sv.raster_camera;
+    const f3 dxC = mk3(rc[0], rc[4], rc[8]), dyC = mk3(rc[1], rc[5], rc[9]);
+    const affine CW = load_affine(sv.camera_world);
+    const f3 dx = normalize3(xf_vector(CW, normalize3(add3(pCam, dxC))));
+    const f3 dy = normalize3(xf_vector(CW, normalize3(add3(pCam, dyC))));
+    si_differentials(s, o, dx, o, dy);
+    if (!sphere) {
+        q[0] = s.u; q[1] = s.v;
+        q[2] = s.dudx; q[3] = s.dvdx;
+        q[4] = s.dudy; q[5] = s.dvdy;
+        return;
+    }
+    const affine inv = load_affine(sv.g.sphere_xform + 24 * (size_t)shape + 12);
+    float st[2], sx[2], sy[2];
+    sphere_st(inv, s.p, st);
+    const float dt = 0.1f;
+    sphere_st(inv, add3(s.p, mul3(s.dpdx, dt)), sx);
+    sphere_st(inv, add3(s.p, mul3(s.dpdy, dt)), sy);
+    float dstdx[2] = {(sx[0] - st[0]) / dt, (sx[1] - st[1]) / dt};
+    float dstdy[2] = {(sy[0] - st[0]) / dt, (sy[1] - st[1]) / dt};
+    if (dstdx[1] > .5f) dstdx[1] = 1 - dstdx[1];
+    else if (dstdx[1] < -.5f) dstdx[1] = -dstdx[1] + 1;
+    if (dstdy[1] > .5f) dstdy[1] = 1 - dstdy[1];
+    else if (dstdy[1] < -.5f) dstdy[1] = -dstdy[1] + 1;
+    q[0] = st[0]; q[1] = st[1];
+    q[2] = dstdx[0]; q[3] = dstdx[1];
+    q[4] = dstdy[0]; q[5] = dstdy[1];
+}
 
 WPBR_D void shade1_body(const RenderLaunch& L, uint32_t i) {
     bool active = i < L.n;
@@ -559,8 +803,23 @@ WPBR_D void shade1_body(const RenderLaunch& L, uint32_t i) {
             if (mid == 0xFFFFFFFFu) {
                 flags |= F_DONE;
             } else {
-                const DevMaterial m = sv.materials[mid];
+                DevMaterial m = sv.materials[mid];
                 ws.mat[i] = mid;
+                if (m.tex_k >= 0 || m.tex_r >= 0) {
+                    // a textured metal (JobGenerateBSDFMaterialMetal, CMaterialMetal.cpp:24-52)
+                    float q[6];
+                    const float bt3[3] = {ws.bary[i], ws.bary[n + i], ws.bary[2 * (size_t)n + i]};
+                    camera_hit_texture_point(L, i, shape, o, d, t, bt3, q);
+                    if (m.tex_r >= 0) {
+                        const float3 r = tex_lookup(sv.textures[m.tex_r], q[0], q[1], q[2], q[3], q[4], q[5]);
+                        m.alpha = roughness_to_alpha(clampf(r.x * 90, 0.0f, 90.0f));
+                    }
+                    if (m.tex_k >= 0) {
+                        const float3 k = tex_lookup(sv.textures[m.tex_k], q[0], q[1], q[2], q[3], q[4], q[5]);
+                        ws.tk[i] = k.x; ws.tk[n + i] = k.y; ws.tk[2 * (size_t)n + i] = k.z;
+                        flags |= F_TEXK;
+                    }
+                }
                 f3 ss = normalize3(si.dpdu);
                 f3 ns = si.ns;
                 f3 ts = cross3(ns, ss);
@@ -736,13 +995,20 @@ WPBR_D void sample_radiance(const RenderLaunch& L, uint32_t i, float& X, float& 
             }
         }
         if (useL || useB) {
+            TexK tkb;
+            const TexK* tkp = nullptr;
+            if (flags & F_TEXK) {
+                const float rgb[3] = {ws.tk[i], ws.tk[n + i], ws.tk[2 * (size_t)n + i]};
+                rgb_basis(rgb, tkb.b, tkb.w);
+                tkp = &tkb;
+            }
             for (int k = 0; k < NSPEC; k++) {
                 float Lk = 0.0f;
-                if (useL) Lk += (Le[k] * bsdf_val(sv, m, lcos, lsc, lac, lzero, 0, k)) * sL;
+                if (useL) Lk += (Le[k] * bsdf_val(sv, m, lcos, lsc, lac, lzero, tkp, k)) * sL;
                 if (useB) {
                     float li_k = envPick ? illum_at(sv, eb, ew, k) * sv.env_emit[k] : Lb[k];
                     float v = (m.type == MAT_SPECULAR) ? (bzero ? 0.0f * bac : (1.0f * bsc) * bac)
-                                                       : bsdf_val(sv, m, bcos, bsc, bac, bzero, 0, k);
+                                                       : bsdf_val(sv, m, bcos, bsc, bac, bzero, tkp, k);
                     Lk += (li_k * v) * sB;
                 }
                 if (Lk != 0.0f) black = false;
@@ -924,175 +1190,7 @@ __global__ __launch_bounds__(256) void k_film_rgb8(const float4* __restrict__ fi
 }
 
 // ---- interactions at the collision boundary (wpbr_scene_collide) ------------------
-// The interaction runCollisionSystem attaches to each ray cast, in the SoA layout
-// of WPBR_SI_* (wpbr.h): CSurfaceInteraction for bSurfInteraction rays
-// (CSTriangle.cpp:149-319; CSSphere.cpp:122-186 then CTransform.h:41-66), the plain
-// CInteraction fields for the others (SLBVH.cpp:55-59), zeros on a miss; the
-// differentials of the rays that carry a CRayDifferential
-// (CSufraceInteraction.h:98-148).  Default-constructed members are zero: the
-// sphere's world interaction keeps uv = 0 and dpdx = dpdy = 0 (the transform
-// leaves uv uncopied and sets dpdy from dpdx), and a triangle's geometric
-// dndu / dndv stay 0.
-struct FullSI {
-    f3 p, wo, n, dpdu, dpdv, dndu, dndv, sn, sdpdu, sdpdv, sdndu, sdndv, dpdx, dpdy;
-    float time, u, v, dudx, dvdx, dudy, dvdy;
-};
-
-WPBR_D FullSI fsi_zero() {
-    FullSI s;
-    const f3 z = mk3(0.0f, 0.0f, 0.0f);
-    s.p = s.wo = s.n = s.dpdu = s.dpdv = s.dndu = s.dndv = z;
-    s.sn = s.sdpdu = s.sdpdv = s.sdndu = s.sdndv = s.dpdx = s.dpdy = z;
-    s.time = s.u = s.v = s.dudx = s.dvdx = s.dudy = s.dvdy = 0.0f;
-    return s;
-}
-
-// CSurfaceInteraction's constructor (CSufraceInteraction.h:58-83).
-WPBR_D void fsi_construct(FullSI& s, f3 p, float u, float v, f3 wo, f3 dpdu, f3 dpdv, f3 dndu, f3 dndv,
-                          float time) {
-    s.p = p; s.u = u; s.v = v; s.wo = wo; s.time = time;
-    s.dpdu = dpdu; s.dpdv = dpdv; s.dndu = dndu; s.dndv = dndv;
-    s.n = normalize3(cross3(dpdu, dpdv));
-    s.sn = s.n; s.sdpdu = dpdu; s.sdpdv = dpdv; s.sdndu = dndu; s.sdndv = dndv;
-}
-
-WPBR_D FullSI si_full_triangle(const SceneView& sv, uint32_t tri, f3 rd, const float* b, float t) {
-    const float* v = sv.g.tri_verts + 9 * (size_t)tri;
-    const f3 P0 = mk3(v[0], v[1], v[2]), P1 = mk3(v[3], v[4], v[5]), P2 = mk3(v[6], v[7], v[8]);
-    const uint8_t flags = sv.tri_flags ? sv.tri_flags[tri] : (uint8_t)0;
-    float uv[3][2] = {{0, 0}, {1, 0}, {1, 1}};
-    if (flags & 1) {
-        const float* u = sv.tri_uv + 6 * (size_t)tri;
-        for (int i = 0; i < 3; i++) { uv[i][0] = u[2 * i]; uv[i][1] = u[2 * i + 1]; }
-    }
-    const f3 dp02 = sub3(P0, P2), dp12 = sub3(P1, P2);
-    float duv02[2] = {uv[0][0] - uv[2][0], uv[0][1] - uv[2][1]};
-    float duv12[2] = {uv[1][0] - uv[2][0], uv[1][1] - uv[2][1]};
-    float det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
-    if (det == 0) {
-        const float k = (float)(1.0 * 0.001);
-        uv[0][0] = 0; uv[0][1] = 0; uv[1][0] = k; uv[1][1] = 0; uv[2][0] = k; uv[2][1] = k;
-        duv02[0] = uv[0][0] - uv[2][0]; duv02[1] = uv[0][1] - uv[2][1];
-        duv12[0] = uv[1][0] - uv[2][0]; duv12[1] = uv[1][1] - uv[2][1];
-        det = duv02[0] * duv12[1] - duv02[1] * duv12[0];
-    }
-    const float detInv = 1 / det;
-    const f3 dpdu = mul3(sub3(mul3(dp02, duv12[1]), mul3(dp12, duv02[1])), detInv);
-    const f3 dpdv = mul3(add3(mul3(neg3(dp02), duv12[0]), mul3(dp12, duv02[0])), detInv);
-    const f3 pHit = add3(add3(mul3(P0, b[0]), mul3(P1, b[1])), mul3(P2, b[2]));
-    const float uh = (uv[0][0] * b[0] + uv[1][0] * b[1]) + uv[2][0] * b[2];
-    const float vh = (uv[0][1] * b[0] + uv[1][1] * b[1]) + uv[2][1] * b[2];
-    FullSI s = fsi_zero();
-    const f3 z = mk3(0.0f, 0.0f, 0.0f);
-    fsi_construct(s, pHit, uh, vh, neg3(rd), dpdu, dpdv, z, z, t);
-    s.n = s.sn = normalize3(cross3(dp02, dp12));
-    if (flags & 2) {
-        const float* nn = sv.tri_normal + 9 * (size_t)tri;
-        const float* tt = sv.tri_tangent + 9 * (size_t)tri;
-        const f3 N0 = mk3(nn[0], nn[1], nn[2]), N1 = mk3(nn[3], nn[4], nn[5]), N2 = mk3(nn[6], nn[7], nn[8]);
-        const f3 T0 = mk3(tt[0], tt[1], tt[2]), T1 = mk3(tt[3], tt[4], tt[5]), T2 = mk3(tt[6], tt[7], tt[8]);
-        f3 ns = add3(add3(mul3(N0, b[0]), mul3(N1, b[1])), mul3(N2, b[2]));
-        ns = is_zero(ns) ? s.n : normalize3(ns);
-        f3 ss = add3(add3(mul3(T0, b[0]), mul3(T1, b[1])), mul3(T2, b[2]));
-        ss = is_zero(ss) ? normalize3(s.dpdu) : normalize3(ss);
-        f3 ts = cross3(ss, ns);
-        if (len2(ts) > 0) {
-            ts = normalize3(ts);
-            ss = cross3(ts, ns);
-        }
-        const f3 dn02 = sub3(N0, N2), dn12 = sub3(N1, N2);
-        f3 dndu = z, dndv = z;
-        const float det2 = duv02[0] * duv12[1] - duv02[1] * duv12[0];
-        if (det2 != 0) {
-            const float inv2 = 1 / det2;
-            dndu = mul3(sub3(mul3(dn02, duv12[1]), mul3(dn12, duv02[1])), inv2);
-            dndv = mul3(add3(mul3(neg3(dn02), duv12[0]), mul3(dn12, duv02[0])), inv2);
-        }
-        s.sn = normalize3(cross3(ss, ts));
-        s.sdpdu = ss;
-        s.sdpdv = ts;
-        s.sdndu = dndu;
-        s.sdndv = dndv;
-        s.n = dot3(s.n, s.sn) < 0.0f ? neg3(s.n) : s.n;
-    }
-    return s;
-}
-
-WPBR_D FullSI si_full_sphere(const SceneView& sv, uint32_t sid, f3 o, f3 d, float t) {
-    const float* xf = sv.g.sphere_xform + 24 * (size_t)sid;
-    const affine W = load_affine(xf), Inv = load_affine(xf + 12);
-    const float radius = sv.g.sphere_radius[sid];
-    const f3 oL = xf_point(Inv, o), dL = xf_vector(Inv, d);
-    const f3 pH = add3(oL, mul3(dL, t));
-    float phi = cr_atan2(pH.y, pH.x);
-    if (phi < 0.0f) phi = (float)((double)phi + 2 * PI_REF);
-    const float u = (float)((double)phi / (2 * PI_REF));
-    const float theta = cr_acos(pH.z / radius);
-    const float v = (float)((double)theta / PI_REF);
-    const float invZR = (float)(1.0 / (double)sqrtf(pH.x * pH.x + pH.y * pH.y));
-    const float cphi = pH.x * invZR, sphi = pH.y * invZR;
-    const f3 dpdu = mul3(mk3(-pH.y, pH.x, 0.0f), (float)(2 * PI_REF));
-    const f3 dpdv = mul3(mk3(pH.z * cphi, pH.z * sphi, -radius * cr_sin(theta)), (float)PI_REF);
-    const f3 d2Pduu = mul3(mk3(pH.x, pH.y, 0.0f), (float)(-2 * PI_REF * 2 * PI_REF));
-    const f3 d2Pduv = mul3(mul3(mul3(mul3(mk3(-sphi, cphi, 0.0f), (float)PI_REF), pH.z), 2.0f), (float)PI_REF);
-    const f3 d2Pdvv = mul3(mul3(mk3(pH.x, pH.y, pH.z), (float)(-PI_REF)), (float)PI_REF);
-    const float E = dot3(dpdu, dpdu), F = dot3(dpdu, dpdv), G = dot3(dpdv, dpdv);
-    const f3 N = normalize3(cross3(dpdu, dpdv));
-    const float e = dot3(N, d2Pduu), f = dot3(N, d2Pduv), g = dot3(N, d2Pdvv);
-    const float invEGF2 = 1 / (E * G - F * F);
-    const f3 dndu = add3(mul3(dpdu, (f * F - e * G) * invEGF2), mul3(dpdv, (e * F - f * E) * invEGF2));
-    const f3 dndv = add3(mul3(dpdu, (g * F - f * G) * invEGF2), mul3(dpdv, (f * F - g * E) * invEGF2));
-    FullSI l = fsi_zero();
-    fsi_construct(l, pH, u, v, neg3(dL), dpdu, dpdv, dndu, dndv, t);
-    // CTransform::operator()(const CSurfaceInteraction&) into a fresh interaction
-    FullSI s = fsi_zero();
-    s.time = l.time;
-    s.dndu = xf_normal(Inv, l.dndu);
-    s.dndv = xf_normal(Inv, l.dndv);
-    s.dpdu = xf_vector(W, l.dpdu);
-    s.dpdv = xf_vector(W, l.dpdv);
-    s.p = xf_point(W, l.p);
-    s.dpdx = xf_vector(W, l.dpdx);
-    s.dpdy = xf_vector(W, l.dpdx);
-    s.n = normalize3(xf_normal(Inv, l.n));
-    s.wo = normalize3(xf_vector(W, l.wo));
-    s.sdndu = xf_vector(W, l.sdndu);
-    s.sdndv = xf_vector(W, l.sdndv);
-    s.sdpdu = xf_vector(W, l.sdpdu);
-    s.sdpdv = xf_vector(W, l.sdpdv);
-    s.sn = normalize3(xf_normal(Inv, l.sn));
-    return s;
-}
-
-// pbrt-v3 SolveLinearSystem2x2 (WoodenMath's solveLinearSystem2x2).
-WPBR_D bool solve2x2(float a00, float a01, float a10, float a11, float b0, float b1, float* x0, float* x1) {
-    const float det = a00 * a11 - a01 * a10;
-    if (fabsf(det) < 1e-10f) return false;
-    *x0 = (a11 * b0 - a01 * b1) / det;
-    *x1 = (a00 * b1 - a10 * b0) / det;
-    return !(__builtin_isnan(*x0) || __builtin_isnan(*x1));
-}
-
-// CSurfaceInteraction::computeDifferentials (CSufraceInteraction.h:98-148).
-WPBR_D void si_differentials(FullSI& s, f3 xo, f3 xd, f3 yo, f3 yd) {
-    const float dd = dot3(s.n, s.p);
-    const float tx = -(dot3(s.n, xo) - dd) / dot3(s.n, xd);
-    const f3 px = add3(xo, mul3(xd, tx));
-    const float ty = -(dot3(s.n, yo) - dd) / dot3(s.n, yd);
-    const f3 py = add3(yo, mul3(yd, ty));
-    s.dpdx = sub3(px, s.p);
-    s.dpdy = sub3(py, s.p);
-    int d0, d1;
-    if (fabsf(s.n.x) > fabsf(s.n.y) && fabsf(s.n.x) > fabsf(s.n.z)) { d0 = 1; d1 = 2; }
-    else if (fabsf(s.n.y) > fabsf(s.n.z)) { d0 = 0; d1 = 2; }
-    else { d0 = 0; d1 = 1; }
-    const float a00 = comp(s.dpdu, d0), a01 = comp(s.dpdv, d0), a10 = comp(s.dpdu, d1), a11 = comp(s.dpdv, d1);
-    if (!solve2x2(a00, a01, a10, a11, comp(px, d0) - comp(s.p, d0), comp(px, d1) - comp(s.p, d1), &s.dudx, &s.dvdx))
-        s.dudx = s.dvdx = 0;
-    if (!solve2x2(a00, a01, a10, a11, comp(py, d0) - comp(s.p, d0), comp(py, d1) - comp(s.p, d1), &s.dudy, &s.dvdy))
-        s.dudy = s.dvdy = 0;
-}
-
+// (the interaction helpers are above the kernels; this is the boundary kernel)
 WPBR_D void put3(float* si, size_t n, size_t i, int f, f3 v) {
     si[(size_t)f * n + i] = v.x;
     si[(size_t)(f + 1) * n + i] = v.y;
@@ -1247,10 +1345,10 @@ WPBR_D void bsdf_request(const SceneView& sv, const BsdfArgs& a, uint32_t i, int
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int k = 16 * j + 4 * sub;
-            out[4 * j] = make_float4(bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k),
-                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k + 1),
-                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k + 2),
-                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, 0u, k + 3));
+            out[4 * j] = make_float4(bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, nullptr, k),
+                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, nullptr, k + 1),
+                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, nullptr, k + 2),
+                                     bsdf_val(sv, m, bt.cos_fr, bt.sc, bt.ac, bt.zero, nullptr, k + 3));
         }
     } else {
         const float4 v4 = make_float4(v, v, v, v);

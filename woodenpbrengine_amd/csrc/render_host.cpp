@@ -84,6 +84,37 @@ void from_rgb_refl(const float* rgb, float* s) {
     }
 }
 
+// Device materials and conductor spectra (eta[64], k[64] each) from the
+// descriptor's 9-float records: conductor spectra via fromRGB(Reflectance); a
+// textured conductor (CMaterialMetal.cpp:24-38) takes the constant spectrum of
+// its eta (the reference's Spectrum(1.5f)) and, without a metallic texture,
+// fromRGB of its k RGB.
+void build_materials(const float* h, uint32_t nm, const int32_t* mat_tex, std::vector<DevMaterial>& mats,
+                     std::vector<float>& specs) {
+    mats.assign(nm, DevMaterial{});
+    specs.clear();
+    for (uint32_t m = 0; m < nm; m++) {
+        const float* src = h + 9 * (size_t)m;
+        std::memcpy(&mats[m].type, &src[0], 4);
+        mats[m].alpha = src[1];
+        mats[m].eta = src[2];
+        mats[m].spec = 0;
+        mats[m].tex_k = mat_tex ? mat_tex[2 * (size_t)m] : -1;
+        mats[m].tex_r = mat_tex ? mat_tex[2 * (size_t)m + 1] : -1;
+        if (mats[m].type == MAT_CONDUCTOR) {
+            mats[m].spec = (uint32_t)(specs.size() / (2 * NSPEC));
+            float e[NSPEC], k[NSPEC];
+            if (mats[m].tex_k >= 0 || mats[m].tex_r >= 0)
+                for (int i = 0; i < NSPEC; i++) e[i] = src[2];
+            else
+                from_rgb_refl(src + 3, e);
+            from_rgb_refl(src + 6, k);
+            specs.insert(specs.end(), e, e + NSPEC);
+            specs.insert(specs.end(), k, k + NSPEC);
+        }
+    }
+}
+
 // Bilinear mip-0 lookup (CSTextureProcess.h:58-81), indices clamped to the texture.
 void env_bilerp_h(const float* rgbTex, int W, int H, float s_, float t_, float* out) {
     float s = s_ * (float)W - 0.5f, t = t_ * (float)H - 0.5f;
@@ -177,6 +208,7 @@ size_t carve_wave(char* p, uint32_t n, WaveState& w) {
     w.b_zero = (uint32_t*)take(n);
     w.b_wi = (float*)take(3 * (size_t)n);
     w.le = (float*)take(3 * (size_t)n);
+    w.tk = (float*)take(3 * (size_t)n);
     w.qrays = (float*)take(12 * (size_t)n);
     w.qhit = (uint32_t*)take(2 * (size_t)n);
     w.qt = (float*)take(2 * (size_t)n);
@@ -256,6 +288,24 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
     for (uint64_t s = 0; s < n; s++)
         if (d->shape_material[s] != 0xFFFFFFFFu && d->shape_material[s] >= d->n_materials)
             return set_err(ctx, WPBR_ERR_INVALID_ARG, "material index out of range");
+    if (d->n_textures) {
+        if (!d->textures || !d->material_textures)
+            return set_err(ctx, WPBR_ERR_INVALID_ARG, "textures: null texture or binding array");
+        for (uint32_t t = 0; t < d->n_textures; t++)
+            if (!d->textures[t]) return set_err(ctx, WPBR_ERR_INVALID_ARG, "textures: null texture");
+        for (uint32_t m = 0; m < d->n_materials; m++) {
+            uint32_t type;
+            std::memcpy(&type, d->materials + 9 * (size_t)m, 4);
+            for (int k = 0; k < 2; k++) {
+                const int32_t ti = d->material_textures[2 * (size_t)m + k];
+                if (ti < -1 || ti >= (int32_t)d->n_textures)
+                    return set_err(ctx, WPBR_ERR_INVALID_ARG, "material " + std::to_string(m) + ": no such texture");
+                if (ti >= 0 && type != MAT_CONDUCTOR)
+                    return set_err(ctx, WPBR_ERR_INVALID_ARG,
+                                   "material " + std::to_string(m) + ": only conductors bind textures");
+            }
+        }
+    }
     hipSetDevice(ctx->device);
     wpbr_scene* sc = new wpbr_scene();
     sc->ctx = ctx;
@@ -286,24 +336,18 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
     v.tri_normal = (const float*)up(d->tri_normal, d->tri_normal ? 36 * (size_t)nt : 0);
     v.tri_tangent = (const float*)up(d->tri_tangent, d->tri_tangent ? 36 * (size_t)nt : 0);
     v.shape_material = (const uint32_t*)up(d->shape_material, 4 * (size_t)n);
-    // materials: conductor spectra via fromRGB(Reflectance)
-    std::vector<DevMaterial> mats(d->n_materials);
-    std::vector<float> specs;
-    for (uint32_t m = 0; m < d->n_materials; m++) {
-        const float* src = d->materials + 9 * (size_t)m;
-        std::memcpy(&mats[m].type, &src[0], 4);
-        mats[m].alpha = src[1];
-        mats[m].eta = src[2];
-        mats[m].spec = 0;
-        if (mats[m].type == MAT_CONDUCTOR) {
-            mats[m].spec = (uint32_t)(specs.size() / (2 * NSPEC));
-            float e[NSPEC], k[NSPEC];
-            from_rgb_refl(src + 3, e);
-            from_rgb_refl(src + 6, k);
-            specs.insert(specs.end(), e, e + NSPEC);
-            specs.insert(specs.end(), k, k + NSPEC);
-        }
+    // materials: conductor spectra via fromRGB(Reflectance); texture bindings
+    sc->h_mat_tex.assign(2 * (size_t)d->n_materials, -1);
+    std::vector<TexView> views(d->n_textures);
+    if (d->n_textures) {
+        std::memcpy(sc->h_mat_tex.data(), d->material_textures, 8 * (size_t)d->n_materials);
+        for (uint32_t t = 0; t < d->n_textures; t++) views[t] = *texture_view(d->textures[t]);
+        for (int32_t ti : sc->h_mat_tex) sc->textured = sc->textured || ti >= 0;
     }
+    v.textures = (const TexView*)up(views.data(), sizeof(TexView) * views.size());
+    std::vector<DevMaterial> mats;
+    std::vector<float> specs;
+    build_materials(d->materials, d->n_materials, sc->h_mat_tex.data(), mats, specs);
     v.materials = (const DevMaterial*)up(mats.data(), sizeof(DevMaterial) * mats.size());
     v.mat_spec = (const float*)up(specs.data(), 4 * specs.size());
     v.n_spec_floats = (uint32_t)specs.size();
@@ -335,6 +379,7 @@ wpbr_status scene_upload(wpbr_ctx* ctx, const wpbr_scene_desc* d, wpbr_scene** o
     const HostTables& T = host_tables();
     v.xyz = (const float*)up(T.xyz, sizeof(T.xyz));
     v.illum = (const float*)up(T.illum, sizeof(T.illum));
+    v.refl = (const float*)up(T.refl, sizeof(T.refl));
     v.raster_camera = (const float*)up(d->raster_camera, 64);
     v.camera_world = (const float*)up(d->camera_world, 48);
     v.width = d->width;

@@ -41,6 +41,9 @@ struct HostTables {
 const HostTables& host_tables();
 // fromRGB(rgb, Reflectance) (CSpectrumSampled.h:110-158).
 void from_rgb_refl(const float* rgb, float* s);
+void build_materials(const float* h, uint32_t nm, const int32_t* mat_tex, std::vector<DevMaterial>& mats,
+                     std::vector<float>& specs);
+const TexView* texture_view(const wpbr_texture* t);
 // Bilinear mip-0 lookup (CSTextureProcess.h:58-81), indices clamped to the texture.
 void env_bilerp_h(const float* rgbTex, int W, int H, float s_, float t_, float* out);
 // The env light's sampling distribution (CLightInfiniteArea.cpp:44-70).
@@ -60,6 +63,8 @@ struct wpbr_scene {
     // parameters (host copies) and the writable buffers behind sv's pointers
     std::vector<float> h_materials;  // n_materials x 9, the descriptor's layout
     std::vector<float> h_env;        // env_h x env_w x 3
+    std::vector<int32_t> h_mat_tex;  // n_materials x 2: metallic, roughness texture (-1 none)
+    bool textured = false;           // some material binds a texture
     wpbr::DevMaterial* d_materials = nullptr;
     float* d_mat_spec = nullptr;
     float* d_env_rgb = nullptr;

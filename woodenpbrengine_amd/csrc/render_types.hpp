@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "texture_device.hpp"
 #include "wpbr_internal.hpp"
 
 namespace wpbr {
@@ -14,6 +15,8 @@ struct DevMaterial {
     float alpha;
     float eta;      // dielectric etaT (etaI = 1)
     uint32_t spec;  // conductor: index of its eta/k spectra in mat_spec (2 x 64 floats)
+    int32_t tex_k;  // conductor: metallic texture (k = fromRGB of its value), -1 none
+    int32_t tex_r;  // conductor: roughness texture (alpha from its red value), -1 none
 };
 
 struct SceneView {
@@ -39,6 +42,8 @@ struct SceneView {
     const float* env_marg_int;   // 1
     const float* xyz;            // 3 x 64 (X, Y, Z)
     const float* illum;          // 7 x 64 Smits illuminant bases
+    const float* refl;           // 7 x 64 Smits reflectance bases
+    const TexView* textures;     // the scene's textures (textured metals)
     const float* raster_camera;  // 16
     const float* camera_world;   // 12
     uint32_t width, height;
@@ -83,6 +88,7 @@ struct WaveState {
     uint32_t* b_zero;
     float* b_wi;          // 3 x N world direction
     float* le;            // XYZ of the emitted light for primary misses (3 x N)
+    float* tk;            // 3 x N: metallic texture value at a textured conductor hit (its k RGB)
     // compacted secondary ray queue
     float* qrays;         // 6 x 2N
     uint32_t* qhit;       // 2N

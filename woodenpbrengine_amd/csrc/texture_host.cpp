@@ -84,6 +84,10 @@ struct wpbr_texture {
     DevTexture t{};
 };
 
+namespace wpbr {
+const TexView* texture_view(const wpbr_texture* t) { return &t->t.view; }
+}  // namespace wpbr
+
 extern "C" {
 
 wpbr_status wpbr_texture_create(wpbr_ctx* ctx, uint32_t width, uint32_t height, const float* rgb,

@@ -5,7 +5,7 @@
 // usage: wpbr_render_demo SCENE OUT [BSDF_IN BSDF_OUT]
 // SCENE (little-endian): the fields of wpbr_scene_desc in declaration order,
 //     each uint32_t field as 4 bytes and each array as u64 byte count + bytes
-//     (count 0: null).
+//     (count 0: null); no textures (n_textures 0).
 // OUT: the film (width*height*4 f32), then the 8-bit film (width*height*3 u8);
 //     the PNG of JobOutputFilm is written to OUT.png.
 // BSDF_IN: u32 n, f32 frame[9 n] (rows ss, ts, ns; SoA), f32 wo[3 n], f32 wi[3 n],
@@ -65,7 +65,12 @@ bool read_desc(Reader& r, wpbr_scene_desc& d) {
     r.u32(d.spp);
     r.u32(d.seed);
     r.u32(d.sampler);
-    return r.ok;
+    // textures are device objects: a serialised scene has none
+    const uint8_t* unused = nullptr;
+    r.u32(d.n_textures);
+    r.arr(unused);
+    r.arr(d.material_textures);
+    return r.ok && d.n_textures == 0;
 }
 
 template <class T>

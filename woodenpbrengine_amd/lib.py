@@ -94,7 +94,8 @@ class SceneDesc(C.Structure):
                 ("env_w", C.c_uint32), ("env_h", C.c_uint32), ("env_xform", C.c_void_p),
                 ("raster_camera", C.c_void_p), ("camera_world", C.c_void_p),
                 ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
-                ("seed", C.c_uint32), ("sampler", C.c_uint32)]
+                ("seed", C.c_uint32), ("sampler", C.c_uint32),
+                ("n_textures", C.c_uint32), ("textures", C.c_void_p), ("material_textures", C.c_void_p)]
 
 
 _lib = None
@@ -530,19 +531,24 @@ def write_png(path, rgb8):
         raise WpbrError(st, "cannot write %s" % path)
 
 
-def scene_desc(s):
-    """wpbr_scene_desc of a woodenpbrengine_amd.scenes.RenderScene; returns
+def scene_desc(s, textures=()):
+    """wpbr_scene_desc of a woodenpbrengine_amd.scenes.RenderScene and its
+    device textures (Texture objects, one per s.textures entry); returns
     (desc, keepalive arrays)."""
     keep = [np.ascontiguousarray(a) for a in (
         s.sphere_xform, s.sphere_radius, s.tri_verts, s.tri_flags, s.tri_uv, s.tri_normal,
         s.tri_tangent, s.shape_material, s.material_table, s.light_shape, s.light_emit,
         s.env_emit, s.env_rgb, s.env_xform, s.raster_camera, s.camera_world)]
     (sx, sr, tv, tf, tuv, tn, tt, sm, mt, ls, le, ee, er, ex, rc, cw) = keep
+    handles = (C.c_void_p * max(1, len(textures)))(*[t.h for t in textures])
+    mtex = np.ascontiguousarray(getattr(s, "material_textures", np.full((len(mt), 2), -1)), np.int32)
+    keep += [handles, mtex]
     d = SceneDesc(len(sr), _ptr(sx), _ptr(sr), len(tv), _ptr(tv), _ptr(tf), _ptr(tuv), _ptr(tn),
                   _ptr(tt), _ptr(sm), len(mt), _ptr(mt), len(ls), _ptr(ls), _ptr(le),
                   1 if s.env is not None else 0, _ptr(ee), _ptr(er), er.shape[1], er.shape[0],
                   _ptr(ex), _ptr(rc), _ptr(cw), s.width, s.height, s.spp, s.seed,
-                  getattr(s, "sampler", 0))
+                  getattr(s, "sampler", 0), len(textures), C.cast(handles, C.c_void_p) if textures else None,
+                  _ptr(mtex) if textures else None)
     return d, keep
 
 
@@ -555,7 +561,9 @@ class Renderer:
     def __init__(self, ctx, scene):
         self.ctx = ctx
         self.scene = scene
-        d, keep = scene_desc(scene)
+        # the device textures live as long as the scene
+        self.textures = [Texture(ctx, t) for t in getattr(scene, "textures", [])]
+        d, keep = scene_desc(scene, self.textures)
         h = C.c_void_p()
         ctx.check(ctx.lib.wpbr_scene_create(ctx.h, C.byref(d), C.byref(h)))
         del keep
@@ -665,6 +673,9 @@ class Renderer:
         if self.h:
             self.ctx.lib.wpbr_scene_destroy(self.h)
             self.h = None
+        for t in self.textures:
+            t.destroy()
+        self.textures = []
 
 
 class DiffRenderer(Renderer):
@@ -753,7 +764,8 @@ class BatchRenderer:
         self.ctx = ctx
         self.scenes = list(scenes)
         S = len(self.scenes)
-        descs, keeps = zip(*[scene_desc(s) for s in self.scenes])
+        self.textures = [[Texture(ctx, t) for t in getattr(s, "textures", [])] for s in self.scenes]
+        descs, keeps = zip(*[scene_desc(s, tx) for s, tx in zip(self.scenes, self.textures)])
         arr = (SceneDesc * S)(*descs)
         hs = (C.c_void_p * S)()
         ctx.check(ctx.lib.wpbr_scenes_create(ctx.h, S, arr, hs))
@@ -790,3 +802,7 @@ class BatchRenderer:
             if h:
                 self.ctx.lib.wpbr_scene_destroy(h)
         self.hs = []
+        for tx in self.textures:
+            for t in tx:
+                t.destroy()
+        self.textures = []

@@ -271,14 +271,28 @@ class RenderScene:
         self.width = self.height = self.spp = 0
         self.seed = 0
         self.sampler = 0         # 0 independent streams, 1 / 2 stratified arrays
+        self.textures = []       # RGB textures (h x w x 3 float32, linear)
+        self.mat_tex = []        # per material: (metallic, roughness) texture index, -1 none
 
-    def material(self, mtype, alpha=0.1, eta=1.5, eta_rgb=(1.5, 1.5, 1.5), k_rgb=(0, 0, 0)):
+    def material(self, mtype, alpha=0.1, eta=1.5, eta_rgb=(1.5, 1.5, 1.5), k_rgb=(0, 0, 0), tex_k=-1,
+                 tex_r=-1):
         m = np.zeros(9, np.float32)
         m[0] = np.array([mtype], np.uint32).view(np.float32)[0]
         m[1], m[2] = alpha, eta
         m[3:6], m[6:9] = eta_rgb, k_rgb
         self.materials.append(m)
+        self.mat_tex.append((tex_k, tex_r))
         return len(self.materials) - 1
+
+    def add_texture(self, rgb):
+        self.textures.append(np.ascontiguousarray(rgb, np.float32))
+        return len(self.textures) - 1
+
+    def textured_metal(self, tex_k, tex_r, eta=1.5):
+        """CMaterialMetal (CMaterialMetal.cpp:24-52): k from the metallic
+        texture, alpha from the roughness texture's red value, eta constant."""
+        return self.material(MAT_CONDUCTOR, alpha=0.1, eta=eta, eta_rgb=(eta, eta, eta), tex_k=tex_k,
+                             tex_r=tex_r)
 
     def add_sphere(self, center, radius, material=None):
         self.spheres.append((np.asarray(center, float), float(radius), material))
@@ -317,6 +331,7 @@ class RenderScene:
         self.shape_material = np.concatenate([sph_mat] + mats).astype(np.uint32)
         self.material_table = (np.stack(self.materials) if self.materials
                                else np.zeros((0, 9), np.float32))
+        self.material_textures = np.array(self.mat_tex, np.int32).reshape(-1, 2)
         self.light_shape = np.array([s for s, _ in self.lights], np.uint32)
         self.light_emit = (np.stack([e for _, e in self.lights]).astype(np.float32)
                            if self.lights else np.zeros((0, N_SPECTRAL), np.float32))
@@ -332,6 +347,55 @@ class RenderScene:
     @property
     def n_shapes(self):
         return len(self.sphere_radius) + len(self.tri_verts)
+
+
+def metallic_texture(width, height, seed):
+    """A checkerboard of two conductor k colours with noise (linear RGB)."""
+    g = rng(seed)
+    y, x = np.mgrid[0:height, 0:width]
+    cell = ((x * 8 // width) + (y * 4 // height)) % 2
+    a, b = np.array([3.6, 2.6, 2.3]), np.array([0.9, 1.4, 3.2])
+    rgb = np.where(cell[..., None] == 1, a, b) * (1 + 0.3 * g.random((height, width, 1)))
+    return rgb.astype(np.float32)
+
+
+def roughness_texture(width, height, seed):
+    """Smooth-to-rough stripes with noise in the red channel (0 .. 0.6)."""
+    g = rng(seed)
+    y, x = np.mgrid[0:height, 0:width]
+    r = 0.3 + 0.3 * np.sin(2 * np.pi * (3 * x / width + 2 * y / height)) * (0.7 + 0.3 * g.random((height, width)))
+    return np.stack([r, 0.5 * r, 0.25 * r], -1).astype(np.float32)
+
+
+def textured_scene(width=128, height=128, spp=4, seed=SEED_BASE + 5):
+    """The plain scene with textured metals (CMaterialMetal: metallic and
+    roughness textures, rectangular) on the ground quad (its UVs), on a sphere
+    (the sphere mapping) and on the icosphere (each triangle's default UVs)."""
+    g = rng(seed)
+    s = RenderScene()
+    tk = s.add_texture(metallic_texture(256, 128, seed + 1))
+    tr = s.add_texture(roughness_texture(64, 96, seed + 2))
+    tk2 = s.add_texture(metallic_texture(32, 32, seed + 3))
+    ground = s.textured_metal(tk, tr)
+    ball = s.textured_metal(tk2, tr, eta=1.2)
+    rough_only = s.textured_metal(-1, tr)
+    diel = s.material(MAT_DIELECTRIC, alpha=0.1, eta=1.5)
+    s.add_sphere((-1.6, -0.5, 0.4), 0.5, ball)
+    s.add_sphere((1.6, -0.5, -0.3), 0.5, diel)
+    s.add_light_sphere((0.6, 2.0, -1.2), 0.3, blackbody_spectrum(6500.0, 40.0))
+    ico = icosphere(8)
+    rot = random_rotations(g, 1)[0]
+    v = ico.reshape(-1, 3).astype(np.float64)
+    v = np.stack([rot[r, 0] * v[:, 0] + rot[r, 1] * v[:, 1] + rot[r, 2] * v[:, 2]
+                  for r in range(3)], axis=1)
+    s.add_mesh(v.reshape(-1, 9).astype(np.float32), rough_only)
+    qv, quv, qn, qt = ground_quad(12.0, 12.0, -1.0)
+    s.add_mesh(qv, ground, quv, qn, qt)
+    s.materials[rough_only][6:9] = CU_K
+    s.env = (env_map(256, 128, seed + 100), np.ones(N_SPECTRAL, np.float32), identity_xform())
+    s.cam = camera(width, height, (0.0, 0.9, -4.6), (0.0, -0.2, 0.0))
+    s.width, s.height, s.spp, s.seed = width, height, spp, seed
+    return s.finalize()
 
 
 def c0_scene(width=128, height=128, spp=4, seed=SEED_BASE + 0):
