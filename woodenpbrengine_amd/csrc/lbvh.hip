@@ -665,7 +665,12 @@ WPBR_D float wave_max_f(float v) {
     return v;
 }
 
-WPBR_D bbox tb_box(const SahLds& L, uint32_t t) {
+// The split helpers below read an LDS layout LDS with members tbox (6 floats
+// per treelet root), perm, bucket, la, lb (indexed by position) and bucket
+// statistics rows bcount / bmin / bmax: the workgroup's SahLds or one wave's
+// WaveRangeLds.
+template <class LDS>
+WPBR_D bbox tb_box(const LDS& L, uint32_t t) {
     const float* p = L.tbox + 6 * t;
     bbox b;
     b.mn[0] = p[0]; b.mn[1] = p[1]; b.mn[2] = p[2];
@@ -674,7 +679,8 @@ WPBR_D bbox tb_box(const SahLds& L, uint32_t t) {
 }
 
 // Treelet root centroid on one axis: (pMin + pMax) * 0.5 (SLBVH.cpp:472, 487).
-WPBR_D float tb_cen(const SahLds& L, uint32_t t, int dim) {
+template <class LDS>
+WPBR_D float tb_cen(const LDS& L, uint32_t t, int dim) {
     const float* p = L.tbox + 6 * t;
     return (float)((double)(p[dim] + p[3 + dim]) * 0.5);
 }
@@ -771,7 +777,8 @@ WPBR_D int sah_best(float cost) {
 // one prefix and one suffix sweep of unions: min/max are exact, so these boxes
 // equal the reference's per-split unions up to the sign of a zero coordinate,
 // which no extent (and hence no area or cost) can see.
-WPBR_D uint32_t lane_split(SahLds& L, uint32_t i0, uint32_t i1, int* dimOut, bbox* boundsOut) {
+template <class LDS>
+WPBR_D uint32_t lane_split(LDS& L, uint32_t i0, uint32_t i1, int* dimOut, bbox* boundsOut) {
     bbox bounds = box_empty(), cb = box_empty();
     for (uint32_t i = i0; i <= i1; i++) {
         uint32_t t = L.perm[i];
@@ -983,10 +990,11 @@ WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint3
     __syncthreads();
 }
 
-// One SAH split of perm[i0..i1] by one wave.
-WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint32_t i0, uint32_t i1, uint32_t dep,
-                           uint32_t rt) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// One SAH split of perm[i0..i1] by one wave with bucket statistics row `wave`;
+// returns the split position (every lane), its axis and the range's bounds.
+template <class LDS>
+WPBR_D uint32_t wave_split_mid(LDS& L, int wave, uint32_t i0, uint32_t i1, int* dimOut, bbox* boundsOut) {
+    const int lane = threadIdx.x & 63;
     float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
     float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
     for (uint32_t i = i0 + lane; i <= i1; i += 64) {
@@ -1086,6 +1094,18 @@ WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint32
         __builtin_amdgcn_wave_barrier();
         mid = i0 + T;
     }
+    *dimOut = dim;
+    *boundsOut = bounds;
+    return mid;
+}
+
+// One SAH split of perm[i0..i1] by one wave, its node and its two child tasks.
+WPBR_D void sah_wave_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint32_t i0, uint32_t i1, uint32_t dep,
+                           uint32_t rt) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int dim;
+    bbox bounds;
+    const uint32_t mid = wave_split_mid(L, wave, i0, i1, &dim, &bounds);
     if (lane == 0) {
         sah_node(s, TL, i0, i1, mid, dim, bounds, dep, rt);
         sah_children(TL, L, i0, i1, mid, dep, rt);
@@ -1934,7 +1954,7 @@ __global__ __launch_bounds__(256) void k_bvh4(LbvhScratch s) { bvh4_body(s, bloc
 //                   std::partition in rank form (the r-th misplaced position
 //                   left of the split swaps with the r-th misplaced position
 //                   right of it counted from the right)
-//   kr_sub          ranges below REF_CUT, one workgroup each in LDS
+//   kr_sub_wave     ranges below REF_CUT, one wave each in LDS
 //   kr_fin_*        preorder layout with a device-wide prefix of the treelet
 //                   node counts; kr_scatter_ref writes the treelet nodes
 // Ranges and chunks of level k live in list k & 1; the level's split pushes
@@ -2553,57 +2573,141 @@ __global__ void kr_active(const LbvhScratch* S, uint32_t n_scenes, uint32_t list
     if (threadIdx.x == 0) *out = s_t;
 }
 
-// ---- ranges below REF_CUT: one workgroup each, in LDS ----------------------
-// The range's positions are local 0 .. m - 1 (their boxes staged from gp_box);
-// the recorded nodes and leaf depths are offset back to the range.
-WPBR_D void sah_sub_ref_one(LbvhScratch& s, SahSubLds& S, uint32_t st) {
-    SahLds& L = S.L;
-    const int tid = threadIdx.x;
-    SahTasks T{S.ti0, S.ti1, S.tdep, S.trt, &S.uctr};
-    if (tid == 0) {
-        S.r[0] = s.sub_i0[st];
-        S.r[1] = s.sub_i1[st];
-        S.r[2] = s.sub_dep[st];
-        S.r[3] = s.sub_rt[st];
-        S.ti0[0] = 0;
-        S.ti1[0] = S.r[1] - S.r[0];
-        S.tdep[0] = 0;
-        S.trt[0] = 0;
-        S.uctr = s.sub_ub[st];
-        L.task_next = 1;
-        L.nbig = 0;
-        L.nbig_next = 0;
+
+// ---- ranges below REF_CUT: one wave each ---------------------------------------
+// Each wave of a workgroup takes its own deferred range (at most WR_CAP roots)
+// in its own LDS: a stack of pending tasks, split by the whole wave while
+// larger than WR_LANE_MAX roots; the small tasks then go to one lane each, to
+// completion.  Tasks are packed as i0 | i1 << 8 | depth << 16 | right turns << 24,
+// all relative to the range (a range of m roots has depth below m).
+constexpr uint32_t WR_CAP = 256;
+constexpr uint32_t WR_WAVES = 4;
+constexpr uint32_t WR_LANE_MAX = 8;
+struct WaveRangeLds {
+    alignas(16) float tbox[6 * WR_CAP];
+    uint16_t perm[WR_CAP];
+    uint8_t bucket[WR_CAP];
+    uint16_t la[WR_CAP], lb[WR_CAP];     // partition lists; then leaf depth / right turns
+    uint32_t stack[WR_CAP];              // pending tasks
+    uint32_t small[WR_CAP];              // tasks for the lanes
+    uint32_t aux[LANE_STACK * 64];       // the lanes' DFS stacks; then the range's old g_perm
+    uint32_t bcount[1][NB];
+    uint32_t bmin[1][NB][3];
+    uint32_t bmax[1][NB][3];
+    uint32_t uctr;
+};
+static_assert(LANE_STACK * 64 >= WR_CAP, "aux holds a range's old g_perm");
+static_assert(REF_CUT <= WR_CAP, "a deferred range fits a wave's LDS");
+
+WPBR_D uint32_t wr_pack(uint32_t a, uint32_t b, uint32_t d, uint32_t r) { return a | (b << 8) | (d << 16) | (r << 24); }
+
+// One small task (at most WR_LANE_MAX roots) by one lane, to completion: the
+// larger child of a split waits on the lane's stack.
+WPBR_D void wr_lane_task(LbvhScratch& s, WaveRangeLds& L, const SahTasks& T, uint32_t task, int lane) {
+    const uint32_t t0 = task & 255u, t1 = (task >> 8) & 255u, d0 = (task >> 16) & 255u, r0 = task >> 24;
+    uint32_t a = t0, b = t1, dd = 0, dr = 0;
+    int sp = 0;
+    while (true) {
+        if (a == b) {
+            L.la[a] = (uint16_t)(d0 + dd);
+            L.lb[a] = (uint16_t)(r0 + dr);
+            if (sp == 0) break;
+            const uint32_t e = L.aux[(--sp) * 64 + lane];
+            a = t0 + (e & 31u);
+            b = t0 + ((e >> 5) & 31u);
+            dd = (e >> 10) & 31u;
+            dr = (e >> 15) & 31u;
+            continue;
+        }
+        int dim;
+        bbox bounds;
+        const uint32_t mid = lane_split(L, a, b, &dim, &bounds);
+        sah_node(s, T, a, b, mid, dim, bounds, d0 + dd, r0 + dr);
+        const uint32_t nl = mid - a, nr = b + 1 - mid;
+        uint32_t pa, pb, pr;  // the child that waits (depth dd + 1)
+        if (nl <= nr) {
+            pa = mid; pb = b; pr = dr + 1;
+            b = mid - 1;
+        } else {
+            pa = a; pb = mid - 1; pr = dr;
+            a = mid; dr = dr + 1;
+        }
+        dd++;
+        if (sp < LANE_STACK)
+            L.aux[(sp++) * 64 + lane] = (pa - t0) | ((pb - t0) << 5) | (dd << 10) | (pr << 15);
+        else
+            atomicOr(s.err, 4u);
     }
-    __syncthreads();
-    const uint32_t i0 = S.r[0], m = S.r[1] - S.r[0] + 1;
+}
+
+WPBR_D void wr_range(LbvhScratch& s, WaveRangeLds& L, uint32_t st) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t i0 = s.sub_i0[st], m = s.sub_i1[st] - i0 + 1;
+    SahTasks T{nullptr, nullptr, nullptr, nullptr, &L.uctr};
     T.pos_base = i0;
-    T.dep_base = S.r[2];
-    T.rt_base = S.r[3];
-    for (uint32_t q = tid; q < m; q += SAH_BLOCK) {
+    T.dep_base = s.sub_dep[st];
+    T.rt_base = s.sub_rt[st];
+    if (m > WR_CAP) {  // the level kernels defer only ranges below REF_CUT
+        if (lane == 0) atomicOr(s.err, 4u);
+        return;
+    }
+    for (uint32_t q = lane; q < m; q += 64) {
         L.perm[q] = (uint16_t)q;
         for (int c = 0; c < 6; c++) L.tbox[6 * q + c] = s.gp_box[6 * (size_t)(i0 + q) + c];
     }
-    __threadfence_block();
-    __syncthreads();
-    sah_run(s, L, T, 0, 1, LANE_TASK_MAX_SUB, 0);
-    for (uint32_t q = tid; q < m; q += SAH_BLOCK) L.aux[q] = s.g_perm[i0 + q];
-    __syncthreads();
-    for (uint32_t q = tid; q < m; q += SAH_BLOCK) {
+    if (lane == 0) {
+        L.uctr = s.sub_ub[st];
+        L.stack[0] = wr_pack(0, m - 1, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t sp = 1, nsmall = 0;  // uniform across the wave
+    while (sp || nsmall) {
+        if (sp && nsmall < 64) {
+            const uint32_t task = L.stack[--sp];
+            const uint32_t a = task & 255u, b = (task >> 8) & 255u, d = (task >> 16) & 255u, r = task >> 24;
+            __builtin_amdgcn_wave_barrier();
+            if (b - a + 1 <= WR_LANE_MAX) {
+                if (lane == 0) L.small[nsmall] = task;
+                nsmall++;
+            } else {
+                int dim;
+                bbox bounds;
+                const uint32_t mid = wave_split_mid(L, 0, a, b, &dim, &bounds);
+                if (lane == 0) {
+                    sah_node(s, T, a, b, mid, dim, bounds, d, r);
+                    L.stack[sp] = wr_pack(a, mid - 1, d + 1, r);
+                    L.stack[sp + 1] = wr_pack(mid, b, d + 1, r + 1);
+                }
+                sp += 2;
+            }
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
+        // a batch of up to 64 small tasks, one per lane
+        const uint32_t nb = nsmall < 64 ? nsmall : 64;
+        if ((uint32_t)lane < nb) wr_lane_task(s, L, T, L.small[nsmall - 1 - lane], lane);
+        nsmall -= nb;
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (uint32_t q = lane; q < m; q += 64) L.aux[q] = s.g_perm[i0 + q];
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t q = lane; q < m; q += 64) {
         s.g_perm[i0 + q] = L.aux[L.perm[q]];
         s.g_la[i0 + q] = T.dep_base + L.la[q];
         s.g_lb[i0 + q] = T.rt_base + L.lb[q];
     }
-    __syncthreads();
-}
-static_assert(REF_CUT <= SAH_CUT, "deferred ranges fit the sub-range kernel's task list");
-__global__ __launch_bounds__(SAH_BLOCK) void kr_sub(const LbvhScratch* S, uint32_t G) {
-    __shared__ SahSubLds L;
-    const uint32_t sc = blockIdx.x / G, rj = blockIdx.x - sc * G;
-    LbvhScratch s = S[sc];
-    const uint32_t nsub = *s.sub_n;
-    for (uint32_t st = rj; st < nsub; st += G) sah_sub_ref_one(s, L, st);
+    __builtin_amdgcn_wave_barrier();
 }
 
+// G workgroups per scene, WR_WAVES ranges in flight per workgroup.
+__global__ __launch_bounds__(64 * WR_WAVES) void kr_sub_wave(const LbvhScratch* S, uint32_t G) {
+    __shared__ WaveRangeLds W[WR_WAVES];
+    const uint32_t sc = blockIdx.x / G, rj = blockIdx.x - sc * G;
+    LbvhScratch s = S[sc];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint32_t nsub = *s.sub_n;
+    for (uint32_t st = rj * WR_WAVES + wave; st < nsub; st += G * WR_WAVES) wr_range(s, W[wave], st);
+}
 // ---- preorder layout ---------------------------------------------------------
 // P = exclusive prefix of t_nodes over final positions; a treelet at final
 // position i with leaf depth d and r right turns starts at d + (i - r) + P[i],
@@ -2730,7 +2834,7 @@ static RefGrid ref_grid(uint32_t max_n, uint32_t n_scenes) {
     g.treelets = g.slots;
     g.emit_big = clampu(max_n / 512 + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.level = clampu(max_n / REF_CHUNK + 1, 1, n_scenes > 1 ? 64u : 1024u);
-    g.sub = clampu(max_n / 512 + 1, 1, n_scenes > 1 ? 16u : 256u);
+    g.sub = clampu(max_n / (64 * WR_WAVES) + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.fin = clampu(max_n / REF_FIN_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.upper = clampu(max_n / REF_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
     return g;
@@ -2771,7 +2875,7 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
         // level; deeper chains of ranges of at least REF_CUT roots are refused
         if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
     }
-    hipLaunchKernelGGL(kr_sub, dim3(ns * g.sub), dim3(SAH_BLOCK), 0, st, S, g.sub);
+    hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
     hipLaunchKernelGGL(kr_fin_sum, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
     hipLaunchKernelGGL(kr_fin_scan, dim3(ns), dim3(1024), 0, st, S);
     hipLaunchKernelGGL(kr_fin_place, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
