@@ -137,6 +137,7 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.head_blk = c.take<uint32_t>((N + 255) / 256);
     s.bigt = c.take<uint32_t>(N / 30 + 1);
     s.bigt_n = c.take<uint32_t>(1);
+    s.big_pre = c.take<uint32_t>(N / 30 + 2);
     s.gp_box = c.take<float>(6 * (size_t)CT);
     s.gp_bucket = c.take<uint8_t>(CT);
     s.pl_a = c.take<uint32_t>(CT);

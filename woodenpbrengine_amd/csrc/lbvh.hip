@@ -2797,27 +2797,77 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_fin_upper(const LbvhScratch* S, 
         store_box(s.node_bounds + 6 * (size_t)off, load_box(s.u_box + 6 * (size_t)u));
     }
 }
-// Flatten (SLBVH.cpp:432-455) of the treelet nodes, one thread per treelet.
+// Flatten (SLBVH.cpp:432-455) of the treelet nodes: record q of treelet t to
+// its final position.
+WPBR_D void scatter_record(const LbvhScratch& s, uint32_t t, uint32_t q) {
+    const uint32_t base = s.t_slot[t], fbase = s.t_final[t];
+    const uint32_t meta = s.r_meta[base + q];
+    const uint32_t fi = fbase + s.r_off[base + q];
+    store_box(s.node_bounds + 6 * (size_t)fi, load_box(s.r_box + 6 * (size_t)(base + q)));
+    if (meta & 0x80000000u) {
+        s.node_offset[fi] = s.r_start[base + q];
+        s.node_nshapes[fi] = s.r_n[base + q];
+        s.node_axis[fi] = 0;
+    } else {
+        const uint32_t c0 = s.r_c0[base + q];
+        s.node_offset[fi] = fbase + s.r_off[base + c0 + 1];
+        s.node_nshapes[fi] = 0;
+        s.node_axis[fi] = (uint8_t)((meta >> 8) & 3u);
+    }
+}
+// Treelets of one node, one thread each.
 __global__ __launch_bounds__(REF_BLOCK) void kr_scatter_ref(const LbvhScratch* S, uint32_t G) {
     REF_SCENE(S, G);
     const uint32_t T = *s.n_treelets;
-    for (uint32_t t = rj * REF_BLOCK + threadIdx.x; t < T; t += G * REF_BLOCK) {
-        const uint32_t base = s.t_slot[t], cnt = s.t_nodes[t], fbase = s.t_final[t];
-        for (uint32_t q = 0; q < cnt; q++) {
-            const uint32_t meta = s.r_meta[base + q];
-            const uint32_t fi = fbase + s.r_off[base + q];
-            store_box(s.node_bounds + 6 * (size_t)fi, load_box(s.r_box + 6 * (size_t)(base + q)));
-            if (meta & 0x80000000u) {
-                s.node_offset[fi] = s.r_start[base + q];
-                s.node_nshapes[fi] = s.r_n[base + q];
-                s.node_axis[fi] = 0;
-            } else {
-                const uint32_t c0 = s.r_c0[base + q];
-                s.node_offset[fi] = fbase + s.r_off[base + c0 + 1];
-                s.node_nshapes[fi] = 0;
-                s.node_axis[fi] = (uint8_t)((meta >> 8) & 3u);
-            }
+    for (uint32_t t = rj * REF_BLOCK + threadIdx.x; t < T; t += G * REF_BLOCK)
+        if (s.t_nodes[t] == 1) scatter_record(s, t, 0);
+}
+// Exclusive prefix of the node counts of the big treelets with more than one
+// node (one workgroup per scene; each thread sums a run of the list).
+__global__ __launch_bounds__(1024) void kr_big_scan(const LbvhScratch* S) {
+    const LbvhScratch& s = S[blockIdx.x];
+    __shared__ uint32_t s_w[16];
+    const uint32_t nb = *s.bigt_n, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t per = (nb + 1023) / 1024, b0 = min(nb, tid * per), b1 = min(nb, b0 + per);
+    auto nodes = [&](uint32_t q) {
+        const uint32_t c = s.t_nodes[s.bigt[q]];
+        return c > 1 ? c : 0u;
+    };
+    uint32_t loc = 0;
+    for (uint32_t q = b0; q < b1; q++) loc += nodes(q);
+    uint32_t inc = loc;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t x = __shfl_up(inc, off, 64);
+        if (lane >= (uint32_t)off) inc += x;
+    }
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (uint32_t k = 0; k < 16; k++) {
+        before += k < w ? s_w[k] : 0u;
+        total += s_w[k];
+    }
+    uint32_t run = before + inc - loc;
+    for (uint32_t q = b0; q < b1; q++) {
+        s.big_pre[q] = run;
+        run += nodes(q);
+    }
+    if (tid == 0) s.big_pre[nb] = total;
+}
+// The big treelets' records, spread over every workgroup of the scene: record
+// g of the flattened list belongs to the big treelet q with big_pre[q] <= g <
+// big_pre[q + 1] (binary search).
+__global__ __launch_bounds__(REF_BLOCK) void kr_scatter_big(const LbvhScratch* S, uint32_t G) {
+    REF_SCENE(S, G);
+    const uint32_t nb = *s.bigt_n, total = s.big_pre[nb];
+    for (uint32_t g = rj * REF_BLOCK + threadIdx.x; g < total; g += G * REF_BLOCK) {
+        uint32_t lo = 0, hi = nb;  // big_pre[lo] <= g < big_pre[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (s.big_pre[mid] <= g) lo = mid;
+            else hi = mid;
         }
+        scatter_record(s, s.bigt[lo], g - s.big_pre[lo]);
     }
 }
 
@@ -3140,6 +3190,8 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
         wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st);
         if (e) return e;
         hipLaunchKernelGGL(kr_scatter_ref, dim3(S * g.upper), dim3(REF_BLOCK), 0, st, B.s, g.upper);
+        hipLaunchKernelGGL(kr_big_scan, dim3(S), dim3(1024), 0, st, B.s);
+        hipLaunchKernelGGL(kr_scatter_big, dim3(S * g.upper), dim3(REF_BLOCK), 0, st, B.s, g.upper);
     } else {
         hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
         hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
@@ -3215,6 +3267,8 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const G
                 if (e) return e;
                 rec(9);
                 hipLaunchKernelGGL(kr_scatter_ref, dim3(rg.upper), dim3(REF_BLOCK), 0, st, d_self, rg.upper);
+                hipLaunchKernelGGL(kr_big_scan, dim3(1), dim3(1024), 0, st, d_self);
+                hipLaunchKernelGGL(kr_scatter_big, dim3(rg.upper), dim3(REF_BLOCK), 0, st, d_self, rg.upper);
             } else {
                 hipLaunchKernelGGL(k_sah_top, dim3(1), dim3(SAH_BLOCK), 0, st, s);
                 hipLaunchKernelGGL(k_sah_sub, dim3(SAH_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, s);

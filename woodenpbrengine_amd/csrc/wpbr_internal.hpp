@@ -85,6 +85,7 @@ struct LbvhScratch {
     uint32_t* head_blk;   // per 256 slots: treelet heads, then their exclusive prefix
     uint32_t* bigt;       // treelets of at least 30 shapes (one workgroup each)
     uint32_t* bigt_n;
+    uint32_t* big_pre;    // bigt_n + 1: exclusive prefix of the big treelets' node counts (flatten)
     // level-synchronous upper SAH over the ranges of at least REF_CUT positions
     float* gp_box;        // cap_t x 6: root box of the treelet at each position
     uint8_t* gp_bucket;   // cap_t: SAH bucket of each position
