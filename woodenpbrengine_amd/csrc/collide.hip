@@ -190,6 +190,15 @@ WPBR_D void ray_end(const RayState& R, const TraceArgs& a, uint32_t i, size_t rs
     }
 }
 
+// Speculative traversal (Aila and Laine 2009): a lane that reaches a primitive
+// range while other lanes of its wave are still looking for one parks it and
+// keeps walking records; the wave turns to the primitives once every lane holds
+// a range (or has none left).  Ranges are still tested in depth-first order
+// (the parked one first), and records walked while a range is parked are tested
+// against a best t that is not yet lowered by it, so no record that could hold
+// the winner is skipped: the closest hit is unchanged.
+constexpr bool TRACE_SPECULATIVE = true;
+
 // One step of the while-while loop: records until a primitive range (or the
 // end), then primitive ranges until a record (or the end).
 template <int BLOCK, int SD, int WIDTH>
@@ -197,10 +206,11 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
     uint32_t node = R.node;
     const f3 o = R.o, inv = R.inv;
     const bool nx = R.nx, ny = R.ny, nz = R.nz;
-    if (WIDTH == 2) {
-        // Binary records: test both children, near first by dirIsNeg[axis].
-        while (node != NODE_DONE && !(node & 0x80000000u)) {
-            const float4* r = a.trav + 4 * (size_t)node;
+    // the next node after record `nd`: its nearest hit child, the others pushed
+    auto visit = [&](uint32_t nd) -> uint32_t {
+        if (WIDTH == 2) {
+            // Binary records: test both children, near first by dirIsNeg[axis].
+            const float4* r = a.trav + 4 * (size_t)nd;
             float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
             float lim = R.tBest * PRUNE_SLACK;
             bool h0 = slab_hit(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, o, inv, nx, ny, nz, lim);
@@ -210,17 +220,15 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
             bool negAxis = (axis == 0) ? nx : ((axis == 1) ? ny : nz);
             uint32_t nearW = negAxis ? w1 : w0, farW = negAxis ? w0 : w1;
             bool hn = negAxis ? h1 : h0, hf = negAxis ? h0 : h1;
-            if (hn && hf) { S.push(R.sp, farW); node = nearW; }
-            else if (hn) node = nearW;
-            else if (hf) node = farW;
-            else node = S.pop(R.sp);
-        }
-    } else {
-        // 4-wide records (the grandchildren of a binary record): the pair of the
-        // near child first, the near grandchild first inside each pair; hits
-        // after the first are pushed in reverse visiting order.
-        while (node != NODE_DONE && !(node & 0x80000000u)) {
-            const float4* r = a.trav4 + 8 * (size_t)node;
+            if (hn && hf) { S.push(R.sp, farW); return nearW; }
+            if (hn) return nearW;
+            if (hf) return farW;
+            return S.pop(R.sp);
+        } else {
+            // 4-wide records (the grandchildren of a binary record): the pair of the
+            // near child first, the near grandchild first inside each pair; hits
+            // after the first are pushed in reverse visiting order.
+            const float4* r = a.trav4 + 8 * (size_t)nd;
             float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5], r6 = r[6], r7 = r[7];
             float lim = R.tBest * PRUNE_SLACK;
             uint32_t w[4] = {__float_as_uint(r6.x), __float_as_uint(r6.y), __float_as_uint(r6.z),
@@ -249,12 +257,23 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
                     next = wq;
                 }
             }
-            node = (next != NODE_DONE) ? next : S.pop(R.sp);
+            return (next != NODE_DONE) ? next : S.pop(R.sp);
         }
+    };
+    uint32_t parked = NODE_DONE;
+    if (TRACE_SPECULATIVE) {
+        while (true) {
+            if (node != NODE_DONE && !(node & 0x80000000u)) node = visit(node);
+            else if (node != NODE_DONE && parked == NODE_DONE) { parked = node; node = S.pop(R.sp); }
+            if (__all(parked != NODE_DONE || node == NODE_DONE)) break;
+        }
+    } else {
+        while (node != NODE_DONE && !(node & 0x80000000u)) node = visit(node);
     }
-    // Primitive ranges.
-    while (node != NODE_DONE && (node & 0x80000000u)) {
-        uint32_t first = node & 0x0FFFFFFFu, cnt = ((node >> 28) & 7u) + 1u;
+    // The primitives of range `leaf`; true when the ray is settled or restarts
+    // (R.node set).
+    auto leaf_prims = [&](uint32_t leaf_word) -> bool {
+        uint32_t first = leaf_word & 0x0FFFFFFFu, cnt = ((leaf_word >> 28) & 7u) + 1u;
         for (uint32_t j = first; j < first + cnt; j++) {
             const float4* p = a.prims + 3 * (size_t)j;
             float4 v0 = p[0], v1 = p[1], v2 = p[2];
@@ -273,7 +292,7 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
                         R.tBest = t;
                         R.node = NODE_DONE;
                         R.sp = 0;
-                        return;
+                        return true;
                     }
                     // equal t: the candidate order decides; redo as a closest-hit query
                     R.occ = false;
@@ -281,7 +300,7 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
                     R.best = R.bestSlot = R.bestLeaf = 0xFFFFFFFFu;
                     R.sp = 0;
                     R.node = a.root_word;
-                    return;
+                    return true;
                 }
                 continue;
             }
@@ -295,6 +314,12 @@ WPBR_D void ray_step(RayState& R, const TraceArgs& a, LaneStack<BLOCK, SD>& S) {
                 R.bb0 = b3[0]; R.bb1 = b3[1]; R.bb2 = b3[2];
             }
         }
+        return false;
+    };
+    // Primitive ranges in depth-first order: the parked one, then the current run.
+    if (parked != NODE_DONE && leaf_prims(parked)) return;
+    while (node != NODE_DONE && (node & 0x80000000u)) {
+        if (leaf_prims(node)) return;
         node = S.pop(R.sp);
     }
     R.node = node;
