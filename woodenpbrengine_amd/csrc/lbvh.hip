@@ -907,7 +907,13 @@ WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint3
     const int dim = box_max_extent(cb);
     const float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
     const float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
-    for (uint32_t i = i0 + tid; i <= i1; i += SAH_BLOCK) {
+    // each thread takes an odd-length run of positions: the lanes of one
+    // instruction then hold treelets far apart, in different buckets, so fewer
+    // of their LDS atomics meet on one address
+    const uint32_t run = ((i1 - i0) / SAH_BLOCK + 1) | 1u;
+    for (uint32_t j = 0; j < run; j++) {
+        const uint32_t i = i0 + tid * run + j;
+        if (i > i1) break;
         uint32_t tr = L.perm[i];
         bbox rb = tb_box(L, tr);
         uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
@@ -990,8 +996,19 @@ WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint3
     __syncthreads();
 }
 
-// One SAH split of perm[i0..i1] by one wave with bucket statistics row `wave`;
-// returns the split position (every lane), its axis and the range's bounds.
+// Ranges up to this size get their split costs lane by lane (below); larger
+// ones through bucket statistics.
+constexpr uint32_t SPLIT_LANES_MAX = 64;
+
+// One SAH split of perm[i0..i1] by one wave; returns the split position (every
+// lane), its axis and the range's bounds.  Split costs (SLBVH.cpp:504-538):
+//  - ranges of at most SPLIT_LANES_MAX treelets: lane 4s + q unions the boxes
+//    of positions i0 + q, i0 + q + 4, ... on either side of split s, and the
+//    four lanes of split s merge theirs; no bucket statistics, no atomics;
+//  - larger ranges: bucket statistics in row `wave` (LDS atomics), then split
+//    s's unions of buckets 0..s and s+1..NB-1 by lane s.
+// Either way a side's box is the union of the same treelet boxes (min/max are
+// exact), so the costs, and the first minimum, are those of the reference.
 template <class LDS>
 WPBR_D uint32_t wave_split_mid(LDS& L, int wave, uint32_t i0, uint32_t i1, int* dimOut, bbox* boundsOut) {
     const int lane = threadIdx.x & 63;
@@ -1012,58 +1029,103 @@ WPBR_D uint32_t wave_split_mid(LDS& L, int wave, uint32_t i0, uint32_t i1, int* 
     bbox bounds, cb;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        bounds.mn[c] = wave_min_f(bmn[c]);
-        bounds.mx[c] = wave_max_f(bmx[c]);
-        cb.mn[c] = wave_min_f(cmn[c]);
-        cb.mx[c] = wave_max_f(cmx[c]);
+        bounds.mn[c] = wave_min_dpp(bmn[c]);
+        bounds.mx[c] = wave_max_dpp(bmx[c]);
+        cb.mn[c] = wave_min_dpp(cmn[c]);
+        cb.mx[c] = wave_max_dpp(cmx[c]);
     }
     int dim = box_max_extent(cb);
     float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
     float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
-    // per-wave statistics in row `wave`, merged into a private row 0 copy below
-    if (lane < NB) {
-        L.bcount[wave][lane] = 0;
-        for (int c = 0; c < 3; c++) {
-            L.bmin[wave][lane][c] = f2ord(FLT_MAX_);
-            L.bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    for (uint32_t i = i0 + lane; i <= i1; i += 64) {
-        uint32_t tr = L.perm[i];
-        bbox rb = tb_box(L, tr);
-        uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
-        L.bucket[i] = (uint8_t)ib;
-        atomicAdd(&L.bcount[wave][ib], 1u);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            atomicMin(&L.bmin[wave][ib][c], f2ord(rb.mn[c]));
-            atomicMax(&L.bmax[wave][ib][c], f2ord(rb.mx[c]));
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
+    const uint32_t size = i1 - i0 + 1;
+    const bool by_lane = size <= SPLIT_LANES_MAX;
     float cost = INF;
     uint32_t cnt_le = 0;
-    if (lane < NB - 1) {
+    if (by_lane) {
+        for (uint32_t i = i0 + lane; i <= i1; i += 64)
+            L.bucket[i] = (uint8_t)sah_bucket(tb_cen(L, L.perm[i], dim), cbmn, dist);
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t sp = (uint32_t)lane >> 2, q = (uint32_t)lane & 3u;
         bbox b0 = box_empty(), b1 = box_empty();
         uint32_t c0 = 0, c1 = 0;
-        for (int j = 0; j < NB; j++) {
-            bbox bj;
-            for (int c = 0; c < 3; c++) {
-                bj.mn[c] = ord2f(L.bmin[wave][j][c]);
-                bj.mx[c] = ord2f(L.bmax[wave][j][c]);
+        if (sp < NB - 1)
+            for (uint32_t i = i0 + q; i <= i1; i += 4) {
+                const bbox tb = tb_box(L, L.perm[i]);
+                if (L.bucket[i] <= sp) { b0 = box_union(b0, tb); c0++; }
+                else { b1 = box_union(b1, tb); c1++; }
             }
-            if (j <= lane) { b0 = box_union(b0, bj); c0 += L.bcount[wave][j]; }
-            else { b1 = box_union(b1, bj); c1 += L.bcount[wave][j]; }
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            b0.mn[c] = smin(b0.mn[c], quad_xor1(b0.mn[c])); b0.mx[c] = smax(b0.mx[c], quad_xor1(b0.mx[c]));
+            b1.mn[c] = smin(b1.mn[c], quad_xor1(b1.mn[c])); b1.mx[c] = smax(b1.mx[c], quad_xor1(b1.mx[c]));
         }
-        float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
-        float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
-        cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
-        cnt_le = c0;
+        c0 += quad_xor1(c0); c1 += quad_xor1(c1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            b0.mn[c] = smin(b0.mn[c], quad_xor2(b0.mn[c])); b0.mx[c] = smax(b0.mx[c], quad_xor2(b0.mx[c]));
+            b1.mn[c] = smin(b1.mn[c], quad_xor2(b1.mn[c])); b1.mx[c] = smax(b1.mx[c], quad_xor2(b1.mx[c]));
+        }
+        c0 += quad_xor2(c0); c1 += quad_xor2(c1);
+        if (sp < NB - 1) {
+            float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+            float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+            cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
+            cnt_le = c0;
+        }
+    } else {
+        // per-wave statistics in row `wave`
+        if (lane < NB) {
+            L.bcount[wave][lane] = 0;
+            for (int c = 0; c < 3; c++) {
+                L.bmin[wave][lane][c] = f2ord(FLT_MAX_);
+                L.bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // odd-length runs per lane, as in the workgroup split
+        const uint32_t run = ((size - 1) / 64 + 1) | 1u;
+        for (uint32_t j = 0; j < run; j++) {
+            const uint32_t i = i0 + (uint32_t)lane * run + j;
+            if (i > i1) break;
+            uint32_t tr = L.perm[i];
+            bbox rb = tb_box(L, tr);
+            uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
+            L.bucket[i] = (uint8_t)ib;
+            atomicAdd(&L.bcount[wave][ib], 1u);
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                atomicMin(&L.bmin[wave][ib][c], f2ord(rb.mn[c]));
+                atomicMax(&L.bmax[wave][ib][c], f2ord(rb.mx[c]));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane < NB - 1) {
+            bbox b0 = box_empty(), b1 = box_empty();
+            uint32_t c0 = 0, c1 = 0;
+            for (int j = 0; j < NB; j++) {
+                bbox bj;
+                for (int c = 0; c < 3; c++) {
+                    bj.mn[c] = ord2f(L.bmin[wave][j][c]);
+                    bj.mx[c] = ord2f(L.bmax[wave][j][c]);
+                }
+                if (j <= lane) { b0 = box_union(b0, bj); c0 += L.bcount[wave][j]; }
+                else { b1 = box_union(b1, bj); c1 += L.bcount[wave][j]; }
+            }
+            float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+            float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+            cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
+            cnt_le = c0;
+        }
     }
-    int best = sah_best(cost);
-    uint32_t T = (uint32_t)__shfl((int)cnt_le, best);
-    uint32_t size = i1 - i0 + 1;
+    // first minimum over the NB - 1 costs (in lanes 4s, or lanes s)
+    const int cs = by_lane ? 4 : 1;
+    int best = 0;
+    float cbest = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cost), 0));
+    for (int sp = 1; sp < NB - 1; sp++) {
+        const float c = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cost), sp * cs));
+        if (c < cbest) { best = sp; cbest = c; }
+    }
+    const uint32_t T = (uint32_t)__builtin_amdgcn_readlane((int)cnt_le, best * cs);
     uint32_t mid;
     if (T == size) {
         mid = size / 2 + i0;
@@ -2281,15 +2343,21 @@ WPBR_D RefRange ref_range(const LbvhScratch& s, uint32_t list, uint32_t c) {
 // Bucket of every position and the bucket statistics of every range
 // (SLBVH.cpp:465-500): counts, bucket boxes, bucket centroid bounds (for the
 // children's centroid bounds) and the range's box.
+// The workgroup keeps ST_COPIES copies of the statistics, lane l adding to copy
+// l % ST_COPIES: neighbouring positions mostly share a bucket, and the copies
+// spread their LDS atomics over ST_COPIES addresses.
+constexpr uint32_t ST_COPIES = 8;
 __global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, uint32_t G, uint32_t level) {
     REF_SCENE(S, G);
-    __shared__ uint32_t st[REF_ST_WORDS];
+    __shared__ uint32_t st[ST_COPIES * REF_ST_WORDS];
     const uint32_t cur = level & 1u;
     const uint32_t nch = s.lv_n[2 + cur];
     if (rj == 0 && threadIdx.x == 0) { s.lv_n[cur ^ 1u] = 0; s.lv_n[2 + (cur ^ 1u)] = 0; }
+    uint32_t* const mine = st + (lane_id() % ST_COPIES) * REF_ST_WORDS;
     for (uint32_t c = rj; c < nch; c += G) {
         const RefRange R = ref_range(s, cur, c);
-        for (uint32_t k = threadIdx.x; k < REF_ST_WORDS; k += REF_BLOCK) st[k] = stat_init_word(k);
+        for (uint32_t k = threadIdx.x; k < ST_COPIES * REF_ST_WORDS; k += REF_BLOCK)
+            st[k] = stat_init_word(k % REF_ST_WORDS);
         const bbox cb = ord_box(R.rec + RG_CB);
         const int dim = box_max_extent(cb);
         const float cmin = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
@@ -2301,7 +2369,7 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, u
             const f3 cc = box_cen(b);
             const uint32_t ib = sah_bucket(comp(cc, dim), cmin, dist);
             s.gp_bucket[p] = (uint8_t)ib;
-            uint32_t* sb = st + 13 * ib;
+            uint32_t* sb = mine + 13 * ib;
             atomicAdd(&sb[0], 1u);
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -2328,10 +2396,15 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, u
         __syncthreads();
         uint32_t* gs = ref_stat(s, cur, R.r);
         for (uint32_t k = threadIdx.x; k < ST_BOX + 6; k += REF_BLOCK) {
-            const uint32_t x = st[k];
+            const bool cnt = k < ST_BOX && k % 13 == 0, mx = stat_is_max(k);
+            uint32_t x = st[k];
+            for (uint32_t q = 1; q < ST_COPIES; q++) {
+                const uint32_t y = st[q * REF_ST_WORDS + k];
+                x = cnt ? x + y : (mx ? max(x, y) : min(x, y));
+            }
             if (x == stat_init_word(k)) continue;
-            if (k < ST_BOX && k % 13 == 0) atomicAdd(&gs[k], x);
-            else if (stat_is_max(k)) atomicMax(&gs[k], x);
+            if (cnt) atomicAdd(&gs[k], x);
+            else if (mx) atomicMax(&gs[k], x);
             else atomicMin(&gs[k], x);
         }
         __syncthreads();

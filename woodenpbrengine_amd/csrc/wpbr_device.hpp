@@ -132,4 +132,32 @@ WPBR_D uint32_t lane_id() { return __lane_id(); }
 WPBR_D uint32_t popc64(uint64_t m) { return __popcll(m); }
 WPBR_D uint64_t lanemask_lt() { uint32_t l = __lane_id(); return l ? (~0ull >> (64 - l)) : 0ull; }
 
+// Wave-wide smin / smax of a float through DPP: row_shr 1, 2, 4, 8 leave each
+// row's result in its lane 15, row_bcast 15 and 31 carry rows 0..2 into lane
+// 63, which every lane reads.  Lanes whose DPP source lies outside the row
+// take the identity (+-inf).  All 64 lanes must be active.
+template <bool MAX>
+WPBR_D float wave_reduce_dpp(float v) {
+    const int id = __float_as_int(MAX ? -INF : INF);
+    auto op = [](float a, float b) { return MAX ? smax(a, b) : smin(a, b); };
+#define WPBR_DPP_STEP(ctrl, rows) \
+    v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), ctrl, rows, 0xF, false)))
+    WPBR_DPP_STEP(0x111, 0xF);
+    WPBR_DPP_STEP(0x112, 0xF);
+    WPBR_DPP_STEP(0x114, 0xF);
+    WPBR_DPP_STEP(0x118, 0xF);
+    WPBR_DPP_STEP(0x142, 0xA);
+    WPBR_DPP_STEP(0x143, 0xC);
+#undef WPBR_DPP_STEP
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+WPBR_D float wave_min_dpp(float v) { return wave_reduce_dpp<false>(v); }
+WPBR_D float wave_max_dpp(float v) { return wave_reduce_dpp<true>(v); }
+
+// Lane value of the lane `lane ^ 1` / `lane ^ 2` (within quads).
+WPBR_D float quad_xor1(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0xB1, 0xF, 0xF, false)); }
+WPBR_D float quad_xor2(float v) { return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x4E, 0xF, 0xF, false)); }
+WPBR_D uint32_t quad_xor1(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false); }
+WPBR_D uint32_t quad_xor2(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false); }
+
 }  // namespace wpbr
