@@ -231,6 +231,16 @@ wpbr_status wpbr_open(int device, wpbr_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
     c->loss_scratch = noalloc_scratch(device);  // null: wpbr_film_loss_noalloc reports it
+    {
+        wpbr::LevelSync& ls = c->level_sync;
+        if (hipHostMalloc((void**)&ls.host, 64, hipHostMallocDefault) != hipSuccess) ls.host = nullptr;
+        if (ls.host && (hipEventCreateWithFlags(&ls.ev[0], hipEventDisableTiming) != hipSuccess ||
+                        hipEventCreateWithFlags(&ls.ev[1], hipEventDisableTiming) != hipSuccess)) {
+            if (ls.ev[0]) (void)hipEventDestroy(ls.ev[0]);
+            (void)hipHostFree(ls.host);
+            ls = wpbr::LevelSync{nullptr, {nullptr, nullptr}};
+        }
+    }
     *out = c;
     return WPBR_OK;
 }
@@ -244,6 +254,11 @@ void wpbr_close(wpbr_ctx* ctx) {
     if (ctx->batch_mem) hipFree(ctx->batch_mem);
     if (ctx->bsdf_blk) hipFree(ctx->bsdf_blk);
     ctx_release_render(ctx);
+    if (ctx->level_sync.host) {
+        hipEventDestroy(ctx->level_sync.ev[0]);
+        hipEventDestroy(ctx->level_sync.ev[1]);
+        hipHostFree(ctx->level_sync.host);
+    }
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     hipStreamDestroy(ctx->stream);
@@ -370,7 +385,8 @@ wpbr_status wpbr_lbvh_build_stage(wpbr_ctx* ctx, const wpbr_geometry* geom, wpbr
             g.sphere_xform != b->geom.sphere_xform || g.tri_verts != b->geom.tri_verts)
             return set_err(ctx, WPBR_ERR_INVALID_ARG, "geometry changed between lbvh stages");
     }
-    wpbr_status s = lbvh_launch_stage(b->s, b->d_self, b->geom, b->n, ctx->stream, stage, b->ev, wpbr_lbvh::NEV);
+    wpbr_status s = lbvh_launch_stage(b->s, b->d_self, b->geom, b->n, ctx->stream, stage, b->ev, wpbr_lbvh::NEV,
+                                      ctx->level_sync.host ? &ctx->level_sync : nullptr);
     if (s) {
         b->next_stage = 0;
         return set_err(ctx, s, "lbvh launch failed");
@@ -474,7 +490,8 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
     if (s) return s;
     hipEventRecord(ctx->ev0, ctx->stream);
     const uint32_t max_n = *std::max_element(nv.begin(), nv.end());
-    s = lbvh_launch_segmented(B, grid, (uint32_t)bvhs[0]->order, max_n, ctx->stream);
+    s = lbvh_launch_segmented(B, grid, (uint32_t)bvhs[0]->order, max_n, ctx->stream,
+                              ctx->level_sync.host ? &ctx->level_sync : nullptr);
     hipEventRecord(ctx->ev1, ctx->stream);
     if (s) return set_err(ctx, s, std::string("segmented lbvh launch failed: ") + hipGetErrorString(last_launch_error));
     std::vector<uint32_t> info(5 * (size_t)n_scenes);

@@ -9,8 +9,11 @@
 //                   histograms of the radix sort
 //   k_onesweep x4   a4: stable LSD radix sort, 8-bit digits, one pass each,
 //                   decoupled look-back, wave-ballot ranking, LDS-staged scatter.
-//                   A stable sort of (code, index) has a unique result, equal to
-//                   the reference's 5 x 6-bit passes (SLBVH.cpp:245-283)
+//                   This chain is the full key order (all 30 bits), a documented
+//                   alternative.  The reference's 5 x 6-bit ping-pong
+//                   (SLBVH.cpp:245-283) leaves pass 3's output in
+//                   shapeMortonCodes, a stable sort on bits 0..23 only: the
+//                   reference key order below (three passes) reproduces that
 //   k_slot_prims    a4: treelet = run of equal top-12 bits (SLBVH.cpp:299-319);
 //                   shape bounds gathered into slot order (SLBVH.cpp:231-324),
 //                   with the traversal layout's primitive records
@@ -2364,8 +2367,19 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, u
         const float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cmin;
         __syncthreads();
         uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-        for (uint32_t p = R.p0 + threadIdx.x; p < R.p1; p += REF_BLOCK) {
-            const bbox b = load_box(s.gp_box + 6 * (size_t)p);
+        // the chunk's boxes first, all loads in flight at once
+        constexpr uint32_t PER = REF_CHUNK / REF_BLOCK;
+        bbox bx[PER];
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) {
+            const uint32_t p = R.p0 + threadIdx.x + j * REF_BLOCK;
+            if (p < R.p1) bx[j] = load_box(s.gp_box + 6 * (size_t)p);
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < PER; j++) {
+            const uint32_t p = R.p0 + threadIdx.x + j * REF_BLOCK;
+            if (p >= R.p1) break;
+            const bbox& b = bx[j];
             const f3 cc = box_cen(b);
             const uint32_t ib = sah_bucket(comp(cc, dim), cmin, dist);
             s.gp_bucket[p] = (uint8_t)ib;
@@ -2779,7 +2793,17 @@ __global__ __launch_bounds__(64 * WR_WAVES) void kr_sub_wave(const LbvhScratch* 
     LbvhScratch s = S[sc];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nsub = *s.sub_n;
-    for (uint32_t st = rj * WR_WAVES + wave; st < nsub; st += G * WR_WAVES) wr_range(s, W[wave], st);
+    (void)rj;
+    // each wave takes the next range when it has finished one (the scene's
+    // range cursor is tile_counter[7], zeroed with the build's counters; the
+    // sort passes use words 0..3)
+    for (;;) {
+        uint32_t st = 0;
+        if (lane_id() == 0) st = atomicAdd(s.tile_counter + 7, 1u);
+        st = (uint32_t)__builtin_amdgcn_readfirstlane((int)st);
+        if (st >= nsub) break;
+        wr_range(s, W[wave], st);
+    }
 }
 // ---- preorder layout ---------------------------------------------------------
 // P = exclusive prefix of t_nodes over final positions; a treelet at final
@@ -2956,7 +2980,9 @@ static RefGrid ref_grid(uint32_t max_n, uint32_t n_scenes) {
     g.slots = clampu((max_n + REF_BLOCK - 1) / REF_BLOCK, 1, cap);
     g.treelets = g.slots;
     g.emit_big = clampu(max_n / 512 + 1, 1, n_scenes > 1 ? 64u : 1024u);
-    g.level = clampu(max_n / REF_CHUNK + 1, 1, n_scenes > 1 ? 64u : 1024u);
+    // about one chunk per workgroup at every level (a level holds up to
+    // n / REF_CUT ranges): the per-chunk steps are latency-bound
+    g.level = clampu(max_n / REF_CUT + 1, 1, n_scenes > 1 ? 128u : 4096u);
     g.sub = clampu(max_n / (64 * WR_WAVES) + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.fin = clampu(max_n / REF_FIN_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.upper = clampu(max_n / REF_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
@@ -2980,8 +3006,10 @@ static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g,
 constexpr uint32_t REF_LEVELS_PER_CHECK = 4;
 constexpr uint32_t REF_MAX_LEVELS = 4096;
 static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const RefGrid& g, uint32_t* d_active,
-                                    hipStream_t st) {
-    for (uint32_t level = 0;; ) {
+                                    hipStream_t st, const LevelSync* ls) {
+    uint32_t level = 0;
+    // REF_LEVELS_PER_CHECK levels, then the next level's range count into d_active
+    auto block = [&]() {
         for (uint32_t k = 0; k < REF_LEVELS_PER_CHECK; k++, level++) {
             hipLaunchKernelGGL(kr_lv_stats, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
             hipLaunchKernelGGL(kr_lv_split, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
@@ -2989,14 +3017,36 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
             hipLaunchKernelGGL(kr_lv_swap, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
         }
         hipLaunchKernelGGL(kr_active, dim3(1), dim3(256), 0, st, S, ns, level & 1u, d_active);
-        uint32_t active = 0;
-        if (hipMemcpyAsync(&active, d_active, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess)
+    };
+    // both children of a split are non-empty, so a level range shrinks every
+    // level; deeper chains of ranges of at least REF_CUT roots are refused
+    if (!ls) {
+        for (;;) {
+            block();
+            uint32_t active = 0;
+            if (hipMemcpyAsync(&active, d_active, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return WPBR_ERR_HIP;
+            if (!active) break;
+            if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
+        }
+    } else {
+        // block k's count goes to host word k & 1 behind event k & 1; the host
+        // reads it while block k + 1 runs (kernels of an empty level return at
+        // once, so a block queued after the last active level changes nothing)
+        block();
+        if (hipMemcpyAsync(ls->host, d_active, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipEventRecord(ls->ev[0], st) != hipSuccess)
             return WPBR_ERR_HIP;
-        if (!active) break;
-        // both children of a split are non-empty, so a level range shrinks every
-        // level; deeper chains of ranges of at least REF_CUT roots are refused
-        if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
+        for (uint32_t k = 0;; k++) {
+            const uint32_t nx = (k + 1) & 1u;
+            block();
+            if (hipMemcpyAsync(ls->host + nx, d_active, 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipEventRecord(ls->ev[nx], st) != hipSuccess || hipEventSynchronize(ls->ev[k & 1u]) != hipSuccess)
+                return WPBR_ERR_HIP;
+            if (!ls->host[k & 1u]) break;
+            if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
+        }
     }
     hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
     hipLaunchKernelGGL(kr_fin_sum, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
@@ -3247,7 +3297,7 @@ void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts) {
 }
 
 wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint32_t order, uint32_t max_n,
-                                  hipStream_t st) {
+                                  hipStream_t st, const LevelSync* ls) {
     const uint32_t S = B.n_scenes;
     const bool ref = order == WPBR_LBVH_ORDER_REFERENCE;
     hipLaunchKernelGGL(ks_bounds, dim3(grid[SEG_BOUNDS]), dim3(256), 0, st, B);
@@ -3260,7 +3310,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
         const RefGrid g = ref_grid(max_n, S);
         ref_launch_treelets(B.s, S, g, st);
         ref_launch_emit(B.s, S, g, st);
-        wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st);
+        wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st, ls);
         if (e) return e;
         hipLaunchKernelGGL(kr_scatter_ref, dim3(S * g.upper), dim3(REF_BLOCK), 0, st, B.s, g.upper);
         hipLaunchKernelGGL(kr_big_scan, dim3(S), dim3(1024), 0, st, B.s);
@@ -3293,7 +3343,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
 //   5 upper SAH + flatten     JobBuildUpperSAH (+ the derived traversal layout)
 // ev[0..11] bracket the kernels (the per-kernel times of wpbr_lbvh_kernel_times).
 wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const GeomView& g, uint32_t n,
-                              hipStream_t st, int stage, hipEvent_t* ev, int nev) {
+                              hipStream_t st, int stage, hipEvent_t* ev, int nev, const LevelSync* ls) {
     auto rec = [&](int i) { if (ev && i < nev) (void)hipEventRecord(ev[i], st); };
     const bool ref = s.order == WPBR_LBVH_ORDER_REFERENCE;
     const RefGrid rg = ref_grid(n, 1);
@@ -3336,7 +3386,7 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const G
             break;
         case 5: {
             if (ref) {
-                wpbr_status e = ref_launch_upper(d_self, 1, rg, s.err + 3, st);
+                wpbr_status e = ref_launch_upper(d_self, 1, rg, s.err + 3, st, ls);
                 if (e) return e;
                 rec(9);
                 hipLaunchKernelGGL(kr_scatter_ref, dim3(rg.upper), dim3(REF_BLOCK), 0, st, d_self, rg.upper);

@@ -36,6 +36,9 @@ struct wpbr_ctx {
     // segmented LBVH builds: per-scene descriptors and block maps
     void* batch_mem = nullptr;
     size_t batch_cap = 0;
+    // the reference-order LBVH level loop's overlapped range-count read-back
+    // (host null: the loop waits for each count)
+    wpbr::LevelSync level_sync{nullptr, {nullptr, nullptr}};
 };
 
 struct wpbr_lbvh {

@@ -161,9 +161,17 @@ struct LbvhBatch {
     SegMap map[SEG_KINDS];
 };
 void lbvh_seg_blocks(uint32_t n, int status_words, uint32_t* counts);
+// The reference order's level loop reads back the next level's range count
+// every few levels.  With a LevelSync (two words of pinned host memory and two
+// events) the read-back of one block of levels overlaps the next block's
+// kernels; without one the host waits for each count.
+struct LevelSync {
+    uint32_t* host;
+    hipEvent_t ev[2];
+};
 // order: the key order of every scene of the batch; max_n: the largest scene.
 wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint32_t order, uint32_t max_n,
-                                  hipStream_t st);
+                                  hipStream_t st, const LevelSync* ls = nullptr);
 
 // wpbr_film_loss_noalloc: the file-scope scratch holds this many per-block loss
 // partials (256 pixels each: films of up to 16 777 216 pixels), then their sum
@@ -181,7 +189,7 @@ constexpr uint32_t SAH_SEG_SUB_BLOCKS = 256;  // segmented build, over all scene
 // d_self: a device copy of s (the reference order's kernels read the scratch
 // through it).
 wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const GeomView& g, uint32_t n,
-                              hipStream_t st, int stage, hipEvent_t* ev, int nev);
+                              hipStream_t st, int stage, hipEvent_t* ev, int nev, const LevelSync* ls = nullptr);
 
 struct TraceArgs {
     GeomView g;
