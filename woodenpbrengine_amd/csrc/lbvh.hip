@@ -2012,10 +2012,11 @@ __global__ __launch_bounds__(256) void k_bvh4(LbvhScratch s) { bvh4_body(s, bloc
 //                   run heads (SLBVH.cpp:299-319), the upper SAH's first range
 //   kr_emit_small / kr_emit_big   treelets below 30 shapes are one leaf
 //                   (SLBVH.cpp:356-367), larger ones get emitLBVH by a workgroup
-//   kr_lv_stats / kr_lv_split / kr_lv_lists / kr_lv_swap   one level of the
-//                   upper SAH (SLBVH.cpp:457-563) over every range of at least
-//                   REF_CUT roots: bucket statistics, the split (cost, first
-//                   minimum, midpoint fallback) and the libstdc++ two-pointer
+//   kr_lv_stats / kr_lv_ranges / kr_lv_count / kr_lv_lists / kr_lv_swap
+//                   one level of the upper SAH (SLBVH.cpp:457-563) over every
+//                   range of at least REF_CUT roots: bucket statistics, the
+//                   split (cost, first minimum, midpoint fallback) one thread
+//                   per range, misplaced counts per chunk and the libstdc++ two-pointer
 //                   std::partition in rank form (the r-th misplaced position
 //                   left of the split swaps with the r-th misplaced position
 //                   right of it counted from the right)
@@ -2033,9 +2034,10 @@ enum {
     RG_BEST,           // split bucket
     RG_L,              // positions left of the split (= size: midpoint fallback)
     RG_ML,             // misplaced pairs
-    RG_MID
+    RG_MID,
+    RG_UB              // first upper node of the range's subtree (a range of m roots makes m - 1)
 };
-static_assert(RG_MID < (int)REF_RG_WORDS, "range record");
+static_assert(RG_UB < (int)REF_RG_WORDS, "range record");
 // Statistics: per bucket 13 words (count, box min xyz / max xyz, centroid min
 // xyz / max xyz as order-preserving uints), then the range's box.
 constexpr uint32_t ST_BOX = NB * 13;
@@ -2156,7 +2158,9 @@ __global__ __launch_bounds__(1024) void kr_head_scan(const LbvhScratch* S) {
         *s.n_treelets = T;
         *s.bigt_n = 0;
         *s.sub_n = 0;
-        *s.upper_count = 0;
+        // upper nodes are numbered by range: a range of m roots owns m - 1
+        // consecutive numbers from its first (the root range's is 0)
+        *s.upper_count = T > 1 ? T - 1 : 0;
         for (int k = 0; k < 4; k++) s.lv_n[k] = 0;
         if (T == 1) {
             s.g_la[0] = 0;
@@ -2168,7 +2172,6 @@ __global__ __launch_bounds__(1024) void kr_head_scan(const LbvhScratch* S) {
             s.sub_rt[0] = 0;
             s.sub_ub[0] = 0;
             *s.sub_n = 1;
-            *s.upper_count = T - 1;
         } else {
             uint32_t* rec = ref_rec(s, 0, 0);
             rec[RG_I0] = 0;
@@ -2181,6 +2184,7 @@ __global__ __launch_bounds__(1024) void kr_head_scan(const LbvhScratch* S) {
             }
             rec[RG_CH] = 0;
             rec[RG_NCH] = (T + REF_CHUNK - 1) / REF_CHUNK;
+            rec[RG_UB] = 0;
             s.lv_n[0] = 1;
             s.lv_n[2] = rec[RG_NCH];
         }
@@ -2425,129 +2429,198 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, u
     }
 }
 
-// The split of every range (SLBVH.cpp:504-553), made identically by every
-// workgroup of the range from its statistics; the range's first chunk records
-// the upper node and its children (next-level ranges, deferred LDS ranges or
-// single positions); every chunk counts its misplaced positions.
-__global__ __launch_bounds__(REF_BLOCK) void kr_lv_split(const LbvhScratch* S, uint32_t G, uint32_t level) {
+// Exclusive prefixes of three counts over the wave, with bases for the whole
+// wave taken from three counters (lanes 0..2 issue the three device atomics at
+// once).  All 64 lanes must call it.
+WPBR_D void wave_alloc3(uint32_t* c0, uint32_t* c1, uint32_t* c2, const uint32_t (&want)[3], uint32_t (&got)[3]) {
+    const int lane = (int)lane_id();
+    uint32_t incl[3] = {want[0], want[1], want[2]};
+    for (int d = 1; d < 64; d <<= 1)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const uint32_t x = (uint32_t)__shfl_up((int)incl[k], d);
+            if (lane >= d) incl[k] += x;
+        }
+    uint32_t tot[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) tot[k] = (uint32_t)__builtin_amdgcn_readlane((int)incl[k], 63);
+    uint32_t b = 0;
+    if (lane < 3) {
+        uint32_t* ctr = lane == 0 ? c0 : (lane == 1 ? c1 : c2);
+        const uint32_t t = lane == 0 ? tot[0] : (lane == 1 ? tot[1] : tot[2]);
+        if (t) b = atomicAdd(ctr, t);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) got[k] = (uint32_t)__builtin_amdgcn_readlane((int)b, k) + incl[k] - want[k];
+}
+
+// The split of every range (SLBVH.cpp:504-553), one lane per range: each
+// workgroup stages the statistics and records of LV_RPB consecutive ranges in
+// LDS (coalesced), then lane r of wave 0 computes range r's split costs
+// (prefix and suffix sweeps of the bucket boxes; min/max are exact, so the
+// sides' boxes, areas and costs are those of the reference's per-split
+// unions), the first minimum and the midpoint fallback, the upper node
+// (numbered from the range's first number: the left subtree's follow it, then
+// the right subtree's) and the children: single positions, deferred ranges
+// below REF_CUT, or next-level ranges.  List slots and chunks are taken with
+// one atomic per counter and wave.
+constexpr uint32_t LV_RPB = 64;
+__global__ __launch_bounds__(REF_BLOCK) void kr_lv_ranges(const LbvhScratch* S, uint32_t G, uint32_t level) {
     REF_SCENE(S, G);
-    __shared__ uint32_t s_split[4];
-    __shared__ uint32_t s_child[2][4];  // next-level range: index, first chunk, chunks, valid
+    __shared__ uint32_t l_st[LV_RPB * REF_ST_WORDS];
+    __shared__ uint32_t l_rec[LV_RPB * REF_RG_WORDS];
+    const uint32_t cur = level & 1u, nxt = cur ^ 1u;
+    const uint32_t nr = s.lv_n[cur];
+    for (uint32_t r0 = rj * LV_RPB; r0 < nr; r0 += G * LV_RPB) {
+        const uint32_t cnt = min(LV_RPB, nr - r0);
+        {
+            const uint32_t* gst = ref_stat(s, cur, r0);
+            const uint32_t* grc = ref_rec(s, cur, r0);
+            for (uint32_t q = threadIdx.x; q < cnt * REF_ST_WORDS; q += REF_BLOCK) l_st[q] = gst[q];
+            for (uint32_t q = threadIdx.x; q < cnt * REF_RG_WORDS; q += REF_BLOCK) l_rec[q] = grc[q];
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const uint32_t lr = threadIdx.x, r = r0 + lr;
+            const bool valid = lr < cnt;
+            const uint32_t* gs = l_st + lr * REF_ST_WORDS;
+            const uint32_t* rec = l_rec + lr * REF_RG_WORDS;
+            uint32_t i0 = 0, i1 = 0, mid = 0, best = 0, ub = 0, dep = 0, rt = 0;
+            bool fallback = false;
+            if (valid) {
+                i0 = rec[RG_I0];
+                i1 = rec[RG_I1];
+                dep = rec[RG_DEP];
+                rt = rec[RG_RT];
+                ub = rec[RG_UB];
+                const uint32_t m = i1 - i0 + 1;
+                const bbox bounds = ord_box(gs + ST_BOX);
+                const float area = box_area(bounds);
+                float la[NB - 1], ra[NB - 1];
+                uint32_t cle[NB - 1];
+                {
+                    bbox acc = box_empty();
+                    uint32_t c = 0;
+#pragma unroll
+                    for (int i = 0; i < NB - 1; i++) {
+                        acc = box_union(acc, ord_box(gs + 13 * i + 1));
+                        c += gs[13 * i];
+                        la[i] = (c == 0) ? 0.0f : (float)c * box_area(acc);
+                        cle[i] = c;
+                    }
+                    acc = box_empty();
+                    c = 0;
+#pragma unroll
+                    for (int i = NB - 1; i >= 1; i--) {
+                        acc = box_union(acc, ord_box(gs + 13 * i + 1));
+                        c += gs[13 * i];
+                        ra[i - 1] = (c == 0) ? 0.0f : (float)c * box_area(acc);
+                    }
+                }
+                float cbest = 0.0f;
+                uint32_t L = 0;
+#pragma unroll
+                for (int i = 0; i < NB - 1; i++) {
+                    const float cost = (float)(0.125 + (double)((la[i] + ra[i]) / area));
+                    if (i == 0 || cost < cbest) { best = (uint32_t)i; cbest = cost; L = cle[i]; }
+                }
+                fallback = L == m;
+                mid = fallback ? m / 2 + i0 : i0 + L;
+                s.u_i0[ub] = i0;
+                s.u_i1[ub] = i1;
+                s.u_mid[ub] = mid;
+                s.u_dep[ub] = dep;
+                s.u_rt[ub] = rt;
+                s.u_axis[ub] = (uint32_t)box_max_extent(ord_box(rec + RG_CB));
+                store_box(s.u_box + 6 * (size_t)ub, bounds);
+                uint32_t* grec = ref_rec(s, cur, r);
+                grec[RG_BEST] = best;
+                grec[RG_L] = L;
+                grec[RG_MID] = mid;
+                grec[RG_ML] = 0;
+            }
+            // children: kind 0 none / single position, 1 deferred, 2 next level
+            uint32_t kind[2] = {0, 0}, cn[2] = {0, 0};
+            if (valid)
+                for (int k = 0; k < 2; k++) {
+                    const uint32_t a = k == 0 ? i0 : mid, b = k == 0 ? mid - 1 : i1;
+                    const uint32_t size = b - a + 1;
+                    if (size == 1) {
+                        s.g_la[a] = dep + 1;
+                        s.g_lb[a] = rt + (uint32_t)k;
+                    } else if (size < REF_CUT) {
+                        kind[k] = 1;
+                    } else {
+                        kind[k] = 2;
+                        cn[k] = (size + REF_CHUNK - 1) / REF_CHUNK;
+                    }
+                }
+            const uint32_t want[3] = {(uint32_t)(kind[0] == 1) + (uint32_t)(kind[1] == 1),
+                                      (uint32_t)(kind[0] == 2) + (uint32_t)(kind[1] == 2), cn[0] + cn[1]};
+            uint32_t got[3];
+            wave_alloc3(s.sub_n, &s.lv_n[nxt], &s.lv_n[2 + nxt], want, got);
+            if (valid) {
+                uint32_t ne = got[0], nrr = got[1], nc = got[2];
+                for (int k = 0; k < 2; k++) {
+                    if (!kind[k]) continue;
+                    const uint32_t a = k == 0 ? i0 : mid, b = k == 0 ? mid - 1 : i1;
+                    const uint32_t cub = k == 0 ? ub + 1 : ub + (mid - i0);
+                    const uint32_t cdep = dep + 1, crt = rt + (uint32_t)k;
+                    if (kind[k] == 1) {
+                        s.sub_i0[ne] = a;
+                        s.sub_i1[ne] = b;
+                        s.sub_dep[ne] = cdep;
+                        s.sub_rt[ne] = crt;
+                        s.sub_ub[ne] = cub;
+                        ne++;
+                        continue;
+                    }
+                    uint32_t* cr = ref_rec(s, nxt, nrr);
+                    cr[RG_I0] = a;
+                    cr[RG_I1] = b;
+                    cr[RG_DEP] = cdep;
+                    cr[RG_RT] = crt;
+                    cr[RG_CH] = nc;
+                    cr[RG_NCH] = cn[k];
+                    cr[RG_UB] = cub;
+                    // the child's centroid bounds: the union of its buckets' (all
+                    // centroids coincide when the midpoint fallback splits)
+                    for (int q = 0; q < 6; q++) {
+                        uint32_t x = rec[RG_CB + q];
+                        if (!fallback) {
+                            x = q < 3 ? 0xFFFFFFFFu : 0u;
+                            const int j0 = k == 0 ? 0 : (int)best + 1, j1 = k == 0 ? (int)best : NB - 1;
+                            for (int j = j0; j <= j1; j++) {
+                                if (gs[13 * j] == 0) continue;
+                                const uint32_t y = gs[13 * j + 7 + q];
+                                x = q < 3 ? min(x, y) : max(x, y);
+                            }
+                        }
+                        cr[RG_CB + q] = x;
+                    }
+                    nrr++;
+                    nc += cn[k];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Per chunk of the level: its misplaced positions (left of the split with a
+// bucket above it, right of it with a bucket at or below it); then the next
+// level's chunk-to-range map and zeroed statistics.
+__global__ __launch_bounds__(REF_BLOCK) void kr_lv_count(const LbvhScratch* S, uint32_t G, uint32_t level) {
+    REF_SCENE(S, G);
     __shared__ uint32_t s_w[REF_BLOCK / 64];
     const uint32_t cur = level & 1u, nxt = cur ^ 1u;
     const uint32_t nch = s.lv_n[2 + cur];
     const int lane = threadIdx.x & 63;
     for (uint32_t c = rj; c < nch; c += G) {
         const RefRange R = ref_range(s, cur, c);
-        const uint32_t m = R.i1 - R.i0 + 1;
-        const uint32_t* gs = ref_stat(s, cur, R.r);
-        if (threadIdx.x < 64) {
-            const bbox bounds = ord_box(gs + ST_BOX);
-            float cost = INF;
-            uint32_t cnt_le = 0;
-            if (lane < NB - 1) {
-                bbox b0 = box_empty(), b1 = box_empty();
-                uint32_t c0 = 0, c1 = 0;
-                for (int j = 0; j < NB; j++) {
-                    const bbox bj = ord_box(gs + 13 * j + 1);
-                    const uint32_t cj = gs[13 * j];
-                    if (j <= lane) { b0 = box_union(b0, bj); c0 += cj; }
-                    else { b1 = box_union(b1, bj); c1 += cj; }
-                }
-                const float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
-                const float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
-                cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
-                cnt_le = c0;
-            }
-            const int best = sah_best(cost);
-            const uint32_t L = (uint32_t)__shfl((int)cnt_le, best);
-            if (lane == 0) {
-                s_split[0] = (uint32_t)best;
-                s_split[1] = L;
-                s_split[2] = (L == m) ? m / 2 + R.i0 : R.i0 + L;
-            }
-        }
-        __syncthreads();
-        const uint32_t best = s_split[0], L = s_split[1], mid = s_split[2];
-        const bool fallback = L == m;
-        if (c == R.ch0 && threadIdx.x < 2) {
-            const uint32_t dep = R.rec[RG_DEP], rt = R.rec[RG_RT];
-            const int k = (int)threadIdx.x;
-            uint32_t* rec = const_cast<uint32_t*>(R.rec);
-            if (k == 0) {
-                const bbox cb = ord_box(R.rec + RG_CB);
-                const uint32_t node = atomicAdd(s.upper_count, 1u);
-                s.u_i0[node] = R.i0;
-                s.u_i1[node] = R.i1;
-                s.u_mid[node] = mid;
-                s.u_dep[node] = dep;
-                s.u_rt[node] = rt;
-                s.u_axis[node] = (uint32_t)box_max_extent(cb);
-                store_box(s.u_box + 6 * (size_t)node, ord_box(gs + ST_BOX));
-                rec[RG_BEST] = best;
-                rec[RG_L] = L;
-                rec[RG_MID] = mid;
-                rec[RG_ML] = 0;
-            }
-            const uint32_t a = k == 0 ? R.i0 : mid, b = k == 0 ? mid - 1 : R.i1;
-            const uint32_t cdep = dep + 1, crt = rt + (uint32_t)k;
-            const uint32_t size = b - a + 1;
-            s_child[k][3] = 0;
-            if (size == 1) {
-                s.g_la[a] = cdep;
-                s.g_lb[a] = crt;
-            } else if (size < REF_CUT) {
-                const uint32_t e = atomicAdd(s.sub_n, 1u);
-                s.sub_i0[e] = a;
-                s.sub_i1[e] = b;
-                s.sub_dep[e] = cdep;
-                s.sub_rt[e] = crt;
-                s.sub_ub[e] = atomicAdd(s.upper_count, size - 1);
-            } else {
-                const uint32_t rr = atomicAdd(&s.lv_n[nxt], 1u);
-                const uint32_t cn = (size + REF_CHUNK - 1) / REF_CHUNK;
-                const uint32_t c0 = atomicAdd(&s.lv_n[2 + nxt], cn);
-                uint32_t* cr = ref_rec(s, nxt, rr);
-                cr[RG_I0] = a;
-                cr[RG_I1] = b;
-                cr[RG_DEP] = cdep;
-                cr[RG_RT] = crt;
-                cr[RG_CH] = c0;
-                cr[RG_NCH] = cn;
-                // children's centroid bounds: the union of their buckets' (all
-                // centroids coincide when the midpoint fallback splits)
-                for (int q = 0; q < 6; q++) {
-                    uint32_t x = R.rec[RG_CB + q];
-                    if (!fallback) {
-                        x = q < 3 ? 0xFFFFFFFFu : 0u;
-                        const int j0 = k == 0 ? 0 : (int)best + 1, j1 = k == 0 ? (int)best : NB - 1;
-                        for (int j = j0; j <= j1; j++) {
-                            if (gs[13 * j] == 0) continue;
-                            const uint32_t y = gs[13 * j + 7 + q];
-                            x = q < 3 ? min(x, y) : max(x, y);
-                        }
-                    }
-                    cr[RG_CB + q] = x;
-                }
-                s_child[k][0] = rr;
-                s_child[k][1] = c0;
-                s_child[k][2] = cn;
-                s_child[k][3] = 1;
-            }
-        }
-        __syncthreads();
-        if (c == R.ch0) {
-            for (int k = 0; k < 2; k++) {
-                if (!s_child[k][3]) continue;
-                const uint32_t rr = s_child[k][0], c0 = s_child[k][1], cn = s_child[k][2];
-                for (uint32_t q = threadIdx.x; q < cn; q += REF_BLOCK) s.ch_range[(size_t)nxt * s.cap_c + c0 + q] = rr;
-                uint32_t* st = ref_stat(s, nxt, rr);
-                for (uint32_t q = threadIdx.x; q < REF_ST_WORDS; q += REF_BLOCK) st[q] = stat_init_word(q);
-            }
-        }
-        // misplaced positions of this chunk
+        const uint32_t m = R.i1 - R.i0 + 1, L = R.rec[RG_L], best = R.rec[RG_BEST];
         uint32_t cl = 0, cr = 0;
-        if (!fallback)
+        if (L != m)
             for (uint32_t p = R.p0 + threadIdx.x; p < R.p1; p += REF_BLOCK) {
                 const uint32_t ib = s.gp_bucket[p];
                 if (p < R.i0 + L) cl += ib > best;
@@ -2566,6 +2639,14 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_split(const LbvhScratch* S, u
             s.ch_cnt[2 * (size_t)c + 1] = tr;
         }
         __syncthreads();
+    }
+    const uint32_t nr2 = s.lv_n[nxt];
+    for (uint32_t r = rj; r < nr2; r += G) {
+        const uint32_t* cr = ref_rec(s, nxt, r);
+        const uint32_t c0 = cr[RG_CH], cn = cr[RG_NCH];
+        for (uint32_t q = threadIdx.x; q < cn; q += REF_BLOCK) s.ch_range[(size_t)nxt * s.cap_c + c0 + q] = r;
+        uint32_t* st = ref_stat(s, nxt, r);
+        for (uint32_t q = threadIdx.x; q < REF_ST_WORDS; q += REF_BLOCK) st[q] = stat_init_word(q);
     }
 }
 static_assert(REF_CHUNK < 65536, "chunk counts packed in 16 bits");
@@ -2971,7 +3052,7 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_scatter_big(const LbvhScratch* S
 // Host side of the reference order: blocks per scene of each kernel, the
 // launch sequence of the sort-and-treelet, emit and upper-SAH stages.
 struct RefGrid {
-    uint32_t slots, treelets, emit_big, level, sub, fin, upper;
+    uint32_t slots, treelets, emit_big, level, ranges, sub, fin, upper;
 };
 static RefGrid ref_grid(uint32_t max_n, uint32_t n_scenes) {
     auto clampu = [](uint64_t v, uint32_t lo, uint32_t hi) { return (uint32_t)std::min<uint64_t>(hi, std::max<uint64_t>(lo, v)); };
@@ -2983,6 +3064,8 @@ static RefGrid ref_grid(uint32_t max_n, uint32_t n_scenes) {
     // about one chunk per workgroup at every level (a level holds up to
     // n / REF_CUT ranges): the per-chunk steps are latency-bound
     g.level = clampu(max_n / REF_CUT + 1, 1, n_scenes > 1 ? 128u : 4096u);
+    // one thread per range (a level holds up to n / REF_CUT ranges)
+    g.ranges = clampu(max_n / (REF_CUT * LV_RPB) + 1, 1, 256u);
     g.sub = clampu(max_n / (64 * WR_WAVES) + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.fin = clampu(max_n / REF_FIN_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.upper = clampu(max_n / REF_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
@@ -3012,7 +3095,8 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
     auto block = [&]() {
         for (uint32_t k = 0; k < REF_LEVELS_PER_CHECK; k++, level++) {
             hipLaunchKernelGGL(kr_lv_stats, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
-            hipLaunchKernelGGL(kr_lv_split, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
+            hipLaunchKernelGGL(kr_lv_ranges, dim3(ns * g.ranges), dim3(REF_BLOCK), 0, st, S, g.ranges, level);
+            hipLaunchKernelGGL(kr_lv_count, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
             hipLaunchKernelGGL(kr_lv_lists, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
             hipLaunchKernelGGL(kr_lv_swap, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
         }

@@ -106,7 +106,7 @@ struct LbvhScratch {
 constexpr uint32_t REF_CUT = 256;
 constexpr uint32_t REF_CHUNK = 2048;
 constexpr uint32_t REF_FIN_BLOCK = 1024;
-constexpr uint32_t REF_RG_WORDS = 16;
+constexpr uint32_t REF_RG_WORDS = 20;
 constexpr uint32_t REF_ST_WORDS = 164;
 inline uint32_t ref_cap_r(uint32_t cap_t) { return cap_t / REF_CUT + 2; }
 inline uint32_t ref_cap_c(uint32_t cap_t) { return cap_t / REF_CHUNK + ref_cap_r(cap_t) + 1; }
