@@ -74,7 +74,7 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.cap_c = ref_cap_c(CT);
     s.slot_bounds = c.take<float>(6 * N);
     s.centroid = c.take<float>(3 * N);
-    s.cb_ord = c.take<uint32_t>(8 + 6 * 512);  // also the reference order's 2 x 256 root partials
+    s.cb_ord = c.take<uint32_t>(8 + 6 * 2048);  // also the reference order's 2 x 1024 root partials
     s.morton = c.take<uint32_t>(N);
     s.keys_a = c.take<uint32_t>(N);
     s.vals_a = c.take<uint32_t>(N);
