@@ -1304,7 +1304,13 @@ WPBR_D void sah_top_body(LbvhScratch& s) {
     __syncthreads();
     const SahTasks T{s.task_i0, s.task_i1, s.task_dep, s.task_rt, s.upper_count};
     sah_run(s, L, T, 0, 1, LANE_TASK_MAX, SAH_CUT);
-    for (uint32_t i = tid; i < k; i += SAH_BLOCK) s.g_perm[i] = L.perm[i];
+    // positions' treelets and root boxes, for the deferred ranges' waves
+    for (uint32_t i = tid; i < k; i += SAH_BLOCK) {
+        const uint32_t t = L.perm[i];
+        s.g_perm[i] = t;
+#pragma unroll
+        for (int c = 0; c < 6; c++) s.gp_box[6 * (size_t)i + c] = L.tbox[6 * t + c];
+    }
 }
 
 // Deferred range st of a scene, by the whole workgroup; its task list and node
@@ -3305,16 +3311,19 @@ __global__ void ks_sah_sub_count(LbvhBatch B) {
 
 // Every scene's deferred ranges from one work list over a fixed grid (few
 // workgroups: each holds a CU's LDS, and launching them is not free).
-__global__ __launch_bounds__(SAH_BLOCK) void ks_sah_sub(LbvhBatch B) {
-    __shared__ SahSubLds S;
-    __shared__ uint32_t s_sc;
+// The deferred ranges of every scene, one wave each (the scenes' ranges in one
+// list: range g belongs to the scene whose prefix of range counts holds it).
+// One workgroup per range (as k_sah_sub) was measured slower here: the 64
+// scenes' deep, unbalanced ranges keep every workgroup's barriers on their
+// critical paths (segmented full-order build 1.76 -> 1.68 ms).
+__global__ __launch_bounds__(64 * WR_WAVES) void ks_sub_wave(LbvhBatch B) {
+    __shared__ WaveRangeLds W[WR_WAVES];
+    const uint32_t wave = threadIdx.x >> 6;
     const uint32_t total = B.subwork[B.n_scenes];
-    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
-        if (threadIdx.x == 0) s_sc = work_scene(B.subwork, B.n_scenes, g);
-        __syncthreads();
-        const uint32_t sc = __builtin_amdgcn_readfirstlane(s_sc);
+    for (uint32_t g = blockIdx.x * WR_WAVES + wave; g < total; g += gridDim.x * WR_WAVES) {
+        const uint32_t sc = work_scene(B.subwork, B.n_scenes, g);
         LbvhScratch s = B.s[sc];
-        sah_sub_one(s, S, g - B.subwork[sc]);
+        wr_range(s, W[wave], g - B.subwork[sc]);
     }
 }
 constexpr uint32_t SEG_SAH_FINAL_BLOCKS = 4;  // per scene
@@ -3407,7 +3416,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
         hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
-        hipLaunchKernelGGL(ks_sah_sub, dim3(SAH_SEG_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
+        hipLaunchKernelGGL(ks_sub_wave, dim3(SAH_SEG_SUB_BLOCKS), dim3(64 * WR_WAVES), 0, st, B);
         hipLaunchKernelGGL(ks_sah_final, dim3(S * SEG_SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
     }
@@ -3481,6 +3490,9 @@ wpbr_status lbvh_launch_stage(LbvhScratch& s, const LbvhScratch* d_self, const G
                 hipLaunchKernelGGL(kr_scatter_big, dim3(rg.upper), dim3(REF_BLOCK), 0, st, d_self, rg.upper);
             } else {
                 hipLaunchKernelGGL(k_sah_top, dim3(1), dim3(SAH_BLOCK), 0, st, s);
+                // the deferred ranges (16..32 of about 128..255 roots at 1M shapes):
+                // one workgroup each (one wave each, as in the segmented build,
+                // measured 191 -> 256 us for the upper SAH)
                 hipLaunchKernelGGL(k_sah_sub, dim3(SAH_SUB_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
                 hipLaunchKernelGGL(k_sah_final, dim3(SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, s);
                 rec(9);
