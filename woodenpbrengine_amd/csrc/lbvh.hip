@@ -1548,7 +1548,7 @@ WPBR_D void write_pair(const LbvhScratch& s, uint32_t r, bbox b0, bbox b1, uint3
 // Leaves of at most 4 shapes are a primitive range of their parent's record;
 // leaves of 5 .. BIG_LEAF shapes get one wave each (k_subtree_leaf); larger
 // ones (only when all their codes are equal) a workgroup (k_subtree_big).
-constexpr uint32_t SUBTREE_BLOCKS = 1024;
+constexpr uint32_t SUBTREE_BLOCKS = 8192;
 
 // One thread per reference node (grid-stride over the node count): interior
 // nodes get their record (both child boxes and words; a child leaf of at most
@@ -1991,7 +1991,7 @@ WPBR_D void bvh4_record(const LbvhScratch& s, uint32_t r) {
     if (!(q.w1 & 0x80000000u)) k1 = load_rec(s, q.w1);
     put_rec4(s, r, q, k0, k1);
 }
-constexpr uint32_t BVH4_BLOCKS = 1024;
+constexpr uint32_t BVH4_BLOCKS = 8192;
 WPBR_D void bvh4_body(const LbvhScratch& s, uint32_t bid, uint32_t nblk) {
     const uint32_t nn = *s.n_nodes;
     for (uint32_t r = bid * blockDim.x + threadIdx.x; r < nn; r += nblk * blockDim.x) {
