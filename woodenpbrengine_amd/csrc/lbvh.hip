@@ -2292,7 +2292,9 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_emit_small(const LbvhScratch* S,
 }
 
 // Treelets of at least 30 shapes: emitLBVH by one workgroup each.
-__global__ __launch_bounds__(EMIT_BLOCK) void kr_emit_big(const LbvhScratch* S, uint32_t G) {
+// trees = 0: the trees were built by ks_emit_trees (segmented build); only the
+// per-treelet epilogue and the centroid partials remain.
+__global__ __launch_bounds__(EMIT_BLOCK) void kr_emit_big(const LbvhScratch* S, uint32_t G, uint32_t trees) {
     REF_SCENE(S, G);
     __shared__ uint32_t s_red[6];
     const uint32_t nbig = *s.bigt_n, T = *s.n_treelets, n = s.n;
@@ -2301,7 +2303,7 @@ __global__ __launch_bounds__(EMIT_BLOCK) void kr_emit_big(const LbvhScratch* S, 
         const uint32_t t = s.bigt[q];
         const uint32_t start = s.t_start[t];
         const uint32_t end = t + 1 < T ? s.t_start[t + 1] : n;
-        emit_tree(s, s.keys_a, t, start, end - start);
+        if (trees) emit_tree(s, s.keys_a, t, start, end - start);
         __syncthreads();
         if (threadIdx.x == 0) {
             const bbox b = load_box(s.t_box + 6 * (size_t)t);
@@ -3087,10 +3089,14 @@ static void ref_launch_treelets(const LbvhScratch* S, uint32_t ns, const RefGrid
     hipLaunchKernelGGL(kr_head_scan, dim3(ns), dim3(1024), 0, st, S);
     hipLaunchKernelGGL(kr_treelets, dim3(ns * g.treelets), dim3(REF_BLOCK), 0, st, S, g.treelets);
 }
-static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g, hipStream_t st) {
+static void ks_launch_emit_trees(const LbvhBatch& B, hipStream_t st);
+static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g, hipStream_t st,
+                            const LbvhBatch* B = nullptr) {
     // exactly g.parts blocks per scene each: every partial slot is written
     hipLaunchKernelGGL(kr_emit_small, dim3(ns * g.parts), dim3(REF_BLOCK), 0, st, S, g.parts);
-    hipLaunchKernelGGL(kr_emit_big, dim3(ns * g.parts), dim3(EMIT_BLOCK), 0, st, S, g.parts);
+    // segmented: every scene's large treelets from one list over all waves
+    if (B) ks_launch_emit_trees(*B, st);
+    hipLaunchKernelGGL(kr_emit_big, dim3(ns * g.parts), dim3(EMIT_BLOCK), 0, st, S, g.parts, B ? 0u : 1u);
     hipLaunchKernelGGL(kr_root_cb, dim3(ns), dim3(64), 0, st, S, g.parts);
 }
 // The level loop: four levels between host checks of the next level's range
@@ -3326,6 +3332,45 @@ __global__ __launch_bounds__(64 * WR_WAVES) void ks_sub_wave(LbvhBatch B) {
         wr_range(s, W[wave], g - B.subwork[sc]);
     }
 }
+// Reference order, segmented: the scenes' treelets of at least 30 shapes in one
+// list (prefix of the scenes' counts in subwork), one wave each.  A per-scene
+// grid left the scenes with many large treelets to a few waves.
+__global__ void ks_big_count(LbvhBatch B) {
+    __shared__ uint32_t s_sum[1024];
+    uint32_t run = 0;
+    for (uint32_t base = 0; base < B.n_scenes; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        s_sum[threadIdx.x] = i < B.n_scenes ? *B.s[i].bigt_n : 0u;
+        __syncthreads();
+        for (uint32_t off = 1; off < blockDim.x; off <<= 1) {
+            uint32_t v = threadIdx.x >= off ? s_sum[threadIdx.x - off] : 0;
+            __syncthreads();
+            s_sum[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (i < B.n_scenes) B.subwork[i + 1] = run + s_sum[threadIdx.x];
+        if (base == 0 && threadIdx.x == 0) B.subwork[0] = 0;
+        run += s_sum[blockDim.x - 1];
+        __syncthreads();
+    }
+}
+constexpr uint32_t SEG_EMIT_TREE_BLOCKS = 8192;
+__global__ __launch_bounds__(EMIT_BLOCK) void ks_emit_trees(LbvhBatch B) {
+    const uint32_t total = B.subwork[B.n_scenes];
+    for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
+        const uint32_t sc = __builtin_amdgcn_readfirstlane(work_scene(B.subwork, B.n_scenes, g));
+        const LbvhScratch& s = B.s[sc];
+        const uint32_t t = s.bigt[g - B.subwork[sc]], T = *s.n_treelets;
+        const uint32_t start = s.t_start[t];
+        const uint32_t end = t + 1 < T ? s.t_start[t + 1] : s.n;
+        emit_tree(s, s.keys_a, t, start, end - start);
+        __syncthreads();
+    }
+}
+static void ks_launch_emit_trees(const LbvhBatch& B, hipStream_t st) {
+    hipLaunchKernelGGL(ks_big_count, dim3(1), dim3(1024), 0, st, B);
+    hipLaunchKernelGGL(ks_emit_trees, dim3(SEG_EMIT_TREE_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
+}
 constexpr uint32_t SEG_SAH_FINAL_BLOCKS = 4;  // per scene
 __global__ __launch_bounds__(SAH_BLOCK) void ks_sah_final(LbvhBatch B) {
     LbvhScratch s = B.s[blockIdx.x / SEG_SAH_FINAL_BLOCKS];
@@ -3405,7 +3450,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
     if (ref) {
         const RefGrid g = ref_grid(max_n, S);
         ref_launch_treelets(B.s, S, g, st);
-        ref_launch_emit(B.s, S, g, st);
+        ref_launch_emit(B.s, S, g, st, &B);
         wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st, ls);
         if (e) return e;
         hipLaunchKernelGGL(kr_scatter_ref, dim3(S * g.upper), dim3(REF_BLOCK), 0, st, B.s, g.upper);
