@@ -3101,10 +3101,12 @@ static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g,
 }
 // The level loop: four levels between host checks of the next level's range
 // count (kernels of an empty level return at once).
+constexpr uint32_t SEG_REF_SUB_BLOCKS = 4096;  // segmented reference order: deferred-range workgroups
 constexpr uint32_t REF_LEVELS_PER_CHECK = 2;  // measured against 4: 2.24 -> 2.22 ms at 1M shapes
 constexpr uint32_t REF_MAX_LEVELS = 4096;
+static void ks_launch_sub_wave(const LbvhBatch& B, uint32_t blocks, hipStream_t st);
 static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const RefGrid& g, uint32_t* d_active,
-                                    hipStream_t st, const LevelSync* ls) {
+                                    hipStream_t st, const LevelSync* ls, const LbvhBatch* B = nullptr) {
     uint32_t level = 0;
     // REF_LEVELS_PER_CHECK levels, then the next level's range count into d_active
     auto block = [&]() {
@@ -3147,7 +3149,10 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
             if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
         }
     }
-    hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
+    // the deferred ranges, one wave each: per scene from its cursor, or for a
+    // segmented build from one list over all scenes
+    if (B) ks_launch_sub_wave(*B, SEG_REF_SUB_BLOCKS, st);
+    else hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
     hipLaunchKernelGGL(kr_fin_sum, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
     hipLaunchKernelGGL(kr_fin_scan, dim3(ns), dim3(1024), 0, st, S);
     hipLaunchKernelGGL(kr_fin_place, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
@@ -3367,6 +3372,10 @@ __global__ __launch_bounds__(EMIT_BLOCK) void ks_emit_trees(LbvhBatch B) {
         __syncthreads();
     }
 }
+static void ks_launch_sub_wave(const LbvhBatch& B, uint32_t blocks, hipStream_t st) {
+    hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
+    hipLaunchKernelGGL(ks_sub_wave, dim3(blocks), dim3(64 * WR_WAVES), 0, st, B);
+}
 static void ks_launch_emit_trees(const LbvhBatch& B, hipStream_t st) {
     hipLaunchKernelGGL(ks_big_count, dim3(1), dim3(1024), 0, st, B);
     hipLaunchKernelGGL(ks_emit_trees, dim3(SEG_EMIT_TREE_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
@@ -3451,7 +3460,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
         const RefGrid g = ref_grid(max_n, S);
         ref_launch_treelets(B.s, S, g, st);
         ref_launch_emit(B.s, S, g, st, &B);
-        wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st, ls);
+        wpbr_status e = ref_launch_upper(B.s, S, g, B.info + 5 * (size_t)S, st, ls, &B);
         if (e) return e;
         hipLaunchKernelGGL(kr_scatter_ref, dim3(S * g.upper), dim3(REF_BLOCK), 0, st, B.s, g.upper);
         hipLaunchKernelGGL(kr_big_scan, dim3(S), dim3(1024), 0, st, B.s);
@@ -3460,8 +3469,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
         hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
         hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
-        hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
-        hipLaunchKernelGGL(ks_sub_wave, dim3(SAH_SEG_SUB_BLOCKS), dim3(64 * WR_WAVES), 0, st, B);
+        ks_launch_sub_wave(B, SAH_SEG_SUB_BLOCKS, st);
         hipLaunchKernelGGL(ks_sah_final, dim3(S * SEG_SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
     }
