@@ -553,8 +553,8 @@ def main():
                           % (s_rays / s_launch),
                 "avg_launch_us": avg_launch_ms * 1e3,
                 "limiter": "latency/L2: divergence and the latency of node and primitive gathers, not HBM "
-                           "bandwidth (profiles/render_c2_pmc.txt: 28 % VALU busy, 28 % of lanes active, "
-                           "L1 94 %, L2 56 %, 254 GB/s of HBM-side traffic)"}
+                           "bandwidth (profiles/render_c2_pmc.txt: 29 % VALU busy, 33 % of lanes active, "
+                           "L1 95 %, L2 57 %, 262 GB/s of HBM-side traffic)"}
     mb = {}
     if not args.no_microbench:
         mb = bench_microbench(lib, scenes, ctx, args.steps)
