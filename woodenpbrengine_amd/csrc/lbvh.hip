@@ -2771,6 +2771,10 @@ struct WaveRangeLds {
     uint32_t bcount[1][NB];
     uint32_t bmin[1][NB][3];
     uint32_t bmax[1][NB][3];
+    uint32_t mid[16];                    // tasks of ROW_MIN..ROW_MAX roots waiting for a row split
+    float rcost[4][16];                  // row splits: split costs and left counts per row
+    uint32_t rcnt[4][16];
+    uint32_t rout[4][2];                 // row splits: each row's two children
     uint32_t uctr;
 };
 static_assert(LANE_STACK * 64 >= WR_CAP, "aux holds a range's old g_perm");
@@ -2817,6 +2821,111 @@ WPBR_D void wr_lane_task(LbvhScratch& s, WaveRangeLds& L, const SahTasks& T, uin
     }
 }
 
+// Tasks of ROW_MIN..ROW_MAX roots are split four at a time, one per 16-lane
+// row of the wave (a wave split of one such task spends most of its time in
+// work that does not depend on its size): lane j of a row holds root j of the
+// row's task; the range box and centroid bounds are reduced over the row by
+// DPP rotations; lane s < 11 of the row sums split s's two sides over the
+// task's roots in position order; lane 0 of the row takes the first minimum of
+// the eleven costs in order (SLBVH.cpp:540-548) and the partition is the
+// libstdc++ two-pointer swap in rank form, over row-masked ballots.
+constexpr uint32_t ROW_MIN = WR_LANE_MAX + 1, ROW_MAX = 16;
+template <bool MAX>
+WPBR_D float row_allreduce(float v) {
+    auto op = [](float a, float b) { return MAX ? smax(a, b) : smin(a, b); };
+#define WPBR_ROW_ROR(n) v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + (n), 0xF, 0xF, false)))
+    WPBR_ROW_ROR(8);
+    WPBR_ROW_ROR(4);
+    WPBR_ROW_ROR(2);
+    WPBR_ROW_ROR(1);
+#undef WPBR_ROW_ROR
+    return v;
+}
+// Splits L.mid[base .. base + nt) (nt <= 4); row k's children go to L.rout[k].
+WPBR_D void wr_row_split(LbvhScratch& s, WaveRangeLds& L, const SahTasks& T, uint32_t base, uint32_t nt) {
+    const uint32_t lane = threadIdx.x & 63, row = lane >> 4, j = lane & 15;
+    const bool act = row < nt;
+    const uint32_t task = act ? L.mid[base + row] : 0u;
+    const uint32_t a = task & 255u, b = (task >> 8) & 255u, d = (task >> 16) & 255u, r = task >> 24;
+    const uint32_t m = act ? b - a + 1 : 0u;
+    const bool mine = j < m;
+    float v[12];
+#pragma unroll
+    for (int c = 0; c < 3; c++) { v[c] = FLT_MAX_; v[3 + c] = FLT_LOWEST_; v[6 + c] = FLT_MAX_; v[9 + c] = FLT_LOWEST_; }
+    if (mine) {
+        const uint32_t t = L.perm[a + j];
+        const bbox rb = tb_box(L, t);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            v[c] = rb.mn[c];
+            v[3 + c] = rb.mx[c];
+            v[6 + c] = v[9 + c] = tb_cen(L, t, c);
+        }
+    }
+    bbox bounds, cb;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        bounds.mn[c] = row_allreduce<false>(v[c]);
+        bounds.mx[c] = row_allreduce<true>(v[3 + c]);
+        cb.mn[c] = row_allreduce<false>(v[6 + c]);
+        cb.mx[c] = row_allreduce<true>(v[9 + c]);
+    }
+    const int dim = box_max_extent(cb);
+    const float cbmn = dim == 0 ? cb.mn[0] : (dim == 1 ? cb.mn[1] : cb.mn[2]);
+    const float dist = (dim == 0 ? cb.mx[0] : (dim == 1 ? cb.mx[1] : cb.mx[2])) - cbmn;
+    if (mine) L.bucket[a + j] = (uint8_t)sah_bucket(tb_cen(L, L.perm[a + j], dim), cbmn, dist);
+    __builtin_amdgcn_wave_barrier();
+    if (act && j < NB - 1) {
+        bbox b0 = box_empty(), b1 = box_empty();
+        uint32_t c0 = 0, c1 = 0;
+        for (uint32_t k = 0; k < m; k++) {
+            const bbox tb = tb_box(L, L.perm[a + k]);
+            if (L.bucket[a + k] <= j) { b0 = box_union(b0, tb); c0++; }
+            else { b1 = box_union(b1, tb); c1++; }
+        }
+        const float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+        const float rr = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+        L.rcost[row][j] = (float)(0.125 + (double)((l + rr) / box_area(bounds)));
+        L.rcnt[row][j] = c0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t best = 0;
+    if (act && j == 0) {
+        float cbest = L.rcost[row][0];
+        for (uint32_t i = 1; i < NB - 1; i++)
+            if (L.rcost[row][i] < cbest) { best = i; cbest = L.rcost[row][i]; }
+        L.rout[row][0] = best;
+    }
+    __builtin_amdgcn_wave_barrier();
+    best = act ? L.rout[row][0] : 0u;
+    const uint32_t Tn = act ? L.rcnt[row][best] : 0u;
+    const uint64_t rmask = 0xFFFFull << (16 * row);
+    // misplaced positions: left of the split with a bucket above best, right of
+    // it (counted from the right) with a bucket at or below best
+    const bool part = act && Tn != m;
+    const bool badl = part && j < Tn && L.bucket[a + j] > best;
+    const bool badr = part && j < m - Tn && L.bucket[b - j] <= best;
+    const uint64_t ml = ballot64(badl) & rmask, mr = ballot64(badr) & rmask;
+    const uint64_t below = lanemask_lt() & rmask;
+    if (badl) L.la[a + popc64(ml & below)] = (uint16_t)(a + j);
+    if (badr) L.lb[a + popc64(mr & below)] = (uint16_t)(b - j);
+    __builtin_amdgcn_wave_barrier();
+    if (part && j < popc64(ml)) {
+        const uint32_t pa = L.la[a + j], pb = L.lb[a + j];
+        const uint16_t ta = L.perm[pa], tb = L.perm[pb];
+        L.perm[pa] = tb;
+        L.perm[pb] = ta;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (act && j == 0) {
+        const uint32_t midp = (Tn == m) ? m / 2 + a : a + Tn;
+        sah_node(s, T, a, b, midp, dim, bounds, d, r);
+        L.rout[row][0] = wr_pack(a, midp - 1, d + 1, r);
+        L.rout[row][1] = wr_pack(midp, b, d + 1, r + 1);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 WPBR_D void wr_range(LbvhScratch& s, WaveRangeLds& L, uint32_t st) {
     const int lane = threadIdx.x & 63;
     const uint32_t i0 = s.sub_i0[st], m = s.sub_i1[st] - i0 + 1;
@@ -2837,8 +2946,32 @@ WPBR_D void wr_range(LbvhScratch& s, WaveRangeLds& L, uint32_t st) {
         L.stack[0] = wr_pack(0, m - 1, 0, 0);
     }
     __builtin_amdgcn_wave_barrier();
-    uint32_t sp = 1, nsmall = 0;  // uniform across the wave
-    while (sp || nsmall) {
+    uint32_t sp = 1, nsmall = 0, nmid = 0;  // uniform across the wave
+    while (sp || nsmall || nmid) {
+        if (nmid >= 4 || (nmid && !sp)) {
+            // the last (up to) four tasks of ROW_MIN..ROW_MAX roots, one per
+            // row; their children go to the lane list or back to this list (a
+            // task of at most ROW_MAX roots has at most one child above
+            // WR_LANE_MAX, so the list never grows past seven)
+            const uint32_t nt = nmid < 4 ? nmid : 4u, base = nmid - nt;
+            wr_row_split(s, L, T, base, nt);
+            uint32_t ns = nsmall, nm = base;
+            for (uint32_t k = 0; k < 2 * nt; k++) {
+                const uint32_t c = L.rout[k >> 1][k & 1];
+                const uint32_t sz = ((c >> 8) & 255u) - (c & 255u) + 1;
+                if (sz <= WR_LANE_MAX) {
+                    if (lane == 0) L.small[ns] = c;
+                    ns++;
+                } else {
+                    if (lane == 0) L.mid[nm] = c;
+                    nm++;
+                }
+            }
+            nsmall = ns;
+            nmid = nm;
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
         if (sp && nsmall < 64) {
             const uint32_t task = L.stack[--sp];
             const uint32_t a = task & 255u, b = (task >> 8) & 255u, d = (task >> 16) & 255u, r = task >> 24;
@@ -2846,6 +2979,9 @@ WPBR_D void wr_range(LbvhScratch& s, WaveRangeLds& L, uint32_t st) {
             if (b - a + 1 <= WR_LANE_MAX) {
                 if (lane == 0) L.small[nsmall] = task;
                 nsmall++;
+            } else if (b - a + 1 <= ROW_MAX) {
+                if (lane == 0) L.mid[nmid] = task;
+                nmid++;
             } else {
                 int dim;
                 bbox bounds;
@@ -3327,12 +3463,13 @@ __global__ void ks_sah_sub_count(LbvhBatch B) {
 // One workgroup per range (as k_sah_sub) was measured slower here: the 64
 // scenes' deep, unbalanced ranges keep every workgroup's barriers on their
 // critical paths (segmented full-order build 1.76 -> 1.68 ms).
-__global__ __launch_bounds__(64 * WR_WAVES) void ks_sub_wave(LbvhBatch B) {
+__global__ __launch_bounds__(64 * WR_WAVES, 3) void ks_sub_wave(LbvhBatch B) {
     __shared__ WaveRangeLds W[WR_WAVES];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t total = B.subwork[B.n_scenes];
     for (uint32_t g = blockIdx.x * WR_WAVES + wave; g < total; g += gridDim.x * WR_WAVES) {
-        const uint32_t sc = work_scene(B.subwork, B.n_scenes, g);
+        // wave-uniform scene: its scratch pointers stay in scalar registers
+        const uint32_t sc = __builtin_amdgcn_readfirstlane(work_scene(B.subwork, B.n_scenes, g));
         LbvhScratch s = B.s[sc];
         wr_range(s, W[wave], g - B.subwork[sc]);
     }
