@@ -643,15 +643,18 @@ constexpr uint32_t LANE_TASK_MAX_SUB = 8;   // lane tasks of sah_sub
 constexpr uint32_t SAH_SUB_TASKS = 2 * SAH_CUT;  // a deferred range makes fewer tasks than this
 static_assert(LANE_STACK * SAH_BLOCK <= 4100, "lane stacks share the prefix-sum array");
 
+// Bucket statistics rows: SC copies per wave, lane l of a wave adding to copy
+// l % SC, so that fewer lanes of one LDS atomic meet on one address.
 struct SahLds {
+    static constexpr int SC = 4;
     alignas(16) float tbox[6 * 4096];  // treelet root boxes
     uint16_t perm[4096];
     uint8_t bucket[4096];
     uint16_t la[4096], lb[4096];            // partition lists; then leaf depth / right turns
     uint32_t aux[4100];         // lane stacks (LANE_STACK x SAH_BLOCK); then the prefix sums P[0..k]
-    uint32_t bcount[SAH_WAVES][NB];
-    uint32_t bmin[SAH_WAVES][NB][3];
-    uint32_t bmax[SAH_WAVES][NB][3];
+    uint32_t bcount[SAH_WAVES * SC][NB];
+    uint32_t bmin[SAH_WAVES * SC][NB][3];
+    uint32_t bmax[SAH_WAVES * SC][NB][3];
     float red[SAH_WAVES][12];
     float fin[12];
     uint32_t wcnt[SAH_WAVES];
@@ -739,23 +742,84 @@ WPBR_D void sah_children(const SahTasks& T, SahLds& L, uint32_t i0, uint32_t i1,
     if (nb) atomicAdd(&L.nbig_next, nb);
 }
 
-// The reference's split cost (SLBVH.cpp:504-538) for split `lane` (< NB-1) from
-// merged bucket statistics row 0; lanes >= NB-1 return +inf.
-WPBR_D float sah_cost(const SahLds& L, int lane, const bbox& bounds, uint32_t* cnt_le) {
+// Treelet root centroid on one axis from its box: (pMin + pMax) * 0.5
+// (SLBVH.cpp:472, 487), as tb_cen.
+WPBR_D float box_cen(const bbox& b, int c) { return (float)((double)(b.mn[c] + b.mx[c]) * 0.5); }
+
+// Inclusive scans over the 16 lanes of each row: UP from lane 0 (row_shr), or
+// down from lane 15 (row_shl); lanes whose source lies outside the row take the
+// identity.  All lanes must be active.
+template <bool UP, bool MAX>
+WPBR_D float row_scan(float v) {
+    const int id = __float_as_int(MAX ? FLT_LOWEST_ : FLT_MAX_);
+    auto op = [](float a, float b) { return MAX ? smax(a, b) : smin(a, b); };
+#define WPBR_ROW_SCAN(d) v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), (UP ? 0x110 : 0x100) + (d), 0xF, 0xF, false)))
+    WPBR_ROW_SCAN(1);
+    WPBR_ROW_SCAN(2);
+    WPBR_ROW_SCAN(4);
+    WPBR_ROW_SCAN(8);
+#undef WPBR_ROW_SCAN
+    return v;
+}
+template <bool UP>
+WPBR_D uint32_t row_scan_add(uint32_t v) {
+#define WPBR_ROW_SCAN(d) v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, (UP ? 0x110 : 0x100) + (d), 0xF, 0xF, false)
+    WPBR_ROW_SCAN(1);
+    WPBR_ROW_SCAN(2);
+    WPBR_ROW_SCAN(4);
+    WPBR_ROW_SCAN(8);
+#undef WPBR_ROW_SCAN
+    return v;
+}
+// Lane value of lane + 1 in the row (identity for lane 15).
+WPBR_D float row_next(float v, float id) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(v), 0x101, 0xF, 0xF, false));
+}
+WPBR_D uint32_t row_next(uint32_t v) { return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x101, 0xF, 0xF, false); }
+
+// Bucket j's statistics for lane j < NB of a wave, merged over ROWS
+// consecutive statistics rows (count, and the box from its order-preserving
+// min / max words); the other lanes get the identity (count 0, empty box).
+template <int ROWS>
+WPBR_D void bucket_lane_load(const uint32_t (*bc)[NB], const uint32_t (*bmn)[NB][3], const uint32_t (*bmx)[NB][3],
+                             int lane, uint32_t& cnt, float (&mn)[3], float (&mx)[3]) {
+    cnt = 0;
+    uint32_t wn[3], wx[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) { wn[c] = f2ord(FLT_MAX_); wx[c] = f2ord(FLT_LOWEST_); }
+    if (lane < NB)
+#pragma unroll
+        for (int r = 0; r < ROWS; r++) {
+            cnt += bc[r][lane];
+#pragma unroll
+            for (int c = 0; c < 3; c++) { wn[c] = min(wn[c], bmn[r][lane][c]); wx[c] = max(wx[c], bmx[r][lane][c]); }
+        }
+#pragma unroll
+    for (int c = 0; c < 3; c++) { mn[c] = ord2f(wn[c]); mx[c] = ord2f(wx[c]); }
+}
+
+// The reference's split cost (SLBVH.cpp:504-538) for split `lane` (< NB-1), lane
+// j < NB holding bucket j (bucket_lane_load): the two sides of every split come
+// from an inclusive prefix and an inclusive suffix over the row's lanes (DPP),
+// lane s taking the prefix at s and the suffix at s + 1.  A side's box is the
+// union of the same bucket boxes as the reference's loop (min/max are exact), so
+// the cost is the reference's.  Lanes >= NB-1 return +inf.  All lanes must be
+// active.
+WPBR_D float sah_cost_lanes(uint32_t cnt, const float (&mn)[3], const float (&mx)[3], int lane, const bbox& bounds,
+                            uint32_t* cnt_le) {
+    bbox b0, b1;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        b0.mn[c] = row_scan<true, false>(mn[c]);
+        b0.mx[c] = row_scan<true, true>(mx[c]);
+        b1.mn[c] = row_next(row_scan<false, false>(mn[c]), FLT_MAX_);
+        b1.mx[c] = row_next(row_scan<false, true>(mx[c]), FLT_LOWEST_);
+    }
+    const uint32_t c0 = row_scan_add<true>(cnt);
+    const uint32_t c1 = row_next(row_scan_add<false>(cnt));
     float cost = INF;
     *cnt_le = 0;
     if (lane < NB - 1) {
-        bbox b0 = box_empty(), b1 = box_empty();
-        uint32_t c0 = 0, c1 = 0;
-        for (int j = 0; j < NB; j++) {
-            bbox bj;
-            for (int c = 0; c < 3; c++) {
-                bj.mn[c] = ord2f(L.bmin[0][j][c]);
-                bj.mx[c] = ord2f(L.bmax[0][j][c]);
-            }
-            if (j <= lane) { b0 = box_union(b0, bj); c0 += L.bcount[0][j]; }
-            else { b1 = box_union(b1, bj); c1 += L.bcount[0][j]; }
-        }
         float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
         float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
         cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
@@ -886,11 +950,13 @@ WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint3
     if (lane == 0)
 #pragma unroll
         for (int q = 0; q < 12; q++) L.red[wave][q] = v[q];
-    if (lane < NB) {
-        L.bcount[wave][lane] = 0;
+    constexpr int SC = SahLds::SC;
+    if (lane < NB * SC) {
+        const int row = wave * SC + lane / NB, b = lane % NB;
+        L.bcount[row][b] = 0;
         for (int c = 0; c < 3; c++) {
-            L.bmin[wave][lane][c] = f2ord(FLT_MAX_);
-            L.bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
+            L.bmin[row][b][c] = f2ord(FLT_MAX_);
+            L.bmax[row][b][c] = f2ord(FLT_LOWEST_);
         }
     }
     __syncthreads();
@@ -921,29 +987,20 @@ WPBR_D void sah_block_split(LbvhScratch& s, SahLds& L, const SahTasks& TL, uint3
         bbox rb = tb_box(L, tr);
         uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
         L.bucket[i] = (uint8_t)ib;
-        atomicAdd(&L.bcount[wave][ib], 1u);
+        const int row = wave * SC + (lane % SC);
+        atomicAdd(&L.bcount[row][ib], 1u);
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-            atomicMin(&L.bmin[wave][ib][c], f2ord(rb.mn[c]));
-            atomicMax(&L.bmax[wave][ib][c], f2ord(rb.mx[c]));
+            atomicMin(&L.bmin[row][ib][c], f2ord(rb.mn[c]));
+            atomicMax(&L.bmax[row][ib][c], f2ord(rb.mx[c]));
         }
-    }
-    __syncthreads();
-    if (tid < NB) {
-        uint32_t cnt = L.bcount[0][tid];
-        uint32_t mn[3], mx[3];
-        for (int c = 0; c < 3; c++) { mn[c] = L.bmin[0][tid][c]; mx[c] = L.bmax[0][tid][c]; }
-        for (int w = 1; w < SAH_WAVES; w++) {
-            cnt += L.bcount[w][tid];
-            for (int c = 0; c < 3; c++) { mn[c] = min(mn[c], L.bmin[w][tid][c]); mx[c] = max(mx[c], L.bmax[w][tid][c]); }
-        }
-        L.bcount[0][tid] = cnt;
-        for (int c = 0; c < 3; c++) { L.bmin[0][tid][c] = mn[c]; L.bmax[0][tid][c] = mx[c]; }
     }
     __syncthreads();
     if (wave == 0) {
-        uint32_t cnt_le;
-        float cost = sah_cost(L, lane, bounds, &cnt_le);
+        uint32_t cnt, cnt_le;
+        float mn[3], mx[3];
+        bucket_lane_load<SAH_WAVES * SC>(L.bcount, L.bmin, L.bmax, lane, cnt, mn, mx);
+        float cost = sah_cost_lanes(cnt, mn, mx, lane, bounds, &cnt_le);
         int best = sah_best(cost);
         uint32_t T = (uint32_t)__shfl((int)cnt_le, best);
         if (lane == 0) { L.split[0] = (uint32_t)best; L.split[1] = T; }
@@ -1017,18 +1074,30 @@ WPBR_D uint32_t wave_split_mid(LDS& L, int wave, uint32_t i0, uint32_t i1, int* 
     const int lane = threadIdx.x & 63;
     float bmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, bmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
     float cmn[3] = {FLT_MAX_, FLT_MAX_, FLT_MAX_}, cmx[3] = {FLT_LOWEST_, FLT_LOWEST_, FLT_LOWEST_};
-    for (uint32_t i = i0 + lane; i <= i1; i += 64) {
-        uint32_t tr = L.perm[i];
-        bbox rb = tb_box(L, tr);
+    auto acc = [&](const bbox& rb) {
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             bmn[c] = smin(bmn[c], rb.mn[c]);
             bmx[c] = smax(bmx[c], rb.mx[c]);
-            float cc = tb_cen(L, tr, c);
+            const float cc = box_cen(rb, c);
             cmn[c] = smin(cmn[c], cc);
             cmx[c] = smax(cmx[c], cc);
         }
+    };
+    // range box and centroid bounds: four positions per lane per step, their
+    // LDS loads issued before any is used (the walk is bound by LDS latency)
+    uint32_t i = i0 + lane;
+    for (; i + 192 <= i1; i += 256) {
+        uint32_t tr[4];
+        bbox rb[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) tr[q] = L.perm[i + 64 * q];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rb[q] = tb_box(L, tr[q]);
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc(rb[q]);
     }
+    for (; i <= i1; i += 64) acc(tb_box(L, L.perm[i]));
     bbox bounds, cb;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -1076,49 +1145,41 @@ WPBR_D uint32_t wave_split_mid(LDS& L, int wave, uint32_t i0, uint32_t i1, int* 
             cnt_le = c0;
         }
     } else {
-        // per-wave statistics in row `wave`
-        if (lane < NB) {
-            L.bcount[wave][lane] = 0;
+        // per-wave statistics in rows wave * SC .. wave * SC + SC - 1
+        constexpr int SC = LDS::SC;
+        const int row0 = wave * SC;
+        if (lane < NB * SC) {
+            const int row = row0 + lane / NB, b = lane % NB;
+            L.bcount[row][b] = 0;
             for (int c = 0; c < 3; c++) {
-                L.bmin[wave][lane][c] = f2ord(FLT_MAX_);
-                L.bmax[wave][lane][c] = f2ord(FLT_LOWEST_);
+                L.bmin[row][b][c] = f2ord(FLT_MAX_);
+                L.bmax[row][b][c] = f2ord(FLT_LOWEST_);
             }
         }
         __builtin_amdgcn_wave_barrier();
-        // odd-length runs per lane, as in the workgroup split
+        // odd-length runs per lane, as in the workgroup split (consecutive
+        // positions of a lane often share a bucket: their atomics are kept
+        // apart, one position per step)
         const uint32_t run = ((size - 1) / 64 + 1) | 1u;
+        const int row = row0 + (lane % SC);
         for (uint32_t j = 0; j < run; j++) {
             const uint32_t i = i0 + (uint32_t)lane * run + j;
             if (i > i1) break;
-            uint32_t tr = L.perm[i];
-            bbox rb = tb_box(L, tr);
-            uint32_t ib = sah_bucket(tb_cen(L, tr, dim), cbmn, dist);
+            const bbox rb = tb_box(L, L.perm[i]);
+            const uint32_t ib = sah_bucket(box_cen(rb, dim), cbmn, dist);
             L.bucket[i] = (uint8_t)ib;
-            atomicAdd(&L.bcount[wave][ib], 1u);
+            atomicAdd(&L.bcount[row][ib], 1u);
 #pragma unroll
             for (int c = 0; c < 3; c++) {
-                atomicMin(&L.bmin[wave][ib][c], f2ord(rb.mn[c]));
-                atomicMax(&L.bmax[wave][ib][c], f2ord(rb.mx[c]));
+                atomicMin(&L.bmin[row][ib][c], f2ord(rb.mn[c]));
+                atomicMax(&L.bmax[row][ib][c], f2ord(rb.mx[c]));
             }
         }
         __builtin_amdgcn_wave_barrier();
-        if (lane < NB - 1) {
-            bbox b0 = box_empty(), b1 = box_empty();
-            uint32_t c0 = 0, c1 = 0;
-            for (int j = 0; j < NB; j++) {
-                bbox bj;
-                for (int c = 0; c < 3; c++) {
-                    bj.mn[c] = ord2f(L.bmin[wave][j][c]);
-                    bj.mx[c] = ord2f(L.bmax[wave][j][c]);
-                }
-                if (j <= lane) { b0 = box_union(b0, bj); c0 += L.bcount[wave][j]; }
-                else { b1 = box_union(b1, bj); c1 += L.bcount[wave][j]; }
-            }
-            float l = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
-            float r = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
-            cost = (float)(0.125 + (double)((l + r) / box_area(bounds)));
-            cnt_le = c0;
-        }
+        uint32_t cnt;
+        float mn[3], mx[3];
+        bucket_lane_load<SC>(L.bcount + row0, L.bmin + row0, L.bmax + row0, lane, cnt, mn, mx);
+        cost = sah_cost_lanes(cnt, mn, mx, lane, bounds, &cnt_le);
     }
     // first minimum over the NB - 1 costs (in lanes 4s, or lanes s)
     const int cs = by_lane ? 4 : 1;
@@ -1133,21 +1194,42 @@ WPBR_D uint32_t wave_split_mid(LDS& L, int wave, uint32_t i0, uint32_t i1, int* 
     if (T == size) {
         mid = size / 2 + i0;
     } else {
+        // misplaced positions in rank order, four 64-position chunks per step
+        const uint64_t lt = lanemask_lt();
         uint32_t rl = 0;
-        for (uint32_t c = 0; c * 64 < T; c++) {
-            uint32_t i = i0 + c * 64 + lane;
-            bool bad = (c * 64 + lane < T) && (L.bucket[i] > (uint32_t)best);
-            uint64_t m = ballot64(bad);
-            if (bad) L.la[i0 + rl + popc64(m & lanemask_lt())] = (uint16_t)i;
-            rl += popc64(m);
+        for (uint32_t c = 0; c < T; c += 256) {
+            uint32_t bk[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t k = c + 64 * q + lane;
+                bk[q] = k < T ? L.bucket[i0 + k] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t k = c + 64 * q + lane;
+                const bool bad = k < T && bk[q] > (uint32_t)best;
+                const uint64_t m = ballot64(bad);
+                if (bad) L.la[i0 + rl + popc64(m & lt)] = (uint16_t)(i0 + k);
+                rl += popc64(m);
+            }
         }
-        uint32_t rr = 0, rsz = size - T;
-        for (uint32_t c = 0; c * 64 < rsz; c++) {
-            uint32_t i = i1 - (c * 64 + lane);
-            bool bad = (c * 64 + lane < rsz) && (L.bucket[i] <= (uint32_t)best);
-            uint64_t m = ballot64(bad);
-            if (bad) L.lb[i0 + rr + popc64(m & lanemask_lt())] = (uint16_t)i;
-            rr += popc64(m);
+        uint32_t rr = 0;
+        const uint32_t rsz = size - T;
+        for (uint32_t c = 0; c < rsz; c += 256) {
+            uint32_t bk[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t k = c + 64 * q + lane;
+                bk[q] = k < rsz ? L.bucket[i1 - k] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t k = c + 64 * q + lane;
+                const bool bad = k < rsz && bk[q] <= (uint32_t)best;
+                const uint64_t m = ballot64(bad);
+                if (bad) L.lb[i0 + rr + popc64(m & lt)] = (uint16_t)(i1 - k);
+                rr += popc64(m);
+            }
         }
         __builtin_amdgcn_wave_barrier();
         for (uint32_t r = lane; r < rl; r += 64) {
@@ -2761,6 +2843,7 @@ constexpr uint32_t WR_CAP = 256;
 constexpr uint32_t WR_WAVES = 4;
 constexpr uint32_t WR_LANE_MAX = 8;
 struct WaveRangeLds {
+    static constexpr int SC = 1;  // bucket statistics copies
     alignas(16) float tbox[6 * WR_CAP];
     uint16_t perm[WR_CAP];
     uint8_t bucket[WR_CAP];
@@ -2768,9 +2851,9 @@ struct WaveRangeLds {
     uint32_t stack[WR_CAP];              // pending tasks
     uint32_t small[WR_CAP];              // tasks for the lanes
     uint32_t aux[LANE_STACK * 64];       // the lanes' DFS stacks; then the range's old g_perm
-    uint32_t bcount[1][NB];
-    uint32_t bmin[1][NB][3];
-    uint32_t bmax[1][NB][3];
+    uint32_t bcount[SC][NB];
+    uint32_t bmin[SC][NB][3];
+    uint32_t bmax[SC][NB][3];
     uint32_t mid[16];                    // tasks of ROW_MIN..ROW_MAX roots waiting for a row split
     float rcost[4][16];                  // row splits: split costs and left counts per row
     uint32_t rcnt[4][16];
