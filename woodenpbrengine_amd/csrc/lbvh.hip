@@ -2545,156 +2545,170 @@ WPBR_D void wave_alloc3(uint32_t* c0, uint32_t* c1, uint32_t* c2, const uint32_t
     for (int k = 0; k < 3; k++) got[k] = (uint32_t)__builtin_amdgcn_readlane((int)b, k) + incl[k] - want[k];
 }
 
-// The split of every range (SLBVH.cpp:504-553), one lane per range: each
-// workgroup stages the statistics and records of LV_RPB consecutive ranges in
-// LDS (coalesced), then lane r of wave 0 computes range r's split costs
-// (prefix and suffix sweeps of the bucket boxes; min/max are exact, so the
-// sides' boxes, areas and costs are those of the reference's per-split
-// unions), the first minimum and the midpoint fallback, the upper node
-// (numbered from the range's first number: the left subtree's follow it, then
-// the right subtree's) and the children: single positions, deferred ranges
-// below REF_CUT, or next-level ranges.  List slots and chunks are taken with
-// one atomic per counter and wave.
-constexpr uint32_t LV_RPB = 64;
+// Order-preserving words' inclusive min (MAX false) or max scan over the 16
+// lanes of each row, up from lane 0 or down from lane 15 (row_scan's integer
+// form).  All lanes must be active.
+template <bool UP, bool MAX>
+WPBR_D uint32_t row_scan_u(uint32_t v) {
+    const int id = MAX ? 0 : -1;
+#define WPBR_ROW_SCAN(d)                                                                                          \
+    {                                                                                                             \
+        const uint32_t x = (uint32_t)__builtin_amdgcn_update_dpp(id, (int)v, (UP ? 0x110 : 0x100) + (d), 0xF, 0xF, \
+                                                                 false);                                          \
+        v = MAX ? max(v, x) : min(v, x);                                                                          \
+    }
+    WPBR_ROW_SCAN(1);
+    WPBR_ROW_SCAN(2);
+    WPBR_ROW_SCAN(4);
+    WPBR_ROW_SCAN(8);
+#undef WPBR_ROW_SCAN
+    return v;
+}
+
+// The split of every range (SLBVH.cpp:504-553), one 16-lane row per range:
+// lane j < NB of the row holds bucket j's statistics; the side boxes and counts
+// of the eleven splits are inclusive prefix and suffix scans over the row (DPP;
+// min/max are exact, so the sides' boxes, areas and costs are those of the
+// reference's per-split unions), and every lane of the row takes the first
+// minimum of the costs in order and the midpoint fallback.  The row's first
+// lane then writes the upper node (numbered from the range's first number: the
+// left subtree's follow it, then the right subtree's) and the children: single
+// positions, deferred ranges below REF_CUT, or next-level ranges, whose
+// centroid bounds are the same scans over their buckets' centroid bounds.  List
+// slots and chunks are taken with one atomic per counter and wave.
+constexpr uint32_t LV_RPB = 16;  // ranges per workgroup step: four per wave
 __global__ __launch_bounds__(REF_BLOCK) void kr_lv_ranges(const LbvhScratch* S, uint32_t G, uint32_t level) {
     REF_SCENE(S, G);
-    __shared__ uint32_t l_st[LV_RPB * REF_ST_WORDS];
-    __shared__ uint32_t l_rec[LV_RPB * REF_RG_WORDS];
     const uint32_t cur = level & 1u, nxt = cur ^ 1u;
     const uint32_t nr = s.lv_n[cur];
+    const uint32_t lane = threadIdx.x & 63u, row = lane >> 4, j = lane & 15u, wave = threadIdx.x >> 6;
+    const uint32_t rbase = lane & ~15u;
     for (uint32_t r0 = rj * LV_RPB; r0 < nr; r0 += G * LV_RPB) {
-        const uint32_t cnt = min(LV_RPB, nr - r0);
-        {
-            const uint32_t* gst = ref_stat(s, cur, r0);
-            const uint32_t* grc = ref_rec(s, cur, r0);
-            for (uint32_t q = threadIdx.x; q < cnt * REF_ST_WORDS; q += REF_BLOCK) l_st[q] = gst[q];
-            for (uint32_t q = threadIdx.x; q < cnt * REF_RG_WORDS; q += REF_BLOCK) l_rec[q] = grc[q];
+        const uint32_t r = r0 + 4 * wave + row;
+        const bool valid = r < nr;
+        const uint32_t* gs = ref_stat(s, cur, valid ? r : 0u);
+        const uint32_t* rec = ref_rec(s, cur, valid ? r : 0u);
+        // bucket j of the row's range (identities for the row's other lanes)
+        const bool has = valid && j < (uint32_t)NB;
+        const uint32_t* bj = gs + 13 * (has ? j : 0u);
+        const uint32_t cnt = has ? bj[0] : 0u;
+        float mn[3], mx[3];
+        uint32_t cw[6];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            mn[c] = has ? ord2f(bj[1 + c]) : FLT_MAX_;
+            mx[c] = has ? ord2f(bj[4 + c]) : FLT_LOWEST_;
         }
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            const uint32_t lr = threadIdx.x, r = r0 + lr;
-            const bool valid = lr < cnt;
-            const uint32_t* gs = l_st + lr * REF_ST_WORDS;
-            const uint32_t* rec = l_rec + lr * REF_RG_WORDS;
-            uint32_t i0 = 0, i1 = 0, mid = 0, best = 0, ub = 0, dep = 0, rt = 0;
-            bool fallback = false;
-            if (valid) {
-                i0 = rec[RG_I0];
-                i1 = rec[RG_I1];
-                dep = rec[RG_DEP];
-                rt = rec[RG_RT];
-                ub = rec[RG_UB];
-                const uint32_t m = i1 - i0 + 1;
-                const bbox bounds = ord_box(gs + ST_BOX);
-                const float area = box_area(bounds);
-                float la[NB - 1], ra[NB - 1];
-                uint32_t cle[NB - 1];
-                {
-                    bbox acc = box_empty();
-                    uint32_t c = 0;
 #pragma unroll
-                    for (int i = 0; i < NB - 1; i++) {
-                        acc = box_union(acc, ord_box(gs + 13 * i + 1));
-                        c += gs[13 * i];
-                        la[i] = (c == 0) ? 0.0f : (float)c * box_area(acc);
-                        cle[i] = c;
-                    }
-                    acc = box_empty();
-                    c = 0;
+        for (int q = 0; q < 6; q++) cw[q] = (has && cnt != 0u) ? bj[7 + q] : (q < 3 ? 0xFFFFFFFFu : 0u);
+        const bbox bounds = ord_box(gs + ST_BOX);
+        bbox b0, b1;
 #pragma unroll
-                    for (int i = NB - 1; i >= 1; i--) {
-                        acc = box_union(acc, ord_box(gs + 13 * i + 1));
-                        c += gs[13 * i];
-                        ra[i - 1] = (c == 0) ? 0.0f : (float)c * box_area(acc);
-                    }
-                }
-                float cbest = 0.0f;
-                uint32_t L = 0;
+        for (int c = 0; c < 3; c++) {
+            b0.mn[c] = row_scan<true, false>(mn[c]);
+            b0.mx[c] = row_scan<true, true>(mx[c]);
+            b1.mn[c] = row_next(row_scan<false, false>(mn[c]), FLT_MAX_);
+            b1.mx[c] = row_next(row_scan<false, true>(mx[c]), FLT_LOWEST_);
+        }
+        const uint32_t c0 = row_scan_add<true>(cnt);
+        const uint32_t c1 = row_next(row_scan_add<false>(cnt));
+        const float la = (c0 == 0) ? 0.0f : (float)c0 * box_area(b0);
+        const float ra = (c1 == 0) ? 0.0f : (float)c1 * box_area(b1);
+        const float cost = (float)(0.125 + (double)((la + ra) / box_area(bounds)));
+        // first minimum over splits 0 .. NB - 2, in order
+        uint32_t best = 0;
+        float cbest = __shfl(cost, (int)rbase);
+        for (uint32_t i = 1; i < (uint32_t)NB - 1; i++) {
+            const float ci = __shfl(cost, (int)(rbase + i));
+            if (ci < cbest) { best = i; cbest = ci; }
+        }
+        const uint32_t L = (uint32_t)__shfl((int)c0, (int)(rbase + best));
+        // the children's centroid bounds: buckets 0 .. best and best + 1 .. NB - 1
+        uint32_t ccb[2][6];
 #pragma unroll
-                for (int i = 0; i < NB - 1; i++) {
-                    const float cost = (float)(0.125 + (double)((la[i] + ra[i]) / area));
-                    if (i == 0 || cost < cbest) { best = (uint32_t)i; cbest = cost; L = cle[i]; }
-                }
-                fallback = L == m;
-                mid = fallback ? m / 2 + i0 : i0 + L;
-                s.u_i0[ub] = i0;
-                s.u_i1[ub] = i1;
-                s.u_mid[ub] = mid;
-                s.u_dep[ub] = dep;
-                s.u_rt[ub] = rt;
-                s.u_axis[ub] = (uint32_t)box_max_extent(ord_box(rec + RG_CB));
-                store_box(s.u_box + 6 * (size_t)ub, bounds);
-                uint32_t* grec = ref_rec(s, cur, r);
-                grec[RG_BEST] = best;
-                grec[RG_L] = L;
-                grec[RG_MID] = mid;
-                grec[RG_ML] = 0;
-            }
-            // children: kind 0 none / single position, 1 deferred, 2 next level
-            uint32_t kind[2] = {0, 0}, cn[2] = {0, 0};
-            if (valid)
-                for (int k = 0; k < 2; k++) {
-                    const uint32_t a = k == 0 ? i0 : mid, b = k == 0 ? mid - 1 : i1;
-                    const uint32_t size = b - a + 1;
-                    if (size == 1) {
-                        s.g_la[a] = dep + 1;
-                        s.g_lb[a] = rt + (uint32_t)k;
-                    } else if (size < REF_CUT) {
-                        kind[k] = 1;
-                    } else {
-                        kind[k] = 2;
-                        cn[k] = (size + REF_CHUNK - 1) / REF_CHUNK;
-                    }
-                }
-            const uint32_t want[3] = {(uint32_t)(kind[0] == 1) + (uint32_t)(kind[1] == 1),
-                                      (uint32_t)(kind[0] == 2) + (uint32_t)(kind[1] == 2), cn[0] + cn[1]};
-            uint32_t got[3];
-            wave_alloc3(s.sub_n, &s.lv_n[nxt], &s.lv_n[2 + nxt], want, got);
-            if (valid) {
-                uint32_t ne = got[0], nrr = got[1], nc = got[2];
-                for (int k = 0; k < 2; k++) {
-                    if (!kind[k]) continue;
-                    const uint32_t a = k == 0 ? i0 : mid, b = k == 0 ? mid - 1 : i1;
-                    const uint32_t cub = k == 0 ? ub + 1 : ub + (mid - i0);
-                    const uint32_t cdep = dep + 1, crt = rt + (uint32_t)k;
-                    if (kind[k] == 1) {
-                        s.sub_i0[ne] = a;
-                        s.sub_i1[ne] = b;
-                        s.sub_dep[ne] = cdep;
-                        s.sub_rt[ne] = crt;
-                        s.sub_ub[ne] = cub;
-                        ne++;
-                        continue;
-                    }
-                    uint32_t* cr = ref_rec(s, nxt, nrr);
-                    cr[RG_I0] = a;
-                    cr[RG_I1] = b;
-                    cr[RG_DEP] = cdep;
-                    cr[RG_RT] = crt;
-                    cr[RG_CH] = nc;
-                    cr[RG_NCH] = cn[k];
-                    cr[RG_UB] = cub;
-                    // the child's centroid bounds: the union of its buckets' (all
-                    // centroids coincide when the midpoint fallback splits)
-                    for (int q = 0; q < 6; q++) {
-                        uint32_t x = rec[RG_CB + q];
-                        if (!fallback) {
-                            x = q < 3 ? 0xFFFFFFFFu : 0u;
-                            const int j0 = k == 0 ? 0 : (int)best + 1, j1 = k == 0 ? (int)best : NB - 1;
-                            for (int j = j0; j <= j1; j++) {
-                                if (gs[13 * j] == 0) continue;
-                                const uint32_t y = gs[13 * j + 7 + q];
-                                x = q < 3 ? min(x, y) : max(x, y);
-                            }
-                        }
-                        cr[RG_CB + q] = x;
-                    }
-                    nrr++;
-                    nc += cn[k];
+        for (int q = 0; q < 6; q++) {
+            const uint32_t up = q < 3 ? row_scan_u<true, false>(cw[q]) : row_scan_u<true, true>(cw[q]);
+            const uint32_t dn = q < 3 ? row_scan_u<false, false>(cw[q]) : row_scan_u<false, true>(cw[q]);
+            ccb[0][q] = (uint32_t)__shfl((int)up, (int)(rbase + best));
+            ccb[1][q] = (uint32_t)__shfl((int)dn, (int)(rbase + best + 1));
+        }
+        const bool lead = valid && j == 0;
+        uint32_t i0 = 0, i1 = 0, mid = 0, ub = 0, dep = 0, rt = 0;
+        bool fallback = false;
+        if (lead) {
+            i0 = rec[RG_I0];
+            i1 = rec[RG_I1];
+            dep = rec[RG_DEP];
+            rt = rec[RG_RT];
+            ub = rec[RG_UB];
+            const uint32_t m = i1 - i0 + 1;
+            fallback = L == m;
+            mid = fallback ? m / 2 + i0 : i0 + L;
+            s.u_i0[ub] = i0;
+            s.u_i1[ub] = i1;
+            s.u_mid[ub] = mid;
+            s.u_dep[ub] = dep;
+            s.u_rt[ub] = rt;
+            s.u_axis[ub] = (uint32_t)box_max_extent(ord_box(rec + RG_CB));
+            store_box(s.u_box + 6 * (size_t)ub, bounds);
+            uint32_t* grec = ref_rec(s, cur, r);
+            grec[RG_BEST] = best;
+            grec[RG_L] = L;
+            grec[RG_MID] = mid;
+            grec[RG_ML] = 0;
+        }
+        // children: kind 0 none / single position, 1 deferred, 2 next level
+        uint32_t kind[2] = {0, 0}, cn[2] = {0, 0};
+        if (lead)
+            for (int k = 0; k < 2; k++) {
+                const uint32_t a = k == 0 ? i0 : mid, b = k == 0 ? mid - 1 : i1;
+                const uint32_t size = b - a + 1;
+                if (size == 1) {
+                    s.g_la[a] = dep + 1;
+                    s.g_lb[a] = rt + (uint32_t)k;
+                } else if (size < REF_CUT) {
+                    kind[k] = 1;
+                } else {
+                    kind[k] = 2;
+                    cn[k] = (size + REF_CHUNK - 1) / REF_CHUNK;
                 }
             }
+        const uint32_t want[3] = {(uint32_t)(kind[0] == 1) + (uint32_t)(kind[1] == 1),
+                                  (uint32_t)(kind[0] == 2) + (uint32_t)(kind[1] == 2), cn[0] + cn[1]};
+        uint32_t got[3];
+        wave_alloc3(s.sub_n, &s.lv_n[nxt], &s.lv_n[2 + nxt], want, got);
+        if (lead) {
+            uint32_t ne = got[0], nrr = got[1], nc = got[2];
+            for (int k = 0; k < 2; k++) {
+                if (!kind[k]) continue;
+                const uint32_t a = k == 0 ? i0 : mid, b = k == 0 ? mid - 1 : i1;
+                const uint32_t cub = k == 0 ? ub + 1 : ub + (mid - i0);
+                const uint32_t cdep = dep + 1, crt = rt + (uint32_t)k;
+                if (kind[k] == 1) {
+                    s.sub_i0[ne] = a;
+                    s.sub_i1[ne] = b;
+                    s.sub_dep[ne] = cdep;
+                    s.sub_rt[ne] = crt;
+                    s.sub_ub[ne] = cub;
+                    ne++;
+                    continue;
+                }
+                uint32_t* cr = ref_rec(s, nxt, nrr);
+                cr[RG_I0] = a;
+                cr[RG_I1] = b;
+                cr[RG_DEP] = cdep;
+                cr[RG_RT] = crt;
+                cr[RG_CH] = nc;
+                cr[RG_NCH] = cn[k];
+                cr[RG_UB] = cub;
+                // the child's centroid bounds: the union of its buckets' (all
+                // centroids coincide when the midpoint fallback splits)
+#pragma unroll
+                for (int q = 0; q < 6; q++) cr[RG_CB + q] = fallback ? rec[RG_CB + q] : ccb[k][q];
+                nrr++;
+                nc += cn[k];
+            }
         }
-        __syncthreads();
     }
 }
 
