@@ -3306,7 +3306,7 @@ static RefGrid ref_grid(uint32_t max_n, uint32_t n_scenes) {
     // about one chunk per workgroup at every level (a level holds up to
     // n / REF_CUT ranges): the per-chunk steps are latency-bound
     g.level = clampu(max_n / REF_CUT + 1, 1, n_scenes > 1 ? 128u : 4096u);
-    // one thread per range (a level holds up to n / REF_CUT ranges)
+    // one 16-lane row per range (a level holds up to n / REF_CUT ranges)
     g.ranges = clampu(max_n / (REF_CUT * LV_RPB) + 1, 1, 256u);
     g.sub = clampu(max_n / (64 * WR_WAVES) + 1, 1, n_scenes > 1 ? 64u : 1024u);
     g.fin = clampu(max_n / REF_FIN_BLOCK + 1, 1, n_scenes > 1 ? 64u : 1024u);
