@@ -2003,7 +2003,7 @@ WPBR_D void subtree_leaf_body(const LbvhScratch& s, const uint32_t* __restrict__
         }
     }
 }
-__global__ __launch_bounds__(64 * LEAF_WAVES) void k_subtree_leaf(LbvhScratch s, const uint32_t* __restrict__ codes) {
+__global__ __launch_bounds__(64 * LEAF_WAVES, 4) void k_subtree_leaf(LbvhScratch s, const uint32_t* __restrict__ codes) {
     subtree_leaf_body(s, codes, blockIdx.x, gridDim.x);
 }
 
@@ -3635,7 +3635,7 @@ __global__ __launch_bounds__(256) void ks_subtree(LbvhBatch B) {
     subtree_body(s, bid, nblk);
 }
 
-__global__ __launch_bounds__(64 * LEAF_WAVES) void ks_subtree_leaf(LbvhBatch B) {
+__global__ __launch_bounds__(64 * LEAF_WAVES, 4) void ks_subtree_leaf(LbvhBatch B) {
     SEG_BLOCK(B.map[SEG_SUBTREE_LEAF]);
     const LbvhScratch s = B.s[sc];
     subtree_leaf_body(s, s.keys_a, bid, nblk);
