@@ -2338,7 +2338,7 @@ WPBR_D void cb_block_write(const LbvhScratch& s, CbAcc a, uint32_t part, uint32_
 // Treelets below 30 shapes: one leaf each (emitLBVH's first case,
 // SLBVH.cpp:356-367); larger ones are listed for kr_emit_big.  Positions start
 // as treelet order.
-__global__ __launch_bounds__(REF_BLOCK) void kr_emit_small(const LbvhScratch* S, uint32_t G) {
+__global__ __launch_bounds__(REF_BLOCK, 7) void kr_emit_small(const LbvhScratch* S, uint32_t G) {
     REF_SCENE(S, G);
     __shared__ uint32_t s_red[6 * REF_BLOCK / 64];
     const uint32_t T = *s.n_treelets, n = s.n;
@@ -3495,7 +3495,7 @@ WPBR_D uint32_t work_scene(const uint32_t* work, uint32_t n_scenes, uint32_t g) 
 }
 
 constexpr uint32_t SEG_EMIT_BLOCKS = 8192;
-__global__ __launch_bounds__(EMIT_BLOCK) void ks_emit(LbvhBatch B) {
+__global__ __launch_bounds__(EMIT_BLOCK, 5) void ks_emit(LbvhBatch B) {
     const uint32_t total = B.work[B.n_scenes];
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         // the (g - work[sc])-th occupied bin of scene sc, two presence words per lane
@@ -3594,7 +3594,7 @@ __global__ void ks_big_count(LbvhBatch B) {
     }
 }
 constexpr uint32_t SEG_EMIT_TREE_BLOCKS = 8192;
-__global__ __launch_bounds__(EMIT_BLOCK) void ks_emit_trees(LbvhBatch B) {
+__global__ __launch_bounds__(EMIT_BLOCK, 7) void ks_emit_trees(LbvhBatch B) {
     const uint32_t total = B.subwork[B.n_scenes];
     for (uint32_t g = blockIdx.x; g < total; g += gridDim.x) {
         const uint32_t sc = __builtin_amdgcn_readfirstlane(work_scene(B.subwork, B.n_scenes, g));
