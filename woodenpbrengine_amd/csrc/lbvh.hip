@@ -3110,6 +3110,45 @@ WPBR_D void wr_range(LbvhScratch& s, WaveRangeLds& L, uint32_t st) {
 }
 
 // G workgroups per scene, WR_WAVES ranges in flight per workgroup.
+// The deferred ranges of a scene in order of decreasing size (a counting sort
+// on the size, 2 .. REF_CUT - 1, into pl_a, free once the level loop is done),
+// so that the range waves take the longest first and finish together.  Ranges
+// are disjoint and numbered in advance, so the order changes no result.
+__global__ __launch_bounds__(1024) void kr_sub_order(const LbvhScratch* S) {
+    __shared__ uint32_t h[REF_CUT];
+    const LbvhScratch& s = S[blockIdx.x];
+    const uint32_t nsub = *s.sub_n;
+    for (uint32_t b = threadIdx.x; b < REF_CUT; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < nsub; e += blockDim.x) {
+        const uint32_t m = s.sub_i1[e] - s.sub_i0[e] + 1;
+        atomicAdd(&h[REF_CUT - 1 - min(m, REF_CUT - 1)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // exclusive prefix over the REF_CUT bins, one wave (4 bins per lane)
+        constexpr uint32_t PB = REF_CUT / 64;
+        const uint32_t l = threadIdx.x;
+        uint32_t v[PB], t = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < PB; q++) { v[q] = h[PB * l + q]; t += v[q]; }
+        uint32_t incl = t;
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t x = (uint32_t)__shfl_up((int)incl, d);
+            if ((int)l >= d) incl += x;
+        }
+        uint32_t run = incl - t;
+#pragma unroll
+        for (uint32_t q = 0; q < PB; q++) { h[PB * l + q] = run; run += v[q]; }
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < nsub; e += blockDim.x) {
+        const uint32_t m = s.sub_i1[e] - s.sub_i0[e] + 1;
+        s.pl_a[atomicAdd(&h[REF_CUT - 1 - min(m, REF_CUT - 1)], 1u)] = e;
+    }
+}
+static_assert(REF_CUT % 64 == 0, "size bins per lane");
+
 __global__ __launch_bounds__(64 * WR_WAVES) void kr_sub_wave(const LbvhScratch* S, uint32_t G) {
     __shared__ WaveRangeLds W[WR_WAVES];
     const uint32_t sc = blockIdx.x / G, rj = blockIdx.x - sc * G;
@@ -3117,15 +3156,15 @@ __global__ __launch_bounds__(64 * WR_WAVES) void kr_sub_wave(const LbvhScratch* 
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t nsub = *s.sub_n;
     (void)rj;
-    // each wave takes the next range when it has finished one (the scene's
-    // range cursor is tile_counter[7], zeroed with the build's counters; the
-    // sort passes use words 0..3)
+    // each wave takes the next range in kr_sub_order's order when it has
+    // finished one (the scene's cursor is tile_counter[7], zeroed with the
+    // build's counters; the sort passes use words 0..3)
     for (;;) {
-        uint32_t st = 0;
-        if (lane_id() == 0) st = atomicAdd(s.tile_counter + 7, 1u);
-        st = (uint32_t)__builtin_amdgcn_readfirstlane((int)st);
-        if (st >= nsub) break;
-        wr_range(s, W[wave], st);
+        uint32_t k = 0;
+        if (lane_id() == 0) k = atomicAdd(s.tile_counter + 7, 1u);
+        k = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+        if (k >= nsub) break;
+        wr_range(s, W[wave], s.pl_a[k]);
     }
 }
 // ---- preorder layout ---------------------------------------------------------
@@ -3385,7 +3424,10 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
     // the deferred ranges, one wave each: per scene from its cursor, or for a
     // segmented build from one list over all scenes
     if (B) ks_launch_sub_wave(*B, SEG_REF_SUB_BLOCKS, st);
-    else hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
+    else {
+        hipLaunchKernelGGL(kr_sub_order, dim3(ns), dim3(1024), 0, st, S);
+        hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
+    }
     hipLaunchKernelGGL(kr_fin_sum, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
     hipLaunchKernelGGL(kr_fin_scan, dim3(ns), dim3(1024), 0, st, S);
     hipLaunchKernelGGL(kr_fin_place, dim3(ns * g.fin), dim3(REF_BLOCK), 0, st, S, g.fin);
