@@ -3109,7 +3109,6 @@ WPBR_D void wr_range(LbvhScratch& s, WaveRangeLds& L, uint32_t st) {
     __builtin_amdgcn_wave_barrier();
 }
 
-// G workgroups per scene, WR_WAVES ranges in flight per workgroup.
 // The deferred ranges of a scene in order of decreasing size (a counting sort
 // on the size, 2 .. REF_CUT - 1, into pl_a, free once the level loop is done),
 // so that the range waves take the longest first and finish together.  Ranges
@@ -3149,6 +3148,7 @@ __global__ __launch_bounds__(1024) void kr_sub_order(const LbvhScratch* S) {
 }
 static_assert(REF_CUT % 64 == 0, "size bins per lane");
 
+// G workgroups per scene, WR_WAVES ranges in flight per workgroup.
 __global__ __launch_bounds__(64 * WR_WAVES) void kr_sub_wave(const LbvhScratch* S, uint32_t G) {
     __shared__ WaveRangeLds W[WR_WAVES];
     const uint32_t sc = blockIdx.x / G, rj = blockIdx.x - sc * G;
