@@ -146,7 +146,7 @@ static size_t lbvh_layout(LbvhScratch& s, char* base, uint32_t n) {
     s.lv_rec = c.take<uint32_t>(2 * (size_t)s.cap_r * REF_RG_WORDS);
     s.lv_stat = c.take<uint32_t>(2 * (size_t)s.cap_r * REF_ST_WORDS);
     s.ch_range = c.take<uint32_t>(2 * (size_t)s.cap_c);
-    s.ch_cnt = c.take<uint32_t>(2 * (size_t)s.cap_c);
+    s.ch_cnt = c.take<uint32_t>(REF_CH_WORDS * (size_t)s.cap_c);
     s.fin_p = c.take<uint32_t>((size_t)CT + 1);
     s.fin_blk = c.take<uint32_t>((size_t)CT / REF_FIN_BLOCK + 1);
     return c.off + 256;

@@ -2100,15 +2100,16 @@ __global__ __launch_bounds__(256) void k_bvh4(LbvhScratch s) { bvh4_body(s, bloc
 //                   run heads (SLBVH.cpp:299-319), the upper SAH's first range
 //   kr_emit_small / kr_emit_big   treelets below 30 shapes are one leaf
 //                   (SLBVH.cpp:356-367), larger ones get emitLBVH by a workgroup
-//   kr_lv_stats / kr_lv_ranges / kr_lv_count / kr_lv_lists / kr_lv_swap
+//   kr_lv_stats / kr_lv_ranges / kr_lv_lists / kr_lv_swap
 //                   one level of the upper SAH (SLBVH.cpp:457-563) over every
-//                   range of at least REF_CUT roots: bucket statistics, the
-//                   split (cost, first minimum, midpoint fallback) one thread
-//                   per range, misplaced counts per chunk and the libstdc++ two-pointer
-//                   std::partition in rank form (the r-th misplaced position
-//                   left of the split swaps with the r-th misplaced position
-//                   right of it counted from the right)
-//   kr_sub_wave     ranges below REF_CUT, one wave each in LDS
+//                   range of at least REF_CUT roots: bucket statistics (and each
+//                   chunk's bucket counts), the split (cost, first minimum,
+//                   midpoint fallback) one 16-lane row per range, and the
+//                   libstdc++ two-pointer std::partition in rank form (the r-th
+//                   misplaced position left of the split swaps with the r-th
+//                   misplaced position right of it counted from the right)
+//   kr_sub_order / kr_sub_wave   ranges below REF_CUT, largest first, one
+//                   wave each in LDS
 //   kr_fin_*        preorder layout with a device-wide prefix of the treelet
 //                   node counts; kr_scatter_ref writes the treelet nodes
 // Ranges and chunks of level k live in list k & 1; the level's split pushes
@@ -2130,6 +2131,7 @@ static_assert(RG_UB < (int)REF_RG_WORDS, "range record");
 // xyz / max xyz as order-preserving uints), then the range's box.
 constexpr uint32_t ST_BOX = NB * 13;
 static_assert(ST_BOX + 6 <= REF_ST_WORDS, "statistics record");
+static_assert(REF_CH_WORDS == (uint32_t)NB, "a chunk's bucket counts");
 
 #define REF_SCENE(S, G)                                   \
     const uint32_t sc = blockIdx.x / (G);                 \
@@ -2511,6 +2513,8 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_stats(const LbvhScratch* S, u
                 const uint32_t y = st[q * REF_ST_WORDS + k];
                 x = cnt ? x + y : (mx ? max(x, y) : min(x, y));
             }
+            // the chunk's own bucket counts, for the partition's ranks
+            if (cnt) s.ch_cnt[(size_t)c * REF_CH_WORDS + k / 13] = x;
             if (x == stat_init_word(k)) continue;
             if (cnt) atomicAdd(&gs[k], x);
             else if (mx) atomicMax(&gs[k], x);
@@ -2712,39 +2716,9 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_ranges(const LbvhScratch* S, 
     }
 }
 
-// Per chunk of the level: its misplaced positions (left of the split with a
-// bucket above it, right of it with a bucket at or below it); then the next
-// level's chunk-to-range map and zeroed statistics.
-__global__ __launch_bounds__(REF_BLOCK) void kr_lv_count(const LbvhScratch* S, uint32_t G, uint32_t level) {
-    REF_SCENE(S, G);
-    __shared__ uint32_t s_w[REF_BLOCK / 64];
-    const uint32_t cur = level & 1u, nxt = cur ^ 1u;
-    const uint32_t nch = s.lv_n[2 + cur];
-    const int lane = threadIdx.x & 63;
-    for (uint32_t c = rj; c < nch; c += G) {
-        const RefRange R = ref_range(s, cur, c);
-        const uint32_t m = R.i1 - R.i0 + 1, L = R.rec[RG_L], best = R.rec[RG_BEST];
-        uint32_t cl = 0, cr = 0;
-        if (L != m)
-            for (uint32_t p = R.p0 + threadIdx.x; p < R.p1; p += REF_BLOCK) {
-                const uint32_t ib = s.gp_bucket[p];
-                if (p < R.i0 + L) cl += ib > best;
-                else cr += ib <= best;
-            }
-        for (int off = 32; off >= 1; off >>= 1) {
-            cl += (uint32_t)__shfl_xor((int)cl, off);
-            cr += (uint32_t)__shfl_xor((int)cr, off);
-        }
-        if (lane == 0) s_w[threadIdx.x >> 6] = cl | (cr << 16);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t tl = 0, tr = 0;
-            for (int w = 0; w < REF_BLOCK / 64; w++) { tl += s_w[w] & 0xFFFFu; tr += s_w[w] >> 16; }
-            s.ch_cnt[2 * (size_t)c] = tl;
-            s.ch_cnt[2 * (size_t)c + 1] = tr;
-        }
-        __syncthreads();
-    }
+// The next level's chunk-to-range map and zeroed statistics (its ranges are
+// written by kr_lv_ranges; its first kernel reads them).
+WPBR_D void ref_next_level_init(const LbvhScratch& s, uint32_t nxt, uint32_t rj, uint32_t G) {
     const uint32_t nr2 = s.lv_n[nxt];
     for (uint32_t r = rj; r < nr2; r += G) {
         const uint32_t* cr = ref_rec(s, nxt, r);
@@ -2754,7 +2728,7 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_count(const LbvhScratch* S, u
         for (uint32_t q = threadIdx.x; q < REF_ST_WORDS; q += REF_BLOCK) st[q] = stat_init_word(q);
     }
 }
-static_assert(REF_CHUNK < 65536, "chunk counts packed in 16 bits");
+
 
 // Ranks of the misplaced positions: pl_a[i0 + r] = the r-th misplaced position
 // left of the split (from the left), pl_b[i0 + r] = the r-th misplaced
@@ -2770,16 +2744,36 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_lists(const LbvhScratch* S, u
         const RefRange R = ref_range(s, cur, c);
         const uint32_t m = R.i1 - R.i0 + 1, L = R.rec[RG_L], best = R.rec[RG_BEST];
         if (L == m) continue;  // uniform over the workgroup
-        // misplaced counts of the range's chunks before (left) and after (right) this one
+        // misplaced counts of the range's chunks before (left) and after (right)
+        // this one: a chunk wholly left of the split counts its positions in
+        // buckets above best, one wholly right of it those at or below best
+        // (from its bucket counts); the chunk holding the split is counted
+        // position by position
+        const uint32_t split = R.i0 + L;
         uint32_t bl = 0, br = 0;
         for (uint32_t q = R.ch0 + threadIdx.x; q < R.ch0 + R.nch; q += REF_BLOCK) {
-            if (q < c) bl += s.ch_cnt[2 * (size_t)q];
-            if (q > c) br += s.ch_cnt[2 * (size_t)q + 1];
+            if (q == c) continue;
+            const uint32_t qp0 = R.i0 + (q - R.ch0) * REF_CHUNK, qp1 = min(qp0 + REF_CHUNK, R.i1 + 1);
+            const uint32_t* h = s.ch_cnt + (size_t)q * REF_CH_WORDS;
+            if (q < c && qp1 <= split)
+                for (uint32_t b = best + 1; b < REF_CH_WORDS; b++) bl += h[b];
+            if (q > c && qp0 >= split)
+                for (uint32_t b = 0; b <= best; b++) br += h[b];
+        }
+        const uint32_t sc0 = R.ch0 + (split - R.i0) / REF_CHUNK;  // the chunk holding the split
+        if (sc0 != c && sc0 < R.ch0 + R.nch) {
+            const uint32_t sp0 = R.i0 + (sc0 - R.ch0) * REF_CHUNK, sp1 = min(sp0 + REF_CHUNK, R.i1 + 1);
+            if (sp0 < split) {
+                for (uint32_t p = sp0 + threadIdx.x; p < sp1; p += REF_BLOCK) {
+                    const uint32_t ib = s.gp_bucket[p];
+                    if (sc0 < c && p < split) bl += ib > best;
+                    if (sc0 > c && p >= split) br += ib <= best;
+                }
+            }
         }
         uint32_t tbl, tbr;
         (void)block_excl_scan(bl, s_w, &tbl);
         (void)block_excl_scan(br, s_w, &tbr);
-        const uint32_t split = R.i0 + L;
         const uint32_t q0 = R.p0 + threadIdx.x * REF_PER;
         uint32_t fl = 0, fr = 0;  // flags of this thread's positions
 #pragma unroll
@@ -2815,6 +2809,7 @@ __global__ __launch_bounds__(REF_BLOCK) void kr_lv_swap(const LbvhScratch* S, ui
     REF_SCENE(S, G);
     const uint32_t cur = level & 1u;
     const uint32_t nch = s.lv_n[2 + cur];
+    ref_next_level_init(s, cur ^ 1u, rj, G);
     for (uint32_t c = rj; c < nch; c += G) {
         const RefRange R = ref_range(s, cur, c);
         const uint32_t m = R.i1 - R.i0 + 1, L = R.rec[RG_L];
@@ -3385,7 +3380,6 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
         for (uint32_t k = 0; k < REF_LEVELS_PER_CHECK; k++, level++) {
             hipLaunchKernelGGL(kr_lv_stats, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
             hipLaunchKernelGGL(kr_lv_ranges, dim3(ns * g.ranges), dim3(REF_BLOCK), 0, st, S, g.ranges, level);
-            hipLaunchKernelGGL(kr_lv_count, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
             hipLaunchKernelGGL(kr_lv_lists, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
             hipLaunchKernelGGL(kr_lv_swap, dim3(ns * g.level), dim3(REF_BLOCK), 0, st, S, g.level, level);
         }

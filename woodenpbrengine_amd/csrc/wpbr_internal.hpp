@@ -95,7 +95,7 @@ struct LbvhScratch {
     uint32_t* lv_rec;     // 2 x cap_r x REF_RG_WORDS range records
     uint32_t* lv_stat;    // 2 x cap_r x REF_ST_WORDS bucket statistics
     uint32_t* ch_range;   // 2 x cap_c: range of each chunk
-    uint32_t* ch_cnt;     // cap_c x 2: misplaced positions left / right per chunk
+    uint32_t* ch_cnt;     // cap_c x REF_CH_WORDS: bucket counts of each chunk
     uint32_t* fin_p;      // cap_t + 1: prefix of treelet node counts over final positions
     uint32_t* fin_blk;    // per REF_FIN_BLOCK positions: node sums, then their exclusive prefix
 };
@@ -108,6 +108,7 @@ constexpr uint32_t REF_CHUNK = 2048;
 constexpr uint32_t REF_FIN_BLOCK = 1024;
 constexpr uint32_t REF_RG_WORDS = 20;
 constexpr uint32_t REF_ST_WORDS = 164;
+constexpr uint32_t REF_CH_WORDS = 12;  // per chunk: its positions' count in each SAH bucket
 inline uint32_t ref_cap_r(uint32_t cap_t) { return cap_t / REF_CUT + 2; }
 inline uint32_t ref_cap_c(uint32_t cap_t) { return cap_t / REF_CHUNK + ref_cap_r(cap_t) + 1; }
 
