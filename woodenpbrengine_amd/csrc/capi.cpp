@@ -448,7 +448,7 @@ wpbr_status wpbr_lbvh_build_segmented(wpbr_ctx* ctx, uint32_t n_scenes, const wp
     auto reserve = [&](size_t b) { size_t o = (bytes + 255) & ~size_t(255); bytes = o + b; return o; };
     const size_t o_s = reserve(sizeof(LbvhScratch) * n_scenes), o_g = reserve(sizeof(GeomView) * n_scenes),
                  o_n = reserve(4 * (size_t)n_scenes), o_work = reserve(4 * ((size_t)n_scenes + 1)),
-                 o_subwork = reserve(4 * ((size_t)n_scenes + 1)),
+                 o_subwork = reserve(4 * ((size_t)n_scenes + 2)),  // + the range cursor
                  o_info = reserve(20 * (size_t)n_scenes + 4);  // + the level loop's range count
     size_t o_first[SEG_KINDS], o_count[SEG_KINDS], o_scene[SEG_KINDS];
     for (int k = 0; k < SEG_KINDS; k++) {

@@ -3371,7 +3371,7 @@ static void ref_launch_emit(const LbvhScratch* S, uint32_t ns, const RefGrid& g,
 constexpr uint32_t SEG_REF_SUB_BLOCKS = 4096;  // segmented reference order: deferred-range workgroups
 constexpr uint32_t REF_LEVELS_PER_CHECK = 2;  // measured against 4: 2.24 -> 2.22 ms at 1M shapes
 constexpr uint32_t REF_MAX_LEVELS = 4096;
-static void ks_launch_sub_wave(const LbvhBatch& B, uint32_t blocks, hipStream_t st);
+static void ks_launch_sub_wave(const LbvhBatch& B, uint32_t blocks, hipStream_t st, bool cursor);
 static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const RefGrid& g, uint32_t* d_active,
                                     hipStream_t st, const LevelSync* ls, const LbvhBatch* B = nullptr) {
     uint32_t level = 0;
@@ -3415,9 +3415,10 @@ static wpbr_status ref_launch_upper(const LbvhScratch* S, uint32_t ns, const Ref
             if (level >= REF_MAX_LEVELS) return WPBR_ERR_HIP;
         }
     }
-    // the deferred ranges, one wave each: per scene from its cursor, or for a
-    // segmented build from one list over all scenes
-    if (B) ks_launch_sub_wave(*B, SEG_REF_SUB_BLOCKS, st);
+    // the deferred ranges, one wave each: per scene from its cursor, largest
+    // first, or for a segmented build from one list over all scenes (the
+    // per-scene ordered form measured 2.75 -> 2.89 ms on the 64-scene batch)
+    if (B) ks_launch_sub_wave(*B, SEG_REF_SUB_BLOCKS, st, false);
     else {
         hipLaunchKernelGGL(kr_sub_order, dim3(ns), dim3(1024), 0, st, S);
         hipLaunchKernelGGL(kr_sub_wave, dim3(ns * g.sub), dim3(64 * WR_WAVES), 0, st, S, g.sub);
@@ -3568,10 +3569,12 @@ __global__ __launch_bounds__(SAH_BLOCK) void ks_sah_top(LbvhBatch B) {
     LbvhScratch s = B.s[blockIdx.x];
     sah_top_body(s);
 }
-// subwork[i] = deferred SAH ranges of scenes 0 .. i-1 (one workgroup).
+// subwork[i] = deferred SAH ranges of scenes 0 .. i-1 (one workgroup);
+// subwork[n_scenes + 1] = 0, the range cursor of ks_sub_wave.
 __global__ void ks_sah_sub_count(LbvhBatch B) {
     __shared__ uint32_t s_sum[1024];
     uint32_t run = 0;
+    if (threadIdx.x == 0) B.subwork[B.n_scenes + 1] = 0;
     for (uint32_t base = 0; base < B.n_scenes; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         s_sum[threadIdx.x] = i < B.n_scenes ? *B.s[i].sub_n : 0u;
@@ -3596,11 +3599,22 @@ __global__ void ks_sah_sub_count(LbvhBatch B) {
 // One workgroup per range (as k_sah_sub) was measured slower here: the 64
 // scenes' deep, unbalanced ranges keep every workgroup's barriers on their
 // critical paths (segmented full-order build 1.76 -> 1.68 ms).
+// CURSOR: each wave takes the next range of the list when it has finished one
+// (the full order's few large ranges: segmented build 1.57 -> 1.56 ms);
+// otherwise the waves stride over the list (the reference order's many small
+// ranges: the shared cursor measured 2.75 -> 2.83 ms).
+template <bool CURSOR>
 __global__ __launch_bounds__(64 * WR_WAVES, 3) void ks_sub_wave(LbvhBatch B) {
     __shared__ WaveRangeLds W[WR_WAVES];
     const uint32_t wave = threadIdx.x >> 6;
     const uint32_t total = B.subwork[B.n_scenes];
-    for (uint32_t g = blockIdx.x * WR_WAVES + wave; g < total; g += gridDim.x * WR_WAVES) {
+    for (uint32_t g = blockIdx.x * WR_WAVES + wave;; g += gridDim.x * WR_WAVES) {
+        if (CURSOR) {
+            uint32_t k = 0;
+            if (lane_id() == 0) k = atomicAdd(B.subwork + B.n_scenes + 1, 1u);
+            g = (uint32_t)__builtin_amdgcn_readfirstlane((int)k);
+        }
+        if (g >= total) break;
         // wave-uniform scene: its scratch pointers stay in scalar registers
         const uint32_t sc = __builtin_amdgcn_readfirstlane(work_scene(B.subwork, B.n_scenes, g));
         LbvhScratch s = B.s[sc];
@@ -3642,9 +3656,10 @@ __global__ __launch_bounds__(EMIT_BLOCK, 7) void ks_emit_trees(LbvhBatch B) {
         __syncthreads();
     }
 }
-static void ks_launch_sub_wave(const LbvhBatch& B, uint32_t blocks, hipStream_t st) {
+static void ks_launch_sub_wave(const LbvhBatch& B, uint32_t blocks, hipStream_t st, bool cursor) {
     hipLaunchKernelGGL(ks_sah_sub_count, dim3(1), dim3(1024), 0, st, B);
-    hipLaunchKernelGGL(ks_sub_wave, dim3(blocks), dim3(64 * WR_WAVES), 0, st, B);
+    if (cursor) hipLaunchKernelGGL(ks_sub_wave<true>, dim3(blocks), dim3(64 * WR_WAVES), 0, st, B);
+    else hipLaunchKernelGGL(ks_sub_wave<false>, dim3(blocks), dim3(64 * WR_WAVES), 0, st, B);
 }
 static void ks_launch_emit_trees(const LbvhBatch& B, hipStream_t st) {
     hipLaunchKernelGGL(ks_big_count, dim3(1), dim3(1024), 0, st, B);
@@ -3739,7 +3754,7 @@ wpbr_status lbvh_launch_segmented(const LbvhBatch& B, const uint32_t* grid, uint
         hipLaunchKernelGGL(ks_treelet_count, dim3(1), dim3(1024), 0, st, B);
         hipLaunchKernelGGL(ks_emit, dim3(SEG_EMIT_BLOCKS), dim3(EMIT_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_sah_top, dim3(S), dim3(SAH_BLOCK), 0, st, B);
-        ks_launch_sub_wave(B, SAH_SEG_SUB_BLOCKS, st);
+        ks_launch_sub_wave(B, SAH_SEG_SUB_BLOCKS, st, true);
         hipLaunchKernelGGL(ks_sah_final, dim3(S * SEG_SAH_FINAL_BLOCKS), dim3(SAH_BLOCK), 0, st, B);
         hipLaunchKernelGGL(ks_scatter, dim3(4096), dim3(256), 0, st, B);
     }

@@ -157,7 +157,7 @@ struct LbvhBatch {
     const uint32_t* n;
     uint32_t n_scenes;
     uint32_t* work;         // n_scenes + 1: treelets of the scenes before each scene
-    uint32_t* subwork;      // n_scenes + 1: deferred SAH ranges of the scenes before each scene
+    uint32_t* subwork;      // n_scenes + 2: deferred SAH ranges of the scenes before each scene, then a cursor
     uint32_t* info;         // 5 per scene: n_nodes, n_treelets, err, nshapes[0], offset[0]
     SegMap map[SEG_KINDS];
 };
