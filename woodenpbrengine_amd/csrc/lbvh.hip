@@ -1815,8 +1815,8 @@ WPBR_D void leaf_run_records(const LbvhScratch& s, const LeafLds& W, uint32_t lb
 
 // The sub-tree of a reference leaf of 5 .. WD shapes (slots off .. off + n - 1,
 // Morton-ordered) by WD lanes, lane j holding slot j: binary and 4-wide records
-// of all its nodes.  WD = 32 runs two leaves per wave, one per half (`i` is the
-// half's leaf, or none).  It is the tree emitLBVH would continue below its leaf
+// of all its nodes.  WD = 32 (16) runs two (four) leaves per wave, one per
+// group of WD lanes (`i` is the group's leaf, or none).  It is the tree emitLBVH would continue below its leaf
 // size (SLBVH.cpp:374-401): a range whose codes differ splits at its highest
 // differing bit (child 0 below child 1 on axis bit % 3); a range of equal codes
 // splits in halves; ranges of at most 4 slots are primitive ranges.  The splits
@@ -1830,7 +1830,7 @@ constexpr uint32_t NO_LEAF = 0xFFFFFFFFu;
 template <int WD>
 WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ codes, LeafLds& W, uint32_t i,
                          uint32_t nn) {
-    constexpr uint64_t HMASK = WD == 64 ? ~0ull : 0xFFFFFFFFull;
+    constexpr uint64_t HMASK = WD == 64 ? ~0ull : ((1ull << WD) - 1ull);
     const uint32_t lane = lane_id(), j = lane % WD, lb = lane - j, sh = lb;  // lb: the half's first lane
     const bool has = i != NO_LEAF;
     const uint32_t n = has ? s.node_nshapes[i] : 0u, off = has ? s.node_offset[i] : 0u;
@@ -1969,8 +1969,8 @@ WPBR_D void subtree_leaf(const LbvhScratch& s, const uint32_t* __restrict__ code
 }
 
 // Reference leaves of 5 .. BIG_LEAF shapes: every wave scans LEAF_CHUNK
-// consecutive nodes (grid-stride); leaves of at most 32 shapes go two per wave,
-// larger ones one per wave.
+// consecutive nodes (grid-stride); leaves of at most 16 shapes go four per wave,
+// of at most 32 two per wave, larger ones one per wave.
 constexpr uint32_t LEAF_WAVES = 4;
 constexpr uint32_t LEAF_BLOCKS = 2048;
 constexpr uint32_t LEAF_CHUNK = 32;
@@ -1981,8 +1981,19 @@ WPBR_D void subtree_leaf_body(const LbvhScratch& s, const uint32_t* __restrict__
     for (uint32_t c0 = (bid * LEAF_WAVES + wave) * LEAF_CHUNK; c0 < nn; c0 += nblk * LEAF_WAVES * LEAF_CHUNK) {
         const uint32_t i = c0 + lane;
         const uint32_t ns = (lane < LEAF_CHUNK && i < nn) ? s.node_nshapes[i] : 0u;
-        uint64_t m2 = ballot64(ns > RANGE_MAX && ns <= 32);
+        uint64_t m4 = ballot64(ns > RANGE_MAX && ns <= 16);
+        uint64_t m2 = ballot64(ns > 16 && ns <= 32);
         uint64_t m1 = ballot64(ns > 32 && ns <= BIG_LEAF);
+        while (m4) {
+            uint32_t mine = NO_LEAF;
+            for (uint32_t q = 0; q < 4 && m4; q++) {
+                const uint32_t k = (uint32_t)(__ffsll((unsigned long long)m4) - 1);
+                m4 &= m4 - 1;
+                if ((lane >> 4) == q) mine = c0 + k;
+            }
+            subtree_leaf<16>(s, codes, W[wave], mine, nn);
+            __builtin_amdgcn_wave_barrier();
+        }
         while (m2) {
             const uint32_t ka = (uint32_t)(__ffsll((unsigned long long)m2) - 1);
             m2 &= m2 - 1;
